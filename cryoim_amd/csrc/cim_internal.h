@@ -1,0 +1,141 @@
+// Internal declarations shared by the HIP translation units of libcryoim_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cryoim.h"
+#include "cim_symops.h"
+
+namespace cim {
+
+// Growable device buffer owned by a context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Twiddle table and factorisation of one FFT length.
+struct FftPlan {
+    int n = 0;
+    int nfac = 0;
+    int fac[16];
+    float2* tw = nullptr;   // exp(-2 pi i k / n), k = 0..n-1 (device)
+};
+
+// Reconstruction accumulator: float4 {fa, fb, wg, 0} voxels covering
+// x, y in [-(maxR), maxR+1] and z in [0, maxR+1] around the origin.
+struct RecState {
+    bool active = false;
+    int box = 0, maxR = 0, layout = 0, nops = 0;
+    float rcut = 0.f, wg_init = 0.f;
+    int dxy = 0, dz = 0;            // accumulator dims
+    DevBuf acc;                     // float4 * dxy*dxy*dz
+    DevBuf ops;                     // double[nops*9]
+    DevBuf samples;                 // int2 (hx, ky) per half-plane sample
+    int nsamples = 0;
+    long long inserted = 0;
+};
+
+// Template bank for the common-lines search.
+struct SearchState {
+    bool ready = false;
+    int box = 0, T = 0, minR = 0, maxR = 0;
+    DevBuf tft;        // float2 [T][box*box] template FTs (masked, FFT, centre shift)
+    DevBuf tfcut;      // float [T][nr] Fcut fitted without CTF (load_template.f90:44-47)
+    std::vector<float> tang;   // T*3 template angles
+};
+
+struct KTimer {
+    double total_ms = 0.0;
+    long long launches = 0;
+};
+
+}  // namespace cim
+
+struct cim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<int, cim::FftPlan> plans;
+    cim::RecState rec;
+    cim::SearchState srch;
+    // scratch buffers reused across calls
+    cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
+    // kernel timing
+    bool timing = false;
+    std::map<std::string, cim::KTimer> timers;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace cim {
+
+int set_error(cim_ctx* c, int code, const char* fmt, ...);
+
+#define CIM_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return cim::set_error(ctx, CIM_E_HIP, "%s failed: %s (%s:%d)", #call,      \
+                                  hipGetErrorString(e_), __FILE__, __LINE__);          \
+    } while (0)
+
+#define CIM_ALLOC(buf, n)                                                              \
+    do {                                                                               \
+        if ((buf).ensure(n) != hipSuccess)                                             \
+            return cim::set_error(ctx, CIM_E_NOMEM, "device allocation of %zu bytes failed", \
+                                  (size_t)(n));                                        \
+    } while (0)
+
+// Kernel timing bracket (enabled with cim_timing).
+struct TimedLaunch {
+    cim_ctx* ctx;
+    const char* name;
+    TimedLaunch(cim_ctx* c, const char* n);
+    ~TimedLaunch();
+};
+
+// FFT (fft.hip)
+int fft_plan(cim_ctx* ctx, int n, FftPlan** out);
+// In-place transforms of interleaved complex data on ctx->stream.
+int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_out);
+int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
+
+// split <-> interleaved conversion (prep.hip)
+int interleave(cim_ctx* ctx, const float* re, const float* im, float2* out, long long count);
+int deinterleave(cim_ctx* ctx, const float2* in, float* re, float* im, long long count);
+
+// Per-particle preprocessing to a centred FT (prep.hip).
+// mode_norm: normalise first; mask centre from parts[CIM_P_MASKX/Y].
+int preprocess_images(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev,
+                      const float* parts_dev, float2* ft_out, int do_norm, int do_mask,
+                      int conj_out);
+// Multiply centred FTs by exp(-i phase) of img2d_FT_centshift (recentre=1).
+int centre_shift(cim_ctx* ctx, float2* ft, int box, int n, const float* parts_dev, int recentre);
+// Fcut of FT_amp_expfit for n centred FTs (optionally multiplied by the CTF
+// of parts_dev record (p % nparts_ctf)).
+int amp_expfit(cim_ctx* ctx, const float2* ft, int box, int n, int minR, int maxR, float* fcut_dev);
+// General form: item i at ft + i*stride, rows of ld, zero frequency at (cidx, cidx).
+int amp_expfit_general(cim_ctx* ctx, const float2* ft, long long stride, int ld, int cidx, int nitems, int minR,
+                       int maxR, float* fcut_dev);
+
+}  // namespace cim

@@ -1,0 +1,267 @@
+"""ctypes binding of libcryoim_hip.so (include/cryoim.h).
+
+The HIP library is the only compute path: if it is missing or fails to
+load, every call raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcryoim_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cryoim.h")
+
+PART_STRIDE = 12
+P_THETA, P_PHI, P_OMEGA, P_X, P_Y, P_PR, P_DF1, P_DF2, P_ASTIG, P_MASKX, P_MASKY, P_USE = range(12)
+
+
+class CimParams(ctypes.Structure):
+    _fields_ = [
+        ("box", ctypes.c_int), ("minR", ctypes.c_int), ("maxR", ctypes.c_int),
+        ("apply_ctf", ctypes.c_int), ("mask_on", ctypes.c_int), ("norm_on", ctypes.c_int),
+        ("ncycle", ctypes.c_int), ("conj_out", ctypes.c_int), ("reserved_i", ctypes.c_int * 8),
+        ("apix", ctypes.c_float), ("imgmask", ctypes.c_float), ("vol", ctypes.c_float),
+        ("cs", ctypes.c_float), ("ampwg", ctypes.c_float), ("df_scale", ctypes.c_float),
+        ("searchstep", ctypes.c_float), ("delta_angle", ctypes.c_float),
+        ("delta_trans", ctypes.c_float), ("reserved_f", ctypes.c_float * 7),
+    ]
+
+
+def make_params(box, minR=0, maxR=None, apply_ctf=True, mask_on=True, norm_on=False, ncycle=6,
+                conj_out=True, apix=1.0, imgmask=None, vol=300.0, cs=2.7, ampwg=0.1, df_scale=1.0e4,
+                searchstep=2.0, delta_angle=0.0, delta_trans=0.0) -> CimParams:
+    p = CimParams()
+    p.box = int(box)
+    p.minR = int(minR)
+    p.maxR = int(maxR if maxR is not None else box // 2 - 2)
+    p.apply_ctf = int(bool(apply_ctf))
+    p.mask_on = int(bool(mask_on))
+    p.norm_on = int(bool(norm_on))
+    p.ncycle = int(ncycle)
+    p.conj_out = int(bool(conj_out))
+    p.apix = float(apix)
+    p.imgmask = float(imgmask if imgmask is not None else box / 2 - 2)
+    p.vol, p.cs, p.ampwg, p.df_scale = float(vol), float(cs), float(ampwg), float(df_scale)
+    p.searchstep = float(searchstep)
+    p.delta_angle, p.delta_trans = float(delta_angle), float(delta_trans)
+    return p
+
+
+class CimError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise CimError(f"{LIB_PATH} is not built (run __graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, ip, fp, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+    P = ctypes.POINTER(CimParams)
+    sig = {
+        "cim_init": (ip, [ctypes.POINTER(vp), ip]),
+        "cim_finalize": (ip, [vp]),
+        "cim_last_error": (ctypes.c_char_p, [vp]),
+        "cim_get_stream": (vp, [vp]),
+        "cim_sync": (ip, [vp]),
+        "cim_timing": (ip, [vp, ip]),
+        "cim_timing_get": (ip, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_longlong)]),
+        "cim_fft2d": (ip, [vp, fp, fp, ip, ip, ip, ip]),
+        "cim_fft3d": (ip, [vp, fp, fp, ip, ip, ip]),
+        "cim_fft2d_dev": (ip, [vp, vp, ip, ip, ip, ip]),
+        "cim_fft3d_dev": (ip, [vp, vp, ip, ip, ip]),
+        "cim_rec_begin": (ip, [vp, ip, ip, ip, ctypes.c_float, ctypes.c_float, ip, dp]),
+        "cim_rec_insert": (ip, [vp, P, ip, fp, fp]),
+        "cim_rec_insert_dev": (ip, [vp, P, ip, vp, fp]),
+        "cim_rec_insert_ft": (ip, [vp, ip, fp, fp, fp, fp]),
+        "cim_rec_get_tnf": (ip, [vp, fp, fp, fp]),
+        "cim_rec_add_tnf": (ip, [vp, fp, fp, fp]),
+        "cim_rec_finish": (ip, [vp, fp]),
+        "cim_rec_finish_dev": (ip, [vp, vp]),
+        "cim_search_templates": (ip, [vp, P, ip, fp, fp]),
+        "cim_search_global": (ip, [vp, P, ip, fp, fp]),
+        "cim_search_global_dev": (ip, [vp, P, ip, vp, fp]),
+        "cim_search_pr_array": (ip, [vp, P, fp, fp, fp]),
+        "cim_refine_local": (ip, [vp, P, ip, fp, fp]),
+    }
+    for name, (res, args) in sig.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def header_functions(path: str = HEADER):
+    """Names of the functions include/cryoim.h declares."""
+    import re
+    txt = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|const char\s*\*)\s*(cim_\w+)\s*\(", txt, re.M)))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Context:
+    """One cim_ctx on one device; all calls are synchronous."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        rc = self.lib.cim_init(ctypes.byref(self.h), int(device))
+        if rc != 0:
+            raise CimError(f"cim_init(device={device}) failed with {rc} (no HIP device?)")
+
+    def close(self):
+        if self.h:
+            self.lib.cim_finalize(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.cim_last_error(self.h)
+            raise CimError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.cim_get_stream(self.h) or 0
+
+    def sync(self):
+        self._check(self.lib.cim_sync(self.h), "cim_sync")
+
+    def timing(self, on: bool):
+        self._check(self.lib.cim_timing(self.h, int(on)), "cim_timing")
+
+    def timing_get(self, name: str):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(self.lib.cim_timing_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+                    "cim_timing_get")
+        return ms.value, n.value
+
+    # -- FFT -------------------------------------------------------------
+    def fft2d(self, re: np.ndarray, im: np.ndarray, direction: int = 1, conj_out: bool = False):
+        re, im = _f32(re).copy(), _f32(im).copy()
+        n = re.shape[-1]
+        batch = re.size // (n * n)
+        self._check(self.lib.cim_fft2d(self.h, _fptr(re), _fptr(im), n, batch, direction, int(conj_out)),
+                    "cim_fft2d")
+        return re, im
+
+    def fft3d(self, re: np.ndarray, im: np.ndarray, direction: int = 1, conj_out: bool = False):
+        re, im = _f32(re).copy(), _f32(im).copy()
+        n = re.shape[-1]
+        self._check(self.lib.cim_fft3d(self.h, _fptr(re), _fptr(im), n, direction, int(conj_out)), "cim_fft3d")
+        return re, im
+
+    # -- reconstruction ---------------------------------------------------
+    def rec_begin(self, box: int, maxR: int, layout: int = 0, rcut: Optional[float] = None,
+                  wg_init: float = 0.01, ops: Optional[np.ndarray] = None):
+        if ops is None:
+            nops, optr = 60, None
+        else:
+            o = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 9)
+            self._ops_keep = o
+            nops, optr = o.shape[0], o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rc = self.lib.cim_rec_begin(self.h, box, maxR, layout, float(maxR if rcut is None else rcut),
+                                    float(wg_init), nops, optr)
+        self._check(rc, "cim_rec_begin")
+
+    def rec_insert(self, params: CimParams, images, parts: np.ndarray):
+        parts = _f32(parts).reshape(-1, PART_STRIDE)
+        n = parts.shape[0]
+        if hasattr(images, "data_ptr"):       # device tensor
+            rc = self.lib.cim_rec_insert_dev(self.h, ctypes.byref(params), n, ctypes.c_void_p(images.data_ptr()),
+                                             _fptr(parts))
+        else:
+            im = _f32(images)
+            rc = self.lib.cim_rec_insert(self.h, ctypes.byref(params), n, _fptr(im), _fptr(parts))
+        self._check(rc, "cim_rec_insert")
+
+    def rec_insert_ft(self, ft_re, ft_im, euler, ctf=None):
+        re, im, eu = _f32(ft_re), _f32(ft_im), _f32(euler).reshape(-1, 3)
+        c = None if ctf is None else _f32(ctf)
+        rc = self.lib.cim_rec_insert_ft(self.h, eu.shape[0], _fptr(re), _fptr(im),
+                                        None if c is None else _fptr(c), _fptr(eu))
+        self._check(rc, "cim_rec_insert_ft")
+
+    def rec_get_tnf(self, maxR: int):
+        n = (2 * maxR + 1) ** 2 * (maxR + 1)
+        fa, fb, wg = (np.empty(n, np.float32) for _ in range(3))
+        self._check(self.lib.cim_rec_get_tnf(self.h, _fptr(fa), _fptr(fb), _fptr(wg)), "cim_rec_get_tnf")
+        shape = (maxR + 1, 2 * maxR + 1, 2 * maxR + 1)
+        return fa.reshape(shape), fb.reshape(shape), wg.reshape(shape)
+
+    def rec_add_tnf(self, fa, fb, wg):
+        fa, fb, wg = _f32(fa), _f32(fb), _f32(wg)
+        self._check(self.lib.cim_rec_add_tnf(self.h, _fptr(fa), _fptr(fb), _fptr(wg)), "cim_rec_add_tnf")
+
+    def rec_finish(self, box: int, out=None):
+        if out is not None and hasattr(out, "data_ptr"):
+            self._check(self.lib.cim_rec_finish_dev(self.h, ctypes.c_void_p(out.data_ptr())), "cim_rec_finish_dev")
+            return out
+        m = np.empty((box, box, box), np.float32)
+        self._check(self.lib.cim_rec_finish(self.h, _fptr(m)), "cim_rec_finish")
+        return m
+
+    # -- search -------------------------------------------------------------
+    def search_templates(self, params: CimParams, images, tort):
+        im = _f32(images)
+        to = _f32(tort).reshape(-1, 5)
+        self._check(self.lib.cim_search_templates(self.h, ctypes.byref(params), to.shape[0], _fptr(im), _fptr(to)),
+                    "cim_search_templates")
+
+    def search_global(self, params: CimParams, images, parts: np.ndarray) -> np.ndarray:
+        parts = _f32(parts).reshape(-1, PART_STRIDE).copy()
+        n = parts.shape[0]
+        if hasattr(images, "data_ptr"):
+            rc = self.lib.cim_search_global_dev(self.h, ctypes.byref(params), n, ctypes.c_void_p(images.data_ptr()),
+                                                _fptr(parts))
+        else:
+            im = _f32(images)
+            rc = self.lib.cim_search_global(self.h, ctypes.byref(params), n, _fptr(im), _fptr(parts))
+        self._check(rc, "cim_search_global")
+        return parts
+
+    def search_pr_array(self, params: CimParams, image, part, shape) -> np.ndarray:
+        im = _f32(image)
+        pa = _f32(part).reshape(PART_STRIDE)
+        out = np.empty(int(np.prod(shape)), np.float32)
+        self._check(self.lib.cim_search_pr_array(self.h, ctypes.byref(params), _fptr(im), _fptr(pa), _fptr(out)),
+                    "cim_search_pr_array")
+        return out.reshape(shape)
+
+    def refine_local(self, params: CimParams, images, parts: np.ndarray) -> np.ndarray:
+        parts = _f32(parts).reshape(-1, PART_STRIDE).copy()
+        im = _f32(images)
+        self._check(self.lib.cim_refine_local(self.h, ctypes.byref(params), parts.shape[0], _fptr(im), _fptr(parts)),
+                    "cim_refine_local")
+        return parts

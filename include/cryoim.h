@@ -1,0 +1,160 @@
+/*
+ * cryoim.h -- C ABI of libcryoim_hip.so, the MI355X (gfx950) implementation
+ * of cryoIM's icosahedral search and direct Fourier inversion hot path.
+ *
+ * Conventions (all kept from the reference):
+ *   - images are box*box row-major f32 with x fastest: the memory order of
+ *     the reference's flat a(i), i = ky*box + hx + 1;
+ *   - Fourier data cross the boundary as split re/im arrays (a, b), centred
+ *     (DC at index box/2 on each axis), forward transforms scaled 1/N;
+ *   - angles in degrees, ZYZ (euler_mat.f90:3-21); shifts in pixels.
+ * Every call is synchronous at the boundary, returns 0 on success or a
+ * negative CIM_E* code, and never aborts; cim_last_error() explains the
+ * failure.  The caller owns every host array; the library owns every
+ * device buffer it allocates inside the context.  Pointers marked "dev" in
+ * the *_dev variants are device pointers (hipMalloc memory of the device
+ * the context was created on).
+ *
+ * Reference interfaces replaced (citations into chenwenyuan2009/cryoIM):
+ *   FFT layer ........ icosprocess/SRC/FFTW.f90:179-219, img2d_FFT.f90:4-32,
+ *                      local_refine/SRC/img3d_FFT.f90:2-31
+ *   insert ........... icosprocess/SRC/reconstruct.f90:53-149 (and
+ *                      reconstruct_substep.f90, reconstruct_half.f90)
+ *   fold + inversion . icosprocess/SRC/reconstruct.f90:151-247
+ *   global search .... icosprocess/SRC/ort_globesearch.f90:34-134
+ *   local refinement . icosprocess/SRC/ort_refinement.f90:31-124
+ *   slice + PR ....... local_refine/SRC/refine_PR.f90:69-245
+ */
+#ifndef CRYOIM_H
+#define CRYOIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CIM_OK 0
+#define CIM_E_ARG -1      /* invalid argument or size */
+#define CIM_E_HIP -2      /* HIP runtime error */
+#define CIM_E_STATE -3    /* call out of order (e.g. insert before begin) */
+#define CIM_E_NOMEM -4    /* device allocation failed */
+
+typedef struct cim_ctx cim_ctx;
+
+/* Scalar parameters shared by the entry points.  Field meaning follows the
+ * reference's key=value parameters after calc_initialvalues
+ * (icosprocess/SRC/0calc_initialvalues.f90:1-21). */
+typedef struct cim_params {
+    int box;          /* FFTsize (image edge, pixels) */
+    int minR;         /* inner search radius (Fourier pixels) */
+    int maxR;         /* outer search / insert radius (Fourier pixels) */
+    int apply_ctf;    /* 1: multiply by getCTF2d (applyCTF=y) */
+    int mask_on;      /* 1: soft circular mask (imgmask given/default y) */
+    int norm_on;      /* 1: zero-mean unit-RMS normalisation before masking */
+    int ncycle;       /* refinement cycles (Ncycle) */
+    int conj_out;     /* FFT stores b = -Im (1, icosprocess) or +Im (0, local_refine) */
+    int reserved_i[8];
+    float apix;       /* pixel size (A) */
+    float imgmask;    /* mask radius in pixels */
+    float vol;        /* accelerating voltage (kV) */
+    float cs;         /* spherical aberration (mm) */
+    float ampwg;      /* amplitude contrast */
+    float df_scale;   /* defocus unit to A: 1e4 (um, icosprocess) or 1 (A) */
+    float searchstep; /* global search grid step (deg) */
+    float delta_angle;/* refinement angle step (deg) */
+    float delta_trans;/* refinement shift step (px) */
+    float reserved_f[7];
+} cim_params;
+
+/* Particle records: CIM_PART_STRIDE floats per particle. */
+#define CIM_PART_STRIDE 12
+#define CIM_P_THETA 0
+#define CIM_P_PHI 1
+#define CIM_P_OMEGA 2
+#define CIM_P_X 3
+#define CIM_P_Y 4
+#define CIM_P_PR 5
+#define CIM_P_DF1 6
+#define CIM_P_DF2 7
+#define CIM_P_ASTIG 8
+#define CIM_P_MASKX 9     /* mask centre x (see cim_rec_insert) */
+#define CIM_P_MASKY 10
+#define CIM_P_USE 11      /* 1.0 = insert, 0.0 = skip (PR/boundX filter) */
+
+/* ---- context ---------------------------------------------------------- */
+int cim_init(cim_ctx** ctx, int device);
+int cim_finalize(cim_ctx* ctx);
+const char* cim_last_error(const cim_ctx* ctx);
+/* hipStream_t every kernel of this context runs on. */
+void* cim_get_stream(cim_ctx* ctx);
+int cim_sync(cim_ctx* ctx);
+/* Average duration (ms) of the named kernel family over the launches timed
+ * since the last reset; timing is off until cim_timing(ctx, 1). */
+int cim_timing(cim_ctx* ctx, int on);
+int cim_timing_get(cim_ctx* ctx, const char* name, double* avg_ms, long long* launches);
+
+/* ---- FFT: replaces FFTcmplx2d_f/B, FFTcmplx3d_f/B and img2d_FFT ------- */
+/* dir = 1 forward (scaled 1/N), -1 backward (unscaled).  conj_out = 1
+ * negates the imaginary part of the result (img2d_FFT.f90:19).  Sizes:
+ * any n whose prime factors are 2, 3, 5 and n <= 1024. */
+int cim_fft2d(cim_ctx* ctx, float* re, float* im, int n, int batch, int dir, int conj_out);
+int cim_fft3d(cim_ctx* ctx, float* re, float* im, int n, int dir, int conj_out);
+/* Same on interleaved complex device data (n*n*batch or n^3 float2). */
+int cim_fft2d_dev(cim_ctx* ctx, void* data, int n, int batch, int dir, int conj_out);
+int cim_fft3d_dev(cim_ctx* ctx, void* data, int n, int dir, int conj_out);
+
+/* ---- direct Fourier inversion ---------------------------------------- */
+/* layout: 0 = full box accumulator as reconstruct.f90 (offset box/2),
+ *         1 = compact tnf accumulator as reconstruct_substep.f90 (offset maxR).
+ * rcut: radius cut of inserted samples (maxR, maxR-1 or maxR-2 in the
+ * reference variants).  wg_init: initial weight (0.01 reconstruct.f90:28,
+ * 0 substep).  ops: nops row-major 3x3 float64 symmetry operators, or NULL
+ * with nops = 60 for the icosahedral group (icos2f_sym_mat order). */
+int cim_rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init,
+                  int nops, const double* ops);
+/* Full per-particle pipeline of reconstruct.f90:77-147: optional norm, CTF,
+ * mask (centre CIM_P_MASKX/Y, which the caller sets to the first particle's
+ * shift to reproduce reconstruct.f90:92), checkerboard, FFT, centre shift
+ * by (x, y), 60-fold insert.  images: n*box*box real.  parts: n records. */
+int cim_rec_insert(cim_ctx* ctx, const cim_params* p, int n, const float* images, const float* parts);
+int cim_rec_insert_dev(cim_ctx* ctx, const cim_params* p, int n, const void* images_dev,
+                       const float* parts);
+/* Insert already-transformed particles: centred FT split arrays (n*box*box),
+ * optional CTF images (n*box*box, NULL = 1) and angles (n*3). */
+int cim_rec_insert_ft(cim_ctx* ctx, int n, const float* ft_re, const float* ft_im,
+                      const float* ctf, const float* euler);
+/* Copy the accumulated sums out in tnf order ((2maxR+1)^2 (maxR+1) each,
+ * lz slowest, hx fastest; reconstruct_substep.f90:148-155). */
+int cim_rec_get_tnf(cim_ctx* ctx, float* fa, float* fb, float* wg);
+/* Add tnf-order partial sums into the accumulator (merge of several
+ * reconstruct_substep outputs, merge_reconstruct.f90:23-34). */
+int cim_rec_add_tnf(cim_ctx* ctx, const float* fa, const float* fb, const float* wg);
+/* Fold of the lz=0 plane, division by the weights (float64), (-1)^(h+k+l),
+ * Hermitian completion, inverse 3D FFT and checkerboard: the real map,
+ * box^3 f32 (reconstruct.f90:151-247). */
+int cim_rec_finish(cim_ctx* ctx, float* map);
+int cim_rec_finish_dev(cim_ctx* ctx, void* map_dev);
+
+/* ---- icosprocess global search (mode 4) and refinement (mode 5) ------- */
+/* Template bank of load_template.f90:18-48: T real images (templatestck)
+ * with their ortfile angles and shifts (tort: T*5 theta phi omega x y);
+ * masked, checkerboarded, transformed and centre-shifted on the device. */
+int cim_search_templates(cim_ctx* ctx, const cim_params* p, int T, const float* images,
+                         const float* tort);
+/* Exhaustive common-lines phase-residual search over the AU grid
+ * (ort_globesearch.f90:34-134).  parts in/out: theta, phi, omega and PR are
+ * replaced by the search result; x, y are used as given. */
+int cim_search_global(cim_ctx* ctx, const cim_params* p, int n, const float* images, float* parts);
+int cim_search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const void* images_dev,
+                          float* parts);
+/* Full phase-residual map of one particle (Nomega x (2Nphi+1) x (Ntheta+1),
+ * omega fastest, 1000 outside the AU) -- the PR_array of ort_globesearch. */
+int cim_search_pr_array(cim_ctx* ctx, const cim_params* p, const float* image, const float* part,
+                        float* pr_array);
+/* Coordinate-descent refinement of orientation and centre
+ * (ort_refinement.f90:31-124, ort_cent_11.f90, return_icos_asymmetry_unit.f90). */
+int cim_refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* images, float* parts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,59 @@
+#!/bin/bash
+# Builds the reference's own Fortran programs (icosprocess, local_refine)
+# from the sources where they lie under /root/reference, plus the unit
+# drivers in oracle/ref_drivers/ that call individual reference routines.
+# Everything it writes goes to oracle/_ref/ (git-ignored).  Nothing here is
+# copied into the repository.  Needs amdflang (ROCm) and MKL's FFTW3
+# wrappers (libmkl_rt from /opt/conda/lib).  local_reconstruct is not built:
+# its 0input_parameters.f90:32 needs an ifort extension flang rejects.
+set -euo pipefail
+REF=${CIM_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+FC=${FC:-amdflang}
+MKL=${MKL_DIR:-/opt/conda/lib}
+[ -d "$REF/icosprocess/SRC" ] || { echo "reference sources not found at $REF"; exit 3; }
+mkdir -p "$OUT/icos" "$OUT/lref"
+FFLAGS="-O2"
+LIBS="-L$MKL -lmkl_rt -Wl,-rpath,$MKL"
+
+build_objs() {   # $1 = object dir, rest = sources (modules first)
+  local od=$1; shift
+  for f in "$@"; do
+    local o="$od/$(basename "${f%.f90}").o"
+    if [ ! -f "$o" ] || [ "$f" -nt "$o" ]; then
+      $FC $FFLAGS -J "$od" -c "$f" -o "$o"
+    fi
+  done
+}
+
+# icosprocess: modules first, then everything else in SRC and icosprocess/
+S=$REF/icosprocess/SRC
+ICOS_SRCS=("$S/0common_icosproc.f90" "$S/FFTW.f90")
+for f in "$S"/*.f90 "$REF"/icosprocess/icosprocess/*.f90; do
+  case "$(basename "$f")" in 0common_icosproc.f90|FFTW.f90) ;; *) ICOS_SRCS+=("$f");; esac
+done
+build_objs "$OUT/icos" "${ICOS_SRCS[@]}"
+ICOS_LIBOBJS=$(ls "$OUT"/icos/*.o | grep -v '/0main.o$')
+$FC $FFLAGS -o "$OUT/icosprocess" $ICOS_LIBOBJS "$OUT/icos/0main.o" $LIBS
+
+# local_refine
+L=$REF/local_refine/SRC
+LREF_SRCS=("$L/0common_loc_refine.f90" "$L/FFTW.f90")
+for f in "$L"/*.f90; do
+  case "$(basename "$f")" in 0common_loc_refine.f90|FFTW.f90) ;; *) LREF_SRCS+=("$f");; esac
+done
+build_objs "$OUT/lref" "${LREF_SRCS[@]}"
+LREF_LIBOBJS=$(ls "$OUT"/lref/*.o | grep -v '/0main.o$')
+$FC $FFLAGS -o "$OUT/local_refine" $LREF_LIBOBJS "$OUT/lref/0main.o" $LIBS
+
+# unit drivers (own code) linked against the reference objects
+for d in "$HERE"/ref_drivers/icos_*.f90; do
+  [ -f "$d" ] || continue
+  $FC $FFLAGS -J "$OUT/icos" -o "$OUT/$(basename "${d%.f90}")" "$d" $ICOS_LIBOBJS $LIBS
+done
+for d in "$HERE"/ref_drivers/lref_*.f90; do
+  [ -f "$d" ] || continue
+  $FC $FFLAGS -J "$OUT/lref" -o "$OUT/$(basename "${d%.f90}")" "$d" $LREF_LIBOBJS $LIBS
+done
+echo "reference built into $OUT"

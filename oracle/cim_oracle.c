@@ -1,0 +1,890 @@
+/*
+ * cim_oracle.c -- CPU restatement of cryoIM's icosahedral search and
+ * direct-Fourier-inversion arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library
+ * is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product path never does.
+ *
+ * It is compiled twice by oracle/Makefile:
+ *   liboracle64.so  REAL=double  (the FP64 oracle every error is stated against)
+ *   liboracle32.so  REAL=float   (FP32-faithful: same operation order as the
+ *                                 Fortran, which computes in default REAL*4)
+ * FFTs are always evaluated in double precision (own mixed-radix FFT,
+ * pinned by a brute-force DFT in the tests); the FFT provider of the
+ * reference (Intel MKL's FFTW3 wrapper) is not available, so FFT rounding
+ * parity is "unpinned" and only the convention (sign, 1/N scaling,
+ * conjugated output) is restated.
+ *
+ * Layout: images are nx*nx row-major with the x index fastest, i.e. the
+ * same memory order as the reference's flat column-major a(i),
+ * i = ky*nx + hx + 1.  Complex data are split re/im arrays (a, b), exactly
+ * as in the reference.  Rotation matrices are row-major R[3*i+j] =
+ * RotMat(i+1, j+1).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL double
+#endif
+
+#define EXPORT __attribute__((visibility("default")))
+
+typedef REAL real;
+
+static real r_pi(void) { return (real)4.0 * (real)atan2((real)1.0, (real)1.0); }
+static real r_deg2rad(void) { return (real)atan2((real)1.0, (real)1.0) / (real)45.0; }
+
+EXPORT int o_real_size(void) { return (int)sizeof(real); }
+
+/* ------------------------------------------------------------------ */
+/* Rotations                                                           */
+/* ------------------------------------------------------------------ */
+
+/* euler_mat: icosprocess/SRC/euler_mat.f90:3-21 (ZYZ, degrees). */
+EXPORT void o_euler_mat(real theta0, real phi0, real omega0, real* R)
+{
+    real d2r = r_deg2rad();
+    real th = theta0 * d2r, ph = phi0 * d2r, om = omega0 * d2r;
+    real c1 = (real)cos(ph), c2 = (real)cos(th), c3 = (real)cos(om);
+    real s1 = (real)sin(ph), s2 = (real)sin(th), s3 = (real)sin(om);
+    R[0] = c1 * c2 * c3 - s1 * s3;  R[1] = -c1 * c2 * s3 - c3 * s1; R[2] = c1 * s2;
+    R[3] = c2 * c3 * s1 + c1 * s3;  R[4] = -c2 * s1 * s3 + c1 * c3; R[5] = s1 * s2;
+    R[6] = -c3 * s2;                R[7] = s2 * s3;                 R[8] = c2;
+}
+
+/* 60 icosahedral rotations, 2-fold axes on x, y, z and 5-fold axes along
+ * (+-1, 0, +-phi) and its cyclic permutations -- the orientation of
+ * icos2f_sym_mat (icosprocess/SRC/icos2f_sym_mat.f90:1-248).  The group is
+ * generated here by closure of two generators and then put into the
+ * reference's element order with an order table; see cim_symops.h. */
+#include "../cryoim_amd/csrc/cim_symops.h"
+
+EXPORT void o_icos_ops(real* ops)
+{
+    double m[540];
+    cim_icos60_ref_order(m);
+    for (int i = 0; i < 540; ++i) ops[i] = (real)m[i];
+}
+
+static void matvec(const real* M, const real* x, real* y)
+{
+    for (int i = 0; i < 3; ++i) y[i] = M[3 * i] * x[0] + M[3 * i + 1] * x[1] + M[3 * i + 2] * x[2];
+}
+static void mattvec(const real* M, const real* x, real* y)
+{
+    for (int i = 0; i < 3; ++i) y[i] = M[i] * x[0] + M[3 + i] * x[1] + M[6 + i] * x[2];
+}
+static void matmul3(const real* A, const real* B, real* C)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+/* icos_crosslin_angle: icosprocess/SRC/icos_crosslin_angle.f90:2-37.
+ * nops = 60 (icosahedral) or the number of rows of ops.  alpha = 1000 marks
+ * a skipped line (|z1 x S z2| < 0.5, :25-29). */
+EXPORT void o_crosslin_angle(real th1, real ph1, real om1, real th2, real ph2, real om2,
+                             int nops, const real* ops, real* alpha1, real* alpha2)
+{
+    real r2d = (real)45.0 / (real)atan2((real)1.0, (real)1.0);
+    real E1[9], E2[9], z0[3] = {0, 0, 1}, z1[3], z2[3];
+    o_euler_mat(th1, ph1, om1, E1);
+    o_euler_mat(th2, ph2, om2, E2);
+    matvec(E1, z0, z1);
+    matvec(E2, z0, z2);
+    for (int i = 0; i < nops; ++i) {
+        const real* S = ops + 9 * i;
+        real z2a[3], v[3], v10[3], t[3], v20[3];
+        matvec(S, z2, z2a);
+        v[0] = z1[1] * z2a[2] - z1[2] * z2a[1];
+        v[1] = z1[2] * z2a[0] - z1[0] * z2a[2];
+        v[2] = z1[0] * z2a[1] - z1[1] * z2a[0];
+        real vv = (real)sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (vv < (real)0.5) { alpha1[i] = 1000; alpha2[i] = 1000; continue; }
+        v[0] /= vv; v[1] /= vv; v[2] /= vv;
+        mattvec(E1, v, v10);
+        mattvec(S, v, t);
+        mattvec(E2, t, v20);
+        alpha1[i] = (real)atan2(v10[1], v10[0]) * r2d;
+        alpha2[i] = (real)atan2(v20[1], v20[0]) * r2d;
+    }
+}
+
+/* Asymmetric-unit test used by ort_globesearch.f90:17-18,80-82 and
+ * return_icos_asymmetry_unit.f90:9-12,22-27. */
+static int in_au(real x, real y, real z)
+{
+    real d2r = r_deg2rad();
+    real s31 = (real)sin((real)31.717 * d2r), c31 = (real)cos((real)31.717 * d2r);
+    real s69 = (real)sin((real)69.09 * d2r), c69 = (real)cos((real)69.09 * d2r);
+    real A1 = -s31 * c69, B1 = -c31 * c69, C1 = s31 * s69;
+    real A2 = -s31 * c69, B2 = c31 * c69, C2 = s31 * s69;
+    real z1 = -(A1 * x + B1 * y) / C1;
+    real z2 = -(A2 * x + B2 * y) / C2;
+    return (z >= 0) && (z <= z1) && (z <= z2);
+}
+
+/* return_icos_asymmetry_unit: icosprocess/SRC/return_icos_asymmetry_unit.f90:1-33.
+ * Falls through to the last operator when none lands in the AU (:17-29). */
+EXPORT void o_return_au(real* theta, real* phi, real* omega)
+{
+    real ops[540], E[9], M[9];
+    real d2r = r_deg2rad();
+    o_icos_ops(ops);
+    o_euler_mat(*theta, *phi, *omega, E);
+    real th = 0, ph = 0, om = 0;
+    for (int i = 0; i < 60; ++i) {
+        matmul3(ops + 9 * i, E, M);
+        th = (real)acos(M[8]);
+        ph = (real)atan2(M[5], M[2]);
+        om = (real)atan2(M[7], -M[6]);
+        real x = (real)sin(th) * (real)cos(ph), y = (real)sin(th) * (real)sin(ph), z = (real)cos(th);
+        if (in_au(x, y, z)) break;
+    }
+    *theta = th / d2r;
+    *phi = ph / d2r;
+    *omega = om / d2r;
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT (double precision, own mixed radix)                             */
+/* ------------------------------------------------------------------ */
+
+/* out[k] = sum_j in[j*stride] exp(sign*2*pi*i*j*k/n); recursive
+ * decimation in time over the smallest prime factor. */
+static void dft_rec(const double* inr, const double* ini, int n, int stride, int sign,
+                    double* outr, double* outi, double* tmpr, double* tmpi)
+{
+    if (n == 1) { outr[0] = inr[0]; outi[0] = ini[0]; return; }
+    int p = 0;
+    for (int f = 2; f * f <= n; ++f) if (n % f == 0) { p = f; break; }
+    if (p == 0) p = n;                       /* n prime: direct DFT */
+    int m = n / p;
+    if (m == 1) {
+        for (int k = 0; k < n; ++k) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < n; ++j) {
+                double ang = sign * 2.0 * M_PI * (double)((long long)j * k % n) / n;
+                double c = cos(ang), s = sin(ang);
+                sr += inr[j * stride] * c - ini[j * stride] * s;
+                si += inr[j * stride] * s + ini[j * stride] * c;
+            }
+            outr[k] = sr; outi[k] = si;
+        }
+        return;
+    }
+    /* sub-transforms of the p decimated sequences, stored in tmp */
+    for (int q = 0; q < p; ++q)
+        dft_rec(inr + q * stride, ini + q * stride, m, stride * p, sign,
+                tmpr + q * m, tmpi + q * m, outr, outi);
+    for (int k = 0; k < n; ++k) {
+        double sr = 0, si = 0;
+        for (int q = 0; q < p; ++q) {
+            double ang = sign * 2.0 * M_PI * (double)((long long)q * k % n) / n;
+            double c = cos(ang), s = sin(ang);
+            double xr = tmpr[q * m + (k % m)], xi = tmpi[q * m + (k % m)];
+            sr += xr * c - xi * s;
+            si += xr * s + xi * c;
+        }
+        outr[k] = sr; outi[k] = si;
+    }
+}
+
+/* In-place 1D transform of n points at stride s, count lines at line stride ls. */
+static void fft_lines(double* re, double* im, int n, long s, long count, long ls, int sign)
+{
+    double* buf = (double*)malloc(sizeof(double) * 6 * n);
+    double *ir = buf, *ii = buf + n, *orr = buf + 2 * n, *oi = buf + 3 * n, *tr = buf + 4 * n, *ti = buf + 5 * n;
+    for (long l = 0; l < count; ++l) {
+        double* r = re + l * ls;
+        double* i = im + l * ls;
+        for (int j = 0; j < n; ++j) { ir[j] = r[j * s]; ii[j] = i[j * s]; }
+        dft_rec(ir, ii, n, 1, sign, orr, oi, tr, ti);
+        for (int j = 0; j < n; ++j) { r[j * s] = orr[j]; i[j * s] = oi[j]; }
+    }
+    free(buf);
+}
+
+/* Generic d-dimensional cube transform (d = 1, 2, 3) of edge n; sign -1 is
+ * FFTW_FORWARD, +1 FFTW_BACKWARD.  Unscaled. */
+EXPORT void o_dft_cube(double* re, double* im, int n, int d, int sign)
+{
+    long n2 = (long)n * n;
+    if (d == 1) { fft_lines(re, im, n, 1, 1, n, sign); return; }
+    if (d == 2) {
+        fft_lines(re, im, n, 1, n, n, sign);          /* rows (x)    */
+        fft_lines(re, im, n, n, n, 1, sign);          /* columns (y) */
+        return;
+    }
+    fft_lines(re, im, n, 1, n2, n, sign);             /* x */
+    for (int z = 0; z < n; ++z)
+        fft_lines(re + z * n2, im + z * n2, n, n, n, 1, sign);   /* y */
+    fft_lines(re, im, n, n2, n2, 1, sign);            /* z */
+}
+
+/* Brute-force DFT of a 2D image, for pinning the FFT in tests. */
+EXPORT void o_dft2d_brute(const double* re, const double* im, int n, int sign, double* ore, double* oim)
+{
+    for (int ky = 0; ky < n; ++ky)
+        for (int kx = 0; kx < n; ++kx) {
+            double sr = 0, si = 0;
+            for (int y = 0; y < n; ++y)
+                for (int x = 0; x < n; ++x) {
+                    long ph = ((long)kx * x + (long)ky * y) % n;
+                    double ang = sign * 2.0 * M_PI * (double)ph / n;
+                    double c = cos(ang), s = sin(ang);
+                    sr += re[y * n + x] * c - im[y * n + x] * s;
+                    si += re[y * n + x] * s + im[y * n + x] * c;
+                }
+            ore[ky * n + kx] = sr; oim[ky * n + kx] = si;
+        }
+}
+
+/* img2d_FFT: icosprocess/SRC/img2d_FFT.f90:4-32 with FFTW.f90:179-193.
+ * dir = 1: forward, scaled 1/N; dir = -1: backward, unscaled.
+ * conj_out = 1 stores b = -Im (icosprocess, local_reconstruct),
+ * conj_out = 0 stores b = +Im (local_refine/SRC/img2d_FFT.f90:19). */
+EXPORT void o_img2d_fft(real* a, real* b, int nx, int dir, int conj_out)
+{
+    long N = (long)nx * nx;
+    double* re = (double*)malloc(sizeof(double) * N);
+    double* im = (double*)malloc(sizeof(double) * N);
+    for (long i = 0; i < N; ++i) { re[i] = a[i]; im[i] = b[i]; }
+    o_dft_cube(re, im, nx, 2, dir == 1 ? -1 : 1);
+    double sc = dir == 1 ? 1.0 / (double)N : 1.0;
+    for (long i = 0; i < N; ++i) {
+        a[i] = (real)(re[i] * sc);
+        b[i] = (real)(conj_out ? -im[i] * sc : im[i] * sc);
+    }
+    free(re); free(im);
+}
+
+/* img3d_FFT / FFTcmplx3d_f/B: local_refine/SRC/img3d_FFT.f90:2-31,
+ * icosprocess/SRC/FFTW.f90:205-219. Same conventions as o_img2d_fft. */
+EXPORT void o_img3d_fft(real* a, real* b, int nx, int dir, int conj_out)
+{
+    long N = (long)nx * nx * nx;
+    double* re = (double*)malloc(sizeof(double) * N);
+    double* im = (double*)malloc(sizeof(double) * N);
+    for (long i = 0; i < N; ++i) { re[i] = a[i]; im[i] = b[i]; }
+    o_dft_cube(re, im, nx, 3, dir == 1 ? -1 : 1);
+    double sc = dir == 1 ? 1.0 / (double)N : 1.0;
+    for (long i = 0; i < N; ++i) {
+        a[i] = (real)(re[i] * sc);
+        b[i] = (real)(conj_out ? -im[i] * sc : im[i] * sc);
+    }
+    free(re); free(im);
+}
+
+/* ------------------------------------------------------------------ */
+/* Per-image operations                                                */
+/* ------------------------------------------------------------------ */
+
+/* getCTF2d: icosprocess/SRC/getCTF2d.f90:2-29.  df_scale = 1e4 for the
+ * icosprocess variant (defocus in um, :14-15), 1 for the local programs
+ * (local_refine/SRC/getCTF2d.f90:14-15, defocus in A). */
+EXPORT void o_getctf2d(int nx, real df10, real df20, real astig0, real apix, real vol,
+                       real Cs0, real ampwg, real df_scale, real* ctf)
+{
+    real PI = r_pi();
+    real d2r = PI / (real)180.0;
+    real lam = (real)0.387818 / (real)sqrt((real)1.0 * vol * ((real)1.0 + (real)0.978459E-3 * vol));
+    real Cs = Cs0 * (real)1.0e7;
+    real df1 = df10 * df_scale, df2 = df20 * df_scale;
+    real ast = astig0 * d2r;
+    real sq = (real)sqrt((real)1.0 - ampwg * ampwg);
+    for (long i = 0; i < (long)nx * nx; ++i) {
+        int ky = (int)(i / nx) - nx / 2;
+        int hx = (int)(i % nx) - nx / 2;
+        real ph = (real)atan2((real)ky, (real)hx);
+        if (ph < 0) ph = ph + (real)2.0 * PI;
+        ph = ph - ast;
+        real c = (real)cos(ph), s = (real)sin(ph);
+        real df = df1 * c * c + df2 * s * s;
+        real res = (real)sqrt((real)(hx * hx + ky * ky)) / (real)nx / apix;
+        real phase = (real)2.0 * PI * (-Cs * lam * lam * lam * res * res * res * res / (real)4 +
+                                       df * lam * res * res / (real)2);
+        ctf[i] = -(sq * (real)sin(phase) + ampwg * (real)cos(phase));
+    }
+}
+
+/* img2d_mask: icosprocess/SRC/img2d_mask.f90:2-49 (ring mean summed in
+ * double precision, cosine edge of half width 3 around the mask radius). */
+EXPORT void o_img2d_mask(real* d, int nx, real mask, real ox, real oy)
+{
+    int halfw = 6;
+    real pi = r_pi();
+    real R1 = mask - (real)(halfw / 2);
+    if (R1 <= 0) R1 = 0;
+    real R2 = mask + (real)(halfw / 2);
+    if (R2 > (real)(nx / 2)) R2 = (real)(nx / 2);
+    double summ = 0;
+    long num = 0;
+    real x0 = ox + (real)(nx / 2), y0 = oy + (real)(nx / 2);
+    for (long i = 0; i < (long)nx * nx; ++i) {
+        int iy = (int)(i / nx), ix = (int)(i % nx);
+        real yy = (-y0 + iy) * (-y0 + iy), xx = (-x0 + ix) * (-x0 + ix);
+        real ra = (real)sqrt(yy + xx);
+        if (ra > R1 && ra <= R2) { summ += d[i]; num++; }
+    }
+    if (num != 0) summ = summ / (double)(float)num;
+    for (long i = 0; i < (long)nx * nx; ++i) {
+        int iy = (int)(i / nx), ix = (int)(i % nx);
+        real yy = (-y0 + iy) * (-y0 + iy), xx = (-x0 + ix) * (-x0 + ix);
+        real ra = (real)sqrt(yy + xx);
+        if (ra > R1 && ra <= R2) {
+            real edge = ((real)1.0 + (real)cos(pi * (ra - R1) / (real)halfw)) / (real)2.0;
+            d[i] = d[i] * edge + ((real)1.0 - edge) * (real)summ;
+        }
+        if (ra > R2) d[i] = (real)summ;
+    }
+}
+
+/* img2d_norm: icosprocess/SRC/img2d_norm.f90:1-22. */
+EXPORT void o_img2d_norm(real* d, int nx)
+{
+    long N = (long)nx * nx;
+    real avg = 0, dev = 0;
+    for (long i = 0; i < N; ++i) avg += d[i];
+    avg = avg / (real)N;
+    for (long i = 0; i < N; ++i) d[i] -= avg;
+    for (long i = 0; i < N; ++i) dev += d[i] * d[i];
+    dev = (real)sqrt(dev / (real)N);
+    for (long i = 0; i < N; ++i) d[i] /= dev;
+}
+
+/* Multiply by (-1)^(x+y): the shift2d arrays of ort_globesearch.f90:25-29. */
+EXPORT void o_checker2d(real* d, int nx)
+{
+    for (long i = 0; i < (long)nx * nx; ++i)
+        if (((i / nx) + (i % nx)) & 1) d[i] = -d[i];
+}
+
+/* img2d_FT_centshift (recentre = 1; icosprocess/SRC/img2d_FT_centshift.f90:2-18,
+ * phase uses x0 + nx/2) and img2d_FTphase (recentre = 0;
+ * local_refine/SRC/img2d_FTphase.f90:1-22, phase uses x0). */
+EXPORT void o_ft_shift(real* a, real* b, int nx, real x0, real y0, int recentre)
+{
+    real pi = r_pi();
+    real xs = recentre ? x0 + (real)(nx / 2) : x0;
+    real ys = recentre ? y0 + (real)(nx / 2) : y0;
+    for (long i = 0; i < (long)nx * nx; ++i) {
+        int ky = (int)(i / nx) - nx / 2, hx = (int)(i % nx) - nx / 2;
+        real ph = (real)2.0 * pi * (xs * hx + ys * ky) / (real)nx;
+        real c = (real)cos(ph), s = (real)sin(ph);
+        real fr = a[i] * c + b[i] * s;
+        real fi = -a[i] * s + b[i] * c;
+        a[i] = fr; b[i] = fi;
+    }
+}
+
+/* FT_amp_expfit: icosprocess/SRC/FT_amp_expfit.f90:2-47.  Fcut has
+ * maxR-minR+1 entries for rings minR..maxR. */
+EXPORT void o_amp_expfit(const real* a, const real* b, int nx, int minR, int maxR, real* Fcut)
+{
+    int nr = maxR - minR + 1;
+    real* Favg = (real*)calloc(nr, sizeof(real));
+    real* resLN = (real*)calloc(nr, sizeof(real));
+    int* Num = (int*)calloc(nr, sizeof(int));
+    for (int ky = 0; ky <= maxR; ++ky)
+        for (int hx = -maxR; hx <= maxR; ++hx) {
+            real rad = (real)sqrt((real)(hx * hx + ky * ky));
+            int ir = (int)(rad + (real)0.5);
+            if (ir >= minR && ir <= maxR) {
+                long i = (long)(ky + nx / 2) * nx + (hx + nx / 2);
+                real amp = (real)sqrt(a[i] * a[i] + b[i] * b[i]);
+                Favg[ir - minR] += amp;
+                Num[ir - minR] += 1;
+                resLN[ir - minR] += rad;
+            }
+        }
+    for (int k = 0; k < nr; ++k) {
+        if (Num[k] > 0 && Favg[k] > 0) {
+            resLN[k] = resLN[k] / (real)Num[k];
+            Favg[k] = (real)log(Favg[k] / (real)Num[k]);
+        } else {
+            Favg[k] = 0;
+        }
+    }
+    /* 2-parameter normal equations for log|F| = x1*r + x2 (:36-42) */
+    real a11 = 0, a12 = 0, a22 = 0, b1 = 0, b2 = 0;
+    for (int k = 0; k < nr; ++k) {
+        a11 += resLN[k] * resLN[k];
+        a12 += resLN[k];
+        a22 += 1;
+        b1 += resLN[k] * Favg[k];
+        b2 += Favg[k];
+    }
+    real x2 = (b2 - b1 * a12 / a11) / (a22 - a12 * a12 / a11);
+    real x1 = (b1 - a12 * x2) / a11;
+    for (int k = 0; k < nr; ++k) Fcut[k] = (real)exp(resLN[k] * x1 + x2) * (real)0.8;
+    free(Favg); free(resLN); free(Num);
+}
+
+/* ------------------------------------------------------------------ */
+/* Common-lines phase residual (icosprocess search)                    */
+/* ------------------------------------------------------------------ */
+
+/* Bilinear sample of a centred FT at (x, y): img2d_fill_line.f90:17-31. */
+static void bilin(const real* a, const real* b, int nx, real x, real y, real* re, real* im)
+{
+    int lx = (int)x, ly = (int)y;
+    real ax = x - (real)lx, ay = y - (real)ly;
+    long n1 = (long)nx * ly + lx, n2 = (long)nx * (ly + 1) + lx;
+    *re = ((real)1.0 - ax) * ((real)1.0 - ay) * a[n1] + ax * ((real)1.0 - ay) * a[n1 + 1] +
+          ((real)1.0 - ax) * ay * a[n2] + ax * ay * a[n2 + 1];
+    *im = ((real)1.0 - ax) * ((real)1.0 - ay) * b[n1] + ax * ((real)1.0 - ay) * b[n1 + 1] +
+          ((real)1.0 - ax) * ay * b[n2] + ax * ay * b[n2 + 1];
+}
+
+/* Phase difference folded into [0, pi]: cal_crosslin_PR_omega.f90:57-60. */
+static real phase_diff(real re1, real im1, real re2, real im2)
+{
+    real pi = r_pi(), tpi = (real)2.0 * pi;
+    real p1 = (real)atan2(im1, re1), p2 = (real)atan2(im2, re2);
+    real diff = (real)fabs(p2 - p1);
+    diff = (real)fmod(diff, tpi);
+    if (diff > pi) diff = tpi - diff;
+    return diff;
+}
+
+/* cal_crosslin_PR_omega for one trial direction (theta, phi, omega=0):
+ * icosprocess/SRC/cal_crosslin_PR_omega.f90:3-82.
+ *   pa,pb     particle FT (masked, checkerboarded, FFT, centre-shifted)
+ *   Fcut0     particle Fcut (rings minR..maxR)
+ *   ta,tb     T template FTs, nx*nx each; tctf_fcut T Fcut rows used as-is
+ *             (the caller has applied the CTF and refitted Fcut when
+ *             applyCTF is on -- identical to the per-call refit at :34-38)
+ *   teul      T*3 template angles
+ *   PR        out, Nomega entries (999 where the gate leaves nothing) */
+EXPORT void o_crosslin_pr_omega(const real* pa, const real* pb, const real* Fcut0,
+                                int T, const real* ta, const real* tb, const real* tFcut,
+                                const real* teul, int nx, int minR, int maxR,
+                                int nops, const real* ops, real theta, real phi,
+                                int Nomega, real step, real* PR)
+{
+    real pi = r_pi();
+    real r2d = (real)180.0 / pi;
+    real d2r = r_deg2rad();
+    int nr = maxR - minR + 1;
+    real* PR2 = (real*)calloc(Nomega, sizeof(real));
+    real* lin2 = (real*)malloc(sizeof(real) * 2 * nr);
+    real* coss = (real*)malloc(sizeof(real) * Nomega);
+    real* sinn = (real*)malloc(sizeof(real) * Nomega);
+    real al1[64 * 6], al2[64 * 6];
+    for (int k = 0; k < Nomega; ++k) PR[k] = 0;
+    long N2 = (long)nx * nx;
+    real half = (real)(nx / 2);
+    for (int t = 0; t < T; ++t) {
+        const real* a2 = ta + t * N2;
+        const real* b2 = tb + t * N2;
+        const real* Fc = tFcut + t * nr;
+        o_crosslin_angle(theta, phi, (real)0.0, teul[3 * t], teul[3 * t + 1], teul[3 * t + 2],
+                         nops, ops, al1, al2);
+        for (int j = 0; j < nops; ++j) {
+            if (!(al1[j] < (real)999.0 && al2[j] < (real)999.0)) continue;
+            /* template line (img2d_fill_line.f90:13-31) */
+            for (int FR = minR; FR <= maxR; ++FR) {
+                real x = (real)FR * (real)cos(al2[j] * d2r) + half;
+                real y = (real)FR * (real)sin(al2[j] * d2r) + half;
+                bilin(a2, b2, nx, x, y, &lin2[2 * (FR - minR)], &lin2[2 * (FR - minR) + 1]);
+            }
+            /* particle line for every omega (img2d_fill_lin_omegearray.f90:13-37) */
+            for (int k = 0; k < Nomega; ++k) {
+                real om = al1[j] - step * (real)k;
+                coss[k] = (real)cos(om * d2r);
+                sinn[k] = (real)sin(om * d2r);
+            }
+            for (int FR = minR; FR <= maxR; ++FR) {
+                real t_re = lin2[2 * (FR - minR)], t_im = lin2[2 * (FR - minR) + 1];
+                real amp2 = (real)sqrt(t_re * t_re + t_im * t_im);
+                if (!(amp2 >= Fc[FR - minR])) continue;
+                for (int k = 0; k < Nomega; ++k) {
+                    real x = (real)FR * coss[k] + half;
+                    real y = (real)FR * sinn[k] + half;
+                    real p_re, p_im;
+                    bilin(pa, pb, nx, x, y, &p_re, &p_im);
+                    real amp1 = (real)sqrt(p_re * p_re + p_im * p_im);
+                    if (!(amp1 >= Fcut0[FR - minR])) continue;
+                    real diff = phase_diff(p_re, p_im, t_re, t_im);
+                    real w = (real)0.5 * (amp1 + amp2);
+                    PR[k] += diff * w;
+                    PR2[k] += w;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < Nomega; ++k)
+        PR[k] = PR2[k] > (real)1.0e-10 ? PR[k] / PR2[k] * r2d : (real)999.0;
+    free(PR2); free(lin2); free(coss); free(sinn);
+}
+
+/* Grid sizes of ort_globesearch.f90:19-21. */
+EXPORT void o_search_grid(real step, int* Ntheta, int* Nphi, int* Nomega)
+{
+    *Ntheta = (int)((real)21.0 / step);
+    *Nphi = (int)((real)32.0 / step);
+    *Nomega = (int)((real)360.0 / step);
+}
+
+/* AU membership of grid point (itheta, iphi): ort_globesearch.f90:68-82. */
+EXPORT int o_grid_in_au(real step, int itheta, int iphi)
+{
+    real d2r = r_deg2rad();
+    real theta = (real)69.0 + step * (real)itheta;
+    real tmp = theta * d2r;
+    real z = (real)cos(tmp), z0 = (real)sin(tmp);
+    real phi = step * (real)iphi;
+    tmp = phi * d2r;
+    return in_au(z0 * (real)cos(tmp), z0 * (real)sin(tmp), z);
+}
+
+/* ort_globesearch for one particle: icosprocess/SRC/ort_globesearch.f90:68-106.
+ * PR_array (Nomega x (2Nphi+1) x (Ntheta+1), omega fastest) is optional
+ * output (may be NULL).  Returns theta, phi, omega, PR of the first
+ * minimum in column-major order (Fortran minloc). */
+EXPORT void o_globesearch(const real* pa, const real* pb, const real* Fcut0,
+                          int T, const real* ta, const real* tb, const real* tFcut, const real* teul,
+                          int nx, int minR, int maxR, int nops, const real* ops, real step,
+                          real* out4, real* PR_array_out)
+{
+    int Nt, Np, No;
+    o_search_grid(step, &Nt, &Np, &No);
+    long plane = (long)No * (2 * Np + 1);
+    long total = plane * (Nt + 1);
+    real* arr = (real*)malloc(sizeof(real) * total);
+    real* pro = (real*)malloc(sizeof(real) * No);
+    for (long i = 0; i < total; ++i) arr[i] = (real)1000.0;
+    for (int it = 0; it <= Nt; ++it) {
+        real theta = (real)69.0 + step * (real)it;
+        for (int ip = 0; ip <= Np; ++ip) {
+            if (!o_grid_in_au(step, it, ip)) continue;
+            real phi = step * (real)ip;
+            o_crosslin_pr_omega(pa, pb, Fcut0, T, ta, tb, tFcut, teul, nx, minR, maxR, nops, ops,
+                                theta, phi, No, step, pro);
+            memcpy(arr + it * plane + (long)(ip + Np) * No, pro, sizeof(real) * No);
+            o_crosslin_pr_omega(pa, pb, Fcut0, T, ta, tb, tFcut, teul, nx, minR, maxR, nops, ops,
+                                theta, -phi, No, step, pro);
+            memcpy(arr + it * plane + (long)(-ip + Np) * No, pro, sizeof(real) * No);
+        }
+    }
+    long best = 0;
+    for (long i = 1; i < total; ++i) if (arr[i] < arr[best]) best = i;
+    int it = (int)(best / plane);
+    int ip = (int)((best % plane) / No) - Np;
+    int io = (int)(best % No);
+    out4[0] = (real)69.0 + step * (real)it;
+    out4[1] = step * (real)ip;
+    out4[2] = step * (real)io;
+    out4[3] = arr[best];
+    if (PR_array_out) memcpy(PR_array_out, arr, sizeof(real) * total);
+    free(arr); free(pro);
+}
+
+/* cal_crosslin_PR: icosprocess/SRC/cal_crosslin_PR.f90:2-71.  pa,pb is the
+ * particle FT without centre shift; the trial shift (x, y) is applied here
+ * (:22).  Returns 1000 when the gate leaves nothing (:64-68). */
+EXPORT real o_crosslin_pr(const real* pa, const real* pb, const real* Fcut0,
+                          int T, const real* ta, const real* tb, const real* tFcut, const real* teul,
+                          int nx, int minR, int maxR, int nops, const real* ops,
+                          real theta, real phi, real omega, real x, real y)
+{
+    real pi = r_pi();
+    real d2r = r_deg2rad();
+    long N2 = (long)nx * nx;
+    int nr = maxR - minR + 1;
+    real* a1 = (real*)malloc(sizeof(real) * N2);
+    real* b1 = (real*)malloc(sizeof(real) * N2);
+    memcpy(a1, pa, sizeof(real) * N2);
+    memcpy(b1, pb, sizeof(real) * N2);
+    o_ft_shift(a1, b1, nx, x, y, 1);
+    real al1[64 * 6], al2[64 * 6];
+    real PR1 = 0, PR2 = 0;
+    real half = (real)(nx / 2);
+    for (int t = 0; t < T; ++t) {
+        const real* a2 = ta + t * N2;
+        const real* b2 = tb + t * N2;
+        const real* Fc = tFcut + t * nr;
+        o_crosslin_angle(theta, phi, omega, teul[3 * t], teul[3 * t + 1], teul[3 * t + 2],
+                         nops, ops, al1, al2);
+        for (int j = 0; j < nops; ++j) {
+            if (!(al1[j] < (real)999.0 && al2[j] < (real)999.0)) continue;
+            for (int FR = minR; FR <= maxR; ++FR) {
+                real p_re, p_im, t_re, t_im;
+                bilin(a1, b1, nx, (real)FR * (real)cos(al1[j] * d2r) + half,
+                      (real)FR * (real)sin(al1[j] * d2r) + half, &p_re, &p_im);
+                bilin(a2, b2, nx, (real)FR * (real)cos(al2[j] * d2r) + half,
+                      (real)FR * (real)sin(al2[j] * d2r) + half, &t_re, &t_im);
+                real amp1 = (real)sqrt(p_re * p_re + p_im * p_im);
+                real amp2 = (real)sqrt(t_re * t_re + t_im * t_im);
+                if (amp1 >= Fcut0[FR - minR] && amp2 >= Fc[FR - minR]) {
+                    real diff = phase_diff(p_re, p_im, t_re, t_im);
+                    real w = (real)0.5 * (amp1 + amp2);
+                    PR1 += diff * w;
+                    PR2 += w;
+                }
+            }
+        }
+    }
+    free(a1); free(b1);
+    return PR2 > (real)1.0e-10 ? PR1 / PR2 * (real)180.0 / pi : (real)1000.0;
+}
+
+/* ort_cent_11 (icosprocess variant, with AU folding):
+ * icosprocess/SRC/ort_cent_11.f90:1-94.  oc is 11 x 5 (theta, phi, omega, x, y). */
+EXPORT void o_ort_cent_11(real th0, real ph0, real om0, real x0, real y0, real dang, real dtran, real* oc)
+{
+    o_return_au(&th0, &ph0, &om0);
+    for (int jj = 0; jj <= 10; ++jj) {
+        real* o = oc + 5 * jj;
+        o[0] = th0; o[1] = ph0; o[2] = om0; o[3] = x0; o[4] = y0;
+        switch (jj) {
+            case 1: o[0] = th0 - dang; break;
+            case 2: o[0] = th0 + dang; break;
+            case 3: o[1] = ph0 - dang; break;
+            case 4: o[1] = ph0 + dang; break;
+            case 5: o[2] = om0 - dang; break;
+            case 6: o[2] = om0 + dang; break;
+            case 7: o[3] = x0 - dtran; break;
+            case 8: o[3] = x0 + dtran; break;
+            case 9: o[4] = y0 - dtran; break;
+            case 10: o[4] = y0 + dtran; break;
+        }
+        o_return_au(&o[0], &o[1], &o[2]);
+    }
+}
+
+/* ort_refinement for one particle: icosprocess/SRC/ort_refinement.f90:69-95.
+ * io5 = (theta, phi, omega, x, y) in/out; returns PR0. */
+EXPORT real o_refine_particle(const real* pa, const real* pb, const real* Fcut0,
+                              int T, const real* ta, const real* tb, const real* tFcut, const real* teul,
+                              int nx, int minR, int maxR, int nops, const real* ops,
+                              int Ncycle, real deltaAngle, real deltaTrans, real* io5)
+{
+    real oc[55], PR[11];
+    real PR0 = 0;
+    for (int n = 1; n <= Ncycle; ++n) {
+        real f = 1;
+        for (int k = 1; k < n; ++k) f *= (real)0.8;
+        real dang = deltaAngle * f, dtran = deltaTrans * f;
+        o_ort_cent_11(io5[0], io5[1], io5[2], io5[3], io5[4], dang, dtran, oc);
+        for (int jj = 0; jj <= 10; ++jj)
+            PR[jj] = o_crosslin_pr(pa, pb, Fcut0, T, ta, tb, tFcut, teul, nx, minR, maxR, nops, ops,
+                                   oc[5 * jj], oc[5 * jj + 1], oc[5 * jj + 2], oc[5 * jj + 3], oc[5 * jj + 4]);
+        int best = 0;
+        for (int jj = 1; jj <= 10; ++jj) if (PR[jj] < PR[best]) best = jj;
+        PR0 = PR[best];
+        for (int q = 0; q < 5; ++q) io5[q] = oc[5 * best + q];
+    }
+    return PR0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Direct Fourier inversion                                            */
+/* ------------------------------------------------------------------ */
+
+/* Insert of one particle: icosprocess/SRC/reconstruct.f90:103-147 (layout
+ * full: volume nx^3, offset nx/2, radius cut rcut) and
+ * reconstruct_substep.f90:101-145 (layout compact: (2R+1)^2 (R+1), offset
+ * R = maxR).  a,b: centred particle FT after centre shift; ctf may be NULL
+ * (ctf = 1).  R: particle rotation (euler_mat). */
+EXPORT void o_insert(real* fa, real* fb, real* wg, int compact, int nx, int maxR, real rcut,
+                     const real* a, const real* b, const real* ctf, const real* R,
+                     int nops, const real* ops)
+{
+    long D = compact ? 2 * maxR + 1 : nx;
+    long D2 = D * D;
+    real off = compact ? (real)maxR : (real)(nx / 2);
+    for (int ky = -maxR; ky <= maxR; ++ky)
+        for (int hx = 0; hx <= maxR; ++hx) {
+            real dr = (real)sqrt((real)(hx * hx + ky * ky));
+            if (!(dr <= rcut)) continue;
+            long i2 = (long)(ky + nx / 2) * nx + (hx + nx / 2);
+            real x[3] = {(real)hx, (real)ky, 0}, x00[3];
+            matvec(R, x, x00);
+            real c = ctf ? ctf[i2] : (real)1.0;
+            for (int s = 0; s < nops; ++s) {
+                real x0[3];
+                matvec(ops + 9 * s, x00, x0);
+                int signz = 1;
+                if (x0[2] < 0) { x0[0] = -x0[0]; x0[1] = -x0[1]; x0[2] = -x0[2]; signz = -1; }
+                x0[0] += off; x0[1] += off;
+                if (!compact) x0[2] += off;
+                int lx[3] = {(int)x0[0], (int)x0[1], (int)x0[2]};
+                real ax[3] = {x0[0] - lx[0], x0[1] - lx[1], x0[2] - lx[2]};
+                real w[8];
+                w[0] = ((real)1 - ax[0]) * ((real)1 - ax[1]) * ((real)1 - ax[2]);
+                w[1] = ax[0] * ((real)1 - ax[1]) * ((real)1 - ax[2]);
+                w[2] = ((real)1 - ax[0]) * ax[1] * ((real)1 - ax[2]);
+                w[3] = ax[0] * ax[1] * ((real)1 - ax[2]);
+                w[4] = ((real)1 - ax[0]) * ((real)1 - ax[1]) * ax[2];
+                w[5] = ax[0] * ((real)1 - ax[1]) * ax[2];
+                w[6] = ((real)1 - ax[0]) * ax[1] * ax[2];
+                w[7] = ax[0] * ax[1] * ax[2];
+                long c1 = lx[2] * D2 + lx[1] * D + lx[0];
+                long id[8] = {c1, c1 + 1, c1 + D, c1 + D + 1, c1 + D2, c1 + D2 + 1, c1 + D2 + D, c1 + D2 + D + 1};
+                for (int j = 0; j < 8; ++j) {
+                    fa[id[j]] += a[i2] * w[j] * c;
+                    fb[id[j]] += b[i2] * w[j] * c * (real)signz;
+                    wg[id[j]] += w[j] * c * c;
+                }
+            }
+        }
+}
+
+/* Fold of the lz = 0 plane, division by the weights and (-1)^(h+k+l):
+ * icosprocess/SRC/reconstruct.f90:151-237.  Full nx^3 volumes in place. */
+EXPORT void o_fold_normalise(real* fa, real* fb, real* wg, int nx, int maxR)
+{
+    long n2 = (long)nx * nx;
+    int c = nx / 2;
+#define IDX(h, k, l) ((long)((l) + c) * n2 + (long)((k) + c) * nx + ((h) + c))
+    int lz = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        int kylo = pass == 0 ? 1 : 0, kyhi = pass == 0 ? maxR : 0;
+        for (int ky = kylo; ky <= kyhi; ++ky) {
+            int hxlo = pass == 0 ? -maxR : 1;
+            for (int hx = hxlo; hx <= maxR; ++hx) {
+                long n1 = IDX(hx, ky, lz), nn2 = IDX(-hx, -ky, -lz);
+                real dr = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+                if (dr <= (real)maxR) {
+                    wg[n1] = wg[n1] + wg[nn2];
+                    fa[n1] = (fa[n1] + fa[nn2]) / wg[n1];
+                    fb[n1] = (fb[n1] - fb[nn2]) / wg[n1];
+                    if (((hx + ky + lz) % 2) != 0) { fa[n1] = -fa[n1]; fb[n1] = -fb[n1]; }
+                    fa[nn2] = fa[n1];
+                    fb[nn2] = -fb[n1];
+                } else {
+                    fa[n1] = 0; fb[n1] = 0; fa[nn2] = 0; fb[nn2] = 0;
+                }
+            }
+        }
+    }
+    {
+        long n1 = IDX(0, 0, 0);
+        fa[n1] = fa[n1] / wg[n1];
+        fb[n1] = 0;
+    }
+    for (lz = 1; lz <= maxR; ++lz)
+        for (int ky = -maxR; ky <= maxR; ++ky)
+            for (int hx = -maxR; hx <= maxR; ++hx) {
+                long n1 = IDX(hx, ky, lz), nn2 = IDX(-hx, -ky, -lz);
+                real dr = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+                if (dr <= (real)maxR) {
+                    fa[n1] = fa[n1] / wg[n1];
+                    fb[n1] = fb[n1] / wg[n1];
+                    if (((hx + ky + lz) % 2) != 0) { fa[n1] = -fa[n1]; fb[n1] = -fb[n1]; }
+                    fa[nn2] = fa[n1];
+                    fb[nn2] = -fb[n1];
+                } else {
+                    fa[n1] = 0; fb[n1] = 0; fa[nn2] = 0; fb[nn2] = 0;
+                }
+            }
+#undef IDX
+}
+
+/* cmplx(fa, -fb) -> backward 3D FFT -> shift3dComplex -> real:
+ * icosprocess/SRC/reconstruct.f90:240-247, FFTW.f90:213-219,304-321. */
+EXPORT void o_volume_to_map(const real* fa, const real* fb, int nx, real* map)
+{
+    long N = (long)nx * nx * nx, n2 = (long)nx * nx;
+    double* re = (double*)malloc(sizeof(double) * N);
+    double* im = (double*)malloc(sizeof(double) * N);
+    for (long i = 0; i < N; ++i) { re[i] = fa[i]; im[i] = -(double)fb[i]; }
+    o_dft_cube(re, im, nx, 3, +1);
+    for (long i = 0; i < N; ++i) {
+        long z = i / n2, y = (i % n2) / nx, x = i % nx;
+        map[i] = (real)(((x + y + z) & 1) ? -re[i] : re[i]);
+    }
+    free(re); free(im);
+}
+
+/* ------------------------------------------------------------------ */
+/* Central-slice phase residual (local_refine)                         */
+/* ------------------------------------------------------------------ */
+
+/* FFT3d_slice: local_refine/SRC/FFT3d_slice.f90:3-78. */
+EXPORT void o_fft3d_slice(const real* fr, const real* fi, int nx, real theta, real phi, real omega,
+                          real* oa, real* ob)
+{
+    real R[9];
+    o_euler_mat(theta, phi, omega, R);
+    long n2 = (long)nx * nx;
+    long n = 0;
+    for (long i = 0; i < n2; ++i) { oa[i] = 0; ob[i] = 0; }
+    for (int iy = -nx / 2; iy <= nx / 2 - 1; ++iy)
+        for (int ix = -nx / 2; ix <= nx / 2 - 1; ++ix, ++n) {
+            real xp[3] = {(real)ix, (real)iy, 0}, X[3];
+            matvec(R, xp, X);
+            if ((real)sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]) > (real)(nx / 2 - 2)) continue;
+            X[0] += (real)(nx / 2); X[1] += (real)(nx / 2); X[2] += (real)(nx / 2);
+            int hx = (int)X[0], ky = (int)X[1], lz = (int)X[2];
+            real xb = X[0] - hx, yb = X[1] - ky, zb = X[2] - lz;
+            long b0 = (long)lz * n2 + (long)ky * nx + hx;
+            long o[8] = {b0, b0 + 1, b0 + nx, b0 + nx + 1, b0 + n2, b0 + n2 + 1, b0 + n2 + nx, b0 + n2 + nx + 1};
+            real w[8] = {((real)1 - xb) * ((real)1 - yb) * ((real)1 - zb), xb * ((real)1 - yb) * ((real)1 - zb),
+                         ((real)1 - xb) * yb * ((real)1 - zb), xb * yb * ((real)1 - zb),
+                         ((real)1 - xb) * ((real)1 - yb) * zb, xb * ((real)1 - yb) * zb,
+                         ((real)1 - xb) * yb * zb, xb * yb * zb};
+            real sr = 0, si = 0;
+            for (int q = 0; q < 8; ++q) { sr += w[q] * fr[o[q]]; si += w[q] * fi[o[q]]; }
+            oa[n] = sr; ob[n] = si;
+        }
+}
+
+/* cal_PR_FT: local_refine/SRC/cal_PR_FT.f90:1-36 (weight |F1|+|F2|). */
+EXPORT real o_pr_ft(const real* a1, const real* b1, const real* a2, const real* b2, int nx,
+                    real minR, real maxR)
+{
+    real pi = r_pi();
+    real PR1 = 0, PR2 = 0;
+    for (long i = 0; i < (long)nx * nx; ++i) {
+        int ky = (int)(i / nx) - nx / 2, hx = (int)(i % nx) - nx / 2;
+        real dr = (real)sqrt((real)(hx * hx + ky * ky));
+        if (dr >= minR && dr <= maxR) {
+            real amp1 = (real)sqrt(a1[i] * a1[i] + b1[i] * b1[i]);
+            real amp2 = (real)sqrt(a2[i] * a2[i] + b2[i] * b2[i]);
+            real diff = phase_diff(a1[i], b1[i], a2[i], b2[i]);
+            real w = amp1 + amp2;
+            PR1 = diff * w + PR1;
+            PR2 = w + PR2;
+        }
+    }
+    return PR2 > (real)1.0e-10 ? PR1 / PR2 * (real)180.0 / pi : (real)1000.0;
+}
+
+/* img3d_softmask: local_refine/SRC/img3d_softmask.f90:3-46 (sum in REAL). */
+EXPORT void o_img3d_softmask(real* d, int nx, real mask)
+{
+    int halfw = 6;
+    real pi = r_pi();
+    real inn = mask - (real)(halfw / 2);
+    if (inn <= 0) inn = 0;
+    real out = mask + (real)(halfw / 2);
+    if (out > (real)(nx / 2)) out = (real)(nx / 2);
+    long n2 = (long)nx * nx, N = n2 * nx;
+    long num = 0;
+    real summ = 0;
+    for (long i = 0; i < N; ++i) {
+        int iz = (int)(i / n2) - nx / 2;
+        if ((real)iz <= out) {
+            int iy = (int)((i % n2) / nx) - nx / 2, ix = (int)(i % nx) - nx / 2;
+            real r = (real)sqrt((real)(ix * ix + iy * iy + iz * iz));
+            if (r > inn && r <= out) { summ = d[i] + summ; num++; }
+        }
+    }
+    if (num != 0) summ = summ / (real)num;
+    for (long i = 0; i < N; ++i) {
+        int iz = (int)(i / n2) - nx / 2, iy = (int)((i % n2) / nx) - nx / 2, ix = (int)(i % nx) - nx / 2;
+        real r = (real)sqrt((real)(ix * ix + iy * iy + iz * iz));
+        if (r > inn && r <= out) {
+            real edge = ((real)1.0 + (real)cos(pi * (r - inn) / (real)halfw)) / (real)2.0;
+            d[i] = d[i] * edge + ((real)1.0 - edge) * summ;
+        }
+        if (r > out) d[i] = summ;
+    }
+}

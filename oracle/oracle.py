@@ -1,0 +1,219 @@
+"""ctypes front end of the C oracle (oracle/cim_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  Never part of the product
+path.  ``Oracle(64)`` is the FP64 oracle, ``Oracle(32)`` the FP32-faithful
+restatement.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class Oracle:
+    def __init__(self, bits: int = 64):
+        path = os.path.join(HERE, f"liboracle{bits}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        self.L = ctypes.CDLL(path)
+        self.dt = np.float64 if bits == 64 else np.float32
+        self.ct = ctypes.c_double if bits == 64 else ctypes.c_float
+        assert self.L.o_real_size() == np.dtype(self.dt).itemsize
+        ret_real = ["o_crosslin_pr", "o_refine_particle", "o_pr_ft"]
+        for f in ret_real:
+            getattr(self.L, f).restype = self.ct
+
+    # helpers
+    def _a(self, x):
+        return np.ascontiguousarray(x, dtype=self.dt)
+
+    def _p(self, a):
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    def _r(self, v):
+        return self.ct(float(v))
+
+    # ---- geometry --------------------------------------------------------
+    def euler_mat(self, th, ph, om):
+        R = np.zeros(9, self.dt)
+        self.L.o_euler_mat(self._r(th), self._r(ph), self._r(om), self._p(R))
+        return R.reshape(3, 3)
+
+    def icos_ops(self):
+        o = np.zeros(540, self.dt)
+        self.L.o_icos_ops(self._p(o))
+        return o.reshape(60, 3, 3)
+
+    def crosslin_angle(self, a1, a2, ops=None):
+        ops = self.icos_ops() if ops is None else self._a(ops)
+        ops = self._a(ops).reshape(-1, 9)
+        n = ops.shape[0]
+        al1, al2 = np.zeros(n, self.dt), np.zeros(n, self.dt)
+        self.L.o_crosslin_angle(*[self._r(v) for v in (*a1, *a2)], n, self._p(ops), self._p(al1), self._p(al2))
+        return al1, al2
+
+    def return_au(self, th, ph, om):
+        v = [self.ct(float(th)), self.ct(float(ph)), self.ct(float(om))]
+        self.L.o_return_au(ctypes.byref(v[0]), ctypes.byref(v[1]), ctypes.byref(v[2]))
+        return v[0].value, v[1].value, v[2].value
+
+    # ---- images ------------------------------------------------------------
+    def ctf2d(self, nx, df1, df2, astig, apix, vol=300.0, cs=2.7, ampwg=0.1, df_scale=1.0e4):
+        c = np.zeros(nx * nx, self.dt)
+        self.L.o_getctf2d(nx, *[self._r(v) for v in (df1, df2, astig, apix, vol, cs, ampwg, df_scale)], self._p(c))
+        return c.reshape(nx, nx)
+
+    def mask(self, img, mask, ox, oy):
+        a = self._a(img).copy()
+        self.L.o_img2d_mask(self._p(a), a.shape[-1], self._r(mask), self._r(ox), self._r(oy))
+        return a
+
+    def norm(self, img):
+        a = self._a(img).copy()
+        self.L.o_img2d_norm(self._p(a), a.shape[-1])
+        return a
+
+    def checker(self, img):
+        a = self._a(img).copy()
+        self.L.o_checker2d(self._p(a), a.shape[-1])
+        return a
+
+    def fft2(self, a, b, direction=1, conj_out=True):
+        a, b = self._a(a).copy(), self._a(b).copy()
+        self.L.o_img2d_fft(self._p(a), self._p(b), a.shape[-1], direction, int(conj_out))
+        return a, b
+
+    def fft3(self, a, b, direction=1, conj_out=False):
+        a, b = self._a(a).copy(), self._a(b).copy()
+        self.L.o_img3d_fft(self._p(a), self._p(b), a.shape[-1], direction, int(conj_out))
+        return a, b
+
+    def dft2_brute(self, re, im, sign):
+        re, im = np.ascontiguousarray(re, np.float64), np.ascontiguousarray(im, np.float64)
+        o1, o2 = np.zeros_like(re), np.zeros_like(im)
+        self.L.o_dft2d_brute(self._p(re), self._p(im), re.shape[-1], sign, self._p(o1), self._p(o2))
+        return o1, o2
+
+    def ft_shift(self, a, b, x0, y0, recentre=True):
+        a, b = self._a(a).copy(), self._a(b).copy()
+        self.L.o_ft_shift(self._p(a), self._p(b), a.shape[-1], self._r(x0), self._r(y0), int(recentre))
+        return a, b
+
+    def amp_expfit(self, a, b, minR, maxR):
+        a, b = self._a(a), self._a(b)
+        f = np.zeros(maxR - minR + 1, self.dt)
+        self.L.o_amp_expfit(self._p(a), self._p(b), a.shape[-1], minR, maxR, self._p(f))
+        return f
+
+    # ---- pipelines -------------------------------------------------------------
+    def particle_ft(self, img, x, y, mask_r=None, mask_xy=None, norm=False, conj_out=True, shift=True):
+        """norm -> mask -> checkerboard -> FFT -> centre shift (reconstruct.f90:79-96)."""
+        a = self._a(img).copy()
+        if norm:
+            a = self.norm(a)
+        if mask_r is not None:
+            mx, my = (x, y) if mask_xy is None else mask_xy
+            a = self.mask(a, mask_r, mx, my)
+        a = self.checker(a)
+        fa, fb = self.fft2(a, np.zeros_like(a), 1, conj_out)
+        if shift:
+            fa, fb = self.ft_shift(fa, fb, x, y, True)
+        return fa, fb
+
+    def insert(self, vol, a, b, ctf, angles, maxR, rcut=None, compact=False, ops=None):
+        fa, fb, wg = vol
+        nx = a.shape[-1]
+        ops = self.icos_ops() if ops is None else self._a(ops)
+        ops = self._a(ops).reshape(-1, 9)
+        R = self._a(self.euler_mat(*angles)).reshape(9)
+        c = None if ctf is None else self._a(ctf)
+        self.L.o_insert(self._p(fa), self._p(fb), self._p(wg), int(compact), nx, maxR,
+                        self._r(maxR if rcut is None else rcut), self._p(self._a(a)), self._p(self._a(b)),
+                        None if c is None else self._p(c), self._p(R), ops.shape[0], self._p(ops))
+
+    def new_volume(self, nx, maxR, compact=False, wg_init=0.01):
+        if compact:
+            shape = (maxR + 1, 2 * maxR + 1, 2 * maxR + 1)
+        else:
+            shape = (nx, nx, nx)
+        return (np.zeros(shape, self.dt), np.zeros(shape, self.dt), np.full(shape, wg_init, self.dt))
+
+    def fold_and_map(self, vol, maxR):
+        fa, fb, wg = (v.copy() for v in vol)
+        nx = fa.shape[-1]
+        self.L.o_fold_normalise(self._p(fa), self._p(fb), self._p(wg), nx, maxR)
+        m = np.zeros_like(fa)
+        self.L.o_volume_to_map(self._p(fa), self._p(fb), nx, self._p(m))
+        return m
+
+    # ---- common lines ------------------------------------------------------------
+    def search_grid(self, step):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.L.o_search_grid(self._r(step), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def pr_omega(self, pa, pb, fcut0, ta, tb, tfcut, teul, minR, maxR, theta, phi, step):
+        nx = pa.shape[-1]
+        _, _, No = self.search_grid(step)
+        ops = self.icos_ops().reshape(-1)
+        out = np.zeros(No, self.dt)
+        T = len(teul)
+        self.L.o_crosslin_pr_omega(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)), T,
+                                   self._p(self._a(ta)), self._p(self._a(tb)), self._p(self._a(tfcut)),
+                                   self._p(self._a(teul)), nx, minR, maxR, 60, self._p(self._a(ops)),
+                                   self._r(theta), self._r(phi), No, self._r(step), self._p(out))
+        return out
+
+    def globesearch(self, pa, pb, fcut0, ta, tb, tfcut, teul, minR, maxR, step, want_array=False):
+        nx = pa.shape[-1]
+        Nt, Np, No = self.search_grid(step)
+        ops = self._a(self.icos_ops().reshape(-1))
+        out4 = np.zeros(4, self.dt)
+        arr = np.zeros((Nt + 1) * (2 * Np + 1) * No, self.dt) if want_array else None
+        self.L.o_globesearch(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)), len(teul),
+                             self._p(self._a(ta)), self._p(self._a(tb)), self._p(self._a(tfcut)),
+                             self._p(self._a(teul)), nx, minR, maxR, 60, self._p(ops), self._r(step),
+                             self._p(out4), None if arr is None else self._p(arr))
+        if want_array:
+            return out4, arr.reshape(Nt + 1, 2 * Np + 1, No)
+        return out4
+
+    def crosslin_pr(self, pa, pb, fcut0, ta, tb, tfcut, teul, minR, maxR, th, ph, om, x, y):
+        nx = pa.shape[-1]
+        ops = self._a(self.icos_ops().reshape(-1))
+        return self.L.o_crosslin_pr(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)), len(teul),
+                                    self._p(self._a(ta)), self._p(self._a(tb)), self._p(self._a(tfcut)),
+                                    self._p(self._a(teul)), nx, minR, maxR, 60, self._p(ops),
+                                    *[self._r(v) for v in (th, ph, om, x, y)])
+
+    def refine_particle(self, pa, pb, fcut0, ta, tb, tfcut, teul, minR, maxR, ncycle, dang, dtran, start5):
+        nx = pa.shape[-1]
+        ops = self._a(self.icos_ops().reshape(-1))
+        io5 = self._a(start5).copy()
+        pr = self.L.o_refine_particle(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)),
+                                      len(teul), self._p(self._a(ta)), self._p(self._a(tb)),
+                                      self._p(self._a(tfcut)), self._p(self._a(teul)), nx, minR, maxR, 60,
+                                      self._p(ops), ncycle, self._r(dang), self._r(dtran), self._p(io5))
+        return io5, pr
+
+    # ---- local refine ----------------------------------------------------------------
+    def fft3d_slice(self, fr, fi, th, ph, om):
+        nx = fr.shape[-1]
+        a, b = np.zeros((nx, nx), self.dt), np.zeros((nx, nx), self.dt)
+        self.L.o_fft3d_slice(self._p(self._a(fr)), self._p(self._a(fi)), nx, self._r(th), self._r(ph),
+                             self._r(om), self._p(a), self._p(b))
+        return a, b
+
+    def pr_ft(self, a1, b1, a2, b2, minR, maxR):
+        nx = a1.shape[-1]
+        return self.L.o_pr_ft(*[self._p(self._a(v)) for v in (a1, b1, a2, b2)], nx, self._r(minR), self._r(maxR))
+
+    def softmask3d(self, vol, mask):
+        v = self._a(vol).copy()
+        self.L.o_img3d_softmask(self._p(v), v.shape[-1], self._r(mask))
+        return v

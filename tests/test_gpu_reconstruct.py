@@ -1,0 +1,110 @@
+"""HIP direct Fourier inversion against the FP64 oracle of
+icosprocess/SRC/reconstruct.f90 (full box, mode 1) and
+reconstruct_substep.f90 (compact tnf accumulator)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from cryoim_amd import lib, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(box=64, n=24, seed=11, ctf=True):
+    ph = synth.make_phantom(box, radius=(box * 0.28, box * 0.4), sigma=(1.5, 2.5), seed=seed)
+    ps = synth.make_particles(ph, n, seed=seed + 1, shift_max=2.0, snr=0.5, ctf=ctf, apix=1.5,
+                              df_range=(1.0, 2.5))
+    return ps
+
+
+def _parts(ps, mask_first=True):
+    n = len(ps.images)
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 0:3] = ps.angles
+    P[:, 3:5] = ps.shifts
+    P[:, 6:8] = ps.df
+    P[:, 8] = ps.astig
+    if mask_first:                     # reconstruct.f90:92 masks around the first particle's shift
+        P[:, 9:11] = ps.shifts[0]
+    else:
+        P[:, 9:11] = ps.shifts
+    P[:, 11] = 1.0
+    return P
+
+
+def _oracle_volume(O, ps, maxR, mask_r, apix, compact=False, rcut=None, wg_init=0.01, ctf=True):
+    box = ps.images.shape[-1]
+    vol = O.new_volume(box, maxR, compact=compact, wg_init=wg_init)
+    for i in range(len(ps.images)):
+        x, y = ps.shifts[i]
+        a, b = O.particle_ft(ps.images[i], x, y, mask_r=mask_r, mask_xy=tuple(ps.shifts[0]), norm=True)
+        c = O.ctf2d(box, ps.df[i, 0], ps.df[i, 1], 0.0, apix) if ctf else None
+        O.insert(vol, a, b, c, ps.angles[i], maxR, rcut=rcut, compact=compact)
+    return vol
+
+
+@pytest.mark.parametrize("box,maxR", [(64, 24), (96, 30)])
+def test_reconstruct_map_matches_fp64_oracle(ctx, O64, box, maxR):
+    ps = _data(box=box, n=20)
+    apix, mask_r = 1.5, box / 2 - 4
+    p = lib.make_params(box, 0, maxR, apply_ctf=True, mask_on=True, norm_on=True, apix=apix, imgmask=mask_r)
+    ctx.rec_begin(box, maxR, layout=0, rcut=maxR, wg_init=0.01)
+    ctx.rec_insert(p, ps.images, _parts(ps))
+    gmap = ctx.rec_finish(box)
+    vol = _oracle_volume(O64, ps, maxR, mask_r, apix)
+    omap = O64.fold_and_map(vol, maxR)
+    err = rel_l2(gmap, omap)
+    print(f"box {box} maxR {maxR}: map rel-L2 vs FP64 oracle = {err:.3e}")
+    assert err < 1e-4
+    # the accumulated sums themselves (tnf block of the full volume)
+    fa, fb, wg = ctx.rec_get_tnf(maxR)
+    c = box // 2
+    sl = (slice(c, c + maxR + 1), slice(c - maxR, c + maxR + 1), slice(c - maxR, c + maxR + 1))
+    for g, o in ((fa, vol[0][sl]), (fb, vol[1][sl]), (wg, vol[2][sl])):
+        assert rel_l2(g, o) < 1e-4
+
+
+def test_reconstruct_substep_tnf(ctx, O64):
+    box, maxR = 64, 24
+    ps = _data(box=box, n=16, seed=5)
+    apix, mask_r = 1.5, box / 2 - 4
+    p = lib.make_params(box, 0, maxR, apply_ctf=True, mask_on=True, norm_on=True, apix=apix, imgmask=mask_r)
+    ctx.rec_begin(box, maxR, layout=1, rcut=maxR - 2, wg_init=0.0)
+    ctx.rec_insert(p, ps.images, _parts(ps))
+    fa, fb, wg = ctx.rec_get_tnf(maxR)
+    vol = _oracle_volume(O64, ps, maxR, mask_r, apix, compact=True, rcut=maxR - 2, wg_init=0.0)
+    for g, o in ((fa, vol[0]), (fb, vol[1]), (wg, vol[2])):
+        assert rel_l2(g, o) < 1e-4
+
+
+def test_partition_of_unity_weights(ctx):
+    """sum(wg) = 60 * sum(ctf^2) over inserted samples (+ init), independent of the oracle."""
+    box, maxR = 64, 20
+    rng = np.random.default_rng(3)
+    n = 5
+    re = rng.standard_normal((n, box, box)).astype(np.float32)
+    im = rng.standard_normal((n, box, box)).astype(np.float32)
+    ctf = rng.uniform(-1, 1, (n, box, box)).astype(np.float32)
+    eul = np.array([[70, 10, 30], [80, -5, 200], [75, 3, 100], [88, 20, 5], [71, -15, 300]], np.float32)
+    ctx.rec_begin(box, maxR, layout=1, rcut=maxR - 2, wg_init=0.0)
+    ctx.rec_insert_ft(re, im, eul, ctf)
+    fa, fb, wg = ctx.rec_get_tnf(maxR)
+    c = box // 2
+    tot = 0.0
+    for ky in range(-maxR, maxR + 1):
+        for hx in range(0, maxR + 1):
+            if np.sqrt(np.float32(hx * hx + ky * ky)) <= maxR - 2:
+                tot += float(np.sum(ctf[:, ky + c, hx + c].astype(np.float64) ** 2))
+    assert abs(wg.sum(dtype=np.float64) - 60 * tot) / (60 * tot) < 1e-5
+
+
+def test_unused_particles_are_skipped(ctx):
+    box, maxR = 64, 20
+    ps = _data(box=box, n=6, seed=9)
+    p = lib.make_params(box, 0, maxR, apply_ctf=True, norm_on=True, apix=1.5)
+    P = _parts(ps)
+    P[:, 11] = 0.0
+    ctx.rec_begin(box, maxR, layout=1, rcut=maxR - 2, wg_init=0.0)
+    ctx.rec_insert(p, ps.images, P)
+    fa, fb, wg = ctx.rec_get_tnf(maxR)
+    assert not fa.any() and not wg.any()
