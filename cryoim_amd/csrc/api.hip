@@ -91,6 +91,7 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->rec.samples.release();
     ctx->srch.tft.release();
     ctx->srch.tfcut.release();
+    ctx->srch.wk.release();
     for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->aux, &ctx->aux2, &ctx->parts, &ctx->out}) b->release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);

@@ -56,6 +56,16 @@ struct RecState {
     long long inserted = 0;
 };
 
+// Per-batch working set of the searches.
+struct SearchBatch {
+    DevBuf pcrop, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi;
+    void release() {
+        for (DevBuf* b : {&pcrop, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
+                          &fv, &fi})
+            b->release();
+    }
+};
+
 // Template bank for the common-lines search.
 struct SearchState {
     bool ready = false;
@@ -63,6 +73,7 @@ struct SearchState {
     DevBuf tft;        // float2 [T][box*box] template FTs (masked, FFT, centre shift)
     DevBuf tfcut;      // float [T][nr] Fcut fitted without CTF (load_template.f90:44-47)
     std::vector<float> tang;   // T*3 template angles
+    SearchBatch wk;
 };
 
 struct KTimer {

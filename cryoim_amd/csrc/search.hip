@@ -1,0 +1,937 @@
+// icosprocess common-lines phase-residual search on gfx950.
+//
+// Global search (ort_globesearch.f90:34-134 with cal_crosslin_PR_omega.f90):
+// one workgroup per (AU grid point, particle).  Each thread owns one (or
+// VPT) in-plane angle omega, so the phase-residual sums of
+// cal_crosslin_PR_omega.f90:48-69 accumulate in registers.  The particle
+// transform inside radius maxR+2 is staged once per workgroup in LDS as a
+// disk (rows of varying length), so every bilinear tap of
+// img2d_fill_lin_omegearray.f90:18-36 is an LDS read; the template lines of
+// img2d_fill_line.f90 are sampled cooperatively into LDS per chunk of
+// common lines.  The argmin over (omega, phi, theta) keeps Fortran minloc
+// semantics (first minimum in column-major order) by reducing
+// (value, linear index) pairs.
+//
+// Local refinement (ort_refinement.f90:69-95): one workgroup per particle
+// runs the whole coordinate descent; the 11 candidates of a cycle are scored
+// together by cal_crosslin_PR (cal_crosslin_PR.f90:22-68) with the trial
+// centre shift applied to the taps.
+#include <cmath>
+
+#include "cim_device.h"
+
+namespace cim {
+
+int upload_parts(cim_ctx* ctx, int n, const float* parts);
+int check_params_public(cim_ctx* ctx, const cim_params* p);
+
+// ---- geometry shared by both kernels (float32, as the reference) ---------
+
+__device__ __forceinline__ void euler_f(float th0, float ph0, float om0, float* R) {
+    const float d2r = 0.0174532925199432957692f;   // atan2(1,1)/45
+    float th = th0 * d2r, ph = ph0 * d2r, om = om0 * d2r;
+    float c1 = cosf(ph), c2 = cosf(th), c3 = cosf(om), s1 = sinf(ph), s2 = sinf(th), s3 = sinf(om);
+    R[0] = c1 * c2 * c3 - s1 * s3; R[1] = -c1 * c2 * s3 - c3 * s1; R[2] = c1 * s2;
+    R[3] = c2 * c3 * s1 + c1 * s3; R[4] = -c2 * s1 * s3 + c1 * c3; R[5] = s1 * s2;
+    R[6] = -c3 * s2;               R[7] = s2 * s3;                 R[8] = c2;
+}
+
+// icos_crosslin_angle.f90:11-34 for one operator S (row-major float[9]).
+__device__ __forceinline__ void crosslin_one(const float* E1, const float* E2, const float* S, float* a1, float* a2) {
+    const float r2d = 57.2957795130823208768f;
+    float z1[3] = {E1[2], E1[5], E1[8]};
+    float z2[3] = {E2[2], E2[5], E2[8]};
+    float z2a[3];
+    for (int i = 0; i < 3; ++i) z2a[i] = S[3 * i] * z2[0] + S[3 * i + 1] * z2[1] + S[3 * i + 2] * z2[2];
+    float v[3] = {z1[1] * z2a[2] - z1[2] * z2a[1], z1[2] * z2a[0] - z1[0] * z2a[2], z1[0] * z2a[1] - z1[1] * z2a[0]};
+    float vv = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (vv < 0.5f) {
+        *a1 = 1000.f;
+        *a2 = 1000.f;
+        return;
+    }
+    v[0] /= vv; v[1] /= vv; v[2] /= vv;
+    float v10x = E1[0] * v[0] + E1[3] * v[1] + E1[6] * v[2];
+    float v10y = E1[1] * v[0] + E1[4] * v[1] + E1[7] * v[2];
+    float t[3];
+    for (int i = 0; i < 3; ++i) t[i] = S[i] * v[0] + S[3 + i] * v[1] + S[6 + i] * v[2];
+    float v20x = E2[0] * t[0] + E2[3] * t[1] + E2[6] * t[2];
+    float v20y = E2[1] * t[0] + E2[4] * t[1] + E2[7] * t[2];
+    *a1 = atan2f(v10y, v10x) * r2d;
+    *a2 = atan2f(v20y, v20x) * r2d;
+}
+
+__device__ __forceinline__ float fold_phase(float d) {
+    const float pi = 3.14159265358979323846f, tpi = 6.28318530717958647692f;
+    d = fabsf(d);
+    if (d >= tpi) d -= tpi;
+    if (d > pi) d = tpi - d;
+    return d;
+}
+
+// Bilinear sample (img2d_fill_line.f90:17-31) of a crop with rows of Dc
+// elements whose index 0 is relative coordinate -(R+1).
+__device__ __forceinline__ float2 crop_bilin(const float2* __restrict__ crop, int Dc, int R, float x, float y,
+                                             int ic) {
+    int lx = (int)x, ly = (int)y;
+    float ax = x - (float)lx, ay = y - (float)ly;
+    int cx = lx - ic + R + 1, cy = ly - ic + R + 1;
+    const float2* r0 = crop + (long long)cy * Dc + cx;
+    const float2* r1 = r0 + Dc;
+    float2 a = r0[0], b = r0[1], c = r1[0], d = r1[1];
+    float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay, w11 = ax * ay;
+    return make_float2(w00 * a.x + w10 * b.x + w01 * c.x + w11 * d.x, w00 * a.y + w10 * b.y + w01 * c.y + w11 * d.y);
+}
+
+// ---- crops ----------------------------------------------------------------
+
+// Crop rel [-(R+1), R+2] around the centre of n centred FTs, optionally
+// multiplied by the CTF of particle record (item / T) (template x CTF case:
+// item = p*T + t, source = src[t]).
+__global__ void k_crop(const float2* __restrict__ src, long long src_stride, int box, int R, int Dc,
+                       float2* __restrict__ dst, int T, const float* __restrict__ ctf_parts, CtfConst cc, int use_ctf) {
+    const int item = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Dc * Dc) return;
+    const int cy = q / Dc, cx = q % Dc;
+    const int c = box / 2;
+    const int hx = cx - R - 1, ky = cy - R - 1;
+    const int ax = hx + c, ay = ky + c;
+    const int t = item % T, p = item / T;
+    const float2* s = src + (long long)(use_ctf ? t : item) * src_stride;
+    float2 v = make_float2(0.f, 0.f);
+    if (ax >= 0 && ax < box && ay >= 0 && ay < box) v = s[(long long)ay * box + ax];
+    if (use_ctf) {
+        const float* pr = ctf_parts + (long long)p * CIM_PART_STRIDE;
+        float cv = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+        v.x *= cv;
+        v.y *= cv;
+    }
+    dst[(long long)item * Dc * Dc + q] = v;
+}
+
+// ---- global search kernel -------------------------------------------------
+
+struct SearchArgs {
+    const float2* pcrop;      // n x Dc^2 particle crops (centre-shifted)
+    const float* fcut0;       // n x nr
+    const float2* tcrop;      // template crops; particle p, template t at tcrop + p*tc_pstride + t*Dc^2
+    long long tc_pstride;
+    const float* tfcut;       // Fcut of template t for particle p at tfcut + p*tf_pstride + t*nr
+    long long tf_pstride;
+    const float* tang;        // T x 3 template angles
+    int T;
+    const float* pts;         // npts x 2 (theta, phi)
+    const int* pbase;         // npts: linear index of (omega=0, phi, theta) in PR_array
+    int npts;
+    const float* ops;         // 60 x 9 float
+    int nops;
+    const int* rows;          // nrows x 3: lo, hi, base (disk layout)
+    int nrows, disk_total;
+    int R, minR, nr, Dc, Nomega, box, chunk;
+    float step;
+    float* outv;              // n x npts
+    int* outi;
+    float* prfull;            // optional n x npts x Nomega
+};
+
+template <int VPT, bool USE_LDS>
+__global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
+    extern __shared__ float4 smem4[];
+    char* sm = reinterpret_cast<char*>(smem4);
+    float2* disk = reinterpret_cast<float2*>(sm);
+    size_t off = USE_LDS ? sizeof(float2) * (size_t)a.disk_total : 0;
+    float2* tline = reinterpret_cast<float2*>(sm + off);
+    off += sizeof(float2) * (size_t)a.chunk * a.nr;
+    float* alpha1 = reinterpret_cast<float*>(sm + off);
+    float* alpha2 = alpha1 + 64;
+    float* fc0 = alpha2 + 64;
+    int* rbase = reinterpret_cast<int*>(fc0 + a.nr);
+    float* redv = reinterpret_cast<float*>(rbase + a.nrows);
+    int* redi = reinterpret_cast<int*>(redv + 16);
+
+    const int pt = blockIdx.x, p = blockIdx.y;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const float2* pc = a.pcrop + (long long)p * a.Dc * a.Dc;
+    const int ic = a.box / 2;
+    const float cf = (float)ic;
+    const float d2r = 0.0174532925199432957692f;
+
+    for (int i = tid; i < a.nr; i += bd) fc0[i] = a.fcut0[(long long)p * a.nr + i];
+    if (USE_LDS) {
+        for (int r = tid; r < a.nrows; r += bd) rbase[r] = a.rows[3 * r + 2];
+        for (int r = 0; r < a.nrows; ++r) {
+            const int lo = a.rows[3 * r], hi = a.rows[3 * r + 1], base = a.rows[3 * r + 2];
+            for (int x = lo + tid; x <= hi; x += bd) disk[base + x] = pc[(long long)r * a.Dc + (x + a.R + 1)];
+        }
+    }
+    const float theta = a.pts[2 * pt], phi = a.pts[2 * pt + 1];
+    float E1[9];
+    euler_f(theta, phi, 0.f, E1);
+
+    float num[VPT], den[VPT];
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) { num[v] = 0.f; den[v] = 0.f; }
+    __syncthreads();
+
+    for (int t = 0; t < a.T; ++t) {
+        const float2* tc = a.tcrop + (long long)p * a.tc_pstride + (long long)t * a.Dc * a.Dc;
+        const float* tf = a.tfcut + (long long)p * a.tf_pstride + (long long)t * a.nr;
+        if (tid < a.nops) {
+            float E2[9];
+            euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
+            crosslin_one(E1, E2, a.ops + 9 * tid, &alpha1[tid], &alpha2[tid]);
+        }
+        __syncthreads();
+        for (int j0 = 0; j0 < a.nops; j0 += a.chunk) {
+            const int jn = min(a.chunk, a.nops - j0);
+            for (int q = tid; q < jn * a.nr; q += bd) {
+                const int jj = q / a.nr, fr = q - jj * a.nr;
+                const int j = j0 + jj;
+                float2 res = make_float2(0.f, -1.f);
+                if (alpha1[j] < 999.f && alpha2[j] < 999.f) {
+                    const float FR = (float)(a.minR + fr);
+                    const float ang = alpha2[j] * d2r;
+                    float x = FR * cosf(ang) + cf, y = FR * sinf(ang) + cf;
+                    float2 z = crop_bilin(tc, a.Dc, a.R, x, y, ic);
+                    float amp2 = sqrtf(z.x * z.x + z.y * z.y);
+                    if (amp2 >= tf[fr]) res = make_float2(atan2f(z.y, z.x), amp2);
+                }
+                tline[q] = res;
+            }
+            __syncthreads();
+            for (int jj = 0; jj < jn; ++jj) {
+                const int j = j0 + jj;
+                const float a1 = alpha1[j];
+                if (!(a1 < 999.f && alpha2[j] < 999.f)) continue;
+                float cs[VPT], sn[VPT];
+#pragma unroll
+                for (int v = 0; v < VPT; ++v) {
+                    const int k = tid + v * bd;
+                    const float om = (a1 - a.step * (float)k) * d2r;
+                    cs[v] = cosf(om);
+                    sn[v] = sinf(om);
+                }
+                const float2* tl = tline + jj * a.nr;
+                for (int fr = 0; fr < a.nr; ++fr) {
+                    const float2 tv = tl[fr];
+                    if (tv.y < 0.f) continue;
+                    const float FR = (float)(a.minR + fr);
+                    const float f0 = fc0[fr];
+#pragma unroll
+                    for (int v = 0; v < VPT; ++v) {
+                        const int k = tid + v * bd;
+                        if (k >= a.Nomega) continue;
+                        const float x = FR * cs[v] + cf, y = FR * sn[v] + cf;
+                        float2 z;
+                        if (USE_LDS) {
+                            const int lx = (int)x, ly = (int)y;
+                            const float ax = x - (float)lx, ay = y - (float)ly;
+                            const int r = ly - ic + a.R + 1;
+                            const int i0 = rbase[r] + (lx - ic), i1 = rbase[r + 1] + (lx - ic);
+                            const float2 q00 = disk[i0], q10 = disk[i0 + 1], q01 = disk[i1], q11 = disk[i1 + 1];
+                            const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay);
+                            const float w01 = (1.f - ax) * ay, w11 = ax * ay;
+                            z = make_float2(w00 * q00.x + w10 * q10.x + w01 * q01.x + w11 * q11.x,
+                                            w00 * q00.y + w10 * q10.y + w01 * q01.y + w11 * q11.y);
+                        } else {
+                            z = crop_bilin(pc, a.Dc, a.R, x, y, ic);
+                        }
+                        const float amp1 = sqrtf(z.x * z.x + z.y * z.y);
+                        if (amp1 >= f0) {
+                            const float diff = fold_phase(tv.x - atan2f(z.y, z.x));
+                            const float w = 0.5f * (amp1 + tv.y);
+                            num[v] += diff * w;
+                            den[v] += w;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // PR per omega, then the first minimum of this (theta, phi) column
+    const float r2d = 57.2957795130823208768f;
+    float best = 3.0e38f;
+    int bidx = 0x7fffffff;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+        const int k = tid + v * bd;
+        if (k < a.Nomega) {
+            const float pr = den[v] > 1.0e-10f ? num[v] / den[v] * r2d : 999.f;
+            if (a.prfull) a.prfull[((long long)p * a.npts + pt) * a.Nomega + k] = pr;
+            const int li = a.pbase[pt] + k;
+            if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
+        }
+    }
+    // wave reduction
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bidx, o, 64);
+        if (ov < best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+    }
+    const int w = tid >> 6;
+    if ((tid & 63) == 0) { redv[w] = best; redi[w] = bidx; }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (bd + 63) >> 6;
+        for (int i = 1; i < nw; ++i)
+            if (redv[i] < best || (redv[i] == best && redi[i] < bidx)) { best = redv[i]; bidx = redi[i]; }
+        a.outv[(long long)p * a.npts + pt] = best;
+        a.outi[(long long)p * a.npts + pt] = bidx;
+    }
+}
+
+// First minimum over the AU points of each particle.
+__global__ void k_argmin(const float* __restrict__ v, const int* __restrict__ ix, int npts, float* __restrict__ ov,
+                         int* __restrict__ oi) {
+    __shared__ float sv[16];
+    __shared__ int si[16];
+    const int p = blockIdx.x;
+    float best = 3.0e38f;
+    int bidx = 0x7fffffff;
+    for (int i = threadIdx.x; i < npts; i += blockDim.x) {
+        const float x = v[(long long)p * npts + i];
+        const int k = ix[(long long)p * npts + i];
+        if (x < best || (x == best && k < bidx)) { best = x; bidx = k; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float xv = __shfl_xor(best, o, 64);
+        const int xi = __shfl_xor(bidx, o, 64);
+        if (xv < best || (xv == best && xi < bidx)) { best = xv; bidx = xi; }
+    }
+    if ((threadIdx.x & 63) == 0) { sv[threadIdx.x >> 6] = best; si[threadIdx.x >> 6] = bidx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+            if (sv[i] < best || (sv[i] == best && si[i] < bidx)) { best = sv[i]; bidx = si[i]; }
+        ov[p] = best;
+        oi[p] = bidx;
+    }
+}
+
+// ---- host: grid, disk layout, template bank ---------------------------------
+
+struct SearchGrid {
+    int Nt = 0, Np = 0, No = 0;
+    std::vector<float> pts;   // theta, phi
+    std::vector<int> base;
+};
+
+// ort_globesearch.f90:16-21,68-82 evaluated in float32 like the reference.
+static SearchGrid make_grid(float step) {
+    SearchGrid g;
+    g.Nt = (int)(21.0f / step);
+    g.Np = (int)(32.0f / step);
+    g.No = (int)(360.0f / step);
+    const float d2r = atan2f(1.0f, 1.0f) / 45.0f;
+    const float A1 = -sinf(31.717f * d2r) * cosf(69.09f * d2r), B1 = -cosf(31.717f * d2r) * cosf(69.09f * d2r);
+    const float C1 = sinf(31.717f * d2r) * sinf(69.09f * d2r);
+    const float A2 = -sinf(31.717f * d2r) * cosf(69.09f * d2r), B2 = cosf(31.717f * d2r) * cosf(69.09f * d2r);
+    const float C2 = sinf(31.717f * d2r) * sinf(69.09f * d2r);
+    for (int it = 0; it <= g.Nt; ++it) {
+        const float theta = 69.0f + step * (float)it;
+        float tmp = theta * d2r;
+        const float z = cosf(tmp), z0 = sinf(tmp);
+        for (int ip = 0; ip <= g.Np; ++ip) {
+            const float phi = step * (float)ip;
+            tmp = phi * d2r;
+            const float x = z0 * cosf(tmp), y = z0 * sinf(tmp);
+            const float z1 = -(A1 * x + B1 * y) / C1, z2 = -(A2 * x + B2 * y) / C2;
+            if (!((z >= 0.f) && (z <= z1) && (z <= z2))) continue;
+            for (int s = 0; s < (ip == 0 ? 1 : 2); ++s) {
+                const int sip = s == 0 ? ip : -ip;
+                g.pts.push_back(theta);
+                g.pts.push_back(s == 0 ? phi : -phi);
+                g.base.push_back(g.No * ((sip + g.Np) + (2 * g.Np + 1) * it));
+            }
+        }
+    }
+    return g;
+}
+
+// Disk of taps reachable from radius <= R (plus a one-pixel margin) as rows
+// lo..hi of relative x with LDS base offsets.
+static void make_disk(int R, std::vector<int>& rows, int& total) {
+    const int nrows = 2 * R + 4;
+    rows.assign(3 * nrows, 0);
+    total = 0;
+    for (int r = 0; r < nrows; ++r) {
+        const int ly = r - R - 1;
+        double ymin = (ly - 1 <= 0 && 0 <= ly + 1) ? 0.0 : std::min(std::abs(ly - 1), std::abs(ly + 1));
+        double xm = std::sqrt(std::max(0.0, (double)R * R - ymin * ymin));
+        int lo = (int)std::floor(-xm) - 1, hi = (int)std::floor(xm) + 2;
+        lo = std::max(lo, -R - 1);
+        hi = std::min(hi, R + 2);
+        rows[3 * r] = lo;
+        rows[3 * r + 1] = hi;
+        rows[3 * r + 2] = total - lo;
+        total += hi - lo + 1;
+    }
+}
+
+static int prep_templates(cim_ctx* ctx, const cim_params* p, int T, const float* images, const float* tort) {
+    SearchState& s = ctx->srch;
+    const int box = p->box;
+    const long long N = (long long)box * box;
+    const int nr = p->maxR - p->minR + 1;
+    CIM_ALLOC(ctx->img, sizeof(float) * N * T);
+    CIM_HIP(hipMemcpyAsync(ctx->img.ptr, images, sizeof(float) * N * T, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<float> parts((size_t)T * CIM_PART_STRIDE, 0.f);
+    s.tang.assign(3 * T, 0.f);
+    for (int t = 0; t < T; ++t) {
+        float* q = &parts[(size_t)t * CIM_PART_STRIDE];
+        q[CIM_P_THETA] = tort[5 * t];
+        q[CIM_P_PHI] = tort[5 * t + 1];
+        q[CIM_P_OMEGA] = tort[5 * t + 2];
+        q[CIM_P_X] = tort[5 * t + 3];
+        q[CIM_P_Y] = tort[5 * t + 4];
+        q[CIM_P_MASKX] = tort[5 * t + 3];
+        q[CIM_P_MASKY] = tort[5 * t + 4];
+        for (int k = 0; k < 3; ++k) s.tang[3 * t + k] = tort[5 * t + k];
+    }
+    int rc = upload_parts(ctx, T, parts.data());
+    if (rc) return rc;
+    CIM_ALLOC(s.tft, sizeof(float2) * N * T);
+    rc = preprocess_images(ctx, p, T, ctx->img.as<float>(), ctx->parts.as<float>(), s.tft.as<float2>(), 0,
+                           p->mask_on, 1);
+    if (rc) return rc;
+    rc = centre_shift(ctx, s.tft.as<float2>(), box, T, ctx->parts.as<float>(), 1);
+    if (rc) return rc;
+    CIM_ALLOC(s.tfcut, sizeof(float) * nr * T);
+    rc = amp_expfit(ctx, s.tft.as<float2>(), box, T, p->minR, p->maxR, s.tfcut.as<float>());
+    if (rc) return rc;
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    s.box = box;
+    s.T = T;
+    s.minR = p->minR;
+    s.maxR = p->maxR;
+    s.ready = true;
+    return CIM_OK;
+}
+
+
+static int launch_search(cim_ctx* ctx, SearchArgs& a, int n, int lds_bytes, bool use_lds, int vpt, int bd) {
+    dim3 grid(a.npts, n);
+#define CIM_LAUNCH(V, L)                                                                                      \
+    do {                                                                                                      \
+        CIM_HIP(hipFuncSetAttribute((const void*)k_cl_search<V, L>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                    lds_bytes));                                                              \
+        hipLaunchKernelGGL((k_cl_search<V, L>), grid, dim3(bd), lds_bytes, ctx->stream, a);                   \
+    } while (0)
+    if (use_lds) {
+        if (vpt == 1) CIM_LAUNCH(1, true);
+        else if (vpt == 2) CIM_LAUNCH(2, true);
+        else CIM_LAUNCH(4, true);
+    } else {
+        if (vpt == 1) CIM_LAUNCH(1, false);
+        else if (vpt == 2) CIM_LAUNCH(2, false);
+        else CIM_LAUNCH(4, false);
+    }
+#undef CIM_LAUNCH
+    CIM_HIP(hipGetLastError());
+    return CIM_OK;
+}
+
+// Global search of n particles already on the device (images_dev) with
+// host records parts (in/out).  pr_array: optional, for n == 1.
+static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts,
+                             float* pr_array) {
+    SearchState& s = ctx->srch;
+    if (!s.ready) return set_error(ctx, CIM_E_STATE, "cim_search_templates has not been called");
+    if (p->box != s.box || p->minR != s.minR || p->maxR != s.maxR)
+        return set_error(ctx, CIM_E_ARG, "box/minR/maxR differ from the template bank");
+    if (!(p->searchstep > 0.f)) return set_error(ctx, CIM_E_ARG, "searchstep must be > 0");
+    if (n <= 0) return CIM_OK;
+    const int box = p->box, R = p->maxR, nr = R - p->minR + 1, T = s.T;
+    const int Dc = 2 * R + 4;
+    const long long N = (long long)box * box;
+    SearchGrid g = make_grid(p->searchstep);
+    if (g.pts.empty()) return set_error(ctx, CIM_E_ARG, "search grid has no AU point for step %g", p->searchstep);
+    const int npts = (int)g.base.size();
+    const int vpt = g.No <= 1024 ? 1 : (g.No <= 2048 ? 2 : 4);
+    if (g.No > 4096) return set_error(ctx, CIM_E_ARG, "searchstep %g too fine (Nomega > 4096)", p->searchstep);
+    int bd = (g.No + vpt - 1) / vpt;
+    bd = ((bd + 63) / 64) * 64;
+    std::vector<int> rows;
+    int disk_total = 0;
+    make_disk(R, rows, disk_total);
+    const int nrows = 2 * R + 4;
+    const size_t misc = sizeof(float) * (64 + 64 + nr) + sizeof(int) * nrows + 16 * 8 + 64;
+    const size_t lds_cap = 160 * 1024;
+    bool use_lds = sizeof(float2) * (size_t)disk_total + misc + sizeof(float2) * 4 * nr <= lds_cap;
+    size_t rem = lds_cap - misc - (use_lds ? sizeof(float2) * (size_t)disk_total : 0);
+    int chunk = (int)std::min<size_t>(60, rem / (sizeof(float2) * nr));
+    if (chunk < 1) return set_error(ctx, CIM_E_ARG, "maxR %d too large for the LDS line buffer", R);
+    const int lds_bytes = (int)(misc + sizeof(float2) * (size_t)chunk * nr +
+                                (use_lds ? sizeof(float2) * (size_t)disk_total : 0));
+
+    // device copies of the small tables
+    std::vector<float> ops(540);
+    {
+        double d[540];
+        cim_icos60_ref_order(d);
+        for (int i = 0; i < 540; ++i) ops[i] = (float)d[i];
+    }
+    SearchBatch& sb = ctx->srch.wk;
+    CIM_ALLOC(sb.ops, sizeof(float) * 540);
+    CIM_ALLOC(sb.rows, sizeof(int) * rows.size());
+    CIM_ALLOC(sb.pts, sizeof(float) * g.pts.size());
+    CIM_ALLOC(sb.pbase, sizeof(int) * g.base.size());
+    CIM_ALLOC(sb.tang, sizeof(float) * 3 * T);
+    CIM_HIP(hipMemcpyAsync(sb.ops.ptr, ops.data(), sizeof(float) * 540, hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipMemcpyAsync(sb.rows.ptr, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipMemcpyAsync(sb.pts.ptr, g.pts.data(), sizeof(float) * g.pts.size(), hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipMemcpyAsync(sb.pbase.ptr, g.base.data(), sizeof(int) * g.base.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    CIM_HIP(hipMemcpyAsync(sb.tang.ptr, s.tang.data(), sizeof(float) * 3 * T, hipMemcpyHostToDevice, ctx->stream));
+
+    const int batch = std::max(1, std::min(n, 256));
+    CtfConst cc = make_ctf_const(p);
+    for (int b0 = 0; b0 < n; b0 += batch) {
+        const int nb = std::min(batch, n - b0);
+        // records: mask around each particle's own centre (ort_globesearch.f90:50)
+        std::vector<float> pr(parts + (size_t)b0 * CIM_PART_STRIDE, parts + (size_t)(b0 + nb) * CIM_PART_STRIDE);
+        for (int i = 0; i < nb; ++i) {
+            pr[i * CIM_PART_STRIDE + CIM_P_MASKX] = pr[i * CIM_PART_STRIDE + CIM_P_X];
+            pr[i * CIM_PART_STRIDE + CIM_P_MASKY] = pr[i * CIM_PART_STRIDE + CIM_P_Y];
+        }
+        int rc = upload_parts(ctx, nb, pr.data());
+        if (rc) return rc;
+        CIM_ALLOC(ctx->ft, sizeof(float2) * N * nb);
+        float2* ft = ctx->ft.as<float2>();
+        // the global search masks unconditionally (ort_globesearch.f90:50, unlike mode 5)
+        rc = preprocess_images(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), ft, 0, 1, 1);
+        if (rc) return rc;
+        CIM_ALLOC(sb.fcut0, sizeof(float) * nr * nb);
+        rc = amp_expfit(ctx, ft, box, nb, p->minR, R, sb.fcut0.as<float>());
+        if (rc) return rc;
+        rc = centre_shift(ctx, ft, box, nb, ctx->parts.as<float>(), 1);
+        if (rc) return rc;
+        CIM_ALLOC(sb.pcrop, sizeof(float2) * (size_t)Dc * Dc * nb);
+        hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream, ft, N, box, R, Dc,
+                           sb.pcrop.as<float2>(), 1, (const float*)nullptr, cc, 0);
+        CIM_HIP(hipGetLastError());
+        SearchArgs a{};
+        if (p->apply_ctf) {
+            CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * nb * T);
+            hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb * T), dim3(256), 0, ctx->stream,
+                               s.tft.as<float2>(), N, box, R, Dc, sb.tcrop.as<float2>(), T, ctx->parts.as<float>(), cc,
+                               1);
+            CIM_HIP(hipGetLastError());
+            CIM_ALLOC(sb.tfcut, sizeof(float) * nr * nb * T);
+            rc = amp_expfit_general(ctx, sb.tcrop.as<float2>(), (long long)Dc * Dc, Dc, R + 1, nb * T, p->minR, R,
+                                    sb.tfcut.as<float>());
+            if (rc) return rc;
+            a.tc_pstride = (long long)T * Dc * Dc;
+            a.tf_pstride = (long long)T * nr;
+        } else {
+            CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * T);
+            hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, T), dim3(256), 0, ctx->stream, s.tft.as<float2>(), N,
+                               box, R, Dc, sb.tcrop.as<float2>(), 1, (const float*)nullptr, cc, 0);
+            CIM_HIP(hipGetLastError());
+            CIM_ALLOC(sb.tfcut, sizeof(float) * nr * T);
+            CIM_HIP(hipMemcpyAsync(sb.tfcut.ptr, s.tfcut.ptr, sizeof(float) * nr * T, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+            a.tc_pstride = 0;
+            a.tf_pstride = 0;
+        }
+        CIM_ALLOC(sb.outv, sizeof(float) * (size_t)npts * nb);
+        CIM_ALLOC(sb.outi, sizeof(int) * (size_t)npts * nb);
+        if (pr_array) CIM_ALLOC(sb.prfull, sizeof(float) * (size_t)npts * g.No * nb);
+        a.pcrop = sb.pcrop.as<float2>();
+        a.fcut0 = sb.fcut0.as<float>();
+        a.tcrop = sb.tcrop.as<float2>();
+        a.tfcut = sb.tfcut.as<float>();
+        a.tang = sb.tang.as<float>();
+        a.T = T;
+        a.pts = sb.pts.as<float>();
+        a.pbase = sb.pbase.as<int>();
+        a.npts = npts;
+        a.ops = sb.ops.as<float>();
+        a.nops = 60;
+        a.rows = sb.rows.as<int>();
+        a.nrows = nrows;
+        a.disk_total = disk_total;
+        a.R = R;
+        a.minR = p->minR;
+        a.nr = nr;
+        a.Dc = Dc;
+        a.Nomega = g.No;
+        a.box = box;
+        a.chunk = chunk;
+        a.step = p->searchstep;
+        a.outv = sb.outv.as<float>();
+        a.outi = sb.outi.as<int>();
+        a.prfull = pr_array ? sb.prfull.as<float>() : nullptr;
+        {
+            TimedLaunch tl(ctx, "cl_search");
+            rc = launch_search(ctx, a, nb, lds_bytes, use_lds, vpt, bd);
+            if (rc) return rc;
+        }
+        CIM_ALLOC(sb.fv, sizeof(float) * nb);
+        CIM_ALLOC(sb.fi, sizeof(int) * nb);
+        hipLaunchKernelGGL(k_argmin, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(), npts,
+                           sb.fv.as<float>(), sb.fi.as<int>());
+        CIM_HIP(hipGetLastError());
+        std::vector<float> hv(nb);
+        std::vector<int> hi(nb);
+        CIM_HIP(hipMemcpyAsync(hv.data(), sb.fv.ptr, sizeof(float) * nb, hipMemcpyDeviceToHost, ctx->stream));
+        CIM_HIP(hipMemcpyAsync(hi.data(), sb.fi.ptr, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<float> full;
+        if (pr_array) {
+            full.resize((size_t)npts * g.No);
+            CIM_HIP(hipMemcpyAsync(full.data(), sb.prfull.ptr, sizeof(float) * full.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        }
+        CIM_HIP(hipStreamSynchronize(ctx->stream));
+        const int plane = g.No * (2 * g.Np + 1);
+        for (int i = 0; i < nb; ++i) {
+            float* q = parts + (size_t)(b0 + i) * CIM_PART_STRIDE;
+            const int idx = hi[i];
+            const int it = idx / plane, ip = (idx % plane) / g.No - g.Np, io = idx % g.No;
+            q[CIM_P_THETA] = 69.0f + p->searchstep * (float)it;
+            q[CIM_P_PHI] = p->searchstep * (float)ip;
+            q[CIM_P_OMEGA] = p->searchstep * (float)io;
+            q[CIM_P_PR] = hv[i];
+        }
+        if (pr_array) {
+            const size_t total = (size_t)plane * (g.Nt + 1);
+            for (size_t i = 0; i < total; ++i) pr_array[i] = 1000.f;
+            for (int pt = 0; pt < npts; ++pt)
+                for (int k = 0; k < g.No; ++k) pr_array[g.base[pt] + k] = full[(size_t)pt * g.No + k];
+        }
+    }
+    return CIM_OK;
+}
+
+// ---- local refinement (mode 5) ------------------------------------------------
+
+struct RefineArgs {
+    const float2* ft;        // n x box^2 particle FTs (masked, FFT, no centre shift)
+    const float* fcut0;      // n x nr
+    const float2* tft;       // template FTs; particle p, template t at tft + p*t_pstride + t*tstride
+    long long t_pstride, tstride;
+    int t_ld, t_c;           // template rows and centre index (box, box/2 or crop)
+    const float* tfcut;      // p*tf_pstride + t*nr
+    long long tf_pstride;
+    const float* tang;
+    int T;
+    const float* ops;        // 60 x 9
+    float* parts;            // n x CIM_PART_STRIDE in/out
+    int box, minR, nr, ncycle;
+    float dang0, dtran0;
+};
+
+// return_icos_asymmetry_unit.f90:14-31 (float32)
+__device__ void return_au_f(const float* ops, float* th, float* ph, float* om) {
+    const float d2r = 0.0174532925199432957692f;
+    const float s31 = sinf(31.717f * d2r), c31 = cosf(31.717f * d2r), s69 = sinf(69.090f * d2r),
+                c69 = cosf(69.090f * d2r);
+    const float A1 = -s31 * c69, B1 = -c31 * c69, C1 = s31 * s69, A2 = -s31 * c69, B2 = c31 * c69, C2 = s31 * s69;
+    float E[9];
+    euler_f(*th, *ph, *om, E);
+    float t = 0.f, p = 0.f, o = 0.f;
+    for (int i = 0; i < 60; ++i) {
+        const float* S = ops + 9 * i;
+        float M[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) M[3 * r + c] = S[3 * r] * E[c] + S[3 * r + 1] * E[3 + c] + S[3 * r + 2] * E[6 + c];
+        t = acosf(M[8]);
+        p = atan2f(M[5], M[2]);
+        o = atan2f(M[7], -M[6]);
+        const float x = sinf(t) * cosf(p), y = sinf(t) * sinf(p), z = cosf(t);
+        const float z1 = -(A1 * x + B1 * y) / C1, z2 = -(A2 * x + B2 * y) / C2;
+        if ((z >= 0.f) && (z <= z1) && (z <= z2)) break;
+    }
+    *th = t / d2r;
+    *ph = p / d2r;
+    *om = o / d2r;
+}
+
+__global__ void __launch_bounds__(256) k_refine(RefineArgs a) {
+    __shared__ float cand[11][5];
+    __shared__ float al1[11][64], al2[11][64];
+    __shared__ float acc_n[11][4], acc_d[11][4];
+    __shared__ float cur[5];
+    __shared__ float fc0s[520];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    const int ic = a.box / 2;
+    const float cf = (float)ic;
+    const float d2r = 0.0174532925199432957692f;
+    const float2* pf = a.ft + (long long)p * a.box * a.box;
+    float* rec = a.parts + (long long)p * CIM_PART_STRIDE;
+    for (int i = tid; i < a.nr; i += blockDim.x) fc0s[i] = a.fcut0[(long long)p * a.nr + i];
+    if (tid < 5) cur[tid] = rec[tid];
+    __syncthreads();
+    float PR0 = 0.f;
+    float f = 1.f;
+    for (int cyc = 1; cyc <= a.ncycle; ++cyc) {
+        const float dang = a.dang0 * f, dtran = a.dtran0 * f;
+        f *= 0.8f;
+        // ort_cent_11.f90:6-91
+        if (tid == 0) {
+            float th0 = cur[0], ph0 = cur[1], om0 = cur[2];
+            return_au_f(a.ops, &th0, &ph0, &om0);
+            for (int jj = 0; jj <= 10; ++jj) {
+                float o[5] = {th0, ph0, om0, cur[3], cur[4]};
+                switch (jj) {
+                    case 1: o[0] = th0 - dang; break;
+                    case 2: o[0] = th0 + dang; break;
+                    case 3: o[1] = ph0 - dang; break;
+                    case 4: o[1] = ph0 + dang; break;
+                    case 5: o[2] = om0 - dang; break;
+                    case 6: o[2] = om0 + dang; break;
+                    case 7: o[3] = cur[3] - dtran; break;
+                    case 8: o[3] = cur[3] + dtran; break;
+                    case 9: o[4] = cur[4] - dtran; break;
+                    case 10: o[4] = cur[4] + dtran; break;
+                }
+                cand[jj][0] = o[0]; cand[jj][1] = o[1]; cand[jj][2] = o[2]; cand[jj][3] = o[3]; cand[jj][4] = o[4];
+            }
+        }
+        __syncthreads();
+        if (tid >= 1 && tid <= 10) return_au_f(a.ops, &cand[tid][0], &cand[tid][1], &cand[tid][2]);
+        __syncthreads();
+        float num[11], den[11];
+        for (int c = 0; c < 11; ++c) { num[c] = 0.f; den[c] = 0.f; }
+        for (int t = 0; t < a.T; ++t) {
+            const float2* tf = a.tft + (long long)p * a.t_pstride + (long long)t * a.tstride;
+            const float* tfc = a.tfcut + (long long)p * a.tf_pstride + (long long)t * a.nr;
+            for (int q = tid; q < 11 * 60; q += blockDim.x) {
+                const int c = q / 60, j = q % 60;
+                float E1[9], E2[9];
+                euler_f(cand[c][0], cand[c][1], cand[c][2], E1);
+                euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
+                crosslin_one(E1, E2, a.ops + 9 * j, &al1[c][j], &al2[c][j]);
+            }
+            __syncthreads();
+            // samples: (line j, radius) per thread, all 11 candidates
+            for (int q = tid; q < 60 * a.nr; q += blockDim.x) {
+                const int j = q / a.nr, fr = q % a.nr;
+                const float FR = (float)(a.minR + fr);
+                for (int c = 0; c < 11; ++c) {
+                    const float A1 = al1[c][j], A2 = al2[c][j];
+                    if (!(A1 < 999.f && A2 < 999.f)) continue;
+                    // particle tap with the trial shift (cal_crosslin_PR.f90:22,41)
+                    float x = FR * cosf(A1 * d2r) + cf, y = FR * sinf(A1 * d2r) + cf;
+                    int lx = (int)x, ly = (int)y;
+                    float ax = x - (float)lx, ay = y - (float)ly;
+                    float2 zt[4];
+                    const int dx[4] = {0, 1, 0, 1}, dy[4] = {0, 0, 1, 1};
+                    for (int k = 0; k < 4; ++k) {
+                        const int X = lx + dx[k], Y = ly + dy[k];
+                        float2 v = pf[(long long)Y * a.box + X];
+                        float2 cs = shift_phase(X - ic, Y - ic, a.box, cand[c][3], cand[c][4], 1);
+                        zt[k] = apply_shift(v, cs);
+                    }
+                    const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay,
+                                w11 = ax * ay;
+                    float re1 = w00 * zt[0].x + w10 * zt[1].x + w01 * zt[2].x + w11 * zt[3].x;
+                    float im1 = w00 * zt[0].y + w10 * zt[1].y + w01 * zt[2].y + w11 * zt[3].y;
+                    // template tap
+                    x = FR * cosf(A2 * d2r) + cf;
+                    y = FR * sinf(A2 * d2r) + cf;
+                    lx = (int)x;
+                    ly = (int)y;
+                    ax = x - (float)lx;
+                    ay = y - (float)ly;
+                    const int tx = lx - ic + a.t_c, ty = ly - ic + a.t_c;
+                    const float2* r0 = tf + (long long)ty * a.t_ld + tx;
+                    const float2* r1 = r0 + a.t_ld;
+                    const float u00 = (1.f - ax) * (1.f - ay), u10 = ax * (1.f - ay), u01 = (1.f - ax) * ay,
+                                u11 = ax * ay;
+                    float re2 = u00 * r0[0].x + u10 * r0[1].x + u01 * r1[0].x + u11 * r1[1].x;
+                    float im2 = u00 * r0[0].y + u10 * r0[1].y + u01 * r1[0].y + u11 * r1[1].y;
+                    const float amp1 = sqrtf(re1 * re1 + im1 * im1), amp2 = sqrtf(re2 * re2 + im2 * im2);
+                    if (amp1 >= fc0s[fr] && amp2 >= tfc[fr]) {
+                        const float diff = fold_phase(atan2f(im2, re2) - atan2f(im1, re1));
+                        const float w = 0.5f * (amp1 + amp2);
+                        num[c] += diff * w;
+                        den[c] += w;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // block reduction of the 11 (num, den) pairs
+        for (int c = 0; c < 11; ++c) {
+            float nv = wave_sum(num[c]), dv = wave_sum(den[c]);
+            if ((tid & 63) == 0) { acc_n[c][tid >> 6] = nv; acc_d[c][tid >> 6] = dv; }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float best = 0.f;
+            int bi = 0;
+            for (int c = 0; c < 11; ++c) {
+                float nn = 0.f, dd = 0.f;
+                for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { nn += acc_n[c][w]; dd += acc_d[c][w]; }
+                const float pr = dd > 1.0e-10f ? nn / dd * 57.2957795130823208768f : 1000.f;
+                if (c == 0 || pr < best) { best = pr; bi = c; }
+            }
+            PR0 = best;
+            for (int k = 0; k < 5; ++k) cur[k] = cand[bi][k];
+            acc_n[0][0] = best;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int k = 0; k < 5; ++k) rec[k] = cur[k];
+        rec[CIM_P_PR] = PR0;
+    }
+}
+
+static int refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* images, float* parts) {
+    SearchState& s = ctx->srch;
+    if (!s.ready) return set_error(ctx, CIM_E_STATE, "cim_search_templates has not been called");
+    if (p->box != s.box || p->minR != s.minR || p->maxR != s.maxR)
+        return set_error(ctx, CIM_E_ARG, "box/minR/maxR differ from the template bank");
+    if (n <= 0) return CIM_OK;
+    const int box = p->box, R = p->maxR, nr = R - p->minR + 1, T = s.T;
+    if (nr > 520) return set_error(ctx, CIM_E_ARG, "too many radii (%d)", nr);
+    const long long N = (long long)box * box;
+    CIM_ALLOC(ctx->img, sizeof(float) * N * n);
+    CIM_HIP(hipMemcpyAsync(ctx->img.ptr, images, sizeof(float) * N * n, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<float> pr(parts, parts + (size_t)n * CIM_PART_STRIDE);
+    for (int i = 0; i < n; ++i) {
+        pr[i * CIM_PART_STRIDE + CIM_P_MASKX] = pr[i * CIM_PART_STRIDE + CIM_P_X];
+        pr[i * CIM_PART_STRIDE + CIM_P_MASKY] = pr[i * CIM_PART_STRIDE + CIM_P_Y];
+    }
+    int rc = upload_parts(ctx, n, pr.data());
+    if (rc) return rc;
+    CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
+    float2* ft = ctx->ft.as<float2>();
+    // mask only when imgmask is on (ort_refinement.f90:46), no centre shift (:50)
+    rc = preprocess_images(ctx, p, n, ctx->img.as<float>(), ctx->parts.as<float>(), ft, 0, p->mask_on, 1);
+    if (rc) return rc;
+    SearchBatch& sb = ctx->srch.wk;
+    CIM_ALLOC(sb.fcut0, sizeof(float) * nr * n);
+    rc = amp_expfit(ctx, ft, box, n, p->minR, R, sb.fcut0.as<float>());
+    if (rc) return rc;
+    CtfConst cc = make_ctf_const(p);
+    RefineArgs a{};
+    const int Dc = 2 * R + 4;
+    if (p->apply_ctf) {
+        CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * n * T);
+        hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, n * T), dim3(256), 0, ctx->stream, s.tft.as<float2>(), N,
+                           box, R, Dc, sb.tcrop.as<float2>(), T, ctx->parts.as<float>(), cc, 1);
+        CIM_HIP(hipGetLastError());
+        CIM_ALLOC(sb.tfcut, sizeof(float) * nr * n * T);
+        rc = amp_expfit_general(ctx, sb.tcrop.as<float2>(), (long long)Dc * Dc, Dc, R + 1, n * T, p->minR, R,
+                                sb.tfcut.as<float>());
+        if (rc) return rc;
+        a.tft = sb.tcrop.as<float2>();
+        a.t_pstride = (long long)T * Dc * Dc;
+        a.tstride = (long long)Dc * Dc;
+        a.t_ld = Dc;
+        a.t_c = R + 1;
+        a.tfcut = sb.tfcut.as<float>();
+        a.tf_pstride = (long long)T * nr;
+    } else {
+        a.tft = s.tft.as<float2>();
+        a.t_pstride = 0;
+        a.tstride = N;
+        a.t_ld = box;
+        a.t_c = box / 2;
+        a.tfcut = s.tfcut.as<float>();
+        a.tf_pstride = 0;
+    }
+    std::vector<float> ops(540);
+    {
+        double d[540];
+        cim_icos60_ref_order(d);
+        for (int i = 0; i < 540; ++i) ops[i] = (float)d[i];
+    }
+    CIM_ALLOC(sb.ops, sizeof(float) * 540);
+    CIM_ALLOC(sb.tang, sizeof(float) * 3 * T);
+    CIM_HIP(hipMemcpyAsync(sb.ops.ptr, ops.data(), sizeof(float) * 540, hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipMemcpyAsync(sb.tang.ptr, s.tang.data(), sizeof(float) * 3 * T, hipMemcpyHostToDevice, ctx->stream));
+    a.ft = ft;
+    a.fcut0 = sb.fcut0.as<float>();
+    a.tang = sb.tang.as<float>();
+    a.T = T;
+    a.ops = sb.ops.as<float>();
+    a.parts = ctx->parts.as<float>();
+    a.box = box;
+    a.minR = p->minR;
+    a.nr = nr;
+    a.ncycle = p->ncycle;
+    a.dang0 = p->delta_angle;
+    a.dtran0 = p->delta_trans;
+    {
+        TimedLaunch tl(ctx, "refine");
+        hipLaunchKernelGGL(k_refine, dim3(n), dim3(256), 0, ctx->stream, a);
+        CIM_HIP(hipGetLastError());
+    }
+    CIM_HIP(hipMemcpyAsync(pr.data(), ctx->parts.ptr, sizeof(float) * CIM_PART_STRIDE * n, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i)
+        for (int k : {CIM_P_THETA, CIM_P_PHI, CIM_P_OMEGA, CIM_P_X, CIM_P_Y, CIM_P_PR})
+            parts[i * CIM_PART_STRIDE + k] = pr[i * CIM_PART_STRIDE + k];
+    return CIM_OK;
+}
+
+}  // namespace cim
+
+using namespace cim;
+
+extern "C" {
+
+int cim_search_templates(cim_ctx* ctx, const cim_params* p, int T, const float* images, const float* tort) {
+    if (!ctx) return CIM_E_ARG;
+    int rc = check_params_public(ctx, p);
+    if (rc) return rc;
+    if (T < 1 || !images || !tort) return set_error(ctx, CIM_E_ARG, "need T >= 1 templates");
+    CIM_HIP(hipSetDevice(ctx->device));
+    return prep_templates(ctx, p, T, images, tort);
+}
+
+int cim_search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const void* images_dev, float* parts) {
+    if (!ctx) return CIM_E_ARG;
+    int rc = check_params_public(ctx, p);
+    if (rc) return rc;
+    if (!images_dev || !parts) return set_error(ctx, CIM_E_ARG, "NULL images or parts");
+    CIM_HIP(hipSetDevice(ctx->device));
+    return search_global_dev(ctx, p, n, (const float*)images_dev, parts, nullptr);
+}
+
+int cim_search_global(cim_ctx* ctx, const cim_params* p, int n, const float* images, float* parts) {
+    if (!ctx) return CIM_E_ARG;
+    int rc = check_params_public(ctx, p);
+    if (rc) return rc;
+    if (!images || !parts) return set_error(ctx, CIM_E_ARG, "NULL images or parts");
+    if (n <= 0) return CIM_OK;
+    CIM_HIP(hipSetDevice(ctx->device));
+    const long long N = (long long)p->box * p->box;
+    CIM_ALLOC(ctx->img, sizeof(float) * N * n);
+    CIM_HIP(hipMemcpyAsync(ctx->img.ptr, images, sizeof(float) * N * n, hipMemcpyHostToDevice, ctx->stream));
+    return search_global_dev(ctx, p, n, ctx->img.as<float>(), parts, nullptr);
+}
+
+int cim_search_pr_array(cim_ctx* ctx, const cim_params* p, const float* image, const float* part, float* pr_array) {
+    if (!ctx) return CIM_E_ARG;
+    int rc = check_params_public(ctx, p);
+    if (rc) return rc;
+    if (!image || !part || !pr_array) return set_error(ctx, CIM_E_ARG, "NULL argument");
+    CIM_HIP(hipSetDevice(ctx->device));
+    const long long N = (long long)p->box * p->box;
+    CIM_ALLOC(ctx->img, sizeof(float) * N);
+    CIM_HIP(hipMemcpyAsync(ctx->img.ptr, image, sizeof(float) * N, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<float> rec(part, part + CIM_PART_STRIDE);
+    return search_global_dev(ctx, p, 1, ctx->img.as<float>(), rec.data(), pr_array);
+}
+
+int cim_refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* images, float* parts) {
+    if (!ctx) return CIM_E_ARG;
+    int rc = check_params_public(ctx, p);
+    if (rc) return rc;
+    if (!images || !parts) return set_error(ctx, CIM_E_ARG, "NULL images or parts");
+    CIM_HIP(hipSetDevice(ctx->device));
+    return refine_local(ctx, p, n, images, parts);
+}
+
+}  // extern "C"

@@ -115,16 +115,20 @@ EXPORT void o_crosslin_angle(real th1, real ph1, real om1, real th2, real ph2, r
 }
 
 /* Asymmetric-unit test used by ort_globesearch.f90:17-18,80-82 and
- * return_icos_asymmetry_unit.f90:9-12,22-27. */
-static int in_au(real x, real y, real z)
+ * return_icos_asymmetry_unit.f90:9-12,22-27.  Evaluated in float32 in both
+ * oracle builds: it is a discrete decision the reference takes in REAL*4,
+ * and at theta = 90 deg cos() rounds to -4.4e-8 < 0, so REAL*4 excludes
+ * the whole equator row of the search grid where float64 would keep it. */
+static int in_au(real xr, real yr, real zr)
 {
-    real d2r = r_deg2rad();
-    real s31 = (real)sin((real)31.717 * d2r), c31 = (real)cos((real)31.717 * d2r);
-    real s69 = (real)sin((real)69.09 * d2r), c69 = (real)cos((real)69.09 * d2r);
-    real A1 = -s31 * c69, B1 = -c31 * c69, C1 = s31 * s69;
-    real A2 = -s31 * c69, B2 = c31 * c69, C2 = s31 * s69;
-    real z1 = -(A1 * x + B1 * y) / C1;
-    real z2 = -(A2 * x + B2 * y) / C2;
+    float x = (float)xr, y = (float)yr, z = (float)zr;
+    float d2r = atan2f(1.0f, 1.0f) / 45.0f;
+    float s31 = sinf(31.717f * d2r), c31 = cosf(31.717f * d2r);
+    float s69 = sinf(69.09f * d2r), c69 = cosf(69.09f * d2r);
+    float A1 = -s31 * c69, B1 = -c31 * c69, C1 = s31 * s69;
+    float A2 = -s31 * c69, B2 = c31 * c69, C2 = s31 * s69;
+    float z1 = -(A1 * x + B1 * y) / C1;
+    float z2 = -(A2 * x + B2 * y) / C2;
     return (z >= 0) && (z <= z1) && (z <= z2);
 }
 
@@ -531,16 +535,17 @@ EXPORT void o_search_grid(real step, int* Ntheta, int* Nphi, int* Nomega)
     *Nomega = (int)((real)360.0 / step);
 }
 
-/* AU membership of grid point (itheta, iphi): ort_globesearch.f90:68-82. */
+/* AU membership of grid point (itheta, iphi): ort_globesearch.f90:68-82,
+ * in float32 as the reference (see in_au). */
 EXPORT int o_grid_in_au(real step, int itheta, int iphi)
 {
-    real d2r = r_deg2rad();
-    real theta = (real)69.0 + step * (real)itheta;
-    real tmp = theta * d2r;
-    real z = (real)cos(tmp), z0 = (real)sin(tmp);
-    real phi = step * (real)iphi;
+    float d2r = atan2f(1.0f, 1.0f) / 45.0f;
+    float theta = 69.0f + (float)step * (float)itheta;
+    float tmp = theta * d2r;
+    float z = cosf(tmp), z0 = sinf(tmp);
+    float phi = (float)step * (float)iphi;
     tmp = phi * d2r;
-    return in_au(z0 * (real)cos(tmp), z0 * (real)sin(tmp), z);
+    return in_au(z0 * cosf(tmp), z0 * sinf(tmp), z);
 }
 
 /* ort_globesearch for one particle: icosprocess/SRC/ort_globesearch.f90:68-106.

@@ -107,7 +107,7 @@ class Oracle:
     def amp_expfit(self, a, b, minR, maxR):
         a, b = self._a(a), self._a(b)
         f = np.zeros(maxR - minR + 1, self.dt)
-        self.L.o_amp_expfit(self._p(a), self._p(b), a.shape[-1], minR, maxR, self._p(f))
+        self.L.o_amp_expfit(self._p(a), self._p(b), a.shape[-1], int(minR), int(maxR), self._p(f))
         return f
 
     # ---- pipelines -------------------------------------------------------------
@@ -132,7 +132,7 @@ class Oracle:
         ops = self._a(ops).reshape(-1, 9)
         R = self._a(self.euler_mat(*angles)).reshape(9)
         c = None if ctf is None else self._a(ctf)
-        self.L.o_insert(self._p(fa), self._p(fb), self._p(wg), int(compact), nx, maxR,
+        self.L.o_insert(self._p(fa), self._p(fb), self._p(wg), int(compact), nx, int(maxR),
                         self._r(maxR if rcut is None else rcut), self._p(self._a(a)), self._p(self._a(b)),
                         None if c is None else self._p(c), self._p(R), ops.shape[0], self._p(ops))
 
@@ -146,7 +146,7 @@ class Oracle:
     def fold_and_map(self, vol, maxR):
         fa, fb, wg = (v.copy() for v in vol)
         nx = fa.shape[-1]
-        self.L.o_fold_normalise(self._p(fa), self._p(fb), self._p(wg), nx, maxR)
+        self.L.o_fold_normalise(self._p(fa), self._p(fb), self._p(wg), nx, int(maxR))
         m = np.zeros_like(fa)
         self.L.o_volume_to_map(self._p(fa), self._p(fb), nx, self._p(m))
         return m
@@ -165,7 +165,7 @@ class Oracle:
         T = len(teul)
         self.L.o_crosslin_pr_omega(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)), T,
                                    self._p(self._a(ta)), self._p(self._a(tb)), self._p(self._a(tfcut)),
-                                   self._p(self._a(teul)), nx, minR, maxR, 60, self._p(self._a(ops)),
+                                   self._p(self._a(teul)), nx, int(minR), int(maxR), 60, self._p(self._a(ops)),
                                    self._r(theta), self._r(phi), No, self._r(step), self._p(out))
         return out
 
@@ -177,7 +177,7 @@ class Oracle:
         arr = np.zeros((Nt + 1) * (2 * Np + 1) * No, self.dt) if want_array else None
         self.L.o_globesearch(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)), len(teul),
                              self._p(self._a(ta)), self._p(self._a(tb)), self._p(self._a(tfcut)),
-                             self._p(self._a(teul)), nx, minR, maxR, 60, self._p(ops), self._r(step),
+                             self._p(self._a(teul)), nx, int(minR), int(maxR), 60, self._p(ops), self._r(step),
                              self._p(out4), None if arr is None else self._p(arr))
         if want_array:
             return out4, arr.reshape(Nt + 1, 2 * Np + 1, No)
@@ -188,7 +188,7 @@ class Oracle:
         ops = self._a(self.icos_ops().reshape(-1))
         return self.L.o_crosslin_pr(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)), len(teul),
                                     self._p(self._a(ta)), self._p(self._a(tb)), self._p(self._a(tfcut)),
-                                    self._p(self._a(teul)), nx, minR, maxR, 60, self._p(ops),
+                                    self._p(self._a(teul)), nx, int(minR), int(maxR), 60, self._p(ops),
                                     *[self._r(v) for v in (th, ph, om, x, y)])
 
     def refine_particle(self, pa, pb, fcut0, ta, tb, tfcut, teul, minR, maxR, ncycle, dang, dtran, start5):
@@ -197,7 +197,7 @@ class Oracle:
         io5 = self._a(start5).copy()
         pr = self.L.o_refine_particle(self._p(self._a(pa)), self._p(self._a(pb)), self._p(self._a(fcut0)),
                                       len(teul), self._p(self._a(ta)), self._p(self._a(tb)),
-                                      self._p(self._a(tfcut)), self._p(self._a(teul)), nx, minR, maxR, 60,
+                                      self._p(self._a(tfcut)), self._p(self._a(teul)), nx, int(minR), int(maxR), 60,
                                       self._p(ops), ncycle, self._r(dang), self._r(dtran), self._p(io5))
         return io5, pr
 

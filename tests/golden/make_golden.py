@@ -1,0 +1,234 @@
+"""Record golden vectors from the reference's own Fortran (built by
+oracle/build_ref.sh into oracle/_ref/).  Runs only where /root/reference was
+available to build it; the GPU box and the CPU test suite only read the
+fixtures this script writes (tests/golden/*.npz).
+
+Two kinds of vectors:
+* unit vectors: oracle/_ref/icos_units and lref_units call one reference
+  routine on seeded inputs (symmetry table, euler_mat, getCTF2d, img2d_mask,
+  img2d_norm, img2d_FFT, img2d_FT_centshift, FT_amp_expfit,
+  icos_crosslin_angle, return_icos_asymmetry_unit, cal_crosslin_PR_omega,
+  cal_crosslin_PR, FFT3d_slice, cal_PR_FT, img2d_FTphase, img3d_softmask,
+  img3d_FFT);
+* program vectors: the real icosprocess (modes 1, 1+subrec, 4, 5) and
+  local_refine executables run on a small synthetic data set written in the
+  reference's own MRC / ortfile formats.
+
+Usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from cryoim_amd import formats, synth  # noqa: E402
+
+f4 = np.float32
+i4 = np.int32
+
+
+def _run_unit(exe, case, payload: bytes, nout: int) -> np.ndarray:
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(payload)
+        subprocess.run([os.path.join(REF, exe), case, fin, fout], check=True, stdout=subprocess.DEVNULL)
+        out = np.fromfile(fout, dtype="<f4")
+    assert out.size == nout, (case, out.size, nout)
+    return out
+
+
+def _b(*arrs) -> bytes:
+    return b"".join(np.ascontiguousarray(a).astype(a.dtype.newbyteorder("<")).tobytes() for a in arrs)
+
+
+def unit_vectors(rng) -> dict:
+    g = {}
+    g["symops"] = _run_unit("icos_units", "symops", b"", 540).reshape(60, 3, 3)
+    ang = np.c_[rng.uniform(0, 180, 16), rng.uniform(-180, 180, 16), rng.uniform(0, 360, 16)].astype(f4)
+    g["euler_in"] = ang
+    g["euler_out"] = _run_unit("icos_units", "euler", _b(np.array([16], i4), ang), 16 * 9).reshape(16, 3, 3)
+    # CTF (icosprocess units: um, kV, mm)
+    ctfp = np.array([[1.5, 1.7, 30.0, 1.5, 300.0, 2.7, 0.1], [2.8, 2.8, 0.0, 2.0, 200.0, 2.0, 0.07]], f4)
+    g["ctf_in"] = ctfp
+    g["ctf_out"] = np.stack([_run_unit("icos_units", "ctf", _b(np.array([40], i4), c), 1600).reshape(40, 40)
+                             for c in ctfp])
+    img = rng.standard_normal((2, 40, 40)).astype(f4) + 0.3
+    mp = np.array([[14.0, 1.5, -2.0], [17.0, 0.0, 0.0]], f4)
+    g["mask_in"], g["mask_par"] = img, mp
+    g["mask_out"] = np.stack([_run_unit("icos_units", "mask", _b(np.array([40], i4), mp[k], img[k]), 1600)
+                              .reshape(40, 40) for k in range(2)])
+    g["norm_in"] = img[0] * 3 + 1
+    g["norm_out"] = _run_unit("icos_units", "norm", _b(np.array([40], i4), g["norm_in"]), 1600).reshape(40, 40)
+    a = rng.standard_normal((24, 24)).astype(f4)
+    b = rng.standard_normal((24, 24)).astype(f4)
+    o = _run_unit("icos_units", "fft2", _b(np.array([24], i4), a, b), 4 * 576).reshape(4, 24, 24)
+    g["fft2_in"] = np.stack([a, b])
+    g["fft2_out"] = o                      # fwd a, fwd b, then backward of that
+    g["shift_in"] = np.stack([a, b])
+    g["shift_xy"] = np.array([2.3, -1.7], f4)
+    g["shift_out"] = _run_unit("icos_units", "centshift", _b(np.array([24], i4), g["shift_xy"], a, b),
+                               2 * 576).reshape(2, 24, 24)
+    # FT_amp_expfit on a synthetic decaying transform
+    nx, mn, mx = 64, 1, 20
+    ky, hx = np.meshgrid(np.arange(nx) - nx // 2, np.arange(nx) - nx // 2, indexing="ij")
+    r = np.sqrt(hx ** 2 + ky ** 2)
+    amp = np.exp(-0.08 * r) * (1 + 0.3 * rng.standard_normal((nx, nx)))
+    ph = rng.uniform(-np.pi, np.pi, (nx, nx))
+    ea, eb = (amp * np.cos(ph)).astype(f4), (amp * np.sin(ph)).astype(f4)
+    g["expfit_in"] = np.stack([ea, eb])
+    g["expfit_r"] = np.array([mn, mx], i4)
+    g["expfit_out"] = _run_unit("icos_units", "expfit", _b(np.array([nx, mn, mx], i4), ea, eb), mx - mn + 1)
+    pairs = np.c_[rng.uniform(60, 95, 12), rng.uniform(-40, 40, 12), rng.uniform(0, 360, 12),
+                  rng.uniform(0, 180, 12), rng.uniform(-180, 180, 12), rng.uniform(0, 360, 12)].astype(f4)
+    g["crosslin_in"] = pairs
+    g["crosslin_out"] = _run_unit("icos_units", "crosslin", _b(np.array([12], i4), pairs), 12 * 120).reshape(12, 2, 60)
+    au = np.c_[rng.uniform(0, 180, 30), rng.uniform(-180, 180, 30), rng.uniform(0, 360, 30)].astype(f4)
+    g["au_in"] = au
+    g["au_out"] = _run_unit("icos_units", "au", _b(np.array([30], i4), au), 90).reshape(30, 3)
+    return g
+
+
+def pr_vectors(rng) -> dict:
+    """cal_crosslin_PR_omega / cal_crosslin_PR on synthetic transforms."""
+    from oracle import Oracle
+    O = Oracle(32)
+    g = {}
+    box, mn, mx, T, step = 48, 1, 10, 2, 6.0
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=5)
+    tang = np.array([[75.0, 10.0, 30.0], [85.0, -12.0, 200.0]])
+    timgs = synth.project(ph, tang)
+    ps = synth.make_particles(ph, 1, seed=6, shift_max=1.5, snr=2.0, ctf=True, apix=2.0, df_range=(1.0, 2.0))
+    x, y = ps.shifts[0]
+    a0, b0 = O.particle_ft(ps.images[0], x, y, mask_r=20.0, shift=False)
+    fc0 = O.amp_expfit(a0, b0, mn, mx)
+    pa, pb = O.ft_shift(a0, b0, x, y, True)
+    ta, tb, tf = [], [], []
+    for t in range(T):
+        a, b = O.particle_ft(timgs[t], 0.0, 0.0, mask_r=20.0)
+        ta.append(a); tb.append(b); tf.append(O.amp_expfit(a, b, mn, mx))
+    ta, tb, tf = (np.array(v, f4) for v in (ta, tb, tf))
+    ctf = O.ctf2d(box, ps.df[0, 0], ps.df[0, 1], 0.0, 2.0).astype(f4)
+    g.update(pr_box=np.array([box, mn, mx, T], i4), pr_step=np.array([step], f4), pr_pa=pa, pr_pb=pb,
+             pr_a0=a0, pr_b0=b0, pr_fc0=fc0, pr_ta=ta, pr_tb=tb, pr_tfcut=tf, pr_tang=tang.astype(f4), pr_ctf=ctf)
+    cases = [(78.0, 6.0, 0.0, 0.0, 0.0), (84.0, -18.0, 0.0, 0.0, 0.0)]
+    outs = []
+    for actf in (1, 0):
+        for (th, phv, om, xx, yy) in cases:
+            payload = _b(np.array([box, mn, mx, T, actf], i4), np.array([step, th, phv, om, xx, yy], f4), pa, pb,
+                         fc0, *[v for t in range(T) for v in (ta[t], tb[t])], tang.astype(f4), tf, ctf)
+            outs.append(_run_unit("icos_units", "pr_omega", payload, int(360 / step)))
+    g["pr_omega_cases"] = np.array(cases, f4)
+    g["pr_omega_out"] = np.array(outs).reshape(2, len(cases), -1)
+    pcases = [(79.0, 5.0, 40.0, 0.7, -0.4), (72.0, -3.0, 300.0, -1.2, 0.9)]
+    pouts = []
+    for actf in (1, 0):
+        for (th, phv, om, xx, yy) in pcases:
+            payload = _b(np.array([box, mn, mx, T, actf], i4), np.array([step, th, phv, om, xx, yy], f4), a0, b0,
+                         fc0, *[v for t in range(T) for v in (ta[t], tb[t])], tang.astype(f4), tf, ctf)
+            pouts.append(_run_unit("icos_units", "pr", payload, 1)[0])
+    g["pr_cases"] = np.array(pcases, f4)
+    g["pr_out"] = np.array(pouts, f4).reshape(2, len(pcases))
+    return g
+
+
+def lref_vectors(rng) -> dict:
+    g = {}
+    nx = 24
+    fr = rng.standard_normal((nx, nx, nx)).astype(f4)
+    fi = rng.standard_normal((nx, nx, nx)).astype(f4)
+    ang = np.array([[33.0, 71.0, 250.0], [90.0, 0.0, 0.0]], f4)
+    g["slice_in"] = np.stack([fr, fi])
+    g["slice_ang"] = ang
+    g["slice_out"] = np.stack([_run_unit("lref_units", "slice", _b(np.array([nx], i4), a, fr, fi), 2 * nx * nx)
+                               .reshape(2, nx, nx) for a in ang])
+    a1, b1, a2, b2 = (rng.standard_normal((32, 32)).astype(f4) for _ in range(4))
+    g["prft_in"] = np.stack([a1, b1, a2, b2])
+    g["prft_r"] = np.array([[-1.0, 9.5], [3.0, 14.0]], f4)
+    g["prft_out"] = np.array([_run_unit("lref_units", "prft", _b(np.array([32], i4), r, a1, b1, a2, b2), 1)[0]
+                              for r in g["prft_r"]], f4)
+    g["ftphase_xy"] = np.array([1.3, -2.2], f4)
+    g["ftphase_out"] = _run_unit("lref_units", "ftphase", _b(np.array([32], i4), g["ftphase_xy"], a1, b1),
+                                 2 * 1024).reshape(2, 32, 32)
+    vol = rng.standard_normal((nx, nx, nx)).astype(f4) + 0.5
+    g["softmask_in"] = vol
+    g["softmask_r"] = np.array([8.5], f4)
+    g["softmask_out"] = _run_unit("lref_units", "softmask", _b(np.array([nx], i4), g["softmask_r"], vol),
+                                  nx ** 3).reshape(nx, nx, nx)
+    v12r, v12i = rng.standard_normal((2, 12, 12, 12)).astype(f4)
+    g["fft3_in"] = np.stack([v12r, v12i])
+    g["fft3_out"] = _run_unit("lref_units", "fft3", _b(np.array([12], i4), v12r, v12i), 2 * 1728).reshape(2, 12, 12, 12)
+    return g
+
+
+def program_vectors() -> dict:
+    """Real icosprocess / local_refine runs on a 48-pixel synthetic set."""
+    g = {}
+    box, apix, n = 48, 2.0, 30
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    tang = np.array([[72.0, 8.0, 20.0], [80.0, -15.0, 140.0], [88.0, 3.0, 300.0]])
+    timgs = synth.project(ph, tang)
+    g.update(prog_images=ps.images, prog_angles=ps.angles.astype(f4), prog_shifts=ps.shifts.astype(f4),
+             prog_df=ps.df.astype(f4), prog_tang=tang.astype(f4), prog_timgs=timgs.astype(f4),
+             prog_box=np.array([box], i4), prog_apix=np.array([apix], f4))
+    exe = os.path.join(REF, "icosprocess")
+    with tempfile.TemporaryDirectory() as d:
+        stk = "stack.mrc"
+        formats.write_mrc(os.path.join(d, stk), ps.images)
+        formats.write_mrc(os.path.join(d, "tmpl.mrc"), timgs)
+        formats.write_template_ort(os.path.join(d, "tort.txt"), tang)
+        parts = [formats.Particle(stk, i + 1, *ps.angles[i], *ps.shifts[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+        common = ["apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt", f"last={n}"]
+
+        def run(args):
+            r = subprocess.run([exe, *args], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            return r.stdout
+
+        run(["mode=1", "result3d=map.mrc", *common])
+        g["mode1_map"] = formats.read_mrc(os.path.join(d, "map.mrc"))
+        run(["mode=1", "subrec=y", "result3d=sub.tnf", *common])
+        t = formats.read_tnf(os.path.join(d, "sub.tnf"))
+        g["mode1sub_fa"], g["mode1sub_fb"], g["mode1sub_wg"] = t["fa"], t["fb"], t["wg"]
+        g["mode1sub_hdr"] = np.array([t["fft_size"], t["max_r"]], i4)
+        # search on the first 6 particles, 6 degree grid
+        run(["mode=4", "templateort=tort.txt", "templatestck=tmpl.mrc", "searchstep=6.0", "newortfile=m4.txt",
+             "apix=2.0", "maxRes=6.0", "minRes=100", "imgmask=40", "ortfile=ort.txt", "last=6"])
+        m4 = formats.read_ortfile900(os.path.join(d, "m4.txt"))
+        g["mode4_out"] = np.array([[p.theta, p.phi, p.omega, p.x, p.y, p.pr] for p in m4], f4)
+        run(["mode=5", "templateort=tort.txt", "templatestck=tmpl.mrc", "Ncycle=3", "newortfile=m5.txt",
+             "apix=2.0", "maxRes=6.0", "minRes=100", "imgmask=40", "ortfile=ort.txt", "last=6"])
+        m5 = formats.read_ortfile900(os.path.join(d, "m5.txt"))
+        g["mode5_out"] = np.array([[p.theta, p.phi, p.omega, p.x, p.y, p.pr] for p in m5], f4)
+    return g
+
+
+def main():
+    rng = np.random.default_rng(20261203)
+    if not os.path.exists(os.path.join(REF, "icos_units")):
+        raise SystemExit("oracle/_ref is not built; run oracle/build_ref.sh first")
+    np.savez_compressed(os.path.join(HERE, "units.npz"), **unit_vectors(rng))
+    np.savez_compressed(os.path.join(HERE, "pr.npz"), **pr_vectors(rng))
+    np.savez_compressed(os.path.join(HERE, "lref_units.npz"), **lref_vectors(rng))
+    np.savez_compressed(os.path.join(HERE, "programs.npz"), **program_vectors())
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

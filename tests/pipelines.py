@@ -1,0 +1,64 @@
+"""Oracle-side restatements of whole icosprocess modes, used as checkers.
+
+mode1_map / mode1_substep follow icosprocess/SRC/reconstruct.f90:53-247 and
+reconstruct_substep.f90:50-147; mode4 follows ort_globesearch.f90:34-106
+with load_template.f90:35-48; mode5 follows ort_refinement.f90:31-95.
+Parameters are derived as calc_initialvalues does
+(icosprocess/SRC/0calc_initialvalues.f90:6-21).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def derived(box, apix, maxRes=8.0, minRes=300.0, imgmask_A=None):
+    """maxR/minR, mask radius (px), refinement steps as calc_initialvalues."""
+    f = np.float32
+    maxres_px = f(apix) * f(box) / f(maxRes)
+    minres_px = f(apix) * f(box) / f(minRes)
+    maxR = int(maxres_px)
+    minR = int(minres_px)
+    if maxR % 2:
+        maxR += 1
+    mask = f(box // 2 - 2) if imgmask_A is None else f(imgmask_A)
+    mask = mask / f(apix)                      # imgmask='y' by default: always divided by apix
+    dang = f(360.0) / (maxres_px * np.sqrt(f(5.0)) * f(3.14159))
+    dtran = f(box) / (maxres_px * np.sqrt(f(5.0)))
+    return dict(maxR=maxR, minR=minR, mask=float(mask), dang=float(dang), dtran=float(dtran))
+
+
+def mode1_volume(O, images, angles, shifts, df, apix, maxR, mask_r, compact=False, ctf=True):
+    box = images.shape[-1]
+    rcut = maxR - 2 if compact else maxR
+    vol = O.new_volume(box, maxR, compact=compact, wg_init=0.0 if compact else 0.01)
+    for i in range(len(images)):
+        x, y = shifts[i]
+        a, b = O.particle_ft(images[i], x, y, mask_r=mask_r, mask_xy=tuple(shifts[0]), norm=True)
+        c = O.ctf2d(box, df[i, 0], df[i, 1], 0.0, apix) if ctf else None
+        O.insert(vol, a, b, c, angles[i], maxR, rcut=rcut, compact=compact)
+    return vol
+
+
+def templates(O, timgs, tshifts, mask_r):
+    ta, tb = [], []
+    for t in range(len(timgs)):
+        a, b = O.particle_ft(timgs[t], tshifts[t][0], tshifts[t][1], mask_r=mask_r)
+        ta.append(a)
+        tb.append(b)
+    return np.array(ta), np.array(tb)
+
+
+def template_inputs(O, ta, tb, ctf, minR, maxR):
+    """Template x CTF and its refitted Fcut (cal_crosslin_PR_omega.f90:34-38)."""
+    if ctf is not None:
+        ta, tb = ta * ctf, tb * ctf
+    tf = np.array([O.amp_expfit(ta[t], tb[t], minR, maxR) for t in range(len(ta))])
+    return ta, tb, tf
+
+
+def particle_inputs(O, img, x, y, mask_r, minR, maxR):
+    """Masked, checkerboarded FT; Fcut before the centre shift; shifted FT."""
+    a0, b0 = O.particle_ft(img, x, y, mask_r=mask_r, shift=False)
+    fc0 = O.amp_expfit(a0, b0, minR, maxR)
+    pa, pb = O.ft_shift(a0, b0, x, y, True)
+    return (a0, b0), (pa, pb), fc0

@@ -1,0 +1,105 @@
+"""HIP common-lines search (icosprocess modes 4 and 5) against the FP64
+oracle restatement of ort_globesearch / cal_crosslin_PR_omega /
+ort_refinement on the same seeded inputs."""
+import numpy as np
+import pytest
+
+from cryoim_amd import geometry, lib, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(box=64, maxR=12, minR=1, T=3, n=2, seed=21, apix=2.0, ctf=True):
+    ph = synth.make_phantom(box, radius=(box * 0.28, box * 0.4), sigma=(1.5, 2.5), seed=seed)
+    rng = np.random.default_rng(seed)
+    tang = synth.random_au_orientations(T, rng)
+    tsh = rng.uniform(-1, 1, (T, 2))
+    timgs = synth.project(ph, tang, tsh)
+    ps = synth.make_particles(ph, n, seed=seed + 1, shift_max=2.0, snr=1.0, ctf=ctf, apix=apix,
+                              df_range=(1.0, 2.0))
+    return ph, tang, tsh, timgs, ps
+
+
+def _oracle_inputs(O, tang, tsh, timgs, ps, i, minR, maxR, mask_r, apix, ctf=True):
+    box = timgs.shape[-1]
+    ta, tb, tfc = [], [], []
+    c = O.ctf2d(box, ps.df[i, 0], ps.df[i, 1], 0.0, apix) if ctf else None
+    for t in range(len(tang)):
+        a, b = O.particle_ft(timgs[t], tsh[t, 0], tsh[t, 1], mask_r=mask_r)
+        if ctf:
+            a, b = a * c, b * c
+        ta.append(a)
+        tb.append(b)
+        tfc.append(O.amp_expfit(a, b, minR, maxR))
+    x, y = ps.shifts[i]
+    a0, b0 = O.particle_ft(ps.images[i], x, y, mask_r=mask_r, shift=False)
+    fc0 = O.amp_expfit(a0, b0, minR, maxR)
+    pa, pb = O.ft_shift(a0, b0, x, y, True)
+    return (a0, b0), (pa, pb), fc0, np.array(ta), np.array(tb), np.array(tfc)
+
+
+def _parts(ps):
+    P = np.zeros((len(ps.images), lib.PART_STRIDE), np.float32)
+    P[:, 0:3] = ps.angles
+    P[:, 3:5] = ps.shifts
+    P[:, 6:8] = ps.df
+    P[:, 11] = 1
+    return P
+
+
+@pytest.mark.parametrize("ctf", [True, False])
+def test_pr_array_matches_oracle(ctx, O64, ctf):
+    box, minR, maxR, step, apix = 64, 1, 12, 3.0, 2.0
+    mask_r = box / 2 - 4
+    ph, tang, tsh, timgs, ps = _setup(box, maxR, minR, T=3, n=1, ctf=ctf, apix=apix)
+    p = lib.make_params(box, minR, maxR, apply_ctf=ctf, mask_on=True, apix=apix, imgmask=mask_r,
+                        searchstep=step)
+    ctx.search_templates(p, timgs, np.concatenate([tang, tsh], axis=1))
+    Nt, Np, No = geometry.search_grid(step)
+    g = ctx.search_pr_array(p, ps.images[0], _parts(ps)[0], (Nt + 1, 2 * Np + 1, No))
+    _, (pa, pb), fc0, ta, tb, tfc = _oracle_inputs(O64, tang, tsh, timgs, ps, 0, minR, maxR, mask_r, apix, ctf)
+    o4, arr = O64.globesearch(pa, pb, fc0, ta, tb, tfc, tang, minR, maxR, step, want_array=True)
+    inside = arr < 999.5
+    assert (g < 999.5).sum() == inside.sum()
+    err = np.abs(g[inside] - arr[inside])
+    print(f"PR max |GPU - FP64 oracle| = {err.max():.4f} deg, mean {err.mean():.5f} deg")
+    assert err.max() < 0.05
+    assert np.array_equal(g >= 999.5, ~inside)
+
+
+def test_global_search_argmin(ctx, O64):
+    box, minR, maxR, step, apix = 64, 1, 12, 3.0, 2.0
+    mask_r = box / 2 - 4
+    ph, tang, tsh, timgs, ps = _setup(box, maxR, minR, T=3, n=3, seed=33, apix=apix)
+    p = lib.make_params(box, minR, maxR, apply_ctf=True, mask_on=True, apix=apix, imgmask=mask_r,
+                        searchstep=step)
+    ctx.search_templates(p, timgs, np.concatenate([tang, tsh], axis=1))
+    out = ctx.search_global(p, ps.images, _parts(ps))
+    for i in range(3):
+        _, (pa, pb), fc0, ta, tb, tfc = _oracle_inputs(O64, tang, tsh, timgs, ps, i, minR, maxR, mask_r, apix)
+        o4, arr = O64.globesearch(pa, pb, fc0, ta, tb, tfc, tang, minR, maxR, step, want_array=True)
+        # decisions agree unless the oracle's best and runner-up are within the float noise
+        srt = np.sort(arr.ravel())
+        if srt[1] - srt[0] > 0.02:
+            assert np.allclose(out[i, 0:3], o4[:3], atol=1e-3), (out[i, 0:3], o4[:3])
+        assert abs(out[i, 5] - o4[3]) < 0.05
+
+
+def test_local_refinement_matches_oracle(ctx, O64):
+    box, minR, maxR, apix = 64, 1, 12, 2.0
+    mask_r = box / 2 - 4
+    ph, tang, tsh, timgs, ps = _setup(box, maxR, minR, T=3, n=2, seed=41, apix=apix)
+    dang = 360.0 / (maxR * np.sqrt(5.0) * 3.14159)
+    dtran = box / (maxR * np.sqrt(5.0))
+    p = lib.make_params(box, minR, maxR, apply_ctf=True, mask_on=True, apix=apix, imgmask=mask_r,
+                        ncycle=3, delta_angle=dang, delta_trans=dtran)
+    ctx.search_templates(p, timgs, np.concatenate([tang, tsh], axis=1))
+    P = _parts(ps)
+    P[:, 0:3] += np.array([1.5, -1.0, 2.0], np.float32)      # start near the truth
+    out = ctx.refine_local(p, ps.images, P)
+    for i in range(2):
+        (a0, b0), _, fc0, ta, tb, tfc = _oracle_inputs(O64, tang, tsh, timgs, ps, i, minR, maxR, mask_r, apix)
+        io5, pr = O64.refine_particle(a0, b0, fc0, ta, tb, tfc, tang, minR, maxR, 3, dang, dtran,
+                                      np.array([*P[i, 0:5]], np.float64))
+        assert np.allclose(out[i, 0:5], io5, atol=2e-3), (out[i, 0:5], io5)
+        assert abs(out[i, 5] - pr) < 0.05
