@@ -16,19 +16,45 @@ int set_error(cim_ctx* c, int code, const char* fmt, ...) {
     return code;
 }
 
+static hipEvent_t take_event(cim_ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
 TimedLaunch::TimedLaunch(cim_ctx* c, const char* n) : ctx(c), name(n) {
-    if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
+    if (!ctx->timing) return;
+    a = take_event(ctx);
+    if (a) (void)hipEventRecord(a, ctx->stream);
 }
 
 TimedLaunch::~TimedLaunch() {
-    if (!ctx->timing) return;
-    (void)hipEventRecord(ctx->ev1, ctx->stream);
-    (void)hipEventSynchronize(ctx->ev1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    KTimer& t = ctx->timers[name];
-    t.total_ms += ms;
-    t.launches += 1;
+    if (!ctx->timing || !a) return;
+    hipEvent_t b = take_event(ctx);
+    if (!b) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({name, a, b});
+}
+
+void resolve_timing(cim_ctx* c) {
+    if (c->pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            KTimer& t = c->timers[p.name];
+            t.total_ms += ms;
+            t.launches += 1;
+        }
+        c->event_pool.push_back(p.a);
+        c->event_pool.push_back(p.b);
+    }
+    c->pending.clear();
 }
 
 int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init, int nops, const double* ops);
@@ -72,8 +98,7 @@ int cim_init(cim_ctx** out, int device) {
     if (hipSetDevice(device) != hipSuccess) return CIM_E_HIP;
     cim_ctx* ctx = new cim_ctx();
     ctx->device = device;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return CIM_E_HIP;
     }
@@ -93,8 +118,8 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->srch.tfcut.release();
     ctx->srch.wk.release();
     for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->aux, &ctx->aux2, &ctx->parts, &ctx->out}) b->release();
-    (void)hipEventDestroy(ctx->ev0);
-    (void)hipEventDestroy(ctx->ev1);
+    resolve_timing(ctx);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CIM_OK;
@@ -112,6 +137,7 @@ int cim_sync(cim_ctx* ctx) {
 
 int cim_timing(cim_ctx* ctx, int on) {
     if (!ctx) return CIM_E_ARG;
+    resolve_timing(ctx);
     ctx->timing = on != 0;
     ctx->timers.clear();
     return CIM_OK;
@@ -119,6 +145,7 @@ int cim_timing(cim_ctx* ctx, int on) {
 
 int cim_timing_get(cim_ctx* ctx, const char* name, double* avg_ms, long long* launches) {
     if (!ctx || !name) return CIM_E_ARG;
+    resolve_timing(ctx);
     auto it = ctx->timers.find(name);
     if (it == ctx->timers.end()) {
         if (avg_ms) *avg_ms = 0.0;
@@ -127,6 +154,19 @@ int cim_timing_get(cim_ctx* ctx, const char* name, double* avg_ms, long long* la
     }
     if (avg_ms) *avg_ms = it->second.launches ? it->second.total_ms / it->second.launches : 0.0;
     if (launches) *launches = it->second.launches;
+    return CIM_OK;
+}
+
+int cim_counter_get(cim_ctx* ctx, const char* name, long long* value) {
+    if (!ctx || !name || !value) return CIM_E_ARG;
+    auto it = ctx->counters.find(name);
+    *value = it == ctx->counters.end() ? 0 : it->second;
+    return CIM_OK;
+}
+
+int cim_counter_reset(cim_ctx* ctx) {
+    if (!ctx) return CIM_E_ARG;
+    ctx->counters.clear();
     return CIM_OK;
 }
 

@@ -58,10 +58,10 @@ struct RecState {
 
 // Per-batch working set of the searches.
 struct SearchBatch {
-    DevBuf pcrop, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi;
+    DevBuf pcrop, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter;
     void release() {
         for (DevBuf* b : {&pcrop, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
-                          &fv, &fi})
+                          &fv, &fi, &counter})
             b->release();
     }
 };
@@ -92,15 +92,26 @@ struct cim_ctx {
     cim::SearchState srch;
     // scratch buffers reused across calls
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
-    // kernel timing
+    // kernel timing: event pairs recorded around launches, resolved lazily
     bool timing = false;
     std::map<std::string, cim::KTimer> timers;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    // autotuned launch shapes
+    std::map<std::string, int> tuned;
+    // work counters (evaluated samples, inserted samples, ...)
+    std::map<std::string, long long> counters;
 };
 
 namespace cim {
 
 int set_error(cim_ctx* c, int code, const char* fmt, ...);
+// Resolve recorded timing events into ctx->timers (synchronises the stream).
+void resolve_timing(cim_ctx* c);
 
 #define CIM_HIP(call)                                                                  \
     do {                                                                               \
@@ -121,6 +132,7 @@ int set_error(cim_ctx* c, int code, const char* fmt, ...);
 struct TimedLaunch {
     cim_ctx* ctx;
     const char* name;
+    hipEvent_t a = nullptr;
     TimedLaunch(cim_ctx* c, const char* n);
     ~TimedLaunch();
 };

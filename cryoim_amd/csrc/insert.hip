@@ -254,6 +254,8 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
                        g);
     CIM_HIP(hipGetLastError());
     r.inserted += n;
+    ctx->counters["insert_samples"] += (long long)n * r.nsamples;
+    ctx->counters["insert_particles"] += n;
     return CIM_OK;
 }
 

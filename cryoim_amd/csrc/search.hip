@@ -61,6 +61,30 @@ __device__ __forceinline__ void crosslin_one(const float* E1, const float* E2, c
     *a2 = atan2f(v20y, v20x) * r2d;
 }
 
+// atan(y/x) for y >= 0, any x, in [0, pi]: octant reduction to t in [0, 1]
+// and t * P(t^2), P of degree 7 fitted to atan(t)/t on [0, 1] (weighted
+// least squares, max abs error 2e-7 rad evaluated in float32).  With
+// y = |Im(z1 conj z2)| and x = Re(z1 conj z2) this is the phase difference
+// of two samples folded into [0, pi], which cal_crosslin_PR_omega.f90:57-60
+// computes from two atan2 calls.
+__device__ __forceinline__ float atan2_upper(float y, float x) {
+    const float ax = fabsf(x);
+    const float mn = fminf(ax, y), mx = fmaxf(ax, y);
+    const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+    const float s = t * t;
+    float q = -0.004780458752065897f;
+    q = fmaf(q, s, 0.02455713413655758f);
+    q = fmaf(q, s, -0.05990472808480263f);
+    q = fmaf(q, s, 0.09942759573459625f);
+    q = fmaf(q, s, -0.14029419422149658f);
+    q = fmaf(q, s, 0.19971375167369843f);
+    q = fmaf(q, s, -0.3333209455013275f);
+    q = fmaf(q, s, 0.9999999403953552f);
+    float r = t * q;
+    r = y > ax ? 1.57079632679489661923f - r : r;
+    return x < 0.f ? 3.14159265358979323846f - r : r;
+}
+
 __device__ __forceinline__ float fold_phase(float d) {
     const float pi = 3.14159265358979323846f, tpi = 6.28318530717958647692f;
     d = fabsf(d);
@@ -133,6 +157,7 @@ struct SearchArgs {
     float* outv;              // n x npts
     int* outi;
     float* prfull;            // optional n x npts x Nomega
+    unsigned long long* nsamples;   // optional: evaluated (line, radius, omega) samples
 };
 
 template <int VPT, bool USE_LDS>
@@ -141,8 +166,9 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     char* sm = reinterpret_cast<char*>(smem4);
     float2* disk = reinterpret_cast<float2*>(sm);
     size_t off = USE_LDS ? sizeof(float2) * (size_t)a.disk_total : 0;
-    float2* tline = reinterpret_cast<float2*>(sm + off);
-    off += sizeof(float2) * (size_t)a.chunk * a.nr;
+    // template line record per (line, radius): re, im, |F| (-1 if gated out), particle Fcut
+    float4* tline = reinterpret_cast<float4*>(sm + off);
+    off += sizeof(float4) * (size_t)a.chunk * a.nr;
     float* alpha1 = reinterpret_cast<float*>(sm + off);
     float* alpha2 = alpha1 + 64;
     float* fc0 = alpha2 + 64;
@@ -172,6 +198,7 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     float num[VPT], den[VPT];
 #pragma unroll
     for (int v = 0; v < VPT; ++v) { num[v] = 0.f; den[v] = 0.f; }
+    unsigned long long pairs = 0;   // (line, radius) pairs that pass the template gate
     __syncthreads();
 
     for (int t = 0; t < a.T; ++t) {
@@ -188,14 +215,14 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
             for (int q = tid; q < jn * a.nr; q += bd) {
                 const int jj = q / a.nr, fr = q - jj * a.nr;
                 const int j = j0 + jj;
-                float2 res = make_float2(0.f, -1.f);
+                float4 res = make_float4(0.f, 0.f, -1.f, fc0[fr]);
                 if (alpha1[j] < 999.f && alpha2[j] < 999.f) {
                     const float FR = (float)(a.minR + fr);
                     const float ang = alpha2[j] * d2r;
                     float x = FR * cosf(ang) + cf, y = FR * sinf(ang) + cf;
                     float2 z = crop_bilin(tc, a.Dc, a.R, x, y, ic);
                     float amp2 = sqrtf(z.x * z.x + z.y * z.y);
-                    if (amp2 >= tf[fr]) res = make_float2(atan2f(z.y, z.x), amp2);
+                    if (amp2 >= tf[fr]) res = make_float4(z.x, z.y, amp2, fc0[fr]);
                 }
                 tline[q] = res;
             }
@@ -212,12 +239,13 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
                     cs[v] = cosf(om);
                     sn[v] = sinf(om);
                 }
-                const float2* tl = tline + jj * a.nr;
+                const float4* tl = tline + jj * a.nr;
                 for (int fr = 0; fr < a.nr; ++fr) {
-                    const float2 tv = tl[fr];
-                    if (tv.y < 0.f) continue;
+                    const float4 tv = tl[fr];
+                    if (tv.z < 0.f) continue;
+                    ++pairs;
                     const float FR = (float)(a.minR + fr);
-                    const float f0 = fc0[fr];
+                    const float f0 = tv.w;
 #pragma unroll
                     for (int v = 0; v < VPT; ++v) {
                         const int k = tid + v * bd;
@@ -237,10 +265,12 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
                         } else {
                             z = crop_bilin(pc, a.Dc, a.R, x, y, ic);
                         }
-                        const float amp1 = sqrtf(z.x * z.x + z.y * z.y);
+                        const float amp1 = __builtin_amdgcn_sqrtf(z.x * z.x + z.y * z.y);
                         if (amp1 >= f0) {
-                            const float diff = fold_phase(tv.x - atan2f(z.y, z.x));
-                            const float w = 0.5f * (amp1 + tv.y);
+                            const float cr = fabsf(z.x * tv.y - z.y * tv.x);
+                            const float dt = z.x * tv.x + z.y * tv.y;
+                            const float diff = atan2_upper(cr, dt);
+                            const float w = 0.5f * (amp1 + tv.z);
                             num[v] += diff * w;
                             den[v] += w;
                         }
@@ -281,6 +311,7 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
             if (redv[i] < best || (redv[i] == best && redi[i] < bidx)) { best = redv[i]; bidx = redi[i]; }
         a.outv[(long long)p * a.npts + pt] = best;
         a.outi[(long long)p * a.npts + pt] = bidx;
+        if (a.nsamples) atomicAdd(a.nsamples, pairs * (unsigned long long)a.Nomega);
     }
 }
 
@@ -425,14 +456,81 @@ static int launch_search(cim_ctx* ctx, SearchArgs& a, int n, int lds_bytes, bool
     if (use_lds) {
         if (vpt == 1) CIM_LAUNCH(1, true);
         else if (vpt == 2) CIM_LAUNCH(2, true);
+        else if (vpt == 3) CIM_LAUNCH(3, true);
         else CIM_LAUNCH(4, true);
     } else {
         if (vpt == 1) CIM_LAUNCH(1, false);
         else if (vpt == 2) CIM_LAUNCH(2, false);
+        else if (vpt == 3) CIM_LAUNCH(3, false);
         else CIM_LAUNCH(4, false);
     }
 #undef CIM_LAUNCH
     CIM_HIP(hipGetLastError());
+    return CIM_OK;
+}
+
+static int block_for(int No, int vpt) {
+    int bd = (No + vpt - 1) / vpt;
+    return ((bd + 63) / 64) * 64;
+}
+
+// Autotuner: the launch shape -- omega values per thread (with it the
+// workgroup size) and whether the particle disk is staged in LDS or read
+// through the caches -- is chosen by timing one-particle launches of every
+// candidate on the first batch, and cached per (maxR, Nomega, T).
+// CIM_SEARCH_VPT=<1..4> and CIM_SEARCH_LDS=<0|1> force a choice.
+struct SearchShape {
+    int vpt;
+    bool lds;
+    int lds_bytes;
+};
+
+static int tune_shape(cim_ctx* ctx, SearchArgs a, int lds_with, int lds_without, bool lds_fits, int No,
+                      SearchShape* out) {
+    const char* ev = getenv("CIM_SEARCH_VPT");
+    const char* el = getenv("CIM_SEARCH_LDS");
+    char key[96];
+    snprintf(key, sizeof(key), "shape:%d:%d:%d", a.R, No, a.T);
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end() && !ev && !el) {
+        out->vpt = it->second & 7;
+        out->lds = (it->second >> 3) & 1;
+        out->lds_bytes = out->lds ? lds_with : lds_without;
+        return CIM_OK;
+    }
+    a.nsamples = nullptr;
+    hipEvent_t e0, e1;
+    CIM_HIP(hipEventCreate(&e0));
+    CIM_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    SearchShape bs{0, false, 0};
+    for (int l = 1; l >= 0; --l) {
+        if (l == 1 && !lds_fits) continue;
+        if (el && atoi(el) != l) continue;
+        for (int v = 1; v <= 4; ++v) {
+            const int bd = block_for(No, v);
+            if (bd > 1024 || (v > 1 && block_for(No, v) == block_for(No, v - 1))) continue;
+            if (ev && atoi(ev) != v) continue;
+            const int lb = l ? lds_with : lds_without;
+            CIM_HIP(hipEventRecord(e0, ctx->stream));
+            int rc = launch_search(ctx, a, 1, lb, l == 1, v, bd);
+            if (rc) return rc;
+            CIM_HIP(hipEventRecord(e1, ctx->stream));
+            CIM_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CIM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (getenv("CIM_TUNE_VERBOSE")) fprintf(stderr, "cim tune: vpt %d lds %d block %d: %.3f ms\n", v, l, bd, ms);
+            if (ms < best) {
+                best = ms;
+                bs = SearchShape{v, l == 1, lb};
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (bs.vpt == 0) return set_error(ctx, CIM_E_ARG, "no launch shape fits Nomega %d", No);
+    if (!ev && !el) ctx->tuned[key] = bs.vpt | (bs.lds ? 8 : 0);
+    *out = bs;
     return CIM_OK;
 }
 
@@ -452,22 +550,20 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     SearchGrid g = make_grid(p->searchstep);
     if (g.pts.empty()) return set_error(ctx, CIM_E_ARG, "search grid has no AU point for step %g", p->searchstep);
     const int npts = (int)g.base.size();
-    const int vpt = g.No <= 1024 ? 1 : (g.No <= 2048 ? 2 : 4);
     if (g.No > 4096) return set_error(ctx, CIM_E_ARG, "searchstep %g too fine (Nomega > 4096)", p->searchstep);
-    int bd = (g.No + vpt - 1) / vpt;
-    bd = ((bd + 63) / 64) * 64;
     std::vector<int> rows;
     int disk_total = 0;
     make_disk(R, rows, disk_total);
     const int nrows = 2 * R + 4;
     const size_t misc = sizeof(float) * (64 + 64 + nr) + sizeof(int) * nrows + 16 * 8 + 64;
     const size_t lds_cap = 160 * 1024;
-    bool use_lds = sizeof(float2) * (size_t)disk_total + misc + sizeof(float2) * 4 * nr <= lds_cap;
-    size_t rem = lds_cap - misc - (use_lds ? sizeof(float2) * (size_t)disk_total : 0);
-    int chunk = (int)std::min<size_t>(60, rem / (sizeof(float2) * nr));
+    const bool lds_fits = sizeof(float2) * (size_t)disk_total + misc + sizeof(float4) * 4 * nr <= lds_cap;
+    size_t rem = lds_cap - misc - (lds_fits ? sizeof(float2) * (size_t)disk_total : 0);
+    // one line-chunk size for both variants so the tuned kernels do the same work
+    int chunk = (int)std::min<size_t>(60, rem / (sizeof(float4) * nr));
     if (chunk < 1) return set_error(ctx, CIM_E_ARG, "maxR %d too large for the LDS line buffer", R);
-    const int lds_bytes = (int)(misc + sizeof(float2) * (size_t)chunk * nr +
-                                (use_lds ? sizeof(float2) * (size_t)disk_total : 0));
+    const int lds_without = (int)(misc + sizeof(float4) * (size_t)chunk * nr);
+    const int lds_with = lds_without + (int)(sizeof(float2) * (size_t)disk_total);
 
     // device copies of the small tables
     std::vector<float> ops(540);
@@ -567,9 +663,17 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         a.outv = sb.outv.as<float>();
         a.outi = sb.outi.as<int>();
         a.prfull = pr_array ? sb.prfull.as<float>() : nullptr;
+        CIM_ALLOC(sb.counter, sizeof(unsigned long long));
+        CIM_HIP(hipMemsetAsync(sb.counter.ptr, 0, sizeof(unsigned long long), ctx->stream));
+        a.nsamples = sb.counter.as<unsigned long long>();
+        SearchShape sh{};
+        rc = tune_shape(ctx, a, lds_with, lds_without, lds_fits, g.No, &sh);
+        if (rc) return rc;
+        ctx->counters["cl_search_vpt"] = sh.vpt;
+        ctx->counters["cl_search_lds"] = sh.lds;
         {
             TimedLaunch tl(ctx, "cl_search");
-            rc = launch_search(ctx, a, nb, lds_bytes, use_lds, vpt, bd);
+            rc = launch_search(ctx, a, nb, sh.lds_bytes, sh.lds, sh.vpt, block_for(g.No, sh.vpt));
             if (rc) return rc;
         }
         CIM_ALLOC(sb.fv, sizeof(float) * nb);
@@ -587,7 +691,11 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             CIM_HIP(hipMemcpyAsync(full.data(), sb.prfull.ptr, sizeof(float) * full.size(), hipMemcpyDeviceToHost,
                                    ctx->stream));
         }
+        unsigned long long hcount = 0;
+        CIM_HIP(hipMemcpyAsync(&hcount, sb.counter.ptr, sizeof(hcount), hipMemcpyDeviceToHost, ctx->stream));
         CIM_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->counters["cl_search_samples"] += (long long)hcount;
+        ctx->counters["cl_search_particles"] += nb;
         const int plane = g.No * (2 * g.Np + 1);
         for (int i = 0; i < nb; ++i) {
             float* q = parts + (size_t)(b0 + i) * CIM_PART_STRIDE;

@@ -76,6 +76,8 @@ def load() -> ctypes.CDLL:
         "cim_timing": (ip, [vp, ip]),
         "cim_timing_get": (ip, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_longlong)]),
+        "cim_counter_get": (ip, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)]),
+        "cim_counter_reset": (ip, [vp]),
         "cim_fft2d": (ip, [vp, fp, fp, ip, ip, ip, ip]),
         "cim_fft3d": (ip, [vp, fp, fp, ip, ip, ip]),
         "cim_fft2d_dev": (ip, [vp, vp, ip, ip, ip, ip]),
@@ -166,6 +168,14 @@ class Context:
         self._check(self.lib.cim_timing_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)),
                     "cim_timing_get")
         return ms.value, n.value
+
+    def counter(self, name: str) -> int:
+        v = ctypes.c_longlong()
+        self._check(self.lib.cim_counter_get(self.h, name.encode(), ctypes.byref(v)), "cim_counter_get")
+        return v.value
+
+    def counter_reset(self):
+        self._check(self.lib.cim_counter_reset(self.h), "cim_counter_reset")
 
     # -- FFT -------------------------------------------------------------
     def fft2d(self, re: np.ndarray, im: np.ndarray, direction: int = 1, conj_out: bool = False):

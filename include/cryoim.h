@@ -91,6 +91,12 @@ int cim_sync(cim_ctx* ctx);
  * since the last reset; timing is off until cim_timing(ctx, 1). */
 int cim_timing(cim_ctx* ctx, int on);
 int cim_timing_get(cim_ctx* ctx, const char* name, double* avg_ms, long long* launches);
+/* Work counters accumulated since the last reset: "cl_search_samples"
+ * (bilinear samples scored by the global search), "cl_search_particles",
+ * "insert_samples" (half-plane samples inserted, each into nops*8 voxels),
+ * "insert_particles". */
+int cim_counter_get(cim_ctx* ctx, const char* name, long long* value);
+int cim_counter_reset(cim_ctx* ctx);
 
 /* ---- FFT: replaces FFTcmplx2d_f/B, FFTcmplx3d_f/B and img2d_FFT ------- */
 /* dir = 1 forward (scaled 1/N), -1 backward (unscaled).  conj_out = 1
