@@ -58,9 +58,9 @@ struct RecState {
 
 // Per-batch working set of the searches.
 struct SearchBatch {
-    DevBuf pcrop, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter;
+    DevBuf pcrop, pcrop4, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter;
     void release() {
-        for (DevBuf* b : {&pcrop, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
+        for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
                           &fv, &fi, &counter})
             b->release();
     }

@@ -138,6 +138,7 @@ __global__ void k_crop(const float2* __restrict__ src, long long src_stride, int
 
 struct SearchArgs {
     const float2* pcrop;      // n x Dc^2 particle crops (centre-shifted)
+    const float4* pcrop4;     // the same, tap-paired
     const float* fcut0;       // n x nr
     const float2* tcrop;      // template crops; particle p, template t at tcrop + p*tc_pstride + t*Dc^2
     long long tc_pstride;
@@ -160,12 +161,22 @@ struct SearchArgs {
     unsigned long long* nsamples;   // optional: evaluated (line, radius, omega) samples
 };
 
-template <int VPT, bool USE_LDS>
+// Particle taps: MODE 0 reads the float2 disk staged in LDS, MODE 1 reads
+// the tap-paired crop (float4 {F(x,y), F(x+1,y)}) through the caches, two
+// 16-byte loads per bilinear sample.
+//
+// PAIR: when the omega grid closes on itself (Nomega even and
+// step*Nomega/2 = 180 deg), column k and column k + Nomega/2 sample the
+// particle transform at mirrored points about the origin.  The transform of
+// a real image (also after the linear-phase centre shift) is Hermitian, so
+// the second sample is the conjugate of the first: one interpolation feeds
+// both columns.
+template <int VPT, int MODE, bool PAIR>
 __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     extern __shared__ float4 smem4[];
     char* sm = reinterpret_cast<char*>(smem4);
     float2* disk = reinterpret_cast<float2*>(sm);
-    size_t off = USE_LDS ? sizeof(float2) * (size_t)a.disk_total : 0;
+    size_t off = MODE == 0 ? sizeof(float2) * (size_t)a.disk_total : 0;
     // template line record per (line, radius): re, im, |F| (-1 if gated out), particle Fcut
     float4* tline = reinterpret_cast<float4*>(sm + off);
     off += sizeof(float4) * (size_t)a.chunk * a.nr;
@@ -179,12 +190,14 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     const int pt = blockIdx.x, p = blockIdx.y;
     const int tid = threadIdx.x, bd = blockDim.x;
     const float2* pc = a.pcrop + (long long)p * a.Dc * a.Dc;
+    const float4* pc4 = a.pcrop4 + (long long)p * a.Dc * a.Dc;
     const int ic = a.box / 2;
     const float cf = (float)ic;
     const float d2r = 0.0174532925199432957692f;
+    const int ncol = PAIR ? a.Nomega / 2 : a.Nomega;
 
     for (int i = tid; i < a.nr; i += bd) fc0[i] = a.fcut0[(long long)p * a.nr + i];
-    if (USE_LDS) {
+    if (MODE == 0) {
         for (int r = tid; r < a.nrows; r += bd) rbase[r] = a.rows[3 * r + 2];
         for (int r = 0; r < a.nrows; ++r) {
             const int lo = a.rows[3 * r], hi = a.rows[3 * r + 1], base = a.rows[3 * r + 2];
@@ -195,9 +208,9 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     float E1[9];
     euler_f(theta, phi, 0.f, E1);
 
-    float num[VPT], den[VPT];
+    float num[VPT], den[VPT], num2[VPT], den2[VPT];
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) { num[v] = 0.f; den[v] = 0.f; }
+    for (int v = 0; v < VPT; ++v) { num[v] = 0.f; den[v] = 0.f; num2[v] = 0.f; den2[v] = 0.f; }
     unsigned long long pairs = 0;   // (line, radius) pairs that pass the template gate
     __syncthreads();
 
@@ -249,30 +262,36 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
 #pragma unroll
                     for (int v = 0; v < VPT; ++v) {
                         const int k = tid + v * bd;
-                        if (k >= a.Nomega) continue;
+                        if (k >= ncol) continue;
                         const float x = FR * cs[v] + cf, y = FR * sn[v] + cf;
-                        float2 z;
-                        if (USE_LDS) {
-                            const int lx = (int)x, ly = (int)y;
-                            const float ax = x - (float)lx, ay = y - (float)ly;
+                        const int lx = (int)x, ly = (int)y;
+                        const float ax = x - (float)lx, ay = y - (float)ly;
+                        float2 q00, q10, q01, q11;
+                        if (MODE == 0) {
                             const int r = ly - ic + a.R + 1;
                             const int i0 = rbase[r] + (lx - ic), i1 = rbase[r + 1] + (lx - ic);
-                            const float2 q00 = disk[i0], q10 = disk[i0 + 1], q01 = disk[i1], q11 = disk[i1 + 1];
-                            const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay);
-                            const float w01 = (1.f - ax) * ay, w11 = ax * ay;
-                            z = make_float2(w00 * q00.x + w10 * q10.x + w01 * q01.x + w11 * q11.x,
-                                            w00 * q00.y + w10 * q10.y + w01 * q01.y + w11 * q11.y);
+                            q00 = disk[i0]; q10 = disk[i0 + 1]; q01 = disk[i1]; q11 = disk[i1 + 1];
                         } else {
-                            z = crop_bilin(pc, a.Dc, a.R, x, y, ic);
+                            const float4* r0 = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
+                            const float4 u = r0[0], w4 = r0[a.Dc];
+                            q00 = make_float2(u.x, u.y); q10 = make_float2(u.z, u.w);
+                            q01 = make_float2(w4.x, w4.y); q11 = make_float2(w4.z, w4.w);
                         }
-                        const float amp1 = __builtin_amdgcn_sqrtf(z.x * z.x + z.y * z.y);
+                        const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay);
+                        const float w01 = (1.f - ax) * ay, w11 = ax * ay;
+                        const float zx = w00 * q00.x + w10 * q10.x + w01 * q01.x + w11 * q11.x;
+                        const float zy = w00 * q00.y + w10 * q10.y + w01 * q01.y + w11 * q11.y;
+                        const float amp1 = __builtin_amdgcn_sqrtf(zx * zx + zy * zy);
                         if (amp1 >= f0) {
-                            const float cr = fabsf(z.x * tv.y - z.y * tv.x);
-                            const float dt = z.x * tv.x + z.y * tv.y;
-                            const float diff = atan2_upper(cr, dt);
                             const float w = 0.5f * (amp1 + tv.z);
-                            num[v] += diff * w;
+                            const float pxy = zx * tv.y, pyx = zy * tv.x, pxx = zx * tv.x, pyy = zy * tv.y;
+                            num[v] += atan2_upper(fabsf(pxy - pyx), pxx + pyy) * w;
                             den[v] += w;
+                            if (PAIR) {
+                                // conjugate sample of column k + Nomega/2
+                                num2[v] += atan2_upper(fabsf(pxy + pyx), pxx - pyy) * w;
+                                den2[v] += w;
+                            }
                         }
                     }
                 }
@@ -288,11 +307,16 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
         const int k = tid + v * bd;
-        if (k < a.Nomega) {
-            const float pr = den[v] > 1.0e-10f ? num[v] / den[v] * r2d : 999.f;
-            if (a.prfull) a.prfull[((long long)p * a.npts + pt) * a.Nomega + k] = pr;
-            const int li = a.pbase[pt] + k;
-            if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
+        if (k < ncol) {
+#pragma unroll
+            for (int h = 0; h < (PAIR ? 2 : 1); ++h) {
+                const float nn = h ? num2[v] : num[v], dd = h ? den2[v] : den[v];
+                const int kk = k + h * ncol;
+                const float pr = dd > 1.0e-10f ? nn / dd * r2d : 999.f;
+                if (a.prfull) a.prfull[((long long)p * a.npts + pt) * a.Nomega + kk] = pr;
+                const int li = a.pbase[pt] + kk;
+                if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
+            }
         }
     }
     // wave reduction
@@ -313,6 +337,17 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
         a.outi[(long long)p * a.npts + pt] = bidx;
         if (a.nsamples) atomicAdd(a.nsamples, pairs * (unsigned long long)a.Nomega);
     }
+}
+
+// Tap-paired copy of the particle crops: {F(x, y), F(x+1, y)} per element.
+__global__ void k_pair_taps(const float2* __restrict__ crop, float4* __restrict__ out, int Dc, int n) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)n * Dc * Dc;
+    if (q >= total) return;
+    const int cx = (int)(q % Dc);
+    const float2 v0 = crop[q];
+    const float2 v1 = cx + 1 < Dc ? crop[q + 1] : make_float2(0.f, 0.f);
+    out[q] = make_float4(v0.x, v0.y, v1.x, v1.y);
 }
 
 // First minimum over the AU points of each particle.
@@ -445,25 +480,29 @@ static int prep_templates(cim_ctx* ctx, const cim_params* p, int T, const float*
 }
 
 
-static int launch_search(cim_ctx* ctx, SearchArgs& a, int n, int lds_bytes, bool use_lds, int vpt, int bd) {
+static int launch_search(cim_ctx* ctx, SearchArgs& a, int n, int lds_bytes, int mode, bool pair, int vpt, int bd) {
     dim3 grid(a.npts, n);
-#define CIM_LAUNCH(V, L)                                                                                      \
-    do {                                                                                                      \
-        CIM_HIP(hipFuncSetAttribute((const void*)k_cl_search<V, L>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                    lds_bytes));                                                              \
-        hipLaunchKernelGGL((k_cl_search<V, L>), grid, dim3(bd), lds_bytes, ctx->stream, a);                   \
+#define CIM_LAUNCH(V, M, P)                                                                                      \
+    do {                                                                                                         \
+        CIM_HIP(hipFuncSetAttribute((const void*)k_cl_search<V, M, P>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                    lds_bytes));                                                                 \
+        hipLaunchKernelGGL((k_cl_search<V, M, P>), grid, dim3(bd), lds_bytes, ctx->stream, a);                   \
     } while (0)
-    if (use_lds) {
-        if (vpt == 1) CIM_LAUNCH(1, true);
-        else if (vpt == 2) CIM_LAUNCH(2, true);
-        else if (vpt == 3) CIM_LAUNCH(3, true);
-        else CIM_LAUNCH(4, true);
+#define CIM_LAUNCH_V(M, P)                  \
+    do {                                    \
+        if (vpt == 1) CIM_LAUNCH(1, M, P);  \
+        else if (vpt == 2) CIM_LAUNCH(2, M, P); \
+        else if (vpt == 3) CIM_LAUNCH(3, M, P); \
+        else CIM_LAUNCH(4, M, P);           \
+    } while (0)
+    if (mode == 0) {
+        if (pair) CIM_LAUNCH_V(0, true);
+        else CIM_LAUNCH_V(0, false);
     } else {
-        if (vpt == 1) CIM_LAUNCH(1, false);
-        else if (vpt == 2) CIM_LAUNCH(2, false);
-        else if (vpt == 3) CIM_LAUNCH(3, false);
-        else CIM_LAUNCH(4, false);
+        if (pair) CIM_LAUNCH_V(1, true);
+        else CIM_LAUNCH_V(1, false);
     }
+#undef CIM_LAUNCH_V
 #undef CIM_LAUNCH
     CIM_HIP(hipGetLastError());
     return CIM_OK;
@@ -483,19 +522,22 @@ struct SearchShape {
     int vpt;
     bool lds;
     int lds_bytes;
+    bool pair;
 };
 
-static int tune_shape(cim_ctx* ctx, SearchArgs a, int lds_with, int lds_without, bool lds_fits, int No,
+static int tune_shape(cim_ctx* ctx, SearchArgs a, int lds_with, int lds_without, bool lds_fits, int No, bool pair,
                       SearchShape* out) {
+    const int ncol = pair ? No / 2 : No;
     const char* ev = getenv("CIM_SEARCH_VPT");
     const char* el = getenv("CIM_SEARCH_LDS");
     char key[96];
-    snprintf(key, sizeof(key), "shape:%d:%d:%d", a.R, No, a.T);
+    snprintf(key, sizeof(key), "shape:%d:%d:%d:%d", a.R, No, a.T, (int)pair);
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end() && !ev && !el) {
         out->vpt = it->second & 7;
         out->lds = (it->second >> 3) & 1;
         out->lds_bytes = out->lds ? lds_with : lds_without;
+        out->pair = pair;
         return CIM_OK;
     }
     a.nsamples = nullptr;
@@ -503,17 +545,17 @@ static int tune_shape(cim_ctx* ctx, SearchArgs a, int lds_with, int lds_without,
     CIM_HIP(hipEventCreate(&e0));
     CIM_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    SearchShape bs{0, false, 0};
+    SearchShape bs{0, false, 0, pair};
     for (int l = 1; l >= 0; --l) {
         if (l == 1 && !lds_fits) continue;
         if (el && atoi(el) != l) continue;
         for (int v = 1; v <= 4; ++v) {
-            const int bd = block_for(No, v);
-            if (bd > 1024 || (v > 1 && block_for(No, v) == block_for(No, v - 1))) continue;
+            const int bd = block_for(ncol, v);
+            if (bd > 1024 || (v > 1 && block_for(ncol, v) == block_for(ncol, v - 1))) continue;
             if (ev && atoi(ev) != v) continue;
             const int lb = l ? lds_with : lds_without;
             CIM_HIP(hipEventRecord(e0, ctx->stream));
-            int rc = launch_search(ctx, a, 1, lb, l == 1, v, bd);
+            int rc = launch_search(ctx, a, 1, lb, l == 1 ? 0 : 1, pair, v, bd);
             if (rc) return rc;
             CIM_HIP(hipEventRecord(e1, ctx->stream));
             CIM_HIP(hipEventSynchronize(e1));
@@ -522,7 +564,7 @@ static int tune_shape(cim_ctx* ctx, SearchArgs a, int lds_with, int lds_without,
             if (getenv("CIM_TUNE_VERBOSE")) fprintf(stderr, "cim tune: vpt %d lds %d block %d: %.3f ms\n", v, l, bd, ms);
             if (ms < best) {
                 best = ms;
-                bs = SearchShape{v, l == 1, lb};
+                bs = SearchShape{v, l == 1, lb, pair};
             }
         }
     }
@@ -551,6 +593,9 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     if (g.pts.empty()) return set_error(ctx, CIM_E_ARG, "search grid has no AU point for step %g", p->searchstep);
     const int npts = (int)g.base.size();
     if (g.No > 4096) return set_error(ctx, CIM_E_ARG, "searchstep %g too fine (Nomega > 4096)", p->searchstep);
+    // Friedel pairing needs the omega grid to close: step * Nomega/2 == 180
+    const bool pair = (g.No % 2 == 0) && std::fabs(p->searchstep * (double)(g.No / 2) - 180.0) < 1e-4 &&
+                      getenv("CIM_SEARCH_NOPAIR") == nullptr;
     std::vector<int> rows;
     int disk_total = 0;
     make_disk(R, rows, disk_total);
@@ -611,6 +656,13 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream, ft, N, box, R, Dc,
                            sb.pcrop.as<float2>(), 1, (const float*)nullptr, cc, 0);
         CIM_HIP(hipGetLastError());
+        CIM_ALLOC(sb.pcrop4, sizeof(float4) * (size_t)Dc * Dc * nb);
+        {
+            const long long tot = (long long)Dc * Dc * nb;
+            hipLaunchKernelGGL(k_pair_taps, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                               sb.pcrop.as<float2>(), sb.pcrop4.as<float4>(), Dc, nb);
+            CIM_HIP(hipGetLastError());
+        }
         SearchArgs a{};
         if (p->apply_ctf) {
             CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * nb * T);
@@ -639,6 +691,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         CIM_ALLOC(sb.outi, sizeof(int) * (size_t)npts * nb);
         if (pr_array) CIM_ALLOC(sb.prfull, sizeof(float) * (size_t)npts * g.No * nb);
         a.pcrop = sb.pcrop.as<float2>();
+        a.pcrop4 = sb.pcrop4.as<float4>();
         a.fcut0 = sb.fcut0.as<float>();
         a.tcrop = sb.tcrop.as<float2>();
         a.tfcut = sb.tfcut.as<float>();
@@ -667,13 +720,15 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         CIM_HIP(hipMemsetAsync(sb.counter.ptr, 0, sizeof(unsigned long long), ctx->stream));
         a.nsamples = sb.counter.as<unsigned long long>();
         SearchShape sh{};
-        rc = tune_shape(ctx, a, lds_with, lds_without, lds_fits, g.No, &sh);
+        rc = tune_shape(ctx, a, lds_with, lds_without, lds_fits, g.No, pair, &sh);
         if (rc) return rc;
         ctx->counters["cl_search_vpt"] = sh.vpt;
         ctx->counters["cl_search_lds"] = sh.lds;
+        ctx->counters["cl_search_pair"] = sh.pair;
         {
             TimedLaunch tl(ctx, "cl_search");
-            rc = launch_search(ctx, a, nb, sh.lds_bytes, sh.lds, sh.vpt, block_for(g.No, sh.vpt));
+            rc = launch_search(ctx, a, nb, sh.lds_bytes, sh.lds ? 0 : 1, sh.pair, sh.vpt,
+                               block_for(pair ? g.No / 2 : g.No, sh.vpt));
             if (rc) return rc;
         }
         CIM_ALLOC(sb.fv, sizeof(float) * nb);

@@ -47,8 +47,20 @@ def _parts(ps):
     return P
 
 
+VARIANTS = {"tuned": {}, "no_pair": {"CIM_SEARCH_NOPAIR": "1"}, "lds": {"CIM_SEARCH_LDS": "1"},
+            "cache_vpt2_nopair": {"CIM_SEARCH_LDS": "0", "CIM_SEARCH_VPT": "2", "CIM_SEARCH_NOPAIR": "1"}}
+
+
+@pytest.fixture(params=sorted(VARIANTS))
+def variant(request, monkeypatch):
+    """Every launch variant of the search kernel (the autotuner picks one per shape)."""
+    for k, v in VARIANTS[request.param].items():
+        monkeypatch.setenv(k, v)
+    return request.param
+
+
 @pytest.mark.parametrize("ctf", [True, False])
-def test_pr_array_matches_oracle(ctx, O64, ctf):
+def test_pr_array_matches_oracle(ctx, O64, ctf, variant):
     box, minR, maxR, step, apix = 64, 1, 12, 3.0, 2.0
     mask_r = box / 2 - 4
     ph, tang, tsh, timgs, ps = _setup(box, maxR, minR, T=3, n=1, ctf=ctf, apix=apix)
