@@ -117,6 +117,7 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->srch.tft.release();
     ctx->srch.tfcut.release();
     ctx->srch.wk.release();
+    ctx->lref.release();
     for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->aux, &ctx->aux2, &ctx->parts, &ctx->out}) b->release();
     resolve_timing(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);

@@ -76,6 +76,16 @@ struct SearchState {
     SearchBatch wk;
 };
 
+// Resident model transform and per-batch buffers of local_refine.
+struct LrefState {
+    bool ready = false;
+    int box = 0;
+    DevBuf model, acc, disk, symops, mparts, g, tfac;
+    void release() {
+        for (DevBuf* b : {&model, &acc, &disk, &symops, &mparts, &g, &tfac}) b->release();
+    }
+};
+
 struct KTimer {
     double total_ms = 0.0;
     long long launches = 0;
@@ -90,6 +100,7 @@ struct cim_ctx {
     std::map<int, cim::FftPlan> plans;
     cim::RecState rec;
     cim::SearchState srch;
+    cim::LrefState lref;
     // scratch buffers reused across calls
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
     // kernel timing: event pairs recorded around launches, resolved lazily

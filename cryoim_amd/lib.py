@@ -17,6 +17,18 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cryoim.h")
 
 PART_STRIDE = 12
 P_THETA, P_PHI, P_OMEGA, P_X, P_Y, P_PR, P_DF1, P_DF2, P_ASTIG, P_MASKX, P_MASKY, P_USE = range(12)
+P_BOXX, P_BOXY = P_MASKX, P_MASKY
+
+
+def sym_ops(sym: str) -> np.ndarray:
+    """Symmetry operators of the local programs (cim_sym_ops), (n, 3, 3)."""
+    L = load()
+    n = ctypes.c_int()
+    if L.cim_sym_ops(sym.encode(), None, ctypes.byref(n)) != 0:
+        raise CimError(f"unknown symmetry {sym!r}")
+    out = np.zeros(9 * n.value, np.float64)
+    L.cim_sym_ops(sym.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n))
+    return out.reshape(-1, 3, 3)
 
 
 class CimParams(ctypes.Structure):
@@ -95,6 +107,9 @@ def load() -> ctypes.CDLL:
         "cim_search_global_dev": (ip, [vp, P, ip, vp, fp]),
         "cim_search_pr_array": (ip, [vp, P, fp, fp, fp]),
         "cim_refine_local": (ip, [vp, P, ip, fp, fp]),
+        "cim_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
+        "cim_lref_model": (ip, [vp, P, fp]),
+        "cim_lref_refine": (ip, [vp, P, ip, fp, ip, fp, ctypes.c_char_p, ctypes.c_float, ctypes.c_float]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):
@@ -268,6 +283,21 @@ class Context:
         self._check(self.lib.cim_search_pr_array(self.h, ctypes.byref(params), _fptr(im), _fptr(pa), _fptr(out)),
                     "cim_search_pr_array")
         return out.reshape(shape)
+
+    # -- local_refine ---------------------------------------------------------
+    def lref_model(self, params: CimParams, vol):
+        v = _f32(vol)
+        self._check(self.lib.cim_lref_model(self.h, ctypes.byref(params), _fptr(v)), "cim_lref_model")
+
+    def lref_refine(self, params: CimParams, images, parts, sym: str = "c1", convF: float = 0.8,
+                    maxres: float = 10.0) -> np.ndarray:
+        im = _f32(images)
+        parts = _f32(parts).reshape(-1, PART_STRIDE).copy()
+        box0 = im.shape[-1]
+        self._check(self.lib.cim_lref_refine(self.h, ctypes.byref(params), parts.shape[0], _fptr(im), box0,
+                                             _fptr(parts), sym.encode(), float(convF), float(maxres)),
+                    "cim_lref_refine")
+        return parts
 
     def refine_local(self, params: CimParams, images, parts: np.ndarray) -> np.ndarray:
         parts = _f32(parts).reshape(-1, PART_STRIDE).copy()

@@ -23,7 +23,8 @@
  *   fold + inversion . icosprocess/SRC/reconstruct.f90:151-247
  *   global search .... icosprocess/SRC/ort_globesearch.f90:34-134
  *   local refinement . icosprocess/SRC/ort_refinement.f90:31-124
- *   slice + PR ....... local_refine/SRC/refine_PR.f90:69-245
+ *   slice + PR ....... local_refine/SRC/refine_PR.f90:69-245, FFT3d_slice.f90,
+ *                      cal_PR_FT.f90, cal_template_FT.f90
  */
 #ifndef CRYOIM_H
 #define CRYOIM_H
@@ -79,6 +80,10 @@ typedef struct cim_params {
 #define CIM_P_MASKX 9     /* mask centre x (see cim_rec_insert) */
 #define CIM_P_MASKY 10
 #define CIM_P_USE 11      /* 1.0 = insert, 0.0 = skip (PR/boundX filter) */
+/* local_refine records reuse slots 9, 10 for the sub-particle box centre in
+ * the full particle image (local_hx0, local_ky0 of the local ortfile). */
+#define CIM_P_BOXX 9
+#define CIM_P_BOXY 10
 
 /* ---- context ---------------------------------------------------------- */
 int cim_init(cim_ctx** ctx, int device);
@@ -159,6 +164,26 @@ int cim_search_pr_array(cim_ctx* ctx, const cim_params* p, const float* image, c
 /* Coordinate-descent refinement of orientation and centre
  * (ort_refinement.f90:31-124, ort_cent_11.f90, return_icos_asymmetry_unit.f90). */
 int cim_refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* images, float* parts);
+
+/* ---- local_refine: central slices + full-plane phase residual ---------- */
+/* Symmetry operators of the local programs (local_refine/SRC/icos2f_sym_matc.f90):
+ * "c1".."c360" (rotations by i*360/N, i = 1..N), "asym", "i2", "i5", and
+ * "icos" (the icos2f_sym_mat table).  ops may be NULL to query nops. */
+int cim_sym_ops(const char* sym, double* ops, int* nops);
+/* Resident model transform (refine_PR.f90:21-54): soft mask of radius
+ * p->imgmask (img3d_softmask.f90), checkerboard, forward 3D FFT (1/N, +Im),
+ * checkerboard.  vol: box^3 real map, box = p->box. */
+int cim_lref_model(cim_ctx* ctx, const cim_params* p, const float* vol);
+/* Refine n sub-particles (refine_PR.f90:69-233).  images: n full particle
+ * images of box0^2; each record gives the local box centre (CIM_P_BOXX/Y),
+ * the starting local theta, phi, omega, x, y (replaced by the result, PR in
+ * CIM_P_PR) and df1, df2 (A, p->df_scale = 1), astig.  p->ncycle cycles of
+ * ort_cent_11 with steps p->delta_angle, p->delta_trans shrunk by convF;
+ * sym != "c1" runs the icos_equalort stage first.  maxres: the resolution
+ * limit in Fourier pixels (apix*box/maxRes) that bounds the residual and sets
+ * the Gaussian low-pass exp(-r^2 / (2 maxres^2)). */
+int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, float* parts,
+                    const char* sym, float convF, float maxres);
 
 #ifdef __cplusplus
 }

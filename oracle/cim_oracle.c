@@ -893,3 +893,151 @@ EXPORT void o_img3d_softmask(real* d, int nx, real mask)
         if (r > out) d[i] = summ;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* local_refine driver                                                 */
+/* ------------------------------------------------------------------ */
+
+/* Symmetry operators of the local programs, in their order:
+ * kind 0 = Cn (local_refine/SRC/icos2f_sym_matc.f90:1-18: rotations by
+ *          i*2pi/N, i = 1..N), kind 1 = asym (:20-36, steps pi/6/N),
+ * kind 2 = I2 (:39-95), kind 3 = I5 (:97-118, I2 conjugated). */
+static void mm3(const real* A, const real* B, real* C) { matmul3(A, B, C); }
+static void tr3(const real* A, real* T)
+{
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) T[3 * j + i] = A[3 * i + j];
+}
+
+EXPORT int o_lref_symops(int kind, int N, real* r)
+{
+    real pi = r_pi();
+    if (kind == 0 || kind == 1) {
+        real ph = kind == 0 ? (real)2.0 * pi / (real)N : pi / (real)6.0 / (real)N;
+        real R1[9] = {(real)cos(ph), -(real)sin(ph), 0, (real)sin(ph), (real)cos(ph), 0, 0, 0, 1};
+        memcpy(r, R1, sizeof(R1));
+        for (int i = 1; i < N; ++i) mm3(r + 9 * (i - 1), R1, r + 9 * i);
+        return N;
+    }
+    real d2r = pi / (real)180.0;
+    real a5 = (real)72.0 * d2r;
+    real v = (real)acos((real)cos((real)60.0 * d2r) / (real)sin((real)36.0 * d2r));
+    real rz5[9] = {(real)cos(a5), -(real)sin(a5), 0, (real)sin(a5), (real)cos(a5), 0, 0, 0, 1};
+    real ryv[9] = {(real)cos(v), 0, (real)sin(v), 0, 1, 0, -(real)sin(v), 0, (real)cos(v)};
+    real t1[9], t2[9], ryvT[9];
+    tr3(ryv, ryvT);
+    mm3(rz5, ryvT, t1);
+    mm3(ryv, t1, t2);
+    memcpy(rz5, t2, sizeof(t2));
+    real rx2[9] = {1, 0, 0, 0, -1, 0, 0, 0, -1};
+    real ry2[9] = {-1, 0, 0, 0, 1, 0, 0, 0, -1};
+    real r111[9] = {0, 0, 1, 1, 0, 0, 0, 1, 0};
+    memset(r, 0, sizeof(real) * 9 * 60);
+    r[0] = r[4] = r[8] = 1;
+    for (int i = 1; i < 5; ++i) mm3(r + 9 * (i - 1), rz5, r + 9 * i);
+    for (int i = 5; i < 10; ++i) mm3(rx2, r + 9 * (i - 5), r + 9 * i);
+    for (int i = 10; i < 20; ++i) mm3(ry2, r + 9 * (i - 10), r + 9 * i);
+    for (int i = 20; i < 40; ++i) mm3(r111, r + 9 * (i - 20), r + 9 * i);
+    for (int i = 40; i < 60; ++i) mm3(r111, r + 9 * (i - 20), r + 9 * i);
+    if (kind == 3) {
+        real rot0[9] = {(real)cos(v), 0, (real)sin(v), 0, 1, 0, -(real)sin(v), 0, (real)cos(v)};
+        real rT[9];
+        tr3(rot0, rT);
+        for (int i = 0; i < 60; ++i) {
+            mm3(r + 9 * i, rT, t1);
+            mm3(rot0, t1, r + 9 * i);
+        }
+    }
+    return 60;
+}
+
+/* icos_equalort: local_refine/SRC/icos_equalort.f90:2-27. */
+EXPORT void o_icos_equalort(real th0, real ph0, real om0, int nsym, const real* ops, real* ang)
+{
+    real E[9], M[9];
+    real r2d = (real)45.0 / (real)atan2((real)1.0, (real)1.0);
+    o_euler_mat(th0, -ph0, (real)180.0 - om0, E);
+    for (int i = 0; i < nsym; ++i) {
+        mm3(ops + 9 * i, E, M);
+        ang[3 * i] = (real)acos(M[8]) * r2d;
+        ang[3 * i + 1] = -(real)atan2(M[5], M[2]) * r2d;
+        ang[3 * i + 2] = (real)atan2(M[7], M[6]) * r2d;
+    }
+}
+
+/* img2d_local_box: local_refine/SRC/img2d_local_box.f90:1-19 (wrap-around cut). */
+EXPORT void o_local_box(const real* img, int n0, int ix, int iy, real* out, int n)
+{
+    for (long i = 0; i < (long)n * n; ++i) {
+        int ky = (int)(i / n), hx = (int)(i % n);
+        hx = hx + ix - n / 2;
+        ky = ky + iy - n / 2;
+        if (hx < 0) hx += n0;
+        if (hx >= n0) hx -= n0;
+        if (ky < 0) ky += n0;
+        if (ky >= n0) ky -= n0;
+        out[i] = img[(long)ky * n0 + hx];
+    }
+}
+
+/* One sub-particle of refine_PR (local_refine/SRC/refine_PR.f90:126-233):
+ *   fr, fi   model transform (nx^3, after softmask/checkerboard/FFT/checkerboard)
+ *   a20,b20  particle transform times w_lp (nx^2)
+ *   ctf,wlp  nx^2
+ *   nsym/ops symmetry stage (nsym = 0 skips it, sym = c1)
+ *   io5      theta phi omega x y in/out; returns the last PR. */
+EXPORT real o_lref_refine(const real* fr, const real* fi, int nx, const real* a20, const real* b20,
+                          const real* ctf, const real* wlp, int nsym, const real* ops, int ncycle,
+                          real refine_angle, real refine_shift, real convF, real maxres, real* io5)
+{
+    long n2 = (long)nx * nx;
+    real* a1 = (real*)malloc(sizeof(real) * n2);
+    real* b1 = (real*)malloc(sizeof(real) * n2);
+    real best = 1000;
+    real th0 = io5[0], ph0 = io5[1], om0 = io5[2], x0 = io5[3], y0 = io5[4];
+#define TEMPLATE_PR(TH, PH, OM, X, Y, OUT)                                           \
+    do {                                                                             \
+        o_fft3d_slice(fr, fi, nx, TH, PH, OM, a1, b1);                               \
+        o_ft_shift(a1, b1, nx, X, Y, 0);                                             \
+        for (long q = 0; q < n2; ++q) {                                              \
+            real s = (((q / nx) + (q % nx)) & 1) ? (real)-1 : (real)1;               \
+            a1[q] = a1[q] * s * ctf[q] * wlp[q];                                     \
+            b1[q] = b1[q] * s * ctf[q] * wlp[q];                                     \
+        }                                                                            \
+        OUT = o_pr_ft(a1, b1, a20, b20, nx, (real)-1.0, maxres);                     \
+    } while (0)
+    if (nsym > 0) {
+        real* ang = (real*)malloc(sizeof(real) * 3 * nsym);
+        o_icos_equalort(th0, ph0, om0, nsym, ops, ang);
+        int bi = 0;
+        for (int i = 0; i < nsym; ++i) {
+            real pr;
+            TEMPLATE_PR(ang[3 * i], ang[3 * i + 1], ang[3 * i + 2], x0, y0, pr);
+            if (i == 0 || pr < best) { best = pr; bi = i; }
+        }
+        th0 = ang[3 * bi]; ph0 = ang[3 * bi + 1]; om0 = ang[3 * bi + 2];
+        free(ang);
+    }
+    for (int c = 1; c <= ncycle; ++c) {
+        real f = 1;
+        for (int k = 1; k < c; ++k) f *= convF;
+        real da = refine_angle * f, dt = refine_shift * f;
+        /* local ort_cent_11 order: local_refine/SRC/ort_cent_11.f90:6-84 */
+        real oc[11][5];
+        for (int jj = 0; jj <= 10; ++jj) {
+            oc[jj][0] = th0; oc[jj][1] = ph0; oc[jj][2] = om0; oc[jj][3] = x0; oc[jj][4] = y0;
+        }
+        oc[1][3] -= dt; oc[2][3] += dt; oc[3][4] -= dt; oc[4][4] += dt;
+        oc[5][0] -= da; oc[6][0] += da; oc[7][1] -= da; oc[8][1] += da; oc[9][2] -= da; oc[10][2] += da;
+        int bi = 0;
+        for (int jj = 0; jj <= 10; ++jj) {
+            real pr;
+            TEMPLATE_PR(oc[jj][0], oc[jj][1], oc[jj][2], oc[jj][3], oc[jj][4], pr);
+            if (jj == 0 || pr < best) { best = pr; bi = jj; }
+        }
+        th0 = oc[bi][0]; ph0 = oc[bi][1]; om0 = oc[bi][2]; x0 = oc[bi][3]; y0 = oc[bi][4];
+    }
+#undef TEMPLATE_PR
+    io5[0] = th0; io5[1] = ph0; io5[2] = om0; io5[3] = x0; io5[4] = y0;
+    free(a1); free(b1);
+    return best;
+}

@@ -217,3 +217,37 @@ class Oracle:
         v = self._a(vol).copy()
         self.L.o_img3d_softmask(self._p(v), v.shape[-1], self._r(mask))
         return v
+
+    # ---- local_refine ----------------------------------------------------------------
+    def local_box(self, img, hx0, ky0, n):
+        a = self._a(img)
+        out = np.zeros((n, n), self.dt)
+        self.L.o_local_box(self._p(a), a.shape[-1], int(hx0), int(ky0), self._p(out), int(n))
+        return out
+
+    def lref_symops(self, kind, n):
+        r = np.zeros(9 * max(n, 60), self.dt)
+        m = self.L.o_lref_symops(int(kind), int(n), self._p(r))
+        return r[:9 * m].reshape(m, 3, 3)
+
+    def icos_equalort(self, th, ph, om, ops):
+        ops = self._a(ops).reshape(-1, 9)
+        out = np.zeros(3 * len(ops), self.dt)
+        self.L.o_icos_equalort(self._r(th), self._r(ph), self._r(om), len(ops), self._p(ops), self._p(out))
+        return out.reshape(-1, 3)
+
+    def lref_refine(self, fr, fi, a20, b20, ctf, wlp, ops, ncycle, refine_angle, refine_shift, convF, maxres,
+                    start5):
+        nx = a20.shape[-1]
+        io5 = self._a(start5).copy()
+        if ops is None:
+            nsym, optr = 0, None
+        else:
+            o = self._a(ops).reshape(-1, 9)
+            nsym, optr = len(o), self._p(o)
+        self.L.o_lref_refine.restype = self.ct
+        pr = self.L.o_lref_refine(self._p(self._a(fr)), self._p(self._a(fi)), nx, self._p(self._a(a20)),
+                                  self._p(self._a(b20)), self._p(self._a(ctf)), self._p(self._a(wlp)), nsym, optr,
+                                  int(ncycle), self._r(refine_angle), self._r(refine_shift), self._r(convF),
+                                  self._r(maxres), self._p(io5))
+        return io5, pr

@@ -168,6 +168,65 @@ def lref_vectors(rng) -> dict:
     v12r, v12i = rng.standard_normal((2, 12, 12, 12)).astype(f4)
     g["fft3_in"] = np.stack([v12r, v12i])
     g["fft3_out"] = _run_unit("lref_units", "fft3", _b(np.array([12], i4), v12r, v12i), 2 * 1728).reshape(2, 12, 12, 12)
+    # symmetry generators of the local programs and icos_equalort
+    for kind, nn in ((0, 5), (1, 4), (2, 60), (3, 60)):
+        g[f"lsym_{kind}_{nn}"] = _run_unit("lref_units", "symops", _b(np.array([kind, nn], i4)),
+                                           9 * nn).reshape(nn, 3, 3)
+    eq_in = np.array([71.0, 23.0, 140.0], f4)
+    g["equalort_in"] = eq_in
+    for kind, nn in ((0, 5), (2, 60), (3, 60)):
+        g[f"equalort_{kind}_{nn}"] = _run_unit("lref_units", "equalort", _b(np.array([kind, nn], i4), eq_in),
+                                               3 * nn).reshape(nn, 3)
+    return g
+
+
+def lref_program_vectors() -> dict:
+    """The real local_refine on sub-particles cut from 48-pixel images
+    against a 32-voxel model (c1, and c5 with the symmetry stage)."""
+    from cryoim_amd import geometry
+    g = {}
+    box, box0, apix, n = 32, 48, 2.0, 6
+    ph = synth.make_phantom(box, nblobs=8, radius=(6, 10), sigma=(1.2, 1.8), seed=17, sym=False)
+    model = synth.phantom_map(ph)
+    rng = np.random.default_rng(18)
+    ang = np.c_[rng.uniform(20, 160, n), rng.uniform(-180, 180, n), rng.uniform(0, 360, n)]
+    sh = rng.uniform(-1.0, 1.0, (n, 2))
+    proj = synth.project(ph, ang, sh, box=box)
+    df = rng.uniform(8000, 20000, n)              # Angstrom, as the local programs take it
+    imgs = np.zeros((n, box0, box0), f4)
+    cen = rng.integers(18, 30, (n, 2))
+    for i in range(n):
+        c = synth.ctf2d(box, df[i], df[i], 0.0, apix, vol=200.0, df_scale=1.0)
+        p = synth.apply_ctf(proj[i], c) + 0.3 * rng.standard_normal((box, box))
+        ys, xs = (np.arange(box) + cen[i, 1] - box // 2) % box0, (np.arange(box) + cen[i, 0] - box // 2) % box0
+        imgs[i][np.ix_(ys, xs)] = p
+    start = ang + rng.uniform(-2.0, 2.0, (n, 3))
+    g.update(lprog_model=model, lprog_images=imgs, lprog_angles=ang.astype(f4), lprog_shifts=sh.astype(f4),
+             lprog_df=df.astype(f4), lprog_cen=cen.astype(i4), lprog_start=start.astype(f4),
+             lprog_box=np.array([box, box0], i4), lprog_apix=np.array([apix], f4))
+    exe = os.path.join(REF, "local_refine")
+    with tempfile.TemporaryDirectory() as d:
+        formats.write_mrc(os.path.join(d, "model.mrc"), model, is_map=True)
+        formats.write_mrc(os.path.join(d, "stk.mrc"), imgs)
+        with open(os.path.join(d, "lort.txt"), "w") as f:
+            for i in range(n):
+                f.write(f"stk.mrc {i + 1} 0 0 0 0 0 0 {i + 1} {i + 1} {df[i]:.3f} {df[i]:.3f} 0 {i + 1} "
+                        f"{cen[i, 0]} {cen[i, 1]} {start[i, 0]:.4f} {start[i, 1]:.4f} {start[i, 2]:.4f} "
+                        f"{sh[i, 0] * 0.5:.4f} {sh[i, 1] * 0.5:.4f} 0 0\n")
+        for tag, extra in (("c1", []), ("c5", ["sym=c5"])):
+            r = subprocess.run([exe, "local_ort0=lort.txt", f"local_newort={tag}.txt", "model3d=model.mrc",
+                                f"apix={apix}", "maxRes=8", "Ncycle=3", "imgmask=24", *extra], cwd=d,
+                               stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            rows = []
+            with open(os.path.join(d, f"{tag}.txt")) as f:
+                for line in f:
+                    t = line[64:].split()
+                    # id th ph om x y PR imgID imgpid df1 df2 ast limgID hx0 ky0 lth lph lom lx ly lz lPR ...
+                    rows.append([float(v) for v in t[15:22]])
+            g[f"lprog_{tag}_out"] = np.array(rows, f4)
+    _ = geometry
     return g
 
 
@@ -225,6 +284,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "pr.npz"), **pr_vectors(rng))
     np.savez_compressed(os.path.join(HERE, "lref_units.npz"), **lref_vectors(rng))
     np.savez_compressed(os.path.join(HERE, "programs.npz"), **program_vectors())
+    np.savez_compressed(os.path.join(HERE, "lref_program.npz"), **lref_program_vectors())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -62,3 +62,42 @@ def particle_inputs(O, img, x, y, mask_r, minR, maxR):
     fc0 = O.amp_expfit(a0, b0, minR, maxR)
     pa, pb = O.ft_shift(a0, b0, x, y, True)
     return (a0, b0), (pa, pb), fc0
+
+
+# ---- local_refine (local_refine/SRC/refine_PR.f90:21-134) -------------------------
+
+def lref_derived(box, apix, maxRes=10.0, imgmask_A=None):
+    """maxRes (px), mask radius, default steps as calc_initialvilues
+    (local_refine/SRC/0calc_initialvalue.f90:61-71)."""
+    f = np.float32
+    maxres = f(apix) * f(box) / f(maxRes)
+    mask = f(box // 2 - 2) if imgmask_A is None else f(imgmask_A) / f(apix)
+    ang = f(360.0) / (maxres * np.sqrt(f(5.0)) * f(3.14159))
+    sh = f(box) / (maxres * np.sqrt(f(5.0)))
+    return dict(maxres=float(maxres), mask=float(mask), refine_angle=float(ang), refine_shift=float(sh))
+
+
+def lref_model_ft(O, model, mask_r):
+    """softmask, checkerboard, forward 3D FFT (+Im), checkerboard (refine_PR.f90:24-54)."""
+    n = model.shape[-1]
+    v = O.softmask3d(model, mask_r)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    s = np.where((x + y + z) % 2, -1.0, 1.0)
+    fr, fi = O.fft3(v * s, np.zeros_like(v), 1, False)
+    return fr * s, fi * s
+
+
+def lref_wlp(n, maxres):
+    ky, hx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    r2 = ((hx - n // 2) ** 2 + (ky - n // 2) ** 2).astype(np.float64) / 2.0 / maxres ** 2
+    return np.exp(-r2)
+
+
+def lref_particle(O, img_full, box, hx0, ky0, x0, y0, mask_r, maxres):
+    a = O.norm(img_full)
+    a = O.local_box(a, hx0, ky0, box)
+    a = O.mask(a, mask_r, x0, y0)
+    a = O.checker(a)
+    fa, fb = O.fft2(a, np.zeros_like(a), 1, False)
+    w = lref_wlp(box, maxres)
+    return fa * w, fb * w, w

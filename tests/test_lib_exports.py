@@ -19,3 +19,16 @@ def test_library_exports_every_declared_symbol():
     L = ctypes.CDLL(lib.LIB_PATH)
     missing = [n for n in lib.header_functions() if not hasattr(L, n)]
     assert not missing, f"not exported: {missing}"
+
+
+def test_local_symmetry_operators_without_gpu():
+    """cim_sym_ops is host-only: the local programs' generators (icos2f_sym_matc.f90)."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    import numpy as np
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lref_units.npz"))
+    for sym, key in (("c5", "lsym_0_5"), ("i2", "lsym_2_60"), ("i5", "lsym_3_60")):
+        assert np.abs(lib.sym_ops(sym) - g[key]).max() < 2e-6
+    assert lib.sym_ops("icos").shape == (60, 3, 3)
+    with pytest.raises(lib.CimError):
+        lib.sym_ops("d7")

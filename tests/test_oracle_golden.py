@@ -321,3 +321,55 @@ def test_mode5_refinement_matches_reference(O32, prog):
                                       d["dang"], d["dtran"], start)
         assert np.allclose(io5, ref[i, :5], atol=2e-3), (i, io5, ref[i])
         assert abs(pr - ref[i, 5]) < 2e-3
+
+
+# ---- local_refine program -----------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def lprog():
+    return np.load(os.path.join(G, "lref_program.npz"))
+
+
+def test_lref_symmetry_generators(O32, O64, lref):
+    for kind, n in ((0, 5), (1, 4), (2, 60), (3, 60)):
+        ref = lref[f"lsym_{kind}_{n}"]
+        assert np.abs(O32.lref_symops(kind, n) - ref).max() < 2e-6
+        ops = O64.lref_symops(kind, n)
+        for S in ops:
+            assert abs(np.linalg.det(S) - 1) < 1e-12
+    # I2 is the icosahedral group of icos2f_sym_mat in another element order
+    a = O64.lref_symops(2, 60).reshape(60, 9)
+    b = O64.icos_ops().reshape(60, 9)
+    assert all(np.abs(b - x).max(axis=1).min() < 1e-9 for x in a)
+
+
+def test_icos_equalort(O32, lref):
+    th, ph, om = lref["equalort_in"]
+    for kind, n in ((0, 5), (2, 60), (3, 60)):
+        out = O32.icos_equalort(th, ph, om, O32.lref_symops(kind, n))
+        d = (out - lref[f"equalort_{kind}_{n}"] + 180) % 360 - 180
+        assert np.abs(d).max() < 2e-3
+
+
+@pytest.mark.parametrize("tag", ["c1", "c5"])
+def test_local_refine_program_matches_reference(tag, O32, lprog):
+    box, box0 = (int(v) for v in lprog["lprog_box"])
+    apix = float(lprog["lprog_apix"][0])
+    d = pipelines.lref_derived(box, apix, maxRes=8.0, imgmask_A=24.0)
+    fr, fi = pipelines.lref_model_ft(O32, lprog["lprog_model"], d["mask"])
+    ops = O32.lref_symops(0, 5) if tag == "c5" else None
+    ref = lprog[f"lprog_{tag}_out"]
+    for i in range(len(ref)):
+        x0, y0 = (round(float(v) * 0.5, 4) for v in lprog["lprog_shifts"][i])
+        hx0, ky0 = lprog["lprog_cen"][i]
+        a20, b20, w = pipelines.lref_particle(O32, lprog["lprog_images"][i], box, hx0, ky0, x0, y0, d["mask"],
+                                              d["maxres"])
+        df = float(lprog["lprog_df"][i])
+        ctf = O32.ctf2d(box, df, df, 0.0, apix, vol=200.0, df_scale=1.0)
+        start = np.array([*lprog["lprog_start"][i], x0, y0], np.float32)
+        start[:3] = np.round(start[:3], 4)
+        io5, pr = O32.lref_refine(fr, fi, a20, b20, ctf, w, ops, 3, d["refine_angle"], d["refine_shift"], 0.8,
+                                  d["maxres"], start)
+        da = (io5[:3] - ref[i, :3] + 180) % 360 - 180
+        assert np.abs(da).max() < 2e-3 and np.abs(io5[3:5] - ref[i, 3:5]).max() < 2e-3, (i, io5, ref[i])
+        assert abs(pr - ref[i, 6]) < 2e-3
