@@ -1,0 +1,379 @@
+! icosprocess_hip: the icosprocess command line (key=value parameters,
+! ortfile / MRC / tnf files, icosprocess/SRC/0main.f90) driving the MI355X
+! library through ISO_C_BINDING for
+!   mode=1            3D reconstruction (reconstruct.f90; subrec=y writes
+!                     the tnf partial volume of reconstruct_substep.f90)
+!   mode=4            global orientation search (ort_globesearch.f90)
+!   mode=5            local orientation/centre refinement (ort_refinement.f90)
+! Parameter keys, defaults and derived values follow input_parameters,
+! initialvalue and calc_initialvalues.  Output files have the reference's
+! formats, so the programs chained after icosprocess read them unchanged.
+program icosprocess_hip
+    use cim_iface
+    implicit none
+    integer, parameter :: NPAR = 32
+    character(len=64) :: coeff(NPAR)
+    character(len=256) :: arg, key, val
+    character(len=256) :: ortfile = '', newortfile = 'newort.dat', result3d = 'map3d.mrc'
+    character(len=256) :: templateort = '', templatestck = '', stck0
+    character(len=4) :: sym = 'icos'
+    character :: applyctf = 'y', imgmask_on = 'y', subrec = 'n', realspaceavg = 'n'
+    integer :: mode = 1, first = 1, last = 0, ncycle = 6, nargs, i, j, k, eq, fftsize, minR, maxR, T
+    real :: apix = 1.0, minres = 300.0, maxres = 8.0, imgmask = -1.0, vol = 300.0, cs = 2.7, ampwg = 0.1
+    real :: pr_threshold = 90.0, searchstep = 2.0, boundx = -1.0, deltaangle, deltatrans
+    logical :: have_mask = .false., have_boundx = .false.
+
+    type particle_t
+        character(len=54) :: stck
+        integer :: pid, imgid, imgpid
+        real :: theta, phi, omega, x, y, pr, df1, df2, astig
+    end type
+    type(particle_t), allocatable :: part(:)
+    type(c_ptr) :: ctx
+    type(cim_params) :: p
+    integer :: nx, ny, nz, rc
+
+    coeff = [character(len=64) :: 'apix', 'minRes', 'maxRes', 'imgmask', 'first', 'last', 'applyCTF', 'Ncycle', &
+             'templatestck', 'vol', 'Cs', 'Bfactor', 'ampwg', 'PR_threshold', 'result3d', 'mode', 'newortfile', &
+             'model3d', 'templateort', 'searchstep', 'realspaceavg', 'ortfile', 'recISAF', 'subrec', 'FSC', &
+             'imgstck', 'sym', 'maxcentshift', 'corrCTF', 'boundX', 'loc_box', 'check']
+    nargs = command_argument_count()
+    if (nargs < 2) then
+        print *, 'usage: icosprocess_hip mode=<1|4|5> ortfile=... [key=value ...] (icosprocess keys)'
+        stop
+    end if
+    do i = 1, nargs
+        call get_command_argument(i, arg)
+        eq = index(arg, '=')
+        key = arg(1:max(eq - 1, 0))
+        val = arg(eq + 1:)
+        if (.not. any(coeff == key)) then
+            print *, 'unknown parameter: ', trim(arg)
+            stop 2
+        end if
+        ! the reference compares several keys against a different spelling
+        ! when it reads the value (Bfactor, FSC, corrCTF, recISAF), so those
+        ! are accepted and ignored, as there (0main.f90:122-174)
+        select case (trim(key))
+        case ('mode'); read(val, *) mode
+        case ('ortfile'); ortfile = val
+        case ('templateort'); templateort = val
+        case ('templatestck'); templatestck = val
+        case ('minRes'); read(val, *) minres
+        case ('maxRes'); read(val, *) maxres
+        case ('imgmask'); read(val, *) imgmask; have_mask = .true.
+        case ('applyCTF'); applyctf = val(1:1)
+        case ('apix'); read(val, *) apix
+        case ('vol'); read(val, *) vol
+        case ('Cs'); read(val, *) cs
+        case ('ampwg'); read(val, *) ampwg
+        case ('first'); read(val, *) first
+        case ('last'); read(val, *) last
+        case ('PR_threshold'); read(val, *) pr_threshold
+        case ('Ncycle'); read(val, *) ncycle
+        case ('result3d'); result3d = val
+        case ('newortfile'); newortfile = val
+        case ('searchstep'); read(val, *) searchstep
+        case ('realspaceavg'); realspaceavg = val(1:1)
+        case ('subrec'); subrec = val(1:1)
+        case ('sym'); sym = val(1:4)
+        case ('boundX'); read(val, *) boundx; have_boundx = .true.
+        end select
+    end do
+    if (trim(sym) /= 'icos') then
+        print *, 'icosprocess_hip runs the icosahedral (sym=icos) path only'
+        stop 2
+    end if
+
+    ! initialvalue: box from the first stack, particle count from the ortfile
+    open(21, file=trim(ortfile), status='old')
+    read(21, *) stck0
+    rewind(21)
+    k = 0
+    do
+        read(21, *, iostat=rc) arg
+        if (rc /= 0) exit
+        k = k + 1
+    end do
+    close(21)
+    if (last <= 0 .or. last > k) last = k
+    call mrc_dims(stck0, nx, ny, nz)
+    fftsize = nx
+    if (.not. have_mask) imgmask = real(fftsize / 2 - 2)
+    if (.not. have_boundx) boundx = real(fftsize / 2 - 2)
+    ! calc_initialvalues (0calc_initialvalues.f90:6-21)
+    minres = apix * fftsize / minres
+    maxres = apix * fftsize / maxres
+    minR = int(minres)
+    maxR = int(maxres)
+    if (mod(maxR, 2) /= 0) maxR = maxR + 1
+    if (imgmask_on == 'y') imgmask = imgmask / apix
+    deltaangle = 360.0 / (maxres * sqrt(5.0) * 3.14159)
+    deltatrans = real(fftsize) / (maxres * sqrt(5.0))
+
+    call load_particles()
+
+    p%box = fftsize
+    p%minR = minR
+    p%maxR = maxR
+    p%apply_ctf = merge(1, 0, applyctf == 'y' .or. applyctf == 'Y')
+    p%mask_on = 1
+    p%conj_out = 1
+    p%apix = apix
+    p%imgmask = imgmask
+    p%vol = vol
+    p%cs = cs
+    p%ampwg = ampwg
+    p%df_scale = 1.0e4
+    p%searchstep = searchstep
+    p%ncycle = ncycle
+    p%delta_angle = deltaangle
+    p%delta_trans = deltatrans
+
+    rc = cim_init(ctx, 0)
+    if (rc /= CIM_OK) then
+        print *, 'no HIP device (cim_init returned', rc, ')'
+        stop 1
+    end if
+    select case (mode)
+    case (1)
+        call run_reconstruct()
+    case (4, 5)
+        call run_search()
+    case default
+        print *, 'icosprocess_hip implements modes 1, 4 and 5'
+        stop 2
+    end select
+    rc = cim_finalize(ctx)
+
+contains
+
+    subroutine mrc_dims(fname, nx, ny, nz)
+        character(len=*), intent(in) :: fname
+        integer, intent(out) :: nx, ny, nz
+        integer :: u
+        open(newunit=u, file=trim(fname), form='unformatted', access='stream', status='old')
+        read(u) nx, ny, nz
+        close(u)
+    end subroutine
+
+    ! load_original_ortcent.f90:12-26 and checkstckfile.f90:6-24
+    subroutine load_particles()
+        integer :: u, i, n, ios
+        allocate(part(first:last))
+        open(newunit=u, file=trim(ortfile), status='old')
+        do i = 1, first - 1
+            read(u, *)
+        end do
+        do i = first, last
+            read(u, *) part(i)%stck, part(i)%pid, part(i)%theta, part(i)%phi, part(i)%omega, part(i)%x, &
+                       part(i)%y, part(i)%pr, part(i)%imgid, part(i)%imgpid, part(i)%df1, part(i)%df2, part(i)%astig
+        end do
+        close(u)
+        do i = first, last
+            open(newunit=u, file=trim(part(i)%stck), form='unformatted', access='stream', status='old', iostat=ios)
+            if (ios /= 0) then
+                print *, 'stack file missing: ', trim(part(i)%stck)
+                stop 1
+            end if
+            read(u) n, n, n
+            close(u)
+            if (part(i)%imgpid > n) then
+                print *, 'stack ', trim(part(i)%stck), ' has ', n, ' images; particle asks for ', part(i)%imgpid
+                stop 1
+            end if
+        end do
+    end subroutine
+
+    subroutine read_image(pt, img)
+        type(particle_t), intent(in) :: pt
+        real(c_float), intent(out) :: img(:)
+        integer :: u
+        integer(8) :: pos
+        pos = 1024_8 + 4_8 * int(fftsize, 8)**2 * int(pt%imgpid - 1, 8) + 1_8
+        open(newunit=u, file=trim(pt%stck), form='unformatted', access='stream', status='old')
+        read(u, pos=pos) img
+        close(u)
+    end subroutine
+
+    ! Records for a batch.  The CTF is recomputed only when imgID changes in
+    ! reconstruct.f90:85-89 and ort_globesearch.f90:61-65, so the defocus of
+    ! the first particle of each micrograph run is carried forward there.
+    subroutine fill_records(i0, i1, rec, cache_ctf, mask_first)
+        integer, intent(in) :: i0, i1
+        real(c_float), intent(out) :: rec(CIM_PART_STRIDE, *)
+        logical, intent(in) :: cache_ctf, mask_first
+        integer :: i, k, imgid0
+        real :: d1, d2, ast
+        imgid0 = -100
+        d1 = 0.0; d2 = 0.0; ast = 0.0
+        do i = first, i1
+            if (.not. cache_ctf .or. part(i)%imgid /= imgid0) then
+                d1 = part(i)%df1; d2 = part(i)%df2; ast = part(i)%astig
+                imgid0 = part(i)%imgid
+            end if
+            if (i < i0) cycle
+            k = i - i0 + 1
+            rec(:, k) = 0.0
+            rec(P_THETA, k) = part(i)%theta
+            rec(P_PHI, k) = part(i)%phi
+            rec(P_OMEGA, k) = part(i)%omega
+            rec(P_X, k) = part(i)%x
+            rec(P_Y, k) = part(i)%y
+            rec(P_PR, k) = part(i)%pr
+            rec(P_DF1, k) = d1
+            rec(P_DF2, k) = d2
+            rec(P_ASTIG, k) = ast
+            if (mask_first) then
+                rec(P_MASKX, k) = part(first)%x
+                rec(P_MASKY, k) = part(first)%y
+            else
+                rec(P_MASKX, k) = part(i)%x
+                rec(P_MASKY, k) = part(i)%y
+            end if
+            rec(P_USE, k) = 1.0
+        end do
+    end subroutine
+
+    subroutine run_reconstruct()
+        integer, parameter :: B = 256
+        real(c_float), allocatable :: imgs(:), rec(:, :), fa(:), fb(:), wg(:), map(:)
+        integer :: i0, i1, i, k, layout, nt, u, used
+        real :: bx
+        logical :: sub
+        sub = subrec == 'y' .or. subrec == 'Y'
+        bx = boundx
+        if (sub) bx = fftsize / 4.0                   ! reconstruct_substep.f90:22 overrides boundX
+        layout = merge(1, 0, sub)
+        p%norm_on = 1
+        p%mask_on = merge(1, 0, imgmask_on == 'y')
+        p%maxR = maxR
+        rc = cim_rec_begin(ctx, fftsize, maxR, layout, real(merge(maxR - 2, maxR, sub), c_float), &
+                           merge(0.0_c_float, 0.01_c_float, sub), 60, c_null_ptr)
+        call cim_check(ctx, rc, 'cim_rec_begin')
+        allocate(imgs(fftsize * fftsize * B), rec(CIM_PART_STRIDE, B))
+        used = 0
+        do i0 = first, last, B
+            i1 = min(last, i0 + B - 1)
+            call fill_records(i0, i1, rec, .not. sub, .true.)
+            do i = i0, i1
+                k = i - i0 + 1
+                call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))
+                if (part(i)%pr <= pr_threshold .and. abs(part(i)%x) <= bx .and. abs(part(i)%y) <= bx) then
+                    used = used + 1
+                else
+                    rec(P_USE, k) = 0.0
+                end if
+            end do
+            rc = cim_rec_insert(ctx, p, i1 - i0 + 1, imgs, rec)
+            call cim_check(ctx, rc, 'cim_rec_insert')
+        end do
+        print *, 'inserted particles:', used
+        if (sub) then
+            nt = (2 * maxR + 1)**2 * (maxR + 1)
+            allocate(fa(nt), fb(nt), wg(nt))
+            rc = cim_rec_get_tnf(ctx, fa, fb, wg)
+            call cim_check(ctx, rc, 'cim_rec_get_tnf')
+            open(newunit=u, file=trim(result3d), form='unformatted', access='stream', status='replace')
+            write(u) fftsize, maxR, apix, imgmask, 0, merge(1, 0, realspaceavg == 'y' .or. realspaceavg == 'Y')
+            write(u) fa
+            write(u) fb
+            write(u) wg
+            close(u)
+        else
+            allocate(map(int(fftsize, 8)**3))
+            rc = cim_rec_finish(ctx, map)
+            call cim_check(ctx, rc, 'cim_rec_finish')
+            call write_map(result3d, map)
+        end if
+    end subroutine
+
+    subroutine write_map(fname, map)
+        character(len=*), intent(in) :: fname
+        real(c_float), intent(in) :: map(:)
+        integer(c_int32_t) :: h(256)
+        integer :: u
+        h = 0
+        h(1:3) = fftsize
+        h(4) = 2
+        h(5:7) = -fftsize / 2
+        h(8:10) = fftsize - 1
+        h(11) = transfer(real((fftsize - 1) * apix, c_float), h(1))
+        h(12) = h(11)
+        h(13) = h(11)
+        h(14:16) = transfer(90.0_c_float, h(1))
+        h(17) = 1; h(18) = 2; h(19) = 3
+        h(50:52) = transfer(real(-fftsize, c_float), h(1))
+        h(53) = transfer('MAP ', h(1))
+        open(newunit=u, file=trim(fname), form='unformatted', access='stream', status='replace')
+        write(u) h
+        write(u) map
+        close(u)
+    end subroutine
+
+    subroutine run_search()
+        integer, parameter :: B = 256
+        real(c_float), allocatable :: imgs(:), rec(:, :), timgs(:), tort(:, :)
+        integer :: i0, i1, i, k, u, pid
+        real :: ox
+        ! template bank (load_template.f90:18-33)
+        open(newunit=u, file=trim(templateort), status='old')
+        T = 0
+        do
+            read(u, *, iostat=rc) pid, ox
+            if (rc /= 0) exit
+            T = T + 1
+        end do
+        rewind(u)
+        allocate(tort(5, T), timgs(fftsize * fftsize * T))
+        do i = 1, T
+            read(u, *) pid, tort(1, i), tort(2, i), tort(3, i), tort(4, i), tort(5, i)
+        end do
+        close(u)
+        open(newunit=u, file=trim(templatestck), form='unformatted', access='stream', status='old')
+        read(u, pos=1025) timgs
+        close(u)
+        p%mask_on = merge(1, 0, imgmask_on == 'y')
+        rc = cim_search_templates(ctx, p, T, timgs, tort)
+        call cim_check(ctx, rc, 'cim_search_templates')
+        allocate(imgs(fftsize * fftsize * B), rec(CIM_PART_STRIDE, B))
+        open(41, file=trim(newortfile))
+        do i0 = first, last, B
+            i1 = min(last, i0 + B - 1)
+            ! mode 4 caches the CTF per micrograph; mode 5 recomputes it per particle
+            call fill_records(i0, i1, rec, mode == 4, .false.)
+            do i = i0, i1
+                k = i - i0 + 1
+                call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))
+            end do
+            if (mode == 4) then
+                rc = cim_search_global(ctx, p, i1 - i0 + 1, imgs, rec)
+            else
+                rc = cim_refine_local(ctx, p, i1 - i0 + 1, imgs, rec)
+            end if
+            call cim_check(ctx, rc, 'search')
+            do i = i0, i1
+                k = i - i0 + 1
+                call write_record(part(i), rec(:, k), mode == 5)
+            end do
+        end do
+        close(41)
+    end subroutine
+
+    ! ort_globesearch.f90:109-139 / ort_refinement.f90:96-122
+    subroutine write_record(pt, r, centres)
+        type(particle_t), intent(in) :: pt
+        real(c_float), intent(in) :: r(:)
+        logical, intent(in) :: centres
+        real :: th, ph, om, x, y, pr
+        th = r(P_THETA); ph = r(P_PHI); om = r(P_OMEGA); pr = r(P_PR)
+        x = pt%x; y = pt%y
+        if (centres) then
+            x = r(P_X); y = r(P_Y)
+        end if
+        write(41, 900) pt%stck, pt%pid, th, ph, om, x, y, pr, pt%imgid, pt%imgpid, pt%df1, pt%df2, pt%astig, &
+                       th - pt%theta, ph - pt%phi, om - pt%omega, x - pt%x, y - pt%y, pr - pt%pr
+900     format(a54,i8,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,i12,i6,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4)
+    end subroutine
+
+end program
