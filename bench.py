@@ -91,10 +91,13 @@ def gpu_run(args, tang, timgs, ps):
     ctx.rec_begin(BOX, INSERT_MAXR, layout=0, rcut=INSERT_MAXR, wg_init=0.01)
     images = torch.from_numpy(ps.images).to("cuda:0")
     if args.calib_copy:
-        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        # 1 GiB copy after 1 GiB of unrelated writes, so the source is read from HBM
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        flush = torch.zeros(256 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)
         dst.copy_(src)
-        del src, dst
+        torch.cuda.synchronize()
+        del src, dst, flush
     torch.cuda.synchronize()
     P = np.zeros((n, lib.PART_STRIDE), np.float32)
     P[:, 3:5] = ps.shifts
@@ -261,6 +264,11 @@ def main():
         except Exception as e:
             cpu = {"value": None, "unit": "particles/s", "cores": 0, "kind": "port", "sample": f"failed: {e}"}
     d = r["d"]
+    traffic = None
+    tj = os.path.join(ROOT, "profiles", "hbm_traffic_box384.json")
+    if os.path.exists(tj):   # rocprofv3 FETCH_SIZE/WRITE_SIZE passes, copy-calibrated (per particle)
+        with open(tj) as f:
+            traffic = json.load(f)["k_cl_search"]["hbm_bytes_per_particle"] * n
     line = {
         "metric": "particles/s search+insert, box 384: chosen vs plain FP32 GPU at FSC>=0.999",
         "value": round(value, 3),
@@ -281,7 +289,8 @@ def main():
                    "insert_maxR": INSERT_MAXR, "searchstep": STEP, "apix": APIX},
         "roofline": {"bound": "mfma", "kernel": "k_cl_search (common-lines phase residual, FP32 VALU)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None if traffic is None else round(traffic),
                      "samples_per_launch": int(samples_per_launch), "flops_per_sample": FLOPS_PER_SAMPLE,
                      "avg_launch_ms": round(r["k_ms"], 3), "tuned_shape": r["shape"]},
         "cpu_baseline": cpu,
