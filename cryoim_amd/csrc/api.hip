@@ -114,6 +114,9 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->rec.acc.release();
     ctx->rec.ops.release();
     ctx->rec.samples.release();
+    ctx->rec.bricks.release();
+    ctx->rec.hxmax.release();
+    ctx->rec.vals.release();
     ctx->srch.tft.release();
     ctx->srch.tfcut.release();
     ctx->srch.wk.release();

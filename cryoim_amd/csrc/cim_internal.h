@@ -53,6 +53,10 @@ struct RecState {
     DevBuf ops;                     // double[nops*9]
     DevBuf samples;                 // int2 (hx, ky) per half-plane sample
     int nsamples = 0;
+    DevBuf hxmax;                   // largest inserted hx per ky (-1: none)
+    DevBuf bricks;                  // int3 origins of the 8^3 bricks a sample can reach
+    int nbricks = 0;
+    DevBuf vals;                    // per-batch float4 sample values on the half-plane grid
     long long inserted = 0;
 };
 

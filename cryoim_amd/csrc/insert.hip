@@ -9,6 +9,7 @@
 // is both the full-box variant restricted to where it is ever written and a
 // superset of the compact tnf block.
 #include <cmath>
+#include <cstring>
 
 #include "cim_device.h"
 
@@ -41,6 +42,31 @@ struct InsertGeom {
     float offz;    // added to z (box/2 full, 0 compact)
 };
 
+// Position of sample (hx, ky) under operator matrix m (float32, the
+// reference's S * R * (hx, ky, 0)), folded into z >= 0 (reconstruct.f90:117-122),
+// as the accumulator index of its lower corner and the trilinear fractions.
+// Both insert kernels use this, so they place every contribution identically.
+__device__ __forceinline__ void place(const float* m, float fh, float fk, const InsertGeom& g, int& gx, int& gy,
+                                      int& gz, float& ax, float& ay, float& az, bool& flip) {
+    float x = m[0] * fh + m[1] * fk;
+    float y = m[3] * fh + m[4] * fk;
+    float z = m[6] * fh + m[7] * fk;
+    flip = z < 0.f;
+    if (flip) {
+        x = -x; y = -y; z = -z;
+    }
+    x += g.off;
+    y += g.off;
+    z += g.offz;
+    const int lx = (int)x, ly = (int)y, lz = (int)z;
+    ax = x - (float)lx;
+    ay = y - (float)ly;
+    az = z - (float)lz;
+    gx = lx - (int)g.off + g.maxR;
+    gy = ly - (int)g.off + g.maxR;
+    gz = lz - (int)g.offz;
+}
+
 // One thread per (half-plane sample, particle); loops over the operators.
 // ft: centred, centre-shifted transforms.  ctf: either computed per sample
 // from the particle record (ctf_img == nullptr && apply_ctf) or read from
@@ -69,20 +95,11 @@ __global__ void __launch_bounds__(256) k_insert(float4* __restrict__ acc, const 
     const long long dxy = g.dxy, dxy2 = dxy * dxy;
     for (int o = 0; o < g.nops; ++o) {
         const float* m = Mp + 9 * o;
-        float x = m[0] * fh + m[1] * fk;
-        float y = m[3] * fh + m[4] * fk;
-        float z = m[6] * fh + m[7] * fk;
-        float sb = vb;
-        if (z < 0.f) {
-            x = -x; y = -y; z = -z;
-            sb = -vb;
-        }
-        x += g.off;
-        y += g.off;
-        z += g.offz;
-        const int lx = (int)x, ly = (int)y, lz = (int)z;
-        const float ax = x - (float)lx, ay = y - (float)ly, az = z - (float)lz;
-        const int gx = lx - (int)g.off + g.maxR, gy = ly - (int)g.off + g.maxR, gz = lz - (int)g.offz;
+        int gx, gy, gz;
+        float ax, ay, az;
+        bool flip;
+        place(m, fh, fk, g, gx, gy, gz, ax, ay, az, flip);
+        const float sb = flip ? -vb : vb;
         float4* b = acc + ((long long)gz * dxy2 + (long long)gy * dxy + gx);
         const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay, w11 = ax * ay;
         const float wz0 = 1.f - az, wz1 = az;
@@ -176,6 +193,154 @@ __global__ void k_tnf_io(float4* __restrict__ acc, float* __restrict__ fa, float
     }
 }
 
+// ---- brick-gather insert ----------------------------------------------------
+//
+// The accumulator is cut into 8x8x8 bricks; one workgroup owns one brick in
+// LDS and gathers every (particle, operator) plane of the batch that passes
+// within reach of it: for each such plane it enumerates the half-plane
+// samples whose folded position can touch the brick (the plane coordinates
+// of the brick's box, widened by one sample), places them with the same
+// place() as the scatter kernel, and adds the corners that fall inside the
+// brick with LDS atomics.  The brick is then added to HBM once, so the
+// insert needs no global atomics at all.
+
+constexpr int BRICK = 8;
+
+// Per-particle sample values {a*ctf, b*ctf, ctf^2} on the (hx, ky) half
+// plane grid hx in [0, maxR], ky in [-maxR, maxR].
+// A sample is inserted iff hx <= hxmax[ky + maxR] (the host's radius test).
+__global__ void k_insert_vals(const float2* __restrict__ ft, int box, int maxR, const int* __restrict__ hxmax,
+                              const float* __restrict__ parts, const float* __restrict__ ctf_img, int apply_ctf,
+                              CtfConst cc, float4* __restrict__ vals) {
+    const int p = blockIdx.y;
+    const int w = maxR + 1, total = (2 * maxR + 1) * w;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const int hx = q % w, ky = q / w - maxR;
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (hx <= hxmax[ky + maxR]) {
+        const long long N = (long long)box * box;
+        const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
+        const float2 v = ft[(long long)p * N + i2];
+        float c = 1.f;
+        if (ctf_img) c = ctf_img[(long long)p * N + i2];
+        else if (apply_ctf) {
+            const float* pr = parts + (long long)p * CIM_PART_STRIDE;
+            c = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+        }
+        out = make_float4(v.x * c, v.y * c, c * c, 1.f);
+    }
+    vals[(long long)p * total + q] = out;
+}
+
+struct BrickArgs {
+    float4* acc;
+    const float4* vals;       // n x (2maxR+1)(maxR+1); w = 1 where the sample is inserted
+    const float* M;           // n x nops x 9
+    const int* bricks;        // nb x 3 brick origins (accumulator indices)
+    int n, dz;
+    InsertGeom g;
+};
+
+__global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
+    __shared__ float sfa[BRICK * BRICK * BRICK], sfb[BRICK * BRICK * BRICK], swg[BRICK * BRICK * BRICK];
+    __shared__ int list[1024];
+    __shared__ int nlist;
+    const int tid = threadIdx.x;
+    const int bx0 = a.bricks[3 * blockIdx.x], by0 = a.bricks[3 * blockIdx.x + 1], bz0 = a.bricks[3 * blockIdx.x + 2];
+    const InsertGeom& g = a.g;
+    const int R = g.maxR;
+    for (int i = tid; i < BRICK * BRICK * BRICK; i += blockDim.x) { sfa[i] = 0.f; sfb[i] = 0.f; swg[i] = 0.f; }
+    // positions (before the index offsets) whose corners can reach the brick:
+    // index gx covers x in [gx - R - 1, gx - R + 1), likewise y; z in [gz - 1, gz + 1)
+    const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - 1)};
+    const float hi[3] = {(float)(bx0 - R + BRICK), (float)(by0 - R + BRICK), (float)(bz0 + BRICK)};
+    const float c[3] = {0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
+    const float reach = 0.5f * (float)(BRICK + 1) * 1.7320508f + 0.5f;
+    const int w = R + 1;
+    const int ntot = a.n * g.nops;
+    const float fR = (float)R;
+    for (int q0 = 0; q0 < ntot; q0 += 1024) {
+        if (tid == 0) nlist = 0;
+        __syncthreads();
+        for (int q = q0 + tid; q < min(ntot, q0 + 1024); q += blockDim.x) {
+            const float* m = a.M + (long long)q * 9;
+            const float d = fabsf(m[2] * c[0] + m[5] * c[1] + m[8] * c[2]);   // plane normal = third column
+            if (d <= reach) list[atomicAdd(&nlist, 1)] = q;
+        }
+        __syncthreads();
+        const int nl = nlist;
+        for (int li = 0; li < nl; ++li) {
+            const int q = list[li];
+            const int p = q / g.nops;
+            const float* m = a.M + (long long)q * 9;
+            // plane coordinates of the box corners: u = col0 . X, v = col1 . X
+            float umin = 3e38f, umax = -3e38f, vmin = 3e38f, vmax = -3e38f;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float X = (k & 1) ? hi[0] : lo[0], Y = (k & 2) ? hi[1] : lo[1], Z = (k & 4) ? hi[2] : lo[2];
+                const float u = m[0] * X + m[3] * Y + m[6] * Z, v = m[1] * X + m[4] * Y + m[7] * Z;
+                umin = fminf(umin, u); umax = fmaxf(umax, u); vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+            }
+            // samples hitting the box directly, and (rect negated) after the z < 0 fold
+            const int u0 = (int)floorf(umin) - 1, u1 = (int)ceilf(umax) + 1;
+            const int v0 = (int)floorf(vmin) - 1, v1 = (int)ceilf(vmax) + 1;
+            if (u1 < -R && -u0 < -R) continue;
+            const int nu = u1 - u0 + 1, nv = v1 - v0 + 1;
+            const int nrect = nu * nv;
+            const float4* pv = a.vals + (long long)p * (2 * R + 1) * w;
+            for (int t = tid; t < 2 * nrect; t += blockDim.x) {
+                int hx, ky;
+                if (t < nrect) {
+                    hx = u0 + t % nu;
+                    ky = v0 + t / nu;
+                } else {
+                    hx = -(u0 + (t - nrect) % nu);
+                    ky = -(v0 + (t - nrect) / nu);
+                    if (hx >= u0 && hx <= u1 && ky >= v0 && ky <= v1) continue;   // already in the direct rect
+                }
+                if (hx < 0 || hx > R || ky < -R || ky > R) continue;
+                const float4 val = pv[(ky + R) * w + hx];
+                if (val.w == 0.f) continue;
+                int gx, gy, gz;
+                float ax, ay, az;
+                bool flip;
+                place(m, (float)hx, (float)ky, g, gx, gy, gz, ax, ay, az, flip);
+                const int ox = gx - bx0, oy = gy - by0, oz = gz - bz0;
+                if (ox < -1 || ox >= BRICK || oy < -1 || oy >= BRICK || oz < -1 || oz >= BRICK) continue;
+                const float va = val.x, vb = flip ? -val.y : val.y, vw = val.z;
+                const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay, w11 = ax * ay;
+                const float wz0 = 1.f - az, wz1 = az;
+                const float wt[8] = {w00 * wz0, w10 * wz0, w01 * wz0, w11 * wz0, w00 * wz1, w10 * wz1, w01 * wz1, w11 * wz1};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int cx = ox + (j & 1), cy = oy + ((j >> 1) & 1), cz = oz + (j >> 2);
+                    if (cx < 0 || cx >= BRICK || cy < 0 || cy >= BRICK || cz < 0 || cz >= BRICK) continue;
+                    const int li3 = (cz * BRICK + cy) * BRICK + cx;
+                    atomicAdd(&sfa[li3], va * wt[j]);
+                    atomicAdd(&sfb[li3], vb * wt[j]);
+                    atomicAdd(&swg[li3], vw * wt[j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // one read-modify-write of the brick in HBM
+    const long long dxy = g.dxy;
+    for (int i = tid; i < BRICK * BRICK * BRICK; i += blockDim.x) {
+        const int cx = i % BRICK, cy = (i / BRICK) % BRICK, cz = i / (BRICK * BRICK);
+        const int X = bx0 + cx, Y = by0 + cy, Z = bz0 + cz;
+        if (X >= g.dxy || Y >= g.dxy || Z >= a.dz) continue;
+        if (sfa[i] == 0.f && sfb[i] == 0.f && swg[i] == 0.f) continue;
+        float4* o = a.acc + ((long long)Z * dxy * dxy + (long long)Y * dxy + X);
+        float4 v = *o;
+        v.x += sfa[i];
+        v.y += sfb[i];
+        v.z += swg[i];
+        *o = v;
+    }
+}
+
 // ---- host side -----------------------------------------------------------
 
 int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init, int nops, const double* ops) {
@@ -213,12 +378,39 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
     CIM_HIP(hipMemcpyAsync(r.ops.ptr, hops.data(), sizeof(double) * 9 * nops, hipMemcpyHostToDevice, ctx->stream));
     // half-plane sample list in the loop order of reconstruct.f90:103-106
     std::vector<int2> smp;
+    std::vector<int> hxmax(2 * maxR + 1, -1);
     for (int ky = -maxR; ky <= maxR; ++ky)
         for (int hx = 0; hx <= maxR; ++hx) {
             float dr = std::sqrt((float)(hx * hx + ky * ky));
-            if (dr <= rcut) smp.push_back(make_int2(hx, ky));
+            if (dr <= rcut) {
+                smp.push_back(make_int2(hx, ky));
+                hxmax[ky + maxR] = hx;
+            }
         }
     r.nsamples = (int)smp.size();
+    CIM_ALLOC(r.hxmax, sizeof(int) * hxmax.size());
+    CIM_HIP(hipMemcpyAsync(r.hxmax.ptr, hxmax.data(), sizeof(int) * hxmax.size(), hipMemcpyHostToDevice, ctx->stream));
+    // bricks of the accumulator that a sample of radius <= rcut can reach
+    std::vector<int> bl;
+    for (int z = 0; z < r.dz; z += BRICK)
+        for (int y = 0; y < r.dxy; y += BRICK)
+            for (int x = 0; x < r.dxy; x += BRICK) {
+                // nearest point of the brick (relative coordinates) to the origin
+                auto nearest = [&](int a0, int n, int shift) {
+                    const int lo = a0 - shift - 1, hi = a0 - shift + n;
+                    return lo > 0 ? lo : (hi < 0 ? hi : 0);
+                };
+                const double nx = nearest(x, BRICK, r.maxR), ny = nearest(y, BRICK, r.maxR), nz = nearest(z, BRICK, 0);
+                if (std::sqrt(nx * nx + ny * ny + nz * nz) <= rcut + 2.0) {
+                    bl.push_back(x);
+                    bl.push_back(y);
+                    bl.push_back(z);
+                }
+            }
+    r.nbricks = (int)(bl.size() / 3);
+    CIM_ALLOC(r.bricks, sizeof(int) * bl.size());
+    CIM_HIP(hipMemcpyAsync(r.bricks.ptr, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
     CIM_ALLOC(r.samples, sizeof(int2) * smp.size());
     CIM_HIP(hipMemcpyAsync(r.samples.ptr, smp.data(), sizeof(int2) * smp.size(), hipMemcpyHostToDevice, ctx->stream));
     CIM_HIP(hipStreamSynchronize(ctx->stream));
@@ -248,11 +440,31 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
     g.offz = r.layout == 0 ? (float)(r.box / 2) : 0.f;
     CtfConst cc{};
     if (p) cc = make_ctf_const(p);
-    TimedLaunch tl(ctx, "insert");
-    hipLaunchKernelGGL(k_insert, dim3((r.nsamples + 255) / 256, n), dim3(256), 0, ctx->stream, r.acc.as<float4>(), ft,
-                       r.samples.as<int2>(), r.nsamples, ctx->aux2.as<float>(), parts_dev, ctf_img_dev, apply_ctf, cc,
-                       g);
-    CIM_HIP(hipGetLastError());
+    const char* mode = getenv("CIM_INSERT");
+    if (mode && strcmp(mode, "scatter") == 0) {
+        TimedLaunch tl(ctx, "insert");
+        hipLaunchKernelGGL(k_insert, dim3((r.nsamples + 255) / 256, n), dim3(256), 0, ctx->stream, r.acc.as<float4>(),
+                           ft, r.samples.as<int2>(), r.nsamples, ctx->aux2.as<float>(), parts_dev, ctf_img_dev,
+                           apply_ctf, cc, g);
+        CIM_HIP(hipGetLastError());
+    } else {
+        const int total = (2 * r.maxR + 1) * (r.maxR + 1);
+        CIM_ALLOC(r.vals, sizeof(float4) * (size_t)total * n);
+        hipLaunchKernelGGL(k_insert_vals, dim3((total + 255) / 256, n), dim3(256), 0, ctx->stream, ft, r.box, r.maxR,
+                           r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, r.vals.as<float4>());
+        CIM_HIP(hipGetLastError());
+        BrickArgs ba;
+        ba.acc = r.acc.as<float4>();
+        ba.vals = r.vals.as<float4>();
+        ba.M = ctx->aux2.as<float>();
+        ba.bricks = r.bricks.as<int>();
+        ba.n = n;
+        ba.dz = r.dz;
+        ba.g = g;
+        TimedLaunch tl(ctx, "insert");
+        hipLaunchKernelGGL(k_insert_brick, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
+        CIM_HIP(hipGetLastError());
+    }
     r.inserted += n;
     ctx->counters["insert_samples"] += (long long)n * r.nsamples;
     ctx->counters["insert_particles"] += n;

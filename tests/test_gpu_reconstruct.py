@@ -1,6 +1,10 @@
 """HIP direct Fourier inversion against the FP64 oracle of
 icosprocess/SRC/reconstruct.f90 (full box, mode 1) and
-reconstruct_substep.f90 (compact tnf accumulator)."""
+reconstruct_substep.f90 (compact tnf accumulator).  Every test runs with
+both insert kernels: the brick gather (default) and the atomic scatter
+(CIM_INSERT=scatter)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -8,6 +12,17 @@ from conftest import rel_l2
 from cryoim_amd import lib, synth
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True, params=["brick", "scatter"])
+def insert_kernel(request):
+    old = os.environ.get("CIM_INSERT")
+    os.environ["CIM_INSERT"] = request.param
+    yield request.param
+    if old is None:
+        os.environ.pop("CIM_INSERT", None)
+    else:
+        os.environ["CIM_INSERT"] = old
 
 
 def _data(box=64, n=24, seed=11, ctf=True):
@@ -108,3 +123,38 @@ def test_unused_particles_are_skipped(ctx):
     ctx.rec_insert(p, ps.images, P)
     fa, fb, wg = ctx.rec_get_tnf(maxR)
     assert not fa.any() and not wg.any()
+
+
+@pytest.mark.parametrize("nops,n", [(1, 3), (7, 40), (60, 19)])
+def test_brick_gather_equals_scatter(ctx, insert_kernel, nops, n):
+    """The two insert kernels place every contribution identically; only the
+    float summation order differs.  Random operators, n*nops across the
+    gather's 1024-plane chunks, both layouts."""
+    if insert_kernel != "brick":
+        pytest.skip("compares the two kernels once")
+    box, maxR = 80, 33
+    rng = np.random.default_rng(nops)
+    re = rng.standard_normal((n, box, box)).astype(np.float32)
+    im = rng.standard_normal((n, box, box)).astype(np.float32)
+    ctf = rng.uniform(-1, 1, (n, box, box)).astype(np.float32)
+    eul = np.stack([rng.uniform(0, 180, n), rng.uniform(0, 360, n), rng.uniform(0, 360, n)], 1).astype(np.float32)
+    q = np.linalg.qr(rng.standard_normal((nops, 3, 3)))[0]
+    q *= np.sign(np.linalg.det(q))[:, None, None]
+    ops = None if nops == 60 else q
+    res = {}
+    for layout, rcut in ((0, float(maxR)), (1, maxR - 2.0)):
+        for kern in ("brick", "scatter"):
+            os.environ["CIM_INSERT"] = kern
+            ctx.rec_begin(box, maxR, layout=layout, rcut=rcut, wg_init=0.0, ops=ops)
+            ctx.rec_insert_ft(re, im, eul, ctf)
+            ctx.rec_insert_ft(re[: n // 2 + 1], im[: n // 2 + 1], eul[: n // 2 + 1], ctf[: n // 2 + 1])
+            res[kern] = ctx.rec_get_tnf(maxR)
+        # float32 sums of up to ~1e3 terms per voxel in different orders
+        for a, b in zip(res["brick"], res["scatter"]):
+            scale = float(np.abs(b).max())
+            err_max = float(np.abs(a - b).max())
+            err_l2 = rel_l2(a, b)
+            print(f"nops {nops} layout {layout}: rel-L2 {err_l2:.2e}, max {err_max / scale:.2e} of max")
+            assert err_l2 < 1e-5
+            assert err_max <= 1e-4 * scale
+            assert float(((a != 0) == (b != 0)).mean()) > 0.999
