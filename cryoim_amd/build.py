@@ -100,8 +100,14 @@ def build_fortran() -> bool:
     os.makedirs(bindir, exist_ok=True)
     modd = os.path.join(OBJDIR, "fmod")
     os.makedirs(modd, exist_ok=True)
-    _run([fc, "-O2", "-J", modd, "-o", os.path.join(bindir, "icosprocess_hip"), *srcs, f"-L{LIBDIR}",
-          "-lcryoim_hip", f"-Wl,-rpath,{LIBDIR}", f"-Wl,-rpath,{ROCM}/lib", f"-L{ROCM}/lib", "-lamdhip64"])
+    link = [f"-L{LIBDIR}", "-lcryoim_hip", f"-Wl,-rpath,{LIBDIR}", f"-Wl,-rpath,{ROCM}/lib", f"-L{ROCM}/lib",
+            "-lamdhip64"]
+    iface_o = os.path.join(modd, "cim_iface.o")
+    _run([fc, "-O2", "-J", modd, "-c", srcs[0], "-o", iface_o])
+    for prog in ("icosprocess_hip", "merge_icosprocess_subrec_hip"):
+        src = os.path.join(fdir, prog + ".f90")
+        if os.path.exists(src):
+            _run([fc, "-O2", "-J", modd, "-o", os.path.join(bindir, prog), src, iface_o, *link])
     return True
 
 

@@ -121,7 +121,9 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->srch.tfcut.release();
     ctx->srch.wk.release();
     ctx->lref.release();
-    for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->aux, &ctx->aux2, &ctx->parts, &ctx->out}) b->release();
+    for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->aux, &ctx->aux2, &ctx->parts, &ctx->out, &ctx->fsc_in,
+                      &ctx->fsc_nrm, &ctx->fsc_acc, &ctx->fsc_sums, &ctx->fsc_wt})
+        b->release();
     resolve_timing(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

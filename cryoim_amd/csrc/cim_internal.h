@@ -107,6 +107,8 @@ struct cim_ctx {
     cim::LrefState lref;
     // scratch buffers reused across calls
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
+    // half-map FSC: input tnf pairs, normalised copies, merged sums, shell sums, weights
+    cim::DevBuf fsc_in, fsc_nrm, fsc_acc, fsc_sums, fsc_wt;
     // kernel timing: event pairs recorded around launches, resolved lazily
     bool timing = false;
     std::map<std::string, cim::KTimer> timers;
@@ -157,6 +159,10 @@ int fft_plan(cim_ctx* ctx, int n, FftPlan** out);
 // In-place transforms of interleaved complex data on ctx->stream.
 int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_out);
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
+// Fold {fa, fb, wg} (pitch dxy, z >= 0) -> complex volume -> inverse FFT ->
+// real map with checkerboard (inside radius rmask only when rmask >= 0).
+int fold_to_map_dev(cim_ctx* ctx, const float4* acc, int dxy, int box, int maxR, int int_radius, float isg,
+                    float rmask, float* map_dev);
 
 // split <-> interleaved conversion (prep.hip)
 int interleave(cim_ctx* ctx, const float* re, const float* im, float2* out, long long count);

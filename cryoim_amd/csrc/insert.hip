@@ -8,6 +8,7 @@
 // z in [0, maxR+1] (every corner a sample of radius <= maxR can reach), which
 // is both the full-box variant restricted to where it is ever written and a
 // superset of the compact tnf block.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -121,8 +122,13 @@ __global__ void k_fill_acc(float4* __restrict__ acc, long long count, float wg0)
 }
 
 // Fold + normalise + Hermitian completion, written as the complex input of
-// the backward FFT: cmplx(fa, -fb) (reconstruct.f90:241).
-__global__ void k_fold(const float4* __restrict__ acc, float2* __restrict__ vol, int box, int maxR, int dxy) {
+// the backward FFT.  isg = -1 writes cmplx(fa, -fb) (reconstruct.f90:241,
+// merge_reconstruct.f90:133), isg = +1 cmplx(fa, fb) (reconstruct_FSC.f90:173).
+// The sphere test is dr <= maxR (reconstruct.f90) or, with int_radius,
+// int(dr) <= maxR (reconstruct_FSC.f90:101, merge_reconstruct.f90:42).
+// acc holds {fa, fb, wg} for x, y in [-maxR, .] (row pitch dxy) and z >= 0.
+__global__ void k_fold(const float4* __restrict__ acc, float2* __restrict__ vol, int box, int maxR, int dxy,
+                       int int_radius, float isg) {
     const long long n2 = (long long)box * box;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n2 * box) return;
@@ -132,7 +138,7 @@ __global__ void k_fold(const float4* __restrict__ acc, float2* __restrict__ vol,
     float2 outv = make_float2(0.f, 0.f);
     if (abs(h) <= maxR && abs(k) <= maxR && abs(l) <= maxR) {
         const float dr = sqrtf((float)(h * h + k * k + l * l));
-        if (dr <= (float)maxR) {
+        if (int_radius ? ((int)dr <= maxR) : (dr <= (float)maxR)) {
             const long long dxy2 = (long long)dxy * dxy;
             auto at = [&](int hh, int kk, int ll) -> float4 {
                 return acc[(long long)ll * dxy2 + (long long)(kk + maxR) * dxy + (hh + maxR)];
@@ -143,7 +149,7 @@ __global__ void k_fold(const float4* __restrict__ acc, float2* __restrict__ vol,
                 const int sg = l > 0 ? 1 : -1;
                 float4 a = at(sg * h, sg * k, sg * l);
                 const double A = (double)a.x / (double)a.z, B = (double)a.y / (double)a.z;
-                outv = make_float2((float)A * s, (l > 0 ? -(float)B : (float)B) * s);
+                outv = make_float2((float)A * s, (l > 0 ? isg : -isg) * (float)B * s);
             } else if (h == 0 && k == 0) {
                 float4 a = at(0, 0, 0);
                 outv = make_float2((float)((double)a.x / (double)a.z), 0.f);
@@ -155,7 +161,7 @@ __global__ void k_fold(const float4* __restrict__ acc, float2* __restrict__ vol,
                 const double W = (double)a1.z + (double)a2.z;
                 const double A = ((double)a1.x + (double)a2.x) / W;
                 const double B = ((double)a1.y - (double)a2.y) / W;
-                outv = make_float2((float)A * s, (is_n1 ? -(float)B : (float)B) * s);
+                outv = make_float2((float)A * s, (is_n1 ? isg : -isg) * (float)B * s);
             }
         }
     }
@@ -171,26 +177,71 @@ __global__ void k_checker_real(const float2* __restrict__ vol, float* __restrict
     map[id] = ((X + Y + Z) & 1) ? -v : v;
 }
 
+// Real part with the checkerboard inside the sphere |x - box/2| <= rmask,
+// zero outside (merge_reconstruct_FSC.f90:188-197).
+__global__ void k_checker_mask(const float2* __restrict__ vol, float* __restrict__ map, int box, float rmask) {
+    const long long n2 = (long long)box * box;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n2 * box) return;
+    const int Z = (int)(id / n2), Y = (int)((id / box) % box), X = (int)(id % box);
+    const int c = box / 2;
+    const float d = sqrtf((float)((X - c) * (X - c) + (Y - c) * (Y - c) + (Z - c) * (Z - c)));
+    const float v = vol[id].x;
+    map[id] = d <= rmask ? (((X + Y + Z) & 1) ? -v : v) : 0.f;
+}
+
+// The (box-8)^3 output block of merge_reconstruct.f90:143-149, whose plane
+// index is i/(box-8)^2 + 4 with the 1-based i: the last voxel of each output
+// plane is read from the next plane.
+__global__ void k_merge_crop(const float* __restrict__ map, float* __restrict__ out, int box) {
+    const long long M = box - 8, M2 = M * M;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // = i - 1
+    if (k >= M2 * M) return;
+    const long long lz = (k + 1) / M2 + 4, ky = (k % M2) / M + 4, hx = (k % M2) % M + 4;
+    out[k] = map[lz * box * box + ky * box + hx];
+}
+
 // Copy between the accumulator and tnf order ((2R+1)^2 (R+1), lz slowest).
+// For the compact layout the readout reproduces the reference's flat
+// indexing of its (2R+1)^2 (R+1) array (reconstruct_substep.f90:133-136,
+// reconstruct_half.f90:127-130): a corner at x = 2R+1 lands on x = 0 of the
+// next row, one at y = 2R+1 on y = 0 of the next plane, and anything past
+// the last plane is lost.  The accumulator keeps those corners in its extra
+// x = 2R+1 / y = 2R+1 cells, so tnf cell f gathers every accumulator cell
+// whose flat reference index is f (their weights minus the initial value).
 __global__ void k_tnf_io(float4* __restrict__ acc, float* __restrict__ fa, float* __restrict__ fb,
-                         float* __restrict__ wg, int maxR, int dxy, int add_into_acc) {
+                         float* __restrict__ wg, int maxR, int dxy, int add_into_acc, int wrap, float wg_init) {
     const int t = 2 * maxR + 1;
     const long long total = (long long)t * t * (maxR + 1);
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= total) return;
     const int lz = (int)(id / ((long long)t * t));
     const int ky = (int)((id / t) % t), hx = (int)(id % t);
-    float4* a = acc + ((long long)lz * dxy * dxy + (long long)ky * dxy + hx);
+    const long long dxy2 = (long long)dxy * dxy;
+    float4* a = acc + ((long long)lz * dxy2 + (long long)ky * dxy + hx);
     if (add_into_acc) {
         a->x += fa[id];
         a->y += fb[id];
         a->z += wg[id];
-    } else {
-        float4 v = *a;
-        fa[id] = v.x;
-        fb[id] = v.y;
-        wg[id] = v.z;
+        return;
     }
+    float4 v = *a;
+    if (wrap) {
+        // cells (x', y', z') with x' <= t, y' <= t and z' t^2 + y' t + x' = id
+        auto add = [&](int x, int y, int z) {
+            const float4 e = acc[(long long)z * dxy2 + (long long)y * dxy + x];
+            v.x += e.x;
+            v.y += e.y;
+            v.z += e.z - wg_init;
+        };
+        if (hx == 0 && ky >= 1) add(t, ky - 1, lz);
+        if (hx == 0 && ky == 0 && lz >= 1) add(t, t - 1, lz - 1);
+        if (ky == 0 && lz >= 1) add(hx, t, lz - 1);
+        if (hx == 0 && ky == 1 && lz >= 1) add(t, t, lz - 1);
+    }
+    fa[id] = v.x;
+    fb[id] = v.y;
+    wg[id] = v.z;
 }
 
 // ---- brick-gather insert ----------------------------------------------------
@@ -348,8 +399,6 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
     if (maxR < 1 || maxR > box / 2 - 2)
         return set_error(ctx, CIM_E_ARG, "maxR %d outside [1, box/2-2] for box %d", maxR, box);
     if (layout != 0 && layout != 1) return set_error(ctx, CIM_E_ARG, "layout must be 0 (full) or 1 (compact)");
-    if (layout == 1 && rcut > (float)(maxR - 2))
-        return set_error(ctx, CIM_E_ARG, "compact layout needs rcut <= maxR-2 (reconstruct_substep.f90:104)");
     if (rcut > (float)maxR) return set_error(ctx, CIM_E_ARG, "rcut %.2f > maxR %d", rcut, maxR);
     if (nops < 1 || nops > 360) return set_error(ctx, CIM_E_ARG, "nops %d outside [1, 360]", nops);
     RecState& r = ctx->rec;
@@ -370,6 +419,21 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
     std::vector<double> hops(9 * nops);
     if (ops) {
         for (int i = 0; i < 9 * nops; ++i) hops[i] = ops[i];
+        // every operator must be a rotation: the accumulator is sized for |S x| = |x| <= maxR
+        for (int o = 0; o < nops; ++o) {
+            const double* S = hops.data() + 9 * o;
+            double dev = 0.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double d = S[i] * S[j] + S[3 + i] * S[3 + j] + S[6 + i] * S[6 + j] - (i == j ? 1.0 : 0.0);
+                    dev = std::max(dev, std::fabs(d));
+                }
+            const double det = S[0] * (S[4] * S[8] - S[5] * S[7]) - S[1] * (S[3] * S[8] - S[5] * S[6]) +
+                               S[2] * (S[3] * S[7] - S[4] * S[6]);
+            if (!(dev < 1e-5) || !(det > 0.0))
+                return set_error(ctx, CIM_E_ARG, "symmetry operator %d is not a proper rotation (|S^T S - I| = %.3g)",
+                                 o, dev);
+        }
     } else {
         if (nops != 60) return set_error(ctx, CIM_E_ARG, "ops == NULL requires nops == 60 (icosahedral)");
         cim_icos60_ref_order(hops.data());
@@ -484,7 +548,7 @@ int rec_tnf_io(cim_ctx* ctx, float* fa, float* fb, float* wg, int add) {
         CIM_HIP(hipMemcpyAsync(d + 2 * total, wg, sizeof(float) * total, hipMemcpyHostToDevice, ctx->stream));
     }
     hipLaunchKernelGGL(k_tnf_io, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, r.acc.as<float4>(), d,
-                       d + total, d + 2 * total, r.maxR, r.dxy, add);
+                       d + total, d + 2 * total, r.maxR, r.dxy, add, r.layout == 1 ? 1 : 0, r.wg_init);
     CIM_HIP(hipGetLastError());
     if (!add) {
         CIM_HIP(hipMemcpyAsync(fa, d, sizeof(float) * total, hipMemcpyDeviceToHost, ctx->stream));
@@ -495,24 +559,50 @@ int rec_tnf_io(cim_ctx* ctx, float* fa, float* fb, float* wg, int add) {
     return CIM_OK;
 }
 
-int rec_finish_dev(cim_ctx* ctx, float* map_dev) {
-    RecState& r = ctx->rec;
-    if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
-    const long long nvox = (long long)r.box * r.box * r.box;
+// Fold {fa, fb, wg} sums (row pitch dxy, z >= 0) into the full complex
+// volume, inverse 3D FFT, real part with the checkerboard (everywhere, or
+// only inside radius rmask when rmask >= 0, zero outside).
+int fold_to_map_dev(cim_ctx* ctx, const float4* acc, int dxy, int box, int maxR, int int_radius, float isg,
+                    float rmask, float* map_dev) {
+    const long long nvox = (long long)box * box * box;
+    const unsigned nb = (unsigned)((nvox + 255) / 256);
     CIM_ALLOC(ctx->ft2, sizeof(float2) * nvox);
     float2* vol = ctx->ft2.as<float2>();
     {
         TimedLaunch tl(ctx, "fold");
-        hipLaunchKernelGGL(k_fold, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream,
-                           r.acc.as<float4>(), vol, r.box, r.maxR, r.dxy);
+        hipLaunchKernelGGL(k_fold, dim3(nb), dim3(256), 0, ctx->stream, acc, vol, box, maxR, dxy, int_radius, isg);
         CIM_HIP(hipGetLastError());
     }
-    int rc = fft3d_dev(ctx, vol, r.box, -1, 0);
+    int rc = fft3d_dev(ctx, vol, box, -1, 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_checker_real, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream, vol, map_dev,
-                       r.box);
+    if (rmask < 0.f)
+        hipLaunchKernelGGL(k_checker_real, dim3(nb), dim3(256), 0, ctx->stream, vol, map_dev, box);
+    else
+        hipLaunchKernelGGL(k_checker_mask, dim3(nb), dim3(256), 0, ctx->stream, vol, map_dev, box, rmask);
     CIM_HIP(hipGetLastError());
     return CIM_OK;
+}
+
+// merge_reconstruct.f90:37-149 on the accumulated sums: integer-radius fold,
+// inverse FFT, checkerboard, (box-8)^3 crop.  out_dev holds (box-8)^3 floats.
+int rec_merge_dev(cim_ctx* ctx, float* out_dev) {
+    RecState& r = ctx->rec;
+    if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
+    const long long nvox = (long long)r.box * r.box * r.box;
+    CIM_ALLOC(ctx->aux, sizeof(float) * nvox);
+    int rc = fold_to_map_dev(ctx, r.acc.as<float4>(), r.dxy, r.box, r.maxR, 1, -1.f, -1.f, ctx->aux.as<float>());
+    if (rc) return rc;
+    const long long m3 = (long long)(r.box - 8) * (r.box - 8) * (r.box - 8);
+    hipLaunchKernelGGL(k_merge_crop, dim3((unsigned)((m3 + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->aux.as<float>(), out_dev, r.box);
+    CIM_HIP(hipGetLastError());
+    return CIM_OK;
+}
+
+int rec_finish_dev(cim_ctx* ctx, float* map_dev) {
+    RecState& r = ctx->rec;
+    if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
+    return fold_to_map_dev(ctx, r.acc.as<float4>(), r.dxy, r.box, r.maxR, 0, -1.f, -1.f, map_dev);
 }
 
 }  // namespace cim

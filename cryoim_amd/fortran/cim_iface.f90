@@ -10,6 +10,7 @@ module cim_iface
 
     integer(c_int), parameter :: CIM_OK = 0
     integer(c_int), parameter :: CIM_PART_STRIDE = 12
+    integer(c_int), parameter :: CIM_FSC_RECONSTRUCT = 0, CIM_FSC_MERGE = 1
     ! 1-based positions of the record fields in rec(:, i)
     integer, parameter :: P_THETA = 1, P_PHI = 2, P_OMEGA = 3, P_X = 4, P_Y = 5, P_PR = 6
     integer, parameter :: P_DF1 = 7, P_DF2 = 8, P_ASTIG = 9, P_MASKX = 10, P_MASKY = 11, P_USE = 12
@@ -73,6 +74,27 @@ module cim_iface
             import :: c_ptr, c_int, c_float
             type(c_ptr), value :: ctx
             real(c_float) :: map(*)
+        end function
+
+        integer(c_int) function cim_rec_add_tnf(ctx, fa, fb, wg) bind(C, name='cim_rec_add_tnf')
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: ctx
+            real(c_float) :: fa(*), fb(*), wg(*)
+        end function
+
+        integer(c_int) function cim_merge_map(ctx, map) bind(C, name='cim_merge_map')
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: ctx
+            real(c_float) :: map(*)
+        end function
+
+        integer(c_int) function cim_fsc_map(ctx, box, maxR, imgmask, variant, ofa, ofb, owg, efa, efb, ewg, &
+                                            fsc, map) bind(C, name='cim_fsc_map')
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: box, maxR, variant
+            real(c_float), value :: imgmask
+            real(c_float) :: ofa(*), ofb(*), owg(*), efa(*), efb(*), ewg(*), fsc(*), map(*)
         end function
 
         integer(c_int) function cim_search_templates(ctx, p, T, images, tort) bind(C, name='cim_search_templates')

@@ -102,6 +102,8 @@ def load() -> ctypes.CDLL:
         "cim_rec_add_tnf": (ip, [vp, fp, fp, fp]),
         "cim_rec_finish": (ip, [vp, fp]),
         "cim_rec_finish_dev": (ip, [vp, vp]),
+        "cim_merge_map": (ip, [vp, fp]),
+        "cim_fsc_map": (ip, [vp, ip, ip, ctypes.c_float, ip, fp, fp, fp, fp, fp, fp, fp, fp]),
         "cim_search_templates": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global_dev": (ip, [vp, P, ip, vp, fp]),
@@ -256,6 +258,26 @@ class Context:
         m = np.empty((box, box, box), np.float32)
         self._check(self.lib.cim_rec_finish(self.h, _fptr(m)), "cim_rec_finish")
         return m
+
+    def merge_map(self, box: int) -> np.ndarray:
+        """merge_reconstruct.f90 map of the accumulated tnf sums, (box-8)^3."""
+        m = np.empty((box - 8,) * 3, np.float32)
+        self._check(self.lib.cim_merge_map(self.h, _fptr(m)), "cim_merge_map")
+        return m
+
+    def fsc_map(self, box: int, maxR: int, odd, even, imgmask: float = 0.0, merge: bool = False):
+        """Half-map FSC + FSC-weighted map: (fsc[maxR+1], map).  merge=False is
+        reconstruct_FSC.f90 (box^3), merge=True merge_reconstruct_FSC.f90 ((box-8)^3)."""
+        arrs = [_f32(a).reshape(-1) for a in (*odd, *even)]
+        n = (2 * maxR + 1) ** 2 * (maxR + 1)
+        if any(a.size != n for a in arrs):
+            raise CimError(f"tnf arrays must hold (2maxR+1)^2 (maxR+1) = {n} values")
+        f = np.empty(maxR + 1, np.float32)
+        m = np.empty((box - 8,) * 3 if merge else (box,) * 3, np.float32)
+        rc = self.lib.cim_fsc_map(self.h, box, maxR, float(imgmask), 1 if merge else 0, *[_fptr(a) for a in arrs],
+                                  _fptr(f), _fptr(m))
+        self._check(rc, "cim_fsc_map")
+        return f, m
 
     # -- search -------------------------------------------------------------
     def search_templates(self, params: CimParams, images, tort):

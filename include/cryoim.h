@@ -117,7 +117,9 @@ int cim_fft3d_dev(cim_ctx* ctx, void* data, int n, int dir, int conj_out);
 /* layout: 0 = full box accumulator as reconstruct.f90 (offset box/2),
  *         1 = compact tnf accumulator as reconstruct_substep.f90 (offset maxR).
  * rcut: radius cut of inserted samples (maxR, maxR-1 or maxR-2 in the
- * reference variants).  wg_init: initial weight (0.01 reconstruct.f90:28,
+ * reference variants; the half maps of reconstruct_half.f90:98 use maxR with
+ * the compact layout, whose edge corners wrap as that array's flat index
+ * does -- see cim_rec_get_tnf).  wg_init: initial weight (0.01 reconstruct.f90:28,
  * 0 substep).  ops: nops row-major 3x3 float64 symmetry operators, or NULL
  * with nops = 60 for the icosahedral group (icos2f_sym_mat order). */
 int cim_rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init,
@@ -134,7 +136,10 @@ int cim_rec_insert_dev(cim_ctx* ctx, const cim_params* p, int n, const void* ima
 int cim_rec_insert_ft(cim_ctx* ctx, int n, const float* ft_re, const float* ft_im,
                       const float* ctf, const float* euler);
 /* Copy the accumulated sums out in tnf order ((2maxR+1)^2 (maxR+1) each,
- * lz slowest, hx fastest; reconstruct_substep.f90:148-155). */
+ * lz slowest, hx fastest; reconstruct_substep.f90:148-155).  With the
+ * compact layout, corners at x or y = maxR+1 land where the reference's flat
+ * index puts them (next row / next plane, reconstruct_half.f90:127-130) and
+ * corners past the last plane are dropped. */
 int cim_rec_get_tnf(cim_ctx* ctx, float* fa, float* fb, float* wg);
 /* Add tnf-order partial sums into the accumulator (merge of several
  * reconstruct_substep outputs, merge_reconstruct.f90:23-34). */
@@ -144,6 +149,31 @@ int cim_rec_add_tnf(cim_ctx* ctx, const float* fa, const float* fb, const float*
  * box^3 f32 (reconstruct.f90:151-247). */
 int cim_rec_finish(cim_ctx* ctx, float* map);
 int cim_rec_finish_dev(cim_ctx* ctx, void* map_dev);
+/* merge_icosprocess_subrec without FSC (merge_reconstruct.f90:14-157): the
+ * caller opens a compact accumulator with weights 0.01
+ * (cim_rec_begin(ctx, box, maxR, 1, maxR, 0.01f, 60, NULL)), adds each
+ * partial tnf with cim_rec_add_tnf, then this folds with the integer shell
+ * test of :42 (int(r) <= maxR), inverse-transforms and writes the
+ * (box-8)^3 block of :143-149, including its plane index i/(box-8)^2+4 on
+ * the 1-based i (the last voxel of each plane comes from the next plane). */
+int cim_merge_map(cim_ctx* ctx, float* map);
+
+/* ---- half-map FSC ------------------------------------------------------ */
+#define CIM_FSC_RECONSTRUCT 0   /* icosprocess reconstruct_FSC.f90:10-197 */
+#define CIM_FSC_MERGE 1         /* merge_reconstruct_FSC.f90:9-210 */
+/* odd/even tnf sums (as written by reconstruct_half.f90:143-151, summed over
+ * files for the merge) -> fsc[0..maxR] = the FSC column of FSC.txt
+ * (shells ir = int(r); reconstruct_FSC.f90:37-63), and the FSC-weighted map:
+ *   CIM_FSC_RECONSTRUCT: box^3 floats, cmplx(fa, fb) inverse-transformed
+ *     (reconstruct_FSC.f90:173-179);
+ *   CIM_FSC_MERGE: the (box-8)^3 floats merge_reconstruct_FSC writes, i.e.
+ *     the first (box-8)^3 values of the flat box^3 map masked to radius
+ *     imgmask (:188-206).
+ * fsc may be NULL.  The weighting keeps the reference's stale-index zeroing
+ * (reconstruct_FSC.f90:82-84). */
+int cim_fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, const float* odd_fa,
+                const float* odd_fb, const float* odd_wg, const float* eve_fa, const float* eve_fb,
+                const float* eve_wg, float* fsc, float* map);
 
 /* ---- icosprocess global search (mode 4) and refinement (mode 5) ------- */
 /* Template bank of load_template.f90:18-48: T real images (templatestck)

@@ -37,6 +37,26 @@ build_objs "$OUT/icos" "${ICOS_SRCS[@]}"
 ICOS_LIBOBJS=$(ls "$OUT"/icos/*.o | grep -v '/0main.o$')
 $FC $FFLAGS -o "$OUT/icosprocess" $ICOS_LIBOBJS "$OUT/icos/0main.o" $LIBS
 
+# icosprocess_fsc: the same objects with a main whose 'FSC' key is parsed.
+# The shipped 0main.f90:155 compares the key with 'fsc' while its key table
+# holds 'FSC' (0input_parameters.f90:30), so FSC=y never reaches
+# reconstruct_half / reconstruct_FSC.  The patched main is a generated file
+# under _ref/ only, made to record golden vectors of that otherwise
+# unreachable path.
+mkdir -p "$OUT/icos_fsc"
+sed "s/\.eq\.'fsc')/.eq.'FSC')/" "$S/0main.f90" > "$OUT/icos_fsc/0main_fsckey.f90"
+if ! grep -q "eq.'FSC')" "$OUT/icos_fsc/0main_fsckey.f90"; then echo "FSC key patch did not apply"; exit 4; fi
+$FC $FFLAGS -J "$OUT/icos" -c "$OUT/icos_fsc/0main_fsckey.f90" -o "$OUT/icos_fsc/0main_fsckey.o"
+$FC $FFLAGS -o "$OUT/icosprocess_fsc" $ICOS_LIBOBJS "$OUT/icos_fsc/0main_fsckey.o" $LIBS
+
+# merge_icosprocess_subrec (tnf merge, plain and FSC-weighted)
+M=$REF/merge_icosprocess_subrec/SRC
+mkdir -p "$OUT/merge"
+build_objs "$OUT/merge" "$M/common_merge_icosprocess_subrec.f90" "$M/FFTW.f90" "$M/FFT1d.f90" "$M/help.f90" \
+  "$M/icos2f_sym_mat.f90" "$M/img3d_icosavg.f90" "$M/mrchead.f90" "$M/merge_reconstruct.f90" \
+  "$M/merge_reconstruct_FSC.f90" "$M/main.f90"
+$FC $FFLAGS -o "$OUT/merge_icosprocess_subrec" "$OUT"/merge/*.o $LIBS
+
 # local_refine
 L=$REF/local_refine/SRC
 LREF_SRCS=("$L/0common_loc_refine.f90" "$L/FFTW.f90")

@@ -703,6 +703,10 @@ EXPORT void o_insert(real* fa, real* fb, real* wg, int compact, int nx, int maxR
 {
     long D = compact ? 2 * maxR + 1 : nx;
     long D2 = D * D;
+    /* The compact array is indexed flat (reconstruct_half.f90:127-130): a
+     * corner at x = 2maxR+1 wraps into the next row, one at y = 2maxR+1 into
+     * the next plane; one past the last plane falls outside the array. */
+    long size = compact ? D2 * (maxR + 1) : D2 * nx;
     real off = compact ? (real)maxR : (real)(nx / 2);
     for (int ky = -maxR; ky <= maxR; ++ky)
         for (int hx = 0; hx <= maxR; ++hx) {
@@ -733,6 +737,7 @@ EXPORT void o_insert(real* fa, real* fb, real* wg, int compact, int nx, int maxR
                 long c1 = lx[2] * D2 + lx[1] * D + lx[0];
                 long id[8] = {c1, c1 + 1, c1 + D, c1 + D + 1, c1 + D2, c1 + D2 + 1, c1 + D2 + D, c1 + D2 + D + 1};
                 for (int j = 0; j < 8; ++j) {
+                    if (id[j] < 0 || id[j] >= size) continue;
                     fa[id[j]] += a[i2] * w[j] * c;
                     fb[id[j]] += b[i2] * w[j] * c * (real)signz;
                     wg[id[j]] += w[j] * c * c;
@@ -743,7 +748,16 @@ EXPORT void o_insert(real* fa, real* fb, real* wg, int compact, int nx, int maxR
 
 /* Fold of the lz = 0 plane, division by the weights and (-1)^(h+k+l):
  * icosprocess/SRC/reconstruct.f90:151-237.  Full nx^3 volumes in place. */
-EXPORT void o_fold_normalise(real* fa, real* fb, real* wg, int nx, int maxR)
+/* int_radius = 0: dr = sqrt(real) <= maxR (reconstruct.f90:151-237).
+ * int_radius = 1: int(sqrt(real)) <= maxR, the shell test of
+ * reconstruct_FSC.f90:101-169 and merge_reconstruct.f90:37-107 (integer dr). */
+static int o_in_sphere(int hx, int ky, int lz, int maxR, int int_radius)
+{
+    real dr = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+    return int_radius ? ((int)dr <= maxR) : (dr <= (real)maxR);
+}
+
+EXPORT void o_fold_normalise(real* fa, real* fb, real* wg, int nx, int maxR, int int_radius)
 {
     long n2 = (long)nx * nx;
     int c = nx / 2;
@@ -755,8 +769,7 @@ EXPORT void o_fold_normalise(real* fa, real* fb, real* wg, int nx, int maxR)
             int hxlo = pass == 0 ? -maxR : 1;
             for (int hx = hxlo; hx <= maxR; ++hx) {
                 long n1 = IDX(hx, ky, lz), nn2 = IDX(-hx, -ky, -lz);
-                real dr = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
-                if (dr <= (real)maxR) {
+                if (o_in_sphere(hx, ky, lz, maxR, int_radius)) {
                     wg[n1] = wg[n1] + wg[nn2];
                     fa[n1] = (fa[n1] + fa[nn2]) / wg[n1];
                     fb[n1] = (fb[n1] - fb[nn2]) / wg[n1];
@@ -778,8 +791,7 @@ EXPORT void o_fold_normalise(real* fa, real* fb, real* wg, int nx, int maxR)
         for (int ky = -maxR; ky <= maxR; ++ky)
             for (int hx = -maxR; hx <= maxR; ++hx) {
                 long n1 = IDX(hx, ky, lz), nn2 = IDX(-hx, -ky, -lz);
-                real dr = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
-                if (dr <= (real)maxR) {
+                if (o_in_sphere(hx, ky, lz, maxR, int_radius)) {
                     fa[n1] = fa[n1] / wg[n1];
                     fb[n1] = fb[n1] / wg[n1];
                     if (((hx + ky + lz) % 2) != 0) { fa[n1] = -fa[n1]; fb[n1] = -fb[n1]; }
@@ -792,20 +804,120 @@ EXPORT void o_fold_normalise(real* fa, real* fb, real* wg, int nx, int maxR)
 #undef IDX
 }
 
-/* cmplx(fa, -fb) -> backward 3D FFT -> shift3dComplex -> real:
- * icosprocess/SRC/reconstruct.f90:240-247, FFTW.f90:213-219,304-321. */
-EXPORT void o_volume_to_map(const real* fa, const real* fb, int nx, real* map)
+/* cmplx(fa, s*fb) -> backward 3D FFT -> shift3dComplex -> real:
+ * s = -1 in reconstruct.f90:240-247 and merge_reconstruct.f90:133-139,
+ * s = +1 in reconstruct_FSC.f90:173-179 (FFTW.f90:213-219,304-321).
+ * merge_reconstruct_FSC.f90:181-185 transforms cmplx(fa, fb) forward with
+ * FFT1d (isn = -1); its real part is that of s = -1 here. */
+EXPORT void o_volume_to_map(const real* fa, const real* fb, int nx, int imag_sign, real* map)
 {
     long N = (long)nx * nx * nx, n2 = (long)nx * nx;
     double* re = (double*)malloc(sizeof(double) * N);
     double* im = (double*)malloc(sizeof(double) * N);
-    for (long i = 0; i < N; ++i) { re[i] = fa[i]; im[i] = -(double)fb[i]; }
+    for (long i = 0; i < N; ++i) { re[i] = fa[i]; im[i] = imag_sign < 0 ? -(double)fb[i] : (double)fb[i]; }
     o_dft_cube(re, im, nx, 3, +1);
     for (long i = 0; i < N; ++i) {
         long z = i / n2, y = (i % n2) / nx, x = i % nx;
         map[i] = (real)(((x + y + z) & 1) ? -re[i] : re[i]);
     }
     free(re); free(im);
+}
+
+/* Half-map FSC and FSC-weighted merge of two tnf volumes (compact layout,
+ * lz slowest): reconstruct_FSC.f90:30-97, identical in
+ * merge_reconstruct_FSC.f90:38-97.  Both weight sums get +1, the shell FSC
+ * uses ir = int(r) over the whole half-cube, and the merged sums are scaled
+ * by sqrt(2 FSC / (1 + FSC)) (0 where FSC <= 0).  The weighting loop's
+ * else-branch zeroes a(i3), b(i3) at the index left from the last in-sphere
+ * voxel, so each in-sphere voxel that is followed (in loop order) by an
+ * out-of-sphere one ends with a = b = 0; that is kept.
+ * fsc_raw[ir] = FSC(ir) as written to FSC.txt; out: merged fa, fb, wg. */
+EXPORT void o_tnf_fsc(int maxR, const real* A1, const real* B1, const real* W1, const real* A2, const real* B2,
+                      const real* W2, real* fsc_raw, real* fa, real* fb, real* wg)
+{
+    const long T = 2 * maxR + 1, n = T * T * (maxR + 1);
+    real* w2 = (real*)malloc(sizeof(real) * n);
+    real* a2 = (real*)malloc(sizeof(real) * n);
+    real* b2 = (real*)malloc(sizeof(real) * n);
+    real* S = (real*)malloc(sizeof(real) * (maxR + 1));
+    real* S1 = (real*)malloc(sizeof(real) * (maxR + 1));
+    real* S2 = (real*)malloc(sizeof(real) * (maxR + 1));
+    for (long i = 0; i < n; ++i) {
+        wg[i] = W1[i] + (real)1;
+        w2[i] = W2[i] + (real)1;
+        fa[i] = A1[i] / wg[i];
+        fb[i] = B1[i] / wg[i];
+        a2[i] = A2[i] / w2[i];
+        b2[i] = B2[i] / w2[i];
+    }
+    for (int r = 0; r <= maxR; ++r) { S[r] = 0; S1[r] = (real)1.0e-10; S2[r] = (real)1.0e-10; }
+    long i3 = 0;
+    for (int lz = 0; lz <= maxR; ++lz)
+        for (int ky = -maxR; ky <= maxR; ++ky)
+            for (int hx = -maxR; hx <= maxR; ++hx) {
+                int ir = (int)(real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+                if (ir > maxR) continue;
+                i3 = (long)lz * T * T + (long)(ky + maxR) * T + (hx + maxR);
+                S[ir] += fa[i3] * a2[i3] + fb[i3] * b2[i3];
+                S1[ir] += fa[i3] * fa[i3] + fb[i3] * fb[i3];
+                S2[ir] += a2[i3] * a2[i3] + b2[i3] * b2[i3];
+            }
+    for (int r = 0; r <= maxR; ++r) {
+        real f = S[r] / (real)sqrt(S1[r] * S2[r]);
+        fsc_raw[r] = f;
+        S[r] = f <= 0 ? (real)0 : (real)sqrt((real)2 * f / ((real)1 + f));
+    }
+    for (long i = 0; i < n; ++i) {
+        fa[i] = fa[i] * wg[i];
+        fb[i] = fb[i] * wg[i];
+        a2[i] = a2[i] * w2[i];
+        b2[i] = b2[i] * w2[i];
+        fa[i] = fa[i] + a2[i];
+        fb[i] = fb[i] + b2[i];
+        wg[i] = wg[i] + w2[i] - (real)2;
+    }
+    for (int lz = 0; lz <= maxR; ++lz)
+        for (int ky = -maxR; ky <= maxR; ++ky)
+            for (int hx = -maxR; hx <= maxR; ++hx) {
+                int ir = (int)(real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+                if (ir <= maxR) {
+                    i3 = (long)lz * T * T + (long)(ky + maxR) * T + (hx + maxR);
+                    fa[i3] = fa[i3] * S[ir];
+                    fb[i3] = fb[i3] * S[ir];
+                    wg[i3] = wg[i3] * S[ir];
+                } else {
+                    fa[i3] = 0;
+                    fb[i3] = 0;
+                }
+            }
+    for (long i = 0; i < n; ++i) wg[i] = wg[i] + (real)1;
+    free(w2); free(a2); free(b2); free(S); free(S1); free(S2);
+}
+
+/* Output crop of merge_reconstruct.f90:143-149: the (nx-8)^3 centre block,
+ * with lz taken as i/(nx-8)^2 + 4 from the 1-based i (not i-1), so the
+ * last voxel of each output plane comes from the next plane. */
+EXPORT void o_merge_crop(const real* map, int nx, real* out)
+{
+    const long M = nx - 8, M2 = M * M, n2 = (long)nx * nx;
+    for (long i = 1; i <= M2 * M; ++i) {
+        long lz = i / M2 + 4, ky = ((i - 1) % M2) / M + 4, hx = ((i - 1) % M2) % M + 4;
+        out[i - 1] = map[lz * n2 + ky * nx + hx];
+    }
+}
+
+/* merge_reconstruct_FSC.f90:188-197: zero outside the sphere of radius
+ * imgmask about (nx/2, nx/2, nx/2) (the checkerboard inside is the one
+ * o_volume_to_map already applies). */
+EXPORT void o_sphere_zero(real* map, int nx, real imgmask)
+{
+    const long n2 = (long)nx * nx;
+    for (long i = 0; i < n2 * nx; ++i) {
+        long lz = i / n2, ky = (i % n2) / nx, hx = i % nx;
+        real d = (real)sqrt((real)((hx - nx / 2) * (hx - nx / 2) + (ky - nx / 2) * (ky - nx / 2) +
+                                   (lz - nx / 2) * (lz - nx / 2)));
+        if (!(d <= imgmask)) map[i] = 0;
+    }
 }
 
 /* ------------------------------------------------------------------ */

@@ -143,13 +143,63 @@ class Oracle:
             shape = (nx, nx, nx)
         return (np.zeros(shape, self.dt), np.zeros(shape, self.dt), np.full(shape, wg_init, self.dt))
 
-    def fold_and_map(self, vol, maxR):
-        fa, fb, wg = (v.copy() for v in vol)
+    def fold_and_map(self, vol, maxR, int_radius=False, imag_sign=-1):
+        """Fold + normalise + inverse FFT of a full-box volume (reconstruct.f90:151-247;
+        int_radius/imag_sign select the reconstruct_FSC / merge variants)."""
+        fa, fb, wg = (self._a(v).copy() for v in vol)
         nx = fa.shape[-1]
-        self.L.o_fold_normalise(self._p(fa), self._p(fb), self._p(wg), nx, int(maxR))
+        self.L.o_fold_normalise(self._p(fa), self._p(fb), self._p(wg), nx, int(maxR), int(bool(int_radius)))
         m = np.zeros_like(fa)
-        self.L.o_volume_to_map(self._p(fa), self._p(fb), nx, self._p(m))
+        self.L.o_volume_to_map(self._p(fa), self._p(fb), nx, int(imag_sign), self._p(m))
         return m
+
+    def embed_tnf(self, fa, fb, wg, nx, maxR, wg_fill=0.0):
+        """Place compact tnf arrays ((maxR+1, 2maxR+1, 2maxR+1), lz slowest) at the
+        centre of an nx^3 volume (lz >= 0 half); weights elsewhere wg_fill."""
+        c = nx // 2
+        out = [np.zeros((nx, nx, nx), self.dt) for _ in range(2)] + [np.full((nx, nx, nx), wg_fill, self.dt)]
+        sl = (slice(c, c + maxR + 1), slice(c - maxR, c + maxR + 1), slice(c - maxR, c + maxR + 1))
+        for o, v in zip(out, (fa, fb, wg)):
+            o[sl] = self._a(v).reshape(maxR + 1, 2 * maxR + 1, 2 * maxR + 1)
+        return tuple(out)
+
+    def tnf_fsc(self, odd, even, maxR):
+        """reconstruct_FSC.f90:30-97: (raw FSC per shell, merged fa, fb, wg)."""
+        ins = [self._a(v).reshape(-1).copy() for v in (*odd, *even)]
+        n = ins[0].size
+        outs = [np.zeros(n, self.dt) for _ in range(3)]
+        f = np.zeros(maxR + 1, self.dt)
+        self.L.o_tnf_fsc(int(maxR), *[self._p(v) for v in ins], self._p(f), *[self._p(o) for o in outs])
+        return (f, *outs)
+
+    def fsc_map(self, odd, even, nx, maxR, imgmask=None, merge=False):
+        """Map of reconstruct_FSC.f90 (merge=False, nx^3) or of
+        merge_reconstruct_FSC.f90 (merge=True: masked, then the first (nx-8)^3
+        values of the flat array, as its writer at :204-206 emits them)."""
+        f, fa, fb, wg = self.tnf_fsc(odd, even, maxR)
+        vol = self.embed_tnf(fa, fb, wg, nx, maxR)
+        if not merge:
+            return f, self.fold_and_map(vol, maxR, int_radius=True, imag_sign=+1)
+        m = self.fold_and_map(vol, maxR, int_radius=True, imag_sign=-1)
+        self.L.o_sphere_zero(self._p(m), nx, self._r(imgmask))
+        M = nx - 8
+        return f, m.reshape(-1)[: M ** 3].reshape(M, M, M).copy()
+
+    def merge_map(self, tnfs, nx, maxR):
+        """merge_reconstruct.f90: sum of tnf triples onto weights 0.01, integer-radius
+        fold, inverse FFT, then the (nx-8)^3 crop of :143-149."""
+        T = 2 * maxR + 1
+        fa, fb = np.zeros((maxR + 1, T, T), self.dt), np.zeros((maxR + 1, T, T), self.dt)
+        wg = np.full((maxR + 1, T, T), 0.01, self.dt)
+        for t in tnfs:
+            fa += self._a(t[0]).reshape(fa.shape)
+            fb += self._a(t[1]).reshape(fa.shape)
+            wg += self._a(t[2]).reshape(fa.shape)
+        vol = self.embed_tnf(fa, fb, wg, nx, maxR, wg_fill=0.01)
+        m = self.fold_and_map(vol, maxR, int_radius=True, imag_sign=-1)
+        out = np.zeros((nx - 8) ** 3, self.dt)
+        self.L.o_merge_crop(self._p(m), nx, self._p(out))
+        return out.reshape(nx - 8, nx - 8, nx - 8)
 
     # ---- common lines ------------------------------------------------------------
     def search_grid(self, step):

@@ -25,6 +25,15 @@ def O32():
     return Oracle(32)
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.fixture(scope="session")
+def prog():
+    """Program vectors of the reference on the 48-pixel synthetic set."""
+    return np.load(os.path.join(GOLDEN, "programs.npz"))
+
+
 @pytest.fixture(scope="session")
 def ctx():
     from cryoim_amd import lib

@@ -12,9 +12,12 @@ Two kinds of vectors:
   img3d_FFT);
 * program vectors: the real icosprocess (modes 1, 1+subrec, 4, 5) and
   local_refine executables run on a small synthetic data set written in the
-  reference's own MRC / ortfile formats.
+  reference's own MRC / ortfile formats;
+* FSC / merge vectors: reconstruct_half + reconstruct_FSC (through
+  oracle/_ref/icosprocess_fsc, whose main parses the FSC key) and the real
+  merge_icosprocess_subrec on half and partial tnf files of the same set.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--only NAME ...]
 """
 from __future__ import annotations
 
@@ -276,15 +279,93 @@ def program_vectors() -> dict:
     return g
 
 
+def _read_fsc_txt(path) -> np.ndarray:
+    """FSC.txt rows 'ir FSC sqrt(2FSC/(1+FSC))' (list-directed, NaN allowed)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) == 3:
+                rows.append([float(v) for v in t])
+    return np.array(rows, np.float64)
+
+
+def fsc_program_vectors() -> dict:
+    """Half maps, FSC weighting and tnf merges on the program_vectors data set."""
+    g = {}
+    box, apix, n = 48, 2.0, 30
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    icos, icos_fsc = os.path.join(REF, "icosprocess"), os.path.join(REF, "icosprocess_fsc")
+    merge = os.path.join(REF, "merge_icosprocess_subrec")
+    with tempfile.TemporaryDirectory() as d:
+        stk = "stack.mrc"
+        formats.write_mrc(os.path.join(d, stk), ps.images)
+        parts = [formats.Particle(stk, i + 1, *ps.angles[i], *ps.shifts[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+        common = ["apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt"]
+
+        def run(exe, args):
+            r = subprocess.run([exe, *args], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            return r.stdout
+
+        def tnf(name, key):
+            t = formats.read_tnf(os.path.join(d, name))
+            g[key + "_fa"], g[key + "_fb"], g[key + "_wg"] = t["fa"], t["fb"], t["wg"]
+            g[key + "_hdr"] = np.array([t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]], i4)
+            g[key + "_hdrf"] = np.array([t["apix"], t["imgmask"]], f4)
+
+        def raw_map(name, m):
+            return np.fromfile(os.path.join(d, name), dtype="<f4", offset=1024, count=m ** 3).reshape(m, m, m)
+
+        # mode 1 with FSC: reconstruct_half over particles 1,3,.. and 2,4,.., then reconstruct_FSC
+        run(icos_fsc, ["mode=1", "FSC=y", "result3d=fsc.mrc", "first=1", f"last={n}", *common])
+        tnf("oddtnf", "half_odd")
+        tnf("evetnf", "half_eve")
+        g["fsc_txt"] = _read_fsc_txt(os.path.join(d, "FSC.txt"))
+        g["fsc_map"] = raw_map("fsc.mrc", box)
+        # subrec + FSC over two particle ranges, merged with FSC weighting (nFSC = 1)
+        run(icos_fsc, ["mode=1", "FSC=y", "subrec=y", "result3d=p1", "first=1", "last=15", *common])
+        run(icos_fsc, ["mode=1", "FSC=y", "subrec=y", "result3d=p2", "first=16", f"last={n}", *common])
+        for p in ("p1", "p2"):
+            tnf(p + "oddtnf", p + "_odd")
+            tnf(p + "evetnf", p + "_eve")
+        run(merge, ["p1", "p2", "mfsc.mrc"])
+        g["mfsc_txt"] = _read_fsc_txt(os.path.join(d, "FSC.txt"))
+        g["mfsc_map"] = raw_map("mfsc.mrc", box - 8)
+        # plain partial reconstructions merged (nFSC = 0)
+        run(icos, ["mode=1", "subrec=y", "result3d=s1.tnf", "first=1", "last=15", *common])
+        run(icos, ["mode=1", "subrec=y", "result3d=s2.tnf", "first=16", f"last={n}", *common])
+        tnf("s1.tnf", "s1")
+        tnf("s2.tnf", "s2")
+        run(merge, ["s1.tnf", "s2.tnf", "m0.mrc"])
+        g["merge_map"] = raw_map("m0.mrc", box - 8)
+        with open(os.path.join(d, "m0.mrc"), "rb") as f:
+            g["merge_hdr"] = np.frombuffer(f.read(1024), dtype="<i4").copy()
+    return g
+
+
 def main():
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", nargs="*", default=None)
+    args = ap.parse_args()
     rng = np.random.default_rng(20261203)
     if not os.path.exists(os.path.join(REF, "icos_units")):
         raise SystemExit("oracle/_ref is not built; run oracle/build_ref.sh first")
-    np.savez_compressed(os.path.join(HERE, "units.npz"), **unit_vectors(rng))
-    np.savez_compressed(os.path.join(HERE, "pr.npz"), **pr_vectors(rng))
-    np.savez_compressed(os.path.join(HERE, "lref_units.npz"), **lref_vectors(rng))
-    np.savez_compressed(os.path.join(HERE, "programs.npz"), **program_vectors())
-    np.savez_compressed(os.path.join(HERE, "lref_program.npz"), **lref_program_vectors())
+    jobs = [("units", lambda: unit_vectors(rng)), ("pr", lambda: pr_vectors(rng)),
+            ("lref_units", lambda: lref_vectors(rng)), ("programs", program_vectors),
+            ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors)]
+    for name, fn in jobs:
+        # the seeded rng is shared by the first three: run them in order even when only one is wanted
+        if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):
+            continue
+        vec = fn()
+        if args.only is None or name in args.only:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **vec)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

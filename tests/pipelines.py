@@ -8,7 +8,13 @@ Parameters are derived as calc_initialvalues does
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
+
+
+def golden_path(name: str) -> str:
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
 
 
 def derived(box, apix, maxRes=8.0, minRes=300.0, imgmask_A=None):
@@ -36,6 +42,20 @@ def mode1_volume(O, images, angles, shifts, df, apix, maxR, mask_r, compact=Fals
         a, b = O.particle_ft(images[i], x, y, mask_r=mask_r, mask_xy=tuple(shifts[0]), norm=True)
         c = O.ctf2d(box, df[i, 0], df[i, 1], 0.0, apix) if ctf else None
         O.insert(vol, a, b, c, angles[i], maxR, rcut=rcut, compact=compact)
+    return vol
+
+
+def half_volume(O, images, angles, shifts, df, apix, maxR, mask_r, parity, ctf=True):
+    """reconstruct_half.f90:47-141: compact sums over particles parity,
+    parity+2, ... with rcut = maxR and weights from 0; the mask is centred on
+    the first particle of the whole range (particle%x of :84)."""
+    box = images.shape[-1]
+    vol = O.new_volume(box, maxR, compact=True, wg_init=0.0)
+    for i in range(parity, len(images), 2):
+        x, y = shifts[i]
+        a, b = O.particle_ft(images[i], x, y, mask_r=mask_r, mask_xy=tuple(shifts[0]), norm=True)
+        c = O.ctf2d(box, df[i, 0], df[i, 1], 0.0, apix) if ctf else None
+        O.insert(vol, a, b, c, angles[i], maxR, rcut=maxR, compact=True)
     return vol
 
 

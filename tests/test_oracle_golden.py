@@ -27,11 +27,6 @@ def lref():
     return np.load(os.path.join(G, "lref_units.npz"))
 
 
-@pytest.fixture(scope="module")
-def prog():
-    return np.load(os.path.join(G, "programs.npz"))
-
-
 # ---- rotations ------------------------------------------------------------
 
 def test_symops_match_reference_table(O64, units):
