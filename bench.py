@@ -1,9 +1,9 @@
-"""Throughput of the icosahedral search + insert hot path on one MI355X.
+"""Throughput of the icosahedral search + insert hot path on one MI355X:
+the chosen path against the plain FP32 path, on the same particles.
 
-Workload (the plain FP32 method at real size): box 384 particles of a seeded
-60-fold Gaussian-blob phantom, CTF-modulated (defocus U(0.8, 3.5) um,
-300 kV, Cs 2.7 mm, A 0.1, 1.5 A/px) with Gaussian noise at SNR 0.05.  One
-step takes a batch of particles through
+Workload: box 384 particles of a seeded 60-fold Gaussian-blob phantom,
+CTF-modulated (defocus U(0.8, 3.5) um, 300 kV, Cs 2.7 mm, A 0.1, 1.5 A/px)
+with Gaussian noise at SNR 0.05.  One step takes a batch of particles through
 
   * the icosprocess global search (mode 4: mask, FFT, Fcut fit, centre shift,
     common-lines phase residual against T templates over the 0.5 deg AU grid
@@ -12,11 +12,19 @@ step takes a batch of particles through
     mask, FFT, centre shift, trilinear insert) at maxR 190 (Nyquist-2) at
     the orientations the search returned.
 
-Inputs are resident in HBM before the timed region.  The metric is
-particles/s over the K timed steps.  The roofline block describes the
-dominant kernel (the common-lines search); the cpu_baseline block times the
-reference's own Fortran (oracle/_ref/icosprocess, MKL FFTW) on a bounded
-sample on the host cores, or the C oracle when that binary cannot run.
+Chosen path (value): the tabulated search (CIM_SEARCH_TABLES: per-particle
+residual tables over ray pairs, every (AU point, omega) scored from them,
+the best 32 AU points re-scored exactly, exhaustive fallback) and the
+compensated FP32 insert (CIM_REC_COMPENSATED).  Plain path: the exhaustive
+search of the reference's method and the plain FP32 insert.  The line
+reports both rates, how many particles get a different orientation, and the
+FSC between the two maps.
+
+Inputs are resident in HBM before the timed region.  The roofline block
+describes the kernel that takes most of a chosen step; the cpu_baseline
+block times the reference's own Fortran (oracle/_ref/icosprocess, MKL FFTW)
+on a bounded sample on the host cores, or the C oracle when that binary
+cannot run.
 """
 from __future__ import annotations
 
@@ -39,11 +47,15 @@ INSERT_MAXR = 190
 STEP = 0.5
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# arithmetic of one scored sample of cal_crosslin_PR_omega.f90:48-69 (see DESIGN.md):
+COMPENSATED = 16          # CIM_REC_COMPENSATED
+# arithmetic of one scored sample of cal_crosslin_PR_omega.f90:48-69:
 # 2 coordinate FMAs (4), bilinear weights (8) and two 4-tap blends (14),
 # |F|^2 + sqrt (4), gate (1), atan2 (1), phase difference and fold (3),
 # weight (2), two accumulations (3)
 FLOPS_PER_SAMPLE = 40
+# insert, per (sample, operator): 8 corners x read-modify-write of fa, fb, wg in f32
+INSERT_BYTES_PER_SAMPLE_OP = 8 * 3 * 8
+METRIC = "particles/s search+insert, box 384: chosen vs plain FP32 GPU at FSC>=0.999"
 
 
 def _parse():
@@ -51,11 +63,12 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16, help="particles per step")
+    ap.add_argument("--batch", type=int, default=64, help="particles per step")
     ap.add_argument("--templates", type=int, default=4, help="template bank size T")
+    ap.add_argument("--topk", type=int, default=32, help="AU points re-scored exactly per particle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
-                    help="run a 512 MiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
+                    help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -77,18 +90,37 @@ def derived():
     return pipelines.derived(BOX, APIX, maxRes=MAXRES, minRes=MINRES, imgmask_A=(BOX / 2 - 4) * APIX)
 
 
+def _fsc(a, b, rmax):
+    """Shell FSC (ir = int(r)) of two real maps on the GPU, shells 0..rmax."""
+    import torch
+    fa, fb = torch.fft.rfftn(a), torch.fft.rfftn(b)
+    n = a.shape[0]
+    f0 = torch.fft.fftfreq(n, d=1.0 / n, device=a.device)
+    f2 = torch.fft.rfftfreq(n, d=1.0 / n, device=a.device)
+    r = torch.sqrt(f0[:, None, None] ** 2 + f0[None, :, None] ** 2 + f2[None, None, :] ** 2).to(torch.int64)
+    r = r.reshape(-1)
+    keep = r <= rmax
+    r = r[keep]
+    fa, fb = fa.reshape(-1)[keep], fb.reshape(-1)[keep]
+    num = torch.bincount(r, weights=(fa * fb.conj()).real.double(), minlength=rmax + 1)
+    da = torch.bincount(r, weights=(fa.abs() ** 2).double(), minlength=rmax + 1)
+    db = torch.bincount(r, weights=(fb.abs() ** 2).double(), minlength=rmax + 1)
+    return (num / torch.sqrt(da * db).clamp_min(1e-300)).cpu().numpy()
+
+
 def gpu_run(args, tang, timgs, ps):
     import torch
     from cryoim_amd import lib
     d = derived()
     ctx = lib.Context(0)
     n = len(ps.images)
-    ps_search = lib.make_params(BOX, d["minR"], d["maxR"], apply_ctf=True, mask_on=True, apix=APIX,
-                                imgmask=d["mask"], searchstep=STEP)
-    ps_insert = lib.make_params(BOX, 0, INSERT_MAXR, apply_ctf=True, mask_on=True, norm_on=True, apix=APIX,
-                                imgmask=d["mask"])
-    ctx.search_templates(ps_search, timgs, np.c_[tang, np.zeros((len(tang), 2))])
-    ctx.rec_begin(BOX, INSERT_MAXR, layout=0, rcut=INSERT_MAXR, wg_init=0.01)
+    common = dict(apply_ctf=True, mask_on=True, apix=APIX, imgmask=d["mask"], searchstep=STEP)
+    p_plain = lib.make_params(BOX, d["minR"], d["maxR"], search_method=lib.SEARCH_PLAIN, **common)
+    p_chosen = lib.make_params(BOX, d["minR"], d["maxR"], search_method=lib.SEARCH_TABLES, search_topk=args.topk,
+                               **common)
+    p_insert = lib.make_params(BOX, 0, INSERT_MAXR, apply_ctf=True, mask_on=True, norm_on=True, apix=APIX,
+                               imgmask=d["mask"])
+    ctx.search_templates(p_chosen, timgs, np.c_[tang, np.zeros((len(tang), 2))])
     images = torch.from_numpy(ps.images).to("cuda:0")
     if args.calib_copy:
         # 1 GiB copy after 1 GiB of unrelated writes, so the source is read from HBM
@@ -106,45 +138,63 @@ def gpu_run(args, tang, timgs, ps):
     P[:, 9:11] = ps.shifts[0]          # reconstruct.f90:92 masks around the first particle
     P[:, 11] = 1.0
 
-    def step():
-        out = ctx.search_global(ps_search, images, P)
+    def step(psearch):
+        out = ctx.search_global(psearch, images, P)
         Q = P.copy()
         Q[:, 0:3] = out[:, 0:3]
-        ctx.rec_insert(ps_insert, images, Q)
+        ctx.rec_insert(p_insert, images, Q)
         return out
 
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    ctx.timing(True)
-    ctx.counter_reset()
     stream = torch.cuda.ExternalStream(ctx.stream)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        out = step()
-    e1.record(stream)
-    ctx.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = e0.elapsed_time(e1)
-    k_ms, k_n = ctx.timing_get("cl_search")
-    ins_ms, ins_n = ctx.timing_get("insert")
-    samples = ctx.counter("cl_search_samples")
-    shape = {"omega_per_thread": ctx.counter("cl_search_vpt"), "lds_disk": bool(ctx.counter("cl_search_lds"))}
-    ins_samples = ctx.counter("insert_samples")
-    # map at the end (once per reconstruction, outside the per-step loop)
-    t1 = time.perf_counter()
-    vol = torch.empty((BOX, BOX, BOX), dtype=torch.float32, device="cuda:0")
-    ctx.rec_finish(BOX, vol)
-    finish_s = time.perf_counter() - t1
-    ctx.timing(False)
+
+    def timed(psearch, layout, steps, warmup):
+        ctx.rec_begin(BOX, INSERT_MAXR, layout=layout, rcut=INSERT_MAXR, wg_init=0.01)
+        for _ in range(warmup):
+            step(psearch)
+        ctx.sync()
+        ctx.timing(True)
+        ctx.counter_reset()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            out = step(psearch)
+        e1.record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_gather", "cl_rescore",
+                                                   "cl_fallback", "insert")}
+        cnt = {k: ctx.counter(k) for k in ("cl_search_samples", "insert_samples", "cl_fallback_particles",
+                                           "cl_table_pairs", "cl_gather_points")}
+        ctx.timing(False)
+        return out, e0.elapsed_time(e1), wall, kt, cnt
+
+    # chosen path, timed
+    out_c, ev_c, wall_c, kt_c, cnt_c = timed(p_chosen, COMPENSATED, args.steps, args.warmup)
+    # plain path on the same particles: one timed step after one untimed
+    out_p, ev_p, wall_p, kt_p, cnt_p = timed(p_plain, 0, 1, 1)
+    # one map per path from one pass over the batch, and their FSC
+    maps = []
+    for psearch_out, layout in ((out_c, COMPENSATED), (out_p, 0)):
+        ctx.rec_begin(BOX, INSERT_MAXR, layout=layout, rcut=INSERT_MAXR, wg_init=0.01)
+        Q = P.copy()
+        Q[:, 0:3] = psearch_out[:, 0:3]
+        ctx.rec_insert(p_insert, images, Q)
+        m = torch.empty((BOX, BOX, BOX), dtype=torch.float32, device="cuda:0")
+        ctx.rec_finish(BOX, m)
+        ctx.sync()
+        maps.append(m)
+    fsc = _fsc(maps[0], maps[1], INSERT_MAXR)
+    map_rel = float((torch.linalg.vector_norm(maps[0] - maps[1]) / torch.linalg.vector_norm(maps[1])).item())
     ctx.close()
-    ang_err = np.abs(((out[:, 0:3] - ps.angles + 180) % 360) - 180)
-    return dict(wall=wall, ev_ms=ev_ms, k_ms=k_ms, k_n=k_n, ins_ms=ins_ms, ins_n=ins_n, samples=samples,
-                ins_samples=ins_samples, finish_s=finish_s, n=n, d=d, ang_err=ang_err, shape=shape)
+    dang = np.abs(((out_c[:, 0:3] - out_p[:, 0:3] + 180) % 360) - 180).max(axis=1)
+    identical = int(np.all(out_c[:, [0, 1, 2, 5]] == out_p[:, [0, 1, 2, 5]], axis=1).sum())
+    within = int((dang <= STEP + 1e-4).sum())
+    ang_err = np.abs(((out_p[:, 0:3] - ps.angles + 180) % 360) - 180)
+    return dict(n=n, d=d, ev_c=ev_c, wall_c=wall_c, kt_c=kt_c, cnt_c=cnt_c, ev_p=ev_p, wall_p=wall_p, kt_p=kt_p,
+                cnt_p=cnt_p, identical=identical, within=within, fsc=fsc, map_rel=map_rel, ang_err=ang_err)
 
 
 def _grid_work(step: float) -> int:
@@ -243,6 +293,37 @@ def _port_baseline(args, tang, timgs, ps):
             "sample": f"C oracle (FP32) on 1 particle, search at {coarse} deg x{scale:.0f} + 1 insert"}
 
 
+def _roofline(r, args):
+    """Roofline of the kernel with the most time in a chosen step."""
+    kt, cnt, n = r["kt_c"], r["cnt_c"], r["n"]
+    tot = {k: v[0] * v[1] for k, v in kt.items() if v[1]}
+    dom = max(tot, key=tot.get)
+    ms, launches = kt[dom]
+    if dom == "insert":
+        per_launch = cnt["insert_samples"] / launches * 60 * INSERT_BYTES_PER_SAMPLE_OP * 2   # hi + lo
+        achieved = per_launch / (ms / 1e3) / 1e9
+        return dom, {"bound": "hbm", "kernel": "k_insert_brick (60-fold trilinear insert, compensated FP32)",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(per_launch),
+                     "bytes_rule": "inserted samples x 60 operators x 8 corners x read+write of fa, fb, wg (f32) "
+                                   "x 2 (hi and lo sums)", "avg_launch_ms": round(ms, 3), "launches": launches}
+    if dom in ("cl_rescore", "cl_search", "cl_fallback"):
+        samples = cnt["cl_search_samples"] / launches
+        achieved = samples * FLOPS_PER_SAMPLE / (ms / 1e3) / 1e12
+        return dom, {"bound": "mfma", "kernel": "k_cl_search (exact phase residual, FP32 VALU)",
+                     "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "avg_launch_ms": round(ms, 3), "launches": launches}
+    # table gather: one 4-byte table entry per (AU point, omega, template, common line)
+    per_launch = cnt["cl_gather_points"] / launches * args.templates * 60 * 4
+    achieved = per_launch / (ms / 1e3) / 1e9
+    return dom, {"bound": "hbm", "kernel": "k_cl_gather (table reads, mostly L2 hits)",
+                 "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                 "avg_launch_ms": round(ms, 3), "launches": launches}
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -253,10 +334,16 @@ def main():
     ph, tang, timgs, ps = make_workload(args.batch, args.templates)
     r = gpu_run(args, tang, timgs, ps)
     n = r["n"]
-    value = n * args.steps / (r["ev_ms"] / 1e3)
-    k_s = r["k_ms"] / 1e3
-    samples_per_launch = r["samples"] / max(1, r["k_n"])
-    achieved = samples_per_launch * FLOPS_PER_SAMPLE / k_s / 1e12 if k_s > 0 else 0.0
+    value = n * args.steps / (r["ev_c"] / 1e3)
+    plain = n / (r["ev_p"] / 1e3)
+    dom, roof = _roofline(r, args)
+    tj = os.path.join(ROOT, "profiles", "hbm_traffic_box384.json")
+    if os.path.exists(tj):   # rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated, per launch
+        with open(tj) as f:
+            tr = json.load(f)
+        key = {"insert": "k_insert_brick", "cl_gather": "k_cl_gather"}.get(dom, "k_cl_search")
+        if key in tr and tr[key].get("batch") == n:
+            roof["traffic"] = int(tr[key]["hbm_bytes_per_launch"])
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -264,42 +351,42 @@ def main():
         except Exception as e:
             cpu = {"value": None, "unit": "particles/s", "cores": 0, "kind": "port", "sample": f"failed: {e}"}
     d = r["d"]
-    traffic = None
-    tj = os.path.join(ROOT, "profiles", "hbm_traffic_box384.json")
-    if os.path.exists(tj):   # rocprofv3 FETCH_SIZE/WRITE_SIZE passes, copy-calibrated (per particle)
-        with open(tj) as f:
-            traffic = json.load(f)["k_cl_search"]["hbm_bytes_per_particle"] * n
+    fsc = r["fsc"]
+    kt = {k: {"avg_ms": round(v[0], 3), "launches": v[1]} for k, v in r["kt_c"].items() if v[1]}
     line = {
-        "metric": "particles/s search+insert, box 384: chosen vs plain FP32 GPU at FSC>=0.999",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "particles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["ev_ms"] / args.steps, 3),
+        "ms_per_step": round(r["ev_c"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded 60-fold Gaussian-blob phantom, analytic projections, CTF, SNR 0.05)",
-        "config": {"workload": f"plain FP32 method, box {BOX}: global common-lines search (0.5 deg AU grid x "
-                               f"720 omega, maxR {d['maxR']}, T={args.templates} templates) + 60-fold insert "
-                               f"(maxR {INSERT_MAXR})",
+        "config": {"workload": f"chosen path vs plain FP32 path, box {BOX}: global common-lines search (0.5 deg "
+                               f"AU grid x 720 omega, maxR {d['maxR']}, T={args.templates} templates; chosen: "
+                               f"tabulated ranking, top-{args.topk} exact re-score) + 60-fold insert (maxR "
+                               f"{INSERT_MAXR}; chosen: compensated FP32)",
                    "box": BOX, "batch": n, "templates": args.templates, "search_maxR": d["maxR"],
-                   "insert_maxR": INSERT_MAXR, "searchstep": STEP, "apix": APIX},
-        "roofline": {"bound": "mfma", "kernel": "k_cl_search (common-lines phase residual, FP32 VALU)",
-                     "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None if traffic is None else round(traffic),
-                     "samples_per_launch": int(samples_per_launch), "flops_per_sample": FLOPS_PER_SAMPLE,
-                     "avg_launch_ms": round(r["k_ms"], 3), "tuned_shape": r["shape"]},
+                   "insert_maxR": INSERT_MAXR, "searchstep": STEP, "apix": APIX, "topk": args.topk},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "extra": {"insert_avg_ms": round(r["ins_ms"], 3),
-                  "insert_GBps_algorithmic": round(r["ins_samples"] / max(1, r["ins_n"]) * 60 * 8 * 3 * 8
-                                                   / (r["ins_ms"] / 1e3) / 1e9, 1) if r["ins_ms"] else None,
-                  "rec_finish_s": round(r["finish_s"], 3),
-                  "wall_s": round(r["wall"], 3),
+        "extra": {"plain_particles_per_s": round(plain, 3),
+                  "chosen_over_plain": round(value / plain, 2),
+                  "plain_ms_per_step": round(r["ev_p"], 3),
+                  "orientation_identical": r["identical"], "orientation_within_one_step": r["within"],
+                  "orientation_misses": n - r["within"], "particles_compared": n,
+                  "fsc_min_to_r190": round(float(np.min(fsc[1:])), 6),
+                  "fsc_at_r190": round(float(fsc[-1]), 6),
+                  "map_rel_l2_chosen_vs_plain": r["map_rel"],
+                  "fallback_particles": int(r["cnt_c"]["cl_fallback_particles"]),
+                  "chosen_kernels": kt,
+                  "plain_search_avg_ms": round(r["kt_p"]["cl_search"][0], 3),
                   "search_angle_err_deg_median": float(np.median(r["ang_err"])),
+                  "wall_s": round(r["wall_c"], 3),
                   "setup_s": round(time.perf_counter() - t_setup, 1)},
     }
     print(json.dumps(line))

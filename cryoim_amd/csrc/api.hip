@@ -112,6 +112,7 @@ int cim_finalize(cim_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->plans) (void)hipFree(kv.second.tw);
     ctx->rec.acc.release();
+    ctx->rec.lo.release();
     ctx->rec.ops.release();
     ctx->rec.samples.release();
     ctx->rec.bricks.release();

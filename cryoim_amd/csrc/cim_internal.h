@@ -50,6 +50,8 @@ struct RecState {
     float rcut = 0.f, wg_init = 0.f;
     int dxy = 0, dz = 0;            // accumulator dims
     DevBuf acc;                     // float4 * dxy*dxy*dz
+    bool comp = false;              // compensated: lo holds the TwoSum residues of acc
+    DevBuf lo;                      // float4 * dxy*dxy*dz (compensated mode)
     DevBuf ops;                     // double[nops*9]
     DevBuf samples;                 // int2 (hx, ky) per half-plane sample
     int nsamples = 0;
@@ -63,9 +65,11 @@ struct RecState {
 // Per-batch working set of the searches.
 struct SearchBatch {
     DevBuf pcrop, pcrop4, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter;
+    // tabulated search: polar rays of particles / templates, residual tables, candidates
+    DevBuf ppol, tpol, table, part, cand, kwin, cv, ci;
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
-                          &fv, &fi, &counter})
+                          &fv, &fi, &counter, &ppol, &tpol, &table, &part, &cand, &kwin, &cv, &ci})
             b->release();
     }
 };
@@ -161,8 +165,8 @@ int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_ou
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
 // Fold {fa, fb, wg} (pitch dxy, z >= 0) -> complex volume -> inverse FFT ->
 // real map with checkerboard (inside radius rmask only when rmask >= 0).
-int fold_to_map_dev(cim_ctx* ctx, const float4* acc, int dxy, int box, int maxR, int int_radius, float isg,
-                    float rmask, float* map_dev);
+int fold_to_map_dev(cim_ctx* ctx, const float4* acc, const float4* lo, int dxy, int box, int maxR, int int_radius,
+                    float isg, float rmask, float* map_dev);
 
 // split <-> interleaved conversion (prep.hip)
 int interleave(cim_ctx* ctx, const float* re, const float* im, float2* out, long long count);

@@ -137,7 +137,8 @@ static int fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, 
     // whose real part is the backward transform of cmplx(fa, -fb).
     const float isg = variant == CIM_FSC_RECONSTRUCT ? 1.f : -1.f;
     const float rmask = variant == CIM_FSC_RECONSTRUCT ? -1.f : imgmask;
-    int rc = fold_to_map_dev(ctx, ctx->fsc_acc.as<float4>(), t, box, maxR, 1, isg, rmask, ctx->out.as<float>());
+    int rc = fold_to_map_dev(ctx, ctx->fsc_acc.as<float4>(), nullptr, t, box, maxR, 1, isg, rmask,
+                             ctx->out.as<float>());
     if (rc) return rc;
     // merge_reconstruct_FSC writes (box-8) planes of (box-8)^2 values from the
     // start of the flat array (:204-206, after maphead set FFTsize = box-8)

@@ -127,8 +127,8 @@ __global__ void k_fill_acc(float4* __restrict__ acc, long long count, float wg0)
 // The sphere test is dr <= maxR (reconstruct.f90) or, with int_radius,
 // int(dr) <= maxR (reconstruct_FSC.f90:101, merge_reconstruct.f90:42).
 // acc holds {fa, fb, wg} for x, y in [-maxR, .] (row pitch dxy) and z >= 0.
-__global__ void k_fold(const float4* __restrict__ acc, float2* __restrict__ vol, int box, int maxR, int dxy,
-                       int int_radius, float isg) {
+__global__ void k_fold(const float4* __restrict__ acc, const float4* __restrict__ lo, float2* __restrict__ vol,
+                       int box, int maxR, int dxy, int int_radius, float isg) {
     const long long n2 = (long long)box * box;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n2 * box) return;
@@ -140,27 +140,37 @@ __global__ void k_fold(const float4* __restrict__ acc, float2* __restrict__ vol,
         const float dr = sqrtf((float)(h * h + k * k + l * l));
         if (int_radius ? ((int)dr <= maxR) : (dr <= (float)maxR)) {
             const long long dxy2 = (long long)dxy * dxy;
-            auto at = [&](int hh, int kk, int ll) -> float4 {
-                return acc[(long long)ll * dxy2 + (long long)(kk + maxR) * dxy + (hh + maxR)];
+            // sums as double: hi + lo in the compensated mode
+            auto at = [&](int hh, int kk, int ll) -> double4 {
+                const long long i = (long long)ll * dxy2 + (long long)(kk + maxR) * dxy + (hh + maxR);
+                const float4 h = acc[i];
+                double4 r = make_double4(h.x, h.y, h.z, 0.0);
+                if (lo) {
+                    const float4 l = lo[i];
+                    r.x += l.x;
+                    r.y += l.y;
+                    r.z += l.z;
+                }
+                return r;
             };
             const float s = ((h + k + l) & 1) ? -1.f : 1.f;
             if (l != 0) {
                 // l > 0 stored directly; l < 0 is the Friedel mate of (-h,-k,-l)
                 const int sg = l > 0 ? 1 : -1;
-                float4 a = at(sg * h, sg * k, sg * l);
-                const double A = (double)a.x / (double)a.z, B = (double)a.y / (double)a.z;
+                const double4 a = at(sg * h, sg * k, sg * l);
+                const double A = a.x / a.z, B = a.y / a.z;
                 outv = make_float2((float)A * s, (l > 0 ? isg : -isg) * (float)B * s);
             } else if (h == 0 && k == 0) {
-                float4 a = at(0, 0, 0);
-                outv = make_float2((float)((double)a.x / (double)a.z), 0.f);
+                const double4 a = at(0, 0, 0);
+                outv = make_float2((float)(a.x / a.z), 0.f);
             } else {
                 // n1: k > 0, or k == 0 and h > 0; n2 = -n1
                 const bool is_n1 = (k > 0) || (k == 0 && h > 0);
                 const int h1 = is_n1 ? h : -h, k1 = is_n1 ? k : -k;
-                float4 a1 = at(h1, k1, 0), a2 = at(-h1, -k1, 0);
-                const double W = (double)a1.z + (double)a2.z;
-                const double A = ((double)a1.x + (double)a2.x) / W;
-                const double B = ((double)a1.y - (double)a2.y) / W;
+                const double4 a1 = at(h1, k1, 0), a2 = at(-h1, -k1, 0);
+                const double W = a1.z + a2.z;
+                const double A = (a1.x + a2.x) / W;
+                const double B = (a1.y - a2.y) / W;
                 outv = make_float2((float)A * s, (is_n1 ? isg : -isg) * (float)B * s);
             }
         }
@@ -209,8 +219,9 @@ __global__ void k_merge_crop(const float* __restrict__ map, float* __restrict__ 
 // the last plane is lost.  The accumulator keeps those corners in its extra
 // x = 2R+1 / y = 2R+1 cells, so tnf cell f gathers every accumulator cell
 // whose flat reference index is f (their weights minus the initial value).
-__global__ void k_tnf_io(float4* __restrict__ acc, float* __restrict__ fa, float* __restrict__ fb,
-                         float* __restrict__ wg, int maxR, int dxy, int add_into_acc, int wrap, float wg_init) {
+__global__ void k_tnf_io(float4* __restrict__ acc, const float4* __restrict__ lo, float* __restrict__ fa,
+                         float* __restrict__ fb, float* __restrict__ wg, int maxR, int dxy, int add_into_acc, int wrap,
+                         float wg_init) {
     const int t = 2 * maxR + 1;
     const long long total = (long long)t * t * (maxR + 1);
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,11 +236,22 @@ __global__ void k_tnf_io(float4* __restrict__ acc, float* __restrict__ fa, float
         a->z += wg[id];
         return;
     }
-    float4 v = *a;
+    // compensated mode: the sum is hi + lo, rounded once to float
+    auto cell = [&](long long i) {
+        float4 e = acc[i];
+        if (lo) {
+            const float4 l = lo[i];
+            e.x = (float)((double)e.x + l.x);
+            e.y = (float)((double)e.y + l.y);
+            e.z = (float)((double)e.z + l.z);
+        }
+        return e;
+    };
+    float4 v = cell((long long)lz * dxy2 + (long long)ky * dxy + hx);
     if (wrap) {
         // cells (x', y', z') with x' <= t, y' <= t and z' t^2 + y' t + x' = id
         auto add = [&](int x, int y, int z) {
-            const float4 e = acc[(long long)z * dxy2 + (long long)y * dxy + x];
+            const float4 e = cell((long long)z * dxy2 + (long long)y * dxy + x);
             v.x += e.x;
             v.y += e.y;
             v.z += e.z - wg_init;
@@ -286,6 +308,7 @@ __global__ void k_insert_vals(const float2* __restrict__ ft, int box, int maxR, 
 
 struct BrickArgs {
     float4* acc;
+    float4* lo;               // compensated mode: TwoSum residues (else nullptr)
     const float4* vals;       // n x (2maxR+1)(maxR+1); w = 1 where the sample is inserted
     const float* M;           // n x nops x 9
     const int* bricks;        // nb x 3 brick origins (accumulator indices)
@@ -383,12 +406,31 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
         const int X = bx0 + cx, Y = by0 + cy, Z = bz0 + cz;
         if (X >= g.dxy || Y >= g.dxy || Z >= a.dz) continue;
         if (sfa[i] == 0.f && sfb[i] == 0.f && swg[i] == 0.f) continue;
-        float4* o = a.acc + ((long long)Z * dxy * dxy + (long long)Y * dxy + X);
-        float4 v = *o;
-        v.x += sfa[i];
-        v.y += sfb[i];
-        v.z += swg[i];
-        *o = v;
+        const long long vi = (long long)Z * dxy * dxy + (long long)Y * dxy + X;
+        float4 v = a.acc[vi];
+        if (a.lo) {
+            // compensated: hi + lo carries the sum; TwoSum (Knuth) keeps the rounding error of each add
+            float4 l = a.lo[vi];
+            auto two_sum = [](float x, float y, float& err) {
+                const float s = __fadd_rn(x, y);
+                const float yy = __fsub_rn(s, x);
+                err = __fadd_rn(__fsub_rn(x, __fsub_rn(s, yy)), __fsub_rn(y, yy));
+                return s;
+            };
+            float e;
+            v.x = two_sum(v.x, sfa[i], e);
+            l.x = __fadd_rn(l.x, e);
+            v.y = two_sum(v.y, sfb[i], e);
+            l.y = __fadd_rn(l.y, e);
+            v.z = two_sum(v.z, swg[i], e);
+            l.z = __fadd_rn(l.z, e);
+            a.lo[vi] = l;
+        } else {
+            v.x += sfa[i];
+            v.y += sfb[i];
+            v.z += swg[i];
+        }
+        a.acc[vi] = v;
     }
 }
 
@@ -398,13 +440,17 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
     if (box < 8 || (box & 1)) return set_error(ctx, CIM_E_ARG, "box %d must be even and >= 8", box);
     if (maxR < 1 || maxR > box / 2 - 2)
         return set_error(ctx, CIM_E_ARG, "maxR %d outside [1, box/2-2] for box %d", maxR, box);
-    if (layout != 0 && layout != 1) return set_error(ctx, CIM_E_ARG, "layout must be 0 (full) or 1 (compact)");
+    const bool comp = (layout & CIM_REC_COMPENSATED) != 0;
+    layout &= ~CIM_REC_COMPENSATED;
+    if (layout != 0 && layout != 1)
+        return set_error(ctx, CIM_E_ARG, "layout must be 0 (full) or 1 (compact), optionally | CIM_REC_COMPENSATED");
     if (rcut > (float)maxR) return set_error(ctx, CIM_E_ARG, "rcut %.2f > maxR %d", rcut, maxR);
     if (nops < 1 || nops > 360) return set_error(ctx, CIM_E_ARG, "nops %d outside [1, 360]", nops);
     RecState& r = ctx->rec;
     r.box = box;
     r.maxR = maxR;
     r.layout = layout;
+    r.comp = comp;
     r.rcut = rcut;
     r.wg_init = wg_init;
     r.nops = nops;
@@ -416,6 +462,12 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
     hipLaunchKernelGGL(k_fill_acc, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream, r.acc.as<float4>(),
                        nvox, wg_init);
     CIM_HIP(hipGetLastError());
+    if (comp) {
+        CIM_ALLOC(r.lo, sizeof(float4) * nvox);
+        hipLaunchKernelGGL(k_fill_acc, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream,
+                           r.lo.as<float4>(), nvox, 0.f);
+        CIM_HIP(hipGetLastError());
+    }
     std::vector<double> hops(9 * nops);
     if (ops) {
         for (int i = 0; i < 9 * nops; ++i) hops[i] = ops[i];
@@ -505,7 +557,7 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
     CtfConst cc{};
     if (p) cc = make_ctf_const(p);
     const char* mode = getenv("CIM_INSERT");
-    if (mode && strcmp(mode, "scatter") == 0) {
+    if (mode && strcmp(mode, "scatter") == 0 && !r.comp) {
         TimedLaunch tl(ctx, "insert");
         hipLaunchKernelGGL(k_insert, dim3((r.nsamples + 255) / 256, n), dim3(256), 0, ctx->stream, r.acc.as<float4>(),
                            ft, r.samples.as<int2>(), r.nsamples, ctx->aux2.as<float>(), parts_dev, ctf_img_dev,
@@ -519,6 +571,7 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
         CIM_HIP(hipGetLastError());
         BrickArgs ba;
         ba.acc = r.acc.as<float4>();
+        ba.lo = r.comp ? r.lo.as<float4>() : nullptr;
         ba.vals = r.vals.as<float4>();
         ba.M = ctx->aux2.as<float>();
         ba.bricks = r.bricks.as<int>();
@@ -547,8 +600,9 @@ int rec_tnf_io(cim_ctx* ctx, float* fa, float* fb, float* wg, int add) {
         CIM_HIP(hipMemcpyAsync(d + total, fb, sizeof(float) * total, hipMemcpyHostToDevice, ctx->stream));
         CIM_HIP(hipMemcpyAsync(d + 2 * total, wg, sizeof(float) * total, hipMemcpyHostToDevice, ctx->stream));
     }
-    hipLaunchKernelGGL(k_tnf_io, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, r.acc.as<float4>(), d,
-                       d + total, d + 2 * total, r.maxR, r.dxy, add, r.layout == 1 ? 1 : 0, r.wg_init);
+    hipLaunchKernelGGL(k_tnf_io, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, r.acc.as<float4>(),
+                       r.comp ? r.lo.as<float4>() : (const float4*)nullptr, d, d + total, d + 2 * total, r.maxR, r.dxy,
+                       add, r.layout == 1 ? 1 : 0, r.wg_init);
     CIM_HIP(hipGetLastError());
     if (!add) {
         CIM_HIP(hipMemcpyAsync(fa, d, sizeof(float) * total, hipMemcpyDeviceToHost, ctx->stream));
@@ -562,15 +616,16 @@ int rec_tnf_io(cim_ctx* ctx, float* fa, float* fb, float* wg, int add) {
 // Fold {fa, fb, wg} sums (row pitch dxy, z >= 0) into the full complex
 // volume, inverse 3D FFT, real part with the checkerboard (everywhere, or
 // only inside radius rmask when rmask >= 0, zero outside).
-int fold_to_map_dev(cim_ctx* ctx, const float4* acc, int dxy, int box, int maxR, int int_radius, float isg,
-                    float rmask, float* map_dev) {
+int fold_to_map_dev(cim_ctx* ctx, const float4* acc, const float4* lo, int dxy, int box, int maxR, int int_radius,
+                    float isg, float rmask, float* map_dev) {
     const long long nvox = (long long)box * box * box;
     const unsigned nb = (unsigned)((nvox + 255) / 256);
     CIM_ALLOC(ctx->ft2, sizeof(float2) * nvox);
     float2* vol = ctx->ft2.as<float2>();
     {
         TimedLaunch tl(ctx, "fold");
-        hipLaunchKernelGGL(k_fold, dim3(nb), dim3(256), 0, ctx->stream, acc, vol, box, maxR, dxy, int_radius, isg);
+        hipLaunchKernelGGL(k_fold, dim3(nb), dim3(256), 0, ctx->stream, acc, lo, vol, box, maxR, dxy, int_radius,
+                           isg);
         CIM_HIP(hipGetLastError());
     }
     int rc = fft3d_dev(ctx, vol, box, -1, 0);
@@ -590,7 +645,8 @@ int rec_merge_dev(cim_ctx* ctx, float* out_dev) {
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
     const long long nvox = (long long)r.box * r.box * r.box;
     CIM_ALLOC(ctx->aux, sizeof(float) * nvox);
-    int rc = fold_to_map_dev(ctx, r.acc.as<float4>(), r.dxy, r.box, r.maxR, 1, -1.f, -1.f, ctx->aux.as<float>());
+    int rc = fold_to_map_dev(ctx, r.acc.as<float4>(), r.comp ? r.lo.as<float4>() : nullptr, r.dxy, r.box, r.maxR, 1,
+                             -1.f, -1.f, ctx->aux.as<float>());
     if (rc) return rc;
     const long long m3 = (long long)(r.box - 8) * (r.box - 8) * (r.box - 8);
     hipLaunchKernelGGL(k_merge_crop, dim3((unsigned)((m3 + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -602,7 +658,8 @@ int rec_merge_dev(cim_ctx* ctx, float* out_dev) {
 int rec_finish_dev(cim_ctx* ctx, float* map_dev) {
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
-    return fold_to_map_dev(ctx, r.acc.as<float4>(), r.dxy, r.box, r.maxR, 0, -1.f, -1.f, map_dev);
+    return fold_to_map_dev(ctx, r.acc.as<float4>(), r.comp ? r.lo.as<float4>() : nullptr, r.dxy, r.box, r.maxR, 0,
+                           -1.f, -1.f, map_dev);
 }
 
 }  // namespace cim

@@ -17,6 +17,7 @@
 // together by cal_crosslin_PR (cal_crosslin_PR.f90:22-68) with the trial
 // centre shift applied to the taps.
 #include <cmath>
+#include <map>
 
 #include "cim_device.h"
 
@@ -159,6 +160,10 @@ struct SearchArgs {
     int* outi;
     float* prfull;            // optional n x npts x Nomega
     unsigned long long* nsamples;   // optional: evaluated (line, radius, omega) samples
+    const int* cand;          // optional n x ncand AU point indices: score only these (grid.x = ncand)
+    int ncand;
+    const int* kwin;          // optional n x ncand first column of a window of wincols columns (mod ncol)
+    int wincols;
 };
 
 // Particle taps: MODE 0 reads the float2 disk staged in LDS, MODE 1 reads
@@ -187,7 +192,9 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     float* redv = reinterpret_cast<float*>(rbase + a.nrows);
     int* redi = reinterpret_cast<int*>(redv + 16);
 
-    const int pt = blockIdx.x, p = blockIdx.y;
+    const int slot = blockIdx.x, p = blockIdx.y;
+    const int pt = a.cand ? a.cand[(long long)p * a.ncand + slot] : slot;
+    const long long oslot = (long long)p * gridDim.x + slot;
     const int tid = threadIdx.x, bd = blockDim.x;
     const float2* pc = a.pcrop + (long long)p * a.Dc * a.Dc;
     const float4* pc4 = a.pcrop4 + (long long)p * a.Dc * a.Dc;
@@ -195,6 +202,10 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     const float cf = (float)ic;
     const float d2r = 0.0174532925199432957692f;
     const int ncol = PAIR ? a.Nomega / 2 : a.Nomega;
+    // columns this workgroup scores: all, or a window starting at kbase (wrapping); the
+    // arithmetic of a column does not depend on which workgroup or lane computes it
+    const int kbase = a.kwin ? a.kwin[(long long)p * a.ncand + slot] : 0;
+    const int nact = a.kwin ? a.wincols : ncol;
 
     for (int i = tid; i < a.nr; i += bd) fc0[i] = a.fcut0[(long long)p * a.nr + i];
     if (MODE == 0) {
@@ -247,7 +258,8 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
                 float cs[VPT], sn[VPT];
 #pragma unroll
                 for (int v = 0; v < VPT; ++v) {
-                    const int k = tid + v * bd;
+                    int k = kbase + tid + v * bd;
+                    if (k >= ncol) k -= ncol;
                     const float om = (a1 - a.step * (float)k) * d2r;
                     cs[v] = cosf(om);
                     sn[v] = sinf(om);
@@ -261,8 +273,7 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
                     const float f0 = tv.w;
 #pragma unroll
                     for (int v = 0; v < VPT; ++v) {
-                        const int k = tid + v * bd;
-                        if (k >= ncol) continue;
+                        if (tid + v * bd >= nact) continue;
                         const float x = FR * cs[v] + cf, y = FR * sn[v] + cf;
                         const int lx = (int)x, ly = (int)y;
                         const float ax = x - (float)lx, ay = y - (float)ly;
@@ -306,14 +317,15 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     int bidx = 0x7fffffff;
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
-        const int k = tid + v * bd;
-        if (k < ncol) {
+        int k = kbase + tid + v * bd;
+        if (k >= ncol) k -= ncol;
+        if (tid + v * bd < nact) {
 #pragma unroll
             for (int h = 0; h < (PAIR ? 2 : 1); ++h) {
                 const float nn = h ? num2[v] : num[v], dd = h ? den2[v] : den[v];
                 const int kk = k + h * ncol;
                 const float pr = dd > 1.0e-10f ? nn / dd * r2d : 999.f;
-                if (a.prfull) a.prfull[((long long)p * a.npts + pt) * a.Nomega + kk] = pr;
+                if (a.prfull && !a.cand) a.prfull[((long long)p * a.npts + pt) * a.Nomega + kk] = pr;
                 const int li = a.pbase[pt] + kk;
                 if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
             }
@@ -333,8 +345,8 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
         const int nw = (bd + 63) >> 6;
         for (int i = 1; i < nw; ++i)
             if (redv[i] < best || (redv[i] == best && redi[i] < bidx)) { best = redv[i]; bidx = redi[i]; }
-        a.outv[(long long)p * a.npts + pt] = best;
-        a.outi[(long long)p * a.npts + pt] = bidx;
+        a.outv[oslot] = best;
+        a.outi[oslot] = bidx;
         if (a.nsamples) atomicAdd(a.nsamples, pairs * (unsigned long long)a.Nomega);
     }
 }
@@ -376,6 +388,331 @@ __global__ void k_argmin(const float* __restrict__ v, const int* __restrict__ ix
             if (sv[i] < best || (sv[i] == best && si[i] < bidx)) { best = sv[i]; bidx = si[i]; }
         ov[p] = best;
         oi[p] = bidx;
+    }
+}
+
+// ---- tabulated search ------------------------------------------------------
+//
+// The phase residual of cal_crosslin_PR_omega.f90:48-69 sums, over common
+// lines and radii, a term that depends only on the particle ray and the
+// template ray the common line picks.  For each (particle, template) the
+// tables hold that sum over radii for every pair of rays on a polar grid of
+// step dpsi = searchstep / sub:
+//   N(jT, jP) = sum_r [gates] |fold(phase_T - phase_P)| (|F_T| + |F_P|) / 2
+//   D(jT, jP) = sum_r [gates] (|F_T| + |F_P|) / 2
+// (gates: |F_T| >= template Fcut, |F_P| >= particle Fcut, as the reference).
+// The score of (AU point, omega_k) is then sum N / sum D over the common
+// lines, with the template ray round(alpha2 / dpsi) and the particle ray
+// round(alpha1 / dpsi) - k * sub.  Rays are sampled exactly as the exact
+// kernel samples them; the only approximation is the rounding of the
+// common-line angles to the polar grid.
+
+constexpr float TWO_PI_F = 6.28318530717958647692f;
+constexpr float GATED = -1.0e30f;
+
+// Polar rays: out[item][fr][j] = {phase, |F|/2 or GATED}, j * dpsi degrees,
+// radius minR + fr, bilinear on the crop as the exact kernel.
+__global__ void k_polar(const float2* __restrict__ crops, long long crop_stride, int Dc, int R, int box, int minR,
+                        int nr, int npsi, float dpsi, const float* __restrict__ gate, long long gate_stride,
+                        float2* __restrict__ out) {
+    const int item = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nr * npsi) return;
+    const int fr = q / npsi, j = q - fr * npsi;
+    const float d2r = 0.0174532925199432957692f;
+    const int ic = box / 2;
+    const float cf = (float)ic;
+    const float FR = (float)(minR + fr);
+    const float ang = ((float)j * dpsi) * d2r;
+    const float x = FR * cosf(ang) + cf, y = FR * sinf(ang) + cf;
+    const float2 z = crop_bilin(crops + (long long)item * crop_stride, Dc, R, x, y, ic);
+    const float amp = sqrtf(z.x * z.x + z.y * z.y);
+    const float h = amp >= gate[(long long)item * gate_stride + fr] ? 0.5f * amp : GATED;
+    out[(long long)item * nr * npsi + q] = make_float2(atan2f(z.y, z.x), h);
+}
+
+// bfloat16 of a finite non-negative float, rounded to nearest even.
+__device__ __forceinline__ unsigned int bf16_bits(float x) {
+    unsigned int u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+// Residual tables, one 64 x 64 tile of (jT, jP) per workgroup, 4 x 4 per
+// thread; radii staged through LDS in chunks.  table[item][jT][jP] holds N
+// (low half) and D (high half) as bfloat16, item = p * T + t: the tables
+// only rank AU points (their 8-bit mantissas perturb a score by ~1e-4 of
+// itself, far below the effect of rounding the angles to the grid).
+// Rays 180 deg apart carry conjugate samples (real images, CTF and the
+// centre-shift ramp keep the transforms Hermitian), and turning both rays of
+// a pair by 180 deg leaves |phase difference| and the weights unchanged:
+// N(jT + npsi/2, jP + npsi/2) = N(jT, jP).  So only jT < npsi/2 is stored.
+constexpr int TAB_RC = 16;
+__global__ void __launch_bounds__(256) k_cl_table(const float2* __restrict__ ppol, const float2* __restrict__ tpol,
+                                                  int tpol_per_particle, int T, int nr, int npsi,
+                                                  unsigned int* __restrict__ table) {
+    __shared__ float2 sT[TAB_RC][64];
+    __shared__ float2 sP[TAB_RC][64];
+    const int item = blockIdx.z, p = item / T, t = item - p * T;
+    const float2* P = ppol + (long long)p * nr * npsi;
+    const float2* Tm = tpol + (long long)(tpol_per_particle ? item : t) * nr * npsi;
+    const int jP0 = blockIdx.x * 64, jT0 = blockIdx.y * 64;
+    const int nrow = npsi / 2;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    float N[4][4], D[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { N[i][j] = 0.f; D[i][j] = 0.f; }
+    for (int r0 = 0; r0 < nr; r0 += TAB_RC) {
+        for (int e = tid; e < TAB_RC * 64; e += 256) {
+            const int rr = e >> 6, c = e & 63, r = r0 + rr;
+            const int jt = jT0 + c, jp = jP0 + c;
+            sT[rr][c] = (r < nr && jt < nrow) ? Tm[(long long)r * npsi + jt] : make_float2(0.f, GATED);
+            sP[rr][c] = (r < nr && jp < npsi) ? P[(long long)r * npsi + jp] : make_float2(0.f, GATED);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int rr = 0; rr < TAB_RC; ++rr) {
+            float2 tv[4], pv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) tv[i] = sT[rr][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) pv[j] = sP[rr][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float d = fabsf(tv[i].x - pv[j].x);
+                    d = fminf(d, TWO_PI_F - d);
+                    const float w = fmaxf(tv[i].y + pv[j].y, 0.f);
+                    N[i][j] = fmaf(d, w, N[i][j]);
+                    D[i][j] += w;
+                }
+        }
+        __syncthreads();
+    }
+    unsigned int* out = table + (long long)item * nrow * npsi;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int jt = jT0 + ty * 4 + i;
+        if (jt >= nrow) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int jp = jP0 + tx * 4 + j;
+            if (jp < npsi) out[(long long)jt * npsi + jp] = bf16_bits(N[i][j]) | (bf16_bits(D[i][j]) << 16);
+        }
+    }
+}
+
+struct GatherArgs {
+    const unsigned int* table;   // n x T x (npsi/2) x npsi, {N, D} as bfloat16
+    const float* tang;        // T x 3
+    const float* pts;         // npts x 2
+    const int* pbase;
+    const float* ops;         // nops x 9
+    int T, nops, npts, npsi, sub, Nomega;
+    float dpsi;
+    int n;                    // particles
+    float2* part;             // n x T x npts x Nomega: per-template {N, D} sums
+    float* outv;              // n x npts: best table score of the AU point
+    int* outi;                // its linear index
+};
+
+__device__ __forceinline__ int wrap_index(int j, int n) {
+    j %= n;
+    return j < 0 ? j + n : j;
+}
+
+// One workgroup per (AU point, particle, template), threads over omega: the
+// template's share of the residual sums per omega goes to part[p][t][pt][k].
+// Workgroups are dealt to the 8 XCDs in turn (workgroup L on XCD L mod 8).
+// When T divides 8, XCD x only ever scores template x mod T (for particles
+// of class x / T), so its L2 holds a single 1 MB table at a time.
+template <int VPT>
+__global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
+    __shared__ int srow[64], sj0[64];
+    __shared__ int snv;
+    const long long L = blockIdx.x;
+    int p, t, pt;
+    if (8 % a.T == 0) {
+        const int x = (int)(L & 7), C = 8 / a.T;
+        const long long r8 = L >> 3;
+        t = x % a.T;
+        pt = (int)(r8 % a.npts);
+        p = x / a.T + C * (int)(r8 / a.npts);
+    } else {
+        pt = (int)(L % a.npts);
+        const long long rem = L / a.npts;
+        t = (int)(rem % a.T);
+        p = (int)(rem / a.T);
+    }
+    if (p >= a.n) return;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int nrow = a.npsi >> 1;
+    // the common lines of this (AU point, template) in operator order, compacted to
+    // (row offset, first column); the operators fit one wave
+    if (tid < 64) {
+        bool ok = false;
+        int jt = 0, j0 = 0;
+        if (tid < a.nops) {
+            float E1[9], E2[9], a1, a2;
+            euler_f(a.pts[2 * pt], a.pts[2 * pt + 1], 0.f, E1);
+            euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
+            crosslin_one(E1, E2, a.ops + 9 * tid, &a1, &a2);
+            ok = a1 < 999.f && a2 < 999.f;
+            jt = wrap_index((int)rintf(a2 / a.dpsi), a.npsi);
+            j0 = wrap_index((int)rintf(a1 / a.dpsi), a.npsi);
+            if (jt >= nrow) {   // N(jT, jP) = N(jT - npsi/2, jP - npsi/2)
+                jt -= nrow;
+                j0 += nrow;
+                if (j0 >= a.npsi) j0 -= a.npsi;
+            }
+        }
+        const unsigned long long mask = __ballot(ok);
+        if (ok) {
+            const int slot = __popcll(mask & ((1ull << tid) - 1ull));
+            srow[slot] = jt * a.npsi;
+            sj0[slot] = j0;
+        }
+        if (tid == 0) snv = __popcll(mask);
+    }
+    __syncthreads();
+    const int nv = snv;
+    const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * a.npsi;
+    float num[VPT], den[VPT];
+    int kk[VPT];
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+        num[v] = 0.f;
+        den[v] = 0.f;
+        kk[v] = (tid + v * bd) * a.sub;
+    }
+    constexpr int G = 8;   // common lines whose loads are issued together
+    int j = 0;
+    for (; j + G <= nv; j += G) {
+        unsigned int e[G][VPT];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const unsigned int* row = tab + srow[j + g];
+            const int j0 = sj0[j + g];
+#pragma unroll
+            for (int v = 0; v < VPT; ++v) {
+                int c = j0 - kk[v];
+                c += c < 0 ? a.npsi : 0;
+                c = c < 0 ? 0 : c;   // lanes past Nomega (k * sub >= npsi) read column 0, unused
+                e[g][v] = row[c];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int v = 0; v < VPT; ++v) {
+                num[v] += __uint_as_float(e[g][v] << 16);
+                den[v] += __uint_as_float(e[g][v] & 0xffff0000u);
+            }
+    }
+    for (; j < nv; ++j) {
+        const unsigned int* row = tab + srow[j];
+        const int j0 = sj0[j];
+#pragma unroll
+        for (int v = 0; v < VPT; ++v) {
+            int c = j0 - kk[v];
+            c += c < 0 ? a.npsi : 0;
+            c = c < 0 ? 0 : c;
+            const unsigned int e = row[c];
+            num[v] += __uint_as_float(e << 16);
+            den[v] += __uint_as_float(e & 0xffff0000u);
+        }
+    }
+    float2* out = a.part + ((long long)(p * a.T + t) * a.npts + pt) * a.Nomega;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+        const int k = tid + v * bd;
+        if (k < a.Nomega) out[k] = make_float2(num[v], den[v]);
+    }
+}
+
+// Sum the template shares (in template order), score, and keep the first
+// minimum over omega of each (AU point, particle).
+template <int VPT>
+__global__ void __launch_bounds__(256) k_cl_combine(GatherArgs a) {
+    __shared__ float redv[4];
+    __shared__ int redi[4];
+    const int pt = blockIdx.x, p = blockIdx.y;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const float r2d = 57.2957795130823208768f;
+    float best = 3.0e38f;
+    int bidx = 0x7fffffff;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+        const int k = tid + v * bd;
+        if (k >= a.Nomega) continue;
+        float nn = 0.f, dd = 0.f;
+        for (int t = 0; t < a.T; ++t) {
+            const float2 e = a.part[((long long)(p * a.T + t) * a.npts + pt) * a.Nomega + k];
+            nn += e.x;
+            dd += e.y;
+        }
+        const float pr = dd > 1.0e-10f ? nn / dd * r2d : 999.f;
+        const int li = a.pbase[pt] + k;
+        if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bidx, o, 64);
+        if (ov < best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+    }
+    if ((tid & 63) == 0) { redv[tid >> 6] = best; redi[tid >> 6] = bidx; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 1; i < (bd >> 6); ++i)
+            if (redv[i] < best || (redv[i] == best && redi[i] < bidx)) { best = redv[i]; bidx = redi[i]; }
+        a.outv[(long long)p * a.npts + pt] = best;
+        a.outi[(long long)p * a.npts + pt] = bidx;
+    }
+}
+
+// The K AU points of lowest table score per particle (ties: lower linear
+// index), in increasing order, and for each the first column of a window of
+// wincols columns (of ncol, the columns per omega half when paired) centred
+// on the omega of its table minimum.
+__global__ void k_topk(const float* __restrict__ v, const int* __restrict__ ix, int npts, int K, int* __restrict__ cand,
+                       int Nomega, int ncol, int wincols, int* __restrict__ kwin) {
+    __shared__ int chosen[256];
+    __shared__ float sv[16];
+    __shared__ int si[16], sp[16];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    for (int kk = 0; kk < K; ++kk) {
+        float best = 3.0e38f;
+        int bidx = 0x7fffffff, bpt = -1;
+        for (int i = tid; i < npts; i += blockDim.x) {
+            bool taken = false;
+            for (int c = 0; c < kk; ++c) taken |= chosen[c] == i;
+            if (taken) continue;
+            const float x = v[(long long)p * npts + i];
+            const int li = ix[(long long)p * npts + i];
+            if (x < best || (x == best && li < bidx)) { best = x; bidx = li; bpt = i; }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float xv = __shfl_xor(best, o, 64);
+            const int xi = __shfl_xor(bidx, o, 64);
+            const int xp = __shfl_xor(bpt, o, 64);
+            if (xv < best || (xv == best && xi < bidx)) { best = xv; bidx = xi; bpt = xp; }
+        }
+        if ((tid & 63) == 0) { sv[tid >> 6] = best; si[tid >> 6] = bidx; sp[tid >> 6] = bpt; }
+        __syncthreads();
+        if (tid == 0) {
+            for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+                if (sv[i] < best || (sv[i] == best && si[i] < bidx)) { best = sv[i]; bidx = si[i]; bpt = sp[i]; }
+            chosen[kk] = bpt < 0 ? 0 : bpt;
+            cand[(long long)p * K + kk] = chosen[kk];
+            const int kb = (bidx == 0x7fffffff ? 0 : bidx % Nomega) % ncol;
+            kwin[(long long)p * K + kk] = ((kb - wincols / 2) % ncol + ncol) % ncol;
+        }
+        __syncthreads();
     }
 }
 
@@ -481,7 +818,7 @@ static int prep_templates(cim_ctx* ctx, const cim_params* p, int T, const float*
 
 
 static int launch_search(cim_ctx* ctx, SearchArgs& a, int n, int lds_bytes, int mode, bool pair, int vpt, int bd) {
-    dim3 grid(a.npts, n);
+    dim3 grid(a.cand ? a.ncand : a.npts, n);
 #define CIM_LAUNCH(V, M, P)                                                                                      \
     do {                                                                                                         \
         CIM_HIP(hipFuncSetAttribute((const void*)k_cl_search<V, M, P>, hipFuncAttributeMaxDynamicSharedMemorySize, \
@@ -630,7 +967,37 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                            ctx->stream));
     CIM_HIP(hipMemcpyAsync(sb.tang.ptr, s.tang.data(), sizeof(float) * 3 * T, hipMemcpyHostToDevice, ctx->stream));
 
-    const int batch = std::max(1, std::min(n, 256));
+    // tabulated search (CIM_SEARCH_TABLES): polar grid of searchstep / sub
+    const bool tables = p->search_method == CIM_SEARCH_TABLES && !pr_array;
+    if (p->search_method != CIM_SEARCH_PLAIN && p->search_method != CIM_SEARCH_TABLES)
+        return set_error(ctx, CIM_E_ARG, "search_method %d unknown", p->search_method);
+    int sub = 1;
+    if (const char* es = getenv("CIM_TABLE_SUB")) sub = std::max(1, std::min(4, atoi(es)));
+    const int npsi = g.No * sub;
+    const float dpsi = p->searchstep / (float)sub;
+    const int topk = std::min(std::min(p->search_topk > 0 ? p->search_topk : 32, npts), 256);
+    if (tables && g.No > 2048) return set_error(ctx, CIM_E_ARG, "searchstep %g too fine for the tables", p->searchstep);
+    // the polar grid must close on itself with rays 180 deg apart on it
+    if (tables && (std::fabs(p->searchstep * (double)g.No - 360.0) > 1e-3 || (npsi & 1)))
+        return set_error(ctx, CIM_E_ARG, "CIM_SEARCH_TABLES needs 360/searchstep to be an even integer (step %g)",
+                         p->searchstep);
+    // fallback to the exhaustive search when the exact minimum ranks in the last quarter of the
+    // candidates.  CIM_TABLE_FALLBACK=<r>: from rank r on; 0: never; negative: every particle.
+    int fb_rank = topk - topk / 4;
+    if (const char* ef = getenv("CIM_TABLE_FALLBACK")) {
+        const int v = atoi(ef);
+        fb_rank = v > 0 ? v : (v == 0 ? topk + 1 : 0);
+    }
+    std::map<int, int> base2pt;
+    for (int i = 0; i < npts; ++i) base2pt[g.base[i]] = i;
+    // exact re-scoring window: wincols columns per (paired) omega half around the table minimum
+    const int ncolx = pair ? g.No / 2 : g.No;
+    int wincols = 64;
+    if (const char* ew = getenv("CIM_TABLE_WINDOW")) wincols = atoi(ew) > 0 ? ((atoi(ew) + 63) / 64) * 64 : ncolx;
+    if (wincols > ncolx) wincols = ncolx;
+
+    // the tables take 4 T npsi^2 bytes per particle (8.3 MB at T = 4, 0.5 deg): batches of 64
+    const int batch = std::max(1, std::min(n, tables ? 64 : 256));
     CtfConst cc = make_ctf_const(p);
     for (int b0 = 0; b0 < n; b0 += batch) {
         const int nb = std::min(batch, n - b0);
@@ -725,16 +1092,88 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         ctx->counters["cl_search_vpt"] = sh.vpt;
         ctx->counters["cl_search_lds"] = sh.lds;
         ctx->counters["cl_search_pair"] = sh.pair;
+        const int bd_exact = block_for(pair ? g.No / 2 : g.No, sh.vpt);
+        int nscore = npts;   // AU points per particle whose exact scores the final argmin reads
+        if (tables) {
+            const int titems = p->apply_ctf ? nb * T : T;
+            CIM_ALLOC(sb.ppol, sizeof(float2) * (size_t)nb * nr * npsi);
+            CIM_ALLOC(sb.tpol, sizeof(float2) * (size_t)titems * nr * npsi);
+            {
+                TimedLaunch tl(ctx, "cl_polar");
+                const unsigned gx = (unsigned)((nr * npsi + 255) / 256);
+                hipLaunchKernelGGL(k_polar, dim3(gx, nb), dim3(256), 0, ctx->stream, sb.pcrop.as<float2>(),
+                                   (long long)Dc * Dc, Dc, R, box, p->minR, nr, npsi, dpsi, sb.fcut0.as<float>(),
+                                   (long long)nr, sb.ppol.as<float2>());
+                CIM_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_polar, dim3(gx, titems), dim3(256), 0, ctx->stream, sb.tcrop.as<float2>(),
+                                   (long long)Dc * Dc, Dc, R, box, p->minR, nr, npsi, dpsi, sb.tfcut.as<float>(),
+                                   (long long)nr, sb.tpol.as<float2>());
+                CIM_HIP(hipGetLastError());
+            }
+            CIM_ALLOC(sb.table, sizeof(unsigned int) * (size_t)nb * T * (npsi / 2) * npsi);
+            {
+                TimedLaunch tl(ctx, "cl_table");
+                const unsigned nt = (unsigned)((npsi + 63) / 64), ntr = (unsigned)((npsi / 2 + 63) / 64);
+                hipLaunchKernelGGL(k_cl_table, dim3(nt, ntr, nb * T), dim3(256), 0, ctx->stream,
+                                   sb.ppol.as<float2>(), sb.tpol.as<float2>(), p->apply_ctf ? 1 : 0, T, nr, npsi,
+                                   sb.table.as<unsigned int>());
+                CIM_HIP(hipGetLastError());
+            }
+            GatherArgs ga{sb.table.as<unsigned int>(), sb.tang.as<float>(), sb.pts.as<float>(), sb.pbase.as<int>(),
+                          sb.ops.as<float>(), T, 60, npts, npsi, sub, g.No, dpsi, nb, nullptr,
+                          sb.outv.as<float>(), sb.outi.as<int>()};
+            CIM_ALLOC(sb.part, sizeof(float2) * (size_t)nb * T * npts * g.No);
+            ga.part = sb.part.as<float2>();
+            {
+                TimedLaunch tl(ctx, "cl_gather");
+                const int vg = (g.No + 255) / 256;
+                const long long nwg = (8 % T == 0) ? 8LL * npts * ((nb + 8 / T - 1) / (8 / T))
+                                                   : (long long)npts * nb * T;
+                const dim3 gg((unsigned)nwg);
+#define CIM_GATHER(V)                                                                       do {                                                                                        hipLaunchKernelGGL(k_cl_gather<V>, gg, dim3(256), 0, ctx->stream, ga);                  hipLaunchKernelGGL(k_cl_combine<V>, dim3(npts, nb), dim3(256), 0, ctx->stream, ga);     } while (0)
+                if (vg <= 1) CIM_GATHER(1);
+                else if (vg == 2) CIM_GATHER(2);
+                else if (vg == 3) CIM_GATHER(3);
+                else if (vg == 4) CIM_GATHER(4);
+                else if (vg <= 6) CIM_GATHER(6);
+                else CIM_GATHER(8);
+#undef CIM_GATHER
+                CIM_HIP(hipGetLastError());
+            }
+            CIM_ALLOC(sb.cand, sizeof(int) * (size_t)nb * topk);
+            CIM_ALLOC(sb.kwin, sizeof(int) * (size_t)nb * topk);
+            hipLaunchKernelGGL(k_topk, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(),
+                               npts, topk, sb.cand.as<int>(), g.No, ncolx, wincols, sb.kwin.as<int>());
+            CIM_HIP(hipGetLastError());
+            ctx->counters["cl_table_pairs"] += (long long)nb * T * npsi * npsi * nr;
+            ctx->counters["cl_gather_points"] += (long long)nb * npts * g.No;
+            a.cand = sb.cand.as<int>();
+            a.ncand = topk;
+            a.prfull = nullptr;
+            if (wincols < ncolx) {
+                a.kwin = sb.kwin.as<int>();
+                a.wincols = wincols;
+                // one-wave workgroups: a short line chunk keeps LDS small so many stay resident
+                // (the chunking does not change any column's accumulation order)
+                a.chunk = std::min(chunk, 6);
+            }
+            nscore = topk;
+        }
         {
-            TimedLaunch tl(ctx, "cl_search");
-            rc = launch_search(ctx, a, nb, sh.lds_bytes, sh.lds ? 0 : 1, sh.pair, sh.vpt,
-                               block_for(pair ? g.No / 2 : g.No, sh.vpt));
+            TimedLaunch tl(ctx, tables ? "cl_rescore" : "cl_search");
+            // windows: one wave of single-column lanes in the same kernel variant
+            if (a.kwin) {
+                const int lw = (int)(misc + sizeof(float4) * (size_t)a.chunk * nr) +
+                               (sh.lds ? (int)(sizeof(float2) * (size_t)disk_total) : 0);
+                rc = launch_search(ctx, a, nb, lw, sh.lds ? 0 : 1, sh.pair, 1, wincols);
+            }
+            else rc = launch_search(ctx, a, nb, sh.lds_bytes, sh.lds ? 0 : 1, sh.pair, sh.vpt, bd_exact);
             if (rc) return rc;
         }
         CIM_ALLOC(sb.fv, sizeof(float) * nb);
         CIM_ALLOC(sb.fi, sizeof(int) * nb);
-        hipLaunchKernelGGL(k_argmin, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(), npts,
-                           sb.fv.as<float>(), sb.fi.as<int>());
+        hipLaunchKernelGGL(k_argmin, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(),
+                           nscore, sb.fv.as<float>(), sb.fi.as<int>());
         CIM_HIP(hipGetLastError());
         std::vector<float> hv(nb);
         std::vector<int> hi(nb);
@@ -745,6 +1184,60 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             full.resize((size_t)npts * g.No);
             CIM_HIP(hipMemcpyAsync(full.data(), sb.prfull.ptr, sizeof(float) * full.size(), hipMemcpyDeviceToHost,
                                    ctx->stream));
+        }
+        std::vector<int> hc, hw;
+        if (tables) {
+            hc.resize((size_t)nb * topk);
+            hw.resize((size_t)nb * topk);
+            CIM_HIP(hipMemcpyAsync(hc.data(), sb.cand.ptr, sizeof(int) * hc.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            CIM_HIP(hipMemcpyAsync(hw.data(), sb.kwin.ptr, sizeof(int) * hw.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        }
+        CIM_HIP(hipStreamSynchronize(ctx->stream));
+        if (tables && fb_rank <= topk) {
+            // Fallback: when the exact minimum sits deep in the candidate list, the tables ranked this
+            // particle poorly; score every AU point exactly instead (the plain search for that particle).
+            for (int i = 0; i < nb; ++i) {
+                const int base = hi[i] - hi[i] % g.No;
+                const int pt = base2pt.count(base) ? base2pt[base] : -1;
+                int rank = topk;
+                for (int c = 0; c < topk; ++c)
+                    if (hc[(size_t)i * topk + c] == pt) { rank = c; break; }
+                bool edge = false;
+                if (a.kwin && rank < topk) {
+                    // exact minimum within 2 columns of its window edge: a lower one may lie outside
+                    const int kb = (hi[i] % g.No) % ncolx;
+                    const int off = ((kb - hw[(size_t)i * topk + rank]) % ncolx + ncolx) % ncolx;
+                    edge = off < 2 || off > wincols - 3;
+                }
+                if (rank < fb_rank && !edge) continue;
+                SearchArgs a1 = a;
+                a1.cand = nullptr;
+                a1.ncand = 0;
+                a1.pcrop += (long long)i * Dc * Dc;
+                a1.pcrop4 += (long long)i * Dc * Dc;
+                a1.fcut0 += (long long)i * nr;
+                a1.tcrop += (long long)i * a.tc_pstride;
+                a1.tfcut += (long long)i * a.tf_pstride;
+                CIM_ALLOC(sb.cv, sizeof(float) * npts);
+                CIM_ALLOC(sb.ci, sizeof(int) * npts);
+                a1.outv = sb.cv.as<float>();
+                a1.outi = sb.ci.as<int>();
+                {
+                    TimedLaunch tl(ctx, "cl_fallback");
+                    rc = launch_search(ctx, a1, 1, sh.lds_bytes, sh.lds ? 0 : 1, sh.pair, sh.vpt, bd_exact);
+                    if (rc) return rc;
+                }
+                hipLaunchKernelGGL(k_argmin, dim3(1), dim3(256), 0, ctx->stream, sb.cv.as<float>(), sb.ci.as<int>(),
+                                   npts, sb.fv.as<float>() + i, sb.fi.as<int>() + i);
+                CIM_HIP(hipGetLastError());
+                CIM_HIP(hipMemcpyAsync(&hv[i], sb.fv.as<float>() + i, sizeof(float), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+                CIM_HIP(hipMemcpyAsync(&hi[i], sb.fi.as<int>() + i, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                CIM_HIP(hipStreamSynchronize(ctx->stream));
+                ctx->counters["cl_fallback_particles"] += 1;
+            }
         }
         unsigned long long hcount = 0;
         CIM_HIP(hipMemcpyAsync(&hcount, sb.counter.ptr, sizeof(hcount), hipMemcpyDeviceToHost, ctx->stream));

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcryoim_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "cryoim.h")
 
 PART_STRIDE = 12
+SEARCH_PLAIN, SEARCH_TABLES = 0, 1
 P_THETA, P_PHI, P_OMEGA, P_X, P_Y, P_PR, P_DF1, P_DF2, P_ASTIG, P_MASKX, P_MASKY, P_USE = range(12)
 P_BOXX, P_BOXY = P_MASKX, P_MASKY
 
@@ -35,7 +36,8 @@ class CimParams(ctypes.Structure):
     _fields_ = [
         ("box", ctypes.c_int), ("minR", ctypes.c_int), ("maxR", ctypes.c_int),
         ("apply_ctf", ctypes.c_int), ("mask_on", ctypes.c_int), ("norm_on", ctypes.c_int),
-        ("ncycle", ctypes.c_int), ("conj_out", ctypes.c_int), ("reserved_i", ctypes.c_int * 8),
+        ("ncycle", ctypes.c_int), ("conj_out", ctypes.c_int), ("search_method", ctypes.c_int),
+        ("search_topk", ctypes.c_int), ("reserved_i", ctypes.c_int * 6),
         ("apix", ctypes.c_float), ("imgmask", ctypes.c_float), ("vol", ctypes.c_float),
         ("cs", ctypes.c_float), ("ampwg", ctypes.c_float), ("df_scale", ctypes.c_float),
         ("searchstep", ctypes.c_float), ("delta_angle", ctypes.c_float),
@@ -45,7 +47,7 @@ class CimParams(ctypes.Structure):
 
 def make_params(box, minR=0, maxR=None, apply_ctf=True, mask_on=True, norm_on=False, ncycle=6,
                 conj_out=True, apix=1.0, imgmask=None, vol=300.0, cs=2.7, ampwg=0.1, df_scale=1.0e4,
-                searchstep=2.0, delta_angle=0.0, delta_trans=0.0) -> CimParams:
+                searchstep=2.0, delta_angle=0.0, delta_trans=0.0, search_method=0, search_topk=0) -> CimParams:
     p = CimParams()
     p.box = int(box)
     p.minR = int(minR)
@@ -60,6 +62,8 @@ def make_params(box, minR=0, maxR=None, apply_ctf=True, mask_on=True, norm_on=Fa
     p.vol, p.cs, p.ampwg, p.df_scale = float(vol), float(cs), float(ampwg), float(df_scale)
     p.searchstep = float(searchstep)
     p.delta_angle, p.delta_trans = float(delta_angle), float(delta_trans)
+    p.search_method = int(search_method)
+    p.search_topk = int(search_topk)
     return p
 
 

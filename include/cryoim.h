@@ -53,7 +53,9 @@ typedef struct cim_params {
     int norm_on;      /* 1: zero-mean unit-RMS normalisation before masking */
     int ncycle;       /* refinement cycles (Ncycle) */
     int conj_out;     /* FFT stores b = -Im (1, icosprocess) or +Im (0, local_refine) */
-    int reserved_i[8];
+    int search_method;/* global search: CIM_SEARCH_PLAIN (0) or CIM_SEARCH_TABLES (1) */
+    int search_topk;  /* CIM_SEARCH_TABLES: AU points re-scored exactly per particle (0 = 32) */
+    int reserved_i[6];
     float apix;       /* pixel size (A) */
     float imgmask;    /* mask radius in pixels */
     float vol;        /* accelerating voltage (kV) */
@@ -65,6 +67,21 @@ typedef struct cim_params {
     float delta_trans;/* refinement shift step (px) */
     float reserved_f[7];
 } cim_params;
+
+/* Global search methods (cim_search_global):
+ * CIM_SEARCH_PLAIN   the reference's exhaustive search: every AU point x
+ *                    omega scored by cal_crosslin_PR_omega's phase residual.
+ * CIM_SEARCH_TABLES  the same residual tabulated per (particle, template)
+ *                    over all pairs of rays on a polar grid of step
+ *                    searchstep; every (AU point, omega) is scored from the
+ *                    tables (common-line angles rounded to the grid), the
+ *                    search_topk best AU points of each particle are scored
+ *                    again exactly as CIM_SEARCH_PLAIN does, and the first
+ *                    minimum among them is returned.  The result equals the
+ *                    plain one whenever the plain minimum lies in one of
+ *                    those AU points. */
+#define CIM_SEARCH_PLAIN 0
+#define CIM_SEARCH_TABLES 1
 
 /* Particle records: CIM_PART_STRIDE floats per particle. */
 #define CIM_PART_STRIDE 12
@@ -114,6 +131,11 @@ int cim_fft2d_dev(cim_ctx* ctx, void* data, int n, int batch, int dir, int conj_
 int cim_fft3d_dev(cim_ctx* ctx, void* data, int n, int dir, int conj_out);
 
 /* ---- direct Fourier inversion ---------------------------------------- */
+/* OR'd into cim_rec_begin's layout: accumulate compensated FP32 sums.  Each
+ * batch's contributions are summed in FP32 per 8^3 brick, then added to a
+ * hi/lo pair per voxel with TwoSum; fold, tnf readout and the map use hi+lo.
+ * Costs a second accumulator (the same size) and its read-modify-write. */
+#define CIM_REC_COMPENSATED 16
 /* layout: 0 = full box accumulator as reconstruct.f90 (offset box/2),
  *         1 = compact tnf accumulator as reconstruct_substep.f90 (offset maxR).
  * rcut: radius cut of inserted samples (maxR, maxR-1 or maxR-2 in the

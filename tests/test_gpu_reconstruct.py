@@ -158,3 +158,49 @@ def test_brick_gather_equals_scatter(ctx, insert_kernel, nops, n):
             assert err_l2 < 1e-5
             assert err_max <= 1e-4 * scale
             assert float(((a != 0) == (b != 0)).mean()) > 0.999
+
+
+COMPENSATED = 16   # CIM_REC_COMPENSATED
+
+
+def test_compensated_sums_do_not_drift(ctx, insert_kernel):
+    """256 one-particle batches of the same particle: the compensated sums stay at
+    256 x the single insert (float64 reference), the plain FP32 sums drift."""
+    if insert_kernel != "brick":
+        pytest.skip("compensation is a brick-insert feature")
+    box, maxR, reps = 64, 20, 256
+    rng = np.random.default_rng(8)
+    re = rng.standard_normal((1, box, box)).astype(np.float32)
+    im = rng.standard_normal((1, box, box)).astype(np.float32)
+    ctf = rng.uniform(0.2, 1.0, (1, box, box)).astype(np.float32)
+    eul = np.array([[73.0, 11.0, 41.0]], np.float32)
+    ctx.rec_begin(box, maxR, layout=1, rcut=maxR - 2, wg_init=0.0)
+    ctx.rec_insert_ft(re, im, eul, ctf)
+    one = [x.astype(np.float64) * reps for x in ctx.rec_get_tnf(maxR)]
+    errs = {}
+    for layout in (1, 1 | COMPENSATED):
+        ctx.rec_begin(box, maxR, layout=layout, rcut=maxR - 2, wg_init=0.0)
+        for _ in range(reps):
+            ctx.rec_insert_ft(re, im, eul, ctf)
+        got = ctx.rec_get_tnf(maxR)
+        errs[layout] = max(rel_l2(g, o) for g, o in zip(got, one))
+    print(f"after {reps} batches: plain FP32 rel-L2 {errs[1]:.2e}, compensated {errs[1 | COMPENSATED]:.2e}")
+    assert errs[1 | COMPENSATED] < 2e-7
+    assert errs[1 | COMPENSATED] < errs[1] / 4
+
+
+def test_compensated_map_matches_fp64_oracle(ctx, O64, insert_kernel):
+    if insert_kernel != "brick":
+        pytest.skip("compensation is a brick-insert feature")
+    box, maxR = 64, 24
+    ps = _data(box=box, n=20)
+    apix, mask_r = 1.5, box / 2 - 4
+    p = lib.make_params(box, 0, maxR, apply_ctf=True, mask_on=True, norm_on=True, apix=apix, imgmask=mask_r)
+    ctx.rec_begin(box, maxR, layout=COMPENSATED, rcut=maxR, wg_init=0.01)
+    for i in range(0, 20, 5):      # four batches
+        ctx.rec_insert(p, ps.images[i:i + 5], _parts(ps)[i:i + 5])
+    gmap = ctx.rec_finish(box)
+    vol = _oracle_volume(O64, ps, maxR, mask_r, apix)
+    err = rel_l2(gmap, O64.fold_and_map(vol, maxR))
+    print(f"compensated map vs FP64 oracle: rel-L2 {err:.2e}")
+    assert err < 1e-5

@@ -126,3 +126,52 @@ def test_local_refinement_matches_oracle(ctx, O64):
                                       np.array([*P[i, 0:5]], np.float64))
         assert np.allclose(out[i, 0:5], io5, atol=2e-3), (out[i, 0:5], io5)
         assert abs(out[i, 5] - pr) < 0.05
+
+
+def _search_pair(ctx, method, topk, box=96, maxR=20, minR=2, T=4, n=24, step=2.0, seed=31, monkeypatch=None):
+    ph, tang, tsh, timgs, ps = _setup(box=box, maxR=maxR, minR=minR, T=T, n=n, seed=seed, apix=2.0)
+    p = lib.make_params(box, minR, maxR, apply_ctf=True, mask_on=True, apix=2.0, imgmask=box / 2 - 4,
+                        searchstep=step, search_method=method, search_topk=topk)
+    ctx.search_templates(p, timgs, np.c_[tang, tsh])
+    ctx.counter_reset()
+    out = ctx.search_global(p, ps.images, _parts(ps))
+    return out, ps
+
+
+@pytest.mark.parametrize("variant_name", ["tuned", "no_pair", "lds"])
+def test_tabulated_search_equals_plain(ctx, monkeypatch, variant_name):
+    """CIM_SEARCH_TABLES re-scores its candidates with the exact kernel, so every
+    particle whose plain minimum is among them gets the plain result bit for bit;
+    on this set that is every particle (K = 32 of 198 AU points, 2 deg grid)."""
+    for k, v in VARIANTS[variant_name].items():
+        monkeypatch.setenv(k, v)
+    plain, ps = _search_pair(ctx, lib.SEARCH_PLAIN, 0)
+    tab, _ = _search_pair(ctx, lib.SEARCH_TABLES, 0)
+    same = np.all(tab[:, [0, 1, 2, 5]] == plain[:, [0, 1, 2, 5]], axis=1)
+    print(f"{variant_name}: identical {same.sum()} of {len(same)}, fallbacks {ctx.counter('cl_fallback_particles')}")
+    assert same.all()
+
+
+def test_tabulated_search_exhaustive_candidates(ctx, monkeypatch):
+    """With every AU point a candidate and full omega windows the tabulated search
+    is the plain search: identical output whatever the tables rank."""
+    monkeypatch.setenv("CIM_TABLE_WINDOW", "4096")
+    monkeypatch.setenv("CIM_TABLE_FALLBACK", "0")
+    plain, _ = _search_pair(ctx, lib.SEARCH_PLAIN, 0, n=6, step=3.0)
+    tab, _ = _search_pair(ctx, lib.SEARCH_TABLES, 256, n=6, step=3.0)
+    assert np.array_equal(tab[:, [0, 1, 2, 5]], plain[:, [0, 1, 2, 5]])
+
+
+def test_tabulated_search_fallback_is_plain(ctx, monkeypatch):
+    """The fallback path (forced for every particle) is the exhaustive search of
+    the plain method, particle by particle: identical output."""
+    monkeypatch.setenv("CIM_TABLE_FALLBACK", "-1")
+    plain, _ = _search_pair(ctx, lib.SEARCH_PLAIN, 0, n=8, step=3.0)
+    tab, _ = _search_pair(ctx, lib.SEARCH_TABLES, 2, n=8, step=3.0)
+    assert ctx.counter("cl_fallback_particles") == 8
+    assert np.array_equal(tab[:, [0, 1, 2, 5]], plain[:, [0, 1, 2, 5]])
+
+
+def test_tabulated_search_needs_closed_grid(ctx):
+    with pytest.raises(lib.CimError):
+        _search_pair(ctx, lib.SEARCH_TABLES, 0, n=2, step=0.7)
