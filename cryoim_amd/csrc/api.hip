@@ -118,6 +118,7 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->rec.bricks.release();
     ctx->rec.hxmax.release();
     ctx->rec.vals.release();
+    ctx->rec.maxv.release();
     ctx->srch.tft.release();
     ctx->srch.tfcut.release();
     ctx->srch.wk.release();

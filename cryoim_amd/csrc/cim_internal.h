@@ -59,6 +59,7 @@ struct RecState {
     DevBuf bricks;                  // int3 origins of the 8^3 bricks a sample can reach
     int nbricks = 0;
     DevBuf vals;                    // per-batch float4 sample values on the half-plane grid
+    DevBuf maxv;                    // largest |sample value| of the batch (float bits)
     long long inserted = 0;
 };
 

@@ -270,45 +270,61 @@ __global__ void k_tnf_io(float4* __restrict__ acc, const float4* __restrict__ lo
 //
 // The accumulator is cut into 8x8x8 bricks; one workgroup owns one brick in
 // LDS and gathers every (particle, operator) plane of the batch that passes
-// within reach of it: for each such plane it enumerates the half-plane
-// samples whose folded position can touch the brick (the plane coordinates
-// of the brick's box, widened by one sample), places them with the same
-// place() as the scatter kernel, and adds the corners that fall inside the
-// brick with LDS atomics.  The brick is then added to HBM once, so the
-// insert needs no global atomics at all.
+// within reach of it: for each such plane it enumerates the lattice points of
+// the plane inside the brick's reach box, row by row, places the samples
+// they stand for with the same place() as the scatter kernel, and adds the
+// corners that fall inside the brick with LDS atomics.  The brick is then
+// added to HBM once, so the insert needs no global atomics at all.
 
 constexpr int BRICK = 8;
 
 // Per-particle sample values {a*ctf, b*ctf, ctf^2} on the (hx, ky) half
 // plane grid hx in [0, maxR], ky in [-maxR, maxR].
 // A sample is inserted iff hx <= hxmax[ky + maxR] (the host's radius test).
-__global__ void k_insert_vals(const float2* __restrict__ ft, int box, int maxR, const int* __restrict__ hxmax,
-                              const float* __restrict__ parts, const float* __restrict__ ctf_img, int apply_ctf,
-                              CtfConst cc, float4* __restrict__ vals) {
+// maxbits collects the largest |value| of the batch (float bits; non-negative
+// floats order as unsigned integers).
+__global__ void __launch_bounds__(256) k_insert_vals(const float2* __restrict__ ft, int box, int maxR,
+                                                     const int* __restrict__ hxmax, const float* __restrict__ parts,
+                                                     const float* __restrict__ ctf_img, int apply_ctf, CtfConst cc,
+                                                     float4* __restrict__ vals, unsigned int* __restrict__ maxbits) {
+    __shared__ float red[4];
     const int p = blockIdx.y;
     const int w = maxR + 1, total = (2 * maxR + 1) * w;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= total) return;
-    const int hx = q % w, ky = q / w - maxR;
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (hx <= hxmax[ky + maxR]) {
-        const long long N = (long long)box * box;
-        const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
-        const float2 v = ft[(long long)p * N + i2];
-        float c = 1.f;
-        if (ctf_img) c = ctf_img[(long long)p * N + i2];
-        else if (apply_ctf) {
-            const float* pr = parts + (long long)p * CIM_PART_STRIDE;
-            c = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+    if (q < total) {
+        const int hx = q % w, ky = q / w - maxR;
+        if (hx <= hxmax[ky + maxR]) {
+            const long long N = (long long)box * box;
+            const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
+            const float2 v = ft[(long long)p * N + i2];
+            float c = 1.f;
+            if (ctf_img) c = ctf_img[(long long)p * N + i2];
+            else if (apply_ctf) {
+                const float* pr = parts + (long long)p * CIM_PART_STRIDE;
+                c = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+            }
+            out = make_float4(v.x * c, v.y * c, c * c, 1.f);
         }
-        out = make_float4(v.x * c, v.y * c, c * c, 1.f);
+        vals[(long long)p * total + q] = out;
     }
-    vals[(long long)p * total + q] = out;
+    float mx = fmaxf(fabsf(out.x), fmaxf(fabsf(out.y), fabsf(out.z)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx = fmaxf(mx, red[i]);
+        atomicMax(maxbits, __float_as_uint(mx));
+    }
 }
+
+__global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
 
 struct BrickArgs {
     float4* acc;
     float4* lo;               // compensated mode: TwoSum residues (else nullptr)
+    const unsigned int* maxbits;   // largest |value| of the batch (float bits)
     const float4* vals;       // n x (2maxR+1)(maxR+1); w = 1 where the sample is inserted
     const float* M;           // n x nops x 9
     const int* bricks;        // nb x 3 brick origins (accumulator indices)
@@ -316,15 +332,35 @@ struct BrickArgs {
     InsertGeom g;
 };
 
+//
+// The brick's sums of a batch are exact: every contribution (rounded to a
+// fixed-point grid of 2^-cap) is added as a 64-bit integer.  cap is the same
+// for the whole launch: the largest |value| of the batch times the most
+// contributions one voxel can take (32 per plane: a 2x2x2 cell around it
+// meets at most ~12 lattice points of a plane, doubled on the hx = 0 column)
+// stays below 2^62.  Integer LDS adds are fast where float LDS atomics are
+// not, and the sums do not depend on the order of the adds.
 __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
-    __shared__ float sfa[BRICK * BRICK * BRICK], sfb[BRICK * BRICK * BRICK], swg[BRICK * BRICK * BRICK];
+    constexpr int NV = BRICK * BRICK * BRICK;
+    __shared__ unsigned long long sacc[3 * NV];
     __shared__ int list[1024];
     __shared__ int nlist;
     const int tid = threadIdx.x;
     const int bx0 = a.bricks[3 * blockIdx.x], by0 = a.bricks[3 * blockIdx.x + 1], bz0 = a.bricks[3 * blockIdx.x + 2];
     const InsertGeom& g = a.g;
     const int R = g.maxR;
-    for (int i = tid; i < BRICK * BRICK * BRICK; i += blockDim.x) { sfa[i] = 0.f; sfb[i] = 0.f; swg[i] = 0.f; }
+    for (int i = tid; i < 3 * NV; i += blockDim.x) sacc[i] = 0ull;
+    int cap;
+    {
+        const float mx = fmaxf(__uint_as_float(*a.maxbits), 1e-30f);
+        int e;
+        (void)frexpf(mx, &e);          // mx < 2^e
+        int cb = 0;
+        while ((1ll << cb) < (long long)a.n * g.nops * 32) ++cb;
+        cap = min(62 - e - cb, 100);
+    }
+    const float scale = ldexpf(1.f, cap);
+    const double inv = ldexp(1.0, -cap);
     // positions (before the index offsets) whose corners can reach the brick:
     // index gx covers x in [gx - R - 1, gx - R + 1), likewise y; z in [gz - 1, gz + 1)
     const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - 1)};
@@ -344,56 +380,96 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
         }
         __syncthreads();
         const int nl = nlist;
-        for (int li = 0; li < nl; ++li) {
+        // Each wave takes its own plane.  The lattice points (h, k) of the plane whose position
+        // x(h, k) = h col0 + k col1 lies in the brick's reach box are enumerated row by row: for row
+        // k the three slab constraints give the exact h interval (widened by one).  A point with
+        // h >= 0 stands for sample (h, k) when that sample is not flipped; a point with h <= 0 for
+        // sample (-h, -k) when it is flipped (z < 0 fold).  So every sample is met once, at the
+        // point where place() puts it (the h = 0 column: (0, k) and (0, -k) both land there).
+        const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+        for (int li = wv; li < nl; li += nwv) {
             const int q = list[li];
             const int p = q / g.nops;
             const float* m = a.M + (long long)q * 9;
-            // plane coordinates of the box corners: u = col0 . X, v = col1 . X
-            float umin = 3e38f, umax = -3e38f, vmin = 3e38f, vmax = -3e38f;
+            float vmin = 3e38f, vmax = -3e38f;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const float X = (k & 1) ? hi[0] : lo[0], Y = (k & 2) ? hi[1] : lo[1], Z = (k & 4) ? hi[2] : lo[2];
-                const float u = m[0] * X + m[3] * Y + m[6] * Z, v = m[1] * X + m[4] * Y + m[7] * Z;
-                umin = fminf(umin, u); umax = fmaxf(umax, u); vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+                const float v = m[1] * X + m[4] * Y + m[7] * Z;
+                vmin = fminf(vmin, v);
+                vmax = fmaxf(vmax, v);
             }
-            // samples hitting the box directly, and (rect negated) after the z < 0 fold
-            const int u0 = (int)floorf(umin) - 1, u1 = (int)ceilf(umax) + 1;
-            const int v0 = (int)floorf(vmin) - 1, v1 = (int)ceilf(vmax) + 1;
-            if (u1 < -R && -u0 < -R) continue;
-            const int nu = u1 - u0 + 1, nv = v1 - v0 + 1;
-            const int nrect = nu * nv;
-            const float4* pv = a.vals + (long long)p * (2 * R + 1) * w;
-            for (int t = tid; t < 2 * nrect; t += blockDim.x) {
-                int hx, ky;
-                if (t < nrect) {
-                    hx = u0 + t % nu;
-                    ky = v0 + t / nu;
-                } else {
-                    hx = -(u0 + (t - nrect) % nu);
-                    ky = -(v0 + (t - nrect) / nu);
-                    if (hx >= u0 && hx <= u1 && ky >= v0 && ky <= v1) continue;   // already in the direct rect
-                }
-                if (hx < 0 || hx > R || ky < -R || ky > R) continue;
-                const float4 val = pv[(ky + R) * w + hx];
-                if (val.w == 0.f) continue;
-                int gx, gy, gz;
-                float ax, ay, az;
-                bool flip;
-                place(m, (float)hx, (float)ky, g, gx, gy, gz, ax, ay, az, flip);
-                const int ox = gx - bx0, oy = gy - by0, oz = gz - bz0;
-                if (ox < -1 || ox >= BRICK || oy < -1 || oy >= BRICK || oz < -1 || oz >= BRICK) continue;
-                const float va = val.x, vb = flip ? -val.y : val.y, vw = val.z;
-                const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay, w11 = ax * ay;
-                const float wz0 = 1.f - az, wz1 = az;
-                const float wt[8] = {w00 * wz0, w10 * wz0, w01 * wz0, w11 * wz0, w00 * wz1, w10 * wz1, w01 * wz1, w11 * wz1};
+            const int ky0 = max((int)floorf(vmin) - 1, -R), ky1 = min((int)ceilf(vmax) + 1, R);
+            const int nrows = ky1 - ky0 + 1;
+            if (nrows <= 0) continue;
+            // row interval of lane r (rows <= 64: the box spans < 18 rows)
+            int ha = 0, cnt = 0;
+            if (lane < nrows) {
+                const float kf = (float)(ky0 + lane);
+                const float c0[3] = {m[0], m[3], m[6]}, c1[3] = {m[1], m[4], m[7]};
+                float ta = -3e38f, tb = 3e38f;
+                bool empty = false;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int cx = ox + (j & 1), cy = oy + ((j >> 1) & 1), cz = oz + (j >> 2);
-                    if (cx < 0 || cx >= BRICK || cy < 0 || cy >= BRICK || cz < 0 || cz >= BRICK) continue;
-                    const int li3 = (cz * BRICK + cy) * BRICK + cx;
-                    atomicAdd(&sfa[li3], va * wt[j]);
-                    atomicAdd(&sfb[li3], vb * wt[j]);
-                    atomicAdd(&swg[li3], vw * wt[j]);
+                for (int i = 0; i < 3; ++i) {
+                    const float base = kf * c1[i];
+                    if (fabsf(c0[i]) > 1e-6f) {
+                        const float t1 = (lo[i] - base) / c0[i], t2 = (hi[i] - base) / c0[i];
+                        ta = fmaxf(ta, fminf(t1, t2));
+                        tb = fminf(tb, fmaxf(t1, t2));
+                    } else if (!(base >= lo[i] - 1e-3f && base <= hi[i] + 1e-3f)) {
+                        empty = true;
+                    }
+                }
+                if (!empty && ta <= tb) {
+                    ha = max((int)ceilf(ta) - 1, -R);
+                    const int hb = min((int)floorf(tb) + 1, R);
+                    cnt = max(hb - ha + 1, 0);
+                }
+            }
+            int wmax = cnt;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o, 64));
+            if (wmax == 0) continue;
+            int w2 = 1;
+            while (w2 < wmax) w2 <<= 1;     // columns per row slot (<= 64)
+            const int rpp = 64 / w2;          // rows per pass
+            const float4* pv = a.vals + (long long)p * (2 * R + 1) * w;
+            for (int r0 = 0; r0 < nrows; r0 += rpp) {
+                const int r = r0 + lane / w2, col = lane % w2;
+                const int rha = __shfl(ha, r < 64 ? r : 0, 64), rcnt = __shfl(cnt, r < 64 ? r : 0, 64);
+                const bool have = r < nrows && col < rcnt;
+                const int hp = rha + col, kp = ky0 + r;
+                // candidate 0: the sample the point stands for; candidate 1: (0, -k) on the h = 0 column
+#pragma unroll
+                for (int cidx = 0; cidx < 2; ++cidx) {
+                    bool ok = have && (cidx == 0 || hp == 0);
+                    const bool mirror = cidx == 1 || hp < 0;
+                    const int hx = mirror ? -hp : hp, ky = mirror ? -kp : kp;
+                    ok = ok && hx >= 0 && hx <= R && ky >= -R && ky <= R;
+                    const float4 vv = ok ? pv[(ky + R) * w + hx] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (!ok || vv.w == 0.f) continue;
+                    int gx, gy, gz;
+                    float ax, ay, az;
+                    bool flip;
+                    place(m, (float)hx, (float)ky, g, gx, gy, gz, ax, ay, az, flip);
+                    if (flip != mirror) continue;
+                    const int ox = gx - bx0, oy = gy - by0, oz = gz - bz0;
+                    if (ox < -1 || ox >= BRICK || oy < -1 || oy >= BRICK || oz < -1 || oz >= BRICK) continue;
+                    const float va = vv.x, vb = flip ? -vv.y : vv.y, vw = vv.z;
+                    const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay,
+                                w11 = ax * ay;
+                    const float wz0 = 1.f - az, wz1 = az;
+                    const float wt[8] = {w00 * wz0, w10 * wz0, w01 * wz0, w11 * wz0,
+                                         w00 * wz1, w10 * wz1, w01 * wz1, w11 * wz1};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int cx = ox + (j & 1), cy = oy + ((j >> 1) & 1), cz = oz + (j >> 2);
+                        if (cx < 0 || cx >= BRICK || cy < 0 || cy >= BRICK || cz < 0 || cz >= BRICK) continue;
+                        const int li3 = (cz * BRICK + cy) * BRICK + cx;
+                        atomicAdd(&sacc[li3], (unsigned long long)__float2ll_rn(va * wt[j] * scale));
+                        atomicAdd(&sacc[NV + li3], (unsigned long long)__float2ll_rn(vb * wt[j] * scale));
+                        atomicAdd(&sacc[2 * NV + li3], (unsigned long long)__float2ll_rn(vw * wt[j] * scale));
+                    }
                 }
             }
         }
@@ -401,34 +477,35 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
     }
     // one read-modify-write of the brick in HBM
     const long long dxy = g.dxy;
-    for (int i = tid; i < BRICK * BRICK * BRICK; i += blockDim.x) {
+    for (int i = tid; i < NV; i += blockDim.x) {
         const int cx = i % BRICK, cy = (i / BRICK) % BRICK, cz = i / (BRICK * BRICK);
         const int X = bx0 + cx, Y = by0 + cy, Z = bz0 + cz;
         if (X >= g.dxy || Y >= g.dxy || Z >= a.dz) continue;
-        if (sfa[i] == 0.f && sfb[i] == 0.f && swg[i] == 0.f) continue;
+        const long long sa = (long long)sacc[i], sb = (long long)sacc[NV + i], sw = (long long)sacc[2 * NV + i];
+        if ((sa | sb | sw) == 0) continue;
+        const double da = (double)sa * inv, db = (double)sb * inv, dw = (double)sw * inv;
         const long long vi = (long long)Z * dxy * dxy + (long long)Y * dxy + X;
         float4 v = a.acc[vi];
         if (a.lo) {
-            // compensated: hi + lo carries the sum; TwoSum (Knuth) keeps the rounding error of each add
+            // compensated: hi + lo carries the sum; TwoSum (Knuth) keeps the rounding error of each add,
+            // and the part of the exact batch sum below float precision goes to lo as well
             float4 l = a.lo[vi];
-            auto two_sum = [](float x, float y, float& err) {
-                const float s = __fadd_rn(x, y);
-                const float yy = __fsub_rn(s, x);
-                err = __fadd_rn(__fsub_rn(x, __fsub_rn(s, yy)), __fsub_rn(y, yy));
-                return s;
+            auto add = [](float& hi, float& lo, double d) {
+                const float y = (float)d;
+                const float s = __fadd_rn(hi, y);
+                const float yy = __fsub_rn(s, hi);
+                const float err = __fadd_rn(__fsub_rn(hi, __fsub_rn(s, yy)), __fsub_rn(y, yy));
+                lo = __fadd_rn(lo, __fadd_rn(err, (float)(d - (double)y)));
+                hi = s;
             };
-            float e;
-            v.x = two_sum(v.x, sfa[i], e);
-            l.x = __fadd_rn(l.x, e);
-            v.y = two_sum(v.y, sfb[i], e);
-            l.y = __fadd_rn(l.y, e);
-            v.z = two_sum(v.z, swg[i], e);
-            l.z = __fadd_rn(l.z, e);
+            add(v.x, l.x, da);
+            add(v.y, l.y, db);
+            add(v.z, l.z, dw);
             a.lo[vi] = l;
         } else {
-            v.x += sfa[i];
-            v.y += sfb[i];
-            v.z += swg[i];
+            v.x = (float)((double)v.x + da);
+            v.y = (float)((double)v.y + db);
+            v.z = (float)((double)v.z + dw);
         }
         a.acc[vi] = v;
     }
@@ -566,12 +643,16 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
     } else {
         const int total = (2 * r.maxR + 1) * (r.maxR + 1);
         CIM_ALLOC(r.vals, sizeof(float4) * (size_t)total * n);
+        CIM_ALLOC(r.maxv, sizeof(unsigned int));
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, ctx->stream, r.maxv.as<unsigned int>(), 0u);
         hipLaunchKernelGGL(k_insert_vals, dim3((total + 255) / 256, n), dim3(256), 0, ctx->stream, ft, r.box, r.maxR,
-                           r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, r.vals.as<float4>());
+                           r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, r.vals.as<float4>(),
+                           r.maxv.as<unsigned int>());
         CIM_HIP(hipGetLastError());
         BrickArgs ba;
         ba.acc = r.acc.as<float4>();
         ba.lo = r.comp ? r.lo.as<float4>() : nullptr;
+        ba.maxbits = r.maxv.as<unsigned int>();
         ba.vals = r.vals.as<float4>();
         ba.M = ctx->aux2.as<float>();
         ba.bricks = r.bricks.as<int>();

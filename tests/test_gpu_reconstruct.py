@@ -149,7 +149,8 @@ def test_brick_gather_equals_scatter(ctx, insert_kernel, nops, n):
             ctx.rec_insert_ft(re, im, eul, ctf)
             ctx.rec_insert_ft(re[: n // 2 + 1], im[: n // 2 + 1], eul[: n // 2 + 1], ctf[: n // 2 + 1])
             res[kern] = ctx.rec_get_tnf(maxR)
-        # float32 sums of up to ~1e3 terms per voxel in different orders
+        # float32 sums of up to ~1e3 terms per voxel in different orders (the brick sums a
+        # batch exactly, so fa, fb that cancel come out 0 where the scatter leaves noise)
         for a, b in zip(res["brick"], res["scatter"]):
             scale = float(np.abs(b).max())
             err_max = float(np.abs(a - b).max())
@@ -157,7 +158,9 @@ def test_brick_gather_equals_scatter(ctx, insert_kernel, nops, n):
             print(f"nops {nops} layout {layout}: rel-L2 {err_l2:.2e}, max {err_max / scale:.2e} of max")
             assert err_l2 < 1e-5
             assert err_max <= 1e-4 * scale
-            assert float(((a != 0) == (b != 0)).mean()) > 0.999
+        # weights are sums of non-negative terms: the same voxels are reached
+        wa, wb = res["brick"][2], res["scatter"][2]
+        assert float(((wa != 0) == (wb != 0)).mean()) > 0.9999
 
 
 COMPENSATED = 16   # CIM_REC_COMPENSATED
