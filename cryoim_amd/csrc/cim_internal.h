@@ -67,10 +67,10 @@ struct RecState {
 struct SearchBatch {
     DevBuf pcrop, pcrop4, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter;
     // tabulated search: polar rays of particles / templates, residual tables, candidates
-    DevBuf ppol, tpol, table, part, cand, kwin, cv, ci;
+    DevBuf ppol, tpol, table, cand, kwin, cv, ci;
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
-                          &fv, &fi, &counter, &ppol, &tpol, &table, &part, &cand, &kwin, &cv, &ci})
+                          &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci})
             b->release();
     }
 };

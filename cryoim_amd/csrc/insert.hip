@@ -346,7 +346,8 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
     __shared__ int list[1024];
     __shared__ int nlist;
     const int tid = threadIdx.x;
-    const int bx0 = a.bricks[3 * blockIdx.x], by0 = a.bricks[3 * blockIdx.x + 1], bz0 = a.bricks[3 * blockIdx.x + 2];
+    const int bi = blockIdx.x;
+    const int bx0 = a.bricks[3 * bi], by0 = a.bricks[3 * bi + 1], bz0 = a.bricks[3 * bi + 2];
     const InsertGeom& g = a.g;
     const int R = g.maxR;
     for (int i = tid; i < 3 * NV; i += blockDim.x) sacc[i] = 0ull;

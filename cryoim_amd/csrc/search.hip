@@ -514,7 +514,6 @@ struct GatherArgs {
     int T, nops, npts, npsi, sub, Nomega;
     float dpsi;
     int n;                    // particles
-    float2* part;             // n x T x npts x Nomega: per-template {N, D} sums
     float* outv;              // n x npts: best table score of the AU point
     int* outi;                // its linear index
 };
@@ -524,62 +523,25 @@ __device__ __forceinline__ int wrap_index(int j, int n) {
     return j < 0 ? j + n : j;
 }
 
-// One workgroup per (AU point, particle, template), threads over omega: the
-// template's share of the residual sums per omega goes to part[p][t][pt][k].
-// Workgroups are dealt to the 8 XCDs in turn (workgroup L on XCD L mod 8).
-// When T divides 8, XCD x only ever scores template x mod T (for particles
-// of class x / T), so its L2 holds a single 1 MB table at a time.
+// One workgroup per (AU point, particle), threads over omega: every template's
+// common lines are summed in registers and the first minimum over omega kept.
+// Workgroups are dealt to the 8 XCDs in turn (workgroup L on XCD L mod 8), so
+// L mod 8 picks the particle within a group of 8 and L / 8 walks its AU
+// points: each XCD reads the tables of one particle at a time.  The valid
+// common lines of a template are compacted in operator order, and the loads
+// of 8 of them are issued together.
 template <int VPT>
 __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     __shared__ int srow[64], sj0[64];
     __shared__ int snv;
+    __shared__ float redv[4];
+    __shared__ int redi[4];
     const long long L = blockIdx.x;
-    int p, t, pt;
-    if (8 % a.T == 0) {
-        const int x = (int)(L & 7), C = 8 / a.T;
-        const long long r8 = L >> 3;
-        t = x % a.T;
-        pt = (int)(r8 % a.npts);
-        p = x / a.T + C * (int)(r8 / a.npts);
-    } else {
-        pt = (int)(L % a.npts);
-        const long long rem = L / a.npts;
-        t = (int)(rem % a.T);
-        p = (int)(rem / a.T);
-    }
+    const long long r8 = L >> 3;
+    const int p = (int)(L & 7) + 8 * (int)(r8 / a.npts), pt = (int)(r8 % a.npts);
     if (p >= a.n) return;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int nrow = a.npsi >> 1;
-    // the common lines of this (AU point, template) in operator order, compacted to
-    // (row offset, first column); the operators fit one wave
-    if (tid < 64) {
-        bool ok = false;
-        int jt = 0, j0 = 0;
-        if (tid < a.nops) {
-            float E1[9], E2[9], a1, a2;
-            euler_f(a.pts[2 * pt], a.pts[2 * pt + 1], 0.f, E1);
-            euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
-            crosslin_one(E1, E2, a.ops + 9 * tid, &a1, &a2);
-            ok = a1 < 999.f && a2 < 999.f;
-            jt = wrap_index((int)rintf(a2 / a.dpsi), a.npsi);
-            j0 = wrap_index((int)rintf(a1 / a.dpsi), a.npsi);
-            if (jt >= nrow) {   // N(jT, jP) = N(jT - npsi/2, jP - npsi/2)
-                jt -= nrow;
-                j0 += nrow;
-                if (j0 >= a.npsi) j0 -= a.npsi;
-            }
-        }
-        const unsigned long long mask = __ballot(ok);
-        if (ok) {
-            const int slot = __popcll(mask & ((1ull << tid) - 1ull));
-            srow[slot] = jt * a.npsi;
-            sj0[slot] = j0;
-        }
-        if (tid == 0) snv = __popcll(mask);
-    }
-    __syncthreads();
-    const int nv = snv;
-    const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * a.npsi;
     float num[VPT], den[VPT];
     int kk[VPT];
 #pragma unroll
@@ -588,59 +550,75 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
         den[v] = 0.f;
         kk[v] = (tid + v * bd) * a.sub;
     }
-    constexpr int G = 8;   // common lines whose loads are issued together
-    int j = 0;
-    for (; j + G <= nv; j += G) {
-        unsigned int e[G][VPT];
+    float E1[9];
+    euler_f(a.pts[2 * pt], a.pts[2 * pt + 1], 0.f, E1);
+    for (int t = 0; t < a.T; ++t) {
+        if (tid < 64) {
+            bool ok = false;
+            int jt = 0, j0 = 0;
+            if (tid < a.nops) {
+                float E2[9], a1, a2;
+                euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
+                crosslin_one(E1, E2, a.ops + 9 * tid, &a1, &a2);
+                ok = a1 < 999.f && a2 < 999.f;
+                jt = wrap_index((int)rintf(a2 / a.dpsi), a.npsi);
+                j0 = wrap_index((int)rintf(a1 / a.dpsi), a.npsi);
+                if (jt >= nrow) {
+                    jt -= nrow;
+                    j0 += nrow;
+                    if (j0 >= a.npsi) j0 -= a.npsi;
+                }
+            }
+            const unsigned long long mask = __ballot(ok);
+            if (ok) {
+                const int slot = __popcll(mask & ((1ull << tid) - 1ull));
+                srow[slot] = jt * a.npsi;
+                sj0[slot] = j0;
+            }
+            if (tid == 0) snv = __popcll(mask);
+        }
+        __syncthreads();
+        const int nv = snv;
+        const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * a.npsi;
+        constexpr int G = 8;
+        int j = 0;
+        for (; j + G <= nv; j += G) {
+            unsigned int e[G][VPT];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const unsigned int* row = tab + srow[j + g];
-            const int j0 = sj0[j + g];
+            for (int g = 0; g < G; ++g) {
+                const unsigned int* row = tab + srow[j + g];
+                const int j0 = sj0[j + g];
+#pragma unroll
+                for (int v = 0; v < VPT; ++v) {
+                    int c = j0 - kk[v];
+                    c += c < 0 ? a.npsi : 0;
+                    c = c < 0 ? 0 : c;
+                    e[g][v] = row[c];
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int v = 0; v < VPT; ++v) {
+                    num[v] += __uint_as_float(e[g][v] << 16);
+                    den[v] += __uint_as_float(e[g][v] & 0xffff0000u);
+                }
+        }
+        for (; j < nv; ++j) {
+            const unsigned int* row = tab + srow[j];
+            const int j0 = sj0[j];
 #pragma unroll
             for (int v = 0; v < VPT; ++v) {
                 int c = j0 - kk[v];
                 c += c < 0 ? a.npsi : 0;
-                c = c < 0 ? 0 : c;   // lanes past Nomega (k * sub >= npsi) read column 0, unused
-                e[g][v] = row[c];
+                c = c < 0 ? 0 : c;
+                const unsigned int e = row[c];
+                num[v] += __uint_as_float(e << 16);
+                den[v] += __uint_as_float(e & 0xffff0000u);
             }
         }
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int v = 0; v < VPT; ++v) {
-                num[v] += __uint_as_float(e[g][v] << 16);
-                den[v] += __uint_as_float(e[g][v] & 0xffff0000u);
-            }
+        __syncthreads();
     }
-    for (; j < nv; ++j) {
-        const unsigned int* row = tab + srow[j];
-        const int j0 = sj0[j];
-#pragma unroll
-        for (int v = 0; v < VPT; ++v) {
-            int c = j0 - kk[v];
-            c += c < 0 ? a.npsi : 0;
-            c = c < 0 ? 0 : c;
-            const unsigned int e = row[c];
-            num[v] += __uint_as_float(e << 16);
-            den[v] += __uint_as_float(e & 0xffff0000u);
-        }
-    }
-    float2* out = a.part + ((long long)(p * a.T + t) * a.npts + pt) * a.Nomega;
-#pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-        const int k = tid + v * bd;
-        if (k < a.Nomega) out[k] = make_float2(num[v], den[v]);
-    }
-}
-
-// Sum the template shares (in template order), score, and keep the first
-// minimum over omega of each (AU point, particle).
-template <int VPT>
-__global__ void __launch_bounds__(256) k_cl_combine(GatherArgs a) {
-    __shared__ float redv[4];
-    __shared__ int redi[4];
-    const int pt = blockIdx.x, p = blockIdx.y;
-    const int tid = threadIdx.x, bd = blockDim.x;
     const float r2d = 57.2957795130823208768f;
     float best = 3.0e38f;
     int bidx = 0x7fffffff;
@@ -648,13 +626,7 @@ __global__ void __launch_bounds__(256) k_cl_combine(GatherArgs a) {
     for (int v = 0; v < VPT; ++v) {
         const int k = tid + v * bd;
         if (k >= a.Nomega) continue;
-        float nn = 0.f, dd = 0.f;
-        for (int t = 0; t < a.T; ++t) {
-            const float2 e = a.part[((long long)(p * a.T + t) * a.npts + pt) * a.Nomega + k];
-            nn += e.x;
-            dd += e.y;
-        }
-        const float pr = dd > 1.0e-10f ? nn / dd * r2d : 999.f;
+        const float pr = den[v] > 1.0e-10f ? num[v] / den[v] * r2d : 999.f;
         const int li = a.pbase[pt] + k;
         if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
     }
@@ -1120,24 +1092,18 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 CIM_HIP(hipGetLastError());
             }
             GatherArgs ga{sb.table.as<unsigned int>(), sb.tang.as<float>(), sb.pts.as<float>(), sb.pbase.as<int>(),
-                          sb.ops.as<float>(), T, 60, npts, npsi, sub, g.No, dpsi, nb, nullptr,
-                          sb.outv.as<float>(), sb.outi.as<int>()};
-            CIM_ALLOC(sb.part, sizeof(float2) * (size_t)nb * T * npts * g.No);
-            ga.part = sb.part.as<float2>();
+                          sb.ops.as<float>(), T, 60, npts, npsi, sub, g.No, dpsi, nb, sb.outv.as<float>(),
+                          sb.outi.as<int>()};
             {
                 TimedLaunch tl(ctx, "cl_gather");
                 const int vg = (g.No + 255) / 256;
-                const long long nwg = (8 % T == 0) ? 8LL * npts * ((nb + 8 / T - 1) / (8 / T))
-                                                   : (long long)npts * nb * T;
-                const dim3 gg((unsigned)nwg);
-#define CIM_GATHER(V)                                                                       do {                                                                                        hipLaunchKernelGGL(k_cl_gather<V>, gg, dim3(256), 0, ctx->stream, ga);                  hipLaunchKernelGGL(k_cl_combine<V>, dim3(npts, nb), dim3(256), 0, ctx->stream, ga);     } while (0)
-                if (vg <= 1) CIM_GATHER(1);
-                else if (vg == 2) CIM_GATHER(2);
-                else if (vg == 3) CIM_GATHER(3);
-                else if (vg == 4) CIM_GATHER(4);
-                else if (vg <= 6) CIM_GATHER(6);
-                else CIM_GATHER(8);
-#undef CIM_GATHER
+                const dim3 gg((unsigned)(8LL * npts * ((nb + 7) / 8)));
+                if (vg <= 1) hipLaunchKernelGGL(k_cl_gather<1>, gg, dim3(256), 0, ctx->stream, ga);
+                else if (vg == 2) hipLaunchKernelGGL(k_cl_gather<2>, gg, dim3(256), 0, ctx->stream, ga);
+                else if (vg == 3) hipLaunchKernelGGL(k_cl_gather<3>, gg, dim3(256), 0, ctx->stream, ga);
+                else if (vg == 4) hipLaunchKernelGGL(k_cl_gather<4>, gg, dim3(256), 0, ctx->stream, ga);
+                else if (vg <= 6) hipLaunchKernelGGL(k_cl_gather<6>, gg, dim3(256), 0, ctx->stream, ga);
+                else hipLaunchKernelGGL(k_cl_gather<8>, gg, dim3(256), 0, ctx->stream, ga);
                 CIM_HIP(hipGetLastError());
             }
             CIM_ALLOC(sb.cand, sizeof(int) * (size_t)nb * topk);
