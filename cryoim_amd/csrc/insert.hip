@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <vector>
 
 #include "cim_device.h"
 
@@ -513,6 +515,57 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
 }
 
 // ---- host side -----------------------------------------------------------
+
+// Symmetry operators of icosprocess's local reconstruction (mode 6,
+// reconstruction_local.f90:30-55 with icosprocess/SRC/icos2f_sym_mat.f90:250-540):
+// z rotations by k * 360/N degrees, each entry evaluated as
+// cos(a * 3.1415926 / 180.0) in REAL*4, for c12, c15, c5a, c6, c3, c4, c2,
+// c8; the identity for c1.  Any other name (the reference has no branch for
+// it there) returns -1.
+static int icosprocess_local_ops(const char* sym, std::vector<double>& r) {
+    std::string s(sym ? sym : "");
+    while (!s.empty() && s.back() == ' ') s.pop_back();
+    int N = 0;
+    float step = 0.f;
+    if (s == "c1") { N = 1; step = 0.f; }
+    else if (s == "c2") { N = 2; step = 180.f; }
+    else if (s == "c3") { N = 3; step = 120.f; }
+    else if (s == "c4") { N = 4; step = 90.f; }
+    else if (s == "c5a") { N = 5; step = 72.f; }
+    else if (s == "c6") { N = 6; step = 60.f; }
+    else if (s == "c8") { N = 8; step = 45.f; }
+    else if (s == "c12") { N = 12; step = 30.f; }
+    else if (s == "c15") { N = 15; step = 24.f; }
+    else return -1;
+    r.assign(9 * (size_t)N, 0.0);
+    for (int k = 0; k < N; ++k) {
+        const float a = (float)k * step;
+        const float rad = a * 3.1415926f / 180.0f;
+        const float c = cosf(rad), sn = sinf(rad);
+        double* m = &r[9 * (size_t)k];
+        m[0] = c;
+        m[1] = -sn;
+        m[3] = sn;
+        m[4] = c;
+        m[8] = 1.0;
+    }
+    return N;
+}
+
+}  // namespace cim
+
+extern "C" int cim_icosprocess_sym_ops(const char* sym, double* ops, int* nops) {
+    std::vector<double> r;
+    const int n = cim::icosprocess_local_ops(sym, r);
+    if (n < 0) return CIM_E_ARG;
+    if (nops) *nops = n;
+    if (ops)
+        for (size_t i = 0; i < r.size(); ++i) ops[i] = r[i];
+    return CIM_OK;
+}
+
+namespace cim {
+
 
 int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init, int nops, const double* ops) {
     if (box < 8 || (box & 1)) return set_error(ctx, CIM_E_ARG, "box %d must be even and >= 8", box);

@@ -5,6 +5,9 @@
 !                     the tnf partial volume of reconstruct_substep.f90)
 !   mode=4            global orientation search (ort_globesearch.f90)
 !   mode=5            local orientation/centre refinement (ort_refinement.f90)
+!   mode=6            local reconstruction of loc_box windows around (x0, y0)
+!                     with a local symmetry (reconstruction_local.f90;
+!                     subrec=y: reconstruction_local_substep.f90)
 ! Parameter keys, defaults and derived values follow input_parameters,
 ! initialvalue and calc_initialvalues.  Output files have the reference's
 ! formats, so the programs chained after icosprocess read them unchanged.
@@ -19,6 +22,7 @@ program icosprocess_hip
     character(len=4) :: sym = 'icos'
     character :: applyctf = 'y', imgmask_on = 'y', subrec = 'n', realspaceavg = 'n'
     integer :: mode = 1, first = 1, last = 0, ncycle = 6, nargs, i, j, k, eq, fftsize, minR, maxR, T
+    integer :: loc_box = -1
     real :: apix = 1.0, minres = 300.0, maxres = 8.0, imgmask = -1.0, vol = 300.0, cs = 2.7, ampwg = 0.1
     real :: pr_threshold = 90.0, searchstep = 2.0, boundx = -1.0, deltaangle, deltatrans
     logical :: have_mask = .false., have_boundx = .false.
@@ -26,7 +30,7 @@ program icosprocess_hip
     type particle_t
         character(len=54) :: stck
         integer :: pid, imgid, imgpid
-        real :: theta, phi, omega, x, y, pr, df1, df2, astig
+        real :: theta, phi, omega, x, y, pr, df1, df2, astig, x0 = 0.0, y0 = 0.0
     end type
     type(particle_t), allocatable :: part(:)
     type(c_ptr) :: ctx
@@ -78,10 +82,11 @@ program icosprocess_hip
         case ('subrec'); subrec = val(1:1)
         case ('sym'); sym = val(1:4)
         case ('boundX'); read(val, *) boundx; have_boundx = .true.
+        case ('loc_box'); read(val, *) loc_box
         end select
     end do
-    if (trim(sym) /= 'icos') then
-        print *, 'icosprocess_hip runs the icosahedral (sym=icos) path only'
+    if (mode /= 6 .and. trim(sym) /= 'icos') then
+        print *, 'icosprocess_hip runs modes 1, 4, 5 with the icosahedral (sym=icos) symmetry only'
         stop 2
     end if
 
@@ -101,9 +106,15 @@ program icosprocess_hip
     fftsize = nx
     if (.not. have_mask) imgmask = real(fftsize / 2 - 2)
     if (.not. have_boundx) boundx = real(fftsize / 2 - 2)
-    ! calc_initialvalues (0calc_initialvalues.f90:6-21)
-    minres = apix * fftsize / minres
-    maxres = apix * fftsize / maxres
+    if (loc_box <= 0) loc_box = fftsize                ! initialvalue: loc_box = FFTsize
+    ! calc_initialvalues (0calc_initialvalues.f90:4-21): mode 6 measures resolution on loc_box
+    if (mode /= 6) then
+        minres = apix * fftsize / minres
+        maxres = apix * fftsize / maxres
+    else
+        minres = apix * loc_box / minres
+        maxres = apix * loc_box / maxres
+    end if
     minR = int(minres)
     maxR = int(maxres)
     if (mod(maxR, 2) /= 0) maxR = maxR + 1
@@ -140,8 +151,10 @@ program icosprocess_hip
         call run_reconstruct()
     case (4, 5)
         call run_search()
+    case (6)
+        call run_local()
     case default
-        print *, 'icosprocess_hip implements modes 1, 4 and 5'
+        print *, 'icosprocess_hip implements modes 1, 4, 5 and 6'
         stop 2
     end select
     rc = cim_finalize(ctx)
@@ -166,8 +179,15 @@ contains
             read(u, *)
         end do
         do i = first, last
-            read(u, *) part(i)%stck, part(i)%pid, part(i)%theta, part(i)%phi, part(i)%omega, part(i)%x, &
-                       part(i)%y, part(i)%pr, part(i)%imgid, part(i)%imgpid, part(i)%df1, part(i)%df2, part(i)%astig
+            if (mode /= 6) then
+                read(u, *) part(i)%stck, part(i)%pid, part(i)%theta, part(i)%phi, part(i)%omega, part(i)%x, &
+                           part(i)%y, part(i)%pr, part(i)%imgid, part(i)%imgpid, part(i)%df1, part(i)%df2, &
+                           part(i)%astig
+            else
+                read(u, *) part(i)%stck, part(i)%pid, part(i)%theta, part(i)%phi, part(i)%omega, part(i)%x, &
+                           part(i)%y, part(i)%pr, part(i)%imgid, part(i)%imgpid, part(i)%df1, part(i)%df2, &
+                           part(i)%astig, part(i)%x0, part(i)%y0
+            end if
         end do
         close(u)
         do i = first, last
@@ -199,16 +219,25 @@ contains
     ! Records for a batch.  The CTF is recomputed only when imgID changes in
     ! reconstruct.f90:85-89 and ort_globesearch.f90:61-65, so the defocus of
     ! the first particle of each micrograph run is carried forward there.
-    subroutine fill_records(i0, i1, rec, cache_ctf, mask_first)
+    ! Particle records for first..i1 written from i0.  bx >= 0 applies the
+    ! reconstruction filter PR <= PR_threshold, |x|, |y| <= bx (unused records
+    ! get P_USE = 0); the per-micrograph CTF cache follows only particles that
+    ! pass it, as the reconstruct loops update imgID0 inside that test
+    ! (reconstruct.f90:77-90, reconstruction_local.f90:96-108).
+    subroutine fill_records(i0, i1, rec, cache_ctf, mask_first, bx)
         integer, intent(in) :: i0, i1
         real(c_float), intent(out) :: rec(CIM_PART_STRIDE, *)
         logical, intent(in) :: cache_ctf, mask_first
+        real, intent(in) :: bx
         integer :: i, k, imgid0
         real :: d1, d2, ast
+        logical :: use
         imgid0 = -100
         d1 = 0.0; d2 = 0.0; ast = 0.0
         do i = first, i1
-            if (.not. cache_ctf .or. part(i)%imgid /= imgid0) then
+            use = .true.
+            if (bx >= 0.0) use = part(i)%pr <= pr_threshold .and. abs(part(i)%x) <= bx .and. abs(part(i)%y) <= bx
+            if (use .and. (.not. cache_ctf .or. part(i)%imgid /= imgid0)) then
                 d1 = part(i)%df1; d2 = part(i)%df2; ast = part(i)%astig
                 imgid0 = part(i)%imgid
             end if
@@ -231,7 +260,7 @@ contains
                 rec(P_MASKX, k) = part(i)%x
                 rec(P_MASKY, k) = part(i)%y
             end if
-            rec(P_USE, k) = 1.0
+            rec(P_USE, k) = merge(1.0, 0.0, use)
         end do
     end subroutine
 
@@ -255,15 +284,11 @@ contains
         used = 0
         do i0 = first, last, B
             i1 = min(last, i0 + B - 1)
-            call fill_records(i0, i1, rec, .not. sub, .true.)
+            call fill_records(i0, i1, rec, .not. sub, .true., bx)
             do i = i0, i1
                 k = i - i0 + 1
                 call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))
-                if (part(i)%pr <= pr_threshold .and. abs(part(i)%x) <= bx .and. abs(part(i)%y) <= bx) then
-                    used = used + 1
-                else
-                    rec(P_USE, k) = 0.0
-                end if
+                if (rec(P_USE, k) > 0.0) used = used + 1
             end do
             rc = cim_rec_insert(ctx, p, i1 - i0 + 1, imgs, rec)
             call cim_check(ctx, rc, 'cim_rec_insert')
@@ -284,27 +309,103 @@ contains
             allocate(map(int(fftsize, 8)**3))
             rc = cim_rec_finish(ctx, map)
             call cim_check(ctx, rc, 'cim_rec_finish')
-            call write_map(result3d, map)
+            call write_map(result3d, map, fftsize)
         end if
     end subroutine
 
-    subroutine write_map(fname, map)
+    ! icosprocess mode 6 (reconstruction_local.f90:65-176, reconstruction_local_substep.f90):
+    ! the loc_box window around (int(x0) + FFTsize/2, int(y0) + FFTsize/2) of each
+    ! particle, masked around its own shift, inserted with the local symmetry.
+    subroutine run_local()
+        integer, parameter :: B = 256
+        real(c_float), allocatable :: img(:), imgs(:), rec(:, :), fa(:), fb(:), wg(:), map(:)
+        real(c_double), target :: ops(9 * 15)
+        integer(c_int) :: nops
+        integer :: i0, i1, i, k, layout, nt, u, used, xc, yc, ix, iy, lb2
+        logical :: sub
+        sub = subrec == 'y'                           ! 0main.f90:254 tests 'y' only
+        rc = cim_icosprocess_sym_ops(trim(sym) // c_null_char, ops, nops)
+        if (rc /= CIM_OK) then
+            print *, 'mode 6 has no local symmetry operators for sym=', trim(sym)
+            stop 2
+        end if
+        layout = merge(1, 0, sub)
+        p%box = loc_box
+        p%maxR = maxR
+        p%norm_on = 1
+        p%mask_on = merge(1, 0, imgmask_on == 'y')
+        rc = cim_rec_begin(ctx, loc_box, maxR, layout, real(merge(maxR - 1, maxR, sub), c_float), 0.01_c_float, &
+                           nops, c_loc(ops))
+        call cim_check(ctx, rc, 'cim_rec_begin')
+        lb2 = loc_box / 2
+        allocate(img(fftsize * fftsize), imgs(loc_box * loc_box * B), rec(CIM_PART_STRIDE, B))
+        used = 0
+        do i0 = first, last, B
+            i1 = min(last, i0 + B - 1)
+            call fill_records(i0, i1, rec, .true., .false., boundx)
+            do i = i0, i1
+                k = i - i0 + 1
+                call read_image(part(i), img)
+                xc = int(part(i)%x0) + fftsize / 2
+                yc = int(part(i)%y0) + fftsize / 2
+                if (xc - lb2 < 0 .or. yc - lb2 < 0 .or. xc + lb2 > fftsize .or. yc + lb2 > fftsize) then
+                    print *, 'particle ', i, ': loc_box window around (x0, y0) leaves the image'
+                    stop 1
+                end if
+                do iy = 0, loc_box - 1
+                    do ix = 0, loc_box - 1
+                        imgs((k - 1) * loc_box * loc_box + iy * loc_box + ix + 1) = &
+                            img((yc - lb2 + iy) * fftsize + (xc - lb2 + ix) + 1)
+                    end do
+                end do
+                if (rec(P_USE, k) > 0.0) used = used + 1
+            end do
+            rc = cim_rec_insert(ctx, p, i1 - i0 + 1, imgs, rec)
+            call cim_check(ctx, rc, 'cim_rec_insert')
+        end do
+        print *, 'usedparticle:', used
+        if (sub) then
+            nt = (2 * maxR + 1)**2 * (maxR + 1)
+            allocate(fa(nt), fb(nt), wg(nt))
+            rc = cim_rec_get_tnf(ctx, fa, fb, wg)
+            call cim_check(ctx, rc, 'cim_rec_get_tnf')
+            open(newunit=u, file=trim(result3d), form='unformatted', access='stream', status='replace')
+            write(u) loc_box, maxR, apix, imgmask, 0, 0
+            write(u) fa
+            write(u) fb
+            write(u) wg
+            close(u)
+        else
+            allocate(map(int(loc_box, 8)**3))
+            rc = cim_rec_finish(ctx, map)
+            call cim_check(ctx, rc, 'cim_rec_finish')
+            call write_map(result3d, map, loc_box)
+        end if
+    end subroutine
+
+    ! MRC header of mrchead.f90:2-58 (maphead) / maphead2.f90 for an n^3 map;
+    ! map(4), machinestamp(4) and label(10,80) are single-character arrays, so
+    ! each holds the first character of the string assigned to it.
+    subroutine write_map(fname, map, n)
         character(len=*), intent(in) :: fname
         real(c_float), intent(in) :: map(:)
+        integer, intent(in) :: n
         integer(c_int32_t) :: h(256)
         integer :: u
         h = 0
-        h(1:3) = fftsize
+        h(1:3) = n
         h(4) = 2
-        h(5:7) = -fftsize / 2
-        h(8:10) = fftsize - 1
-        h(11) = transfer(real((fftsize - 1) * apix, c_float), h(1))
+        h(5:7) = -n / 2
+        h(8:10) = n - 1
+        h(11) = transfer(real((n - 1) * apix, c_float), h(1))
         h(12) = h(11)
         h(13) = h(11)
         h(14:16) = transfer(90.0_c_float, h(1))
         h(17) = 1; h(18) = 2; h(19) = 3
-        h(50:52) = transfer(real(-fftsize, c_float), h(1))
-        h(53) = transfer('MAP ', h(1))
+        h(50:52) = transfer(real(-n, c_float), h(1))
+        h(53) = transfer('mmmm', h(1))
+        h(54) = transfer('llll', h(1))
+        h(57:256) = transfer('HHHH', h(1))
         open(newunit=u, file=trim(fname), form='unformatted', access='stream', status='replace')
         write(u) h
         write(u) map
@@ -341,7 +442,7 @@ contains
         do i0 = first, last, B
             i1 = min(last, i0 + B - 1)
             ! mode 4 caches the CTF per micrograph; mode 5 recomputes it per particle
-            call fill_records(i0, i1, rec, mode == 4, .false.)
+            call fill_records(i0, i1, rec, mode == 4, .false., -1.0)
             do i = i0, i1
                 k = i - i0 + 1
                 call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))

@@ -21,15 +21,23 @@ P_THETA, P_PHI, P_OMEGA, P_X, P_Y, P_PR, P_DF1, P_DF2, P_ASTIG, P_MASKX, P_MASKY
 P_BOXX, P_BOXY = P_MASKX, P_MASKY
 
 
-def sym_ops(sym: str) -> np.ndarray:
-    """Symmetry operators of the local programs (cim_sym_ops), (n, 3, 3)."""
-    L = load()
+def _ops(fn, sym: str) -> np.ndarray:
     n = ctypes.c_int()
-    if L.cim_sym_ops(sym.encode(), None, ctypes.byref(n)) != 0:
+    if fn(sym.encode(), None, ctypes.byref(n)) != 0:
         raise CimError(f"unknown symmetry {sym!r}")
     out = np.zeros(9 * n.value, np.float64)
-    L.cim_sym_ops(sym.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n))
+    fn(sym.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n))
     return out.reshape(-1, 3, 3)
+
+
+def sym_ops(sym: str) -> np.ndarray:
+    """Symmetry operators of the local programs (cim_sym_ops), (n, 3, 3)."""
+    return _ops(load().cim_sym_ops, sym)
+
+
+def icosprocess_sym_ops(sym: str) -> np.ndarray:
+    """Local symmetry operators of icosprocess mode 6 (cim_icosprocess_sym_ops), (n, 3, 3)."""
+    return _ops(load().cim_icosprocess_sym_ops, sym)
 
 
 class CimParams(ctypes.Structure):
@@ -114,6 +122,7 @@ def load() -> ctypes.CDLL:
         "cim_search_pr_array": (ip, [vp, P, fp, fp, fp]),
         "cim_refine_local": (ip, [vp, P, ip, fp, fp]),
         "cim_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
+        "cim_icosprocess_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
         "cim_lref_model": (ip, [vp, P, fp]),
         "cim_lref_refine": (ip, [vp, P, ip, fp, ip, fp, ctypes.c_char_p, ctypes.c_float, ctypes.c_float]),
     }

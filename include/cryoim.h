@@ -179,6 +179,13 @@ int cim_rec_finish_dev(cim_ctx* ctx, void* map_dev);
  * (box-8)^3 block of :143-149, including its plane index i/(box-8)^2+4 on
  * the 1-based i (the last voxel of each plane comes from the next plane). */
 int cim_merge_map(cim_ctx* ctx, float* map);
+/* Symmetry operators of icosprocess's local reconstruction (mode 6,
+ * reconstruction_local.f90:30-55; generators icos2f_sym_mat.f90:250-540):
+ * sym = "c1", "c2", "c3", "c4", "c5a", "c6", "c8", "c12" or "c15", z
+ * rotations by k*360/N degrees evaluated in REAL*4 as the reference.  ops
+ * (may be NULL): nops row-major 3x3 float64, ready for cim_rec_begin.
+ * Returns CIM_E_ARG for names mode 6 has no operators for. */
+int cim_icosprocess_sym_ops(const char* sym, double* ops, int* nops);
 
 /* ---- half-map FSC ------------------------------------------------------ */
 #define CIM_FSC_RECONSTRUCT 0   /* icosprocess reconstruct_FSC.f90:10-197 */

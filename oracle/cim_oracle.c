@@ -894,6 +894,21 @@ EXPORT void o_tnf_fsc(int maxR, const real* A1, const real* B1, const real* W1, 
     free(w2); free(a2); free(b2); free(S); free(S1); free(S2);
 }
 
+/* Local symmetry of icosprocess mode 6 (icos2f_sym_mat.f90:250-540,
+ * reconstruction_local.f90:30-55): N z rotations by k*step degrees, each entry
+ * cos(a*3.1415926/180.0) in REAL*4.  ops: 9*N row-major. */
+EXPORT void o_icosprocess_local_ops(int N, float step, real* ops)
+{
+    for (int k = 0; k < N; ++k) {
+        float a = (float)k * step;
+        float rad = a * 3.1415926f / 180.0f;
+        float c = cosf(rad), sn = sinf(rad);
+        real* m = ops + 9 * k;
+        for (int i = 0; i < 9; ++i) m[i] = 0;
+        m[0] = c; m[1] = -sn; m[3] = sn; m[4] = c; m[8] = 1;
+    }
+}
+
 /* Output crop of merge_reconstruct.f90:143-149: the (nx-8)^3 centre block,
  * with lz taken as i/(nx-8)^2 + 4 from the 1-based i (not i-1), so the
  * last voxel of each output plane comes from the next plane. */

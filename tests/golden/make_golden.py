@@ -348,6 +348,41 @@ def fsc_program_vectors() -> dict:
     return g
 
 
+def local_program_vectors() -> dict:
+    """icosprocess mode 6 (reconstruct_local, reconstruct_local_substep): loc_box windows
+    of the program_vectors images around seeded (x0, y0), local symmetry c5a."""
+    g = {}
+    box, apix, n, lb = 48, 2.0, 30, 32
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    rng = np.random.default_rng(66)
+    x0y0 = rng.uniform(-6.0, 6.0, (n, 2)).astype(f4)
+    g["loc_x0y0"] = x0y0
+    g["loc_box"] = np.array([lb], i4)
+    exe = os.path.join(REF, "icosprocess")
+    with tempfile.TemporaryDirectory() as d:
+        stk = "stack.mrc"
+        formats.write_mrc(os.path.join(d, stk), ps.images)
+        parts = [formats.Particle(stk, i + 1, *ps.angles[i], *ps.shifts[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0, x0=float(x0y0[i, 0]), y0=float(x0y0[i, 1])) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort6.txt"), parts, with_local_box=True)
+        common = ["mode=6", f"loc_box={lb}", "sym=c5a", "apix=2.0", "maxRes=6.0", "imgmask=24", "ortfile=ort6.txt",
+                  f"last={n}"]
+        for args in (["result3d=loc.mrc"], ["subrec=y", "result3d=loc.tnf"]):
+            r = subprocess.run([exe, *common, *args], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+        raw = open(os.path.join(d, "loc.mrc"), "rb").read()
+        g["loc_hdr"] = np.frombuffer(raw[:1024], dtype="<i4").copy()
+        g["loc_map"] = np.frombuffer(raw[1024:], dtype="<f4").reshape(lb, lb, lb).copy()
+        t = formats.read_tnf(os.path.join(d, "loc.tnf"))
+        g["locsub_fa"], g["locsub_fb"], g["locsub_wg"] = t["fa"], t["fb"], t["wg"]
+        g["locsub_hdr"] = np.array([t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]], i4)
+        g["locsub_hdrf"] = np.array([t["apix"], t["imgmask"]], f4)
+    return g
+
+
 def main():
     import argparse
     ap = argparse.ArgumentParser()
@@ -358,7 +393,8 @@ def main():
         raise SystemExit("oracle/_ref is not built; run oracle/build_ref.sh first")
     jobs = [("units", lambda: unit_vectors(rng)), ("pr", lambda: pr_vectors(rng)),
             ("lref_units", lambda: lref_vectors(rng)), ("programs", program_vectors),
-            ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors)]
+            ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors),
+            ("local", local_program_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

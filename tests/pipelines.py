@@ -59,6 +59,33 @@ def half_volume(O, images, angles, shifts, df, apix, maxR, mask_r, parity, ctf=T
     return vol
 
 
+def local_windows(images, x0y0, lb):
+    """reconstruction_local.f90:85-93: the loc_box window of each particle image
+    centred on (int(x0) + box/2, int(y0) + box/2) (Fortran int truncates to zero)."""
+    box = images.shape[-1]
+    out = np.empty((len(images), lb, lb), images.dtype)
+    for i, (x0, y0) in enumerate(x0y0):
+        xc, yc = int(x0) + box // 2, int(y0) + box // 2
+        out[i] = images[i, yc - lb // 2:yc + lb // 2, xc - lb // 2:xc + lb // 2]
+    return out
+
+
+def local_volume(O, images, angles, shifts, df, x0y0, apix, maxR, mask_r, lb, sym, compact=False):
+    """icosprocess mode 6: reconstruction_local.f90:65-176 (full loc_box^3 volume,
+    rcut maxR) or reconstruction_local_substep.f90 (compact, rcut maxR-1); weights
+    start at 0.01 in both, the mask is centred on each particle's own shift and
+    the operators are the local group of icos2f_sym_mat.f90:250-540."""
+    ops = O.icosprocess_local_ops(sym)
+    wins = local_windows(images, x0y0, lb)
+    vol = O.new_volume(lb, maxR, compact=compact, wg_init=0.01)
+    for i in range(len(wins)):
+        x, y = shifts[i]
+        a, b = O.particle_ft(wins[i], x, y, mask_r=mask_r, mask_xy=(x, y), norm=True)
+        c = O.ctf2d(lb, df[i, 0], df[i, 1], 0.0, apix)
+        O.insert(vol, a, b, c, angles[i], maxR, rcut=maxR - 1 if compact else maxR, compact=compact, ops=ops)
+    return vol
+
+
 def templates(O, timgs, tshifts, mask_r):
     ta, tb = [], []
     for t in range(len(timgs)):

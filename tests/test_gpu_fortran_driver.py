@@ -72,3 +72,49 @@ def test_search_modes(workdir, mode):
         got = np.array([o.theta, o.phi, o.omega, o.x, o.y])
         assert np.abs(got - r[:5]).max() < 2e-3, (got, r)     # format 900 keeps 4 decimals
         assert abs(o.pr - r[5]) < 2e-3
+
+
+@pytest.fixture(scope="module")
+def local_ort(workdir):
+    d, g = workdir
+    loc = np.load(os.path.join(ROOT, "tests", "golden", "local.npz"))
+    n = len(g["prog_images"])
+    parts = [formats.Particle("stack.mrc", i + 1, *g["prog_angles"][i], *g["prog_shifts"][i], 0.0, i + 1, i + 1,
+                              g["prog_df"][i, 0], g["prog_df"][i, 1], 0.0, x0=float(loc["loc_x0y0"][i, 0]),
+                              y0=float(loc["loc_x0y0"][i, 1])) for i in range(n)]
+    formats.write_ortfile(os.path.join(d, "ort6.txt"), parts, with_local_box=True)
+    return d, loc
+
+
+LOCAL_ARGS = ["mode=6", "loc_box=32", "sym=c5a", "apix=2.0", "maxRes=6.0", "imgmask=24", "ortfile=ort6.txt",
+              "last=30"]
+
+
+def test_mode6_local_map(local_ort):
+    """reconstruction_local.f90: loc_box^3 map with the c5a local operators."""
+    d, loc = local_ort
+    _run(d, *LOCAL_ARGS, "result3d=loc.mrc")
+    raw = open(os.path.join(d, "loc.mrc"), "rb").read()
+    assert np.array_equal(np.frombuffer(raw[:1024], "<i4"), loc["loc_hdr"])
+    m = np.frombuffer(raw[1024:], "<f4").reshape(loc["loc_map"].shape)
+    err = rel_l2(m, loc["loc_map"])
+    print(f"mode 6 map vs reference: rel-L2 {err:.2e}")
+    assert err < 1e-4
+
+
+def test_mode6_local_substep_tnf(local_ort):
+    """reconstruction_local_substep.f90: compact tnf with its (loc_box, maxR, apix, imgmask) header."""
+    d, loc = local_ort
+    _run(d, *LOCAL_ARGS, "subrec=y", "result3d=loc.tnf")
+    t = formats.read_tnf(os.path.join(d, "loc.tnf"))
+    assert [t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]] == list(loc["locsub_hdr"])
+    assert np.allclose([t["apix"], t["imgmask"]], loc["locsub_hdrf"])
+    for k in ("fa", "fb", "wg"):
+        assert rel_l2(t[k], loc[f"locsub_{k}"]) < 1e-4
+
+
+def test_mode6_rejects_unknown_symmetry(local_ort):
+    d, _ = local_ort
+    r = subprocess.run([EXE, *[a if not a.startswith("sym=") else "sym=c7" for a in LOCAL_ARGS], "result3d=x.mrc"],
+                       cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode != 0 and "local symmetry" in r.stdout

@@ -111,6 +111,35 @@ def test_fsc_map_vs_fp64_oracle_box96(ctx, O64, merge):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("compact", [False, True])
+def test_local_reconstruction_matches_reference(ctx, prog, compact):
+    """icosprocess mode 6 through the C ABI: loc_box windows cut around
+    (x0, y0), the c5a local operators, weights starting at 0.01, mask centred on
+    each particle's own shift; full loc_box^3 map (reconstruction_local.f90) or
+    compact tnf with rcut maxR-1 (reconstruction_local_substep.f90)."""
+    loc = np.load(pipelines.golden_path("local.npz"))
+    apix, lb = float(prog["prog_apix"][0]), int(loc["loc_box"][0])
+    d = pipelines.derived(lb, apix, maxRes=6.0, imgmask_A=24)
+    maxR = d["maxR"]
+    ops = lib.icosprocess_sym_ops("c5a")
+    assert ops.shape == (5, 3, 3)
+    wins = pipelines.local_windows(prog["prog_images"], loc["loc_x0y0"], lb)
+    p = lib.make_params(lb, 0, maxR, apply_ctf=True, mask_on=True, norm_on=True, apix=apix, imgmask=d["mask"])
+    n = len(wins)
+    ctx.rec_begin(lb, maxR, layout=int(compact), rcut=maxR - 1 if compact else maxR, wg_init=0.01, ops=ops)
+    ctx.rec_insert(p, wins, _records(prog, np.arange(n), prog["prog_shifts"]))
+    if compact:
+        for o, k in zip(ctx.rec_get_tnf(maxR), TNF):
+            err = rel_l2(o, loc[f"locsub_{k}"])
+            print(f"local substep {k}: rel-L2 {err:.2e}")
+            assert err < 2e-5
+    else:
+        m = ctx.rec_finish(lb)
+        err = rel_l2(m, loc["loc_map"])
+        print(f"local map: rel-L2 {err:.2e}")
+        assert err < 2e-5
+
+
 def test_rec_begin_rejects_non_rotation_ops(ctx):
     ops = np.eye(3)[None] * 1.02
     with pytest.raises(lib.CimError):

@@ -32,3 +32,18 @@ def test_local_symmetry_operators_without_gpu():
     assert lib.sym_ops("icos").shape == (60, 3, 3)
     with pytest.raises(lib.CimError):
         lib.sym_ops("d7")
+
+
+def test_icosprocess_local_operators_without_gpu(O32):
+    """cim_icosprocess_sym_ops (host-only): the mode-6 local groups, against the
+    oracle's operators, which reproduce the reference's mode-6 map."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    import numpy as np
+    for sym, (N, _) in O32.LOCAL_SYM.items():
+        got = lib.icosprocess_sym_ops(sym)
+        assert got.shape == (N, 3, 3)
+        assert np.abs(got - O32.icosprocess_local_ops(sym)).max() < 1e-7, sym
+    for bad in ("c7", "icos", "i2", ""):
+        with pytest.raises(lib.CimError):
+            lib.icosprocess_sym_ops(bad)

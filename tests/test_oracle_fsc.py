@@ -151,3 +151,50 @@ def test_integer_shell_fold_keeps_the_outer_shell(O64):
     b = O64.fold_and_map(vol, maxR, int_radius=True)
     fa = np.fft.fftn(b) - np.fft.fftn(a)
     assert rel_l2(a, b) > 1e-3 and np.abs(fa).max() > 0
+
+
+# ---- icosprocess mode 6 (local reconstruction) --------------------------------
+
+@pytest.fixture(scope="module")
+def loc():
+    return np.load(pipelines.golden_path("local.npz"))
+
+
+def _local_setup(prog, loc):
+    box, apix = int(prog["prog_box"][0]), float(prog["prog_apix"][0])
+    lb = int(loc["loc_box"][0])
+    d = pipelines.derived(lb, apix, maxRes=6.0, imgmask_A=24)   # calc_initialvalues uses loc_box in mode 6
+    return lb, apix, d
+
+
+def test_local_symmetry_operators(O64):
+    """icos2f_sym_matc*: closed groups of z rotations of order N."""
+    for sym, (N, step) in O64.LOCAL_SYM.items():
+        ops = O64.icosprocess_local_ops(sym)
+        assert ops.shape == (N, 3, 3)
+        for k in range(N):
+            a = np.deg2rad(k * step)
+            assert np.allclose(ops[k], [[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]], atol=2e-7)
+
+
+@pytest.mark.parametrize("bits,tol", [(32, 1e-5), (64, 1e-4)])
+def test_local_map_matches_reference(bits, tol, O32, O64, prog, loc):
+    O = O32 if bits == 32 else O64
+    lb, apix, d = _local_setup(prog, loc)
+    assert int(loc["locsub_hdr"][1]) == d["maxR"]
+    vol = pipelines.local_volume(O, prog["prog_images"], prog["prog_angles"], prog["prog_shifts"], prog["prog_df"],
+                                 loc["loc_x0y0"], apix, d["maxR"], d["mask"], lb, "c5a")
+    m = O.fold_and_map(vol, d["maxR"])
+    err = rel_l2(m, loc["loc_map"])
+    print(f"FP{bits} oracle vs reference mode-6 map: rel-L2 {err:.2e}")
+    assert err < tol
+
+
+@pytest.mark.parametrize("bits,tol", [(32, 1e-5), (64, 1e-4)])
+def test_local_substep_tnf_matches_reference(bits, tol, O32, O64, prog, loc):
+    O = O32 if bits == 32 else O64
+    lb, apix, d = _local_setup(prog, loc)
+    vol = pipelines.local_volume(O, prog["prog_images"], prog["prog_angles"], prog["prog_shifts"], prog["prog_df"],
+                                 loc["loc_x0y0"], apix, d["maxR"], d["mask"], lb, "c5a", compact=True)
+    for o, k in zip(vol, TNF):
+        assert rel_l2(o, loc[f"locsub_{k}"]) < tol
