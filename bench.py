@@ -20,6 +20,10 @@ search of the reference's method and the plain FP32 insert.  The line
 reports both rates, how many particles get a different orientation, and the
 FSC between the two maps.
 
+`--stage center` measures only the centre finding of icosprocess mode 3
+(calc_center against the phantom map, and calc_center_selfrot180) on 224
+of those particles; the default run reports its rates under extra.
+
 Inputs are resident in HBM before the timed region.  The roofline block
 describes the kernel that takes most of a chosen step; the cpu_baseline
 block times the reference's own Fortran (oracle/_ref/icosprocess, MKL FFTW)
@@ -67,6 +71,8 @@ def _parse():
     ap.add_argument("--templates", type=int, default=4, help="template bank size T")
     ap.add_argument("--topk", type=int, default=32, help="AU points re-scored exactly per particle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage", choices=("main", "center"), default="main",
+                    help="center: measure only the centre finding of icosprocess mode 3 at box 384")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -195,6 +201,116 @@ def gpu_run(args, tang, timgs, ps):
     ang_err = np.abs(((out_p[:, 0:3] - ps.angles + 180) % 360) - 180)
     return dict(n=n, d=d, ev_c=ev_c, wall_c=wall_c, kt_c=kt_c, cnt_c=cnt_c, ev_p=ev_p, wall_p=wall_p, kt_p=kt_p,
                 cnt_p=cnt_p, identical=identical, within=within, fsc=fsc, map_rel=map_rel, ang_err=ang_err)
+
+
+CENTER_BATCH = 224       # one working chunk at box 384
+CENTER_METRIC = "particles/s centre finding (icosprocess mode 3, model3d), box 384"
+
+
+def center_run(ph, ps, steps: int, warmup: int):
+    """icosprocess mode 3 on CENTER_BATCH particles resident in HBM: calc_center
+    against the phantom map (CTF, mask, low-pass at maxR) and
+    calc_center_selfrot180 on the same images."""
+    import torch
+    from cryoim_amd import lib, synth
+    d = derived()
+    n = CENTER_BATCH
+    idx = np.arange(n) % len(ps.images)
+    ctx = lib.Context(0)
+    p = lib.make_params(BOX, 0, d["maxR"], apply_ctf=True, mask_on=True, apix=APIX, imgmask=d["mask"])
+    ctx.center_model(p, synth.phantom_map(ph))
+    images = torch.from_numpy(np.ascontiguousarray(ps.images[idx])).to("cuda:0")
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 0:3] = ps.angles[idx]
+    P[:, 3:5] = ps.shifts[idx]
+    P[:, 6:8] = ps.df[idx]
+    P[:, 11] = 1.0
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    res = {}
+    for name, method in (("model", lib.CENTER_MODEL), ("selfrot180", lib.CENTER_SELFROT180)):
+        for _ in range(warmup):
+            ctx.center(p, method, images, P)
+        ctx.sync()
+        ctx.timing(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            out, cc = ctx.center(p, method, images, P)
+        e1.record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        kt = {k: ctx.timing_get(k) for k in ("fft2d", "cen_stage", "cen_slice", "cen_elem", "cen_peak")}
+        ctx.timing(False)
+        err = np.abs(out[:, 3:5] - P[:, 3:5]).max(axis=1)
+        res[name] = {"particles_per_s": round(n / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
+                     "kernels": {k: {"avg_ms": round(v[0], 4), "launches": v[1]} for k, v in kt.items() if v[1]},
+                     "median_shift_error_px": round(float(np.median(err)), 3),
+                     "cc_median": round(float(np.median(cc)), 4)}
+    ctx.close()
+    return res
+
+
+def center_cpu_baseline(ph, ps):
+    """Reference icosprocess mode 3 with model3d on the host cores: the cost
+    per particle is the time of K particles minus that of 1, per process
+    (the difference removes each process's model read and 3D FFT)."""
+    from cryoim_amd import formats, synth
+    exe = os.path.join(ROOT, "oracle", "_ref", "icosprocess")
+    if not os.path.exists(exe):
+        return None
+    d = derived()
+    ncores = min(8, os.cpu_count() or 1)
+    m = len(ps.images)
+    K = 41
+    with tempfile.TemporaryDirectory() as tmp:
+        formats.write_mrc(os.path.join(tmp, "stk.mrc"), ps.images)
+        formats.write_mrc(os.path.join(tmp, "model.mrc"), synth.phantom_map(ph))
+        parts = [formats.Particle("stk.mrc", i + 1, *ps.angles[i % m], *ps.shifts[i % m], 0.0, i + 1, (i % m) + 1,
+                                  ps.df[i % m, 0], ps.df[i % m, 1], 0.0) for i in range(K * ncores)]
+        formats.write_ortfile(os.path.join(tmp, "ort.txt"), parts)
+        t = {}
+        for k in (1, K):
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([exe, "mode=3", "model3d=model.mrc", f"newortfile=c{i}.txt",
+                                       f"first={i * K + 1}", f"last={i * K + k}", f"apix={APIX}",
+                                       f"maxRes={MAXRES}", f"imgmask={d['mask'] * APIX}", "ortfile=ort.txt"],
+                                      cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                     for i in range(ncores)]
+            rcs = [pr.wait() for pr in procs]
+            t[k] = time.perf_counter() - t0
+            if any(rcs):
+                return None
+    per = (t[K] - t[1]) / (K - 1)
+    return {"value": round(ncores / per, 3), "unit": "particles/s", "cores": ncores, "kind": "reference",
+            "sample": (f"icosprocess mode 3 (flang -O2, MKL FFTW) as {ncores} concurrent processes, {K} particles "
+                       f"each ({t[K]:.1f} s) minus 1 each ({t[1]:.1f} s: model read and 3D FFT)")}
+
+
+def center_main(args):
+    ph, _, _, ps = make_workload(args.batch, args.templates)
+    r = center_run(ph, ps, args.steps, args.warmup)
+    m = r["model"]
+    n = CENTER_BATCH
+    fft = m["kernels"]["fft2d"]
+    # one fft2d call: row and column passes, each reading and writing n box^2 complex64
+    nbytes = 4 * n * BOX * BOX * 8
+    achieved = nbytes / (fft["avg_ms"] / 1e3) / 1e9
+    cpu = None if args.no_cpu_baseline else center_cpu_baseline(ph, ps)
+    line = {"metric": CENTER_METRIC, "value": m["particles_per_s"], "unit": "particles/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (the main workload's particles, tiled to the batch)",
+            "config": {"workload": f"calc_center on {n} particles of box {BOX}, maxR {derived()['maxR']}",
+                       "box": BOX, "batch": n},
+            "roofline": {"bound": "hbm", "kernel": "fft2d (k_fft_lines row + column pass)",
+                         "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(achieved / PEAK_HBM_GBS, 3), "traffic": None,
+                         "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": fft["avg_ms"],
+                         "launches": fft["launches"]},
+            "cpu_baseline": cpu,
+            "extra": r}
+    print(json.dumps(line))
 
 
 def _grid_work(step: float) -> int:
@@ -330,9 +446,13 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.stage == "center":
+        center_main(args)
+        return
     t_setup = time.perf_counter()
     ph, tang, timgs, ps = make_workload(args.batch, args.templates)
     r = gpu_run(args, tang, timgs, ps)
+    cen = center_run(ph, ps, args.steps, args.warmup)
     n = r["n"]
     value = n * args.steps / (r["ev_c"] / 1e3)
     plain = n / (r["ev_p"] / 1e3)
@@ -386,6 +506,8 @@ def main():
                   "chosen_kernels": kt,
                   "plain_search_avg_ms": round(r["kt_p"]["cl_search"][0], 3),
                   "search_angle_err_deg_median": float(np.median(r["ang_err"])),
+                  "center_mode3": {k: {kk: v[kk] for kk in ("particles_per_s", "ms_per_step")}
+                                   for k, v in cen.items()},
                   "wall_s": round(r["wall_c"], 3),
                   "setup_s": round(time.perf_counter() - t_setup, 1)},
     }

@@ -69,6 +69,16 @@ __device__ __forceinline__ float2 apply_shift(float2 v, float2 cs) {
     return make_float2(v.x * cs.x + v.y * cs.y, -v.x * cs.y + v.y * cs.x);
 }
 
+// euler_mat.f90:3-21 in float32: R row-major, R[3*i+j] = RotMat(i+1, j+1).
+__device__ __forceinline__ void euler_f(float th0, float ph0, float om0, float* R) {
+    const float d2r = 0.0174532925199432957692f;   // atan2(1,1)/45
+    float th = th0 * d2r, ph = ph0 * d2r, om = om0 * d2r;
+    float c1 = cosf(ph), c2 = cosf(th), c3 = cosf(om), s1 = sinf(ph), s2 = sinf(th), s3 = sinf(om);
+    R[0] = c1 * c2 * c3 - s1 * s3; R[1] = -c1 * c2 * s3 - c3 * s1; R[2] = c1 * s2;
+    R[3] = c2 * c3 * s1 + c1 * s3; R[4] = -c2 * s1 * s3 + c1 * c3; R[5] = s1 * s2;
+    R[6] = -c3 * s2;               R[7] = s2 * s3;                 R[8] = c2;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -95,6 +95,19 @@ struct LrefState {
     }
 };
 
+// Resident model transform and batch buffers of the centre finder.
+struct CenterState {
+    bool ready = false;
+    int box = 0;
+    DevBuf model;      // float2 box^3: {Re, -Im} of the checkerboarded model FT
+    DevBuf x1, x2;     // float2 [chunk][box^2] working images
+    DevBuf acc;        // double[2] shell sums of the model mask
+    DevBuf img, parts, cc;
+    void release() {
+        for (DevBuf* b : {&model, &x1, &x2, &acc, &img, &parts, &cc}) b->release();
+    }
+};
+
 struct KTimer {
     double total_ms = 0.0;
     long long launches = 0;
@@ -110,6 +123,7 @@ struct cim_ctx {
     cim::RecState rec;
     cim::SearchState srch;
     cim::LrefState lref;
+    cim::CenterState cen;
     // scratch buffers reused across calls
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
     // half-map FSC: input tnf pairs, normalised copies, merged sums, shell sums, weights

@@ -19,15 +19,6 @@ namespace cim {
 int upload_parts(cim_ctx* ctx, int n, const float* parts);
 int check_params_public(cim_ctx* ctx, const cim_params* p);
 
-__device__ __forceinline__ void euler_lf(float th0, float ph0, float om0, float* R) {
-    const float d2r = 0.0174532925199432957692f;
-    float th = th0 * d2r, ph = ph0 * d2r, om = om0 * d2r;
-    float c1 = cosf(ph), c2 = cosf(th), c3 = cosf(om), s1 = sinf(ph), s2 = sinf(th), s3 = sinf(om);
-    R[0] = c1 * c2 * c3 - s1 * s3; R[1] = -c1 * c2 * s3 - c3 * s1; R[2] = c1 * s2;
-    R[3] = c2 * c3 * s1 + c1 * s3; R[4] = -c2 * s1 * s3 + c1 * c3; R[5] = s1 * s2;
-    R[6] = -c3 * s2;               R[7] = s2 * s3;                 R[8] = c2;
-}
-
 // ---- model preparation: softmask (img3d_softmask.f90), checkerboard, FFT ----
 
 __global__ void k_softmask_sum(const float* __restrict__ v, int n, float inn, float out, double* __restrict__ acc) {
@@ -227,7 +218,7 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
             if (tid == 0) {
                 const float r2d = 57.2957795130823208768f;
                 float E[9], M[9];
-                euler_lf(cur[0], -cur[1], 180.0f - cur[2], E);
+                euler_f(cur[0], -cur[1], 180.0f - cur[2], E);
                 const float* S = a.symops + 9 * s;
                 for (int i = 0; i < 3; ++i)
                     for (int j = 0; j < 3; ++j) M[3 * i + j] = S[3 * i] * E[j] + S[3 * i + 1] * E[3 + j] + S[3 * i + 2] * E[6 + j];
@@ -237,7 +228,7 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
             }
             __syncthreads();
             float R[9];
-            euler_lf(symang[0], symang[1], symang[2], R);
+            euler_f(symang[0], symang[1], symang[2], R);
             float num[1] = {0.f}, den[1] = {0.f};
             for (int q = tid; q < a.nq; q += blockDim.x) {
                 const int2 hk = a.disk[q];
@@ -285,8 +276,8 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
         }
         __syncthreads();
         float Rs[7][9];
-        euler_lf(cand[0][0], cand[0][1], cand[0][2], Rs[0]);
-        for (int c = 5; c <= 10; ++c) euler_lf(cand[c][0], cand[c][1], cand[c][2], Rs[c - 4]);
+        euler_f(cand[0][0], cand[0][1], cand[0][2], Rs[0]);
+        for (int c = 5; c <= 10; ++c) euler_f(cand[c][0], cand[c][1], cand[c][2], Rs[c - 4]);
         float num[11], den[11];
         for (int c = 0; c < 11; ++c) { num[c] = 0.f; den[c] = 0.f; }
         for (int q = tid; q < a.nq; q += blockDim.x) {

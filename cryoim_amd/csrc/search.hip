@@ -28,15 +28,6 @@ int check_params_public(cim_ctx* ctx, const cim_params* p);
 
 // ---- geometry shared by both kernels (float32, as the reference) ---------
 
-__device__ __forceinline__ void euler_f(float th0, float ph0, float om0, float* R) {
-    const float d2r = 0.0174532925199432957692f;   // atan2(1,1)/45
-    float th = th0 * d2r, ph = ph0 * d2r, om = om0 * d2r;
-    float c1 = cosf(ph), c2 = cosf(th), c3 = cosf(om), s1 = sinf(ph), s2 = sinf(th), s3 = sinf(om);
-    R[0] = c1 * c2 * c3 - s1 * s3; R[1] = -c1 * c2 * s3 - c3 * s1; R[2] = c1 * s2;
-    R[3] = c2 * c3 * s1 + c1 * s3; R[4] = -c2 * s1 * s3 + c1 * c3; R[5] = s1 * s2;
-    R[6] = -c3 * s2;               R[7] = s2 * s3;                 R[8] = c2;
-}
-
 // icos_crosslin_angle.f90:11-34 for one operator S (row-major float[9]).
 __device__ __forceinline__ void crosslin_one(const float* E1, const float* E2, const float* S, float* a1, float* a2) {
     const float r2d = 57.2957795130823208768f;

@@ -11,6 +11,9 @@ module cim_iface
     integer(c_int), parameter :: CIM_OK = 0
     integer(c_int), parameter :: CIM_PART_STRIDE = 12
     integer(c_int), parameter :: CIM_FSC_RECONSTRUCT = 0, CIM_FSC_MERGE = 1
+    ! CIM_CENTER_MODEL / CIM_CENTER_SELFROT180 of cryoim.h (Fortran names are
+    ! case-insensitive, so CIM_CENTER_MODEL would clash with cim_center_model)
+    integer(c_int), parameter :: CIM_CTR_MODEL = 0, CIM_CTR_SELFROT180 = 1
     ! 1-based positions of the record fields in rec(:, i)
     integer, parameter :: P_THETA = 1, P_PHI = 2, P_OMEGA = 3, P_X = 4, P_Y = 5, P_PR = 6
     integer, parameter :: P_DF1 = 7, P_DF2 = 8, P_ASTIG = 9, P_MASKX = 10, P_MASKY = 11, P_USE = 12
@@ -112,6 +115,21 @@ module cim_iface
             type(cim_params) :: p
             integer(c_int), value :: n
             real(c_float) :: images(*), parts(*)
+        end function
+
+        integer(c_int) function cim_center_model(ctx, p, model) bind(C, name='cim_center_model')
+            import :: c_ptr, c_int, c_float, cim_params
+            type(c_ptr), value :: ctx
+            type(cim_params) :: p
+            real(c_float) :: model(*)
+        end function
+
+        integer(c_int) function cim_center(ctx, p, method, n, images, parts, cc) bind(C, name='cim_center')
+            import :: c_ptr, c_int, c_float, cim_params
+            type(c_ptr), value :: ctx
+            type(cim_params) :: p
+            integer(c_int), value :: method, n
+            real(c_float) :: images(*), parts(*), cc(*)
         end function
 
         integer(c_int) function cim_refine_local(ctx, p, n, images, parts) bind(C, name='cim_refine_local')

@@ -4,6 +4,8 @@
 !   mode=1            3D reconstruction (reconstruct.f90; subrec=y writes
 !                     the tnf partial volume of reconstruct_substep.f90)
 !   mode=4            global orientation search (ort_globesearch.f90)
+!   mode=3            centre finding: calc_center.f90 against model3d, or
+!                     calc_center_selfrot180.f90 without it; output_ort.f90
 !   mode=5            local orientation/centre refinement (ort_refinement.f90)
 !   mode=6            local reconstruction of loc_box windows around (x0, y0)
 !                     with a local symmetry (reconstruction_local.f90;
@@ -18,7 +20,7 @@ program icosprocess_hip
     character(len=64) :: coeff(NPAR)
     character(len=256) :: arg, key, val
     character(len=256) :: ortfile = '', newortfile = 'newort.dat', result3d = 'map3d.mrc'
-    character(len=256) :: templateort = '', templatestck = '', stck0
+    character(len=256) :: templateort = '', templatestck = '', stck0, model3d = ''
     character(len=4) :: sym = 'icos'
     character :: applyctf = 'y', imgmask_on = 'y', subrec = 'n', realspaceavg = 'n'
     integer :: mode = 1, first = 1, last = 0, ncycle = 6, nargs, i, j, k, eq, fftsize, minR, maxR, T
@@ -43,7 +45,7 @@ program icosprocess_hip
              'imgstck', 'sym', 'maxcentshift', 'corrCTF', 'boundX', 'loc_box', 'check']
     nargs = command_argument_count()
     if (nargs < 2) then
-        print *, 'usage: icosprocess_hip mode=<1|4|5> ortfile=... [key=value ...] (icosprocess keys)'
+        print *, 'usage: icosprocess_hip mode=<1|3|4|5|6> ortfile=... [key=value ...] (icosprocess keys)'
         stop
     end if
     do i = 1, nargs
@@ -83,10 +85,11 @@ program icosprocess_hip
         case ('sym'); sym = val(1:4)
         case ('boundX'); read(val, *) boundx; have_boundx = .true.
         case ('loc_box'); read(val, *) loc_box
+        case ('model3d'); model3d = val
         end select
     end do
     if (mode /= 6 .and. trim(sym) /= 'icos') then
-        print *, 'icosprocess_hip runs modes 1, 4, 5 with the icosahedral (sym=icos) symmetry only'
+        print *, 'icosprocess_hip runs modes 1, 3, 4, 5 with the icosahedral (sym=icos) symmetry only'
         stop 2
     end if
 
@@ -149,12 +152,14 @@ program icosprocess_hip
     select case (mode)
     case (1)
         call run_reconstruct()
+    case (3)
+        call run_center()
     case (4, 5)
         call run_search()
     case (6)
         call run_local()
     case default
-        print *, 'icosprocess_hip implements modes 1, 4, 5 and 6'
+        print *, 'icosprocess_hip implements modes 1, 3, 4, 5 and 6'
         stop 2
     end select
     rc = cim_finalize(ctx)
@@ -459,6 +464,50 @@ contains
             end do
         end do
         close(41)
+    end subroutine
+
+    ! icosprocess mode 3 (0main.f90:237-241): calc_center when model3d is
+    ! given, calc_center_selfrot180 otherwise, then output_ort.f90:1-17
+    ! (dtheta, dphi, domega stay 0 from load_original_ortcent.f90:7).
+    subroutine run_center()
+        integer, parameter :: B = 1024
+        real(c_float), allocatable :: imgs(:), rec(:, :), vol(:), cc(:)
+        integer :: i0, i1, i, k, u
+        integer(c_int) :: method
+        if (len_trim(model3d) > 0) then
+            method = CIM_CTR_MODEL
+            allocate(vol(int(fftsize, 8)**3))
+            open(newunit=u, file=trim(model3d), form='unformatted', access='stream', status='old')
+            read(u, pos=1025) vol                     ! calc_center.f90:21-23: header, then FFTsize**3 reals
+            close(u)
+            rc = cim_center_model(ctx, p, vol)
+            call cim_check(ctx, rc, 'cim_center_model')
+            deallocate(vol)
+        else
+            method = CIM_CTR_SELFROT180
+        end if
+        allocate(imgs(fftsize * fftsize * B), rec(CIM_PART_STRIDE, B), cc(B))
+        open(41, file=trim(newortfile))
+        do i0 = first, last, B
+            i1 = min(last, i0 + B - 1)
+            call fill_records(i0, i1, rec, .true., .false., -1.0)
+            do i = i0, i1
+                k = i - i0 + 1
+                call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))
+            end do
+            rc = cim_center(ctx, p, method, i1 - i0 + 1, imgs, rec, cc)
+            call cim_check(ctx, rc, 'cim_center')
+            do i = i0, i1
+                k = i - i0 + 1
+                print *, i, rec(P_X, k), rec(P_Y, k), cc(k)
+                write(41, 900) part(i)%stck, part(i)%pid, part(i)%theta, part(i)%phi, part(i)%omega, rec(P_X, k), &
+                               rec(P_Y, k), rec(P_PR, k), part(i)%imgid, part(i)%imgpid, part(i)%df1, part(i)%df2, &
+                               part(i)%astig, 0.0, 0.0, 0.0, rec(P_X, k) - part(i)%x, rec(P_Y, k) - part(i)%y, &
+                               rec(P_PR, k) - part(i)%pr
+            end do
+        end do
+        close(41)
+900     format(a54,i8,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,i12,i6,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4,f10.4)
     end subroutine
 
     ! ort_globesearch.f90:109-139 / ort_refinement.f90:96-122

@@ -17,6 +17,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cryoim.h")
 
 PART_STRIDE = 12
 SEARCH_PLAIN, SEARCH_TABLES = 0, 1
+CENTER_MODEL, CENTER_SELFROT180 = 0, 1
 P_THETA, P_PHI, P_OMEGA, P_X, P_Y, P_PR, P_DF1, P_DF2, P_ASTIG, P_MASKX, P_MASKY, P_USE = range(12)
 P_BOXX, P_BOXY = P_MASKX, P_MASKY
 
@@ -124,6 +125,9 @@ def load() -> ctypes.CDLL:
         "cim_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
         "cim_icosprocess_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
         "cim_lref_model": (ip, [vp, P, fp]),
+        "cim_center_model": (ip, [vp, P, fp]),
+        "cim_center": (ip, [vp, P, ip, ip, fp, fp, fp]),
+        "cim_center_dev": (ip, [vp, P, ip, ip, vp, fp, fp]),
         "cim_lref_refine": (ip, [vp, P, ip, fp, ip, fp, ctypes.c_char_p, ctypes.c_float, ctypes.c_float]),
     }
     for name, (res, args) in sig.items():
@@ -333,6 +337,26 @@ class Context:
                                              _fptr(parts), sym.encode(), float(convF), float(maxres)),
                     "cim_lref_refine")
         return parts
+
+    def center_model(self, params: CimParams, model):
+        """calc_center's model transform (cim_center_model), kept on the device."""
+        v = _f32(model)
+        self._check(self.lib.cim_center_model(self.h, ctypes.byref(params), _fptr(v)), "cim_center_model")
+
+    def center(self, params: CimParams, method: int, images, parts: np.ndarray):
+        """icosprocess mode 3 (cim_center / cim_center_dev): returns the updated
+        records (x, y, PR) and the unclipped fitted peak values."""
+        parts = _f32(parts).reshape(-1, PART_STRIDE).copy()
+        n = parts.shape[0]
+        cc = np.zeros(n, np.float32)
+        if hasattr(images, "data_ptr"):
+            rc = self.lib.cim_center_dev(self.h, ctypes.byref(params), int(method), n,
+                                         ctypes.c_void_p(images.data_ptr()), _fptr(parts), _fptr(cc))
+        else:
+            im = _f32(images)
+            rc = self.lib.cim_center(self.h, ctypes.byref(params), int(method), n, _fptr(im), _fptr(parts), _fptr(cc))
+        self._check(rc, "cim_center")
+        return parts, cc
 
     def refine_local(self, params: CimParams, images, parts: np.ndarray) -> np.ndarray:
         parts = _f32(parts).reshape(-1, PART_STRIDE).copy()

@@ -53,13 +53,19 @@ def phantom_map(ph: Phantom, box: Optional[int] = None) -> np.ndarray:
     n = box or ph.box
     c = n // 2
     ax = np.arange(n, dtype=np.float64) - c
-    vol = np.zeros((n, n, n))
-    for (px, py, pz), s, a in zip(ph.centres, ph.sigmas, ph.amps):
-        gx = np.exp(-(ax - px) ** 2 / (2 * s * s))
-        gy = np.exp(-(ax - py) ** 2 / (2 * s * s))
-        gz = np.exp(-(ax - pz) ** 2 / (2 * s * s))
-        vol += a * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
-    return vol.astype(np.float32)
+    inv = 1.0 / (2 * ph.sigmas ** 2)
+    gx = np.exp(-(ax[None, :] - ph.centres[:, 0:1]) ** 2 * inv[:, None])   # (nb, n)
+    gy = np.exp(-(ax[None, :] - ph.centres[:, 1:2]) ** 2 * inv[:, None])
+    gz = np.exp(-(ax[None, :] - ph.centres[:, 2:3]) ** 2 * inv[:, None]) * ph.amps[:, None]
+    # the map is a sum of nb separable blobs: vol[z, y*n + x] = gz^T (gy (x) gx),
+    # one matrix product per block of blobs
+    vol = np.zeros((n, n * n))
+    step = max(1, (1 << 27) // (n * n))
+    for b0 in range(0, len(inv), step):
+        b1 = min(len(inv), b0 + step)
+        gyx = (gy[b0:b1, :, None] * gx[b0:b1, None, :]).reshape(b1 - b0, n * n)
+        vol += gz[b0:b1].T @ gyx
+    return vol.reshape(n, n, n).astype(np.float32)
 
 
 def project(ph: Phantom, angles: np.ndarray, shifts: Optional[np.ndarray] = None,

@@ -244,6 +244,26 @@ int cim_lref_model(cim_ctx* ctx, const cim_params* p, const float* vol);
 int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, float* parts,
                     const char* sym, float convF, float maxres);
 
+/* ---- centre finding (icosprocess mode 3) -------------------------------- */
+#define CIM_CENTER_MODEL 0        /* calc_center.f90:2-113 (model3d given) */
+#define CIM_CENTER_SELFROT180 1   /* calc_center_selfrot180.f90:1-95 */
+/* Resident model of calc_center.f90:16-41: box^3 real map (the model3d
+ * data, box = p->box), img3d_mask of radius p->imgmask when p->mask_on,
+ * checkerboard, forward 3D FFT (1/N), checkerboard, imaginary part negated. */
+int cim_center_model(cim_ctx* ctx, const cim_params* p, const float* model);
+/* Centre n particles (the loops of calc_center.f90:44-102 and
+ * calc_center_selfrot180.f90:18-92).  images: n*box*box real.  parts: n
+ * records; reads theta, phi, omega, df1, df2, astig (CTF when
+ * p->apply_ctf, CIM_CENTER_MODEL only) and x, y (the mask centre of the
+ * particle image, CIM_CENTER_MODEL); writes x, y (the fitted centre, 0 where
+ * |shift| >= box/16) and CIM_P_PR = acos(cc) in degrees.  Low-pass radius
+ * p->maxR; p->mask_on / p->imgmask as imgmask.  cc (may be NULL): the n
+ * fitted peak values before clipping to [-1, 1]. */
+int cim_center(cim_ctx* ctx, const cim_params* p, int method, int n, const float* images, float* parts, float* cc);
+/* Same with the images already in device memory. */
+int cim_center_dev(cim_ctx* ctx, const cim_params* p, int method, int n, const void* images_dev, float* parts,
+                   float* cc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1168,3 +1168,254 @@ EXPORT real o_lref_refine(const real* fr, const real* fi, int nx, const real* a2
     free(a1); free(b1);
     return best;
 }
+
+/* ------------------------------------------------------------------ */
+/* Centre finding (icosprocess mode 3)                                  */
+/* ------------------------------------------------------------------ */
+
+/* img3d_mask: icosprocess/SRC/img3d_mask.f90:2-49 (shell mean in double,
+ * cosine edge of half width 3 around the mask radius, outside = mean). */
+EXPORT void o_img3d_mask(real* d, int nx, real mask, real ox, real oy, real oz)
+{
+    int halfw = 6;
+    real pi = r_pi();
+    real R1 = mask - (real)(halfw / 2);
+    if (R1 <= 0) R1 = 0;
+    real R2 = mask + (real)(halfw / 2);
+    if (R2 > (real)(nx / 2)) R2 = (real)(nx / 2);
+    double summ = 0;
+    long num = 0;
+    real x0 = ox + (real)(nx / 2), y0 = oy + (real)(nx / 2), z0 = oz + (real)(nx / 2);
+    long n2 = (long)nx * nx, N = n2 * nx;
+    for (long i = 0; i < N; ++i) {
+        int iz = (int)(i / n2), iy = (int)((i % n2) / nx), ix = (int)(i % nx);
+        real yy = (-y0 + iy) * (-y0 + iy), xx = (-x0 + ix) * (-x0 + ix), zz = (-z0 + iz) * (-z0 + iz);
+        real ra = (real)sqrt(xx + yy + zz);
+        if (ra > R1 && ra <= R2) { summ += d[i]; num++; }
+    }
+    if (num != 0) summ = summ / (double)(float)num;
+    for (long i = 0; i < N; ++i) {
+        int iz = (int)(i / n2), iy = (int)((i % n2) / nx), ix = (int)(i % nx);
+        real yy = (-y0 + iy) * (-y0 + iy), xx = (-x0 + ix) * (-x0 + ix), zz = (-z0 + iz) * (-z0 + iz);
+        real ra = (real)sqrt(xx + yy + zz);
+        if (ra > R1 && ra <= R2) {
+            real edge = ((real)1.0 + (real)cos(pi * (ra - R1) / (real)halfw)) / (real)2.0;
+            d[i] = d[i] * edge + ((real)1.0 - edge) * (real)summ;
+        }
+        if (ra > R2) d[i] = (real)summ;
+    }
+}
+
+/* The model transform of calc_center.f90:19-41: optional img3d_mask at the
+ * origin, checkerboard, forward 3D FFT (scaled 1/N^3), checkerboard;
+ * fr = Re, fi = -Im.  vol is overwritten by the masked model. */
+EXPORT void o_center_model(real* vol, int nx, real mask, real* fr, real* fi)
+{
+    long n2 = (long)nx * nx, N = n2 * nx;
+    if (mask >= 0) o_img3d_mask(vol, nx, mask, 0, 0, 0);
+    for (long i = 0; i < N; ++i) {
+        long s = i / n2 + (i % n2) / nx + i % nx;
+        fr[i] = (s & 1) ? -vol[i] : vol[i];
+        fi[i] = 0;
+    }
+    o_img3d_fft(fr, fi, nx, 1, 0);
+    for (long i = 0; i < N; ++i) {
+        long s = i / n2 + (i % n2) / nx + i % nx;
+        if (s & 1) { fr[i] = -fr[i]; fi[i] = -fi[i]; }
+        fi[i] = -fi[i];
+    }
+}
+
+/* img2d_lowpass: icosprocess/SRC/img2d_lowpass.f90:2-37.  Gaussian
+ * exp(-r^2 / (2 lp^2)) about the centred origin between a checkerboarded
+ * forward and backward img2d_FFT; the imaginary part is dropped. */
+EXPORT void o_img2d_lowpass(real* a, int nx, int lp)
+{
+    long N = (long)nx * nx;
+    real* b = (real*)calloc(N, sizeof(real));
+    real lowpass = (real)2.0 * (real)(lp * lp);
+    o_checker2d(a, nx);
+    o_img2d_fft(a, b, nx, 1, 1);
+    for (long i = 0; i < N; ++i) {
+        int ky = (int)(i / nx), hx = (int)(i % nx);
+        real r2 = (real)((hx - nx / 2) * (hx - nx / 2) + (ky - nx / 2) * (ky - nx / 2));
+        real H = (real)exp(-r2 / lowpass);
+        a[i] = a[i] * H / (real)N;
+        b[i] = b[i] * H / (real)N;
+    }
+    o_img2d_fft(a, b, nx, -1, 1);
+    o_checker2d(a, nx);
+    free(b);
+}
+
+/* img2d_applyCTF: icosprocess/SRC/img2d_applyCTF.f90:1-27. */
+EXPORT void o_img2d_applyctf(real* a, int nx, const real* ctf)
+{
+    long N = (long)nx * nx;
+    real* b = (real*)calloc(N, sizeof(real));
+    o_checker2d(a, nx);
+    o_img2d_fft(a, b, nx, 1, 1);
+    for (long i = 0; i < N; ++i) { a[i] *= ctf[i]; b[i] *= ctf[i]; }
+    o_img2d_fft(a, b, nx, -1, 1);
+    o_checker2d(a, nx);
+    free(b);
+}
+
+/* project2d_FT3dTo2d: icosprocess/SRC/project2d_FT3dTo2d.f90:2-94.  Central
+ * slice of (fr, fi) by trilinear interpolation inside radius nx/2-2, then a
+ * checkerboarded backward img2d_FFT; the real part is the projection. */
+EXPORT void o_project2d_ft(const real* fr, const real* fi, int nx, real th, real ph, real om, real* out)
+{
+    long N = (long)nx * nx, n2 = N;
+    real R[9];
+    real* oi = (real*)calloc(N, sizeof(real));
+    o_euler_mat(th, ph, om, R);
+    long n = 0;
+    for (int iy = -nx / 2; iy < nx / 2; ++iy)
+        for (int ix = -nx / 2; ix < nx / 2; ++ix, ++n) {
+            out[n] = 0;
+            real X[3] = {(real)ix, (real)iy, 0}, x[3];
+            matvec(R, X, x);
+            if ((real)sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) > (real)(nx / 2 - 2)) continue;
+            x[0] += (real)(nx / 2); x[1] += (real)(nx / 2); x[2] += (real)(nx / 2);
+            int hx = (int)x[0], ky = (int)x[1], lz = (int)x[2];
+            real xb = x[0] - (real)hx, yb = x[1] - (real)ky, zb = x[2] - (real)lz;
+            long c = (long)lz * n2 + (long)ky * nx + hx;
+            const real* src[2] = {fr, fi};
+            real v[2];
+            for (int k = 0; k < 2; ++k) {
+                const real* f = src[k];
+                v[k] = ((real)1. - xb) * ((real)1. - yb) * ((real)1. - zb) * f[c] +
+                       xb * ((real)1. - yb) * ((real)1. - zb) * f[c + 1] +
+                       ((real)1. - xb) * yb * ((real)1. - zb) * f[c + nx] +
+                       xb * yb * ((real)1. - zb) * f[c + nx + 1] +
+                       ((real)1. - xb) * ((real)1. - yb) * zb * f[c + n2] +
+                       xb * ((real)1. - yb) * zb * f[c + n2 + 1] +
+                       ((real)1. - xb) * yb * zb * f[c + n2 + nx] +
+                       xb * yb * zb * f[c + n2 + nx + 1];
+            }
+            out[n] = v[0];
+            oi[n] = v[1];
+        }
+    o_checker2d(out, nx);
+    o_checker2d(oi, nx);
+    o_img2d_fft(out, oi, nx, -1, 1);
+    o_checker2d(out, nx);
+    free(oi);
+}
+
+/* cross_correlation: icosprocess/SRC/cross_correlation.f90:1-28.  a1 is
+ * overwritten by the correlation map (peak at the centre for zero shift). */
+EXPORT void o_cross_correlation(real* a1, real* a2, int nx)
+{
+    long N = (long)nx * nx;
+    real* b1 = (real*)calloc(N, sizeof(real));
+    real* b2 = (real*)calloc(N, sizeof(real));
+    o_img2d_fft(a1, b1, nx, 1, 1);
+    o_img2d_fft(a2, b2, nx, 1, 1);
+    for (long i = 0; i < N; ++i) {
+        int ky = (int)(i / nx), hx = (int)(i % nx);
+        real x = a1[i] * a2[i] + b1[i] * b2[i];
+        real y = b1[i] * a2[i] - a1[i] * b2[i];
+        a1[i] = ((hx + ky) & 1) ? -x : x;
+        b1[i] = ((hx + ky) & 1) ? -y : y;
+    }
+    o_img2d_fft(a1, b1, nx, -1, 1);
+    free(b1); free(b2);
+}
+
+/* guass_regression: icosprocess/SRC/guass_regression.f90:1-56.  Least-
+ * squares quadric on log z over the 3x3 neighbourhood z[(j+1)*3 + i+1] =
+ * z(i, j); returns the sub-pixel offset and the fitted peak value. */
+EXPORT void o_gauss_regression(const real* z, real* dx, real* dy, real* cc)
+{
+    real c10 = 0, c01 = 0, c11 = 0, c20 = 0, c02 = 0, c00 = 0;
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) c10 += (real)i * (real)log(z[(j + 1) * 3 + i + 1]);
+    c10 /= (real)6.0;
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) c01 += (real)j * (real)log(z[(j + 1) * 3 + i + 1]);
+    c01 /= (real)6.0;
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) c11 += (real)(i * j) * (real)log(z[(j + 1) * 3 + i + 1]);
+    c11 /= (real)4.0;
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) c20 += (real)(3 * i * i - 2) * (real)log(z[(j + 1) * 3 + i + 1]);
+    c20 /= (real)6.0;
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) c02 += (real)(3 * j * j - 2) * (real)log(z[(j + 1) * 3 + i + 1]);
+    c02 /= (real)6.0;
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) c00 += (real)(5 - 3 * i * i - 3 * j * j) * (real)log(z[(j + 1) * 3 + i + 1]);
+    c00 /= (real)9.0;
+    real den = (real)4.0 * c20 * c02 - c11 * c11;
+    *dx = (c11 * c01 - (real)2.0 * c10 * c02) / den;
+    *dy = (c11 * c10 - (real)2.0 * c01 * c20) / den;
+    *cc = (real)exp(c00 - c20 * *dx * *dx - c11 * *dx * *dy - c02 * *dy * *dy);
+}
+
+/* The peak step shared by calc_center.f90:70-91 and
+ * calc_center_selfrot180.f90:58-79: first maximum in memory order, the 3x3
+ * values around it at flat indices (iy0+j)*nx + ix0+i (a neighbour past a
+ * row end reads the adjacent row; one outside the image reads 0), the
+ * quadric fit, the shift (halved for the 180-degree self-correlation),
+ * zeroed at |shift| >= nx/16.  out = {x, y, cc}; cc is not clipped. */
+EXPORT void o_center_peak(const real* ccmap, int nx, int halve, real* out)
+{
+    long N = (long)nx * nx, imax = 0;
+    for (long i = 1; i < N; ++i)
+        if (ccmap[i] > ccmap[imax]) imax = i;
+    int iy0 = (int)(imax / nx), ix0 = (int)(imax % nx);
+    real z[9];
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) {
+            long k = (long)(iy0 + j) * nx + ix0 + i;
+            z[(j + 1) * 3 + i + 1] = (k >= 0 && k < N) ? ccmap[k] : (real)0;
+        }
+    real dx, dy, cc;
+    o_gauss_regression(z, &dx, &dy, &cc);
+    real cx = (real)(ix0 - nx / 2) + dx, cy = (real)(iy0 - nx / 2) + dy;
+    if (halve) { cx = cx / (real)2.0; cy = cy / (real)2.0; }
+    if (fabs(cx) >= (real)nx / (real)16.0) cx = 0;
+    if (fabs(cy) >= (real)nx / (real)16.0) cy = 0;
+    out[0] = cx; out[1] = cy; out[2] = cc;
+}
+
+/* calc_center.f90:44-76 for one particle: img is the particle image
+ * (overwritten), (fr, fi) the model transform, ctf the particle's CTF (NULL:
+ * applyCTF off), mask < 0: imgmask off. */
+EXPORT void o_calc_center_one(const real* fr, const real* fi, real* img, int nx, real x, real y, real th, real ph,
+                              real om, const real* ctf, real mask, int maxR, real* out)
+{
+    long N = (long)nx * nx;
+    real* a2 = (real*)malloc(sizeof(real) * N);
+    if (mask >= 0) o_img2d_mask(img, nx, mask, x, y);
+    o_img2d_lowpass(img, nx, maxR);
+    o_project2d_ft(fr, fi, nx, th, ph, om, a2);
+    if (ctf) o_img2d_applyctf(a2, nx, ctf);
+    if (mask >= 0) o_img2d_mask(a2, nx, mask, 0, 0);
+    o_img2d_lowpass(a2, nx, maxR);
+    o_img2d_norm(img, nx);
+    o_img2d_norm(a2, nx);
+    o_cross_correlation(img, a2, nx);
+    o_center_peak(img, nx, 0, out);
+    free(a2);
+}
+
+/* calc_center_selfrot180.f90:33-79 for one particle (img overwritten). */
+EXPORT void o_center_selfrot180_one(real* img, int nx, real mask, int maxR, real* out)
+{
+    long N = (long)nx * nx;
+    real* a2 = (real*)calloc(N, sizeof(real));
+    o_img2d_lowpass(img, nx, maxR);
+    if (mask >= 0) o_img2d_mask(img, nx, mask, 0, 0);
+    o_img2d_norm(img, nx);
+    for (long i = 0; i < N; ++i) {
+        int ky = (int)(i / nx), hx = (int)(i % nx);
+        if (hx == 0 || ky == 0) img[i] = 0;
+        else a2[(long)(nx - ky) * nx + (nx - hx)] = img[i];
+    }
+    o_cross_correlation(img, a2, nx);
+    o_center_peak(img, nx, 1, out);
+    free(a2);
+}

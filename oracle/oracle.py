@@ -311,3 +311,70 @@ class Oracle:
                                   int(ncycle), self._r(refine_angle), self._r(refine_shift), self._r(convF),
                                   self._r(maxres), self._p(io5))
         return io5, pr
+
+    # ---- centre finding (icosprocess mode 3) -------------------------------
+    def img3d_mask(self, vol, mask, origin=(0.0, 0.0, 0.0)):
+        v = self._a(vol).copy()
+        self.L.o_img3d_mask(self._p(v), v.shape[-1], self._r(mask), *[self._r(o) for o in origin])
+        return v
+
+    def center_model(self, vol, mask=None):
+        """calc_center.f90:19-41: (fr, fi) of the (masked) model, fi = -Im."""
+        v = self._a(vol).copy()
+        fr, fi = np.zeros_like(v), np.zeros_like(v)
+        self.L.o_center_model(self._p(v), v.shape[-1], self._r(-1.0 if mask is None else mask), self._p(fr),
+                              self._p(fi))
+        return fr, fi
+
+    def lowpass(self, img, lp):
+        a = self._a(img).copy()
+        self.L.o_img2d_lowpass(self._p(a), a.shape[-1], int(lp))
+        return a
+
+    def apply_ctf(self, img, ctf):
+        a = self._a(img).copy()
+        self.L.o_img2d_applyctf(self._p(a), a.shape[-1], self._p(self._a(ctf)))
+        return a
+
+    def project2d_ft(self, fr, fi, th, ph, om):
+        fr, fi = self._a(fr), self._a(fi)
+        n = fr.shape[-1]
+        out = np.zeros((n, n), self.dt)
+        self.L.o_project2d_ft(self._p(fr), self._p(fi), n, self._r(th), self._r(ph), self._r(om), self._p(out))
+        return out
+
+    def cross_correlation(self, a1, a2):
+        a, b = self._a(a1).copy(), self._a(a2).copy()
+        self.L.o_cross_correlation(self._p(a), self._p(b), a.shape[-1])
+        return a
+
+    def gauss_regression(self, z):
+        """z[j+1, i+1] = z(i, j) of guass_regression.f90; returns (dx, dy, cc)."""
+        zz = self._a(z).reshape(9)
+        dx, dy, cc = self.ct(), self.ct(), self.ct()
+        self.L.o_gauss_regression(self._p(zz), ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(cc))
+        return dx.value, dy.value, cc.value
+
+    def center_peak(self, ccmap, halve=False):
+        c = self._a(ccmap)
+        out = np.zeros(3, self.dt)
+        self.L.o_center_peak(self._p(c), c.shape[-1], int(halve), self._p(out))
+        return out
+
+    def calc_center_one(self, fr, fi, img, x, y, angles, ctf, mask, maxR):
+        """calc_center.f90:44-76: {x, y, cc} of one particle (cc not clipped)."""
+        a = self._a(img).copy()
+        out = np.zeros(3, self.dt)
+        cp = None if ctf is None else self._p(self._a(ctf))
+        self.L.o_calc_center_one(self._p(self._a(fr)), self._p(self._a(fi)), self._p(a), a.shape[-1], self._r(x),
+                                 self._r(y), *[self._r(v) for v in angles], cp,
+                                 self._r(-1.0 if mask is None else mask), int(maxR), self._p(out))
+        return out
+
+    def center_selfrot180_one(self, img, mask, maxR):
+        """calc_center_selfrot180.f90:33-79: {x, y, cc} of one particle."""
+        a = self._a(img).copy()
+        out = np.zeros(3, self.dt)
+        self.L.o_center_selfrot180_one(self._p(a), a.shape[-1], self._r(-1.0 if mask is None else mask), int(maxR),
+                                       self._p(out))
+        return out

@@ -383,6 +383,56 @@ def local_program_vectors() -> dict:
     return g
 
 
+def _parse_center_lines(out: str, first: int, last: int) -> np.ndarray:
+    """The 'print*, i, centx, centy, cc' lines of calc_center.f90:86 and
+    calc_center_selfrot180.f90:74 (list-directed; NaN allowed)."""
+    rows = {}
+    for line in out.splitlines():
+        t = line.split()
+        if len(t) == 4:
+            try:
+                i = int(t[0])
+                rows[i] = [float(v) for v in t[1:]]
+            except ValueError:
+                continue
+    return np.array([rows[i] for i in range(first, last + 1)], np.float64)
+
+
+def center_program_vectors() -> dict:
+    """icosprocess mode 3 (centre finding) with a model (calc_center) and without
+    (calc_center_selfrot180) on a 48-pixel set with shifts up to 2.5 px."""
+    g = {}
+    box, apix, n = 48, 2.0, 12
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=9, shift_max=2.5, snr=2.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    rng = np.random.default_rng(99)
+    guess = (ps.shifts + rng.uniform(-0.5, 0.5, ps.shifts.shape)).astype(f4)
+    model = synth.phantom_map(ph).astype(f4)
+    g.update(cen_model=model, cen_images=ps.images.astype(f4), cen_angles=ps.angles.astype(f4),
+             cen_shifts=ps.shifts.astype(f4), cen_guess=guess, cen_df=ps.df.astype(f4))
+    exe = os.path.join(REF, "icosprocess")
+    with tempfile.TemporaryDirectory() as d:
+        stk = "stack.mrc"
+        formats.write_mrc(os.path.join(d, stk), ps.images)
+        formats.write_mrc(os.path.join(d, "model.mrc"), model)
+        parts = [formats.Particle(stk, i + 1, *ps.angles[i], *guess[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+        common = ["mode=3", "apix=2.0", "maxRes=6.0", "imgmask=20", "ortfile=ort.txt", f"last={n}"]
+        for key, extra in (("cen_model_out", ["model3d=model.mrc", "newortfile=c1.txt"]),
+                           ("cen_self_out", ["newortfile=c2.txt"])):
+            r = subprocess.run([exe, *common, *extra], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               text=True)
+            g[key] = _parse_center_lines(r.stdout, 1, n)
+            g[key + "_rc"] = np.array([r.returncode], i4)
+            ort = os.path.join(d, extra[-1].split("=")[1])
+            g[key + "_ortfile"] = np.array([os.path.exists(ort)], i4)
+            if os.path.exists(ort):
+                rows = formats.read_ortfile900(ort)
+                g[key + "_ort"] = np.array([[p.x, p.y, p.pr] for p in rows], f4)
+    return g
+
+
 def main():
     import argparse
     ap = argparse.ArgumentParser()
@@ -394,7 +444,7 @@ def main():
     jobs = [("units", lambda: unit_vectors(rng)), ("pr", lambda: pr_vectors(rng)),
             ("lref_units", lambda: lref_vectors(rng)), ("programs", program_vectors),
             ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors),
-            ("local", local_program_vectors)]
+            ("local", local_program_vectors), ("center", center_program_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

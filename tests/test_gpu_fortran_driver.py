@@ -118,3 +118,35 @@ def test_mode6_rejects_unknown_symmetry(local_ort):
     r = subprocess.run([EXE, *[a if not a.startswith("sym=") else "sym=c7" for a in LOCAL_ARGS], "result3d=x.mrc"],
                        cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode != 0 and "local symmetry" in r.stdout
+
+
+@pytest.fixture(scope="module")
+def center_dir():
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (amdflang missing)")
+    c = np.load(os.path.join(ROOT, "tests", "golden", "center.npz"))
+    d = tempfile.mkdtemp(prefix="cimc_")
+    n = len(c["cen_images"])
+    formats.write_mrc(os.path.join(d, "stack.mrc"), c["cen_images"])
+    formats.write_mrc(os.path.join(d, "model.mrc"), c["cen_model"])
+    parts = [formats.Particle("stack.mrc", i + 1, *c["cen_angles"][i], *c["cen_guess"][i], 0.0, i + 1, i + 1,
+                              c["cen_df"][i, 0], c["cen_df"][i, 1], 0.0) for i in range(n)]
+    formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+    return d, c
+
+
+@pytest.mark.parametrize("with_model", [True, False])
+def test_mode3_center(center_dir, with_model):
+    """calc_center (model3d=) / calc_center_selfrot180, then output_ort."""
+    d, c = center_dir
+    key = "cen_model_out" if with_model else "cen_self_out"
+    extra = ["model3d=model.mrc"] if with_model else []
+    out = _run(d, "mode=3", "apix=2.0", "maxRes=6.0", "imgmask=20", "ortfile=ort.txt", "newortfile=c.txt",
+               f"last={len(c['cen_images'])}", *extra)
+    rows = formats.read_ortfile900(os.path.join(d, "c.txt"))
+    got = np.array([[p.x, p.y, p.pr] for p in rows])
+    ref = c[key + "_ort"]
+    assert np.abs(got[:, :2] - ref[:, :2]).max() < 2e-3, (got, ref)
+    assert np.abs(got[:, 2] - ref[:, 2]).max() < 2e-2          # PR = acos(cc): d/dcc ~ 2.3 rad near cc 0.9
+    printed = np.array([[float(v) for v in line.split()[1:]] for line in out.splitlines() if len(line.split()) == 4])
+    assert np.abs(printed - c[key]).max() < 2e-4
