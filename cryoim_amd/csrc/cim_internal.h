@@ -128,6 +128,8 @@ struct cim_ctx {
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
     // half-map FSC: input tnf pairs, normalised copies, merged sums, shell sums, weights
     cim::DevBuf fsc_in, fsc_nrm, fsc_acc, fsc_sums, fsc_wt;
+    // real-space icosahedral average: operators and the sum volume
+    cim::DevBuf avg_ops, avg_acc;
     // kernel timing: event pairs recorded around launches, resolved lazily
     bool timing = false;
     std::map<std::string, cim::KTimer> timers;
@@ -182,6 +184,10 @@ int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
 // real map with checkerboard (inside radius rmask only when rmask >= 0).
 int fold_to_map_dev(cim_ctx* ctx, const float4* acc, const float4* lo, int dxy, int box, int maxR, int int_radius,
                     float isg, float rmask, float* map_dev);
+
+// Real-space icosahedral average (icosavg.hip) of the first n^3 values of
+// map_dev (len values readable), variant CIM_ICOSAVG_FULL / _MERGE.
+int icosavg_dev(cim_ctx* ctx, float* map_dev, long long len, int n, float r0, int variant);
 
 // split <-> interleaved conversion (prep.hip)
 int interleave(cim_ctx* ctx, const float* re, const float* im, float2* out, long long count);

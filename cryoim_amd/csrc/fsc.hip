@@ -15,7 +15,7 @@
 
 namespace cim {
 
-int rec_merge_dev(cim_ctx* ctx, float* out_dev);
+int rec_merge_dev(cim_ctx* ctx, float* out_dev, int nicosavg, float imgmask);
 
 // Weight sums + 1, normalised sums, and the per-shell FP64 sums of
 // a1 a2 + b1 b2, |1|^2, |2|^2 over ir = int(r) <= R
@@ -95,8 +95,8 @@ __global__ void k_fsc_merge(const float* __restrict__ nrm, const float* __restri
     out[id] = make_float4(a, b, __fadd_rn(w, 1.f), 0.f);
 }
 
-static int fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, const float* const in[6], float* fsc,
-                   float* map) {
+static int fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, int nicosavg,
+                   const float* const in[6], float* fsc, float* map) {
     const int t = 2 * maxR + 1;
     const long long n = (long long)t * t * (maxR + 1);
     CIM_ALLOC(ctx->fsc_in, sizeof(float) * 6 * n);
@@ -140,6 +140,15 @@ static int fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, 
     int rc = fold_to_map_dev(ctx, ctx->fsc_acc.as<float4>(), nullptr, t, box, maxR, 1, isg, rmask,
                              ctx->out.as<float>());
     if (rc) return rc;
+    // realspaceavg: icosprocess's img3d_icosavg on the map (reconstruct_FSC.f90:185),
+    // or merge's on the first (box-8)^3 values taken as a cube
+    // (merge_reconstruct_FSC.f90:202, after maphead set FFTsize = box-8)
+    if (nicosavg == 1) {
+        rc = variant == CIM_FSC_RECONSTRUCT
+                 ? icosavg_dev(ctx, ctx->out.as<float>(), nvox, box, imgmask, CIM_ICOSAVG_FULL)
+                 : icosavg_dev(ctx, ctx->out.as<float>(), nvox, box - 8, imgmask, CIM_ICOSAVG_MERGE);
+        if (rc) return rc;
+    }
     // merge_reconstruct_FSC writes (box-8) planes of (box-8)^2 values from the
     // start of the flat array (:204-206, after maphead set FFTsize = box-8)
     const long long nout = variant == CIM_FSC_RECONSTRUCT ? nvox : (long long)(box - 8) * (box - 8) * (box - 8);
@@ -154,7 +163,7 @@ using namespace cim;
 
 extern "C" {
 
-int cim_fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, const float* odd_fa,
+int cim_fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, int nicosavg, const float* odd_fa,
                 const float* odd_fb, const float* odd_wg, const float* eve_fa, const float* eve_fb, const float* eve_wg,
                 float* fsc, float* map) {
     if (!ctx) return CIM_E_ARG;
@@ -167,10 +176,10 @@ int cim_fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, con
         return set_error(ctx, CIM_E_ARG, "maxR %d outside [1, box/2-1] for box %d", maxR, box);
     CIM_HIP(hipSetDevice(ctx->device));
     const float* in[6] = {odd_fa, odd_fb, odd_wg, eve_fa, eve_fb, eve_wg};
-    return fsc_map(ctx, box, maxR, imgmask, variant, in, fsc, map);
+    return fsc_map(ctx, box, maxR, imgmask, variant, nicosavg, in, fsc, map);
 }
 
-int cim_merge_map(cim_ctx* ctx, float* map) {
+int cim_merge_map(cim_ctx* ctx, int nicosavg, float imgmask, float* map) {
     if (!ctx) return CIM_E_ARG;
     if (!map) return set_error(ctx, CIM_E_ARG, "NULL map");
     if (!ctx->rec.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
@@ -178,7 +187,7 @@ int cim_merge_map(cim_ctx* ctx, float* map) {
     CIM_HIP(hipSetDevice(ctx->device));
     const long long m3 = (long long)(ctx->rec.box - 8) * (ctx->rec.box - 8) * (ctx->rec.box - 8);
     CIM_ALLOC(ctx->out, sizeof(float) * m3);
-    int rc = rec_merge_dev(ctx, ctx->out.as<float>());
+    int rc = rec_merge_dev(ctx, ctx->out.as<float>(), nicosavg, imgmask);
     if (rc) return rc;
     CIM_HIP(hipMemcpyAsync(map, ctx->out.ptr, sizeof(float) * m3, hipMemcpyDeviceToHost, ctx->stream));
     CIM_HIP(hipStreamSynchronize(ctx->stream));

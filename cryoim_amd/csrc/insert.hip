@@ -775,7 +775,7 @@ int fold_to_map_dev(cim_ctx* ctx, const float4* acc, const float4* lo, int dxy, 
 
 // merge_reconstruct.f90:37-149 on the accumulated sums: integer-radius fold,
 // inverse FFT, checkerboard, (box-8)^3 crop.  out_dev holds (box-8)^3 floats.
-int rec_merge_dev(cim_ctx* ctx, float* out_dev) {
+int rec_merge_dev(cim_ctx* ctx, float* out_dev, int nicosavg, float imgmask) {
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
     const long long nvox = (long long)r.box * r.box * r.box;
@@ -783,6 +783,9 @@ int rec_merge_dev(cim_ctx* ctx, float* out_dev) {
     int rc = fold_to_map_dev(ctx, r.acc.as<float4>(), r.comp ? r.lo.as<float4>() : nullptr, r.dxy, r.box, r.maxR, 1,
                              -1.f, -1.f, ctx->aux.as<float>());
     if (rc) return rc;
+    // merge_reconstruct.f90:142: merge's img3d_icosavg on the full map, before the crop
+    if (nicosavg == 1 && (rc = icosavg_dev(ctx, ctx->aux.as<float>(), nvox, r.box, imgmask, CIM_ICOSAVG_MERGE)))
+        return rc;
     const long long m3 = (long long)(r.box - 8) * (r.box - 8) * (r.box - 8);
     hipLaunchKernelGGL(k_merge_crop, dim3((unsigned)((m3 + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->aux.as<float>(), out_dev, r.box);

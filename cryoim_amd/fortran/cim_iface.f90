@@ -14,6 +14,7 @@ module cim_iface
     ! CIM_CENTER_MODEL / CIM_CENTER_SELFROT180 of cryoim.h (Fortran names are
     ! case-insensitive, so CIM_CENTER_MODEL would clash with cim_center_model)
     integer(c_int), parameter :: CIM_CTR_MODEL = 0, CIM_CTR_SELFROT180 = 1
+    integer(c_int), parameter :: CIM_ICOSAVG_FULL = 0, CIM_ICOSAVG_MERGE = 1
     ! 1-based positions of the record fields in rec(:, i)
     integer, parameter :: P_THETA = 1, P_PHI = 2, P_OMEGA = 3, P_X = 4, P_Y = 5, P_PR = 6
     integer, parameter :: P_DF1 = 7, P_DF2 = 8, P_ASTIG = 9, P_MASKX = 10, P_MASKY = 11, P_USE = 12
@@ -86,19 +87,30 @@ module cim_iface
             real(c_float) :: fa(*), fb(*), wg(*)
         end function
 
-        integer(c_int) function cim_merge_map(ctx, map) bind(C, name='cim_merge_map')
+        integer(c_int) function cim_merge_map(ctx, nicosavg, imgmask, map) bind(C, name='cim_merge_map')
             import :: c_ptr, c_int, c_float
             type(c_ptr), value :: ctx
+            integer(c_int), value :: nicosavg
+            real(c_float), value :: imgmask
             real(c_float) :: map(*)
         end function
 
-        integer(c_int) function cim_fsc_map(ctx, box, maxR, imgmask, variant, ofa, ofb, owg, efa, efb, ewg, &
-                                            fsc, map) bind(C, name='cim_fsc_map')
+        integer(c_int) function cim_fsc_map(ctx, box, maxR, imgmask, variant, nicosavg, ofa, ofb, owg, efa, efb, &
+                                            ewg, fsc, map) bind(C, name='cim_fsc_map')
             import :: c_ptr, c_int, c_float
             type(c_ptr), value :: ctx
-            integer(c_int), value :: box, maxR, variant
+            integer(c_int), value :: box, maxR, variant, nicosavg
             real(c_float), value :: imgmask
             real(c_float) :: ofa(*), ofb(*), owg(*), efa(*), efb(*), ewg(*), fsc(*), map(*)
+        end function
+
+        integer(c_int) function cim_icosavg(ctx, map, len, n, r0, variant) bind(C, name='cim_icosavg')
+            import :: c_ptr, c_int, c_float, c_long_long
+            type(c_ptr), value :: ctx
+            real(c_float) :: map(*)
+            integer(c_long_long), value :: len
+            integer(c_int), value :: n, variant
+            real(c_float), value :: r0
         end function
 
         integer(c_int) function cim_search_templates(ctx, p, T, images, tort) bind(C, name='cim_search_templates')

@@ -314,6 +314,11 @@ contains
             allocate(map(int(fftsize, 8)**3))
             rc = cim_rec_finish(ctx, map)
             call cim_check(ctx, rc, 'cim_rec_finish')
+            if (realspaceavg == 'y' .or. realspaceavg == 'Y') then
+                ! reconstruct.f90:253-255: img3d_icosavg(fa, FFTsize, imgmask)
+                rc = cim_icosavg(ctx, map, int(size(map), c_long_long), fftsize, imgmask, CIM_ICOSAVG_FULL)
+                call cim_check(ctx, rc, 'cim_icosavg')
+            end if
             call write_map(result3d, map, fftsize)
         end if
     end subroutine

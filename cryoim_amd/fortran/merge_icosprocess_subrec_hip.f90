@@ -8,15 +8,17 @@
 !   nFSC = 1: merge_reconstruct_FSC.f90 -- the <name>oddtnf / <name>evetnf
 !     pairs of every listed name summed, FSC.txt written, and the
 !     FSC-weighted map (cim_fsc_map, CIM_FSC_MERGE).
-! Real-space icosahedral averaging (nicosavg = 1 in the header,
-! img3d_icosavg.f90) is not part of this library: such input is refused.
+! nicosavg = 1 in the header (realspaceavg=y) adds merge's real-space
+! icosahedral average (img3d_icosavg.f90): the plain merge takes it, imgmask
+! and apix from the last file read (merge_reconstruct.f90:26 re-reads each
+! header), the FSC merge from the first (main.f90:21).
 program merge_icosprocess_subrec_hip
     use iso_c_binding
     use cim_iface
     implicit none
     character(len=512), allocatable :: par(:)
-    integer :: nargs, i, u, ios, fftsize, maxr, nfsc, nicosavg, f2, m2, m, r, ntnf
-    real :: apix, imgmask
+    integer :: nargs, i, u, ios, fftsize, maxr, nfsc, nicosavg, f2, m2, m, r, ntnf, last_avg
+    real :: apix, imgmask, last_apix, last_mask
     real(c_float), allocatable :: a0(:), b0(:), w0(:), a1(:), b1(:), w1(:), a2(:), b2(:), w2(:), map(:), fsc(:)
     type(c_ptr) :: ctx
     integer(c_int) :: rc
@@ -38,10 +40,6 @@ program merge_icosprocess_subrec_hip
     read(u) fftsize, maxr, apix, imgmask, nfsc, nicosavg
     close(u)
     print *, fftsize, maxr, apix, imgmask, nfsc, nicosavg
-    if (nicosavg == 1) then
-        print *, 'realspaceavg=y tnf input (img3d_icosavg) is not supported'
-        stop 1
-    end if
     if (nfsc /= 0 .and. nfsc /= 1) stop
     ntnf = (maxr + 1) * (2 * maxr + 1)**2
     m = fftsize - 8
@@ -61,7 +59,8 @@ program merge_icosprocess_subrec_hip
             rc = cim_rec_add_tnf(ctx, a0, b0, w0)
             call cim_check(ctx, rc, 'cim_rec_add_tnf')
         end do
-        rc = cim_merge_map(ctx, map)
+        apix = last_apix
+        rc = cim_merge_map(ctx, last_avg, last_mask, map)
         call cim_check(ctx, rc, 'cim_merge_map')
     else
         allocate(a1(ntnf), b1(ntnf), w1(ntnf), a2(ntnf), b2(ntnf), w2(ntnf), fsc(0:maxr))
@@ -73,7 +72,7 @@ program merge_icosprocess_subrec_hip
             a2 = a2 + a0; b2 = b2 + b0; w2 = w2 + w0
         end do
         print *, 'calculating FSC...'
-        rc = cim_fsc_map(ctx, fftsize, maxr, imgmask, CIM_FSC_MERGE, a1, b1, w1, a2, b2, w2, fsc, map)
+        rc = cim_fsc_map(ctx, fftsize, maxr, imgmask, CIM_FSC_MERGE, nicosavg, a1, b1, w1, a2, b2, w2, fsc, map)
         call cim_check(ctx, rc, 'cim_fsc_map')
         open(newunit=u, file='FSC.txt')
         do r = 0, maxr
@@ -97,6 +96,9 @@ contains
         real :: r1, r2
         open(newunit=v, file=name, form='unformatted', access='stream', status='old')
         read(v) h1, h2, r1, r2, h3, h4
+        last_apix = r1
+        last_mask = r2
+        last_avg = h4
         if (h1 /= fftsize .or. h2 /= maxr) then
             print *, trim(name), ': FFTsize/maxR ', h1, h2, ' differ from the first file ', fftsize, maxr
             stop 1

@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cryoim.h")
 PART_STRIDE = 12
 SEARCH_PLAIN, SEARCH_TABLES = 0, 1
 CENTER_MODEL, CENTER_SELFROT180 = 0, 1
+ICOSAVG_FULL, ICOSAVG_MERGE = 0, 1
 P_THETA, P_PHI, P_OMEGA, P_X, P_Y, P_PR, P_DF1, P_DF2, P_ASTIG, P_MASKX, P_MASKY, P_USE = range(12)
 P_BOXX, P_BOXY = P_MASKX, P_MASKY
 
@@ -115,8 +116,10 @@ def load() -> ctypes.CDLL:
         "cim_rec_add_tnf": (ip, [vp, fp, fp, fp]),
         "cim_rec_finish": (ip, [vp, fp]),
         "cim_rec_finish_dev": (ip, [vp, vp]),
-        "cim_merge_map": (ip, [vp, fp]),
-        "cim_fsc_map": (ip, [vp, ip, ip, ctypes.c_float, ip, fp, fp, fp, fp, fp, fp, fp, fp]),
+        "cim_merge_map": (ip, [vp, ip, ctypes.c_float, fp]),
+        "cim_fsc_map": (ip, [vp, ip, ip, ctypes.c_float, ip, ip, fp, fp, fp, fp, fp, fp, fp, fp]),
+        "cim_icosavg": (ip, [vp, fp, ctypes.c_longlong, ip, ctypes.c_float, ip]),
+        "cim_icosavg_dev": (ip, [vp, vp, ctypes.c_longlong, ip, ctypes.c_float, ip]),
         "cim_search_templates": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global_dev": (ip, [vp, P, ip, vp, fp]),
@@ -276,25 +279,42 @@ class Context:
         self._check(self.lib.cim_rec_finish(self.h, _fptr(m)), "cim_rec_finish")
         return m
 
-    def merge_map(self, box: int) -> np.ndarray:
-        """merge_reconstruct.f90 map of the accumulated tnf sums, (box-8)^3."""
+    def merge_map(self, box: int, icosavg: bool = False, imgmask: float = 0.0) -> np.ndarray:
+        """merge_reconstruct.f90 map of the accumulated tnf sums, (box-8)^3;
+        icosavg: the header's nicosavg = 1 (merge's real-space average, radius imgmask)."""
         m = np.empty((box - 8,) * 3, np.float32)
-        self._check(self.lib.cim_merge_map(self.h, _fptr(m)), "cim_merge_map")
+        self._check(self.lib.cim_merge_map(self.h, int(bool(icosavg)), float(imgmask), _fptr(m)), "cim_merge_map")
         return m
 
-    def fsc_map(self, box: int, maxR: int, odd, even, imgmask: float = 0.0, merge: bool = False):
+    def fsc_map(self, box: int, maxR: int, odd, even, imgmask: float = 0.0, merge: bool = False,
+                icosavg: bool = False):
         """Half-map FSC + FSC-weighted map: (fsc[maxR+1], map).  merge=False is
-        reconstruct_FSC.f90 (box^3), merge=True merge_reconstruct_FSC.f90 ((box-8)^3)."""
+        reconstruct_FSC.f90 (box^3), merge=True merge_reconstruct_FSC.f90 ((box-8)^3);
+        icosavg: realspaceavg=y / nicosavg = 1."""
         arrs = [_f32(a).reshape(-1) for a in (*odd, *even)]
         n = (2 * maxR + 1) ** 2 * (maxR + 1)
         if any(a.size != n for a in arrs):
             raise CimError(f"tnf arrays must hold (2maxR+1)^2 (maxR+1) = {n} values")
         f = np.empty(maxR + 1, np.float32)
         m = np.empty((box - 8,) * 3 if merge else (box,) * 3, np.float32)
-        rc = self.lib.cim_fsc_map(self.h, box, maxR, float(imgmask), 1 if merge else 0, *[_fptr(a) for a in arrs],
-                                  _fptr(f), _fptr(m))
+        rc = self.lib.cim_fsc_map(self.h, box, maxR, float(imgmask), 1 if merge else 0, int(bool(icosavg)),
+                                  *[_fptr(a) for a in arrs], _fptr(f), _fptr(m))
         self._check(rc, "cim_fsc_map")
         return f, m
+
+    def icosavg(self, vol, r0: float, variant: int = 0, n: Optional[int] = None):
+        """Real-space icosahedral average (cim_icosavg / cim_icosavg_dev) of the
+        first n^3 values of vol (n = its box): a new array for a host array,
+        in place for a device tensor (the values past n^3 are read, not written)."""
+        if hasattr(vol, "data_ptr"):
+            nn = vol.shape[-1] if n is None else n
+            self._check(self.lib.cim_icosavg_dev(self.h, ctypes.c_void_p(vol.data_ptr()), vol.numel(), int(nn),
+                                                 float(r0), int(variant)), "cim_icosavg_dev")
+            return vol
+        v = _f32(vol).copy()
+        nn = v.shape[-1] if n is None else n
+        self._check(self.lib.cim_icosavg(self.h, _fptr(v), v.size, int(nn), float(r0), int(variant)), "cim_icosavg")
+        return v
 
     # -- search -------------------------------------------------------------
     def search_templates(self, params: CimParams, images, tort):

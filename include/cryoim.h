@@ -25,6 +25,14 @@
  *   local refinement . icosprocess/SRC/ort_refinement.f90:31-124
  *   slice + PR ....... local_refine/SRC/refine_PR.f90:69-245, FFT3d_slice.f90,
  *                      cal_PR_FT.f90, cal_template_FT.f90
+ *   half maps, FSC ... icosprocess/SRC/reconstruct_half.f90, reconstruct_FSC.f90
+ *   tnf merges ....... merge_icosprocess_subrec/SRC/merge_reconstruct.f90,
+ *                      merge_reconstruct_FSC.f90
+ *   local insert ..... icosprocess/icosprocess/reconstruction_local*.f90
+ *   centre finding ... icosprocess/SRC/calc_center.f90,
+ *                      calc_center_selfrot180.f90
+ *   real-space avg ... icosprocess/SRC/img3d_icosavg.f90,
+ *                      merge_icosprocess_subrec/SRC/img3d_icosavg.f90
  */
 #ifndef CRYOIM_H
 #define CRYOIM_H
@@ -177,8 +185,11 @@ int cim_rec_finish_dev(cim_ctx* ctx, void* map_dev);
  * partial tnf with cim_rec_add_tnf, then this folds with the integer shell
  * test of :42 (int(r) <= maxR), inverse-transforms and writes the
  * (box-8)^3 block of :143-149, including its plane index i/(box-8)^2+4 on
- * the 1-based i (the last voxel of each plane comes from the next plane). */
-int cim_merge_map(cim_ctx* ctx, float* map);
+ * the 1-based i (the last voxel of each plane comes from the next plane).
+ * nicosavg = 1 (the tnf header's flag, written by realspaceavg=y) runs
+ * merge's img3d_icosavg (CIM_ICOSAVG_MERGE, radius imgmask in pixels) on the
+ * full map before the crop (:142). */
+int cim_merge_map(cim_ctx* ctx, int nicosavg, float imgmask, float* map);
 /* Symmetry operators of icosprocess's local reconstruction (mode 6,
  * reconstruction_local.f90:30-55; generators icos2f_sym_mat.f90:250-540):
  * sym = "c1", "c2", "c3", "c4", "c5a", "c6", "c8", "c12" or "c15", z
@@ -186,6 +197,20 @@ int cim_merge_map(cim_ctx* ctx, float* map);
  * (may be NULL): nops row-major 3x3 float64, ready for cim_rec_begin.
  * Returns CIM_E_ARG for names mode 6 has no operators for. */
 int cim_icosprocess_sym_ops(const char* sym, double* ops, int* nops);
+
+/* ---- real-space icosahedral average (realspaceavg=y) ---------------------- */
+#define CIM_ICOSAVG_FULL 0    /* icosprocess/SRC/img3d_icosavg.f90:2-126 */
+#define CIM_ICOSAVG_MERGE 1   /* merge_icosprocess_subrec/SRC/img3d_icosavg.f90:1-91 */
+/* Replaces the first n^3 values of map (taken as an n^3 cube, centre n/2)
+ * by the mean over the 60 icosahedral operators of the trilinearly
+ * interpolated map at S x, within radius r0 (pixels) of the centre and 0
+ * beyond.  CIM_ICOSAVG_MERGE averages the quarter lz, ky >= 0 and fills the
+ * rest with the copies of the reference's second loop, in its order and
+ * flat indexing.  Taps read the flat index the reference reads; len values
+ * of map are readable (reads outside them give 0).  Host array. */
+int cim_icosavg(cim_ctx* ctx, float* map, long long len, int n, float r0, int variant);
+/* Same on a device array. */
+int cim_icosavg_dev(cim_ctx* ctx, void* map_dev, long long len, int n, float r0, int variant);
 
 /* ---- half-map FSC ------------------------------------------------------ */
 #define CIM_FSC_RECONSTRUCT 0   /* icosprocess reconstruct_FSC.f90:10-197 */
@@ -199,8 +224,13 @@ int cim_icosprocess_sym_ops(const char* sym, double* ops, int* nops);
  *     the first (box-8)^3 values of the flat box^3 map masked to radius
  *     imgmask (:188-206).
  * fsc may be NULL.  The weighting keeps the reference's stale-index zeroing
- * (reconstruct_FSC.f90:82-84). */
-int cim_fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, const float* odd_fa,
+ * (reconstruct_FSC.f90:82-84).  nicosavg = 1 (realspaceavg=y) averages the
+ * map in real space: CIM_ICOSAVG_FULL on the box^3 map for
+ * CIM_FSC_RECONSTRUCT (:185), CIM_ICOSAVG_MERGE on the first (box-8)^3
+ * values taken as a (box-8)^3 cube for CIM_FSC_MERGE
+ * (merge_reconstruct_FSC.f90:202, after maphead has set FFTsize = box-8);
+ * radius imgmask. */
+int cim_fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, int nicosavg, const float* odd_fa,
                 const float* odd_fb, const float* odd_wg, const float* eve_fa, const float* eve_fb,
                 const float* eve_wg, float* fsc, float* map);
 

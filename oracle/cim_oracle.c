@@ -1419,3 +1419,96 @@ EXPORT void o_center_selfrot180_one(real* img, int nx, real mask, int maxR, real
     o_center_peak(img, nx, 1, out);
     free(a2);
 }
+
+/* ------------------------------------------------------------------ */
+/* Real-space icosahedral averaging                                     */
+/* ------------------------------------------------------------------ */
+
+/* Adds the eight trilinear taps of one operator to acc, term by term in
+ * the order of img3d_icosavg.f90:75-83 (acc + t1 + ... + t8, left to
+ * right): x = S x0 + nx/2, taps by flat index n1 = iz nx^2 + iy nx + ix
+ * (int() truncation), read from the first len values of img (0 outside). */
+static real icosavg_taps(real acc, const real* img, long len, int nx, const real* S, int hx, int ky, int lz)
+{
+    real x0[3] = {(real)hx, (real)ky, (real)lz}, x[3];
+    matvec(S, x0, x);
+    for (int k = 0; k < 3; ++k) x[k] += (real)(nx / 2);
+    int ix = (int)x[0], iy = (int)x[1], iz = (int)x[2];
+    real ax = x[0] - (real)ix, ay = x[1] - (real)iy, az = x[2] - (real)iz;
+    long n2 = (long)nx * nx;
+    long n1 = (long)iz * n2 + (long)iy * nx + ix;
+    long idx[8] = {n1, n1 + 1, n1 + nx, n1 + nx + 1, n1 + n2, n1 + 1 + n2, n1 + nx + n2, n1 + nx + 1 + n2};
+    real v[8];
+    for (int k = 0; k < 8; ++k) v[k] = (idx[k] >= 0 && idx[k] < len) ? img[idx[k]] : (real)0;
+    acc = acc + ((real)1.0 - ax) * ((real)1.0 - ay) * ((real)1.0 - az) * v[0];
+    acc = acc + ax * ((real)1.0 - ay) * ((real)1.0 - az) * v[1];
+    acc = acc + ((real)1.0 - ax) * ay * ((real)1.0 - az) * v[2];
+    acc = acc + ax * ay * ((real)1.0 - az) * v[3];
+    acc = acc + ((real)1.0 - ax) * ((real)1.0 - ay) * az * v[4];
+    acc = acc + ax * ((real)1.0 - ay) * az * v[5];
+    acc = acc + ((real)1.0 - ax) * ay * az * v[6];
+    acc = acc + ax * ay * az * v[7];
+    return acc;
+}
+
+/* icosprocess/SRC/img3d_icosavg.f90:2-126: every voxel within r0 of the
+ * centre is the sum over the nops operators of the interpolated map at
+ * S x, divided by nops; voxels beyond r0 become 0.  The first nx^3 values of
+ * img are replaced; taps may read anywhere in its first len values. */
+EXPORT void o_img3d_icosavg_full(real* img, long len, int nx, real r0, const real* ops, int nops)
+{
+    long n2 = (long)nx * nx, N = n2 * nx;
+    real* acc = (real*)calloc(N, sizeof(real));
+    for (int lz = -nx / 2; lz < nx / 2; ++lz)
+        for (int ky = -nx / 2; ky < nx / 2; ++ky)
+            for (int hx = -nx / 2; hx < nx / 2; ++hx) {
+                real r = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+                if (r > r0) continue;
+                long i3 = (long)(lz + nx / 2) * n2 + (long)(ky + nx / 2) * nx + (hx + nx / 2);
+                for (int s = 0; s < nops; ++s) acc[i3] = icosavg_taps(acc[i3], img, len, nx, ops + 9 * s, hx, ky, lz);
+            }
+    for (long i = 0; i < N; ++i) img[i] = acc[i] / (real)nops;
+    free(acc);
+}
+
+/* merge_icosprocess_subrec/SRC/img3d_icosavg.f90:1-91: the 60-operator sum
+ * over the quarter lz, ky in [0, nx/2-1] within r0, then the copies of its
+ * second loop in loop order -- to (-hx, -ky, lz), and for lz > 0 also to
+ * (-hx, ky, -lz) and (hx, -ky, -lz) -- by flat index, so an x index of nx
+ * lands on x = 0 of the next row; copies outside the nx^3 array are
+ * dropped.  Divided by 60.  Reads as o_img3d_icosavg_full. */
+EXPORT void o_img3d_icosavg_merge(real* img, long len, int nx, real r0, const real* ops)
+{
+    long n2 = (long)nx * nx, N = n2 * nx;
+    real* acc = (real*)calloc(N, sizeof(real));
+    for (int lz = 0; lz < nx / 2; ++lz)
+        for (int ky = 0; ky < nx / 2; ++ky)
+            for (int hx = -nx / 2; hx < nx / 2; ++hx) {
+                real r = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+                if (r > r0) continue;
+                long i3 = (long)(lz + nx / 2) * n2 + (long)(ky + nx / 2) * nx + (hx + nx / 2);
+                for (int s = 0; s < 60; ++s) acc[i3] = icosavg_taps(acc[i3], img, len, nx, ops + 9 * s, hx, ky, lz);
+            }
+    for (int lz = 0; lz < nx / 2; ++lz)
+        for (int ky = 0; ky < nx / 2; ++ky)
+            for (int hx = -nx / 2; hx < nx / 2; ++hx) {
+                real r = (real)sqrt((real)(hx * hx + ky * ky + lz * lz));
+                if (r > r0) continue;
+                long i3 = (long)(lz + nx / 2) * n2 + (long)(ky + nx / 2) * nx + (hx + nx / 2);
+                long t[3];
+                int nt;
+                if (lz == 0) {
+                    t[0] = (long)(lz + nx / 2) * n2 + (long)(-ky + nx / 2) * nx + (-hx + nx / 2);
+                    nt = 1;
+                } else {
+                    t[0] = (long)(lz + nx / 2) * n2 + (long)(-ky + nx / 2) * nx + (-hx + nx / 2);
+                    t[1] = (long)(-lz + nx / 2) * n2 + (long)(ky + nx / 2) * nx + (-hx + nx / 2);
+                    t[2] = (long)(-lz + nx / 2) * n2 + (long)(-ky + nx / 2) * nx + (hx + nx / 2);
+                    nt = 3;
+                }
+                for (int k = 0; k < nt; ++k)
+                    if (t[k] >= 0 && t[k] < N) acc[t[k]] = acc[i3];
+            }
+    for (long i = 0; i < N; ++i) img[i] = acc[i] / (real)60;
+    free(acc);
+}

@@ -182,22 +182,32 @@ class Oracle:
         self.L.o_tnf_fsc(int(maxR), *[self._p(v) for v in ins], self._p(f), *[self._p(o) for o in outs])
         return (f, *outs)
 
-    def fsc_map(self, odd, even, nx, maxR, imgmask=None, merge=False):
+    def fsc_map(self, odd, even, nx, maxR, imgmask=None, merge=False, icosavg=False):
         """Map of reconstruct_FSC.f90 (merge=False, nx^3) or of
         merge_reconstruct_FSC.f90 (merge=True: masked, then the first (nx-8)^3
-        values of the flat array, as its writer at :204-206 emits them)."""
+        values of the flat array, as its writer at :204-206 emits them).
+        icosavg: realspaceavg / nicosavg = 1 -- icosprocess's img3d_icosavg on
+        the map (reconstruct_FSC.f90:185), or merge's on the first (nx-8)^3
+        values taken as an (nx-8)^3 cube (merge_reconstruct_FSC.f90:202, after
+        maphead has set FFTsize = nx-8)."""
         f, fa, fb, wg = self.tnf_fsc(odd, even, maxR)
         vol = self.embed_tnf(fa, fb, wg, nx, maxR)
         if not merge:
-            return f, self.fold_and_map(vol, maxR, int_radius=True, imag_sign=+1)
+            m = self.fold_and_map(vol, maxR, int_radius=True, imag_sign=+1)
+            if icosavg:
+                m = self.icosavg_full(m, imgmask)
+            return f, m
         m = self.fold_and_map(vol, maxR, int_radius=True, imag_sign=-1)
         self.L.o_sphere_zero(self._p(m), nx, self._r(imgmask))
         M = nx - 8
+        if icosavg:
+            m = self.icosavg_merge(m, imgmask, n=M)
         return f, m.reshape(-1)[: M ** 3].reshape(M, M, M).copy()
 
-    def merge_map(self, tnfs, nx, maxR):
+    def merge_map(self, tnfs, nx, maxR, imgmask=None, icosavg=False):
         """merge_reconstruct.f90: sum of tnf triples onto weights 0.01, integer-radius
-        fold, inverse FFT, then the (nx-8)^3 crop of :143-149."""
+        fold, inverse FFT, [merge's img3d_icosavg when nicosavg = 1, :142], then
+        the (nx-8)^3 crop of :143-149."""
         T = 2 * maxR + 1
         fa, fb = np.zeros((maxR + 1, T, T), self.dt), np.zeros((maxR + 1, T, T), self.dt)
         wg = np.full((maxR + 1, T, T), 0.01, self.dt)
@@ -207,6 +217,8 @@ class Oracle:
             wg += self._a(t[2]).reshape(fa.shape)
         vol = self.embed_tnf(fa, fb, wg, nx, maxR, wg_fill=0.01)
         m = self.fold_and_map(vol, maxR, int_radius=True, imag_sign=-1)
+        if icosavg:
+            m = self.icosavg_merge(m, imgmask)
         out = np.zeros((nx - 8) ** 3, self.dt)
         self.L.o_merge_crop(self._p(m), nx, self._p(out))
         return out.reshape(nx - 8, nx - 8, nx - 8)
@@ -378,3 +390,22 @@ class Oracle:
         self.L.o_center_selfrot180_one(self._p(a), a.shape[-1], self._r(-1.0 if mask is None else mask), int(maxR),
                                        self._p(out))
         return out
+
+    # ---- real-space icosahedral averaging --------------------------------
+    def icosavg_full(self, vol, r0, ops=None, n=None):
+        """icosprocess img3d_icosavg on the first n^3 values of vol (n = its box)."""
+        v = self._a(vol).copy()
+        flat = v.reshape(-1)
+        n = v.shape[-1] if n is None else n
+        o = self._a(self.icos_ops() if ops is None else ops).reshape(-1, 9)
+        self.L.o_img3d_icosavg_full(self._p(flat), flat.size, int(n), self._r(r0), self._p(o), len(o))
+        return v
+
+    def icosavg_merge(self, vol, r0, n=None):
+        """merge_icosprocess_subrec img3d_icosavg on the first n^3 values of vol."""
+        v = self._a(vol).copy()
+        flat = v.reshape(-1)
+        n = v.shape[-1] if n is None else n
+        o = self._a(self.icos_ops()).reshape(-1, 9)
+        self.L.o_img3d_icosavg_merge(self._p(flat), flat.size, int(n), self._r(r0), self._p(o))
+        return v

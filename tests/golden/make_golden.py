@@ -348,6 +348,52 @@ def fsc_program_vectors() -> dict:
     return g
 
 
+def icosavg_program_vectors() -> dict:
+    """Real-space icosahedral averaging (realspaceavg=y) on the program_vectors
+    data set: icosprocess's img3d_icosavg after mode 1 and after
+    reconstruct_FSC, and merge_icosprocess_subrec's img3d_icosavg in the plain
+    and the FSC-weighted merge (tnf headers with nicosavg = 1)."""
+    g = {}
+    box, apix, n = 48, 2.0, 30
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    icos, icos_fsc = os.path.join(REF, "icosprocess"), os.path.join(REF, "icosprocess_fsc")
+    merge = os.path.join(REF, "merge_icosprocess_subrec")
+    with tempfile.TemporaryDirectory() as d:
+        stk = "stack.mrc"
+        formats.write_mrc(os.path.join(d, stk), ps.images)
+        parts = [formats.Particle(stk, i + 1, *ps.angles[i], *ps.shifts[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+        common = ["apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt", "realspaceavg=y"]
+
+        def run(exe, args):
+            r = subprocess.run([exe, *args], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            return r.stdout
+
+        def raw_map(name, m):
+            return np.fromfile(os.path.join(d, name), dtype="<f4", offset=1024, count=m ** 3).reshape(m, m, m)
+
+        run(icos, ["mode=1", "result3d=plain.mrc", f"last={n}", *common[:-1]])
+        g["avg_input_map"] = raw_map("plain.mrc", box)
+        run(icos, ["mode=1", "result3d=avg.mrc", f"last={n}", *common])
+        g["avg_map"] = raw_map("avg.mrc", box)
+        run(icos_fsc, ["mode=1", "FSC=y", "result3d=fsc.mrc", "first=1", f"last={n}", *common])
+        g["fscavg_map"] = raw_map("fsc.mrc", box)
+        run(icos, ["mode=1", "subrec=y", "result3d=s1.tnf", "first=1", "last=15", *common])
+        run(icos, ["mode=1", "subrec=y", "result3d=s2.tnf", "first=16", f"last={n}", *common])
+        g["avg_tnf_hdr"] = np.array([formats.read_tnf(os.path.join(d, "s1.tnf"))["realspaceavg"]], i4)
+        run(merge, ["s1.tnf", "s2.tnf", "m0.mrc"])
+        g["mavg_map"] = raw_map("m0.mrc", box - 8)
+        run(icos_fsc, ["mode=1", "FSC=y", "subrec=y", "result3d=p1", "first=1", "last=15", *common])
+        run(icos_fsc, ["mode=1", "FSC=y", "subrec=y", "result3d=p2", "first=16", f"last={n}", *common])
+        run(merge, ["p1", "p2", "mfsc.mrc"])
+        g["mfscavg_map"] = raw_map("mfsc.mrc", box - 8)
+    return g
+
+
 def local_program_vectors() -> dict:
     """icosprocess mode 6 (reconstruct_local, reconstruct_local_substep): loc_box windows
     of the program_vectors images around seeded (x0, y0), local symmetry c5a."""
@@ -444,7 +490,8 @@ def main():
     jobs = [("units", lambda: unit_vectors(rng)), ("pr", lambda: pr_vectors(rng)),
             ("lref_units", lambda: lref_vectors(rng)), ("programs", program_vectors),
             ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors),
-            ("local", local_program_vectors), ("center", center_program_vectors)]
+            ("local", local_program_vectors), ("center", center_program_vectors),
+            ("icosavg", icosavg_program_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

@@ -49,6 +49,18 @@ def test_mode1_map(workdir):
     assert err < 1e-4
 
 
+def test_mode1_realspaceavg_map(workdir):
+    """reconstruct.f90:253-255: img3d_icosavg on the map before it is written."""
+    d, g = workdir
+    a = np.load(os.path.join(ROOT, "tests", "golden", "icosavg.npz"))
+    _run(d, "mode=1", "result3d=avg.mrc", "apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt", "last=30",
+         "realspaceavg=y")
+    m = formats.read_mrc(os.path.join(d, "avg.mrc"))
+    err = rel_l2(m, a["avg_map"])
+    print(f"mode 1 realspaceavg map vs reference: rel-L2 {err:.2e}")
+    assert err < 1e-4
+
+
 def test_mode1_subrec_tnf(workdir):
     d, g = workdir
     _run(d, "mode=1", "subrec=y", "result3d=sub.tnf", "apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt",

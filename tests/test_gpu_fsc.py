@@ -86,6 +86,60 @@ def test_merge_map_matches_reference(ctx, g):
     assert err < 1e-5
 
 
+@pytest.fixture(scope="module")
+def avg():
+    return np.load(pipelines.golden_path("icosavg.npz"))
+
+
+def test_icosavg_full_matches_reference(ctx, avg):
+    """icosprocess mode 1 with realspaceavg=y (reconstruct.f90:253-255)."""
+    out = ctx.icosavg(avg["avg_input_map"], 20.0, lib.ICOSAVG_FULL)
+    err = rel_l2(out, avg["avg_map"])
+    print(f"img3d_icosavg vs reference: rel-L2 {err:.2e}")
+    assert err < 1e-5
+
+
+def test_merges_with_icosavg_match_reference(ctx, g, avg):
+    """nicosavg = 1: merge_reconstruct, merge_reconstruct_FSC (average of the
+    first (box-8)^3 values as a cube) and icosprocess's reconstruct_FSC."""
+    box, maxR, mask = int(g["s1_hdr"][0]), int(g["s1_hdr"][1]), float(g["s1_hdrf"][1])
+    ctx.rec_begin(box, maxR, layout=1, rcut=maxR, wg_init=0.01)
+    for key in ("s1", "s2"):
+        ctx.rec_add_tnf(*_tnf(g, key))
+    m = ctx.merge_map(box, icosavg=True, imgmask=mask)
+    e1 = rel_l2(m, avg["mavg_map"])
+    odd = [a.astype(np.float32) + b for a, b in zip(_tnf(g, "p1_odd"), _tnf(g, "p2_odd"))]
+    eve = [a.astype(np.float32) + b for a, b in zip(_tnf(g, "p1_eve"), _tnf(g, "p2_eve"))]
+    _, m = ctx.fsc_map(box, maxR, odd, eve, imgmask=mask, merge=True, icosavg=True)
+    e2 = rel_l2(m, avg["mfscavg_map"])
+    _, m = ctx.fsc_map(box, maxR, _tnf(g, "half_odd"), _tnf(g, "half_eve"), imgmask=mask, icosavg=True)
+    e3 = rel_l2(m, avg["fscavg_map"])
+    print(f"averaged merge {e1:.2e}, merge FSC {e2:.2e}, reconstruct_FSC {e3:.2e}")
+    assert max(e1, e2, e3) < 1e-5
+
+
+@pytest.mark.parametrize("variant", [lib.ICOSAVG_FULL, lib.ICOSAVG_MERGE])
+def test_icosavg_vs_fp64_oracle_box128(ctx, O64, variant):
+    """Box 128.  The merge variant runs as merge_reconstruct_FSC calls it: a
+    120^3 cube over the first values of the 128^3 map, radius 62 >= 60, so its
+    taps read past the cube and its hx = -60 copies wrap to the next row."""
+    box = 128
+    rng = np.random.default_rng(41)
+    v = rng.normal(size=(box, box, box)).astype(np.float32)
+    if variant == lib.ICOSAVG_FULL:
+        n, r0 = box, 60.0
+        ref = O64.icosavg_full(v, r0)
+    else:
+        n, r0 = box - 8, 62.0
+        ref = O64.icosavg_merge(v, r0, n=n)
+    out = ctx.icosavg(v, r0, variant, n=n)
+    m3 = n ** 3
+    err = np.abs(out.reshape(-1)[:m3] - ref.reshape(-1)[:m3]).max()
+    print(f"variant {variant}: max |d| {err:.2e} on a unit-variance map")
+    assert err < 2e-5
+    assert np.array_equal(out.reshape(-1)[m3:], v.reshape(-1)[m3:])    # values past n^3 untouched
+
+
 @pytest.mark.parametrize("merge", [False, True])
 def test_fsc_map_vs_fp64_oracle_box96(ctx, O64, merge):
     box, maxR = 96, 40
@@ -150,6 +204,30 @@ def _run(exe, d, *args):
     r = subprocess.run([exe, *args], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode == 0, r.stdout
     return r.stdout
+
+
+def _write_tnf(path, g, key, nicosavg):
+    h, hf = g[key + "_hdr"], g[key + "_hdrf"]
+    formats.write_tnf(path, int(h[0]), int(h[1]), float(hf[0]), float(hf[1]), *_tnf(g, key), fsc=int(h[2]),
+                      realspaceavg=nicosavg)
+
+
+def test_fortran_merge_driver_with_icosavg(g, avg):
+    """realspaceavg=y tnf input (nicosavg = 1) through the merge driver."""
+    if not os.path.exists(MERGE_EXE):
+        pytest.skip("Fortran merge driver not built (amdflang missing)")
+    d = tempfile.mkdtemp(prefix="cimavg_")
+    for key, name in (("s1", "s1.tnf"), ("s2", "s2.tnf")):
+        _write_tnf(os.path.join(d, name), g, key, nicosavg=1)
+    _run(MERGE_EXE, d, "s1.tnf", "s2.tnf", "m0.mrc")
+    m = np.fromfile(os.path.join(d, "m0.mrc"), dtype="<f4", offset=1024).reshape((40,) * 3)
+    assert rel_l2(m, avg["mavg_map"]) < 1e-5
+    for p in ("p1", "p2"):
+        for h in ("odd", "eve"):
+            _write_tnf(os.path.join(d, f"{p}{h}tnf"), g, f"{p}_{h}", nicosavg=1)
+    _run(MERGE_EXE, d, "p1", "p2", "mfsc.mrc")
+    m = np.fromfile(os.path.join(d, "mfsc.mrc"), dtype="<f4", offset=1024).reshape((40,) * 3)
+    assert rel_l2(m, avg["mfscavg_map"]) < 1e-5
 
 
 def test_fortran_merge_driver(g):
