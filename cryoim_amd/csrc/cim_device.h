@@ -13,6 +13,8 @@ struct CtfConst {
     float sq;        // sqrt(1 - ampwg^2)
     float apix;
     float df_scale;  // 1e4 (um) or 1 (A)
+    int corr_cycles; // defocus series: C steps each side (0: one defocus)
+    float corr_step; // step of the series in the df unit
 };
 
 inline CtfConst make_ctf_const(const cim_params* p) {
@@ -24,6 +26,8 @@ inline CtfConst make_ctf_const(const cim_params* p) {
     c.sq = (float)std::sqrt(1.0 - (double)p->ampwg * p->ampwg);
     c.apix = p->apix;
     c.df_scale = p->df_scale;
+    c.corr_cycles = p->corr_cycles;
+    c.corr_step = p->corr_step;
     return c;
 }
 
@@ -44,6 +48,30 @@ __device__ __forceinline__ float ctf_value(int hx, int ky, int nx, float df1, fl
     double s, co;
     sincos(phase, &s, &co);
     return (float)(-((double)c.sq * s + (double)c.ampwg * co));
+}
+
+// CTF factors of one inserted sample: fa, fb take ca and wg takes cw.
+// Without a defocus series ca = ctf and cw = ctf^2.  With C = corr_cycles > 0
+// (reconstruct_corrCTF.f90:80-88, 137-145): ca = the sum over k = -C..C of
+// the CTF at df + k*step (defocus summed in float32 as there), and
+// cw = (2C+1) ctf_C^2 -- every k adds the square of ctf2d, the last CTF the
+// series computed (k = +C), not of its own.
+__device__ __forceinline__ void ctf_factors(int hx, int ky, int nx, const float* pr, const CtfConst& c, float& ca,
+                                            float& cw) {
+    if (c.corr_cycles <= 0) {
+        const float v = ctf_value(hx, ky, nx, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], c);
+        ca = v;
+        cw = v * v;
+        return;
+    }
+    float s = 0.f, last = 0.f;
+    for (int k = -c.corr_cycles; k <= c.corr_cycles; ++k) {
+        const float d1 = pr[CIM_P_DF1] + c.corr_step * (float)k, d2 = pr[CIM_P_DF2] + c.corr_step * (float)k;
+        last = ctf_value(hx, ky, nx, d1, d2, pr[CIM_P_ASTIG], c);
+        s += last;
+    }
+    ca = s;
+    cw = (float)(2 * c.corr_cycles + 1) * (last * last);
 }
 
 // exp(-i * 2 pi * ((x0 + s*nx/2) hx + (y0 + s*nx/2) ky) / nx) with s = recentre.

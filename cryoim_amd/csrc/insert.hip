@@ -86,13 +86,14 @@ __global__ void __launch_bounds__(256) k_insert(float4* __restrict__ acc, const 
     const long long N = (long long)g.box * g.box;
     const long long i2 = (long long)(hk.y + g.box / 2) * g.box + (hk.x + g.box / 2);
     const float2 v = ft[(long long)p * N + i2];
-    float c = 1.f;
-    if (ctf_img) c = ctf_img[(long long)p * N + i2];
-    else if (apply_ctf) {
-        const float* pr = parts + (long long)p * CIM_PART_STRIDE;
-        c = ctf_value(hk.x, hk.y, g.box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+    float ca = 1.f, cw = 1.f;
+    if (ctf_img) {
+        ca = ctf_img[(long long)p * N + i2];
+        cw = ca * ca;
+    } else if (apply_ctf) {
+        ctf_factors(hk.x, hk.y, g.box, parts + (long long)p * CIM_PART_STRIDE, cc, ca, cw);
     }
-    const float va = v.x * c, vb = v.y * c, vw = c * c;
+    const float va = v.x * ca, vb = v.y * ca, vw = cw;
     const float fh = (float)hk.x, fk = (float)hk.y;
     const float* Mp = M + (long long)p * g.nops * 9;
     const long long dxy = g.dxy, dxy2 = dxy * dxy;
@@ -280,7 +281,7 @@ __global__ void k_tnf_io(float4* __restrict__ acc, const float4* __restrict__ lo
 
 constexpr int BRICK = 8;
 
-// Per-particle sample values {a*ctf, b*ctf, ctf^2} on the (hx, ky) half
+// Per-particle sample values {a*ca, b*ca, cw} (ctf_factors) on the (hx, ky) half
 // plane grid hx in [0, maxR], ky in [-maxR, maxR].
 // A sample is inserted iff hx <= hxmax[ky + maxR] (the host's radius test).
 // maxbits collects the largest |value| of the batch (float bits; non-negative
@@ -300,13 +301,14 @@ __global__ void __launch_bounds__(256) k_insert_vals(const float2* __restrict__ 
             const long long N = (long long)box * box;
             const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
             const float2 v = ft[(long long)p * N + i2];
-            float c = 1.f;
-            if (ctf_img) c = ctf_img[(long long)p * N + i2];
-            else if (apply_ctf) {
-                const float* pr = parts + (long long)p * CIM_PART_STRIDE;
-                c = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+            float ca = 1.f, cw = 1.f;
+            if (ctf_img) {
+                ca = ctf_img[(long long)p * N + i2];
+                cw = ca * ca;
+            } else if (apply_ctf) {
+                ctf_factors(hx, ky, box, parts + (long long)p * CIM_PART_STRIDE, cc, ca, cw);
             }
-            out = make_float4(v.x * c, v.y * c, c * c, 1.f);
+            out = make_float4(v.x * ca, v.y * ca, cw, 1.f);
         }
         vals[(long long)p * total + q] = out;
     }

@@ -63,7 +63,9 @@ typedef struct cim_params {
     int conj_out;     /* FFT stores b = -Im (1, icosprocess) or +Im (0, local_refine) */
     int search_method;/* global search: CIM_SEARCH_PLAIN (0) or CIM_SEARCH_TABLES (1) */
     int search_topk;  /* CIM_SEARCH_TABLES: AU points re-scored exactly per particle (0 = 32) */
-    int reserved_i[6];
+    int corr_cycles;  /* insert with a defocus series (reconstruct_corrCTF): C defocus steps on
+                         each side of df; 0 = the plain CTF */
+    int reserved_i[5];
     float apix;       /* pixel size (A) */
     float imgmask;    /* mask radius in pixels */
     float vol;        /* accelerating voltage (kV) */
@@ -73,7 +75,8 @@ typedef struct cim_params {
     float searchstep; /* global search grid step (deg) */
     float delta_angle;/* refinement angle step (deg) */
     float delta_trans;/* refinement shift step (px) */
-    float reserved_f[7];
+    float corr_step;  /* defocus step of the series, in the df unit (corrCTF=step,cycles) */
+    float reserved_f[6];
 } cim_params;
 
 /* Global search methods (cim_search_global):

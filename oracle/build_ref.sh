@@ -49,6 +49,18 @@ if ! grep -q "eq.'FSC')" "$OUT/icos_fsc/0main_fsckey.f90"; then echo "FSC key pa
 $FC $FFLAGS -J "$OUT/icos" -c "$OUT/icos_fsc/0main_fsckey.f90" -o "$OUT/icos_fsc/0main_fsckey.o"
 $FC $FFLAGS -o "$OUT/icosprocess_fsc" $ICOS_LIBOBJS "$OUT/icos_fsc/0main_fsckey.o" $LIBS
 
+# icosprocess_corrctf: the same objects with a main whose 'corrCTF' key is
+# parsed.  0main.f90:171 compares the key with 'corrctf' while the key table
+# holds 'corrCTF' (0input_parameters.f90:34), so corrCTF=step,cycles never
+# reaches reconstruct_corrctf / reconstruct_corr_ctf_substep (:190-197).  As
+# above, the patched main is generated under _ref/ only, to record golden
+# vectors of that path.
+mkdir -p "$OUT/icos_corrctf"
+sed "s/\.eq\.'corrctf')/.eq.'corrCTF')/" "$S/0main.f90" > "$OUT/icos_corrctf/0main_corrctfkey.f90"
+if ! grep -q "eq.'corrCTF')" "$OUT/icos_corrctf/0main_corrctfkey.f90"; then echo "corrCTF key patch did not apply"; exit 4; fi
+$FC $FFLAGS -J "$OUT/icos" -c "$OUT/icos_corrctf/0main_corrctfkey.f90" -o "$OUT/icos_corrctf/0main_corrctfkey.o"
+$FC $FFLAGS -o "$OUT/icosprocess_corrctf" $ICOS_LIBOBJS "$OUT/icos_corrctf/0main_corrctfkey.o" $LIBS
+
 # merge_icosprocess_subrec (tnf merge, plain and FSC-weighted)
 M=$REF/merge_icosprocess_subrec/SRC
 mkdir -p "$OUT/merge"

@@ -697,6 +697,34 @@ EXPORT real o_refine_particle(const real* pa, const real* pb, const real* Fcut0,
  * reconstruct_substep.f90:101-145 (layout compact: (2R+1)^2 (R+1), offset
  * R = maxR).  a,b: centred particle FT after centre shift; ctf may be NULL
  * (ctf = 1).  R: particle rotation (euler_mat). */
+/* The eight corners and trilinear weights of one sample x00 = R (hx, ky, 0)
+ * under operator S (reconstruct.f90:116-138): z < 0 flips the point and
+ * returns signz = -1; offsets off on x, y (and z for the full box). */
+static int place_corners(const real* S, const real* x00, int compact, real off, long D, long D2, long id[8],
+                         real w[8])
+{
+    real x0[3];
+    matvec(S, x00, x0);
+    int signz = 1;
+    if (x0[2] < 0) { x0[0] = -x0[0]; x0[1] = -x0[1]; x0[2] = -x0[2]; signz = -1; }
+    x0[0] += off; x0[1] += off;
+    if (!compact) x0[2] += off;
+    int lx[3] = {(int)x0[0], (int)x0[1], (int)x0[2]};
+    real ax[3] = {x0[0] - lx[0], x0[1] - lx[1], x0[2] - lx[2]};
+    w[0] = ((real)1 - ax[0]) * ((real)1 - ax[1]) * ((real)1 - ax[2]);
+    w[1] = ax[0] * ((real)1 - ax[1]) * ((real)1 - ax[2]);
+    w[2] = ((real)1 - ax[0]) * ax[1] * ((real)1 - ax[2]);
+    w[3] = ax[0] * ax[1] * ((real)1 - ax[2]);
+    w[4] = ((real)1 - ax[0]) * ((real)1 - ax[1]) * ax[2];
+    w[5] = ax[0] * ((real)1 - ax[1]) * ax[2];
+    w[6] = ((real)1 - ax[0]) * ax[1] * ax[2];
+    w[7] = ax[0] * ax[1] * ax[2];
+    long c1 = lx[2] * D2 + lx[1] * D + lx[0];
+    long t[8] = {c1, c1 + 1, c1 + D, c1 + D + 1, c1 + D2, c1 + D2 + 1, c1 + D2 + D, c1 + D2 + D + 1};
+    for (int k = 0; k < 8; ++k) id[k] = t[k];
+    return signz;
+}
+
 EXPORT void o_insert(real* fa, real* fb, real* wg, int compact, int nx, int maxR, real rcut,
                      const real* a, const real* b, const real* ctf, const real* R,
                      int nops, const real* ops)
@@ -717,30 +745,53 @@ EXPORT void o_insert(real* fa, real* fb, real* wg, int compact, int nx, int maxR
             matvec(R, x, x00);
             real c = ctf ? ctf[i2] : (real)1.0;
             for (int s = 0; s < nops; ++s) {
-                real x0[3];
-                matvec(ops + 9 * s, x00, x0);
-                int signz = 1;
-                if (x0[2] < 0) { x0[0] = -x0[0]; x0[1] = -x0[1]; x0[2] = -x0[2]; signz = -1; }
-                x0[0] += off; x0[1] += off;
-                if (!compact) x0[2] += off;
-                int lx[3] = {(int)x0[0], (int)x0[1], (int)x0[2]};
-                real ax[3] = {x0[0] - lx[0], x0[1] - lx[1], x0[2] - lx[2]};
+                long id[8];
                 real w[8];
-                w[0] = ((real)1 - ax[0]) * ((real)1 - ax[1]) * ((real)1 - ax[2]);
-                w[1] = ax[0] * ((real)1 - ax[1]) * ((real)1 - ax[2]);
-                w[2] = ((real)1 - ax[0]) * ax[1] * ((real)1 - ax[2]);
-                w[3] = ax[0] * ax[1] * ((real)1 - ax[2]);
-                w[4] = ((real)1 - ax[0]) * ((real)1 - ax[1]) * ax[2];
-                w[5] = ax[0] * ((real)1 - ax[1]) * ax[2];
-                w[6] = ((real)1 - ax[0]) * ax[1] * ax[2];
-                w[7] = ax[0] * ax[1] * ax[2];
-                long c1 = lx[2] * D2 + lx[1] * D + lx[0];
-                long id[8] = {c1, c1 + 1, c1 + D, c1 + D + 1, c1 + D2, c1 + D2 + 1, c1 + D2 + D, c1 + D2 + D + 1};
+                int signz = place_corners(ops + 9 * s, x00, compact, off, D, D2, id, w);
                 for (int j = 0; j < 8; ++j) {
                     if (id[j] < 0 || id[j] >= size) continue;
                     fa[id[j]] += a[i2] * w[j] * c;
                     fb[id[j]] += b[i2] * w[j] * c * (real)signz;
                     wg[id[j]] += w[j] * c * c;
+                }
+            }
+        }
+}
+
+/* reconstruct_corrCTF.f90:106-150 / reconstruct_corrCTF_substep.f90:103-147:
+ * the insert with a defocus series.  ctf3d holds ncyc CTF images (defocus
+ * df + k step, k = -C..C); fa, fb take a w ctf3d_k for every k, in that order,
+ * and wg takes w ctf2d^2 ncyc times, ctf2d being the last CTF the series
+ * computed (k = +C), not ctf3d_k. */
+EXPORT void o_insert_series(real* fa, real* fb, real* wg, int compact, int nx, int maxR, real rcut,
+                            const real* a, const real* b, const real* ctf3d, int ncyc, const real* ctf2d,
+                            const real* R, int nops, const real* ops)
+{
+    long D = compact ? 2 * maxR + 1 : nx;
+    long D2 = D * D;
+    long size = compact ? D2 * (maxR + 1) : D2 * nx;
+    long N2 = (long)nx * nx;
+    real off = compact ? (real)maxR : (real)(nx / 2);
+    for (int ky = -maxR; ky <= maxR; ++ky)
+        for (int hx = 0; hx <= maxR; ++hx) {
+            real dr = (real)sqrt((real)(hx * hx + ky * ky));
+            if (!(dr <= rcut)) continue;
+            long i2 = (long)(ky + nx / 2) * nx + (hx + nx / 2);
+            real x[3] = {(real)hx, (real)ky, 0}, x00[3];
+            matvec(R, x, x00);
+            for (int s = 0; s < nops; ++s) {
+                long id[8];
+                real w[8];
+                int signz = place_corners(ops + 9 * s, x00, compact, off, D, D2, id, w);
+                for (int j = 0; j < 8; ++j) {
+                    if (id[j] < 0 || id[j] >= size) continue;
+                    real t1 = a[i2] * w[j], t2 = b[i2] * w[j];
+                    for (int k = 0; k < ncyc; ++k) {
+                        real c = ctf3d[k * N2 + i2];
+                        fa[id[j]] = fa[id[j]] + t1 * c;
+                        fb[id[j]] = fb[id[j]] + t2 * c * (real)signz;
+                        wg[id[j]] = wg[id[j]] + w[j] * (ctf2d[i2] * ctf2d[i2]);
+                    }
                 }
             }
         }

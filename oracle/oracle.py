@@ -136,6 +136,18 @@ class Oracle:
                         self._r(maxR if rcut is None else rcut), self._p(self._a(a)), self._p(self._a(b)),
                         None if c is None else self._p(c), self._p(R), ops.shape[0], self._p(ops))
 
+    def insert_series(self, vol, a, b, ctf3d, ctf_last, angles, maxR, rcut=None, compact=False, ops=None):
+        """reconstruct_corrCTF's insert: ctf3d (ncyc, nx, nx) for fa, fb; ctf_last^2 per cycle for wg."""
+        fa, fb, wg = vol
+        nx = a.shape[-1]
+        ops = self._a(self.icos_ops() if ops is None else ops).reshape(-1, 9)
+        R = self._a(self.euler_mat(*angles)).reshape(9)
+        c3 = self._a(ctf3d)
+        self.L.o_insert_series(self._p(fa), self._p(fb), self._p(wg), int(compact), nx, int(maxR),
+                               self._r(maxR if rcut is None else rcut), self._p(self._a(a)), self._p(self._a(b)),
+                               self._p(c3), c3.shape[0], self._p(self._a(ctf_last)), self._p(R), ops.shape[0],
+                               self._p(ops))
+
     def new_volume(self, nx, maxR, compact=False, wg_init=0.01):
         if compact:
             shape = (maxR + 1, 2 * maxR + 1, 2 * maxR + 1)

@@ -394,6 +394,37 @@ def icosavg_program_vectors() -> dict:
     return g
 
 
+def corrctf_program_vectors() -> dict:
+    """reconstruct_corrCTF and reconstruct_corr_CTF_substep (mode 1 with
+    corrCTF=step,cycles: a defocus series of 2 cycles x 0.2 um each side) on
+    the program_vectors data set, by the key-patched icosprocess_corrctf."""
+    g = {}
+    box, apix, n = 48, 2.0, 30
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    exe = os.path.join(REF, "icosprocess_corrctf")
+    g["corr_step_cycles"] = np.array([0.2, 2.0], f4)
+    with tempfile.TemporaryDirectory() as d:
+        stk = "stack.mrc"
+        formats.write_mrc(os.path.join(d, stk), ps.images)
+        parts = [formats.Particle(stk, i + 1, *ps.angles[i], *ps.shifts[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+        common = ["apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt", f"last={n}", "corrCTF=0.2,2"]
+        for args in (["mode=1", "result3d=c.mrc"], ["mode=1", "subrec=y", "result3d=c.tnf"]):
+            r = subprocess.run([exe, *args, *common], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            if "subrec=y" not in args and "reconstruction:" not in r.stdout:   # reconstruct_corrCTF.f90:15
+                raise RuntimeError("corrCTF path not taken:\n" + r.stdout[-2000:])
+        g["corr_map"] = np.fromfile(os.path.join(d, "c.mrc"), dtype="<f4", offset=1024).reshape(box, box, box)
+        t = formats.read_tnf(os.path.join(d, "c.tnf"))
+        g["corrsub_fa"], g["corrsub_fb"], g["corrsub_wg"] = t["fa"], t["fb"], t["wg"]
+        g["corrsub_hdr"] = np.array([t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]], i4)
+    return g
+
+
 def local_program_vectors() -> dict:
     """icosprocess mode 6 (reconstruct_local, reconstruct_local_substep): loc_box windows
     of the program_vectors images around seeded (x0, y0), local symmetry c5a."""
@@ -491,7 +522,7 @@ def main():
             ("lref_units", lambda: lref_vectors(rng)), ("programs", program_vectors),
             ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors),
             ("local", local_program_vectors), ("center", center_program_vectors),
-            ("icosavg", icosavg_program_vectors)]
+            ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

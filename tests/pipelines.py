@@ -45,6 +45,30 @@ def mode1_volume(O, images, angles, shifts, df, apix, maxR, mask_r, compact=Fals
     return vol
 
 
+def corrctf_ctfs(O, box, df1, df2, apix, step, cycles):
+    """The defocus series of reconstruct_corrCTF.f90:80-88: CTF images at
+    df + k step, k = -cycles..cycles (defocus summed in float32), and the last
+    one, which getCTF2d leaves in ctf2d."""
+    f = np.float32
+    c3 = np.array([O.ctf2d(box, f(df1) + f(step) * f(k), f(df2) + f(step) * f(k), 0.0, apix)
+                   for k in range(-cycles, cycles + 1)])
+    return c3, c3[-1]
+
+
+def corrctf_volume(O, images, angles, shifts, df, apix, maxR, mask_r, step, cycles, compact=False):
+    """reconstruct_corrCTF.f90:53-150 (full box, rcut maxR, weights from 0.01) or
+    reconstruct_corr_CTF_substep.f90 (compact, rcut maxR, weights from 0); mask
+    at the first particle (particle%x of :95)."""
+    box = images.shape[-1]
+    vol = O.new_volume(box, maxR, compact=compact, wg_init=0.0 if compact else 0.01)
+    for i in range(len(images)):
+        x, y = shifts[i]
+        a, b = O.particle_ft(images[i], x, y, mask_r=mask_r, mask_xy=tuple(shifts[0]), norm=True)
+        c3, last = corrctf_ctfs(O, box, df[i, 0], df[i, 1], apix, step, cycles)
+        O.insert_series(vol, a, b, c3, last, angles[i], maxR, rcut=maxR, compact=compact)
+    return vol
+
+
 def half_volume(O, images, angles, shifts, df, apix, maxR, mask_r, parity, ctf=True):
     """reconstruct_half.f90:47-141: compact sums over particles parity,
     parity+2, ... with rcut = maxR and weights from 0; the mask is centred on

@@ -207,3 +207,54 @@ def test_compensated_map_matches_fp64_oracle(ctx, O64, insert_kernel):
     err = rel_l2(gmap, O64.fold_and_map(vol, maxR))
     print(f"compensated map vs FP64 oracle: rel-L2 {err:.2e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("compact", [False, True])
+def test_corrctf_matches_reference(ctx, O64, prog, insert_kernel, compact):
+    """reconstruct_corrCTF / reconstruct_corr_CTF_substep (corrCTF=0.2,2): the
+    insert with a defocus series, against the key-patched reference's map and
+    tnf (tests/golden/corrctf.npz) and the FP64 oracle.  The reference's
+    float32 sums (5 defoci x 60 operators x 8 corners per sample, in order)
+    drift 3.4e-5 from the FP64 map on this case, as the FP32 oracle shows by
+    replaying that order; the brick kernel's per-batch sums are exact, so its
+    map sits with the FP64 one (the scatter kernel's float atomics add in
+    arbitrary order, the reference's error class)."""
+    import pipelines
+    cc = np.load(pipelines.golden_path("corrctf.npz"))
+    box, apix = int(prog["prog_box"][0]), float(prog["prog_apix"][0])
+    d = pipelines.derived(box, apix, maxRes=6.0, imgmask_A=40)
+    maxR = d["maxR"]
+    step, cyc = float(cc["corr_step_cycles"][0]), int(cc["corr_step_cycles"][1])
+    p = lib.make_params(box, 0, maxR, apply_ctf=True, mask_on=True, norm_on=True, apix=apix, imgmask=d["mask"],
+                        corr_step=step, corr_cycles=cyc)
+    n = len(prog["prog_images"])
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 0:3] = prog["prog_angles"]
+    P[:, 3:5] = prog["prog_shifts"]
+    P[:, 6:8] = prog["prog_df"]
+    P[:, 9:11] = prog["prog_shifts"][0]      # reconstruct_corrCTF.f90:95 masks at the first particle
+    P[:, 11] = 1.0
+    vol = pipelines.corrctf_volume(O64, prog["prog_images"], prog["prog_angles"], prog["prog_shifts"],
+                                   prog["prog_df"], apix, maxR, d["mask"], step, cyc, compact=compact)
+    tol64 = 5e-6 if insert_kernel == "brick" else 3e-5
+    if compact:
+        ctx.rec_begin(box, maxR, layout=1, rcut=maxR, wg_init=0.0)
+        ctx.rec_insert(p, prog["prog_images"], P)
+        for o, k, v in zip(ctx.rec_get_tnf(maxR), ("fa", "fb", "wg"), vol):
+            e_ref, e_64 = rel_l2(o, cc[f"corrsub_{k}"]), rel_l2(o, v)
+            print(f"corrCTF substep {k}: rel-L2 {e_ref:.2e} vs reference, {e_64:.2e} vs FP64 oracle")
+            assert e_ref < 1e-4 and e_64 < tol64
+    else:
+        ctx.rec_begin(box, maxR, layout=0, rcut=maxR, wg_init=0.01)
+        ctx.rec_insert(p, prog["prog_images"], P)
+        m = ctx.rec_finish(box)
+        e_ref, e_64 = rel_l2(m, cc["corr_map"]), rel_l2(m, O64.fold_and_map(vol, maxR))
+        print(f"corrCTF map: rel-L2 {e_ref:.2e} vs reference, {e_64:.2e} vs FP64 oracle")
+        assert e_ref < 1e-4 and e_64 < tol64
+
+
+def test_corrctf_needs_the_ctf(ctx):
+    p = lib.make_params(64, 0, 20, apply_ctf=False, apix=1.5, corr_step=0.1, corr_cycles=1)
+    ctx.rec_begin(64, 20, layout=1, rcut=18, wg_init=0.0)
+    with pytest.raises(lib.CimError, match="apply_ctf"):
+        ctx.rec_insert(p, np.zeros((1, 64, 64), np.float32), np.zeros((1, lib.PART_STRIDE), np.float32))
