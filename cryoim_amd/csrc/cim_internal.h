@@ -187,7 +187,9 @@ int fold_to_map_dev(cim_ctx* ctx, const float4* acc, const float4* lo, int dxy, 
 
 // Real-space icosahedral average (icosavg.hip) of the first n^3 values of
 // map_dev (len values readable), variant CIM_ICOSAVG_FULL / _MERGE.
-int icosavg_dev(cim_ctx* ctx, float* map_dev, long long len, int n, float r0, int variant);
+// ops: nops row-major float64 operators, or NULL for the 60 icosahedral ones.
+int icosavg_dev(cim_ctx* ctx, float* map_dev, long long len, int n, float r0, int variant, int nops = 0,
+                const double* ops = nullptr);
 
 // split <-> interleaved conversion (prep.hip)
 int interleave(cim_ctx* ctx, const float* re, const float* im, float2* out, long long count);

@@ -1,8 +1,9 @@
 // Real-space icosahedral averaging of a map (realspaceavg=y).
 //
 // CIM_ICOSAVG_FULL: icosprocess/SRC/img3d_icosavg.f90:2-126 -- every voxel
-//   within r0 of the centre becomes the mean over the 60 operators of the
-//   trilinearly interpolated map at S x; voxels beyond r0 become 0.
+//   within r0 of the centre becomes the mean over the operators (the 60 of
+//   the icosahedral group, or the cyclic group of sym) of the trilinearly
+//   interpolated map at S x; voxels beyond r0 become 0.
 // CIM_ICOSAVG_MERGE: merge_icosprocess_subrec/SRC/img3d_icosavg.f90:1-91 --
 //   the same mean over the quarter lz, ky >= 0 only, then the copies of its
 //   second loop (to (-hx, -ky, lz), and for lz > 0 to (-hx, ky, -lz) and
@@ -22,8 +23,9 @@ namespace cim {
 namespace {
 
 // Sum over the operators for voxels of the averaged region (all n^3, or the
-// quarter lz in [0, n/2), ky in [0, n/2), hx in [-n/2, n/2)).  The 60
-// operators (row-major float, the reference's order) are staged in LDS.
+// quarter lz in [0, n/2), ky in [0, n/2), hx in [-n/2, n/2)).  The
+// operators (row-major float, the reference's order, at most 60) are
+// staged in LDS.
 __global__ void __launch_bounds__(256) k_icosavg_sum(const float* __restrict__ in, long long len,
                                                      float* __restrict__ acc, int n, float r0, int quarter,
                                                      const float* __restrict__ ops, int nops) {
@@ -103,14 +105,24 @@ __global__ void k_icosavg_div(const float* __restrict__ acc, float* __restrict__
 
 }  // namespace
 
-int icosavg_dev(cim_ctx* ctx, float* map_dev, long long len, int n, float r0, int variant) {
+int icosavg_dev(cim_ctx* ctx, float* map_dev, long long len, int n, float r0, int variant, int nops,
+                const double* ops) {
     if (variant != CIM_ICOSAVG_FULL && variant != CIM_ICOSAVG_MERGE)
         return set_error(ctx, CIM_E_ARG, "variant %d is neither CIM_ICOSAVG_FULL nor CIM_ICOSAVG_MERGE", variant);
     if (n < 4 || n > 1024 || (n & 1)) return set_error(ctx, CIM_E_ARG, "cube side %d unsupported", n);
     const long long N = (long long)n * n * n;
     if (len < N) return set_error(ctx, CIM_E_ARG, "map of %lld values holds no %d^3 cube", len, n);
-    std::vector<double> d(60 * 9);
-    cim_icos60_ref_order(d.data());
+    if (ops && variant == CIM_ICOSAVG_MERGE)
+        return set_error(ctx, CIM_E_ARG, "merge's average is icosahedral only (ops must be NULL)");
+    if (ops && (nops < 1 || nops > 60)) return set_error(ctx, CIM_E_ARG, "nops %d outside [1, 60]", nops);
+    std::vector<double> d;
+    if (ops) {
+        d.assign(ops, ops + 9 * (size_t)nops);
+    } else {
+        nops = 60;
+        d.resize(60 * 9);
+        cim_icos60_ref_order(d.data());
+    }
     std::vector<float> f(d.begin(), d.end());
     CIM_ALLOC(ctx->avg_ops, sizeof(float) * f.size());
     CIM_HIP(hipMemcpyAsync(ctx->avg_ops.ptr, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -122,7 +134,7 @@ int icosavg_dev(cim_ctx* ctx, float* map_dev, long long len, int n, float r0, in
     {
         TimedLaunch tl(ctx, "icosavg");
         hipLaunchKernelGGL(k_icosavg_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, map_dev,
-                           len, acc, n, r0, quarter, ctx->avg_ops.as<float>(), 60);
+                           len, acc, n, r0, quarter, ctx->avg_ops.as<float>(), nops);
         CIM_HIP(hipGetLastError());
     }
     if (quarter) {
@@ -130,7 +142,7 @@ int icosavg_dev(cim_ctx* ctx, float* map_dev, long long len, int n, float r0, in
         CIM_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_icosavg_div, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, acc, map_dev, N,
-                       60.0f);
+                       (float)nops);
     CIM_HIP(hipGetLastError());
     return CIM_OK;
 }
@@ -141,14 +153,15 @@ using namespace cim;
 
 extern "C" {
 
-int cim_icosavg(cim_ctx* ctx, float* map, long long len, int n, float r0, int variant) {
+int cim_icosavg(cim_ctx* ctx, float* map, long long len, int n, float r0, int variant, int nops,
+                const double* ops) {
     if (!ctx) return CIM_E_ARG;
     if (!map) return set_error(ctx, CIM_E_ARG, "NULL map");
     if (len <= 0 || len > (1LL << 31)) return set_error(ctx, CIM_E_ARG, "map length %lld unsupported", len);
     CIM_HIP(hipSetDevice(ctx->device));
     CIM_ALLOC(ctx->out, sizeof(float) * len);
     CIM_HIP(hipMemcpyAsync(ctx->out.ptr, map, sizeof(float) * len, hipMemcpyHostToDevice, ctx->stream));
-    int rc = icosavg_dev(ctx, ctx->out.as<float>(), len, n, r0, variant);
+    int rc = icosavg_dev(ctx, ctx->out.as<float>(), len, n, r0, variant, nops, ops);
     if (rc) return rc;
     const long long N = (long long)n * n * n;
     CIM_HIP(hipMemcpyAsync(map, ctx->out.ptr, sizeof(float) * N, hipMemcpyDeviceToHost, ctx->stream));
@@ -156,11 +169,12 @@ int cim_icosavg(cim_ctx* ctx, float* map, long long len, int n, float r0, int va
     return CIM_OK;
 }
 
-int cim_icosavg_dev(cim_ctx* ctx, void* map_dev, long long len, int n, float r0, int variant) {
+int cim_icosavg_dev(cim_ctx* ctx, void* map_dev, long long len, int n, float r0, int variant, int nops,
+                    const double* ops) {
     if (!ctx) return CIM_E_ARG;
     if (!map_dev) return set_error(ctx, CIM_E_ARG, "NULL map");
     CIM_HIP(hipSetDevice(ctx->device));
-    return icosavg_dev(ctx, static_cast<float*>(map_dev), len, n, r0, variant);
+    return icosavg_dev(ctx, static_cast<float*>(map_dev), len, n, r0, variant, nops, ops);
 }
 
 }  // extern "C"

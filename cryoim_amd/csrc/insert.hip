@@ -535,13 +535,16 @@ static int icosprocess_local_ops(const char* sym, std::vector<double>& r) {
     else if (s == "c4") { N = 4; step = 90.f; }
     else if (s == "c5a") { N = 5; step = 72.f; }
     else if (s == "c6") { N = 6; step = 60.f; }
+    else if (s == "c7") { N = 7; step = 0.f; }
     else if (s == "c8") { N = 8; step = 45.f; }
     else if (s == "c12") { N = 12; step = 30.f; }
     else if (s == "c15") { N = 15; step = 24.f; }
     else return -1;
     r.assign(9 * (size_t)N, 0.0);
     for (int k = 0; k < N; ++k) {
-        const float a = (float)k * step;
+        // literal angles k*step (icos2f_sym_matc2..c15); c7 computes
+        // float(i-1)*360.0/7.0 (icos2f_sym_mat.f90:520-523)
+        const float a = step > 0.f ? (float)k * step : (float)k * 360.0f / (float)N;
         const float rad = a * 3.1415926f / 180.0f;
         const float c = cosf(rad), sn = sinf(rad);
         double* m = &r[9 * (size_t)k];

@@ -104,12 +104,12 @@ module cim_iface
             real(c_float) :: ofa(*), ofb(*), owg(*), efa(*), efb(*), ewg(*), fsc(*), map(*)
         end function
 
-        integer(c_int) function cim_icosavg(ctx, map, len, n, r0, variant) bind(C, name='cim_icosavg')
+        integer(c_int) function cim_icosavg(ctx, map, len, n, r0, variant, nops, ops) bind(C, name='cim_icosavg')
             import :: c_ptr, c_int, c_float, c_long_long
-            type(c_ptr), value :: ctx
+            type(c_ptr), value :: ctx, ops
             real(c_float) :: map(*)
             integer(c_long_long), value :: len
-            integer(c_int), value :: n, variant
+            integer(c_int), value :: n, variant, nops
             real(c_float), value :: r0
         end function
 

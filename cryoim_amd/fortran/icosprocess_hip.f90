@@ -26,7 +26,7 @@ program icosprocess_hip
     integer :: mode = 1, first = 1, last = 0, ncycle = 6, nargs, i, j, k, eq, fftsize, minR, maxR, T
     integer :: loc_box = -1
     real :: apix = 1.0, minres = 300.0, maxres = 8.0, imgmask = -1.0, vol = 300.0, cs = 2.7, ampwg = 0.1
-    real :: pr_threshold = 90.0, searchstep = 2.0, boundx = -1.0, deltaangle, deltatrans
+    real :: pr_threshold = 90.0, searchstep = 2.0, boundx = -1.0, deltaangle, deltatrans, maxcentshift = 100.0
     logical :: have_mask = .false., have_boundx = .false.
 
     type particle_t
@@ -84,12 +84,13 @@ program icosprocess_hip
         case ('subrec'); subrec = val(1:1)
         case ('sym'); sym = val(1:4)
         case ('boundX'); read(val, *) boundx; have_boundx = .true.
+        case ('maxcentshift'); read(val, *) maxcentshift
         case ('loc_box'); read(val, *) loc_box
         case ('model3d'); model3d = val
         end select
     end do
-    if (mode /= 6 .and. trim(sym) /= 'icos') then
-        print *, 'icosprocess_hip runs modes 1, 3, 4, 5 with the icosahedral (sym=icos) symmetry only'
+    if (mode /= 1 .and. mode /= 6 .and. trim(sym) /= 'icos') then
+        print *, 'icosprocess_hip runs modes 3, 4, 5 with the icosahedral (sym=icos) symmetry only'
         stop 2
     end if
 
@@ -225,15 +226,17 @@ contains
     ! reconstruct.f90:85-89 and ort_globesearch.f90:61-65, so the defocus of
     ! the first particle of each micrograph run is carried forward there.
     ! Particle records for first..i1 written from i0.  bx >= 0 applies the
-    ! reconstruction filter PR <= PR_threshold, |x|, |y| <= bx (unused records
-    ! get P_USE = 0); the per-micrograph CTF cache follows only particles that
-    ! pass it, as the reconstruct loops update imgID0 inside that test
-    ! (reconstruct.f90:77-90, reconstruction_local.f90:96-108).
-    subroutine fill_records(i0, i1, rec, cache_ctf, mask_first, bx)
+    ! reconstruction filter PR <= PR_threshold, |x|, |y| <= bx; mcs >= 0 the
+    ! filter PR <= PR_threshold, sqrt(x^2 + y^2) <= mcs of
+    ! reconstruct_C1_substep.f90:70-71 (unused records get P_USE = 0).  The
+    ! per-micrograph CTF cache follows only particles that pass, as the
+    ! reconstruct loops update imgID0 inside that test (reconstruct.f90:77-90,
+    ! reconstruction_local.f90:96-108).
+    subroutine fill_records(i0, i1, rec, cache_ctf, mask_first, bx, mcs)
         integer, intent(in) :: i0, i1
         real(c_float), intent(out) :: rec(CIM_PART_STRIDE, *)
         logical, intent(in) :: cache_ctf, mask_first
-        real, intent(in) :: bx
+        real, intent(in) :: bx, mcs
         integer :: i, k, imgid0
         real :: d1, d2, ast
         logical :: use
@@ -242,6 +245,7 @@ contains
         do i = first, i1
             use = .true.
             if (bx >= 0.0) use = part(i)%pr <= pr_threshold .and. abs(part(i)%x) <= bx .and. abs(part(i)%y) <= bx
+            if (mcs >= 0.0) use = part(i)%pr <= pr_threshold .and. sqrt(part(i)%x**2 + part(i)%y**2) <= mcs
             if (use .and. (.not. cache_ctf .or. part(i)%imgid /= imgid0)) then
                 d1 = part(i)%df1; d2 = part(i)%df2; ast = part(i)%astig
                 imgid0 = part(i)%imgid
@@ -269,27 +273,63 @@ contains
         end do
     end subroutine
 
+    ! icosprocess mode 1.  sym = icos: reconstruct.f90 / reconstruct_substep.f90.
+    ! Any other sym (0main.f90:199-205): reconstruct_C1.f90 -- the cyclic
+    ! group's operators, mask at each particle's own shift -- or
+    ! reconstruct_C1_substep.f90 -- the identity only, rcut maxR-1, the
+    ! maxcentshift filter; both cache the CTF per micrograph.
     subroutine run_reconstruct()
         integer, parameter :: B = 256
         real(c_float), allocatable :: imgs(:), rec(:, :), fa(:), fb(:), wg(:), map(:)
+        real(c_double), target :: ops(9 * 15)
         integer :: i0, i1, i, k, layout, nt, u, used
-        real :: bx
-        logical :: sub
+        integer(c_int) :: nops
+        real :: bx, mcs, rcut
+        logical :: sub, c1path, cache, mfirst
+        type(c_ptr) :: optr
         sub = subrec == 'y' .or. subrec == 'Y'
+        c1path = trim(sym) /= 'icos'
         bx = boundx
-        if (sub) bx = fftsize / 4.0                   ! reconstruct_substep.f90:22 overrides boundX
+        mcs = -1.0
+        nops = 60
+        optr = c_null_ptr
         layout = merge(1, 0, sub)
+        if (.not. c1path) then
+            if (sub) bx = fftsize / 4.0               ! reconstruct_substep.f90:22 overrides boundX
+            rcut = real(merge(maxR - 2, maxR, sub))
+            cache = .not. sub
+            mfirst = .true.
+        else if (sub) then
+            ops = 0.0_c_double
+            ops(1) = 1.0_c_double; ops(5) = 1.0_c_double; ops(9) = 1.0_c_double
+            nops = 1                                  ! do isym=1,1 (reconstruct_C1_substep.f90:109)
+            optr = c_loc(ops)
+            bx = -1.0
+            mcs = maxcentshift
+            rcut = real(maxR - 1)
+            cache = .true.
+            mfirst = .true.
+        else
+            rc = cim_icosprocess_sym_ops(trim(sym) // c_null_char, ops, nops)
+            if (rc /= CIM_OK) then
+                print *, 'mode 1 has no operators for sym=', trim(sym)
+                stop 2
+            end if
+            optr = c_loc(ops)
+            rcut = real(maxR)
+            cache = .true.
+            mfirst = .false.
+        end if
         p%norm_on = 1
         p%mask_on = merge(1, 0, imgmask_on == 'y')
         p%maxR = maxR
-        rc = cim_rec_begin(ctx, fftsize, maxR, layout, real(merge(maxR - 2, maxR, sub), c_float), &
-                           merge(0.0_c_float, 0.01_c_float, sub), 60, c_null_ptr)
+        rc = cim_rec_begin(ctx, fftsize, maxR, layout, rcut, merge(0.0_c_float, 0.01_c_float, sub), nops, optr)
         call cim_check(ctx, rc, 'cim_rec_begin')
         allocate(imgs(fftsize * fftsize * B), rec(CIM_PART_STRIDE, B))
         used = 0
         do i0 = first, last, B
             i1 = min(last, i0 + B - 1)
-            call fill_records(i0, i1, rec, .not. sub, .true., bx)
+            call fill_records(i0, i1, rec, cache, mfirst, bx, mcs)
             do i = i0, i1
                 k = i - i0 + 1
                 call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))
@@ -315,8 +355,14 @@ contains
             rc = cim_rec_finish(ctx, map)
             call cim_check(ctx, rc, 'cim_rec_finish')
             if (realspaceavg == 'y' .or. realspaceavg == 'Y') then
-                ! reconstruct.f90:253-255: img3d_icosavg(fa, FFTsize, imgmask)
-                rc = cim_icosavg(ctx, map, int(size(map), c_long_long), fftsize, imgmask, CIM_ICOSAVG_FULL)
+                ! reconstruct.f90:253-255, reconstruct_C1.f90:286-288: img3d_icosavg(fa,
+                ! FFTsize, imgmask) over the icosahedral group or the cyclic group of sym
+                if (c1path .and. (trim(sym) == 'c1' .or. trim(sym) == 'c7')) then
+                    print *, 'img3d_icosavg has no operators for sym=', trim(sym)
+                    stop 2
+                end if
+                rc = cim_icosavg(ctx, map, int(size(map), c_long_long), fftsize, imgmask, CIM_ICOSAVG_FULL, &
+                                 merge(nops, 0, c1path), optr)
                 call cim_check(ctx, rc, 'cim_icosavg')
             end if
             call write_map(result3d, map, fftsize)
@@ -335,6 +381,7 @@ contains
         logical :: sub
         sub = subrec == 'y'                           ! 0main.f90:254 tests 'y' only
         rc = cim_icosprocess_sym_ops(trim(sym) // c_null_char, ops, nops)
+        if (trim(sym) == 'c7') rc = -1                ! reconstruction_local has no c7 branch
         if (rc /= CIM_OK) then
             print *, 'mode 6 has no local symmetry operators for sym=', trim(sym)
             stop 2
@@ -352,7 +399,7 @@ contains
         used = 0
         do i0 = first, last, B
             i1 = min(last, i0 + B - 1)
-            call fill_records(i0, i1, rec, .true., .false., boundx)
+            call fill_records(i0, i1, rec, .true., .false., boundx, -1.0)
             do i = i0, i1
                 k = i - i0 + 1
                 call read_image(part(i), img)
@@ -452,7 +499,7 @@ contains
         do i0 = first, last, B
             i1 = min(last, i0 + B - 1)
             ! mode 4 caches the CTF per micrograph; mode 5 recomputes it per particle
-            call fill_records(i0, i1, rec, mode == 4, .false., -1.0)
+            call fill_records(i0, i1, rec, mode == 4, .false., -1.0, -1.0)
             do i = i0, i1
                 k = i - i0 + 1
                 call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))
@@ -495,7 +542,7 @@ contains
         open(41, file=trim(newortfile))
         do i0 = first, last, B
             i1 = min(last, i0 + B - 1)
-            call fill_records(i0, i1, rec, .true., .false., -1.0)
+            call fill_records(i0, i1, rec, .true., .false., -1.0, -1.0)
             do i = i0, i1
                 k = i - i0 + 1
                 call read_image(part(i), imgs((k - 1) * fftsize * fftsize + 1:k * fftsize * fftsize))

@@ -120,8 +120,8 @@ def load() -> ctypes.CDLL:
         "cim_rec_finish_dev": (ip, [vp, vp]),
         "cim_merge_map": (ip, [vp, ip, ctypes.c_float, fp]),
         "cim_fsc_map": (ip, [vp, ip, ip, ctypes.c_float, ip, ip, fp, fp, fp, fp, fp, fp, fp, fp]),
-        "cim_icosavg": (ip, [vp, fp, ctypes.c_longlong, ip, ctypes.c_float, ip]),
-        "cim_icosavg_dev": (ip, [vp, vp, ctypes.c_longlong, ip, ctypes.c_float, ip]),
+        "cim_icosavg": (ip, [vp, fp, ctypes.c_longlong, ip, ctypes.c_float, ip, ip, dp]),
+        "cim_icosavg_dev": (ip, [vp, vp, ctypes.c_longlong, ip, ctypes.c_float, ip, ip, dp]),
         "cim_search_templates": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global_dev": (ip, [vp, P, ip, vp, fp]),
@@ -304,18 +304,25 @@ class Context:
         self._check(rc, "cim_fsc_map")
         return f, m
 
-    def icosavg(self, vol, r0: float, variant: int = 0, n: Optional[int] = None):
-        """Real-space icosahedral average (cim_icosavg / cim_icosavg_dev) of the
-        first n^3 values of vol (n = its box): a new array for a host array,
-        in place for a device tensor (the values past n^3 are read, not written)."""
+    def icosavg(self, vol, r0: float, variant: int = 0, n: Optional[int] = None, ops=None):
+        """Real-space average (cim_icosavg / cim_icosavg_dev) of the first n^3
+        values of vol (n = its box) over ops (default: the 60 icosahedral
+        operators): a new array for a host array, in place for a device tensor
+        (the values past n^3 are read, not written)."""
+        if ops is None:
+            nops, optr = 0, None
+        else:
+            o = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 9)
+            nops, optr = o.shape[0], o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         if hasattr(vol, "data_ptr"):
             nn = vol.shape[-1] if n is None else n
             self._check(self.lib.cim_icosavg_dev(self.h, ctypes.c_void_p(vol.data_ptr()), vol.numel(), int(nn),
-                                                 float(r0), int(variant)), "cim_icosavg_dev")
+                                                 float(r0), int(variant), nops, optr), "cim_icosavg_dev")
             return vol
         v = _f32(vol).copy()
         nn = v.shape[-1] if n is None else n
-        self._check(self.lib.cim_icosavg(self.h, _fptr(v), v.size, int(nn), float(r0), int(variant)), "cim_icosavg")
+        self._check(self.lib.cim_icosavg(self.h, _fptr(v), v.size, int(nn), float(r0), int(variant), nops, optr),
+                    "cim_icosavg")
         return v
 
     # -- search -------------------------------------------------------------

@@ -193,27 +193,33 @@ int cim_rec_finish_dev(cim_ctx* ctx, void* map_dev);
  * merge's img3d_icosavg (CIM_ICOSAVG_MERGE, radius imgmask in pixels) on the
  * full map before the crop (:142). */
 int cim_merge_map(cim_ctx* ctx, int nicosavg, float imgmask, float* map);
-/* Symmetry operators of icosprocess's local reconstruction (mode 6,
- * reconstruction_local.f90:30-55; generators icos2f_sym_mat.f90:250-540):
- * sym = "c1", "c2", "c3", "c4", "c5a", "c6", "c8", "c12" or "c15", z
- * rotations by k*360/N degrees evaluated in REAL*4 as the reference.  ops
- * (may be NULL): nops row-major 3x3 float64, ready for cim_rec_begin.
- * Returns CIM_E_ARG for names mode 6 has no operators for. */
+/* icosprocess's cyclic groups (generators icos2f_sym_mat.f90:250-540), used
+ * by mode 1 with sym != icos (reconstruct_C1.f90:33-59) and mode 6
+ * (reconstruction_local.f90:30-55): sym = "c1", "c2", "c3", "c4", "c5a",
+ * "c6", "c7", "c8", "c12" or "c15", z rotations by k*360/N degrees
+ * evaluated in REAL*4 as the reference (mode 6 has no "c7" branch; the
+ * reference program faults there).  ops (may be NULL): nops row-major 3x3
+ * float64, ready for cim_rec_begin.  Returns CIM_E_ARG for other names. */
 int cim_icosprocess_sym_ops(const char* sym, double* ops, int* nops);
 
 /* ---- real-space icosahedral average (realspaceavg=y) ---------------------- */
 #define CIM_ICOSAVG_FULL 0    /* icosprocess/SRC/img3d_icosavg.f90:2-126 */
 #define CIM_ICOSAVG_MERGE 1   /* merge_icosprocess_subrec/SRC/img3d_icosavg.f90:1-91 */
 /* Replaces the first n^3 values of map (taken as an n^3 cube, centre n/2)
- * by the mean over the 60 icosahedral operators of the trilinearly
- * interpolated map at S x, within radius r0 (pixels) of the centre and 0
- * beyond.  CIM_ICOSAVG_MERGE averages the quarter lz, ky >= 0 and fills the
- * rest with the copies of the reference's second loop, in its order and
- * flat indexing.  Taps read the flat index the reference reads; len values
- * of map are readable (reads outside them give 0).  Host array. */
-int cim_icosavg(cim_ctx* ctx, float* map, long long len, int n, float r0, int variant);
+ * by the mean over the operators of the trilinearly interpolated map at S x,
+ * within radius r0 (pixels) of the centre and 0 beyond.  ops: nops (<= 60)
+ * row-major float64 operators -- icosprocess's img3d_icosavg averages over
+ * the group of its sym (cim_icosprocess_sym_ops for the cyclic ones) -- or
+ * NULL for the 60 icosahedral operators.  CIM_ICOSAVG_MERGE (ops NULL)
+ * averages the quarter lz, ky >= 0 and fills the rest with the copies of the
+ * reference's second loop, in its order and flat indexing.  Taps read the
+ * flat index the reference reads; len values of map are readable (reads
+ * outside them give 0).  Host array. */
+int cim_icosavg(cim_ctx* ctx, float* map, long long len, int n, float r0, int variant, int nops,
+                const double* ops);
 /* Same on a device array. */
-int cim_icosavg_dev(cim_ctx* ctx, void* map_dev, long long len, int n, float r0, int variant);
+int cim_icosavg_dev(cim_ctx* ctx, void* map_dev, long long len, int n, float r0, int variant, int nops,
+                    const double* ops);
 
 /* ---- half-map FSC ------------------------------------------------------ */
 #define CIM_FSC_RECONSTRUCT 0   /* icosprocess reconstruct_FSC.f90:10-197 */

@@ -951,7 +951,9 @@ EXPORT void o_tnf_fsc(int maxR, const real* A1, const real* B1, const real* W1, 
 EXPORT void o_icosprocess_local_ops(int N, float step, real* ops)
 {
     for (int k = 0; k < N; ++k) {
-        float a = (float)k * step;
+        /* step > 0: the literal angles k*step of icos2f_sym_matc2..c15;
+         * step <= 0: icos2f_sym_matc7's float(i-1)*360.0/N (:520-523) */
+        float a = step > 0 ? (float)k * step : (float)k * 360.0f / (float)N;
         float rad = a * 3.1415926f / 180.0f;
         float c = cosf(rad), sn = sinf(rad);
         real* m = ops + 9 * k;

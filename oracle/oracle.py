@@ -176,10 +176,10 @@ class Oracle:
         return tuple(out)
 
     LOCAL_SYM = {"c1": (1, 0.0), "c2": (2, 180.0), "c3": (3, 120.0), "c4": (4, 90.0), "c5a": (5, 72.0),
-                 "c6": (6, 60.0), "c8": (8, 45.0), "c12": (12, 30.0), "c15": (15, 24.0)}
+                 "c6": (6, 60.0), "c7": (7, -1.0), "c8": (8, 45.0), "c12": (12, 30.0), "c15": (15, 24.0)}
 
     def icosprocess_local_ops(self, sym):
-        """Operators of icosprocess mode 6 (icos2f_sym_mat.f90:250-540), (N, 3, 3)."""
+        """icosprocess's cyclic groups (icos2f_sym_mat.f90:250-540), (N, 3, 3)."""
         N, step = self.LOCAL_SYM[sym]
         o = np.zeros(9 * N, self.dt)
         self.L.o_icosprocess_local_ops(N, ctypes.c_float(step), self._p(o))

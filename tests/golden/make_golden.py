@@ -425,6 +425,47 @@ def corrctf_program_vectors() -> dict:
     return g
 
 
+def symc_program_vectors() -> dict:
+    """icosprocess mode 1 with a cyclic sym (reconstruct_C1 for c5a and c7,
+    reconstruct_C1_substep, img3d_icosavg over the group) on the
+    program_vectors data.  (Mode 6 has no c7 operators; the reference program
+    ends in a segmentation fault there.)"""
+    g = {}
+    box, apix, n = 48, 2.0, 30
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    exe = os.path.join(REF, "icosprocess")
+    with tempfile.TemporaryDirectory() as d:
+        stk = "stack.mrc"
+        formats.write_mrc(os.path.join(d, stk), ps.images)
+        parts = [formats.Particle(stk, i + 1, *ps.angles[i], *ps.shifts[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+        common = ["apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt", f"last={n}", "sym=c5a"]
+
+        def run(args):
+            r = subprocess.run([exe, *args], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            return r.stdout
+
+        def raw_map(name, m):
+            return np.fromfile(os.path.join(d, name), dtype="<f4", offset=1024, count=m ** 3).reshape(m, m, m)
+
+        run(["mode=1", "result3d=c.mrc", *common])
+        g["c5a_map"] = raw_map("c.mrc", box)
+        run(["mode=1", "result3d=ca.mrc", "realspaceavg=y", *common])
+        g["c5a_avg_map"] = raw_map("ca.mrc", box)
+        out = run(["mode=1", "subrec=y", "result3d=c.tnf", "maxcentshift=1.2", *common])
+        t = formats.read_tnf(os.path.join(d, "c.tnf"))
+        g["c5asub_fa"], g["c5asub_fb"], g["c5asub_wg"] = t["fa"], t["fb"], t["wg"]
+        g["c5asub_hdr"] = np.array([t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]], i4)
+        g["c5asub_maxcentshift"] = np.array([1.2], f4)
+        run(["mode=1", "result3d=c7.mrc", *common[:-1], "sym=c7"])
+        g["c7_map"] = raw_map("c7.mrc", box)
+    return g
+
+
 def local_program_vectors() -> dict:
     """icosprocess mode 6 (reconstruct_local, reconstruct_local_substep): loc_box windows
     of the program_vectors images around seeded (x0, y0), local symmetry c5a."""
@@ -522,7 +563,8 @@ def main():
             ("lref_units", lambda: lref_vectors(rng)), ("programs", program_vectors),
             ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors),
             ("local", local_program_vectors), ("center", center_program_vectors),
-            ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors)]
+            ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors),
+            ("symc", symc_program_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

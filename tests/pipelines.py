@@ -45,6 +45,32 @@ def mode1_volume(O, images, angles, shifts, df, apix, maxR, mask_r, compact=Fals
     return vol
 
 
+def c1_volume(O, images, angles, shifts, df, apix, maxR, mask_r, sym, substep=False, maxcentshift=100.0,
+              pr=None, pr_threshold=90.0):
+    """icosprocess mode 1 with a cyclic sym.  reconstruct_C1.f90: full box,
+    rcut maxR, weights from 0.01, the group's operators, mask at each
+    particle's own shift; reconstruct_C1_substep.f90: compact, rcut maxR-1,
+    weights from 0, the identity only (its loop is isym = 1, 1), mask at the
+    first particle, particles kept when PR <= PR_threshold and
+    sqrt(x^2 + y^2) <= maxcentshift.  The CTF is cached per micrograph in
+    both; here every particle has its own."""
+    box = images.shape[-1]
+    f = np.float32
+    ops = np.eye(3)[None] if substep else O.icosprocess_local_ops(sym)
+    vol = O.new_volume(box, maxR, compact=substep, wg_init=0.0 if substep else 0.01)
+    for i in range(len(images)):
+        x, y = shifts[i]
+        if substep:
+            p = 0.0 if pr is None else pr[i]
+            if not (f(p) <= f(pr_threshold) and np.sqrt(f(x) ** 2 + f(y) ** 2) <= f(maxcentshift)):
+                continue
+        mxy = tuple(shifts[0]) if substep else (x, y)
+        a, b = O.particle_ft(images[i], x, y, mask_r=mask_r, mask_xy=mxy, norm=True)
+        c = O.ctf2d(box, df[i, 0], df[i, 1], 0.0, apix)
+        O.insert(vol, a, b, c, angles[i], maxR, rcut=maxR - 1 if substep else maxR, compact=substep, ops=ops)
+    return vol
+
+
 def corrctf_ctfs(O, box, df1, df2, apix, step, cycles):
     """The defocus series of reconstruct_corrCTF.f90:80-88: CTF images at
     df + k step, k = -cycles..cycles (defocus summed in float32), and the last

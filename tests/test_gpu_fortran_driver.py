@@ -61,6 +61,33 @@ def test_mode1_realspaceavg_map(workdir):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("sym,key", [("c5a", "c5a_map"), ("c7", "c7_map")])
+def test_mode1_cyclic_sym_map(workdir, sym, key):
+    """sym != icos in mode 1: reconstruct_C1 with the cyclic group."""
+    d, g = workdir
+    sc = np.load(os.path.join(ROOT, "tests", "golden", "symc.npz"))
+    _run(d, "mode=1", f"sym={sym}", f"result3d={sym}.mrc", "apix=2.0", "maxRes=6.0", "imgmask=40",
+         "ortfile=ort.txt", "last=30")
+    err = rel_l2(formats.read_mrc(os.path.join(d, f"{sym}.mrc")), sc[key])
+    print(f"mode 1 sym={sym} map vs reference: rel-L2 {err:.2e}")
+    assert err < 1e-4
+
+
+def test_mode1_cyclic_sym_average_and_substep(workdir):
+    """reconstruct_C1 with realspaceavg=y (average over c5a), and
+    reconstruct_C1_substep (identity only, maxcentshift filter)."""
+    d, g = workdir
+    sc = np.load(os.path.join(ROOT, "tests", "golden", "symc.npz"))
+    common = ["mode=1", "sym=c5a", "apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt", "last=30"]
+    _run(d, *common, "realspaceavg=y", "result3d=ca.mrc")
+    assert rel_l2(formats.read_mrc(os.path.join(d, "ca.mrc")), sc["c5a_avg_map"]) < 1e-4
+    _run(d, *common, "subrec=y", "maxcentshift=1.2", "result3d=c.tnf")
+    t = formats.read_tnf(os.path.join(d, "c.tnf"))
+    assert [t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]] == list(sc["c5asub_hdr"])
+    for k in ("fa", "fb", "wg"):
+        assert rel_l2(t[k], sc[f"c5asub_{k}"]) < 1e-4
+
+
 def test_mode1_subrec_tnf(workdir):
     d, g = workdir
     _run(d, "mode=1", "subrec=y", "result3d=sub.tnf", "apix=2.0", "maxRes=6.0", "imgmask=40", "ortfile=ort.txt",

@@ -99,6 +99,15 @@ def test_icosavg_full_matches_reference(ctx, avg):
     assert err < 1e-5
 
 
+def test_group_average_matches_reference(ctx):
+    """reconstruct_C1.f90:286: img3d_icosavg over the c5a group's 5 operators."""
+    sc = np.load(pipelines.golden_path("symc.npz"))
+    out = ctx.icosavg(sc["c5a_map"], 20.0, lib.ICOSAVG_FULL, ops=lib.icosprocess_sym_ops("c5a"))
+    err = rel_l2(out, sc["c5a_avg_map"])
+    print(f"c5a real-space average vs reference: rel-L2 {err:.2e}")
+    assert err < 1e-5
+
+
 def test_merges_with_icosavg_match_reference(ctx, g, avg):
     """nicosavg = 1: merge_reconstruct, merge_reconstruct_FSC (average of the
     first (box-8)^3 values as a cube) and icosprocess's reconstruct_FSC."""

@@ -44,6 +44,6 @@ def test_icosprocess_local_operators_without_gpu(O32):
         got = lib.icosprocess_sym_ops(sym)
         assert got.shape == (N, 3, 3)
         assert np.abs(got - O32.icosprocess_local_ops(sym)).max() < 1e-7, sym
-    for bad in ("c7", "icos", "i2", ""):
+    for bad in ("c9", "icos", "i2", ""):
         with pytest.raises(lib.CimError):
             lib.icosprocess_sym_ops(bad)

@@ -173,7 +173,7 @@ def test_local_symmetry_operators(O64):
         ops = O64.icosprocess_local_ops(sym)
         assert ops.shape == (N, 3, 3)
         for k in range(N):
-            a = np.deg2rad(k * step)
+            a = np.deg2rad(k * 360.0 / N)
             assert np.allclose(ops[k], [[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]], atol=2e-7)
 
 
