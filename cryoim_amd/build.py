@@ -5,7 +5,8 @@
 * ``oracle/liboracle{64,32}.so`` -- the C oracle (test infrastructure);
 * ``oracle/_ref/`` -- the reference's own Fortran programs, only where the
   reference sources are present (never on the GPU box);
-* ``cryoim_amd/bin/icosprocess_hip`` -- the Fortran driver over the C ABI,
+* ``cryoim_amd/bin/icosprocess_hip``, ``merge_icosprocess_subrec_hip`` and
+  ``local_reconstruct_hip`` -- the Fortran drivers over the C ABI,
   when amdflang is available.
 """
 from __future__ import annotations
@@ -104,7 +105,7 @@ def build_fortran() -> bool:
             "-lamdhip64"]
     iface_o = os.path.join(modd, "cim_iface.o")
     _run([fc, "-O2", "-J", modd, "-c", srcs[0], "-o", iface_o])
-    for prog in ("icosprocess_hip", "merge_icosprocess_subrec_hip"):
+    for prog in ("icosprocess_hip", "merge_icosprocess_subrec_hip", "local_reconstruct_hip"):
         src = os.path.join(fdir, prog + ".f90")
         if os.path.exists(src):
             _run([fc, "-O2", "-J", modd, "-o", os.path.join(bindir, prog), src, iface_o, *link])

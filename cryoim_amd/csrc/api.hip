@@ -71,6 +71,7 @@ static int check_params(cim_ctx* ctx, const cim_params* p) {
     if (p->minR < 0 || p->minR > p->maxR) return set_error(ctx, CIM_E_ARG, "minR %d outside [0, maxR]", p->minR);
     if (p->apply_ctf && !(p->apix > 0.f && p->vol > 0.f))
         return set_error(ctx, CIM_E_ARG, "CTF needs apix > 0 and vol > 0");
+    if (p->norm_on < 0 || p->norm_on > 2) return set_error(ctx, CIM_E_ARG, "norm_on %d is not 0, 1 or 2", p->norm_on);
     if (p->corr_cycles < 0 || p->corr_cycles > 64)
         return set_error(ctx, CIM_E_ARG, "corr_cycles %d outside [0, 64]", p->corr_cycles);
     if (p->corr_cycles > 0 && !p->apply_ctf)

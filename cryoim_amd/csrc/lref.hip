@@ -9,7 +9,9 @@
 // slice sample once and scoring it for every candidate that shares the
 // slice (the 5 shift-only candidates of a cycle share candidate 0's slice,
 // as cal_template_FT.f90:9-22 does).
+#include <array>
 #include <cmath>
+#include <string>
 #include <cstring>
 
 #include "cim_device.h"
@@ -378,11 +380,99 @@ static int local_symops(const char* sym, std::vector<float>& r) {
     return -1;
 }
 
+// ---- local_reconstruct's operators (0calc_initialvalues.f90:33-62), float32 ----
+
+namespace {
+
+using M3 = std::array<float, 9>;   // row-major
+
+M3 mul3(const M3& A, const M3& B) {
+    M3 C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+    return C;
+}
+
+M3 tr3(const M3& A) {
+    return M3{A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8]};
+}
+
+// icos_matrix_i2.f90:1-56: products of the tilted 5-fold, the 2-folds about
+// x and y and the 3-fold permutation, in the reference's order.
+std::vector<M3> icos_matrix_i2() {
+    const float d2r = 4.0f * atan2f(1.0f, 1.0f) / 180.0f;
+    const float a5 = 72.0f * d2r;
+    const float v = acosf(cosf(60.0f * d2r) / sinf(36.0f * d2r));
+    const M3 rz5{cosf(a5), -sinf(a5), 0.f, sinf(a5), cosf(a5), 0.f, 0.f, 0.f, 1.f};
+    const M3 ryv{cosf(v), 0.f, sinf(v), 0.f, 1.f, 0.f, -sinf(v), 0.f, cosf(v)};
+    const M3 z5 = mul3(ryv, mul3(rz5, tr3(ryv)));
+    const M3 rx2{1.f, 0.f, 0.f, 0.f, -1.f, 0.f, 0.f, 0.f, -1.f};
+    const M3 ry2{-1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, -1.f};
+    const M3 r3{0.f, 0.f, 1.f, 1.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+    std::vector<M3> r(60);
+    r[0] = M3{1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+    for (int i = 1; i < 5; ++i) r[i] = mul3(r[i - 1], z5);
+    for (int i = 5; i < 10; ++i) r[i] = mul3(rx2, r[i - 5]);
+    for (int i = 10; i < 20; ++i) r[i] = mul3(ry2, r[i - 10]);
+    for (int i = 20; i < 60; ++i) r[i] = mul3(r3, r[i - 20]);
+    return r;
+}
+
+// local_reconstruct's operator set: I2, I3 (I2 conjugated by a rotation about
+// x by -w, icos_matrix_i3.f90), I5 (about y by v, icos_matrix_i5.f90), or
+// C_matrix (C_matrix.f90: r1 = rotation about z by 2 pi / N, r_i = r_(i-1) r1).
+// Names as 0calc_initialvalues.f90 spells them; -1 for others.
+int local_reconstruct_ops(const char* sym, std::vector<M3>& r) {
+    std::string s(sym ? sym : "");
+    while (!s.empty() && s.back() == ' ') s.pop_back();
+    const float d2r = 4.0f * atan2f(1.0f, 1.0f) / 180.0f;
+    if (s == "i2" || s == "I2" || s == "i3" || s == "I3" || s == "i5" || s == "I5") {
+        r = icos_matrix_i2();
+        if (s == "i2" || s == "I2") return 60;
+        M3 rot0;
+        if (s == "i3" || s == "I3") {
+            const float w = acosf(cosf(36.0f * d2r) / sinf(60.0f * d2r));
+            rot0 = M3{1.f, 0.f, 0.f, 0.f, cosf(-w), sinf(-w), 0.f, -sinf(-w), cosf(-w)};
+        } else {
+            const float v = acosf(cosf(60.0f * d2r) / sinf(36.0f * d2r));
+            rot0 = M3{cosf(v), 0.f, sinf(v), 0.f, 1.f, 0.f, -sinf(v), 0.f, cosf(v)};
+        }
+        for (auto& m : r) m = mul3(rot0, mul3(m, tr3(rot0)));
+        return 60;
+    }
+    int N = 0;
+    for (int n = 1; n <= 15; ++n) {
+        const std::string lo = "c" + std::to_string(n);
+        const std::string up = n < 10 ? "C" + std::to_string(n) : "C_" + std::to_string(n);
+        if (s == lo || s == up) N = n;
+    }
+    if (N == 0) return -1;
+    const float pi = 4.0f * atan2f(1.0f, 1.0f);
+    const float phi = 2.0f * pi / (float)N;
+    r.assign(N, M3{});
+    r[0] = M3{cosf(phi), -sinf(phi), 0.f, sinf(phi), cosf(phi), 0.f, 0.f, 0.f, 1.f};
+    for (int i = 1; i < N; ++i) r[i] = mul3(r[i - 1], r[0]);
+    return N;
+}
+
+}  // namespace
+
 }  // namespace cim
 
 using namespace cim;
 
 extern "C" {
+
+int cim_local_reconstruct_sym_ops(const char* sym, double* ops, int* nops) {
+    std::vector<M3> r;
+    const int n = local_reconstruct_ops(sym, r);
+    if (n < 0) return CIM_E_ARG;
+    if (nops) *nops = n;
+    if (ops)
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < 9; ++i) ops[9 * k + i] = r[k][i];
+    return CIM_OK;
+}
 
 int cim_sym_ops(const char* sym, double* ops, int* nops) {
     std::vector<float> r;

@@ -40,7 +40,9 @@ int deinterleave(cim_ctx* ctx, const float2* in, float* re, float* im, long long
     return CIM_OK;
 }
 
-// norm -> mask -> checkerboard, real image in, complex (v, 0) out.
+// [norm] -> [mask] -> [norm] -> checkerboard, real image in, complex (v, 0)
+// out.  do_norm = 1 normalises before the mask (reconstruct.f90:79-92),
+// 2 after it (local_reconstruct/SRC/reconstruct_substep.f90:93-94).
 __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ imgs, const float* __restrict__ parts,
                                                     float2* __restrict__ out, int box, int do_norm, int do_mask,
                                                     float mask_r) {
@@ -50,7 +52,7 @@ __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ im
     float2* o = out + (long long)blockIdx.x * N;
     const float* pr = parts + (long long)blockIdx.x * CIM_PART_STRIDE;
     double avg = 0.0, inv_dev = 1.0;
-    if (do_norm) {
+    if (do_norm == 1) {
         double s = 0.0;
         for (long long i = threadIdx.x; i < N; i += blockDim.x) s += img[i];
         avg = block_sum(s, sh) / (double)N;
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ im
         ring_mean = cnt != 0.0 ? (float)(ssum / cnt) : 0.f;
     }
     const float pi = 3.14159265358979323846f;
-    for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+    auto masked = [&](long long i) -> float {
         int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
         float v = (float)(((double)img[i] - avg) * inv_dev);
         if (do_mask) {
@@ -98,6 +100,24 @@ __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ im
             }
             if (ra > R2) v = ring_mean;
         }
+        return v;
+    };
+    double avg2 = 0.0, inv_dev2 = 1.0;
+    if (do_norm == 2) {
+        double s = 0.0;
+        for (long long i = threadIdx.x; i < N; i += blockDim.x) s += masked(i);
+        avg2 = block_sum(s, sh) / (double)N;
+        double q = 0.0;
+        for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+            double d = (double)masked(i) - avg2;
+            q += d * d;
+        }
+        inv_dev2 = 1.0 / sqrt(block_sum(q, sh) / (double)N);
+    }
+    for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+        int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
+        float v = masked(i);
+        if (do_norm == 2) v = (float)(((double)v - avg2) * inv_dev2);
         if ((ix + iy) & 1) v = -v;
         o[i] = make_float2(v, 0.f);
     }

@@ -170,6 +170,14 @@ module cim_iface
             real(c_float), value :: convF, maxres
         end function
 
+        integer(c_int) function cim_local_reconstruct_sym_ops(sym, ops, nops) &
+                bind(C, name='cim_local_reconstruct_sym_ops')
+            import :: c_int, c_char, c_double
+            character(kind=c_char) :: sym(*)
+            real(c_double) :: ops(*)
+            integer(c_int) :: nops
+        end function
+
         integer(c_int) function cim_icosprocess_sym_ops(sym, ops, nops) bind(C, name='cim_icosprocess_sym_ops')
             import :: c_int, c_char, c_double
             character(kind=c_char) :: sym(*)

@@ -37,6 +37,11 @@ def sym_ops(sym: str) -> np.ndarray:
     return _ops(load().cim_sym_ops, sym)
 
 
+def local_reconstruct_sym_ops(sym: str) -> np.ndarray:
+    """local_reconstruct's operators (cim_local_reconstruct_sym_ops), (n, 3, 3)."""
+    return _ops(load().cim_local_reconstruct_sym_ops, sym)
+
+
 def icosprocess_sym_ops(sym: str) -> np.ndarray:
     """Local symmetry operators of icosprocess mode 6 (cim_icosprocess_sym_ops), (n, 3, 3)."""
     return _ops(load().cim_icosprocess_sym_ops, sym)
@@ -65,7 +70,7 @@ def make_params(box, minR=0, maxR=None, apply_ctf=True, mask_on=True, norm_on=Fa
     p.maxR = int(maxR if maxR is not None else box // 2 - 2)
     p.apply_ctf = int(bool(apply_ctf))
     p.mask_on = int(bool(mask_on))
-    p.norm_on = int(bool(norm_on))
+    p.norm_on = int(norm_on)          # 0, 1 (before the mask) or 2 (after it); True/False as 1/0
     p.ncycle = int(ncycle)
     p.conj_out = int(bool(conj_out))
     p.apix = float(apix)
@@ -129,6 +134,7 @@ def load() -> ctypes.CDLL:
         "cim_refine_local": (ip, [vp, P, ip, fp, fp]),
         "cim_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
         "cim_icosprocess_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
+        "cim_local_reconstruct_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
         "cim_lref_model": (ip, [vp, P, fp]),
         "cim_center_model": (ip, [vp, P, fp]),
         "cim_center": (ip, [vp, P, ip, ip, fp, fp, fp]),

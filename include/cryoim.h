@@ -33,6 +33,8 @@
  *                      calc_center_selfrot180.f90
  *   real-space avg ... icosprocess/SRC/img3d_icosavg.f90,
  *                      merge_icosprocess_subrec/SRC/img3d_icosavg.f90
+ *   local_reconstruct  local_reconstruct/SRC/reconstruct.f90,
+ *                      reconstruct_substep.f90, C_matrix.f90, icos_matrix_i*.f90
  */
 #ifndef CRYOIM_H
 #define CRYOIM_H
@@ -58,7 +60,8 @@ typedef struct cim_params {
     int maxR;         /* outer search / insert radius (Fourier pixels) */
     int apply_ctf;    /* 1: multiply by getCTF2d (applyCTF=y) */
     int mask_on;      /* 1: soft circular mask (imgmask given/default y) */
-    int norm_on;      /* 1: zero-mean unit-RMS normalisation before masking */
+    int norm_on;      /* 1: zero-mean unit-RMS normalisation before masking;
+                         2: after masking (local_reconstruct's substep) */
     int ncycle;       /* refinement cycles (Ncycle) */
     int conj_out;     /* FFT stores b = -Im (1, icosprocess) or +Im (0, local_refine) */
     int search_method;/* global search: CIM_SEARCH_PLAIN (0) or CIM_SEARCH_TABLES (1) */
@@ -268,6 +271,13 @@ int cim_refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* imag
  * "c1".."c360" (rotations by i*360/N, i = 1..N), "asym", "i2", "i5", and
  * "icos" (the icos2f_sym_mat table).  ops may be NULL to query nops. */
 int cim_sym_ops(const char* sym, double* ops, int* nops);
+/* Symmetry operators of local_reconstruct (0calc_initialvalues.f90:33-62):
+ * "i2"/"I2" (icos_matrix_i2.f90), "i3"/"I3" and "i5"/"I5" (I2 conjugated onto
+ * the 3-fold and 5-fold frames), or "c1".."c15" ("C1".."C9", "C_10".."C_15")
+ * by C_matrix.f90, whose first operator is the rotation by 360/N and whose
+ * last is the identity -- generated in float32 in the reference's order.
+ * ops may be NULL to query nops; CIM_E_ARG for other names. */
+int cim_local_reconstruct_sym_ops(const char* sym, double* ops, int* nops);
 /* Resident model transform (refine_PR.f90:21-54): soft mask of radius
  * p->imgmask (img3d_softmask.f90), checkerboard, forward 3D FFT (1/N, +Im),
  * checkerboard.  vol: box^3 real map, box = p->box. */

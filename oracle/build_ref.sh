@@ -4,8 +4,7 @@
 # drivers in oracle/ref_drivers/ that call individual reference routines.
 # Everything it writes goes to oracle/_ref/ (git-ignored).  Nothing here is
 # copied into the repository.  Needs amdflang (ROCm) and MKL's FFTW3
-# wrappers (libmkl_rt from /opt/conda/lib).  local_reconstruct is not built:
-# its 0input_parameters.f90:32 needs an ifort extension flang rejects.
+# wrappers (libmkl_rt from /opt/conda/lib).
 set -euo pipefail
 REF=${CIM_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -60,6 +59,22 @@ sed "s/\.eq\.'corrctf')/.eq.'corrCTF')/" "$S/0main.f90" > "$OUT/icos_corrctf/0ma
 if ! grep -q "eq.'corrCTF')" "$OUT/icos_corrctf/0main_corrctfkey.f90"; then echo "corrCTF key patch did not apply"; exit 4; fi
 $FC $FFLAGS -J "$OUT/icos" -c "$OUT/icos_corrctf/0main_corrctfkey.f90" -o "$OUT/icos_corrctf/0main_corrctfkey.o"
 $FC $FFLAGS -o "$OUT/icosprocess_corrctf" $ICOS_LIBOBJS "$OUT/icos_corrctf/0main_corrctfkey.o" $LIBS
+
+# local_reconstruct: flang rejects one line of 0input_parameters.f90 (:32,
+# local_sym='c1', a character constant assigned to an undeclared implicit
+# integer that nothing reads).  The program is built from its own sources
+# with a generated copy of that file lacking the line, under _ref/ only.
+LR=$REF/local_reconstruct/SRC
+mkdir -p "$OUT/lrec"
+sed "/^ *local_sym='c1'/d" "$LR/0input_parameters.f90" > "$OUT/lrec/0input_parameters_nolocalsym.f90"
+if grep -q "local_sym='c1'" "$OUT/lrec/0input_parameters_nolocalsym.f90"; then echo "local_sym line not removed"; exit 4; fi
+LREC_SRCS=("$LR/0common_icosproc.f90" "$LR/FFTW.f90")
+for f in "$LR"/*.f90; do
+  case "$(basename "$f")" in 0common_icosproc.f90|FFTW.f90|0input_parameters.f90) ;; *) LREC_SRCS+=("$f");; esac
+done
+LREC_SRCS+=("$OUT/lrec/0input_parameters_nolocalsym.f90")
+build_objs "$OUT/lrec" "${LREC_SRCS[@]}"
+$FC $FFLAGS -o "$OUT/local_reconstruct" "$OUT"/lrec/*.o $LIBS
 
 # merge_icosprocess_subrec (tnf merge, plain and FSC-weighted)
 M=$REF/merge_icosprocess_subrec/SRC

@@ -1565,3 +1565,90 @@ EXPORT void o_img3d_icosavg_merge(real* img, long len, int nx, real r0, const re
     for (long i = 0; i < N; ++i) img[i] = acc[i] / (real)60;
     free(acc);
 }
+
+/* ------------------------------------------------------------------ */
+/* local_reconstruct's symmetry operators                               */
+/* ------------------------------------------------------------------ */
+
+static void mm3t(const real* A, const real* B, real* C)   /* C = A B^T */
+{
+    real T[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T[3 * j + i] = B[3 * i + j];
+    matmul3(A, T, C);
+}
+
+/* C_matrix.f90:1-17: r1 = rotation about z by 2 pi / N, r_i = r_(i-1) r1
+ * (so the identity comes last, up to rounding). */
+static void o_c_matrix(int N, real* r)
+{
+    real pi = r_pi();
+    real phi = (real)2.0 * pi / (real)N;
+    real* m = r;
+    for (int i = 0; i < 9; ++i) m[i] = 0;
+    m[0] = (real)cos(phi); m[1] = -(real)sin(phi);
+    m[3] = (real)sin(phi); m[4] = (real)cos(phi);
+    m[8] = (real)1.0;
+    for (int i = 1; i < N; ++i) matmul3(r + 9 * (i - 1), r, r + 9 * i);
+}
+
+/* icos_matrix_i2.f90:1-56: the 60 operators built from a 5-fold tilted by
+ * v onto the 2-fold frame (ryv rz5 ryv^T), the 2-folds about x and y and the
+ * 3-fold permutation about (1,1,1), in that order of products. */
+static void o_icos_matrix_i2(real* r)
+{
+    real d2r = r_pi() / (real)180.0;
+    real a5 = (real)72.0 * d2r;
+    real v = (real)acos((real)cos((real)60.0 * d2r) / (real)sin((real)36.0 * d2r));
+    real rz5[9] = {(real)cos(a5), -(real)sin(a5), 0, (real)sin(a5), (real)cos(a5), 0, 0, 0, 1};
+    real ryv[9] = {(real)cos(v), 0, (real)sin(v), 0, 1, 0, -(real)sin(v), 0, (real)cos(v)};
+    real t[9], z5[9];
+    mm3t(rz5, ryv, t);           /* rz5 ryv^T */
+    matmul3(ryv, t, z5);         /* ryv (rz5 ryv^T) */
+    real rx2[9] = {1, 0, 0, 0, -1, 0, 0, 0, -1};
+    real ry2[9] = {-1, 0, 0, 0, 1, 0, 0, 0, -1};
+    real r3[9] = {0, 0, 1, 1, 0, 0, 0, 1, 0};
+    for (int i = 0; i < 60 * 9; ++i) r[i] = 0;
+    r[0] = 1; r[4] = 1; r[8] = 1;
+    for (int i = 1; i < 5; ++i) matmul3(r + 9 * (i - 1), z5, r + 9 * i);
+    for (int i = 5; i < 10; ++i) matmul3(rx2, r + 9 * (i - 5), r + 9 * i);
+    for (int i = 10; i < 20; ++i) matmul3(ry2, r + 9 * (i - 10), r + 9 * i);
+    for (int i = 20; i < 40; ++i) matmul3(r3, r + 9 * (i - 20), r + 9 * i);
+    for (int i = 40; i < 60; ++i) matmul3(r3, r + 9 * (i - 20), r + 9 * i);
+}
+
+/* icos_matrix_i3.f90 / icos_matrix_i5.f90: the I2 operators conjugated by a
+ * rotation about x by -w (2f-3f angle) or about y by v (2f-5f angle):
+ * r_i = rot0 (r_i rot0^T). */
+static void o_icos_matrix_conj(real* r, int kind)
+{
+    real d2r = r_pi() / (real)180.0;
+    real rot0[9];
+    o_icos_matrix_i2(r);
+    if (kind == 3) {
+        real w = (real)acos((real)cos((real)36.0 * d2r) / (real)sin((real)60.0 * d2r));
+        real c = (real)cos(-w), s = (real)sin(-w);
+        real m[9] = {1, 0, 0, 0, c, s, 0, -s, c};
+        for (int i = 0; i < 9; ++i) rot0[i] = m[i];
+    } else {
+        real v = (real)acos((real)cos((real)60.0 * d2r) / (real)sin((real)36.0 * d2r));
+        real m[9] = {(real)cos(v), 0, (real)sin(v), 0, 1, 0, -(real)sin(v), 0, (real)cos(v)};
+        for (int i = 0; i < 9; ++i) rot0[i] = m[i];
+    }
+    for (int i = 0; i < 60; ++i) {
+        real t[9];
+        mm3t(r + 9 * i, rot0, t);      /* r_i rot0^T */
+        matmul3(rot0, t, r + 9 * i);   /* rot0 (r_i rot0^T) */
+    }
+}
+
+/* local_reconstruct/SRC/0calc_initialvalues.f90:33-62: kind 2, 3, 5 for
+ * I2 / I3 / I5 (60 operators), 0 for C_matrix with N operators.  Returns
+ * the count. */
+EXPORT int o_local_reconstruct_ops(int kind, int N, real* r)
+{
+    if (kind == 2) { o_icos_matrix_i2(r); return 60; }
+    if (kind == 3 || kind == 5) { o_icos_matrix_conj(r, kind); return 60; }
+    o_c_matrix(N, r);
+    return N;
+}

@@ -111,14 +111,18 @@ class Oracle:
         return f
 
     # ---- pipelines -------------------------------------------------------------
-    def particle_ft(self, img, x, y, mask_r=None, mask_xy=None, norm=False, conj_out=True, shift=True):
-        """norm -> mask -> checkerboard -> FFT -> centre shift (reconstruct.f90:79-96)."""
+    def particle_ft(self, img, x, y, mask_r=None, mask_xy=None, norm=False, conj_out=True, shift=True,
+                    mask_first=False):
+        """norm -> mask -> checkerboard -> FFT -> centre shift (reconstruct.f90:79-96);
+        mask_first: mask, then norm (local_reconstruct/SRC/reconstruct_substep.f90:93-94)."""
         a = self._a(img).copy()
-        if norm:
+        if norm and not mask_first:
             a = self.norm(a)
         if mask_r is not None:
             mx, my = (x, y) if mask_xy is None else mask_xy
             a = self.mask(a, mask_r, mx, my)
+        if norm and mask_first:
+            a = self.norm(a)
         a = self.checker(a)
         fa, fb = self.fft2(a, np.zeros_like(a), 1, conj_out)
         if shift:
@@ -421,3 +425,25 @@ class Oracle:
         o = self._a(self.icos_ops()).reshape(-1, 9)
         self.L.o_img3d_icosavg_merge(self._p(flat), flat.size, int(n), self._r(r0), self._p(o))
         return v
+
+    # ---- local_reconstruct ---------------------------------------------------
+    @staticmethod
+    def local_reconstruct_sym(sym):
+        """(kind, N) of local_reconstruct/SRC/0calc_initialvalues.f90:33-62:
+        I2 / I3 / I5 (60 operators) or C_matrix with N from c1..c15 (the
+        spellings it accepts; a sym it does not know keeps Nsym = 1)."""
+        s = sym.strip()
+        for k in (2, 3, 5):
+            if s in (f"i{k}", f"I{k}"):
+                return k, 60
+        for n in range(1, 16):
+            names = (f"c{n}", f"C{n}") if n < 10 else (f"c{n}", f"C_{n}")
+            if s in names:
+                return 0, n
+        return 0, 1
+
+    def local_reconstruct_ops(self, sym):
+        kind, n = self.local_reconstruct_sym(sym)
+        r = np.zeros(9 * 60, self.dt)
+        m = self.L.o_local_reconstruct_ops(kind, n, self._p(r))
+        return r[:9 * m].reshape(m, 3, 3)

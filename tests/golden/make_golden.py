@@ -233,6 +233,62 @@ def lref_program_vectors() -> dict:
     return g
 
 
+def local_reconstruct_vectors() -> dict:
+    """The real local_reconstruct (built with the one line flang rejects
+    removed) on 40 sub-particles: 32-pixel boxes cut from 48-pixel images at
+    (local_hx0, local_ky0), local orientations and shifts, CTF in Angstrom;
+    maps for local_sym c5, i2, i3, i5, the substep tnf, and Zhigh_df."""
+    g = {}
+    box, box0, apix, n = 32, 48, 2.0, 40
+    ph = synth.make_phantom(box, nblobs=8, radius=(6, 10), sigma=(1.2, 1.8), seed=27, sym=False)
+    rng = np.random.default_rng(28)
+    # rounded to the decimals the local ortfile carries, so the stored values are the ones read
+    ang = np.round(np.c_[rng.uniform(20, 160, n), rng.uniform(-180, 180, n), rng.uniform(0, 360, n)], 4)
+    sh = np.round(rng.uniform(-1.0, 1.0, (n, 2)), 4)
+    proj = synth.project(ph, ang, sh, box=box)
+    df = np.round(rng.uniform(8000, 20000, n), 3)
+    lz = np.round(rng.uniform(-300.0, 300.0, n), 4)
+    imgs = np.zeros((n, box0, box0), f4)
+    cen = rng.integers(14, 34, (n, 2))
+    for i in range(n):
+        c = synth.ctf2d(box, df[i], df[i], 0.0, apix, vol=200.0, df_scale=1.0)
+        p = synth.apply_ctf(proj[i], c) + 0.3 * rng.standard_normal((box, box))
+        ys, xs = (np.arange(box) + cen[i, 1] - box // 2) % box0, (np.arange(box) + cen[i, 0] - box // 2) % box0
+        imgs[i][np.ix_(ys, xs)] = p
+    g.update(lrec_images=imgs, lrec_angles=ang.astype(f4), lrec_shifts=sh.astype(f4), lrec_df=df.astype(f4),
+             lrec_z=lz.astype(f4), lrec_cen=cen.astype(i4), lrec_box=np.array([box, box0], i4),
+             lrec_apix=np.array([apix], f4))
+    exe = os.path.join(REF, "local_reconstruct")
+    with tempfile.TemporaryDirectory() as d:
+        formats.write_mrc(os.path.join(d, "stk.mrc"), imgs)
+        with open(os.path.join(d, "lort.txt"), "w") as f:
+            for i in range(n):
+                f.write(f"stk.mrc {i + 1} 0 0 0 0 0 0 {i + 1} {i + 1} {df[i]:.3f} {df[i]:.3f} 0 {i + 1} "
+                        f"{cen[i, 0]} {cen[i, 1]} {ang[i, 0]:.4f} {ang[i, 1]:.4f} {ang[i, 2]:.4f} "
+                        f"{sh[i, 0]:.4f} {sh[i, 1]:.4f} {lz[i]:.4f} 0\n")
+        common = ["local_ort=lort.txt", f"local_FFTsize={box}", f"apix={apix}", "maxRes=6"]
+        runs = {"c5": ["local_sym=c5"], "i2": ["local_sym=i2"], "i3": ["local_sym=I3"], "i5": ["local_sym=i5"],
+                "zhigh": ["local_sym=c1", "Zhigh_df=y", "sign=-"]}
+        for tag, extra in runs.items():
+            r = subprocess.run([exe, *common, *extra, f"result3d={tag}.mrc"], cwd=d, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            g[f"lrec_{tag}_map"] = np.fromfile(os.path.join(d, f"{tag}.mrc"), dtype="<f4", offset=1024).reshape(
+                box, box, box)
+        with open(os.path.join(d, "c5.mrc"), "rb") as f:
+            g["lrec_hdr"] = np.frombuffer(f.read(1024), dtype="<i4").copy()
+        r = subprocess.run([exe, *common, "local_sym=c5", "subrec=y", "imgmask=24", "result3d=c5.tnf"], cwd=d,
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout)
+        t = formats.read_tnf(os.path.join(d, "c5.tnf"))
+        g["lrecsub_fa"], g["lrecsub_fb"], g["lrecsub_wg"] = t["fa"], t["fb"], t["wg"]
+        g["lrecsub_hdr"] = np.array([t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]], i4)
+        g["lrecsub_hdrf"] = np.array([t["apix"], t["imgmask"]], f4)
+    return g
+
+
 def program_vectors() -> dict:
     """Real icosprocess / local_refine runs on a 48-pixel synthetic set."""
     g = {}
@@ -564,7 +620,7 @@ def main():
             ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors),
             ("local", local_program_vectors), ("center", center_program_vectors),
             ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors),
-            ("symc", symc_program_vectors)]
+            ("symc", symc_program_vectors), ("lrec", local_reconstruct_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

@@ -71,6 +71,33 @@ def c1_volume(O, images, angles, shifts, df, apix, maxR, mask_r, sym, substep=Fa
     return vol
 
 
+def local_reconstruct_volume(O, images0, cen, angles, shifts, df, apix, box, maxR, sym, compact=False,
+                             mask_r=None, z=None, sign=None, vol=200.0):
+    """local_reconstruct (reconstruct.f90:53-147; reconstruct_substep.f90):
+    the box x box sub-particle cut around cen from each full image (periodic,
+    img2d_local_box.f90), normalised; the substep variant masks it at its
+    local shift; FFT, centre shift by the local shift, CTF in Angstrom (df -/+
+    local z with Zhigh_df), insert with local_reconstruct's operators.  The
+    substep masks before it normalises.  Full
+    box with rcut maxR and weights 0.01, or compact with rcut maxR-1 and
+    weights 0.1.  (The full image's own normalisation before the cut does not
+    change the result: the cut is normalised again.)"""
+    ops = O.local_reconstruct_ops(sym)
+    vol_ = O.new_volume(box, maxR, compact=compact, wg_init=0.1 if compact else 0.01)
+    f = np.float32
+    for i in range(len(images0)):
+        a0 = O.local_box(images0[i], int(cen[i, 0]), int(cen[i, 1]), box)
+        x, y = shifts[i]
+        d1 = d2 = f(df[i])
+        if z is not None:
+            d1 = d2 = f(df[i]) - f(z[i]) if sign == "-" else f(df[i]) + f(z[i])
+        c = O.ctf2d(box, d1, d2, 0.0, apix, vol=vol, df_scale=1.0)
+        a, b = O.particle_ft(a0, x, y, mask_r=mask_r if compact else None, mask_xy=(x, y), norm=True,
+                             mask_first=True)
+        O.insert(vol_, a, b, c, angles[i], maxR, rcut=maxR - 1 if compact else maxR, compact=compact, ops=ops)
+    return vol_
+
+
 def corrctf_ctfs(O, box, df1, df2, apix, step, cycles):
     """The defocus series of reconstruct_corrCTF.f90:80-88: CTF images at
     df + k step, k = -cycles..cycles (defocus summed in float32), and the last

@@ -47,3 +47,18 @@ def test_icosprocess_local_operators_without_gpu(O32):
     for bad in ("c9", "icos", "i2", ""):
         with pytest.raises(lib.CimError):
             lib.icosprocess_sym_ops(bad)
+
+
+def test_local_reconstruct_operators_without_gpu(O32):
+    """cim_local_reconstruct_sym_ops (host-only) against the oracle's float32
+    restatement of the same generators."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    import numpy as np
+    for sym in ("i2", "I3", "i5", "c1", "c5", "C7", "C_12", "c15"):
+        got = lib.local_reconstruct_sym_ops(sym)
+        ref = O32.local_reconstruct_ops(sym)
+        assert got.shape == ref.shape, sym
+        assert np.abs(got - ref).max() < 1e-7, sym
+    with pytest.raises(lib.CimError):
+        lib.local_reconstruct_sym_ops("c16")
