@@ -105,7 +105,7 @@ def build_fortran() -> bool:
             "-lamdhip64"]
     iface_o = os.path.join(modd, "cim_iface.o")
     _run([fc, "-O2", "-J", modd, "-c", srcs[0], "-o", iface_o])
-    for prog in ("icosprocess_hip", "merge_icosprocess_subrec_hip", "local_reconstruct_hip"):
+    for prog in ("icosprocess_hip", "merge_icosprocess_subrec_hip", "local_reconstruct_hip", "local_box_hip"):
         src = os.path.join(fdir, prog + ".f90")
         if os.path.exists(src):
             _run([fc, "-O2", "-J", modd, "-o", os.path.join(bindir, prog), src, iface_o, *link])

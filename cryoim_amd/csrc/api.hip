@@ -129,6 +129,7 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->srch.wk.release();
     ctx->lref.release();
     ctx->cen.release();
+    ctx->lbox.release();
     for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->aux, &ctx->aux2, &ctx->parts, &ctx->out, &ctx->fsc_in,
                       &ctx->fsc_nrm, &ctx->fsc_acc, &ctx->fsc_sums, &ctx->fsc_wt, &ctx->avg_ops, &ctx->avg_acc})
         b->release();

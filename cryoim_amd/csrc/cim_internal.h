@@ -108,6 +108,15 @@ struct CenterState {
     }
 };
 
+// Device buffers of local_box: angles and shifts, operators, per-(particle,
+// operator) records, complex working images.
+struct LboxState {
+    DevBuf pr, ops, rec, x;
+    void release() {
+        for (DevBuf* b : {&pr, &ops, &rec, &x}) b->release();
+    }
+};
+
 struct KTimer {
     double total_ms = 0.0;
     long long launches = 0;
@@ -124,6 +133,7 @@ struct cim_ctx {
     cim::SearchState srch;
     cim::LrefState lref;
     cim::CenterState cen;
+    cim::LboxState lbox;
     // scratch buffers reused across calls
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
     // half-map FSC: input tnf pairs, normalised copies, merged sums, shell sums, weights

@@ -421,8 +421,11 @@ std::vector<M3> icos_matrix_i2() {
 // local_reconstruct's operator set: I2, I3 (I2 conjugated by a rotation about
 // x by -w, icos_matrix_i3.f90), I5 (about y by v, icos_matrix_i5.f90), or
 // C_matrix (C_matrix.f90: r1 = rotation about z by 2 pi / N, r_i = r_(i-1) r1).
-// Names as 0calc_initialvalues.f90 spells them; -1 for others.
-int local_reconstruct_ops(const char* sym, std::vector<M3>& r) {
+// Names as 0calc_initialvalues.f90 spells them ("C_10".."C_15"); -1 for
+// others.  box_names: as local_box/SRC/0main.f90:80-122 spells them
+// ("C10".."C15"), and a name it does not know keeps its default Nsym = 60
+// and goes to C_matrix.
+int local_reconstruct_ops(const char* sym, std::vector<M3>& r, bool box_names = false) {
     std::string s(sym ? sym : "");
     while (!s.empty() && s.back() == ' ') s.pop_back();
     const float d2r = 4.0f * atan2f(1.0f, 1.0f) / 180.0f;
@@ -443,9 +446,10 @@ int local_reconstruct_ops(const char* sym, std::vector<M3>& r) {
     int N = 0;
     for (int n = 1; n <= 15; ++n) {
         const std::string lo = "c" + std::to_string(n);
-        const std::string up = n < 10 ? "C" + std::to_string(n) : "C_" + std::to_string(n);
+        const std::string up = n < 10 || box_names ? "C" + std::to_string(n) : "C_" + std::to_string(n);
         if (s == lo || s == up) N = n;
     }
+    if (N == 0 && box_names) N = 60;
     if (N == 0) return -1;
     const float pi = 4.0f * atan2f(1.0f, 1.0f);
     const float phi = 2.0f * pi / (float)N;
@@ -467,6 +471,16 @@ int cim_local_reconstruct_sym_ops(const char* sym, double* ops, int* nops) {
     std::vector<M3> r;
     const int n = local_reconstruct_ops(sym, r);
     if (n < 0) return CIM_E_ARG;
+    if (nops) *nops = n;
+    if (ops)
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < 9; ++i) ops[9 * k + i] = r[k][i];
+    return CIM_OK;
+}
+
+int cim_local_box_sym_ops(const char* sym, double* ops, int* nops) {
+    std::vector<M3> r;
+    const int n = local_reconstruct_ops(sym, r, true);
     if (nops) *nops = n;
     if (ops)
         for (int k = 0; k < n; ++k)

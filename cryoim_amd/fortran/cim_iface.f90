@@ -15,6 +15,7 @@ module cim_iface
     ! case-insensitive, so CIM_CENTER_MODEL would clash with cim_center_model)
     integer(c_int), parameter :: CIM_CTR_MODEL = 0, CIM_CTR_SELFROT180 = 1
     integer(c_int), parameter :: CIM_ICOSAVG_FULL = 0, CIM_ICOSAVG_MERGE = 1
+    integer(c_int), parameter :: CIM_LBOX_REC = 8
     ! 1-based positions of the record fields in rec(:, i)
     integer, parameter :: P_THETA = 1, P_PHI = 2, P_OMEGA = 3, P_X = 4, P_Y = 5, P_PR = 6
     integer, parameter :: P_DF1 = 7, P_DF2 = 8, P_ASTIG = 9, P_MASKX = 10, P_MASKY = 11, P_USE = 12
@@ -183,6 +184,33 @@ module cim_iface
             character(kind=c_char) :: sym(*)
             real(c_double) :: ops(*)
             integer(c_int) :: nops
+        end function
+
+        integer(c_int) function cim_local_box_sym_ops(sym, ops, nops) bind(C, name='cim_local_box_sym_ops')
+            import :: c_int, c_char, c_double
+            character(kind=c_char) :: sym(*)
+            real(c_double) :: ops(*)
+            integer(c_int) :: nops
+        end function
+
+        integer(c_int) function cim_local_box_coords(ctx, n, parts, nops, ops, x0, box, apix, rec) &
+                bind(C, name='cim_local_box_coords')
+            import :: c_ptr, c_int, c_float, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: n, nops, box
+            real(c_float) :: parts(*), x0(3), rec(*)
+            real(c_double) :: ops(*)
+            real(c_float), value :: apix
+        end function
+
+        integer(c_int) function cim_local_box_stack(ctx, n, images, parts, box, nops, ops, x0, sub, apix, boxes, rec) &
+                bind(C, name='cim_local_box_stack')
+            import :: c_ptr, c_int, c_float, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: n, box, nops, sub
+            real(c_float) :: images(*), parts(*), x0(3), boxes(*), rec(*)
+            real(c_double) :: ops(*)
+            real(c_float), value :: apix
         end function
 
         function c_strlen(s) bind(C, name='strlen')

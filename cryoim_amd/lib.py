@@ -19,6 +19,7 @@ PART_STRIDE = 12
 SEARCH_PLAIN, SEARCH_TABLES = 0, 1
 CENTER_MODEL, CENTER_SELFROT180 = 0, 1
 ICOSAVG_FULL, ICOSAVG_MERGE = 0, 1
+LBOX_REC = 8
 P_THETA, P_PHI, P_OMEGA, P_X, P_Y, P_PR, P_DF1, P_DF2, P_ASTIG, P_MASKX, P_MASKY, P_USE = range(12)
 P_BOXX, P_BOXY = P_MASKX, P_MASKY
 
@@ -40,6 +41,11 @@ def sym_ops(sym: str) -> np.ndarray:
 def local_reconstruct_sym_ops(sym: str) -> np.ndarray:
     """local_reconstruct's operators (cim_local_reconstruct_sym_ops), (n, 3, 3)."""
     return _ops(load().cim_local_reconstruct_sym_ops, sym)
+
+
+def local_box_sym_ops(sym: str) -> np.ndarray:
+    """local_box's operators (cim_local_box_sym_ops), (n, 3, 3)."""
+    return _ops(load().cim_local_box_sym_ops, sym)
 
 
 def icosprocess_sym_ops(sym: str) -> np.ndarray:
@@ -135,6 +141,10 @@ def load() -> ctypes.CDLL:
         "cim_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
         "cim_icosprocess_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
         "cim_local_reconstruct_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
+        "cim_local_box_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
+        "cim_local_box_coords": (ip, [vp, ip, fp, ip, dp, fp, ip, ctypes.c_float, fp]),
+        "cim_local_box_stack": (ip, [vp, ip, fp, fp, ip, ip, dp, fp, ip, ctypes.c_float, fp, fp]),
+        "cim_local_box_stack_dev": (ip, [vp, ip, vp, fp, ip, ip, dp, fp, ip, ctypes.c_float, vp, fp]),
         "cim_lref_model": (ip, [vp, P, fp]),
         "cim_center_model": (ip, [vp, P, fp]),
         "cim_center": (ip, [vp, P, ip, ip, fp, fp, fp]),
@@ -330,6 +340,46 @@ class Context:
         self._check(self.lib.cim_icosavg(self.h, _fptr(v), v.size, int(nn), float(r0), int(variant), nops, optr),
                     "cim_icosavg")
         return v
+
+    # -- local_box ----------------------------------------------------------
+    def local_box_coords(self, parts, ops, x0, box: int, apix: float) -> np.ndarray:
+        """cim_local_box_coords: (n, nops, LBOX_REC) records."""
+        parts = _f32(parts).reshape(-1, PART_STRIDE)
+        o = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 9)
+        x = _f32(x0).reshape(3)
+        rec = np.zeros((parts.shape[0], o.shape[0], LBOX_REC), np.float32)
+        rc = self.lib.cim_local_box_coords(self.h, parts.shape[0], _fptr(parts), o.shape[0],
+                                           o.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _fptr(x), int(box),
+                                           float(apix), _fptr(rec))
+        self._check(rc, "cim_local_box_coords")
+        return rec
+
+    def local_box_stack(self, images, parts, ops, x0, sub: int, apix: float, boxes=None, box: Optional[int] = None):
+        """cim_local_box_stack (host images) or cim_local_box_stack_dev (device
+        arrays with data_ptr(): images of n*box^2 floats, boxes of
+        n*nops*sub^2 floats).  box defaults to the images' last dimension.
+        Returns (boxes (n, nops, sub, sub), records (n, nops, LBOX_REC))."""
+        parts = _f32(parts).reshape(-1, PART_STRIDE)
+        n = parts.shape[0]
+        o = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 9)
+        optr = o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        x = _f32(x0).reshape(3)
+        rec = np.zeros((n, o.shape[0], LBOX_REC), np.float32)
+        box = int(images.shape[-1] if box is None else box)
+        if hasattr(images, "data_ptr"):
+            if boxes is None or boxes.numel() != n * o.shape[0] * sub * sub:
+                raise CimError("device images need a device boxes tensor of n*nops*sub^2 floats")
+            rc = self.lib.cim_local_box_stack_dev(self.h, n, ctypes.c_void_p(images.data_ptr()), _fptr(parts), box,
+                                                  o.shape[0], optr, _fptr(x), int(sub), float(apix),
+                                                  ctypes.c_void_p(boxes.data_ptr()), _fptr(rec))
+            self._check(rc, "cim_local_box_stack_dev")
+            return boxes, rec
+        im = _f32(images).reshape(n, box, box)
+        out = np.zeros((n, o.shape[0], sub, sub), np.float32)
+        rc = self.lib.cim_local_box_stack(self.h, n, _fptr(im), _fptr(parts), box, o.shape[0], optr, _fptr(x),
+                                          int(sub), float(apix), _fptr(out), _fptr(rec))
+        self._check(rc, "cim_local_box_stack")
+        return out, rec
 
     # -- search -------------------------------------------------------------
     def search_templates(self, params: CimParams, images, tort):

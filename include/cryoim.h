@@ -35,6 +35,8 @@
  *                      merge_icosprocess_subrec/SRC/img3d_icosavg.f90
  *   local_reconstruct  local_reconstruct/SRC/reconstruct.f90,
  *                      reconstruct_substep.f90, C_matrix.f90, icos_matrix_i*.f90
+ *   local_box ........ local_box/SRC/loc_box_coordinate.f90, loc_box.f90,
+ *                      equalort.f90, img2d_centshift.f90
  */
 #ifndef CRYOIM_H
 #define CRYOIM_H
@@ -312,6 +314,41 @@ int cim_center(cim_ctx* ctx, const cim_params* p, int method, int n, const float
 /* Same with the images already in device memory. */
 int cim_center_dev(cim_ctx* ctx, const cim_params* p, int method, int n, const void* images_dev, float* parts,
                    float* cc);
+
+/* ---- local_box: sub-particle coordinates and boxes per operator ---------- */
+/* Symmetry operators of local_box (local_box/SRC/0main.f90:80-122):
+ * "i2"/"I2", "i3"/"I3", "i5"/"I5" as cim_local_reconstruct_sym_ops, and
+ * "c1".."c15"/"C1".."C15" by C_matrix.f90; any other name keeps local_box's
+ * default Nsym = 60 and gives C_matrix with 60 operators (float32).  ops may
+ * be NULL to query nops.  Never fails. */
+int cim_local_box_sym_ops(const char* sym, double* ops, int* nops);
+/* One record of CIM_LBOX_REC floats per (particle i, operator k), at
+ * rec[(i*nops + k)*CIM_LBOX_REC]:
+ *   0..2  theta, phi, omega of S_k R(particle) (equalort.f90:10-21)
+ *   3, 4  hx0, ky0 = int(c1), int(c2) (truncation toward zero)
+ *   5, 6  c1 - hx0, c2 - ky0
+ *   7     coordinates: (c3 - box/2) * apix; stack: c3 * apix
+ * with c = R_k^T x0 (R_k the matrix of the local angles, x0 = local_x,
+ * local_y, local_z), plus box/2 + (x, y, 0) in coordinate mode. */
+#define CIM_LBOX_REC 8
+/* loc_box_coordinate.f90:16-41: the records of n particles (parts: records
+ * of CIM_PART_STRIDE; theta, phi, omega, x, y read) for nops operators (ops
+ * row-major float64, values of float32 operators).  box = FFTsize of the
+ * particle images, apix in A/pixel. */
+int cim_local_box_coords(cim_ctx* ctx, int n, const float* parts, int nops, const double* ops, const float* x0,
+                         int box, float apix, float* rec);
+/* loc_box.f90:50-91: n images of box^2 (the particle images); each is
+ * shifted by -(x, y) through its transform (img2d_centshift.f90), then for
+ * each operator k the sub x sub box with corner (hx0, ky0) + box/2 - sub/2
+ * (periodic) is cut and normalised to mean 0, rms 1 into
+ * boxes[(i*nops + k)*sub^2].  rec (may be NULL): the stack-mode records.
+ * Requires sub <= box and |x0| <= box - 2 (so every index stays within one
+ * periodic wrap).  box with prime factors 2, 3, 5 only, <= 1024. */
+int cim_local_box_stack(cim_ctx* ctx, int n, const float* images, const float* parts, int box, int nops,
+                        const double* ops, const float* x0, int sub, float apix, float* boxes, float* rec);
+/* Same with images and boxes in device memory. */
+int cim_local_box_stack_dev(cim_ctx* ctx, int n, const void* images_dev, const float* parts, int box, int nops,
+                            const double* ops, const float* x0, int sub, float apix, void* boxes_dev, float* rec);
 
 #ifdef __cplusplus
 }

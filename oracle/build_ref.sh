@@ -76,6 +76,16 @@ LREC_SRCS+=("$OUT/lrec/0input_parameters_nolocalsym.f90")
 build_objs "$OUT/lrec" "${LREC_SRCS[@]}"
 $FC $FFLAGS -o "$OUT/local_reconstruct" "$OUT"/lrec/*.o $LIBS
 
+# local_box (sub-particle coordinates / boxes around a vertex, per operator)
+B=$REF/local_box/SRC
+mkdir -p "$OUT/lbox"
+LBOX_SRCS=("$B/0common_icos_loc_box.f90" "$B/FFTW.f90")
+for f in "$B"/*.f90; do
+  case "$(basename "$f")" in 0common_icos_loc_box.f90|FFTW.f90) ;; *) LBOX_SRCS+=("$f");; esac
+done
+build_objs "$OUT/lbox" "${LBOX_SRCS[@]}"
+$FC $FFLAGS -o "$OUT/local_box" "$OUT"/lbox/*.o $LIBS
+
 # merge_icosprocess_subrec (tnf merge, plain and FSC-weighted)
 M=$REF/merge_icosprocess_subrec/SRC
 mkdir -p "$OUT/merge"

@@ -1652,3 +1652,112 @@ EXPORT int o_local_reconstruct_ops(int kind, int N, real* r)
     o_c_matrix(N, r);
     return N;
 }
+
+/* ------------------------------------------------------------------ */
+/* local_box                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Eularmatrix.f90: euler_mat's ZYZ matrix with deg2rad = 4 atan2(1,1)/180. */
+static void o_eularmatrix(real th0, real ph0, real om0, real* r)
+{
+    real d2r = (real)4.0 * (real)atan2((real)1.0, (real)1.0) / (real)180.0;
+    real th = th0 * d2r, ph = ph0 * d2r, om = om0 * d2r;
+    real cp = (real)cos(ph), ct = (real)cos(th), co = (real)cos(om);
+    real sp = (real)sin(ph), st = (real)sin(th), so = (real)sin(om);
+    r[0] = cp * ct * co - sp * so; r[1] = -cp * ct * so - sp * co; r[2] = cp * st;
+    r[3] = sp * ct * co + cp * so; r[4] = -sp * ct * so + cp * co; r[5] = sp * st;
+    r[6] = -st * co;               r[7] = st * so;                 r[8] = ct;
+}
+
+/* equalort.f90: the orientation of S_i R(theta, phi, omega). */
+static void o_equalort_one(real th, real ph, real om, const real* S, real* ang)
+{
+    real rad2deg = (real)45.0 / (real)atan2((real)1.0, (real)1.0);
+    real E[9], r[9];
+    o_eularmatrix(th, ph, om, E);
+    matmul3(S, E, r);
+    if (r[8] >= (real)1.0) ang[0] = 0;
+    else if (r[8] < (real)-1.0) ang[0] = (real)180.0;
+    else ang[0] = (real)acos(r[8]) * rad2deg;
+    ang[1] = (real)atan2(r[5], r[2]) * rad2deg;
+    ang[2] = (real)atan2(r[7], -r[6]) * rad2deg;
+}
+
+/* loc_box_coordinate.f90:16-41 for one particle: per operator the local
+ * orientation, the sub-particle centre R^T x0 + nx/2 + (x, y) split into
+ * integer (hx0, ky0) and fraction (local x, y), and the height
+ * (z - nx/2) apix.  out: nops x {lth, lph, lom, hx0, ky0, lx, ly, lz}. */
+EXPORT void o_local_box_coords(real th, real ph, real om, real x, real y, int nops, const real* ops, const real* x0,
+                               int nx, real apix, real* out)
+{
+    for (int i = 0; i < nops; ++i) {
+        real ang[3], R[9], c[3];
+        o_equalort_one(th, ph, om, ops + 9 * i, ang);
+        o_eularmatrix(ang[0], ang[1], ang[2], R);
+        mattvec(R, x0, c);
+        for (int k = 0; k < 3; ++k) c[k] = c[k] + (real)(nx / 2);
+        c[0] = c[0] + x;
+        c[1] = c[1] + y;
+        int hx0 = (int)c[0], ky0 = (int)c[1];
+        real* o = out + 8 * i;
+        o[0] = ang[0]; o[1] = ang[1]; o[2] = ang[2];
+        o[3] = (real)hx0; o[4] = (real)ky0;
+        o[5] = c[0] - (real)hx0; o[6] = c[1] - (real)ky0;
+        o[7] = (c[2] - (real)(nx / 2)) * apix;
+    }
+}
+
+/* img2d_centshift.f90 (local_box): checkerboard, forward FFT (b = +Im), the
+ * phase ramp of (x0, y0) without the nx/2 recentring, backward FFT, real
+ * part, checkerboard -- a circular sub-pixel shift of the image. */
+EXPORT void o_img2d_realshift(real* a, int nx, real x0, real y0)
+{
+    long N = (long)nx * nx;
+    real* b = (real*)calloc(N, sizeof(real));
+    real pi = r_pi();
+    o_checker2d(a, nx);
+    o_img2d_fft(a, b, nx, 1, 1);
+    for (long i = 0; i < N; ++i) b[i] = -b[i];
+    for (long i = 0; i < N; ++i) {
+        int ky = (int)(i / nx) - nx / 2, hx = (int)(i % nx) - nx / 2;
+        real phase = (real)2.0 * pi * (x0 * (real)hx + y0 * (real)ky) / (real)nx;
+        real fr = a[i] * (real)cos(phase) + b[i] * (real)sin(phase);
+        real fi = -a[i] * (real)sin(phase) + b[i] * (real)cos(phase);
+        a[i] = fr;
+        b[i] = fi;
+    }
+    o_img2d_fft(a, b, nx, -1, 1);
+    o_checker2d(a, nx);
+    free(b);
+}
+
+/* loc_box.f90:50-91 for one particle: the image shifted by -(x, y) and
+ * normalised; per operator the orientation equalort gives, c = R^T x0 with
+ * R that orientation's matrix, the sub x sub box whose corner is
+ * int(c) + nx/2 - sub/2 (int truncates toward zero; one periodic wrap per
+ * axis, as img2d_local_box), normalised.  boxes: nops x sub^2;
+ * meta: nops x {lth, lph, lom, c1 - int(c1), c2 - int(c2), c3 apix}. */
+EXPORT void o_local_box_stack(const real* img, int nx, real th, real ph, real om, real x, real y, int nops,
+                              const real* ops, const real* x0, int sub, real apix, real* boxes, real* meta)
+{
+    long N = (long)nx * nx;
+    real* a = (real*)malloc(sizeof(real) * N);
+    for (long i = 0; i < N; ++i) a[i] = img[i];
+    o_img2d_realshift(a, nx, -x, -y);
+    o_img2d_norm(a, nx);
+    for (int i = 0; i < nops; ++i) {
+        real ang[3], R[9], c[3];
+        o_equalort_one(th, ph, om, ops + 9 * i, ang);
+        o_eularmatrix(ang[0], ang[1], ang[2], R);
+        mattvec(R, x0, c);
+        int hx0 = (int)c[0], ky0 = (int)c[1];
+        real* m = meta + 6 * i;
+        m[0] = ang[0]; m[1] = ang[1]; m[2] = ang[2];
+        m[3] = c[0] - (real)hx0; m[4] = c[1] - (real)ky0;
+        m[5] = c[2] * apix;
+        real* b = boxes + (long)sub * sub * i;
+        o_local_box(a, nx, hx0 + nx / 2, ky0 + nx / 2, b, sub);
+        o_img2d_norm(b, sub);
+    }
+    free(a);
+}

@@ -447,3 +447,48 @@ class Oracle:
         r = np.zeros(9 * 60, self.dt)
         m = self.L.o_local_reconstruct_ops(kind, n, self._p(r))
         return r[:9 * m].reshape(m, 3, 3)
+
+    # ---- local_box -------------------------------------------------------------------
+    @staticmethod
+    def local_box_sym(sym):
+        """(kind, N) of local_box/SRC/0main.f90:80-122: I2 / I3 / I5 (60
+        operators) or C_matrix with N from c1..c15 (either case); a sym it
+        does not know keeps the default Nsym = 60 and goes to C_matrix."""
+        s = sym.strip()
+        for k in (2, 3, 5):
+            if s in (f"i{k}", f"I{k}"):
+                return k, 60
+        for n in range(1, 16):
+            if s in (f"c{n}", f"C{n}"):
+                return 0, n
+        return 0, 60
+
+    def local_box_ops(self, sym):
+        kind, n = self.local_box_sym(sym)
+        r = np.zeros(9 * 60, self.dt)
+        m = self.L.o_local_reconstruct_ops(kind, n, self._p(r))
+        return r[:9 * m].reshape(m, 3, 3)
+
+    def img2d_realshift(self, img, x0, y0):
+        a = self._a(img).copy()
+        self.L.o_img2d_realshift(self._p(a), a.shape[-1], self._r(x0), self._r(y0))
+        return a
+
+    def local_box_coords(self, ang, xy, ops, x0, nx, apix):
+        """Per operator {lth, lph, lom, hx0, ky0, frac x, frac y, z apix}."""
+        ops = self._a(ops).reshape(-1, 9)
+        out = np.zeros((len(ops), 8), self.dt)
+        self.L.o_local_box_coords(*[self._r(v) for v in (*ang, *xy)], len(ops), self._p(ops),
+                                  self._p(self._a(x0)), int(nx), self._r(apix), self._p(out))
+        return out
+
+    def local_box_stack(self, img, ang, xy, ops, x0, sub, apix):
+        """(boxes [nops, sub, sub], meta [nops, {lth, lph, lom, frac x, frac y, z apix}])."""
+        a = self._a(img)
+        ops = self._a(ops).reshape(-1, 9)
+        boxes = np.zeros((len(ops), sub, sub), self.dt)
+        meta = np.zeros((len(ops), 6), self.dt)
+        self.L.o_local_box_stack(self._p(a), a.shape[-1], *[self._r(v) for v in (*ang, *xy)], len(ops),
+                                 self._p(ops), self._p(self._a(x0)), int(sub), self._r(apix), self._p(boxes),
+                                 self._p(meta))
+        return boxes, meta

@@ -31,6 +31,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(ROOT, "oracle", "_ref")
+REFERENCE_ROOT = os.environ.get("CIM_REFERENCE", "/root/reference")   # read for data fixtures only
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
@@ -286,6 +287,67 @@ def local_reconstruct_vectors() -> dict:
         g["lrecsub_fa"], g["lrecsub_fb"], g["lrecsub_wg"] = t["fa"], t["fb"], t["wg"]
         g["lrecsub_hdr"] = np.array([t["fft_size"], t["max_r"], t["fsc"], t["realspaceavg"]], i4)
         g["lrecsub_hdrf"] = np.array([t["apix"], t["imgmask"]], f4)
+    return g
+
+
+def _read_sym_matrix_file(path) -> np.ndarray:
+    """The numbers of a local_box symmetry-matrix fixture, 3 per line, read as
+    read_sym_matrixa does: column by column (read_sym_matrix.f90:8-12)."""
+    vals = [float(v) for v in open(path).read().split()]
+    m = np.array(vals, np.float64).reshape(-1, 3, 3)        # [op, column, row]
+    return np.transpose(m, (0, 2, 1)).astype(f4)            # [op, row, column]
+
+
+def _parse_local_ort(path, long):
+    """local_box's ortfiles: loc_box_coordinate.f90:45 (long) or loc_box.f90:91."""
+    rows = []
+    for line in open(path):
+        if len(line.rstrip("\n")) < 70:
+            continue
+        rows.append([float(v) for v in line[64:].split()])
+    return np.array(rows, np.float64)
+
+
+def local_box_vectors() -> dict:
+    """The real local_box on the program_vectors particles: local ortfiles
+    (coordinate mode) for sym=i5 and for the 12-vertex matrix fixture, and the
+    sub-particle stack (stack mode) for the fixture."""
+    g = {}
+    box, apix, n = 48, 2.0, 30
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    pr = np.zeros(n, f4)
+    pr[[3, 11, 17]] = 95.0                          # dropped by PR_threshold = 90
+    g.update(lbox_images=ps.images.astype(f4), lbox_angles=np.round(ps.angles, 4).astype(f4),
+             lbox_shifts=np.round(ps.shifts, 4).astype(f4), lbox_df=np.round(ps.df, 4).astype(f4), lbox_pr=pr)
+    fixture = os.path.join(REFERENCE_ROOT, "local_box", "SRC", "icos5f_12vertex_matrix.txt")
+    g["lbox_v12_ops"] = _read_sym_matrix_file(fixture)
+    exe = os.path.join(REF, "local_box")
+    with tempfile.TemporaryDirectory() as d:
+        formats.write_mrc(os.path.join(d, "stk.mrc"), ps.images)
+        parts = [formats.Particle("stk.mrc", i + 1, *g["lbox_angles"][i], *g["lbox_shifts"][i], float(pr[i]), i + 1,
+                                  i + 1, float(g["lbox_df"][i, 0]), float(g["lbox_df"][i, 1]), 0.0)
+                 for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "icos.txt"), parts)
+        with open(os.path.join(d, "v12.txt"), "w") as f:
+            f.write(open(fixture).read())
+
+        def run(args):
+            r = subprocess.run([exe, "icos_ort=icos.txt", f"apix={apix}", *args], cwd=d, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+
+        run(["local_ort=c_i5.txt", "sym=i5", "local_x=0", "local_y=0", "local_z=18"])
+        g["lbox_coords_i5"] = _parse_local_ort(os.path.join(d, "c_i5.txt"), True)
+        run(["local_ort=c_v12.txt", "sym_matrix=v12.txt", "local_x=1.5", "local_y=-2", "local_z=17.5"])
+        g["lbox_coords_v12"] = _parse_local_ort(os.path.join(d, "c_v12.txt"), True)
+        run(["local_ort=s_v12.txt", "local_imgstck=s.mrc", "sub_FFTsize=24", "sym_matrix=v12.txt", "local_x=0",
+             "local_y=0", "local_z=14"])
+        g["lbox_stack_ort"] = _parse_local_ort(os.path.join(d, "s_v12.txt"), False)
+        raw = open(os.path.join(d, "s.mrc"), "rb").read()
+        g["lbox_stack_hdr"] = np.frombuffer(raw[:1024], dtype="<i4").copy()
+        g["lbox_stack"] = np.frombuffer(raw[1024:], dtype="<f4").reshape(-1, 24, 24).copy()
     return g
 
 
@@ -620,7 +682,8 @@ def main():
             ("lref_program", lref_program_vectors), ("fsc", fsc_program_vectors),
             ("local", local_program_vectors), ("center", center_program_vectors),
             ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors),
-            ("symc", symc_program_vectors), ("lrec", local_reconstruct_vectors)]
+            ("symc", symc_program_vectors), ("lrec", local_reconstruct_vectors),
+            ("lbox", local_box_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

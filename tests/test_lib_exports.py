@@ -62,3 +62,17 @@ def test_local_reconstruct_operators_without_gpu(O32):
         assert np.abs(got - ref).max() < 1e-7, sym
     with pytest.raises(lib.CimError):
         lib.local_reconstruct_sym_ops("c16")
+
+
+def test_local_box_operators_without_gpu(O32):
+    """cim_local_box_sym_ops (host-only): local_box's spellings, including the
+    default of 60 C_matrix operators for a name it does not know."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    import numpy as np
+    for sym in ("i2", "I3", "i5", "c1", "C5", "c10", "C12", "c15", "C_12", "d7"):
+        got = lib.local_box_sym_ops(sym)
+        ref = O32.local_box_ops(sym)
+        assert got.shape == ref.shape, sym
+        assert np.abs(got - ref).max() < 1e-7, sym
+    assert lib.local_box_sym_ops("C_12").shape == (60, 3, 3)
