@@ -23,6 +23,8 @@ FSC between the two maps.
 `--stage center` measures only the centre finding of icosprocess mode 3
 (calc_center against the phantom map, and calc_center_selfrot180) on 224
 of those particles; the default run reports its rates under extra.
+`--stage lbox` measures local_box's stack mode on 256 of them: per particle
+the 96-pixel boxes around its 12 five-fold vertices (local_z 140 px).
 
 Inputs are resident in HBM before the timed region.  The roofline block
 describes the kernel that takes most of a chosen step; the cpu_baseline
@@ -71,8 +73,9 @@ def _parse():
     ap.add_argument("--templates", type=int, default=4, help="template bank size T")
     ap.add_argument("--topk", type=int, default=32, help="AU points re-scored exactly per particle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--stage", choices=("main", "center"), default="main",
-                    help="center: measure only the centre finding of icosprocess mode 3 at box 384")
+    ap.add_argument("--stage", choices=("main", "center", "lbox"), default="main",
+                    help="center: only the centre finding of icosprocess mode 3 at box 384; "
+                         "lbox: only local_box's stack mode (12 vertex boxes of 96 pixels per particle)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -275,7 +278,7 @@ def center_cpu_baseline(ph, ps):
             procs = [subprocess.Popen([exe, "mode=3", "model3d=model.mrc", f"newortfile=c{i}.txt",
                                        f"first={i * K + 1}", f"last={i * K + k}", f"apix={APIX}",
                                        f"maxRes={MAXRES}", f"imgmask={d['mask'] * APIX}", "ortfile=ort.txt"],
-                                      cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                                      cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=_ref_env())
                      for i in range(ncores)]
             rcs = [pr.wait() for pr in procs]
             t[k] = time.perf_counter() - t0
@@ -311,6 +314,146 @@ def center_main(args):
             "cpu_baseline": cpu,
             "extra": r}
     print(json.dumps(line))
+
+
+LBOX_BATCH, LBOX_SUB, LBOX_Z = 256, 96, 140.0
+LBOX_METRIC = "particles/s local_box stack mode, box 384: 12 vertex sub-particles of 96^2 per particle"
+
+
+def vertex_ops():
+    """The 12 operators of local_box's I5 set (5-fold along z) that take the
+    z axis to the 12 distinct five-fold vertices: x0 = (0, 0, z) gives one
+    sub-particle box per vertex, as the 12-vertex matrix files do."""
+    from cryoim_amd import lib
+    ops = lib.local_box_sym_ops("i5")
+    keep, seen = [], []
+    for S in ops:
+        v = S[2]                       # S^T z
+        if not any(np.abs(v - u).max() < 1e-3 for u in seen):
+            seen.append(v)
+            keep.append(S)
+    assert len(keep) == 12
+    return np.array(keep)
+
+
+def lbox_run(ps, steps: int, warmup: int):
+    """local_box stack mode on LBOX_BATCH particles resident in HBM."""
+    import torch
+    from cryoim_amd import lib
+    n = LBOX_BATCH
+    idx = np.arange(n) % len(ps.images)
+    ops = vertex_ops()
+    ctx = lib.Context(0)
+    images = torch.from_numpy(np.ascontiguousarray(ps.images[idx])).to("cuda:0")
+    boxes = torch.empty(n * len(ops) * LBOX_SUB * LBOX_SUB, dtype=torch.float32, device="cuda:0")
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 0:3] = ps.angles[idx]
+    P[:, 3:5] = ps.shifts[idx]
+    x0 = (0.0, 0.0, LBOX_Z)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    for _ in range(warmup):
+        ctx.local_box_stack(images, P, ops, x0, LBOX_SUB, APIX, boxes=boxes, box=BOX)
+    ctx.sync()
+    ctx.timing(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        _, rec = ctx.local_box_stack(images, P, ops, x0, LBOX_SUB, APIX, boxes=boxes, box=BOX)
+    e1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    kt = {k: ctx.timing_get(k) for k in ("lbox_geom", "lbox_rows", "lbox_cols", "lbox_cut")}
+    ctx.timing(False)
+    b = boxes.view(n * len(ops), -1)
+    mean_err = float(b.mean(dim=1).abs().max())
+    rms_err = float((b.double().pow(2).mean(dim=1).sqrt() - 1).abs().max())
+    ctx.close()
+    return {"particles_per_s": round(n / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
+            "sub_particles_per_s": round(n * len(ops) / (ms / 1e3), 1),
+            "kernels": {k: {"avg_ms": round(v[0], 4), "launches": v[1]} for k, v in kt.items() if v[1]},
+            "box_mean_max_abs": mean_err, "box_rms_max_dev": rms_err}
+
+
+def lbox_cpu_baseline(ps):
+    """Reference local_box stack mode on the host cores: the cost per
+    particle is the time of K particles minus that of 1, per process (the
+    difference removes each process's stack read and start-up)."""
+    from cryoim_amd import formats
+    exe = os.path.join(ROOT, "oracle", "_ref", "local_box")
+    if not os.path.exists(exe):
+        return None
+    ncores = min(8, os.cpu_count() or 1)
+    m = len(ps.images)
+    K = 121
+    ops = vertex_ops().astype(np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        formats.write_mrc(os.path.join(tmp, "stk.mrc"), ps.images)
+        parts = [formats.Particle("stk.mrc", i + 1, *ps.angles[i % m], *ps.shifts[i % m], 0.0, i + 1, (i % m) + 1,
+                                  ps.df[i % m, 0], ps.df[i % m, 1], 0.0) for i in range(K * ncores)]
+        formats.write_ortfile(os.path.join(tmp, "ort.txt"), parts)
+        with open(os.path.join(tmp, "v12.txt"), "w") as f:
+            for S in ops:
+                for col in S.T:
+                    f.write(" ".join(f"{float(v):.9g}" for v in col) + "\n")
+        t = {}
+        for k in (1, K):
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([exe, "icos_ort=ort.txt", f"local_ort=l{i}.txt", f"local_imgstck=s{i}.mrc",
+                                       f"sub_FFTsize={LBOX_SUB}", "sym_matrix=v12.txt", "local_x=0", "local_y=0",
+                                       f"local_z={LBOX_Z}", f"apix={APIX}", f"first={i * K + 1}",
+                                       f"last={i * K + k}"], cwd=tmp, stdout=subprocess.DEVNULL,
+                                      stderr=subprocess.DEVNULL, env=_ref_env())
+                     for i in range(ncores)]
+            rcs = [pr.wait() for pr in procs]
+            t[k] = time.perf_counter() - t0
+            if any(rcs):
+                return None
+    per = (t[K] - t[1]) / (K - 1)
+    return {"value": round(ncores / per, 3), "unit": "particles/s", "cores": ncores, "kind": "reference",
+            "sample": (f"local_box stack mode (flang -O2, MKL FFTW) as {ncores} concurrent processes, {K} "
+                       f"particles each ({t[K]:.1f} s) minus 1 each ({t[1]:.1f} s: stack read, start-up)")}
+
+
+def lbox_main(args):
+    _, _, _, ps = make_workload(args.batch, args.templates)
+    r = lbox_run(ps, args.steps, args.warmup)
+    n, nops = LBOX_BATCH, 12
+    kt = r["kernels"]
+    # algorithmic bytes per launch of each stage: rows read the real images and
+    # write complex rows; columns read those and write the real shifted images;
+    # the cut reads each shifted image once and writes the boxes
+    b2 = n * BOX * BOX
+    per = {"lbox_rows": b2 * (4 + 8), "lbox_cols": b2 * (8 + 4),
+           "lbox_cut": n * nops * LBOX_SUB * LBOX_SUB * 4 + b2 * 4}
+    dom = max((k for k in per if k in kt), key=lambda k: kt[k]["avg_ms"] * kt[k]["launches"])
+    achieved = per[dom] / (kt[dom]["avg_ms"] / 1e3) / 1e9
+    names = {"lbox_rows": "k_lbox_shift<0> (row transform, ramp, inverse)",
+             "lbox_cols": "k_lbox_shift<1> (column transform, ramp, inverse, real part)",
+             "lbox_cut": "k_lbox_cut (cut + normalise)"}
+    cpu = None if args.no_cpu_baseline else lbox_cpu_baseline(ps)
+    line = {"metric": LBOX_METRIC, "value": r["particles_per_s"], "unit": "particles/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (the main workload's particles, tiled to the batch)",
+            "config": {"workload": f"local_box stack mode on {n} particles of box {BOX}: 12 vertex boxes of "
+                                   f"{LBOX_SUB}^2 at local_z {LBOX_Z} px", "box": BOX, "batch": n,
+                       "sub_FFTsize": LBOX_SUB, "nops": nops},
+            "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 3),
+                         "traffic": None, "algorithmic_bytes_per_launch": per[dom],
+                         "avg_launch_ms": kt[dom]["avg_ms"], "launches": kt[dom]["launches"]},
+            "cpu_baseline": cpu,
+            "extra": r}
+    print(json.dumps(line))
+
+
+def _ref_env():
+    """Environment of the reference processes: MKL and OpenMP on one thread
+    each, so that N concurrent processes use exactly N host cores."""
+    env = dict(os.environ)
+    env.update(MKL_NUM_THREADS="1", OMP_NUM_THREADS="1", MKL_DYNAMIC="FALSE")
+    return env
 
 
 def _grid_work(step: float) -> int:
@@ -352,7 +495,7 @@ def _ref_baseline(exe, args, tang, timgs, ps, ncores):
         procs = [subprocess.Popen([exe, "mode=4", "templateort=tort.txt", "templatestck=tmpl.mrc",
                                    f"searchstep={coarse}", f"newortfile=o{i}.txt", f"first={i + 1}",
                                    f"last={i + 1}", *common], cwd=tmp, stdout=subprocess.DEVNULL,
-                                  stderr=subprocess.DEVNULL) for i in range(ncores)]
+                                  stderr=subprocess.DEVNULL, env=_ref_env()) for i in range(ncores)]
         rcs = [p.wait() for p in procs]
         t_search = time.perf_counter() - t0
         if any(rcs):
@@ -368,7 +511,8 @@ def _ref_baseline(exe, args, tang, timgs, ps, ncores):
         procs = [subprocess.Popen([exe, "mode=1", "subrec=y", f"result3d=s{i}.tnf", f"first={i * nins + 1}",
                                    f"last={(i + 1) * nins}", f"apix={APIX}", f"maxRes={maxres_ins}",
                                    f"imgmask={d['mask'] * APIX}", "ortfile=ort2.txt"], cwd=tmp,
-                                  stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL) for i in range(ncores)]
+                                  stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=_ref_env())
+                 for i in range(ncores)]
         rcs = [p.wait() for p in procs]
         t_ins = time.perf_counter() - t0
         if any(rcs):
@@ -448,6 +592,9 @@ def main():
         sys.exit(2)
     if args.stage == "center":
         center_main(args)
+        return
+    if args.stage == "lbox":
+        lbox_main(args)
         return
     t_setup = time.perf_counter()
     ph, tang, timgs, ps = make_workload(args.batch, args.templates)

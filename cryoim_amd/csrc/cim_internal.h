@@ -109,11 +109,11 @@ struct CenterState {
 };
 
 // Device buffers of local_box: angles and shifts, operators, per-(particle,
-// operator) records, complex working images.
+// operator) records, row-shifted complex images, shifted real images.
 struct LboxState {
-    DevBuf pr, ops, rec, x;
+    DevBuf pr, ops, rec, x, r;
     void release() {
-        for (DevBuf* b : {&pr, &ops, &rec, &x}) b->release();
+        for (DevBuf* b : {&pr, &ops, &rec, &x, &r}) b->release();
     }
 };
 

@@ -12,16 +12,19 @@
 //
 // One thread per (particle, operator) evaluates the geometry in float32 in
 // the reference's order of operations (no contraction).  The stack path
-// runs load+checkerboard, a batched forward 2D FFT, the phase ramp, the
-// backward FFT, then one workgroup per (particle, operator) cuts and
-// normalises its box from the shifted image, which stays in L2 while the 60
-// boxes that overlap it are cut.  The whole-image normalisation of
-// loc_box.f90:52 is an affine map of the image, which the per-box
-// normalisation removes again, so it is not run.
+// shifts each image with two kernels, rows then columns, each running the
+// forward transform, the phase ramp and the backward transform of its axis
+// in LDS (one read and one write of the image per axis), then one
+// workgroup per (particle, operator) cuts and normalises its box from the
+// shifted image, which stays in L2 while the particle's overlapping boxes
+// are cut.  The whole-image normalisation of loc_box.f90:52 is an affine
+// map of the image, which the per-box normalisation removes again, so it
+// is not run.
 #include <algorithm>
 #include <vector>
 
 #include "cim_device.h"
+#include "cim_fft.h"
 
 namespace cim {
 
@@ -95,69 +98,121 @@ __global__ void __launch_bounds__(256) k_lbox_geom(LboxGeomArgs a) {
     o[2] = om;
 }
 
-// img2d_centshift.f90:13-16: checkerboarded image as complex data.
-__global__ void k_lbox_load(const float* __restrict__ img, float2* __restrict__ x, int box, long long total) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int q = (int)(t % ((long long)box * box));
-    const int s = ((q / box) + (q % box)) & 1;
-    const float v = img[t];
-    x[t] = make_float2(s ? -v : v, 0.f);
-}
-
-// img2d_centshift.f90:21-29 on the (Re, +Im) transform: multiply by
-// exp(-i phase), phase = 2 pi (x0 hx + y0 ky) / box with (x0, y0) = -(x, y).
-__global__ void k_lbox_phase(float2* __restrict__ x, int box, long long total, const float* __restrict__ pr,
-                             float twopi) {
+// img2d_centshift.f90:13-34 along one axis for L lines (one workgroup):
+// the checkerboard factor (-1)^e, forward transform (scaled 1/n), the phase
+// ramp exp(-i 2 pi s (k - n/2) / n) with s = -x (rows) or -y (columns) of
+// the line's particle, backward transform, (-1)^e again.  The reference's
+// 2D operator is the product of these row and column operators (the
+// checkerboard, both transforms and the ramp all factor by axis), so rows
+// then columns give its shifted image up to rounding.
+//   COLS = 0: rows of the real images img -> complex rows of z;
+//   COLS = 1: columns of z -> the real part, into out.
+// Line l is row (or column) l % n of particle l / n.
+template <int COLS>
+__global__ void __launch_bounds__(256) k_lbox_shift(const float* __restrict__ img, float2* __restrict__ z,
+                                                    float* __restrict__ out, const float2* __restrict__ tw, int n,
+                                                    FacList fl, long long nlines, int L,
+                                                    const float* __restrict__ pr, float twopi) {
 #pragma clang fp contract(off)
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const long long b2 = (long long)box * box;
-    const int i = (int)(t / b2), q = (int)(t % b2);
-    const int ky = q / box - box / 2, hx = q % box - box / 2;
-    const float x0 = -pr[5 * i + 3], y0 = -pr[5 * i + 4];
-    const float phase = twopi * (x0 * (float)hx + y0 * (float)ky) / (float)box;
-    float s, c;
-    sincosf(phase, &s, &c);
-    const float2 v = x[t];
-    x[t] = make_float2(v.x * c + v.y * s, -v.x * s + v.y * c);
+    extern __shared__ float2 lds[];
+    __shared__ long long lbase[16];   // first element of each line (COLS: column c of particle p)
+    __shared__ float lsh[16];         // -x (rows) or -y (columns) of each line's particle
+    float2* b0 = lds;
+    float2* b1 = lds + (size_t)L * n;
+    const long long l0 = (long long)blockIdx.x * L;
+    const int nl = (int)((nlines - l0) < L ? (nlines - l0) : L);
+    const long long n2 = (long long)n * n;
+    if (threadIdx.x < nl) {
+        const long long l = l0 + threadIdx.x, p = l / n;
+        lbase[threadIdx.x] = COLS ? p * n2 + (l - p * n) : l * n;
+        lsh[threadIdx.x] = -pr[5 * p + (COLS ? 4 : 3)];
+    }
+    __syncthreads();
+    const int lgL = __ffs(L) - 1;
+    if (!COLS) {
+        for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+            const int li = (int)fdiv((unsigned)idx, (unsigned)n, fl.n_m), e = idx - li * n;
+            const float v = img[lbase[li] + e];
+            b0[idx] = make_float2((e & 1) ? -v : v, 0.f);
+        }
+    } else {
+        // L neighbouring columns: each row of the tile is one coalesced segment
+        for (int idx = threadIdx.x; idx < L * n; idx += blockDim.x) {
+            const int e = idx >> lgL, li = idx & (L - 1);
+            if (li >= nl) continue;
+            const float2 v = z[lbase[li] + (long long)e * n];
+            b0[li * n + e] = (e & 1) ? make_float2(-v.x, -v.y) : v;
+        }
+    }
+    for (int idx = nl * n + threadIdx.x; idx < L * n; idx += blockDim.x) b0[idx] = make_float2(0.f, 0.f);
+    __syncthreads();
+    float2* f = lds_fft(b0, b1, n, L, fl, tw, -1.f);
+    const float inv = 1.0f / (float)n;
+    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+        const int li = (int)fdiv((unsigned)idx, (unsigned)n, fl.n_m), k = idx - li * n;
+        const float phase = twopi * (lsh[li] * (float)(k - n / 2)) / (float)n;
+        float sn, cs;
+        sincosf(phase, &sn, &cs);
+        const float2 v = make_float2(f[idx].x * inv, f[idx].y * inv);
+        f[idx] = make_float2(v.x * cs + v.y * sn, -v.x * sn + v.y * cs);
+    }
+    __syncthreads();
+    const float2* r = lds_fft(f, f == b0 ? b1 : b0, n, L, fl, tw, 1.f);
+    if (!COLS) {
+        for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+            const int li = (int)fdiv((unsigned)idx, (unsigned)n, fl.n_m), e = idx - li * n;
+            const float2 v = r[idx];
+            z[lbase[li] + e] = (e & 1) ? make_float2(-v.x, -v.y) : v;
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < L * n; idx += blockDim.x) {
+            const int e = idx >> lgL, li = idx & (L - 1);
+            if (li >= nl) continue;
+            const float v = r[li * n + e].x;
+            out[lbase[li] + (long long)e * n] = (e & 1) ? -v : v;
+        }
+    }
 }
 
 // loc_box.f90:64-86 for one (particle, operator): the sub x sub box of the
-// real, checkerboard-corrected shifted image whose corner is
-// (hx0, ky0) + box/2 - sub/2 (one periodic wrap per axis, then a modulo that
-// only matters for corners the host has already refused), normalised.
-__global__ void __launch_bounds__(256) k_lbox_cut(const float2* __restrict__ x, const float* __restrict__ rec,
+// shifted image whose corner is (hx0, ky0) + box/2 - sub/2 (one periodic
+// wrap per axis, then a modulo that only matters for corners the host has
+// already refused), normalised: mean, then the rms about it, then the
+// normalised values (three passes; the box stays in L2).
+__global__ void __launch_bounds__(256) k_lbox_cut(const float* __restrict__ x, const float* __restrict__ rec,
                                                   int box, int sub, int nops, float* __restrict__ out) {
     __shared__ double sh[4];
     const int t = blockIdx.x;
     const int i = t / nops;
     const float* o = rec + (long long)CIM_LBOX_REC * t;
     const int hx0 = (int)o[3] + box / 2 - sub / 2, ky0 = (int)o[4] + box / 2 - sub / 2;
-    const float2* img = x + (long long)i * box * box;
+    const float* img = x + (long long)i * box * box;
     float* dst = out + (long long)t * sub * sub;
-    const int m = sub * sub;
-    auto value = [&](int q) -> float {
-        int ky = q / sub + ky0, hx = q % sub + hx0;
-        if (hx < 0) hx += box;
-        if (hx >= box) hx -= box;
-        if (ky < 0) ky += box;
-        if (ky >= box) ky -= box;
-        hx = ((hx % box) + box) % box;
-        ky = ((ky % box) + box) % box;
-        const float v = img[(long long)ky * box + hx].x;
-        return ((hx + ky) & 1) ? -v : v;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6, ny = blockDim.x >> 6;
+    auto wrap = [&](int v) {
+        if (v < 0) v += box;
+        if (v >= box) v -= box;
+        return ((v % box) + box) % box;
     };
     float s = 0.f;
-    for (int q = threadIdx.x; q < m; q += blockDim.x) s += value(q);
-    const float avg = (float)(block_sum((double)s, sh) / (double)m);
-    float d = 0.f;
-    for (int q = threadIdx.x; q < m; q += blockDim.x) {
-        const float v = value(q) - avg;
-        d += v * v;
+    for (int r = ty; r < sub; r += ny) {
+        const float* row = img + (long long)wrap(ky0 + r) * box;
+        for (int c = tx; c < sub; c += 64) s += row[wrap(hx0 + c)];
     }
-    const float dev = (float)sqrt(block_sum((double)d, sh) / (double)m);
-    for (int q = threadIdx.x; q < m; q += blockDim.x) dst[q] = (value(q) - avg) / dev;
+    const float avg = (float)(block_sum((double)s, sh) / ((double)sub * sub));
+    float d = 0.f;
+    for (int r = ty; r < sub; r += ny) {
+        const float* row = img + (long long)wrap(ky0 + r) * box;
+        for (int c = tx; c < sub; c += 64) {
+            const float v = row[wrap(hx0 + c)] - avg;
+            d += v * v;
+        }
+    }
+    const float dev = (float)sqrt(block_sum((double)d, sh) / ((double)sub * sub));
+    for (int r = ty; r < sub; r += ny) {
+        const float* row = img + (long long)wrap(ky0 + r) * box;
+        for (int c = tx; c < sub; c += 64) dst[(long long)r * sub + c] = (row[wrap(hx0 + c)] - avg) / dev;
+    }
 }
 
 struct LboxSetup {
@@ -241,29 +296,39 @@ int lbox_stack_dev(cim_ctx* ctx, int n, const float* images_dev, const float* pa
     rc = lbox_geom(ctx, st.a);
     if (rc) return rc;
     const long long b2 = (long long)box * box;
-    const int chunk = (int)std::max(1LL, std::min((long long)n, (512LL << 20) / (8 * b2)));
-    if (n > 0) CIM_ALLOC(ctx->lbox.x, sizeof(float2) * b2 * chunk);
-    float2* x = ctx->lbox.x.as<float2>();
+    const int chunk = (int)std::max(1LL, std::min((long long)n, (512LL << 20) / (12 * b2)));
+    if (n > 0) {
+        CIM_ALLOC(ctx->lbox.x, sizeof(float2) * b2 * chunk);
+        CIM_ALLOC(ctx->lbox.r, sizeof(float) * b2 * chunk);
+    }
+    float2* z = ctx->lbox.x.as<float2>();
+    float* rimg = ctx->lbox.r.as<float>();
     const float twopi = 2.0f * (4.0f * atan2f(1.0f, 1.0f));
+    const FacList fl = fac_list(pl);
+    // lines per workgroup as the batched transforms choose them
+    const int Lr = box >= 1024 ? 1 : (box >= 512 ? 2 : (box >= 256 ? 4 : (box >= 128 ? 8 : 16)));
+    const int Lc = box <= 128 ? 16 : (box <= 512 ? 8 : 4);
     for (int i0 = 0; i0 < n; i0 += chunk) {
         const int nb = std::min(chunk, n - i0);
-        const long long total = b2 * nb;
-        hipLaunchKernelGGL(k_lbox_load, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
-                           images_dev + b2 * i0, x, box, total);
-        CIM_HIP(hipGetLastError());
-        rc = fft2d_dev(ctx, x, box, nb, 1, 0);
-        if (rc) return rc;
+        const long long nlines = (long long)nb * box;
+        const float* prc = st.a.pr + 5LL * i0;
         {
-            TimedLaunch tl(ctx, "lbox_phase");
-            hipLaunchKernelGGL(k_lbox_phase, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, x,
-                               box, total, st.a.pr + 5LL * i0, twopi);
+            TimedLaunch tl(ctx, "lbox_rows");
+            hipLaunchKernelGGL(k_lbox_shift<0>, dim3((unsigned)((nlines + Lr - 1) / Lr)), dim3(256),
+                               sizeof(float2) * 2 * Lr * box, ctx->stream, images_dev + b2 * i0, z, nullptr, pl->tw,
+                               box, fl, nlines, Lr, prc, twopi);
             CIM_HIP(hipGetLastError());
         }
-        rc = fft2d_dev(ctx, x, box, nb, -1, 0);
-        if (rc) return rc;
+        {
+            TimedLaunch tl(ctx, "lbox_cols");
+            hipLaunchKernelGGL(k_lbox_shift<1>, dim3((unsigned)((nlines + Lc - 1) / Lc)), dim3(256),
+                               sizeof(float2) * 2 * Lc * box, ctx->stream, nullptr, z, rimg, pl->tw, box, fl, nlines,
+                               Lc, prc, twopi);
+            CIM_HIP(hipGetLastError());
+        }
         {
             TimedLaunch tl(ctx, "lbox_cut");
-            hipLaunchKernelGGL(k_lbox_cut, dim3((unsigned)(nb * nops)), dim3(256), 0, ctx->stream, x,
+            hipLaunchKernelGGL(k_lbox_cut, dim3((unsigned)(nb * nops)), dim3(256), 0, ctx->stream, rimg,
                                st.a.rec + (long long)CIM_LBOX_REC * nops * i0, box, sub, nops,
                                boxes_dev + (long long)sub * sub * nops * i0);
             CIM_HIP(hipGetLastError());
