@@ -332,6 +332,7 @@ struct BrickArgs {
     const float4* vals;       // n x (2maxR+1)(maxR+1); w = 1 where the sample is inserted
     const float* M;           // n x nops x 9
     const int* bricks;        // nb x 3 brick origins (accumulator indices)
+    const int* hxmax;         // 2maxR+1: sample (hx, ky) is inserted iff hx <= hxmax[ky + maxR]
     int n, dz;
     InsertGeom g;
 };
@@ -344,6 +345,32 @@ struct BrickArgs {
 // meets at most ~12 lattice points of a plane, doubled on the hx = 0 column)
 // stays below 2^62.  Integer LDS adds are fast where float LDS atomics are
 // not, and the sums do not depend on the order of the adds.
+// The terms are also kept below 2^51 in magnitude, so that fix64 converts
+// them exactly.
+__device__ __forceinline__ int brick_cap(const BrickArgs& a) {
+    const float mx = fmaxf(__uint_as_float(*a.maxbits), 1e-30f);
+    int e;
+    (void)frexpf(mx, &e);          // mx < 2^e
+    int cb = 0;
+    while ((1ll << cb) < (long long)a.n * a.g.nops * 32) ++cb;
+    return min(min(62 - e - cb, 51 - e), 100);
+}
+
+// x (|x| < 2^51) rounded to the nearest integer, ties to even, as a 64-bit
+// two's complement value: adding 1.5 * 2^52 in double rounds x to an integer
+// in the low mantissa bits (the round-to-nearest of the add), and the
+// difference of the bit patterns is that integer.
+__device__ __forceinline__ unsigned long long fix64(float x) {
+    const double magic = 6755399441055744.0;
+    return (unsigned long long)(__double_as_longlong((double)x + magic) - __double_as_longlong(magic));
+}
+
+constexpr int BRICK_NV = BRICK * BRICK * BRICK;
+
+// One read-modify-write of the brick's exact batch sums into HBM.
+__device__ void brick_flush(const BrickArgs& a, const unsigned long long* sacc, int bx0, int by0, int bz0,
+                            double inv);
+
 __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
     constexpr int NV = BRICK * BRICK * BRICK;
     __shared__ unsigned long long sacc[3 * NV];
@@ -355,15 +382,7 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
     const InsertGeom& g = a.g;
     const int R = g.maxR;
     for (int i = tid; i < 3 * NV; i += blockDim.x) sacc[i] = 0ull;
-    int cap;
-    {
-        const float mx = fmaxf(__uint_as_float(*a.maxbits), 1e-30f);
-        int e;
-        (void)frexpf(mx, &e);          // mx < 2^e
-        int cb = 0;
-        while ((1ll << cb) < (long long)a.n * g.nops * 32) ++cb;
-        cap = min(62 - e - cb, 100);
-    }
+    const int cap = brick_cap(a);
     const float scale = ldexpf(1.f, cap);
     const double inv = ldexp(1.0, -cap);
     // positions (before the index offsets) whose corners can reach the brick:
@@ -471,16 +490,205 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
                         const int cx = ox + (j & 1), cy = oy + ((j >> 1) & 1), cz = oz + (j >> 2);
                         if (cx < 0 || cx >= BRICK || cy < 0 || cy >= BRICK || cz < 0 || cz >= BRICK) continue;
                         const int li3 = (cz * BRICK + cy) * BRICK + cx;
-                        atomicAdd(&sacc[li3], (unsigned long long)__float2ll_rn(va * wt[j] * scale));
-                        atomicAdd(&sacc[NV + li3], (unsigned long long)__float2ll_rn(vb * wt[j] * scale));
-                        atomicAdd(&sacc[2 * NV + li3], (unsigned long long)__float2ll_rn(vw * wt[j] * scale));
+                        atomicAdd(&sacc[li3], fix64(va * wt[j] * scale));
+                        atomicAdd(&sacc[NV + li3], fix64(vb * wt[j] * scale));
+                        atomicAdd(&sacc[2 * NV + li3], fix64(vw * wt[j] * scale));
                     }
                 }
             }
         }
         __syncthreads();
     }
-    // one read-modify-write of the brick in HBM
+    brick_flush(a, sacc, bx0, by0, bz0, inv);
+}
+
+// The same brick gather with the work of a wave packed into full lanes.
+// Per plane, the lattice points of the rows (the same row intervals as
+// k_insert_brick) are numbered through a wave prefix sum, so each lane takes
+// one point; the point's candidate samples are placed and tested, and the
+// ones whose cell touches the brick go to a queue in LDS (its offsets in
+// the brick, fractions, flip and value index).  Whenever the queue holds 64
+// entries, 64 lanes add their 8 corners.  Every contribution is the same
+// fixed-point term as in k_insert_brick and the sums are exact, so both
+// kernels give the same bits.
+constexpr int QCAP = 192;   // < 64 left over + 2 candidates x 64 lanes
+
+__global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
+    constexpr int NV = BRICK_NV;
+    __shared__ unsigned long long sacc[3 * NV];
+    __shared__ int list[1024];
+    __shared__ int nlist;
+    __shared__ int qpos[4][QCAP];       // (ox+1) | (oy+1) << 4 | (oz+1) << 8 | flip << 12
+    __shared__ unsigned qval[4][QCAP];  // index into a.vals
+    __shared__ float qax[4][QCAP], qay[4][QCAP], qaz[4][QCAP];
+    __shared__ int shx[2 * 512 + 1];    // hxmax (maxR <= 512)
+    const int tid = threadIdx.x;
+    const int bi = blockIdx.x;
+    const int bx0 = a.bricks[3 * bi], by0 = a.bricks[3 * bi + 1], bz0 = a.bricks[3 * bi + 2];
+    const InsertGeom& g = a.g;
+    const int R = g.maxR;
+    for (int i = tid; i < 3 * NV; i += blockDim.x) sacc[i] = 0ull;
+    for (int i = tid; i <= 2 * R; i += blockDim.x) shx[i] = a.hxmax[i];
+    const int cap = brick_cap(a);
+    const float scale = ldexpf(1.f, cap);
+    const double inv = ldexp(1.0, -cap);
+    const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - 1)};
+    const float hi[3] = {(float)(bx0 - R + BRICK), (float)(by0 - R + BRICK), (float)(bz0 + BRICK)};
+    const float c[3] = {0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
+    const float reach = 0.5f * (float)(BRICK + 1) * 1.7320508f + 0.5f;
+    const int w = R + 1;
+    const int ntot = a.n * g.nops;
+    const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    int qn = 0;   // entries in this wave's queue (wave-uniform)
+    // add the corners of the last cnt entries, one entry per lane
+    auto drain = [&](int cnt) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < cnt) {
+            const int e = qn - cnt + lane;
+            const int pk = qpos[wv][e];
+            const int ox = (pk & 15) - 1, oy = ((pk >> 4) & 15) - 1, oz = ((pk >> 8) & 15) - 1;
+            const bool flip = (pk >> 12) & 1;
+            const float4 vv = a.vals[qval[wv][e]];
+            const float ax = qax[wv][e], ay = qay[wv][e], az = qaz[wv][e];
+            const float va = vv.x, vb = flip ? -vv.y : vv.y, vw = vv.z;
+            const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay, w11 = ax * ay;
+            const float wz0 = 1.f - az, wz1 = az;
+            const float wt[8] = {w00 * wz0, w10 * wz0, w01 * wz0, w11 * wz0, w00 * wz1, w10 * wz1, w01 * wz1, w11 * wz1};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int cx = ox + (j & 1), cy = oy + ((j >> 1) & 1), cz = oz + (j >> 2);
+                if (cx < 0 || cx >= BRICK || cy < 0 || cy >= BRICK || cz < 0 || cz >= BRICK) continue;
+                const int li3 = (cz * BRICK + cy) * BRICK + cx;
+                atomicAdd(&sacc[li3], fix64(va * wt[j] * scale));
+                atomicAdd(&sacc[NV + li3], fix64(vb * wt[j] * scale));
+                atomicAdd(&sacc[2 * NV + li3], fix64(vw * wt[j] * scale));
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        qn -= cnt;
+    };
+    for (int q0 = 0; q0 < ntot; q0 += 1024) {
+        if (tid == 0) nlist = 0;
+        __syncthreads();
+        for (int q = q0 + tid; q < min(ntot, q0 + 1024); q += blockDim.x) {
+            const float* m = a.M + (long long)q * 9;
+            const float d = fabsf(m[2] * c[0] + m[5] * c[1] + m[8] * c[2]);
+            if (d <= reach) list[atomicAdd(&nlist, 1)] = q;
+        }
+        __syncthreads();
+        const int nl = nlist;
+        for (int li = wv; li < nl; li += nwv) {
+            const int q = list[li];
+            const int p = q / g.nops;
+            const float* m = a.M + (long long)q * 9;
+            float vmin = 3e38f, vmax = -3e38f;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float X = (k & 1) ? hi[0] : lo[0], Y = (k & 2) ? hi[1] : lo[1], Z = (k & 4) ? hi[2] : lo[2];
+                const float v = m[1] * X + m[4] * Y + m[7] * Z;
+                vmin = fminf(vmin, v);
+                vmax = fmaxf(vmax, v);
+            }
+            // Rows and per-row h intervals of the plane's points in the reach box, with a
+            // margin of 0.01 lattice steps over the float error of the bounds (~1e-4); the
+            // exact test below decides which points count, as in k_insert_brick, whose
+            // wider intervals hold the same points.
+            const int ky0 = max((int)floorf(vmin - 0.01f), -R), ky1 = min((int)ceilf(vmax + 0.01f), R);
+            const int nrows = ky1 - ky0 + 1;
+            if (nrows <= 0) continue;
+            int ha = 0, cnt = 0;
+            if (lane < nrows) {
+                const float kf = (float)(ky0 + lane);
+                const float c0[3] = {m[0], m[3], m[6]}, c1[3] = {m[1], m[4], m[7]};
+                float ta = -3e38f, tb = 3e38f;
+                bool empty = false;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float base = kf * c1[i];
+                    if (fabsf(c0[i]) > 1e-6f) {
+                        const float rc = 1.0f / c0[i];
+                        const float t1 = (lo[i] - base) * rc, t2 = (hi[i] - base) * rc;
+                        ta = fmaxf(ta, fminf(t1, t2));
+                        tb = fminf(tb, fmaxf(t1, t2));
+                    } else if (!(base >= lo[i] - 1e-3f && base <= hi[i] + 1e-3f)) {
+                        empty = true;
+                    }
+                }
+                if (!empty && ta <= tb + 0.02f) {
+                    ha = max((int)ceilf(ta - 0.01f), -R);
+                    const int hb = min((int)floorf(tb + 0.01f), R);
+                    cnt = max(hb - ha + 1, 0);
+                }
+            }
+            // inclusive prefix sum of the row counts over the lanes
+            int pre = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(pre, o, 64);
+                if (lane >= o) pre += u;
+            }
+            const int total = __shfl(pre, 63, 64);
+            const unsigned vbase = (unsigned)p * (unsigned)((2 * R + 1) * w);
+            for (int t0 = 0; t0 < total; t0 += 64) {
+                const int t = t0 + lane;
+                // row of point t: the number of rows whose inclusive prefix is <= t
+                int r = 0;
+#pragma unroll
+                for (int sft = 32; sft > 0; sft >>= 1)
+                    if (__shfl(pre, r + sft - 1, 64) <= t) r += sft;
+                const int rcnt = __shfl(cnt, r, 64), rha = __shfl(ha, r, 64), rpre = __shfl(pre, r, 64);
+                const int hp = rha + (t - (rpre - rcnt)), kp = ky0 + r;
+                // the second candidate, (0, -k), exists on the h = 0 column only
+                const int ncand = __any(t < total && hp == 0) ? 2 : 1;
+                for (int cidx = 0; cidx < ncand; ++cidx) {
+                    bool ok = t < total && (cidx == 0 || hp == 0);
+                    const bool mirror = cidx == 1 || hp < 0;
+                    const int hx = mirror ? -hp : hp, ky = mirror ? -kp : kp;
+                    ok = ok && hx >= 0 && hx <= R && ky >= -R && ky <= R;
+                    const unsigned vi = vbase + (unsigned)((ky + R) * w + hx);
+                    if (ok) ok = hx <= shx[ky + R];   // vals[vi].w != 0
+                    int gx = 0, gy = 0, gz = 0;
+                    float ax = 0.f, ay = 0.f, az = 0.f;
+                    int ox = 0, oy = 0, oz = 0;
+                    bool flip = false;
+                    if (ok) {
+                        place(m, (float)hx, (float)ky, g, gx, gy, gz, ax, ay, az, flip);
+                        ox = gx - bx0;
+                        oy = gy - by0;
+                        oz = gz - bz0;
+                        ok = flip == mirror && ox >= -1 && ox < BRICK && oy >= -1 && oy < BRICK && oz >= -1 &&
+                             oz < BRICK;
+                    }
+                    const unsigned long long mask = __ballot(ok);
+                    if (ok) {
+                        const int e = qn + __popcll(mask & ((1ull << lane) - 1ull));
+                        qpos[wv][e] = (ox + 1) | ((oy + 1) << 4) | ((oz + 1) << 8) | ((int)flip << 12);
+                        qval[wv][e] = vi;
+                        qax[wv][e] = ax;
+                        qay[wv][e] = ay;
+                        qaz[wv][e] = az;
+                    }
+                    qn += __popcll(mask);
+                }
+                while (qn >= 64) drain(64);
+            }
+        }
+        __syncthreads();
+    }
+    if (qn > 0) drain(qn);
+    __syncthreads();
+    brick_flush(a, sacc, bx0, by0, bz0, inv);
+}
+
+__device__ void brick_flush(const BrickArgs& a, const unsigned long long* sacc, int bx0, int by0, int bz0,
+                            double inv) {
+    constexpr int NV = BRICK_NV;
+    const InsertGeom& g = a.g;
+    const int tid = threadIdx.x;
     const long long dxy = g.dxy;
     for (int i = tid; i < NV; i += blockDim.x) {
         const int cx = i % BRICK, cy = (i / BRICK) % BRICK, cz = i / (BRICK * BRICK);
@@ -715,11 +923,16 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
         ba.vals = r.vals.as<float4>();
         ba.M = ctx->aux2.as<float>();
         ba.bricks = r.bricks.as<int>();
+        ba.hxmax = r.hxmax.as<int>();
         ba.n = n;
         ba.dz = r.dz;
         ba.g = g;
+        // the queue kernel indexes the batch's values with 32 bits
+        const bool rows = (mode && strcmp(mode, "brickrows") == 0) || (long long)total * n >= (1ll << 32) ||
+                          r.maxR > 512;
         TimedLaunch tl(ctx, "insert");
-        hipLaunchKernelGGL(k_insert_brick, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
+        if (rows) hipLaunchKernelGGL(k_insert_brick, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
+        else hipLaunchKernelGGL(k_insert_brickq, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
         CIM_HIP(hipGetLastError());
     }
     r.inserted += n;

@@ -483,7 +483,9 @@ __global__ void __launch_bounds__(256) k_cl_table(const float2* __restrict__ ppo
         }
         __syncthreads();
     }
-    unsigned int* out = table + (long long)item * nrow * npsi;
+    // rows of 2 npsi entries, the second half repeating the first, so that the
+    // gather reads the npsi omega columns of a common line without a wrap
+    unsigned int* out = table + (long long)item * nrow * 2 * npsi;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int jt = jT0 + ty * 4 + i;
@@ -491,13 +493,17 @@ __global__ void __launch_bounds__(256) k_cl_table(const float2* __restrict__ ppo
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int jp = jP0 + tx * 4 + j;
-            if (jp < npsi) out[(long long)jt * npsi + jp] = bf16_bits(N[i][j]) | (bf16_bits(D[i][j]) << 16);
+            if (jp < npsi) {
+                const unsigned int e = bf16_bits(N[i][j]) | (bf16_bits(D[i][j]) << 16);
+                out[(long long)jt * 2 * npsi + jp] = e;
+                out[(long long)jt * 2 * npsi + jp + npsi] = e;
+            }
         }
     }
 }
 
 struct GatherArgs {
-    const unsigned int* table;   // n x T x (npsi/2) x npsi, {N, D} as bfloat16
+    const unsigned int* table;   // n x T x (npsi/2) x 2 npsi, {N, D} as bfloat16, rows repeated once
     const float* tang;        // T x 3
     const float* pts;         // npts x 2
     const int* pbase;
@@ -508,6 +514,23 @@ struct GatherArgs {
     float* outv;              // n x npts: best table score of the AU point
     int* outi;                // its linear index
 };
+
+// Add the {N, D} bfloat16 pair of a table entry to the sums: N the low half,
+// D the high half, each widened exactly to float and added with one rounding.
+// kGatherDot2 does both with v_dot2c_f32_bf16 against (1, 0) and (0, 1).
+constexpr bool kGatherDot2 = false;
+typedef short bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void acc_entry(unsigned int e, float& num, float& den) {
+    if (kGatherDot2) {
+        const bf16x2_t v = __builtin_bit_cast(bf16x2_t, e);
+        const bf16x2_t s0 = {(short)0x3f80, 0}, s1 = {0, (short)0x3f80};
+        num = __builtin_amdgcn_fdot2_f32_bf16(v, s0, num, false);
+        den = __builtin_amdgcn_fdot2_f32_bf16(v, s1, den, false);
+    } else {
+        num += __uint_as_float(e << 16);
+        den += __uint_as_float(e & 0xffff0000u);
+    }
+}
 
 __device__ __forceinline__ int wrap_index(int j, int n) {
     j %= n;
@@ -534,12 +557,17 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     const int tid = threadIdx.x, bd = blockDim.x;
     const int nrow = a.npsi >> 1;
     float num[VPT], den[VPT];
-    int kk[VPT];
+    // column of omega k on a line with zero-angle column j0: j0 - k sub (mod npsi), read in
+    // the repeated row as j0 + npsi - maxk + off with off = maxk - k sub in [0, maxk]
+    // (columns past Nomega read column j0 + npsi and are not used)
+    const int maxk = (a.Nomega - 1) * a.sub;
+    unsigned boff[VPT];   // byte offsets
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
         num[v] = 0.f;
         den[v] = 0.f;
-        kk[v] = (tid + v * bd) * a.sub;
+        const int k = tid + v * bd;
+        boff[v] = 4u * (unsigned)(maxk - (k < a.Nomega ? k * a.sub : 0));
     }
     float E1[9];
     euler_f(a.pts[2 * pt], a.pts[2 * pt + 1], 0.f, E1);
@@ -563,50 +591,36 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
             const unsigned long long mask = __ballot(ok);
             if (ok) {
                 const int slot = __popcll(mask & ((1ull << tid) - 1ull));
-                srow[slot] = jt * a.npsi;
-                sj0[slot] = j0;
+                srow[slot] = jt * 2 * a.npsi;
+                sj0[slot] = j0 + a.npsi - maxk;
             }
             if (tid == 0) snv = __popcll(mask);
         }
         __syncthreads();
         const int nv = snv;
-        const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * a.npsi;
+        const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * 2 * a.npsi;
         constexpr int G = 8;
         int j = 0;
         for (; j + G <= nv; j += G) {
             unsigned int e[G][VPT];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const unsigned int* row = tab + srow[j + g];
-                const int j0 = sj0[j + g];
+                // the line's start is the same for the whole workgroup: a scalar base
+                const char* row = (const char*)(tab + __builtin_amdgcn_readfirstlane(srow[j + g] + sj0[j + g]));
 #pragma unroll
-                for (int v = 0; v < VPT; ++v) {
-                    int c = j0 - kk[v];
-                    c += c < 0 ? a.npsi : 0;
-                    c = c < 0 ? 0 : c;
-                    e[g][v] = row[c];
-                }
+                for (int v = 0; v < VPT; ++v) e[g][v] = *(const unsigned int*)(row + boff[v]);
             }
+            // keep all G x VPT loads in flight before the first use
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int g = 0; g < G; ++g)
 #pragma unroll
-                for (int v = 0; v < VPT; ++v) {
-                    num[v] += __uint_as_float(e[g][v] << 16);
-                    den[v] += __uint_as_float(e[g][v] & 0xffff0000u);
-                }
+                for (int v = 0; v < VPT; ++v) acc_entry(e[g][v], num[v], den[v]);
         }
         for (; j < nv; ++j) {
-            const unsigned int* row = tab + srow[j];
-            const int j0 = sj0[j];
+            const char* row = (const char*)(tab + __builtin_amdgcn_readfirstlane(srow[j] + sj0[j]));
 #pragma unroll
-            for (int v = 0; v < VPT; ++v) {
-                int c = j0 - kk[v];
-                c += c < 0 ? a.npsi : 0;
-                c = c < 0 ? 0 : c;
-                const unsigned int e = row[c];
-                num[v] += __uint_as_float(e << 16);
-                den[v] += __uint_as_float(e & 0xffff0000u);
-            }
+            for (int v = 0; v < VPT; ++v) acc_entry(*(const unsigned int*)(row + boff[v]), num[v], den[v]);
         }
         __syncthreads();
     }
@@ -1073,7 +1087,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                    (long long)nr, sb.tpol.as<float2>());
                 CIM_HIP(hipGetLastError());
             }
-            CIM_ALLOC(sb.table, sizeof(unsigned int) * (size_t)nb * T * (npsi / 2) * npsi);
+            CIM_ALLOC(sb.table, sizeof(unsigned int) * (size_t)nb * T * (npsi / 2) * 2 * npsi);
             {
                 TimedLaunch tl(ctx, "cl_table");
                 const unsigned nt = (unsigned)((npsi + 63) / 64), ntr = (unsigned)((npsi / 2 + 63) / 64);

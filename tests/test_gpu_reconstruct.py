@@ -166,6 +166,38 @@ def test_brick_gather_equals_scatter(ctx, insert_kernel, nops, n):
 COMPENSATED = 16   # CIM_REC_COMPENSATED
 
 
+@pytest.mark.parametrize("nops,layout", [(60, 0), (60, 1 | COMPENSATED), (7, 1), (60, 1)])
+def test_queue_kernel_equals_row_kernel(ctx, insert_kernel, nops, layout):
+    """The brick gather packed through per-wave queues (default) and the
+    row-by-row brick gather (CIM_INSERT=brickrows) add the same fixed-point
+    terms into exact sums: the same bits, over three batches (the second
+    spans two 1024-plane chunks)."""
+    if insert_kernel != "brick":
+        pytest.skip("compares the two brick kernels once")
+    box, maxR = 128, 58
+    rng = np.random.default_rng(100 + nops + layout)
+    n = 24
+    re = rng.standard_normal((n, box, box)).astype(np.float32)
+    im = rng.standard_normal((n, box, box)).astype(np.float32)
+    ctf = rng.uniform(-1, 1, (n, box, box)).astype(np.float32)
+    eul = np.stack([rng.uniform(0, 180, n), rng.uniform(0, 360, n), rng.uniform(0, 360, n)], 1).astype(np.float32)
+    q = np.linalg.qr(rng.standard_normal((nops, 3, 3)))[0]
+    q *= np.sign(np.linalg.det(q))[:, None, None]
+    ops = None if nops == 60 else q
+    res = {}
+    for kern in ("brick", "brickrows"):
+        os.environ["CIM_INSERT"] = kern
+        ctx.rec_begin(box, maxR, layout=layout, rcut=maxR - 2.0, wg_init=0.0, ops=ops)
+        ctx.rec_insert_ft(re[:3], im[:3], eul[:3], ctf[:3])
+        ctx.rec_insert_ft(re[3:], im[3:], eul[3:], ctf[3:])
+        ctx.rec_insert_ft(re[:1], im[:1], eul[:1], ctf[:1])
+        res[kern] = ctx.rec_get_tnf(maxR)
+    os.environ["CIM_INSERT"] = "brick"
+    for a, b in zip(res["brick"], res["brickrows"]):
+        assert np.array_equal(a, b)
+    assert np.abs(res["brick"][2]).max() > 0
+
+
 def test_compensated_sums_do_not_drift(ctx, insert_kernel):
     """256 one-particle batches of the same particle: the compensated sums stay at
     256 x the single insert (float64 reference), the plain FP32 sums drift."""
