@@ -67,12 +67,14 @@ METRIC = "particles/s search+insert, box 384: chosen vs plain FP32 GPU at FSC>=0
 def _parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64, help="particles per step")
     ap.add_argument("--templates", type=int, default=4, help="template bank size T")
     ap.add_argument("--topk", type=int, default=32, help="AU points re-scored exactly per particle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="time the chosen path step after step (search, then insert) on one stream")
     ap.add_argument("--stage", choices=("main", "center", "lbox"), default="main",
                     help="center: only the centre finding of icosprocess mode 3 at box 384; "
                          "lbox: only local_box's stack mode (12 vertex boxes of 96 pixels per particle)")
@@ -130,6 +132,9 @@ def gpu_run(args, tang, timgs, ps):
     p_insert = lib.make_params(BOX, 0, INSERT_MAXR, apply_ctf=True, mask_on=True, norm_on=True, apix=APIX,
                                imgmask=d["mask"])
     ctx.search_templates(p_chosen, timgs, np.c_[tang, np.zeros((len(tang), 2))])
+    # second context (its own stream) for the pipelined chosen path: the insert of
+    # batch k-1 runs there while the search of batch k runs on ctx
+    ctx_i = lib.Context(0)
     images = torch.from_numpy(ps.images).to("cuda:0")
     if args.calib_copy:
         # 1 GiB copy after 1 GiB of unrelated writes, so the source is read from HBM
@@ -180,8 +185,61 @@ def gpu_run(args, tang, timgs, ps):
         ctx.timing(False)
         return out, e0.elapsed_time(e1), wall, kt, cnt
 
-    # chosen path, timed
-    out_c, ev_c, wall_c, kt_c, cnt_c = timed(p_chosen, COMPENSATED, args.steps, args.warmup)
+    def timed_pipelined(psearch, layout, steps, warmup):
+        """steps batches through search and insert, the search of batch k on ctx
+        overlapping the insert of batch k-1 on ctx_i (two host threads; the
+        library calls release the GIL).  The timed region holds the first
+        search alone and the last insert alone, so it is exactly steps batches."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        def insert(out):
+            Q = P.copy()
+            Q[:, 0:3] = out[:, 0:3]
+            ctx_i.rec_insert(p_insert, images, Q)
+
+        ctx_i.rec_begin(BOX, INSERT_MAXR, layout=layout, rcut=INSERT_MAXR, wg_init=0.01)
+        for _ in range(warmup):
+            insert(ctx.search_global(psearch, images, P))
+        ctx.sync()
+        ctx_i.sync()
+        for c in (ctx, ctx_i):
+            c.timing(True)
+            c.counter_reset()
+        torch.cuda.synchronize()
+        out = None
+        with ThreadPoolExecutor(max_workers=2) as pool:
+            t0 = time.perf_counter()
+            prev = None
+            for k in range(steps + 1):
+                fs = pool.submit(ctx.search_global, psearch, images, P) if k < steps else None
+                fi = pool.submit(insert, prev) if prev is not None else None
+                nxt = fs.result() if fs is not None else None
+                if fi is not None:
+                    fi.result()
+                if nxt is not None:
+                    out = nxt
+                prev = nxt
+            ctx.sync()
+            ctx_i.sync()
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_gather", "cl_rescore",
+                                                   "cl_fallback")}
+        kt["insert"] = ctx_i.timing_get("insert")
+        cnt = {k: ctx.counter(k) for k in ("cl_search_samples", "cl_fallback_particles", "cl_table_pairs",
+                                           "cl_gather_points")}
+        cnt["insert_samples"] = ctx_i.counter("insert_samples")
+        for c in (ctx, ctx_i):
+            c.timing(False)
+        return out, wall * 1e3, wall, kt, cnt
+
+    # chosen path, timed: pipelined (the value) and sequential (reported beside it)
+    if args.no_pipeline:
+        out_c, ev_c, wall_c, kt_c, cnt_c = timed(p_chosen, COMPENSATED, args.steps, args.warmup)
+        ev_seq = ev_c
+    else:
+        out_c, ev_c, wall_c, kt_c, cnt_c = timed_pipelined(p_chosen, COMPENSATED, args.steps, args.warmup)
+        _, ev_seq, _, _, _ = timed(p_chosen, COMPENSATED, args.steps, 0)
     # plain path on the same particles: one timed step after one untimed
     out_p, ev_p, wall_p, kt_p, cnt_p = timed(p_plain, 0, 1, 1)
     # one map per path from one pass over the batch, and their FSC
@@ -198,12 +256,14 @@ def gpu_run(args, tang, timgs, ps):
     fsc = _fsc(maps[0], maps[1], INSERT_MAXR)
     map_rel = float((torch.linalg.vector_norm(maps[0] - maps[1]) / torch.linalg.vector_norm(maps[1])).item())
     ctx.close()
+    ctx_i.close()
     dang = np.abs(((out_c[:, 0:3] - out_p[:, 0:3] + 180) % 360) - 180).max(axis=1)
     identical = int(np.all(out_c[:, [0, 1, 2, 5]] == out_p[:, [0, 1, 2, 5]], axis=1).sum())
     within = int((dang <= STEP + 1e-4).sum())
     ang_err = np.abs(((out_p[:, 0:3] - ps.angles + 180) % 360) - 180)
     return dict(n=n, d=d, ev_c=ev_c, wall_c=wall_c, kt_c=kt_c, cnt_c=cnt_c, ev_p=ev_p, wall_p=wall_p, kt_p=kt_p,
-                cnt_p=cnt_p, identical=identical, within=within, fsc=fsc, map_rel=map_rel, ang_err=ang_err)
+                cnt_p=cnt_p, identical=identical, within=within, fsc=fsc, map_rel=map_rel, ang_err=ang_err,
+                ev_seq=ev_seq)
 
 
 CENTER_BATCH = 224       # one working chunk at box 384
@@ -562,7 +622,7 @@ def _roofline(r, args):
     if dom == "insert":
         per_launch = cnt["insert_samples"] / launches * 60 * INSERT_BYTES_PER_SAMPLE_OP * 2   # hi + lo
         achieved = per_launch / (ms / 1e3) / 1e9
-        return dom, {"bound": "hbm", "kernel": "k_insert_brick (60-fold trilinear insert, compensated FP32)",
+        return dom, {"bound": "hbm", "kernel": "k_insert_brickq (60-fold trilinear insert, compensated FP32)",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": int(per_launch),
@@ -608,7 +668,7 @@ def main():
     if os.path.exists(tj):   # rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated, per launch
         with open(tj) as f:
             tr = json.load(f)
-        key = {"insert": "k_insert_brick", "cl_gather": "k_cl_gather"}.get(dom, "k_cl_search")
+        key = {"insert": "k_insert_brickq", "cl_gather": "k_cl_gather"}.get(dom, "k_cl_search")
         if key in tr and tr[key].get("batch") == n:
             roof["traffic"] = int(tr[key]["hbm_bytes_per_launch"])
     cpu = None
@@ -636,12 +696,16 @@ def main():
         "config": {"workload": f"chosen path vs plain FP32 path, box {BOX}: global common-lines search (0.5 deg "
                                f"AU grid x 720 omega, maxR {d['maxR']}, T={args.templates} templates; chosen: "
                                f"tabulated ranking, top-{args.topk} exact re-score) + 60-fold insert (maxR "
-                               f"{INSERT_MAXR}; chosen: compensated FP32)",
+                               f"{INSERT_MAXR}; chosen: compensated FP32)" +
+                               ("" if args.no_pipeline else "; chosen steps pipelined: the search of batch k on one "
+                                                            "stream overlaps the insert of batch k-1 on another"),
                    "box": BOX, "batch": n, "templates": args.templates, "search_maxR": d["maxR"],
-                   "insert_maxR": INSERT_MAXR, "searchstep": STEP, "apix": APIX, "topk": args.topk},
+                   "insert_maxR": INSERT_MAXR, "searchstep": STEP, "apix": APIX, "topk": args.topk,
+                   "pipelined": not args.no_pipeline},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": {"plain_particles_per_s": round(plain, 3),
+                  "sequential_particles_per_s": round(n * args.steps / (r["ev_seq"] / 1e3), 3),
                   "chosen_over_plain": round(value / plain, 2),
                   "plain_ms_per_step": round(r["ev_p"], 3),
                   "orientation_identical": r["identical"], "orientation_within_one_step": r["within"],
