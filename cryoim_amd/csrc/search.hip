@@ -513,23 +513,21 @@ struct GatherArgs {
     int n;                    // particles
     float* outv;              // n x npts: best table score of the AU point
     int* outi;                // its linear index
+    unsigned sel_n, sel_d;    // bfloat16 pairs (1, 0) and (0, 1): 0x00003f80, 0x3f800000
 };
 
-// Add the {N, D} bfloat16 pair of a table entry to the sums: N the low half,
-// D the high half, each widened exactly to float and added with one rounding.
-// kGatherDot2 does both with v_dot2c_f32_bf16 against (1, 0) and (0, 1).
-constexpr bool kGatherDot2 = false;
+// Add the {N, D} bfloat16 pair of a table entry to the sums, N the low half
+// and D the high half: one v_dot2c_f32_bf16 each, against the selector pairs
+// (1, 0) and (0, 1).  The selectors come from kernel arguments (registers): as
+// inline constants the compiler encodes them as the f32 1.0, which selects the
+// high half.  The dot product rounds its sum on its own terms, so a table
+// score can differ from plain float adds in the last bit; the scores only
+// rank the AU points, which the exact kernel then re-scores.
 typedef short bf16x2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void acc_entry(unsigned int e, float& num, float& den) {
-    if (kGatherDot2) {
-        const bf16x2_t v = __builtin_bit_cast(bf16x2_t, e);
-        const bf16x2_t s0 = {(short)0x3f80, 0}, s1 = {0, (short)0x3f80};
-        num = __builtin_amdgcn_fdot2_f32_bf16(v, s0, num, false);
-        den = __builtin_amdgcn_fdot2_f32_bf16(v, s1, den, false);
-    } else {
-        num += __uint_as_float(e << 16);
-        den += __uint_as_float(e & 0xffff0000u);
-    }
+__device__ __forceinline__ void acc_entry(unsigned int e, float& num, float& den, unsigned sel_n, unsigned sel_d) {
+    const bf16x2_t v = __builtin_bit_cast(bf16x2_t, e);
+    num = __builtin_amdgcn_fdot2_f32_bf16(v, __builtin_bit_cast(bf16x2_t, sel_n), num, false);
+    den = __builtin_amdgcn_fdot2_f32_bf16(v, __builtin_bit_cast(bf16x2_t, sel_d), den, false);
 }
 
 __device__ __forceinline__ int wrap_index(int j, int n) {
@@ -615,12 +613,12 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
 #pragma unroll
             for (int g = 0; g < G; ++g)
 #pragma unroll
-                for (int v = 0; v < VPT; ++v) acc_entry(e[g][v], num[v], den[v]);
+                for (int v = 0; v < VPT; ++v) acc_entry(e[g][v], num[v], den[v], a.sel_n, a.sel_d);
         }
         for (; j < nv; ++j) {
             const char* row = (const char*)(tab + __builtin_amdgcn_readfirstlane(srow[j] + sj0[j]));
 #pragma unroll
-            for (int v = 0; v < VPT; ++v) acc_entry(*(const unsigned int*)(row + boff[v]), num[v], den[v]);
+            for (int v = 0; v < VPT; ++v) acc_entry(*(const unsigned int*)(row + boff[v]), num[v], den[v], a.sel_n, a.sel_d);
         }
         __syncthreads();
     }
@@ -1098,7 +1096,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             }
             GatherArgs ga{sb.table.as<unsigned int>(), sb.tang.as<float>(), sb.pts.as<float>(), sb.pbase.as<int>(),
                           sb.ops.as<float>(), T, 60, npts, npsi, sub, g.No, dpsi, nb, sb.outv.as<float>(),
-                          sb.outi.as<int>()};
+                          sb.outi.as<int>(), 0x00003f80u, 0x3f800000u};
             {
                 TimedLaunch tl(ctx, "cl_gather");
                 const int vg = (g.No + 255) / 256;
