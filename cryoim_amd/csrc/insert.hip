@@ -535,7 +535,10 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
     const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - 1)};
     const float hi[3] = {(float)(bx0 - R + BRICK), (float)(by0 - R + BRICK), (float)(bz0 + BRICK)};
     const float c[3] = {0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
-    const float reach = 0.5f * (float)(BRICK + 1) * 1.7320508f + 0.5f;
+    // A point whose cell touches the brick lies in the reach box [lo, hi) (half-width
+    // (BRICK+1)/2 about c), so its plane passes within half-width * (|n_x|+|n_y|+|n_z|) of c
+    // (n the plane normal, the third column); 0.01 covers the float error of the plane.
+    const float half = 0.5f * (float)(BRICK + 1) + 0.01f;
     const int w = R + 1;
     const int ntot = a.n * g.nops;
     const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
@@ -577,7 +580,7 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
         for (int q = q0 + tid; q < min(ntot, q0 + 1024); q += blockDim.x) {
             const float* m = a.M + (long long)q * 9;
             const float d = fabsf(m[2] * c[0] + m[5] * c[1] + m[8] * c[2]);
-            if (d <= reach) list[atomicAdd(&nlist, 1)] = q;
+            if (d <= half * (fabsf(m[2]) + fabsf(m[5]) + fabsf(m[8]))) list[atomicAdd(&nlist, 1)] = q;
         }
         __syncthreads();
         const int nl = nlist;

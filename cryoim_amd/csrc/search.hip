@@ -653,8 +653,10 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
 // index), in increasing order, and for each the first column of a window of
 // wincols columns (of ncol, the columns per omega half when paired) centred
 // on the omega of its table minimum.
-__global__ void k_topk(const float* __restrict__ v, const int* __restrict__ ix, int npts, int K, int* __restrict__ cand,
-                       int Nomega, int ncol, int wincols, int* __restrict__ kwin) {
+// The K AU points of lowest table score per particle, any number of points:
+// K rounds, each scanning the points not chosen yet.
+__global__ void k_topk_scan(const float* __restrict__ v, const int* __restrict__ ix, int npts, int K,
+                            int* __restrict__ cand, int Nomega, int ncol, int wincols, int* __restrict__ kwin) {
     __shared__ int chosen[256];
     __shared__ float sv[16];
     __shared__ int si[16], sp[16];
@@ -688,6 +690,65 @@ __global__ void k_topk(const float* __restrict__ v, const int* __restrict__ ix, 
             kwin[(long long)p * K + kk] = ((kb - wincols / 2) % ncol + ncol) % ncol;
         }
         __syncthreads();
+    }
+}
+
+// The same selection with each thread's AU points (i = tid + 256 c, c < TOPK_PER) in
+// registers; a round takes the block's lowest (value, linear index) among the
+// points not yet chosen, and the owner thread retires it.
+constexpr int TOPK_PER = 16;   // npts <= 4096 (else k_topk_scan)
+__global__ void __launch_bounds__(256) k_topk(const float* __restrict__ v, const int* __restrict__ ix, int npts, int K,
+                                              int* __restrict__ cand, int Nomega, int ncol, int wincols,
+                                              int* __restrict__ kwin) {
+    __shared__ float sv[4];
+    __shared__ int si[4], sp[4];
+    __shared__ int win_pt, win_li;
+    const int p = blockIdx.x, tid = threadIdx.x;
+    float xv[TOPK_PER];
+    int xl[TOPK_PER];
+    bool live[TOPK_PER];
+#pragma unroll
+    for (int c = 0; c < TOPK_PER; ++c) {
+        const int i = tid + c * 256;
+        live[c] = i < npts;
+        xv[c] = live[c] ? v[(long long)p * npts + i] : 3.0e38f;
+        xl[c] = live[c] ? ix[(long long)p * npts + i] : 0x7fffffff;
+    }
+    for (int kk = 0; kk < K; ++kk) {
+        float best = 3.0e38f;
+        int bidx = 0x7fffffff, bpt = -1;
+#pragma unroll
+        for (int c = 0; c < TOPK_PER; ++c)
+            if (live[c] && (xv[c] < best || (xv[c] == best && xl[c] < bidx))) {
+                best = xv[c];
+                bidx = xl[c];
+                bpt = tid + c * 256;
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(bidx, o, 64);
+            const int op = __shfl_xor(bpt, o, 64);
+            if (ov < best || (ov == best && oi < bidx)) { best = ov; bidx = oi; bpt = op; }
+        }
+        if ((tid & 63) == 0) { sv[tid >> 6] = best; si[tid >> 6] = bidx; sp[tid >> 6] = bpt; }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < 4; ++w)
+                if (sv[w] < best || (sv[w] == best && si[w] < bidx)) { best = sv[w]; bidx = si[w]; bpt = sp[w]; }
+            const int ch = bpt < 0 ? 0 : bpt;
+            cand[(long long)p * K + kk] = ch;
+            const int kb = (bidx == 0x7fffffff ? 0 : bidx % Nomega) % ncol;
+            kwin[(long long)p * K + kk] = ((kb - wincols / 2) % ncol + ncol) % ncol;
+            win_pt = bpt;
+        }
+        __syncthreads();
+        const int w = win_pt;
+        if (w >= 0 && (w & 255) == tid) {
+#pragma unroll
+            for (int c = 0; c < TOPK_PER; ++c)
+                if (c == (w >> 8)) live[c] = false;
+        }
     }
 }
 
@@ -1111,7 +1172,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             }
             CIM_ALLOC(sb.cand, sizeof(int) * (size_t)nb * topk);
             CIM_ALLOC(sb.kwin, sizeof(int) * (size_t)nb * topk);
-            hipLaunchKernelGGL(k_topk, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(),
+            hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(),
                                npts, topk, sb.cand.as<int>(), g.No, ncolx, wincols, sb.kwin.as<int>());
             CIM_HIP(hipGetLastError());
             ctx->counters["cl_table_pairs"] += (long long)nb * T * npsi * npsi * nr;
