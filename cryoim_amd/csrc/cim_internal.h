@@ -68,9 +68,12 @@ struct SearchBatch {
     DevBuf pcrop, pcrop4, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter;
     // tabulated search: polar rays of particles / templates, residual tables, candidates
     DevBuf ppol, tpol, table, cand, kwin, cv, ci;
+    // coarse-to-fine gather: coarse point list, 3x3 neighbours, per-particle coarse winners and fine lists
+    DevBuf clist, nbr, ccand, ckwin, flist, fcnt;
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
-                          &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci})
+                          &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci, &clist, &nbr, &ccand, &ckwin,
+                          &flist, &fcnt})
             b->release();
     }
 };

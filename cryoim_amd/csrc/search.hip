@@ -513,21 +513,20 @@ struct GatherArgs {
     int n;                    // particles
     float* outv;              // n x npts: best table score of the AU point
     int* outi;                // its linear index
-    unsigned sel_n, sel_d;    // bfloat16 pairs (1, 0) and (0, 1): 0x00003f80, 0x3f800000
+    // AU points to score: all npts (plist null), or plist[p * plstride + j] for j < nlist
+    // (plstride 0: one list for every particle), j < pcnt[p] when pcnt is given
+    const int* plist;
+    int plstride, nlist;
+    const int* pcnt;
 };
 
-// Add the {N, D} bfloat16 pair of a table entry to the sums, N the low half
-// and D the high half: one v_dot2c_f32_bf16 each, against the selector pairs
-// (1, 0) and (0, 1).  The selectors come from kernel arguments (registers): as
-// inline constants the compiler encodes them as the f32 1.0, which selects the
-// high half.  The dot product rounds its sum on its own terms, so a table
-// score can differ from plain float adds in the last bit; the scores only
-// rank the AU points, which the exact kernel then re-scores.
-typedef short bf16x2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void acc_entry(unsigned int e, float& num, float& den, unsigned sel_n, unsigned sel_d) {
-    const bf16x2_t v = __builtin_bit_cast(bf16x2_t, e);
-    num = __builtin_amdgcn_fdot2_f32_bf16(v, __builtin_bit_cast(bf16x2_t, sel_n), num, false);
-    den = __builtin_amdgcn_fdot2_f32_bf16(v, __builtin_bit_cast(bf16x2_t, sel_d), den, false);
+// Add the {N, D} bfloat16 pair of a table entry to the sums: N the low half,
+// D the high half, each widened exactly to float and added with one rounding.
+// (Summing with v_dot2c_f32_bf16 against register-held selectors was measured
+// at the same speed: the gather waits on its table reads, not on its adds.)
+__device__ __forceinline__ void acc_entry(unsigned int e, float& num, float& den) {
+    num += __uint_as_float(e << 16);
+    den += __uint_as_float(e & 0xffff0000u);
 }
 
 __device__ __forceinline__ int wrap_index(int j, int n) {
@@ -550,8 +549,11 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     __shared__ int redi[4];
     const long long L = blockIdx.x;
     const long long r8 = L >> 3;
-    const int p = (int)(L & 7) + 8 * (int)(r8 / a.npts), pt = (int)(r8 % a.npts);
+    const int nl = a.plist ? a.nlist : a.npts;
+    const int p = (int)(L & 7) + 8 * (int)(r8 / nl), slot = (int)(r8 % nl);
     if (p >= a.n) return;
+    if (a.pcnt && slot >= a.pcnt[p]) return;
+    const int pt = a.plist ? a.plist[(long long)p * a.plstride + slot] : slot;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int nrow = a.npsi >> 1;
     float num[VPT], den[VPT];
@@ -613,12 +615,12 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
 #pragma unroll
             for (int g = 0; g < G; ++g)
 #pragma unroll
-                for (int v = 0; v < VPT; ++v) acc_entry(e[g][v], num[v], den[v], a.sel_n, a.sel_d);
+                for (int v = 0; v < VPT; ++v) acc_entry(e[g][v], num[v], den[v]);
         }
         for (; j < nv; ++j) {
             const char* row = (const char*)(tab + __builtin_amdgcn_readfirstlane(srow[j] + sj0[j]));
 #pragma unroll
-            for (int v = 0; v < VPT; ++v) acc_entry(*(const unsigned int*)(row + boff[v]), num[v], den[v], a.sel_n, a.sel_d);
+            for (int v = 0; v < VPT; ++v) acc_entry(*(const unsigned int*)(row + boff[v]), num[v], den[v]);
         }
         __syncthreads();
     }
@@ -653,6 +655,39 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
 // index), in increasing order, and for each the first column of a window of
 // wincols columns (of ncol, the columns per omega half when paired) centred
 // on the omega of its table minimum.
+__global__ void k_fill_scores(float* __restrict__ v, int* __restrict__ ix, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        v[i] = 3.0e38f;
+        ix[i] = 0x7fffffff;
+    }
+}
+
+// Coarse-to-fine gather, second stage: the 3x3 grid neighbours of a
+// particle's M best coarse points that have no score yet, each once, as that
+// particle's fine list (one workgroup per particle; npts <= 8192).  Byte
+// flags in LDS are claimed with an atomic OR on their 32-bit word.
+__global__ void __launch_bounds__(256) k_expand(const int* __restrict__ ccand, int M, const int* __restrict__ nbr,
+                                                const float* __restrict__ v, int npts, int* __restrict__ flist,
+                                                int fstride, int* __restrict__ fcnt) {
+    __shared__ unsigned int seen[8192 / 4];
+    __shared__ int cnt;
+    const int p = blockIdx.x, tid = threadIdx.x;
+    for (int i = tid; i < (npts + 3) / 4; i += blockDim.x) seen[i] = 0u;
+    if (tid == 0) cnt = 0;
+    __syncthreads();
+    for (int e = tid; e < M * 9; e += blockDim.x) {
+        const int c = ccand[(long long)p * M + e / 9];
+        const int q = nbr[(long long)c * 9 + e % 9];
+        if (q < 0 || v[(long long)p * npts + q] < 1.0e38f) continue;
+        const unsigned int bit = 1u << (8 * (q & 3));
+        if (atomicOr(&seen[q >> 2], bit) & bit) continue;
+        flist[(long long)p * fstride + atomicAdd(&cnt, 1)] = q;
+    }
+    __syncthreads();
+    if (tid == 0) fcnt[p] = cnt;
+}
+
 // The K AU points of lowest table score per particle, any number of points:
 // K rounds, each scanning the points not chosen yet.
 __global__ void k_topk_scan(const float* __restrict__ v, const int* __restrict__ ix, int npts, int K,
@@ -758,6 +793,7 @@ struct SearchGrid {
     int Nt = 0, Np = 0, No = 0;
     std::vector<float> pts;   // theta, phi
     std::vector<int> base;
+    std::vector<int> it, sip; // theta index and signed phi index of each point
 };
 
 // ort_globesearch.f90:16-21,68-82 evaluated in float32 like the reference.
@@ -786,10 +822,42 @@ static SearchGrid make_grid(float step) {
                 g.pts.push_back(theta);
                 g.pts.push_back(s == 0 ? phi : -phi);
                 g.base.push_back(g.No * ((sip + g.Np) + (2 * g.Np + 1) * it));
+                g.it.push_back(it);
+                g.sip.push_back(sip);
             }
         }
     }
     return g;
+}
+
+// Coarse-to-fine lists of the tabulated search: the coarse points are the
+// grid points with even theta and phi indices (the grid of twice the step),
+// plus every point none of whose 3x3 grid neighbours is such a point (at the
+// AU border), so every point is coarse or next to a coarse one.  nbr: for each
+// point its 3x3 neighbours in the grid (-1 where outside the AU).
+static void coarse_lists(const SearchGrid& g, std::vector<int>& clist, std::vector<int>& nbr) {
+    const int npts = (int)g.it.size(), W = 2 * g.Np + 1;
+    std::vector<int> id((size_t)(g.Nt + 1) * W, -1);
+    for (int q = 0; q < npts; ++q) id[(size_t)g.it[q] * W + g.sip[q] + g.Np] = q;
+    auto at = [&](int it, int sip) {
+        return (it < 0 || it > g.Nt || sip < -g.Np || sip > g.Np) ? -1 : id[(size_t)it * W + sip + g.Np];
+    };
+    nbr.assign((size_t)npts * 9, -1);
+    for (int q = 0; q < npts; ++q)
+        for (int d = 0; d < 9; ++d) nbr[(size_t)q * 9 + d] = at(g.it[q] + d / 3 - 1, g.sip[q] + d % 3 - 1);
+    std::vector<char> coarse(npts, 0), covered(npts, 0);
+    auto take = [&](int q) {
+        coarse[q] = 1;
+        for (int d = 0; d < 9; ++d)
+            if (nbr[(size_t)q * 9 + d] >= 0) covered[nbr[(size_t)q * 9 + d]] = 1;
+    };
+    for (int q = 0; q < npts; ++q)
+        if (g.it[q] % 2 == 0 && g.sip[q] % 2 == 0) take(q);
+    for (int q = 0; q < npts; ++q)
+        if (!covered[q]) take(q);
+    clist.clear();
+    for (int q = 0; q < npts; ++q)
+        if (coarse[q]) clist.push_back(q);
 }
 
 // Disk of taps reachable from radius <= R (plus a one-pixel margin) as rows
@@ -1035,6 +1103,29 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     // the tables take 4 T npsi^2 bytes per particle (8.3 MB at T = 4, 0.5 deg): batches of 64
     const int batch = std::max(1, std::min(n, tables ? 64 : 256));
     CtfConst cc = make_ctf_const(p);
+
+    // coarse-to-fine table scores: the coarse points (every other theta and phi index,
+    // plus the AU border points no such point neighbours) first, then the unscored 3x3
+    // neighbours of each particle's coarseM best.  CIM_TABLE_COARSE=<M>; 0 scores every point.
+    int coarseM = 64;
+    if (const char* ec = getenv("CIM_TABLE_COARSE")) coarseM = std::max(0, atoi(ec));
+    std::vector<int> clist, nbrv;
+    bool c2f = false;
+    if (tables && coarseM > 0 && npts <= 8192) {
+        coarse_lists(g, clist, nbrv);
+        c2f = (int)clist.size() > coarseM && (int)clist.size() >= topk;
+    }
+    if (c2f) {
+        CIM_ALLOC(sb.clist, sizeof(int) * clist.size());
+        CIM_ALLOC(sb.nbr, sizeof(int) * nbrv.size());
+        CIM_ALLOC(sb.ccand, sizeof(int) * (size_t)batch * coarseM);
+        CIM_ALLOC(sb.ckwin, sizeof(int) * (size_t)batch * coarseM);
+        CIM_ALLOC(sb.flist, sizeof(int) * (size_t)batch * coarseM * 9);
+        CIM_ALLOC(sb.fcnt, sizeof(int) * (size_t)batch);
+        CIM_HIP(hipMemcpyAsync(sb.clist.ptr, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        CIM_HIP(hipMemcpyAsync(sb.nbr.ptr, nbrv.data(), sizeof(int) * nbrv.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
     for (int b0 = 0; b0 < n; b0 += batch) {
         const int nb = std::min(batch, n - b0);
         // records: mask around each particle's own centre (ort_globesearch.f90:50)
@@ -1157,11 +1248,10 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             }
             GatherArgs ga{sb.table.as<unsigned int>(), sb.tang.as<float>(), sb.pts.as<float>(), sb.pbase.as<int>(),
                           sb.ops.as<float>(), T, 60, npts, npsi, sub, g.No, dpsi, nb, sb.outv.as<float>(),
-                          sb.outi.as<int>(), 0x00003f80u, 0x3f800000u};
-            {
-                TimedLaunch tl(ctx, "cl_gather");
+                          sb.outi.as<int>(), nullptr, 0, 0, nullptr};
+            auto gather = [&](int nlist) -> int {
                 const int vg = (g.No + 255) / 256;
-                const dim3 gg((unsigned)(8LL * npts * ((nb + 7) / 8)));
+                const dim3 gg((unsigned)(8LL * nlist * ((nb + 7) / 8)));
                 if (vg <= 1) hipLaunchKernelGGL(k_cl_gather<1>, gg, dim3(256), 0, ctx->stream, ga);
                 else if (vg == 2) hipLaunchKernelGGL(k_cl_gather<2>, gg, dim3(256), 0, ctx->stream, ga);
                 else if (vg == 3) hipLaunchKernelGGL(k_cl_gather<3>, gg, dim3(256), 0, ctx->stream, ga);
@@ -1169,6 +1259,36 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 else if (vg <= 6) hipLaunchKernelGGL(k_cl_gather<6>, gg, dim3(256), 0, ctx->stream, ga);
                 else hipLaunchKernelGGL(k_cl_gather<8>, gg, dim3(256), 0, ctx->stream, ga);
                 CIM_HIP(hipGetLastError());
+                return CIM_OK;
+            };
+            {
+                TimedLaunch tl(ctx, "cl_gather");
+                if (c2f) {
+                    // coarse points, the M best per particle, their unscored neighbours
+                    const long long ns = (long long)nb * npts;
+                    hipLaunchKernelGGL(k_fill_scores, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream,
+                                       sb.outv.as<float>(), sb.outi.as<int>(), ns);
+                    CIM_HIP(hipGetLastError());
+                    ga.plist = sb.clist.as<int>();
+                    ga.plstride = 0;
+                    ga.nlist = (int)clist.size();
+                    if ((rc = gather(ga.nlist))) return rc;
+                    hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0,
+                                       ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(), npts, coarseM,
+                                       sb.ccand.as<int>(), g.No, ncolx, wincols, sb.ckwin.as<int>());
+                    CIM_HIP(hipGetLastError());
+                    hipLaunchKernelGGL(k_expand, dim3(nb), dim3(256), 0, ctx->stream, sb.ccand.as<int>(), coarseM,
+                                       sb.nbr.as<int>(), sb.outv.as<float>(), npts, sb.flist.as<int>(), coarseM * 9,
+                                       sb.fcnt.as<int>());
+                    CIM_HIP(hipGetLastError());
+                    ga.plist = sb.flist.as<int>();
+                    ga.plstride = coarseM * 9;
+                    ga.nlist = coarseM * 9;
+                    ga.pcnt = sb.fcnt.as<int>();
+                    if ((rc = gather(ga.nlist))) return rc;
+                } else if ((rc = gather(npts))) {
+                    return rc;
+                }
             }
             CIM_ALLOC(sb.cand, sizeof(int) * (size_t)nb * topk);
             CIM_ALLOC(sb.kwin, sizeof(int) * (size_t)nb * topk);
@@ -1176,7 +1296,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                npts, topk, sb.cand.as<int>(), g.No, ncolx, wincols, sb.kwin.as<int>());
             CIM_HIP(hipGetLastError());
             ctx->counters["cl_table_pairs"] += (long long)nb * T * npsi * npsi * nr;
-            ctx->counters["cl_gather_points"] += (long long)nb * npts * g.No;
+            if (!c2f) ctx->counters["cl_gather_points"] += (long long)nb * npts * g.No;
             a.cand = sb.cand.as<int>();
             a.ncand = topk;
             a.prfull = nullptr;
@@ -1209,6 +1329,11 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         std::vector<int> hi(nb);
         CIM_HIP(hipMemcpyAsync(hv.data(), sb.fv.ptr, sizeof(float) * nb, hipMemcpyDeviceToHost, ctx->stream));
         CIM_HIP(hipMemcpyAsync(hi.data(), sb.fi.ptr, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<int> hfc;
+        if (c2f) {
+            hfc.resize(nb);
+            CIM_HIP(hipMemcpyAsync(hfc.data(), sb.fcnt.ptr, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream));
+        }
         std::vector<float> full;
         if (pr_array) {
             full.resize((size_t)npts * g.No);
@@ -1225,6 +1350,11 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                    ctx->stream));
         }
         CIM_HIP(hipStreamSynchronize(ctx->stream));
+        if (c2f) {
+            long long scored = (long long)nb * (long long)clist.size();
+            for (int c : hfc) scored += c;
+            ctx->counters["cl_gather_points"] += scored * g.No;
+        }
         if (tables && fb_rank <= topk) {
             // Fallback: when the exact minimum sits deep in the candidate list, the tables ranked this
             // particle poorly; score every AU point exactly instead (the plain search for that particle).

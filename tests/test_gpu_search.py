@@ -152,6 +152,26 @@ def test_tabulated_search_equals_plain(ctx, monkeypatch, variant_name):
     assert same.all()
 
 
+@pytest.mark.parametrize("coarse", ["8", "24"])
+def test_tabulated_search_coarse_to_fine_equals_plain(ctx, monkeypatch, coarse):
+    """The coarse-to-fine table scores (every other theta and phi index first,
+    then the 3x3 neighbours of each particle's M best): on this set every
+    particle still gets the plain result bit for bit (the 2 deg grid has 198 AU
+    points; M = 8 and 24 of its coarse ones are refined)."""
+    monkeypatch.setenv("CIM_TABLE_COARSE", coarse)
+    plain, ps = _search_pair(ctx, lib.SEARCH_PLAIN, 0)
+    ctx.counter_reset()
+    tab, _ = _search_pair(ctx, lib.SEARCH_TABLES, 0)
+    same = np.all(tab[:, [0, 1, 2, 5]] == plain[:, [0, 1, 2, 5]], axis=1)
+    scored = ctx.counter("cl_gather_points")
+    print(f"coarse M={coarse}: identical {same.sum()} of {len(same)}, scored (point, omega) {scored}")
+    assert same.all()
+    monkeypatch.setenv("CIM_TABLE_COARSE", "0")
+    ctx.counter_reset()
+    _search_pair(ctx, lib.SEARCH_TABLES, 0)
+    assert scored < ctx.counter("cl_gather_points")     # fewer points scored than the full gather
+
+
 def test_tabulated_search_exhaustive_candidates(ctx, monkeypatch):
     """With every AU point a candidate and full omega windows the tabulated search
     is the plain search: identical output whatever the tables rank."""
