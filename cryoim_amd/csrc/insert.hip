@@ -555,18 +555,22 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
             const bool flip = (pk >> 12) & 1;
             const float4 vv = a.vals[qval[wv][e]];
             const float ax = qax[wv][e], ay = qay[wv][e], az = qaz[wv][e];
-            const float va = vv.x, vb = flip ? -vv.y : vv.y, vw = vv.z;
+            // values scaled once: (v * scale) * w equals (v * w) * scale for the power of two
+            // scale (a product that differs is below 2^-126 and rounds to a zero term)
+            const float va = vv.x * scale, vb = (flip ? -vv.y : vv.y) * scale, vw = vv.z * scale;
             const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay, w11 = ax * ay;
             const float wz0 = 1.f - az, wz1 = az;
             const float wt[8] = {w00 * wz0, w10 * wz0, w01 * wz0, w11 * wz0, w00 * wz1, w10 * wz1, w01 * wz1, w11 * wz1};
+            const bool inx[2] = {ox >= 0, ox + 1 < BRICK}, iny[2] = {oy >= 0, oy + 1 < BRICK},
+                       inz[2] = {oz >= 0, oz + 1 < BRICK};
+            const int base = (oz * BRICK + oy) * BRICK + ox;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const int cx = ox + (j & 1), cy = oy + ((j >> 1) & 1), cz = oz + (j >> 2);
-                if (cx < 0 || cx >= BRICK || cy < 0 || cy >= BRICK || cz < 0 || cz >= BRICK) continue;
-                const int li3 = (cz * BRICK + cy) * BRICK + cx;
-                atomicAdd(&sacc[li3], fix64(va * wt[j] * scale));
-                atomicAdd(&sacc[NV + li3], fix64(vb * wt[j] * scale));
-                atomicAdd(&sacc[2 * NV + li3], fix64(vw * wt[j] * scale));
+                if (!(inx[j & 1] && iny[(j >> 1) & 1] && inz[j >> 2])) continue;
+                const int li3 = base + (j & 1) + ((j >> 1) & 1) * BRICK + (j >> 2) * BRICK * BRICK;
+                atomicAdd(&sacc[li3], fix64(va * wt[j]));
+                atomicAdd(&sacc[NV + li3], fix64(vb * wt[j]));
+                atomicAdd(&sacc[2 * NV + li3], fix64(vw * wt[j]));
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -588,14 +592,10 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
             const int q = list[li];
             const int p = q / g.nops;
             const float* m = a.M + (long long)q * 9;
-            float vmin = 3e38f, vmax = -3e38f;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float X = (k & 1) ? hi[0] : lo[0], Y = (k & 2) ? hi[1] : lo[1], Z = (k & 4) ? hi[2] : lo[2];
-                const float v = m[1] * X + m[4] * Y + m[7] * Z;
-                vmin = fminf(vmin, v);
-                vmax = fmaxf(vmax, v);
-            }
+            // k = col1 . x over the reach box: its centre value -/+ the half-widths times |col1|
+            const float kc = m[1] * c[0] + m[4] * c[1] + m[7] * c[2];
+            const float kh = 0.5f * (float)(BRICK + 1) * (fabsf(m[1]) + fabsf(m[4]) + fabsf(m[7]));
+            const float vmin = kc - kh, vmax = kc + kh;
             // Rows and per-row h intervals of the plane's points in the reach box, with a
             // margin of 0.01 lattice steps over the float error of the bounds (~1e-4); the
             // exact test below decides which points count, as in k_insert_brick, whose
