@@ -155,6 +155,7 @@ struct SearchArgs {
     int ncand;
     const int* kwin;          // optional n x ncand first column of a window of wincols columns (mod ncol)
     int wincols;
+    int wsplit;               // workgroups per window (grid.x = ncand * wsplit), each wincols / wsplit columns
 };
 
 // Particle taps: MODE 0 reads the float2 disk staged in LDS, MODE 1 reads
@@ -183,9 +184,10 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     float* redv = reinterpret_cast<float*>(rbase + a.nrows);
     int* redi = reinterpret_cast<int*>(redv + 16);
 
-    const int slot = blockIdx.x, p = blockIdx.y;
+    const int split = a.kwin ? a.wsplit : 1;
+    const int slot = blockIdx.x / split, part = blockIdx.x - slot * split, p = blockIdx.y;
     const int pt = a.cand ? a.cand[(long long)p * a.ncand + slot] : slot;
-    const long long oslot = (long long)p * gridDim.x + slot;
+    const long long oslot = (long long)p * gridDim.x + blockIdx.x;
     const int tid = threadIdx.x, bd = blockDim.x;
     const float2* pc = a.pcrop + (long long)p * a.Dc * a.Dc;
     const float4* pc4 = a.pcrop4 + (long long)p * a.Dc * a.Dc;
@@ -195,8 +197,9 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     const int ncol = PAIR ? a.Nomega / 2 : a.Nomega;
     // columns this workgroup scores: all, or a window starting at kbase (wrapping); the
     // arithmetic of a column does not depend on which workgroup or lane computes it
-    const int kbase = a.kwin ? a.kwin[(long long)p * a.ncand + slot] : 0;
-    const int nact = a.kwin ? a.wincols : ncol;
+    const int nact = a.kwin ? a.wincols / split : ncol;
+    int kbase = a.kwin ? a.kwin[(long long)p * a.ncand + slot] + part * nact : 0;
+    if (kbase >= ncol) kbase -= ncol;
 
     for (int i = tid; i < a.nr; i += bd) fc0[i] = a.fcut0[(long long)p * a.nr + i];
     if (MODE == 0) {
@@ -219,10 +222,10 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     for (int t = 0; t < a.T; ++t) {
         const float2* tc = a.tcrop + (long long)p * a.tc_pstride + (long long)t * a.Dc * a.Dc;
         const float* tf = a.tfcut + (long long)p * a.tf_pstride + (long long)t * a.nr;
-        if (tid < a.nops) {
+        for (int j = tid; j < a.nops; j += bd) {
             float E2[9];
             euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
-            crosslin_one(E1, E2, a.ops + 9 * tid, &alpha1[tid], &alpha2[tid]);
+            crosslin_one(E1, E2, a.ops + 9 * j, &alpha1[j], &alpha2[j]);
         }
         __syncthreads();
         for (int j0 = 0; j0 < a.nops; j0 += a.chunk) {
@@ -322,9 +325,11 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
             }
         }
     }
-    // wave reduction
+    // wave reduction (over the lanes the workgroup has: bd is a multiple of 64 or a
+    // power of two below it)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
+        if (o >= bd) continue;
         const float ov = __shfl_xor(best, o, 64);
         const int oi = __shfl_xor(bidx, o, 64);
         if (ov < best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
@@ -922,7 +927,7 @@ static int prep_templates(cim_ctx* ctx, const cim_params* p, int T, const float*
 
 
 static int launch_search(cim_ctx* ctx, SearchArgs& a, int n, int lds_bytes, int mode, bool pair, int vpt, int bd) {
-    dim3 grid(a.cand ? a.ncand : a.npts, n);
+    dim3 grid(a.cand ? a.ncand * (a.kwin ? a.wsplit : 1) : a.npts, n);
 #define CIM_LAUNCH(V, M, P)                                                                                      \
     do {                                                                                                         \
         CIM_HIP(hipFuncSetAttribute((const void*)k_cl_search<V, M, P>, hipFuncAttributeMaxDynamicSharedMemorySize, \
@@ -1181,8 +1186,10 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             a.tc_pstride = 0;
             a.tf_pstride = 0;
         }
-        CIM_ALLOC(sb.outv, sizeof(float) * (size_t)npts * nb);
-        CIM_ALLOC(sb.outi, sizeof(int) * (size_t)npts * nb);
+        // the exact scores: one per AU point, or per (candidate, window part) (at most 4 x 256)
+        const size_t nout = std::max((size_t)npts, (size_t)4 * 256);
+        CIM_ALLOC(sb.outv, sizeof(float) * nout * nb);
+        CIM_ALLOC(sb.outi, sizeof(int) * nout * nb);
         if (pr_array) CIM_ALLOC(sb.prfull, sizeof(float) * (size_t)npts * g.No * nb);
         a.pcrop = sb.pcrop.as<float2>();
         a.pcrop4 = sb.pcrop4.as<float4>();
@@ -1303,11 +1310,17 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             if (wincols < ncolx) {
                 a.kwin = sb.kwin.as<int>();
                 a.wincols = wincols;
+                // each window on wsplit workgroups of wincols / wsplit columns
+                // (CIM_RESCORE_SPLIT=<1|2|4>): more waves, but each lane-idle wave costs issue
+                // slots; measured at box 384, 512 particles: 85 ms (1), 93 ms (2), 136 ms (4)
+                a.wsplit = 1;
+                if (const char* es = getenv("CIM_RESCORE_SPLIT")) a.wsplit = std::max(1, std::min(4, atoi(es)));
+                while (a.wsplit > 1 && wincols % a.wsplit) --a.wsplit;
                 // one-wave workgroups: a short line chunk keeps LDS small so many stay resident
                 // (the chunking does not change any column's accumulation order)
                 a.chunk = std::min(chunk, 6);
             }
-            nscore = topk;
+            nscore = topk * (a.kwin ? a.wsplit : 1);
         }
         {
             TimedLaunch tl(ctx, tables ? "cl_rescore" : "cl_search");
@@ -1315,7 +1328,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             if (a.kwin) {
                 const int lw = (int)(misc + sizeof(float4) * (size_t)a.chunk * nr) +
                                (sh.lds ? (int)(sizeof(float2) * (size_t)disk_total) : 0);
-                rc = launch_search(ctx, a, nb, lw, sh.lds ? 0 : 1, sh.pair, 1, wincols);
+                rc = launch_search(ctx, a, nb, lw, sh.lds ? 0 : 1, sh.pair, 1, wincols / a.wsplit);
             }
             else rc = launch_search(ctx, a, nb, sh.lds_bytes, sh.lds ? 0 : 1, sh.pair, sh.vpt, bd_exact);
             if (rc) return rc;
@@ -1375,6 +1388,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 SearchArgs a1 = a;
                 a1.cand = nullptr;
                 a1.ncand = 0;
+                a1.kwin = nullptr;      // every omega column of every AU point
+                a1.wsplit = 1;
                 a1.pcrop += (long long)i * Dc * Dc;
                 a1.pcrop4 += (long long)i * Dc * Dc;
                 a1.fcut0 += (long long)i * nr;

@@ -182,12 +182,15 @@ def test_tabulated_search_exhaustive_candidates(ctx, monkeypatch):
     assert np.array_equal(tab[:, [0, 1, 2, 5]], plain[:, [0, 1, 2, 5]])
 
 
-def test_tabulated_search_fallback_is_plain(ctx, monkeypatch):
+@pytest.mark.parametrize("step", [3.0, 2.0])
+def test_tabulated_search_fallback_is_plain(ctx, monkeypatch, step):
     """The fallback path (forced for every particle) is the exhaustive search of
-    the plain method, particle by particle: identical output."""
+    the plain method, particle by particle: identical output.  At 2 deg the
+    re-score uses omega windows (64 of 90 paired columns) and the fallback must
+    not inherit them."""
     monkeypatch.setenv("CIM_TABLE_FALLBACK", "-1")
-    plain, _ = _search_pair(ctx, lib.SEARCH_PLAIN, 0, n=8, step=3.0)
-    tab, _ = _search_pair(ctx, lib.SEARCH_TABLES, 2, n=8, step=3.0)
+    plain, _ = _search_pair(ctx, lib.SEARCH_PLAIN, 0, n=8, step=step)
+    tab, _ = _search_pair(ctx, lib.SEARCH_TABLES, 2, n=8, step=step)
     assert ctx.counter("cl_fallback_particles") == 8
     assert np.array_equal(tab[:, [0, 1, 2, 5]], plain[:, [0, 1, 2, 5]])
 
