@@ -13,9 +13,13 @@ with Gaussian noise at SNR 0.05.  One step takes a batch of particles through
     the orientations the search returned.
 
 Chosen path (value): the tabulated search (CIM_SEARCH_TABLES: per-particle
-residual tables over ray pairs, every (AU point, omega) scored from them,
-the best 32 AU points re-scored exactly, exhaustive fallback) and the
-compensated FP32 insert (CIM_REC_COMPENSATED).  Plain path: the exhaustive
+residual tables over ray pairs, (AU point, omega) scored from them coarse to
+fine, the best 32 AU points re-scored exactly, exhaustive fallback) and the
+compensated FP32 insert (CIM_REC_COMPENSATED).  Its steps are pipelined: the
+search of batch k runs on one library context (stream) while the insert of
+batch k-1 runs on another; the timed region holds exactly --steps batches
+through both stages, and the same steps timed one after the other are
+reported as extra.sequential_particles_per_s.  Plain path: the exhaustive
 search of the reference's method and the plain FP32 insert.  The line
 reports both rates, how many particles get a different orientation, and the
 FSC between the two maps.
@@ -27,10 +31,10 @@ of those particles; the default run reports its rates under extra.
 the 96-pixel boxes around its 12 five-fold vertices (local_z 140 px).
 
 Inputs are resident in HBM before the timed region.  The roofline block
-describes the kernel that takes most of a chosen step; the cpu_baseline
-block times the reference's own Fortran (oracle/_ref/icosprocess, MKL FFTW)
-on a bounded sample on the host cores, or the C oracle when that binary
-cannot run.
+describes the kernel that takes most of a chosen step (its HIP-event time in
+the timed steps); the cpu_baseline block times the reference's own Fortran
+(oracle/_ref/icosprocess, MKL FFTW, one thread per process) on a bounded
+sample on the host cores, or the C oracle when that binary cannot run.
 """
 from __future__ import annotations
 
