@@ -308,6 +308,21 @@ def _parse_local_ort(path, long):
     return np.array(rows, np.float64)
 
 
+SYM_FILES = ("icos2f_sym_matrix", "icos2f_sym_matrix_53", "icos2f_12vertex_matrix", "icos5f_12vertex_matrix",
+             "icos3f_20facecenter", "icos2f_30edgecenter")
+
+
+def symfile_vectors() -> dict:
+    """The symmetry-matrix data files shipped in local_box/SRC, as the operators
+    read_sym_matrixa reads from them (column by column), in float64."""
+    g = {}
+    for name in SYM_FILES:
+        vals = [float(v) for v in open(os.path.join(REFERENCE_ROOT, "local_box", "SRC", name + ".txt")).read().split()]
+        m = np.array(vals, np.float64).reshape(-1, 3, 3)          # [op, column, row]
+        g["sym_" + name] = np.transpose(m, (0, 2, 1)).copy()     # [op, row, column]
+    return g
+
+
 def local_box_vectors() -> dict:
     """The real local_box on the program_vectors particles: local ortfiles
     (coordinate mode) for sym=i5 and for the 12-vertex matrix fixture, and the
@@ -683,7 +698,7 @@ def main():
             ("local", local_program_vectors), ("center", center_program_vectors),
             ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors),
             ("symc", symc_program_vectors), ("lrec", local_reconstruct_vectors),
-            ("lbox", local_box_vectors)]
+            ("lbox", local_box_vectors), ("symfiles", symfile_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):
