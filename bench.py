@@ -631,7 +631,11 @@ def _roofline(r, args):
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": int(per_launch),
                      "bytes_rule": "inserted samples x 60 operators x 8 corners x read+write of fa, fb, wg (f32) "
-                                   "x 2 (hi and lo sums)", "avg_launch_ms": round(ms, 3), "launches": launches}
+                                   "x 2 (hi and lo sums)", "avg_launch_ms": round(ms, 3), "launches": launches,
+                     "note": "frac above 1: the rule counts one HBM read-modify-write per corner update (an atomic "
+                             "scatter); the brick gather sums a batch in LDS and touches each accumulator voxel "
+                             "once, so its measured traffic is about a tenth of the rule's bytes and the kernel is "
+                             "bound by VALU issue and LDS atomics instead"}
     if dom in ("cl_rescore", "cl_search", "cl_fallback"):
         samples = cnt["cl_search_samples"] / launches
         achieved = samples * FLOPS_PER_SAMPLE / (ms / 1e3) / 1e12

@@ -121,6 +121,7 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->rec.ops.release();
     ctx->rec.samples.release();
     ctx->rec.bricks.release();
+    ctx->rec.bricksq.release();
     ctx->rec.hxmax.release();
     ctx->rec.vals.release();
     ctx->rec.maxv.release();

@@ -58,6 +58,8 @@ struct RecState {
     DevBuf hxmax;                   // largest inserted hx per ky (-1: none)
     DevBuf bricks;                  // int3 origins of the 8^3 bricks a sample can reach
     int nbricks = 0;
+    DevBuf bricksq;                 // the same for the queue kernel's brick shape, central first
+    int nbricksq = 0, qshape = 0;
     DevBuf vals;                    // per-batch float4 sample values on the half-plane grid
     DevBuf maxv;                    // largest |sample value| of the batch (float bits)
     long long inserted = 0;

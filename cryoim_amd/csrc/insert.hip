@@ -367,9 +367,58 @@ __device__ __forceinline__ unsigned long long fix64(float x) {
 
 constexpr int BRICK_NV = BRICK * BRICK * BRICK;
 
-// One read-modify-write of the brick's exact batch sums into HBM.
+// LDS slot of brick voxel i in the queue kernel: a rotation of each block of
+// 32 slots by 5 x (block index), so the 64-bit atomics of a wave (corners of
+// a patch of one plane) spread over more banks than with the plain layout,
+// whatever the plane's orientation.
+__device__ __forceinline__ int brick_slot(int i) { return (i & ~31) | ((i + 5 * (i >> 5)) & 31); }
+
+// One read-modify-write of the brick's exact batch sums into HBM (brick of
+// BX x BY x BZ voxels at accumulator index (bx0, by0, bz0), x fastest; SWZ:
+// voxel i at LDS slot brick_slot(i)).
+template <int BX, int BY, int BZ, bool SWZ>
 __device__ void brick_flush(const BrickArgs& a, const unsigned long long* sacc, int bx0, int by0, int bz0,
-                            double inv);
+                            double inv) {
+    constexpr int NV = BX * BY * BZ;
+    static_assert(!SWZ || NV % 32 == 0, "slot rotation needs whole blocks of 32");
+    const InsertGeom& g = a.g;
+    const int tid = threadIdx.x;
+    const long long dxy = g.dxy;
+    for (int i = tid; i < NV; i += blockDim.x) {
+        const int cx = i % BX, cy = (i / BX) % BY, cz = i / (BX * BY);
+        const int X = bx0 + cx, Y = by0 + cy, Z = bz0 + cz;
+        if (X >= g.dxy || Y >= g.dxy || Z >= a.dz) continue;
+        const int si = SWZ ? brick_slot(i) : i;
+        const long long sa = (long long)sacc[si], sb = (long long)sacc[NV + si], sw = (long long)sacc[2 * NV + si];
+        if ((sa | sb | sw) == 0) continue;
+        const double da = (double)sa * inv, db = (double)sb * inv, dw = (double)sw * inv;
+        const long long vi = (long long)Z * dxy * dxy + (long long)Y * dxy + X;
+        float4 v = a.acc[vi];
+        if (a.lo) {
+            // compensated: hi + lo carries the sum; TwoSum (Knuth) keeps the rounding error of each add,
+            // and the part of the exact batch sum below float precision goes to lo as well
+            float4 l = a.lo[vi];
+            auto add = [](float& hi, float& lo, double d) {
+                const float y = (float)d;
+                const float s = __fadd_rn(hi, y);
+                const float yy = __fsub_rn(s, hi);
+                const float err = __fadd_rn(__fsub_rn(hi, __fsub_rn(s, yy)), __fsub_rn(y, yy));
+                lo = __fadd_rn(lo, __fadd_rn(err, (float)(d - (double)y)));
+                hi = s;
+            };
+            add(v.x, l.x, da);
+            add(v.y, l.y, db);
+            add(v.z, l.z, dw);
+            a.lo[vi] = l;
+        } else {
+            v.x = (float)((double)v.x + da);
+            v.y = (float)((double)v.y + db);
+            v.z = (float)((double)v.z + dw);
+        }
+        a.acc[vi] = v;
+    }
+}
+
 
 __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
     constexpr int NV = BRICK * BRICK * BRICK;
@@ -499,49 +548,56 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
         }
         __syncthreads();
     }
-    brick_flush(a, sacc, bx0, by0, bz0, inv);
+    brick_flush<BRICK, BRICK, BRICK, false>(a, sacc, bx0, by0, bz0, inv);
 }
 
 // The same brick gather with the work of a wave packed into full lanes.
-// Per plane, the lattice points of the rows (the same row intervals as
-// k_insert_brick) are numbered through a wave prefix sum, so each lane takes
+// Per plane, the lattice points of the rows (row intervals built as in
+// k_insert_brick, over this kernel's brick) are numbered through a wave
+// prefix sum, so each lane takes
 // one point; the point's candidate samples are placed and tested, and the
 // ones whose cell touches the brick go to a queue in LDS (its offsets in
 // the brick, fractions, flip and value index).  Whenever the queue holds 64
 // entries, 64 lanes add their 8 corners.  Every contribution is the same
 // fixed-point term as in k_insert_brick and the sums are exact, so both
-// kernels give the same bits.
-constexpr int QCAP = 192;   // < 64 left over + 2 candidates x 64 lanes
+// kernels give the same bits, whatever the brick shape BX x BY x BZ (each
+// at most 16) and the workgroup size NT.  Bricks larger than 8^3 meet fewer
+// (plane, brick) pairs, whose row set-up otherwise costs as much as the
+// points; the slot rotation (brick_slot) keeps their atomics' bank conflicts
+// at the level of the small bricks.
+constexpr int QCAP = 128;   // < 64 left over + 64 lanes: drained after every candidate pass
 
-__global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
-    constexpr int NV = BRICK_NV;
+template <int BX, int BY, int BZ, int NT>
+__global__ void __launch_bounds__(NT) k_insert_brickq(BrickArgs a) {
+    static_assert(BX <= 16 && BY <= 16 && BZ <= 16 && NT % 64 == 0 && NT <= 1024, "brick shape");
+    constexpr int NV = BX * BY * BZ, NW = NT / 64;
     __shared__ unsigned long long sacc[3 * NV];
     __shared__ int list[1024];
     __shared__ int nlist;
-    __shared__ int qpos[4][QCAP];       // (ox+1) | (oy+1) << 4 | (oz+1) << 8 | flip << 12
-    __shared__ unsigned qval[4][QCAP];  // index into a.vals
-    __shared__ float qax[4][QCAP], qay[4][QCAP], qaz[4][QCAP];
-    __shared__ int shx[2 * 512 + 1];    // hxmax (maxR <= 512)
+    __shared__ int qpos[NW][QCAP];       // (ox+1) | (oy+1) << 5 | (oz+1) << 10 | flip << 15
+    __shared__ unsigned qval[NW][QCAP];  // index into a.vals
+    __shared__ float qax[NW][QCAP], qay[NW][QCAP], qaz[NW][QCAP];
+    __shared__ int shx[2 * 512 + 1];     // hxmax (maxR <= 512)
     const int tid = threadIdx.x;
     const int bi = blockIdx.x;
     const int bx0 = a.bricks[3 * bi], by0 = a.bricks[3 * bi + 1], bz0 = a.bricks[3 * bi + 2];
     const InsertGeom& g = a.g;
     const int R = g.maxR;
-    for (int i = tid; i < 3 * NV; i += blockDim.x) sacc[i] = 0ull;
-    for (int i = tid; i <= 2 * R; i += blockDim.x) shx[i] = a.hxmax[i];
+    for (int i = tid; i < 3 * NV; i += NT) sacc[i] = 0ull;
+    for (int i = tid; i <= 2 * R; i += NT) shx[i] = a.hxmax[i];
     const int cap = brick_cap(a);
     const float scale = ldexpf(1.f, cap);
     const double inv = ldexp(1.0, -cap);
     const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - 1)};
-    const float hi[3] = {(float)(bx0 - R + BRICK), (float)(by0 - R + BRICK), (float)(bz0 + BRICK)};
+    const float hi[3] = {(float)(bx0 - R + BX), (float)(by0 - R + BY), (float)(bz0 + BZ)};
     const float c[3] = {0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
-    // A point whose cell touches the brick lies in the reach box [lo, hi) (half-width
-    // (BRICK+1)/2 about c), so its plane passes within half-width * (|n_x|+|n_y|+|n_z|) of c
-    // (n the plane normal, the third column); 0.01 covers the float error of the plane.
-    const float half = 0.5f * (float)(BRICK + 1) + 0.01f;
+    // A point whose cell touches the brick lies in the reach box [lo, hi) (half-widths
+    // (B+1)/2 about c), so its plane passes within sum_i half_i |n_i| of c (n the plane
+    // normal, the third column); 0.01 covers the float error of the plane.
+    const float hw[3] = {0.5f * (float)(BX + 1), 0.5f * (float)(BY + 1), 0.5f * (float)(BZ + 1)};
     const int w = R + 1;
     const int ntot = a.n * g.nops;
-    const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    const int lane = tid & 63, wv = tid >> 6;
     int qn = 0;   // entries in this wave's queue (wave-uniform)
     // add the corners of the last cnt entries, one entry per lane
     auto drain = [&](int cnt) {
@@ -551,8 +607,8 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
         if (lane < cnt) {
             const int e = qn - cnt + lane;
             const int pk = qpos[wv][e];
-            const int ox = (pk & 15) - 1, oy = ((pk >> 4) & 15) - 1, oz = ((pk >> 8) & 15) - 1;
-            const bool flip = (pk >> 12) & 1;
+            const int ox = (pk & 31) - 1, oy = ((pk >> 5) & 31) - 1, oz = ((pk >> 10) & 31) - 1;
+            const bool flip = (pk >> 15) & 1;
             const float4 vv = a.vals[qval[wv][e]];
             const float ax = qax[wv][e], ay = qay[wv][e], az = qaz[wv][e];
             // values scaled once: (v * scale) * w equals (v * w) * scale for the power of two
@@ -561,13 +617,13 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
             const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay), w01 = (1.f - ax) * ay, w11 = ax * ay;
             const float wz0 = 1.f - az, wz1 = az;
             const float wt[8] = {w00 * wz0, w10 * wz0, w01 * wz0, w11 * wz0, w00 * wz1, w10 * wz1, w01 * wz1, w11 * wz1};
-            const bool inx[2] = {ox >= 0, ox + 1 < BRICK}, iny[2] = {oy >= 0, oy + 1 < BRICK},
-                       inz[2] = {oz >= 0, oz + 1 < BRICK};
-            const int base = (oz * BRICK + oy) * BRICK + ox;
+            const bool inx[2] = {ox >= 0, ox + 1 < BX}, iny[2] = {oy >= 0, oy + 1 < BY},
+                       inz[2] = {oz >= 0, oz + 1 < BZ};
+            const int base = (oz * BY + oy) * BX + ox;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 if (!(inx[j & 1] && iny[(j >> 1) & 1] && inz[j >> 2])) continue;
-                const int li3 = base + (j & 1) + ((j >> 1) & 1) * BRICK + (j >> 2) * BRICK * BRICK;
+                const int li3 = brick_slot(base + (j & 1) + ((j >> 1) & 1) * BX + (j >> 2) * BX * BY);
                 atomicAdd(&sacc[li3], fix64(va * wt[j]));
                 atomicAdd(&sacc[NV + li3], fix64(vb * wt[j]));
                 atomicAdd(&sacc[2 * NV + li3], fix64(vw * wt[j]));
@@ -581,20 +637,21 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
     for (int q0 = 0; q0 < ntot; q0 += 1024) {
         if (tid == 0) nlist = 0;
         __syncthreads();
-        for (int q = q0 + tid; q < min(ntot, q0 + 1024); q += blockDim.x) {
+        for (int q = q0 + tid; q < min(ntot, q0 + 1024); q += NT) {
             const float* m = a.M + (long long)q * 9;
             const float d = fabsf(m[2] * c[0] + m[5] * c[1] + m[8] * c[2]);
-            if (d <= half * (fabsf(m[2]) + fabsf(m[5]) + fabsf(m[8]))) list[atomicAdd(&nlist, 1)] = q;
+            if (d <= (hw[0] + 0.01f) * fabsf(m[2]) + (hw[1] + 0.01f) * fabsf(m[5]) + (hw[2] + 0.01f) * fabsf(m[8]))
+                list[atomicAdd(&nlist, 1)] = q;
         }
         __syncthreads();
         const int nl = nlist;
-        for (int li = wv; li < nl; li += nwv) {
+        for (int li = wv; li < nl; li += NW) {
             const int q = list[li];
             const int p = q / g.nops;
             const float* m = a.M + (long long)q * 9;
             // k = col1 . x over the reach box: its centre value -/+ the half-widths times |col1|
             const float kc = m[1] * c[0] + m[4] * c[1] + m[7] * c[2];
-            const float kh = 0.5f * (float)(BRICK + 1) * (fabsf(m[1]) + fabsf(m[4]) + fabsf(m[7]));
+            const float kh = hw[0] * fabsf(m[1]) + hw[1] * fabsf(m[4]) + hw[2] * fabsf(m[7]);
             const float vmin = kc - kh, vmax = kc + kh;
             // Rows and per-row h intervals of the plane's points in the reach box, with a
             // margin of 0.01 lattice steps over the float error of the bounds (~1e-4); the
@@ -663,68 +720,27 @@ __global__ void __launch_bounds__(256) k_insert_brickq(BrickArgs a) {
                         ox = gx - bx0;
                         oy = gy - by0;
                         oz = gz - bz0;
-                        ok = flip == mirror && ox >= -1 && ox < BRICK && oy >= -1 && oy < BRICK && oz >= -1 &&
-                             oz < BRICK;
+                        ok = flip == mirror && ox >= -1 && ox < BX && oy >= -1 && oy < BY && oz >= -1 && oz < BZ;
                     }
                     const unsigned long long mask = __ballot(ok);
                     if (ok) {
                         const int e = qn + __popcll(mask & ((1ull << lane) - 1ull));
-                        qpos[wv][e] = (ox + 1) | ((oy + 1) << 4) | ((oz + 1) << 8) | ((int)flip << 12);
+                        qpos[wv][e] = (ox + 1) | ((oy + 1) << 5) | ((oz + 1) << 10) | ((int)flip << 15);
                         qval[wv][e] = vi;
                         qax[wv][e] = ax;
                         qay[wv][e] = ay;
                         qaz[wv][e] = az;
                     }
                     qn += __popcll(mask);
+                    if (qn >= 64) drain(64);
                 }
-                while (qn >= 64) drain(64);
             }
         }
         __syncthreads();
     }
     if (qn > 0) drain(qn);
     __syncthreads();
-    brick_flush(a, sacc, bx0, by0, bz0, inv);
-}
-
-__device__ void brick_flush(const BrickArgs& a, const unsigned long long* sacc, int bx0, int by0, int bz0,
-                            double inv) {
-    constexpr int NV = BRICK_NV;
-    const InsertGeom& g = a.g;
-    const int tid = threadIdx.x;
-    const long long dxy = g.dxy;
-    for (int i = tid; i < NV; i += blockDim.x) {
-        const int cx = i % BRICK, cy = (i / BRICK) % BRICK, cz = i / (BRICK * BRICK);
-        const int X = bx0 + cx, Y = by0 + cy, Z = bz0 + cz;
-        if (X >= g.dxy || Y >= g.dxy || Z >= a.dz) continue;
-        const long long sa = (long long)sacc[i], sb = (long long)sacc[NV + i], sw = (long long)sacc[2 * NV + i];
-        if ((sa | sb | sw) == 0) continue;
-        const double da = (double)sa * inv, db = (double)sb * inv, dw = (double)sw * inv;
-        const long long vi = (long long)Z * dxy * dxy + (long long)Y * dxy + X;
-        float4 v = a.acc[vi];
-        if (a.lo) {
-            // compensated: hi + lo carries the sum; TwoSum (Knuth) keeps the rounding error of each add,
-            // and the part of the exact batch sum below float precision goes to lo as well
-            float4 l = a.lo[vi];
-            auto add = [](float& hi, float& lo, double d) {
-                const float y = (float)d;
-                const float s = __fadd_rn(hi, y);
-                const float yy = __fsub_rn(s, hi);
-                const float err = __fadd_rn(__fsub_rn(hi, __fsub_rn(s, yy)), __fsub_rn(y, yy));
-                lo = __fadd_rn(lo, __fadd_rn(err, (float)(d - (double)y)));
-                hi = s;
-            };
-            add(v.x, l.x, da);
-            add(v.y, l.y, db);
-            add(v.z, l.z, dw);
-            a.lo[vi] = l;
-        } else {
-            v.x = (float)((double)v.x + da);
-            v.y = (float)((double)v.y + db);
-            v.z = (float)((double)v.z + dw);
-        }
-        a.acc[vi] = v;
-    }
+    brick_flush<BX, BY, BZ, true>(a, sacc, bx0, by0, bz0, inv);
 }
 
 // ---- host side -----------------------------------------------------------
@@ -782,6 +798,18 @@ extern "C" int cim_icosprocess_sym_ops(const char* sym, double* ops, int* nops) 
 
 namespace cim {
 
+
+// Brick shapes of the queue kernel (x, y, z voxels; workgroup size): CIM_BRICKQ
+// selects one by its index (default 0).
+// Measured at box 384 (maxR 190, 64 particles, compensated): 16x16x8 4.44 ms,
+// 16^3 4.46 ms, 8^3 5.12 ms per launch.
+constexpr int kBrickQ[][4] = {{16, 16, 8, 512}, {16, 16, 16, 1024}, {8, 8, 8, 256}};
+static int brickq_shape() {
+    const char* e = getenv("CIM_BRICKQ");
+    const int n = (int)(sizeof(kBrickQ) / sizeof(kBrickQ[0]));
+    const int v = e ? atoi(e) : 0;
+    return v >= 0 && v < n ? v : 0;
+}
 
 int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init, int nops, const double* ops) {
     if (box < 8 || (box & 1)) return set_error(ctx, CIM_E_ARG, "box %d must be even and >= 8", box);
@@ -853,27 +881,48 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
     r.nsamples = (int)smp.size();
     CIM_ALLOC(r.hxmax, sizeof(int) * hxmax.size());
     CIM_HIP(hipMemcpyAsync(r.hxmax.ptr, hxmax.data(), sizeof(int) * hxmax.size(), hipMemcpyHostToDevice, ctx->stream));
-    // bricks of the accumulator that a sample of radius <= rcut can reach
-    std::vector<int> bl;
-    for (int z = 0; z < r.dz; z += BRICK)
-        for (int y = 0; y < r.dxy; y += BRICK)
-            for (int x = 0; x < r.dxy; x += BRICK) {
-                // nearest point of the brick (relative coordinates) to the origin
-                auto nearest = [&](int a0, int n, int shift) {
-                    const int lo = a0 - shift - 1, hi = a0 - shift + n;
-                    return lo > 0 ? lo : (hi < 0 ? hi : 0);
-                };
-                const double nx = nearest(x, BRICK, r.maxR), ny = nearest(y, BRICK, r.maxR), nz = nearest(z, BRICK, 0);
-                if (std::sqrt(nx * nx + ny * ny + nz * nz) <= rcut + 2.0) {
-                    bl.push_back(x);
-                    bl.push_back(y);
-                    bl.push_back(z);
+    // bricks of the accumulator that a sample of radius <= rcut can reach: 8^3 for the
+    // row kernel, the queue kernel's shape for it
+    auto brick_list = [&](int bx, int by, int bz, bool central_first, DevBuf& buf, int& nb) -> int {
+        std::vector<int> bl;
+        std::vector<std::pair<double, int>> order;
+        for (int z = 0; z < r.dz; z += bz)
+            for (int y = 0; y < r.dxy; y += by)
+                for (int x = 0; x < r.dxy; x += bx) {
+                    // nearest point of the brick's reach (relative coordinates) to the origin
+                    auto nearest = [&](int a0, int n, int shift) {
+                        const int lo = a0 - shift - 1, hi = a0 - shift + n;
+                        return lo > 0 ? lo : (hi < 0 ? hi : 0);
+                    };
+                    const double nx = nearest(x, bx, r.maxR), ny = nearest(y, by, r.maxR), nz = nearest(z, bz, 0);
+                    const double d = std::sqrt(nx * nx + ny * ny + nz * nz);
+                    if (d <= rcut + 2.0) {
+                        order.push_back({d, (int)order.size()});
+                        bl.push_back(x);
+                        bl.push_back(y);
+                        bl.push_back(z);
+                    }
                 }
-            }
-    r.nbricks = (int)(bl.size() / 3);
-    CIM_ALLOC(r.bricks, sizeof(int) * bl.size());
-    CIM_HIP(hipMemcpyAsync(r.bricks.ptr, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice, ctx->stream));
-    CIM_HIP(hipStreamSynchronize(ctx->stream));
+        // central bricks meet the most planes: starting them first shortens the tail
+        // of the launch (the bricks are disjoint, so the order changes no sum)
+        if (central_first) std::stable_sort(order.begin(), order.end());
+        std::vector<int> ol(bl.size());
+        for (size_t i = 0; i < order.size(); ++i)
+            for (int j = 0; j < 3; ++j) ol[3 * i + j] = bl[3 * order[i].second + j];
+        nb = (int)order.size();
+        CIM_ALLOC(buf, sizeof(int) * std::max<size_t>(ol.size(), 3));
+        CIM_HIP(hipMemcpyAsync(buf.ptr, ol.data(), sizeof(int) * ol.size(), hipMemcpyHostToDevice, ctx->stream));
+        CIM_HIP(hipStreamSynchronize(ctx->stream));
+        return CIM_OK;
+    };
+    int rc = brick_list(BRICK, BRICK, BRICK, false, r.bricks, r.nbricks);
+    if (rc != CIM_OK) return rc;
+    const int qs = brickq_shape();
+    const char* ord = getenv("CIM_BRICK_ORDER");
+    const bool central = !(ord && strcmp(ord, "raster") == 0);
+    rc = brick_list(kBrickQ[qs][0], kBrickQ[qs][1], kBrickQ[qs][2], central, r.bricksq, r.nbricksq);
+    if (rc != CIM_OK) return rc;
+    r.qshape = qs;
     CIM_ALLOC(r.samples, sizeof(int2) * smp.size());
     CIM_HIP(hipMemcpyAsync(r.samples.ptr, smp.data(), sizeof(int2) * smp.size(), hipMemcpyHostToDevice, ctx->stream));
     CIM_HIP(hipStreamSynchronize(ctx->stream));
@@ -935,7 +984,15 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
                           r.maxR > 512;
         TimedLaunch tl(ctx, "insert");
         if (rows) hipLaunchKernelGGL(k_insert_brick, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
-        else hipLaunchKernelGGL(k_insert_brickq, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
+        else {
+            ba.bricks = r.bricksq.as<int>();
+            if (r.qshape == 0)
+                hipLaunchKernelGGL((k_insert_brickq<16, 16, 8, 512>), dim3(r.nbricksq), dim3(512), 0, ctx->stream, ba);
+            else if (r.qshape == 1)
+                hipLaunchKernelGGL((k_insert_brickq<16, 16, 16, 1024>), dim3(r.nbricksq), dim3(1024), 0, ctx->stream, ba);
+            else
+                hipLaunchKernelGGL((k_insert_brickq<8, 8, 8, 256>), dim3(r.nbricksq), dim3(256), 0, ctx->stream, ba);
+        }
         CIM_HIP(hipGetLastError());
     }
     r.inserted += n;

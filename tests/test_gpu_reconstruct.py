@@ -171,7 +171,10 @@ def test_queue_kernel_equals_row_kernel(ctx, insert_kernel, nops, layout):
     """The brick gather packed through per-wave queues (default) and the
     row-by-row brick gather (CIM_INSERT=brickrows) add the same fixed-point
     terms into exact sums: the same bits, over three batches (the second
-    spans two 1024-plane chunks)."""
+    spans two 1024-plane chunks), for every brick shape of the queue kernel
+    (CIM_BRICKQ: 16x16x8, 16^3, 8^3; the 16-voxel bricks overhang the
+    118 x 118 x 60 accumulator) and in raster as well as central-first brick
+    order."""
     if insert_kernel != "brick":
         pytest.skip("compares the two brick kernels once")
     box, maxR = 128, 58
@@ -185,17 +188,26 @@ def test_queue_kernel_equals_row_kernel(ctx, insert_kernel, nops, layout):
     q *= np.sign(np.linalg.det(q))[:, None, None]
     ops = None if nops == 60 else q
     res = {}
-    for kern in ("brick", "brickrows"):
-        os.environ["CIM_INSERT"] = kern
-        ctx.rec_begin(box, maxR, layout=layout, rcut=maxR - 2.0, wg_init=0.0, ops=ops)
-        ctx.rec_insert_ft(re[:3], im[:3], eul[:3], ctf[:3])
-        ctx.rec_insert_ft(re[3:], im[3:], eul[3:], ctf[3:])
-        ctx.rec_insert_ft(re[:1], im[:1], eul[:1], ctf[:1])
-        res[kern] = ctx.rec_get_tnf(maxR)
-    os.environ["CIM_INSERT"] = "brick"
-    for a, b in zip(res["brick"], res["brickrows"]):
-        assert np.array_equal(a, b)
-    assert np.abs(res["brick"][2]).max() > 0
+    runs = [("brickrows", "0", "central")] + [("brick", s, o) for s in ("0", "1", "2") for o in ("central", "raster")]
+    try:
+        for kern, shape, order in runs:
+            os.environ["CIM_INSERT"] = kern
+            os.environ["CIM_BRICKQ"] = shape
+            os.environ["CIM_BRICK_ORDER"] = order
+            ctx.rec_begin(box, maxR, layout=layout, rcut=maxR - 2.0, wg_init=0.0, ops=ops)
+            ctx.rec_insert_ft(re[:3], im[:3], eul[:3], ctf[:3])
+            ctx.rec_insert_ft(re[3:], im[3:], eul[3:], ctf[3:])
+            ctx.rec_insert_ft(re[:1], im[:1], eul[:1], ctf[:1])
+            res[(kern, shape, order)] = ctx.rec_get_tnf(maxR)
+    finally:
+        os.environ["CIM_INSERT"] = "brick"
+        os.environ.pop("CIM_BRICKQ", None)
+        os.environ.pop("CIM_BRICK_ORDER", None)
+    ref = res[runs[0]]
+    assert np.abs(ref[2]).max() > 0
+    for key in runs[1:]:
+        for a, b in zip(res[key], ref):
+            assert np.array_equal(a, b), key
 
 
 def test_compensated_sums_do_not_drift(ctx, insert_kernel):
