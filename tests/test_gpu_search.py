@@ -90,6 +90,40 @@ def test_pr_array_matches_oracle(ctx, O64, ctf, variant):
     assert err.max() < 0.5
 
 
+@pytest.mark.parametrize("pair,step", [(True, 1.0), (False, 0.5)])
+def test_launch_shapes_give_the_same_bits(ctx, monkeypatch, pair, step):
+    """Every launch shape of the exact kernel (columns per lane 1-4, particle
+    taps from LDS or through the caches) computes each column with the same
+    arithmetic in the same order, so the PR arrays are bitwise equal: the
+    tabulated path re-scores with one shape and the plain path uses whichever
+    the autotuner picks, and their winners must agree exactly.  The shapes
+    tried are those the autotuner offers for the grid (a column count per lane
+    that gives the same workgroup size as one less is not offered)."""
+    box, minR, maxR, apix = 64, 1, 12, 2.0
+    mask_r = box / 2 - 4
+    ph, tang, tsh, timgs, ps = _setup(box, maxR, minR, T=3, n=1, apix=apix, seed=44)
+    p = lib.make_params(box, minR, maxR, apply_ctf=True, mask_on=True, apix=apix, imgmask=mask_r,
+                        searchstep=step)
+    Nt, Np, No = geometry.search_grid(step)
+    if not pair:
+        monkeypatch.setenv("CIM_SEARCH_NOPAIR", "1")
+    ncol = No // 2 if pair else No
+    block = lambda v: (-(-(-(-ncol // v)) // 64)) * 64
+    vpts = [v for v in range(1, 5) if v == 1 or block(v) != block(v - 1)]
+    res = {}
+    for vpt in vpts:
+        for lds in ("0", "1"):
+            monkeypatch.setenv("CIM_SEARCH_VPT", str(vpt))
+            monkeypatch.setenv("CIM_SEARCH_LDS", lds)
+            ctx.search_templates(p, timgs, np.concatenate([tang, tsh], axis=1))
+            res[(vpt, lds)] = ctx.search_pr_array(p, ps.images[0], _parts(ps)[0], (Nt + 1, 2 * Np + 1, No))
+    assert len(res) >= 6
+    ref = res[(1, "0")]
+    assert (ref < 999.5).any()
+    for k, g in res.items():
+        assert np.array_equal(g, ref), (k, int((g != ref).sum()))
+
+
 def test_global_search_argmin(ctx, O64):
     box, minR, maxR, step, apix = 64, 1, 12, 3.0, 2.0
     mask_r = box / 2 - 4
