@@ -73,7 +73,7 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64, help="particles per step")
+    ap.add_argument("--batch", type=int, default=512, help="particles per step")
     ap.add_argument("--templates", type=int, default=4, help="template bank size T")
     ap.add_argument("--topk", type=int, default=32, help="AU points re-scored exactly per particle")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -1105,8 +1105,13 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     if (const char* ew = getenv("CIM_TABLE_WINDOW")) wincols = atoi(ew) > 0 ? ((atoi(ew) + 63) / 64) * 64 : ncolx;
     if (wincols > ncolx) wincols = ncolx;
 
-    // the tables take 4 T npsi^2 bytes per particle (8.3 MB at T = 4, 0.5 deg): batches of 64
-    const int batch = std::max(1, std::min(n, tables ? 64 : 256));
+    // the tables take 4 T npsi^2 bytes per particle (8.3 MB at T = 4, 0.5 deg): batches of
+    // CIM_TABLE_BATCH (default 512, 4.2 GB) particles.  Large batches keep enough waves of
+    // the one-wave re-score workgroups resident (32 per particle): measured at box 384,
+    // the re-score takes 5.5 ms per 64 particles in batches of 64 and 2.75 ms in batches of 512
+    int tbatch = 512;
+    if (const char* eb = getenv("CIM_TABLE_BATCH")) tbatch = std::max(1, std::min(1024, atoi(eb)));
+    const int batch = std::max(1, std::min(n, tables ? tbatch : 256));
     CtfConst cc = make_ctf_const(p);
 
     // coarse-to-fine table scores: the coarse points (every other theta and phi index,
