@@ -634,8 +634,8 @@ def _roofline(r, args):
                                    "x 2 (hi and lo sums)", "avg_launch_ms": round(ms, 3), "launches": launches,
                      "note": "frac above 1: the rule counts one HBM read-modify-write per corner update (an atomic "
                              "scatter); the brick gather sums a batch in LDS and touches each accumulator voxel "
-                             "once, so its measured traffic is about a tenth of the rule's bytes and the kernel is "
-                             "bound by VALU issue and LDS atomics instead"}
+                             "once, so its measured traffic is a small part of the rule's bytes and the kernel is "
+                             "limited by its VALU work and LDS atomics (and their latencies) instead"}
     if dom in ("cl_rescore", "cl_search", "cl_fallback"):
         samples = cnt["cl_search_samples"] / launches
         achieved = samples * FLOPS_PER_SAMPLE / (ms / 1e3) / 1e12
