@@ -1,6 +1,6 @@
 // Device building blocks of the batched FFTs (fft.hip) shared with kernels
-// that run transforms inside their own workgroups: mixed-radix (4, 2, 3, 5)
-// self-sorting Stockham passes over L lines held in LDS.
+// that run transforms inside their own workgroups: mixed-radix (8, 4, 6, 2, 3,
+// 5) self-sorting Stockham passes over L lines held in LDS.
 #pragma once
 #include "cim_internal.h"
 
@@ -55,7 +55,51 @@ __device__ void stockham_pass(const float2* __restrict__ in, float2* __restrict_
             }
         }
         float2 y[R];
-        if (R == 2) {
+        if (R == 8) {
+            // two 4-point DFTs (even and odd inputs), joined by w^k, w = exp(sgn 2 pi i / 8)
+            const float c = 0.70710678118654752440f;
+            float2 e[4], o[4];
+            {
+                float2 a0 = cadd(v[0], v[4]), a1 = csub(v[0], v[4]);
+                float2 b0 = cadd(v[2], v[6]), b1 = cmuli(csub(v[2], v[6]), sgn);
+                e[0] = cadd(a0, b0); e[2] = csub(a0, b0); e[1] = cadd(a1, b1); e[3] = csub(a1, b1);
+            }
+            {
+                float2 a0 = cadd(v[1], v[5]), a1 = csub(v[1], v[5]);
+                float2 b0 = cadd(v[3], v[7]), b1 = cmuli(csub(v[3], v[7]), sgn);
+                o[0] = cadd(a0, b0); o[2] = csub(a0, b0); o[1] = cadd(a1, b1); o[3] = csub(a1, b1);
+            }
+            // w = c (1 + i sgn), w^2 = i sgn, w^3 = c (-1 + i sgn)
+            const float2 t1 = make_float2(c * (o[1].x - sgn * o[1].y), c * (o[1].y + sgn * o[1].x));
+            const float2 t2 = cmuli(o[2], sgn);
+            const float2 t3 = make_float2(c * (-o[3].x - sgn * o[3].y), c * (-o[3].y + sgn * o[3].x));
+            y[0] = cadd(e[0], o[0]); y[4] = csub(e[0], o[0]);
+            y[1] = cadd(e[1], t1); y[5] = csub(e[1], t1);
+            y[2] = cadd(e[2], t2); y[6] = csub(e[2], t2);
+            y[3] = cadd(e[3], t3); y[7] = csub(e[3], t3);
+        } else if (R == 6) {
+            // two 3-point DFTs (even and odd inputs), joined by w^k, w = exp(sgn pi i / 3)
+            const float h = 0.86602540378443864676f;
+            float2 e[3], o[3];
+            {
+                float2 t1 = cadd(v[2], v[4]);
+                float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+                float2 t3 = cmuli(cscale(csub(v[2], v[4]), h), sgn);
+                e[0] = cadd(v[0], t1); e[1] = cadd(t2, t3); e[2] = csub(t2, t3);
+            }
+            {
+                float2 t1 = cadd(v[3], v[5]);
+                float2 t2 = make_float2(v[1].x - 0.5f * t1.x, v[1].y - 0.5f * t1.y);
+                float2 t3 = cmuli(cscale(csub(v[3], v[5]), h), sgn);
+                o[0] = cadd(v[1], t1); o[1] = cadd(t2, t3); o[2] = csub(t2, t3);
+            }
+            // w = 1/2 + i sgn h, w^2 = -1/2 + i sgn h
+            const float2 t1 = make_float2(0.5f * o[1].x - sgn * h * o[1].y, 0.5f * o[1].y + sgn * h * o[1].x);
+            const float2 t2 = make_float2(-0.5f * o[2].x - sgn * h * o[2].y, -0.5f * o[2].y + sgn * h * o[2].x);
+            y[0] = cadd(e[0], o[0]); y[3] = csub(e[0], o[0]);
+            y[1] = cadd(e[1], t1); y[4] = csub(e[1], t1);
+            y[2] = cadd(e[2], t2); y[5] = csub(e[2], t2);
+        } else if (R == 2) {
             y[0] = cadd(v[0], v[1]);
             y[1] = csub(v[0], v[1]);
         } else if (R == 4) {
@@ -104,7 +148,9 @@ __device__ __forceinline__ float2* lds_fft(float2* b0, float2* b1, int n, int L,
     for (int f = 0; f < fl.nfac; ++f) {
         const int R = fl.fac[f];
         const unsigned nsm = fl.ns_m[f], nbm = fl.nb_m[f];
-        if (R == 4) stockham_pass<4>(in, out, n, L, Ns, nsm, nbm, tw, sgn);
+        if (R == 8) stockham_pass<8>(in, out, n, L, Ns, nsm, nbm, tw, sgn);
+        else if (R == 4) stockham_pass<4>(in, out, n, L, Ns, nsm, nbm, tw, sgn);
+        else if (R == 6) stockham_pass<6>(in, out, n, L, Ns, nsm, nbm, tw, sgn);
         else if (R == 2) stockham_pass<2>(in, out, n, L, Ns, nsm, nbm, tw, sgn);
         else if (R == 3) stockham_pass<3>(in, out, n, L, Ns, nsm, nbm, tw, sgn);
         else stockham_pass<5>(in, out, n, L, Ns, nsm, nbm, tw, sgn);

@@ -75,7 +75,10 @@ int fft_plan(cim_ctx* ctx, int n, FftPlan** out) {
     FftPlan pl;
     pl.n = n;
     int m = n;
+    // fewest passes: radix 8 first (384 = 8 x 8 x 6: three passes through LDS)
+    while (m % 8 == 0) { pl.fac[pl.nfac++] = 8; m /= 8; }
     while (m % 4 == 0) { pl.fac[pl.nfac++] = 4; m /= 4; }
+    while (m % 6 == 0) { pl.fac[pl.nfac++] = 6; m /= 6; }
     while (m % 2 == 0) { pl.fac[pl.nfac++] = 2; m /= 2; }
     while (m % 3 == 0) { pl.fac[pl.nfac++] = 3; m /= 3; }
     while (m % 5 == 0) { pl.fac[pl.nfac++] = 5; m /= 5; }

@@ -9,7 +9,7 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [8, 12, 20, 24, 64, 96, 128, 384, 512])
+@pytest.mark.parametrize("n", [6, 8, 12, 20, 24, 48, 64, 96, 128, 384, 512])
 @pytest.mark.parametrize("conj", [False, True])
 def test_fft2d_forward(ctx, n, conj):
     rng = np.random.default_rng(n)
