@@ -567,8 +567,8 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
 // at the level of the small bricks.
 constexpr int QCAP = 128;   // < 64 left over + 64 lanes: drained after every candidate pass
 
-template <int BX, int BY, int BZ, int NT>
-__global__ void __launch_bounds__(NT) k_insert_brickq(BrickArgs a) {
+template <int BX, int BY, int BZ, int NT, int WPE = 1>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_insert_brickq(BrickArgs a) {
     static_assert(BX <= 16 && BY <= 16 && BZ <= 16 && NT % 64 == 0 && NT <= 1024, "brick shape");
     constexpr int NV = BX * BY * BZ, NW = NT / 64;
     __shared__ unsigned long long sacc[3 * NV];
@@ -803,7 +803,12 @@ namespace cim {
 // selects one by its index (default 0).
 // Measured at box 384 (maxR 190, 64 particles, compensated): 16x16x8 4.44 ms,
 // 16^3 4.46 ms, 8^3 5.12 ms per launch.
-constexpr int kBrickQ[][4] = {{16, 16, 8, 512}, {16, 16, 16, 1024}, {8, 8, 8, 256}};
+// Shapes 3 and 4 (16x8x8, 53 KB and 48 KB of LDS) let three workgroups share a
+// CU: shape 3 asks the compiler for 6 waves per SIMD (78 VGPRs, no spills;
+// shape 0 holds 87 and is held to 4 waves per SIMD by its 77 KB of LDS).
+// Measured at box 384 (maxR 190, 512 particles, compensated, same bits):
+// 16x16x8 34.8 ms, 16x8x8 at 512 threads 33.8 ms, at 384 threads 48.0 ms.
+constexpr int kBrickQ[][4] = {{16, 16, 8, 512}, {16, 16, 16, 1024}, {8, 8, 8, 256}, {16, 8, 8, 512}, {16, 8, 8, 384}};
 static int brickq_shape() {
     const char* e = getenv("CIM_BRICKQ");
     const int n = (int)(sizeof(kBrickQ) / sizeof(kBrickQ[0]));
@@ -990,8 +995,12 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
                 hipLaunchKernelGGL((k_insert_brickq<16, 16, 8, 512>), dim3(r.nbricksq), dim3(512), 0, ctx->stream, ba);
             else if (r.qshape == 1)
                 hipLaunchKernelGGL((k_insert_brickq<16, 16, 16, 1024>), dim3(r.nbricksq), dim3(1024), 0, ctx->stream, ba);
-            else
+            else if (r.qshape == 2)
                 hipLaunchKernelGGL((k_insert_brickq<8, 8, 8, 256>), dim3(r.nbricksq), dim3(256), 0, ctx->stream, ba);
+            else if (r.qshape == 3)
+                hipLaunchKernelGGL((k_insert_brickq<16, 8, 8, 512, 6>), dim3(r.nbricksq), dim3(512), 0, ctx->stream, ba);
+            else
+                hipLaunchKernelGGL((k_insert_brickq<16, 8, 8, 384>), dim3(r.nbricksq), dim3(384), 0, ctx->stream, ba);
         }
         CIM_HIP(hipGetLastError());
     }

@@ -172,7 +172,7 @@ def test_queue_kernel_equals_row_kernel(ctx, insert_kernel, nops, layout):
     row-by-row brick gather (CIM_INSERT=brickrows) add the same fixed-point
     terms into exact sums: the same bits, over three batches (the second
     spans two 1024-plane chunks), for every brick shape of the queue kernel
-    (CIM_BRICKQ: 16x16x8, 16^3, 8^3; the 16-voxel bricks overhang the
+    (CIM_BRICKQ: 16x16x8, 16^3, 8^3, 16x8x8 at 512 and 384 threads; the 16-voxel bricks overhang the
     118 x 118 x 60 accumulator) and in raster as well as central-first brick
     order."""
     if insert_kernel != "brick":
@@ -188,7 +188,8 @@ def test_queue_kernel_equals_row_kernel(ctx, insert_kernel, nops, layout):
     q *= np.sign(np.linalg.det(q))[:, None, None]
     ops = None if nops == 60 else q
     res = {}
-    runs = [("brickrows", "0", "central")] + [("brick", s, o) for s in ("0", "1", "2") for o in ("central", "raster")]
+    runs = [("brickrows", "0", "central")] + [("brick", s, o) for s in ("0", "1", "2", "3", "4")
+                                                for o in ("central", "raster")]
     try:
         for kern, shape, order in runs:
             os.environ["CIM_INSERT"] = kern
