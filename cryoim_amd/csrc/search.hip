@@ -1118,7 +1118,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     // plus the AU border points no such point neighbours) first, then the unscored 3x3
     // neighbours of each particle's coarseM best.  CIM_TABLE_COARSE=<M>; 0 scores every point.
     int coarseM = 64;
-    if (const char* ec = getenv("CIM_TABLE_COARSE")) coarseM = std::max(0, atoi(ec));
+    // (k_topk_scan keeps the chosen points in a 256-entry LDS array)
+    if (const char* ec = getenv("CIM_TABLE_COARSE")) coarseM = std::max(0, std::min(256, atoi(ec)));
     std::vector<int> clist, nbrv;
     bool c2f = false;
     if (tables && coarseM > 0 && npts <= 8192) {

@@ -1,0 +1,224 @@
+"""Time the reference's own programs (oracle/_ref/, built from the reference
+sources by oracle/build_ref.sh) on this container's host cores, on bounded
+samples of the box-384 workloads of bench.py, and write the rates to
+profiles/cpu_reference_box384.json.  bench.py quotes that file beside the C
+oracle port it times itself; the reference never runs where bench.py runs
+on a GPU.
+
+Each program runs as N concurrent single-threaded processes (MKL and OpenMP
+on one thread each) over disjoint particle ranges: the reference's only
+parallel mode.
+
+Usage: python oracle/time_reference.py [--stages main center lbox]
+"""
+from __future__ import annotations
+
+import argparse
+import datetime
+import json
+import os
+import platform
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+REF = os.path.join(HERE, "_ref")
+
+import bench  # noqa: E402
+from bench import APIX, BOX, INSERT_MAXR, LBOX_SUB, LBOX_Z, MAXRES, MINRES, STEP  # noqa: E402
+
+OUT = os.path.join(ROOT, "profiles", "cpu_reference_box384.json")
+
+
+def center_cpu_baseline(ph, ps):
+    """Reference icosprocess mode 3 with model3d on the host cores: the cost
+    per particle is the time of K particles minus that of 1, per process
+    (the difference removes each process's model read and 3D FFT)."""
+    from cryoim_amd import formats, synth
+    exe = os.path.join(REF, "icosprocess")
+    if not os.path.exists(exe):
+        return None
+    d = bench.derived()
+    ncores = min(8, os.cpu_count() or 1)
+    m = len(ps.images)
+    K = 41
+    with tempfile.TemporaryDirectory() as tmp:
+        formats.write_mrc(os.path.join(tmp, "stk.mrc"), ps.images)
+        formats.write_mrc(os.path.join(tmp, "model.mrc"), synth.phantom_map(ph))
+        parts = [formats.Particle("stk.mrc", i + 1, *ps.angles[i % m], *ps.shifts[i % m], 0.0, i + 1, (i % m) + 1,
+                                  ps.df[i % m, 0], ps.df[i % m, 1], 0.0) for i in range(K * ncores)]
+        formats.write_ortfile(os.path.join(tmp, "ort.txt"), parts)
+        t = {}
+        for k in (1, K):
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([exe, "mode=3", "model3d=model.mrc", f"newortfile=c{i}.txt",
+                                       f"first={i * K + 1}", f"last={i * K + k}", f"apix={APIX}",
+                                       f"maxRes={MAXRES}", f"imgmask={d['mask'] * APIX}", "ortfile=ort.txt"],
+                                      cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=_ref_env())
+                     for i in range(ncores)]
+            rcs = [pr.wait() for pr in procs]
+            t[k] = time.perf_counter() - t0
+            if any(rcs):
+                return None
+    per = (t[K] - t[1]) / (K - 1)
+    return {"value": round(ncores / per, 3), "unit": "particles/s", "cores": ncores, "kind": "reference",
+            "sample": (f"icosprocess mode 3 (flang -O2, MKL FFTW) as {ncores} concurrent processes, {K} particles "
+                       f"each ({t[K]:.1f} s) minus 1 each ({t[1]:.1f} s: model read and 3D FFT)")}
+
+
+
+
+def lbox_cpu_baseline(ps):
+    """Reference local_box stack mode on the host cores: the cost per
+    particle is the time of K particles minus that of 1, per process (the
+    difference removes each process's stack read and start-up)."""
+    from cryoim_amd import formats
+    exe = os.path.join(REF, "local_box")
+    if not os.path.exists(exe):
+        return None
+    ncores = min(8, os.cpu_count() or 1)
+    m = len(ps.images)
+    K = 121
+    ops = bench.vertex_ops().astype(np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        formats.write_mrc(os.path.join(tmp, "stk.mrc"), ps.images)
+        parts = [formats.Particle("stk.mrc", i + 1, *ps.angles[i % m], *ps.shifts[i % m], 0.0, i + 1, (i % m) + 1,
+                                  ps.df[i % m, 0], ps.df[i % m, 1], 0.0) for i in range(K * ncores)]
+        formats.write_ortfile(os.path.join(tmp, "ort.txt"), parts)
+        with open(os.path.join(tmp, "v12.txt"), "w") as f:
+            for S in ops:
+                for col in S.T:
+                    f.write(" ".join(f"{float(v):.9g}" for v in col) + "\n")
+        t = {}
+        for k in (1, K):
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([exe, "icos_ort=ort.txt", f"local_ort=l{i}.txt", f"local_imgstck=s{i}.mrc",
+                                       f"sub_FFTsize={LBOX_SUB}", "sym_matrix=v12.txt", "local_x=0", "local_y=0",
+                                       f"local_z={LBOX_Z}", f"apix={APIX}", f"first={i * K + 1}",
+                                       f"last={i * K + k}"], cwd=tmp, stdout=subprocess.DEVNULL,
+                                      stderr=subprocess.DEVNULL, env=_ref_env())
+                     for i in range(ncores)]
+            rcs = [pr.wait() for pr in procs]
+            t[k] = time.perf_counter() - t0
+            if any(rcs):
+                return None
+    per = (t[K] - t[1]) / (K - 1)
+    return {"value": round(ncores / per, 3), "unit": "particles/s", "cores": ncores, "kind": "reference",
+            "sample": (f"local_box stack mode (flang -O2, MKL FFTW) as {ncores} concurrent processes, {K} "
+                       f"particles each ({t[K]:.1f} s) minus 1 each ({t[1]:.1f} s: stack read, start-up)")}
+
+
+
+
+def _ref_env():
+    """Environment of the reference processes: MKL and OpenMP on one thread
+    each, so that N concurrent processes use exactly N host cores."""
+    env = dict(os.environ)
+    env.update(MKL_NUM_THREADS="1", OMP_NUM_THREADS="1", MKL_DYNAMIC="FALSE")
+    return env
+
+
+
+
+def _ref_baseline(exe, args, tang, timgs, ps, ncores):
+    from cryoim_amd import formats
+    d = bench.derived()
+    coarse = 4.0
+    m = len(ps.images)
+    with tempfile.TemporaryDirectory() as tmp:
+        formats.write_mrc(os.path.join(tmp, "stk.mrc"), ps.images)
+        formats.write_mrc(os.path.join(tmp, "tmpl.mrc"), timgs)
+        formats.write_template_ort(os.path.join(tmp, "tort.txt"), tang)
+        parts = [formats.Particle("stk.mrc", i + 1, 0.0, 0.0, 0.0, *ps.shifts[i % m], 0.0, i + 1, (i % m) + 1,
+                                  ps.df[i % m, 0], ps.df[i % m, 1], 0.0) for i in range(ncores)]
+        formats.write_ortfile(os.path.join(tmp, "ort.txt"), parts)
+        common = [f"apix={APIX}", f"maxRes={MAXRES}", f"minRes={MINRES}", f"imgmask={d['mask'] * APIX}",
+                  "ortfile=ort.txt"]
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([exe, "mode=4", "templateort=tort.txt", "templatestck=tmpl.mrc",
+                                   f"searchstep={coarse}", f"newortfile=o{i}.txt", f"first={i + 1}",
+                                   f"last={i + 1}", *common], cwd=tmp, stdout=subprocess.DEVNULL,
+                                  stderr=subprocess.DEVNULL, env=_ref_env()) for i in range(ncores)]
+        rcs = [p.wait() for p in procs]
+        t_search = time.perf_counter() - t0
+        if any(rcs):
+            raise RuntimeError(f"icosprocess mode 4 exit codes {rcs}")
+        # insert: reconstruct_substep at maxR 190 (maxRes = apix*box/190), 2 particles per process
+        maxres_ins = APIX * BOX / INSERT_MAXR
+        nins = 2
+        parts = [formats.Particle("stk.mrc", i + 1, *ps.angles[i % m], *ps.shifts[i % m], 0.0,
+                                  i + 1, (i % m) + 1, ps.df[i % m, 0], ps.df[i % m, 1], 0.0)
+                 for i in range(ncores * nins)]
+        formats.write_ortfile(os.path.join(tmp, "ort2.txt"), parts)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([exe, "mode=1", "subrec=y", f"result3d=s{i}.tnf", f"first={i * nins + 1}",
+                                   f"last={(i + 1) * nins}", f"apix={APIX}", f"maxRes={maxres_ins}",
+                                   f"imgmask={d['mask'] * APIX}", "ortfile=ort2.txt"], cwd=tmp,
+                                  stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=_ref_env())
+                 for i in range(ncores)]
+        rcs = [p.wait() for p in procs]
+        t_ins = time.perf_counter() - t0
+        if any(rcs):
+            raise RuntimeError(f"icosprocess mode 1 exit codes {rcs}")
+    scale = bench._grid_work(STEP) / bench._grid_work(coarse)
+    per_particle_core = t_search * scale + t_ins / nins      # seconds per particle on one core
+    rate = ncores / per_particle_core
+    return {"value": rate, "unit": "particles/s", "cores": ncores, "kind": "reference",
+            "sample": (f"icosprocess (flang -O2, MKL FFTW) as {ncores} concurrent processes: mode 4 on 1 particle "
+                       f"each at searchstep {coarse} ({t_search:.1f} s, x{scale:.0f} to the {STEP} deg grid by "
+                       f"AU-cell x omega count) + mode 1 subrec at maxR {INSERT_MAXR} on {nins} particles each "
+                       f"({t_ins:.1f} s)")}
+
+
+
+
+def _host():
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": cpu, "logical_cpus": os.cpu_count(), "machine": platform.machine()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stages", nargs="*", default=["main", "center", "lbox"])
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--templates", type=int, default=4)
+    args = ap.parse_args()
+    if not os.path.exists(os.path.join(REF, "icosprocess")):
+        raise SystemExit("oracle/_ref is not built; run oracle/build_ref.sh first")
+    res = json.load(open(OUT)) if os.path.exists(OUT) else {}
+    ph, tang, timgs, ps = bench.make_workload(args.batch, args.templates)
+    ncores = min(8, os.cpu_count() or 1)
+    stamp = datetime.date.today().isoformat()
+    if "main" in args.stages:
+        r = _ref_baseline(os.path.join(REF, "icosprocess"), args, tang, timgs, ps, ncores)
+        res["main"] = dict(r, date=stamp, host=_host())
+        print("main", res["main"], flush=True)
+    if "center" in args.stages:
+        r = center_cpu_baseline(ph, ps)
+        res["center"] = dict(r, date=stamp, host=_host())
+        print("center", res["center"], flush=True)
+    if "lbox" in args.stages:
+        r = lbox_cpu_baseline(ps)
+        res["lbox"] = dict(r, date=stamp, host=_host())
+        print("lbox", res["lbox"], flush=True)
+    with open(OUT, "w") as f:
+        json.dump(res, f, indent=1)
+    print(OUT)
+
+
+if __name__ == "__main__":
+    main()

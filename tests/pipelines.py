@@ -225,3 +225,62 @@ def lref_particle(O, img_full, box, hx0, ky0, x0, y0, mask_r, maxres):
     fa, fb = O.fft2(a, np.zeros_like(a), 1, False)
     w = lref_wlp(box, maxres)
     return fa * w, fb * w, w
+
+
+# ---- workloads of the measured configurations --------------------------------------
+
+def config1_workload(n, T=4, seed=101):
+    """Box 128, 1.5 A/px: the 20-blob 60-fold phantom (sigma U(2, 4) px, radius
+    U(40, 55) px), T noise-free templates at random AU orientations, and n
+    particles at uniform AU orientations with +-3 px shifts, defocus U(1, 3) um
+    (300 kV, Cs 2.7 mm, A 0.1) and Gaussian noise at SNR 0.1."""
+    from cryoim_amd import synth
+    box, apix = 128, 1.5
+    ph = synth.make_phantom(box, nblobs=20, sigma=(2.0, 4.0), radius=(40.0, 55.0), seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    tang = synth.random_au_orientations(T, rng)
+    timgs = synth.project(ph, tang)
+    ps = synth.make_particles(ph, n, seed=seed + 2, shift_max=3.0, df_range=(1.0, 3.0), snr=0.1, apix=apix,
+                              mask_radius=box / 2 - 4)
+    d = derived(box, apix, maxRes=8.0, minRes=300.0, imgmask_A=(box / 2 - 4) * apix)
+    return dict(box=box, apix=apix, phantom=ph, tang=tang, timgs=timgs, ps=ps, d=d, step=1.0,
+                insert_maxR=box // 2 - 2)
+
+
+def box384_workload(n, T=4, seed=1):
+    """The box-384 workload of the throughput measurement: phantom radii and
+    sigmas x3, defocus U(0.8, 3.5) um, SNR 0.05, 0.5 deg grid, maxR 72."""
+    from cryoim_amd import synth
+    box, apix = 384, 1.5
+    s = box / 128.0
+    ph = synth.make_phantom(box, nblobs=20, sigma=(2.0 * s, 4.0 * s), radius=(40.0 * s, 55.0 * s), seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    tang = synth.random_au_orientations(T, rng)
+    timgs = synth.project(ph, tang)
+    ps = synth.make_particles(ph, n, seed=seed + 2, shift_max=3.0, df_range=(0.8, 3.5), snr=0.05, apix=apix,
+                              mask_radius=box / 2 - 4)
+    d = derived(box, apix, maxRes=8.0, minRes=300.0, imgmask_A=(box / 2 - 4) * apix)
+    return dict(box=box, apix=apix, phantom=ph, tang=tang, timgs=timgs, ps=ps, d=d, step=0.5, insert_maxR=190)
+
+
+def oracle_search_one(O, w, i):
+    """The oracle's ort_globesearch for particle i of a workload: templates
+    masked, transformed, multiplied by the particle's CTF with Fcut refitted;
+    the particle masked at its own centre (ort_globesearch.f90:50), Fcut
+    before the centre shift.  Returns theta, phi, omega, PR."""
+    box, apix, d, ps = w["box"], w["apix"], w["d"], w["ps"]
+    T = len(w["tang"])
+    ta, tb = templates(O, w["timgs"], np.zeros((T, 2)), d["mask"])
+    ctf = O.ctf2d(box, ps.df[i, 0], ps.df[i, 1], 0.0, apix)
+    a, b, tf = template_inputs(O, ta, tb, ctf, d["minR"], d["maxR"])
+    _, (pa, pb), fc0 = particle_inputs(O, ps.images[i], *ps.shifts[i], d["mask"], d["minR"], d["maxR"])
+    return np.asarray(O.globesearch(pa, pb, fc0, a, b, tf, w["tang"], d["minR"], d["maxR"], w["step"]), np.float64)
+
+
+def angle_steps(a, b, step):
+    """Largest difference of theta, phi, omega (degrees, omega modulo 360)
+    between two sets of orientations, in grid steps."""
+    a = np.asarray(a, np.float64)[:, :3]
+    b = np.asarray(b, np.float64)[:, :3]
+    d = np.abs(((a - b + 180.0) % 360.0) - 180.0)
+    return d.max(axis=1) / step
