@@ -125,6 +125,10 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->rec.hxmax.release();
     ctx->rec.vals.release();
     ctx->rec.maxv.release();
+    ctx->rec.cube.release();
+    ctx->rec.cube_lo.release();
+    ctx->rec.cops.release();
+    ctx->rec.bricksc.release();
     ctx->srch.tft.release();
     ctx->srch.tfcut.release();
     ctx->srch.wk.release();

@@ -63,6 +63,17 @@ struct RecState {
     DevBuf vals;                    // per-batch float4 sample values on the half-plane grid
     DevBuf maxv;                    // largest |sample value| of the batch (float bits)
     long long inserted = 0;
+    // coset insert (full layout, the 60 icosahedral operators): the 5 coset
+    // representatives c_i of the 12 signed-permutation operators T inserted into a
+    // full cube [-maxR, maxR+1]^3 without the z fold, symmetrised by T into acc when
+    // the sums are read
+    bool cosets = false;
+    bool cube_dirty = false;
+    DevBuf cube, cube_lo;           // float4 * dxy^3 (cube_lo: compensated mode)
+    DevBuf cops;                    // double[5*9] coset representatives
+    DevBuf bricksc;                 // int3 brick origins in the cube, central first
+    int nbricksc = 0;
+    int tsym[12][3] = {};           // operator j of T: row i has its +-1 in column tsym[j][i] & 3, sign bit 4
 };
 
 // Per-batch working set of the searches.

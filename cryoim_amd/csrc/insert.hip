@@ -70,6 +70,24 @@ __device__ __forceinline__ void place(const float* m, float fh, float fk, const 
     gz = lz - (int)g.offz;
 }
 
+// Position of sample (hx, ky) under m (the reference's S * R * (hx, ky, 0) in
+// float32) with no fold, for the full-cube accumulator of the coset insert:
+// index of the lower corner (coordinate + R on each axis) and the trilinear
+// fractions x - floor(x), which are exact in float32.
+__device__ __forceinline__ void place_cube(const float* m, float fh, float fk, int R, int& gx, int& gy, int& gz,
+                                           float& ax, float& ay, float& az) {
+    const float x = m[0] * fh + m[1] * fk;
+    const float y = m[3] * fh + m[4] * fk;
+    const float z = m[6] * fh + m[7] * fk;
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    ax = x - fx;
+    ay = y - fy;
+    az = z - fz;
+    gx = (int)fx + R;
+    gy = (int)fy + R;
+    gz = (int)fz + R;
+}
+
 // One thread per (half-plane sample, particle); loops over the operators.
 // ft: centred, centre-shifted transforms.  ctf: either computed per sample
 // from the particle record (ctf_img == nullptr && apply_ctf) or read from
@@ -373,6 +391,17 @@ constexpr int BRICK_NV = BRICK * BRICK * BRICK;
 // whatever the plane's orientation.
 __device__ __forceinline__ int brick_slot(int i) { return (i & ~31) | ((i + 5 * (i >> 5)) & 31); }
 
+// hi + lo += d: TwoSum (Knuth) keeps the rounding error of the float add in lo,
+// together with the part of d below float precision.
+__device__ __forceinline__ void comp_add(float& hi, float& lo, double d) {
+    const float y = (float)d;
+    const float s = __fadd_rn(hi, y);
+    const float yy = __fsub_rn(s, hi);
+    const float err = __fadd_rn(__fsub_rn(hi, __fsub_rn(s, yy)), __fsub_rn(y, yy));
+    lo = __fadd_rn(lo, __fadd_rn(err, (float)(d - (double)y)));
+    hi = s;
+}
+
 // One read-modify-write of the brick's exact batch sums into HBM (brick of
 // BX x BY x BZ voxels at accumulator index (bx0, by0, bz0), x fastest; SWZ:
 // voxel i at LDS slot brick_slot(i)).
@@ -398,17 +427,9 @@ __device__ void brick_flush(const BrickArgs& a, const unsigned long long* sacc, 
             // compensated: hi + lo carries the sum; TwoSum (Knuth) keeps the rounding error of each add,
             // and the part of the exact batch sum below float precision goes to lo as well
             float4 l = a.lo[vi];
-            auto add = [](float& hi, float& lo, double d) {
-                const float y = (float)d;
-                const float s = __fadd_rn(hi, y);
-                const float yy = __fsub_rn(s, hi);
-                const float err = __fadd_rn(__fsub_rn(hi, __fsub_rn(s, yy)), __fsub_rn(y, yy));
-                lo = __fadd_rn(lo, __fadd_rn(err, (float)(d - (double)y)));
-                hi = s;
-            };
-            add(v.x, l.x, da);
-            add(v.y, l.y, db);
-            add(v.z, l.z, dw);
+            comp_add(v.x, l.x, da);
+            comp_add(v.y, l.y, db);
+            comp_add(v.z, l.z, dw);
             a.lo[vi] = l;
         } else {
             v.x = (float)((double)v.x + da);
@@ -567,7 +588,10 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
 // at the level of the small bricks.
 constexpr int QCAP = 128;   // < 64 left over + 64 lanes: drained after every candidate pass
 
-template <int BX, int BY, int BZ, int NT, int WPE = 1>
+// CUBE: the coset insert's accumulator -- the full cube of coordinates
+// [-R, R+1]^3 with no z fold, each plane a half disk (hx >= 0) placed by
+// place_cube; the bricks' z origins are then cube indices too.
+template <int BX, int BY, int BZ, int NT, int WPE = 1, bool CUBE = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_insert_brickq(BrickArgs a) {
     static_assert(BX <= 16 && BY <= 16 && BZ <= 16 && NT % 64 == 0 && NT <= 1024, "brick shape");
     constexpr int NV = BX * BY * BZ, NW = NT / 64;
@@ -588,8 +612,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
     const int cap = brick_cap(a);
     const float scale = ldexpf(1.f, cap);
     const double inv = ldexp(1.0, -cap);
-    const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - 1)};
-    const float hi[3] = {(float)(bx0 - R + BX), (float)(by0 - R + BY), (float)(bz0 + BZ)};
+    const int zsh = CUBE ? R : 0;
+    const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - zsh - 1)};
+    const float hi[3] = {(float)(bx0 - R + BX), (float)(by0 - R + BY), (float)(bz0 - zsh + BZ)};
     const float c[3] = {0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
     // A point whose cell touches the brick lies in the reach box [lo, hi) (half-widths
     // (B+1)/2 about c), so its plane passes within sum_i half_i |n_i| of c (n the plane
@@ -679,7 +704,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
                     }
                 }
                 if (!empty && ta <= tb + 0.02f) {
-                    ha = max((int)ceilf(ta - 0.01f), -R);
+                    ha = max((int)ceilf(ta - 0.01f), CUBE ? 0 : -R);
                     const int hb = min((int)floorf(tb + 0.01f), R);
                     cnt = max(hb - ha + 1, 0);
                 }
@@ -703,10 +728,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
                 const int rcnt = __shfl(cnt, r, 64), rha = __shfl(ha, r, 64), rpre = __shfl(pre, r, 64);
                 const int hp = rha + (t - (rpre - rcnt)), kp = ky0 + r;
                 // the second candidate, (0, -k), exists on the h = 0 column only
-                const int ncand = __any(t < total && hp == 0) ? 2 : 1;
+                // (no mirror in the cube: a plane is the half disk hx >= 0 itself)
+                const int ncand = !CUBE && __any(t < total && hp == 0) ? 2 : 1;
                 for (int cidx = 0; cidx < ncand; ++cidx) {
                     bool ok = t < total && (cidx == 0 || hp == 0);
-                    const bool mirror = cidx == 1 || hp < 0;
+                    const bool mirror = !CUBE && (cidx == 1 || hp < 0);
                     const int hx = mirror ? -hp : hp, ky = mirror ? -kp : kp;
                     ok = ok && hx >= 0 && hx <= R && ky >= -R && ky <= R;
                     const unsigned vi = vbase + (unsigned)((ky + R) * w + hx);
@@ -716,7 +742,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
                     int ox = 0, oy = 0, oz = 0;
                     bool flip = false;
                     if (ok) {
-                        place(m, (float)hx, (float)ky, g, gx, gy, gz, ax, ay, az, flip);
+                        if (CUBE) place_cube(m, (float)hx, (float)ky, R, gx, gy, gz, ax, ay, az);
+                        else place(m, (float)hx, (float)ky, g, gx, gy, gz, ax, ay, az, flip);
                         ox = gx - bx0;
                         oy = gy - by0;
                         oz = gz - bz0;
@@ -741,6 +768,80 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
     if (qn > 0) drain(qn);
     __syncthreads();
     brick_flush<BX, BY, BZ, true>(a, sacc, bx0, by0, bz0, inv);
+}
+
+// ---- coset insert: symmetrisation of the cube into the half-volume sums -------
+//
+// The 12 signed-permutation operators T of the icosahedral group map the
+// lattice onto itself and the trilinear weight w(x) = prod (1 - |x_i|) onto
+// itself: w(g y) = w(y).  With the 60 operators written g_j c_i (g_j in T, c_i
+// the 5 coset representatives), the unfolded sum over the 60 operators
+// U60(x) = sum_j U5(g_j^T x), where U5 is the sum over the c_i alone.  The
+// reference's half-volume sums are U60(x) + conj U60(-x) for z >= 1; on the
+// z = 0 plane they split that sum by the side each sample came from, and only
+// the fold's combination of x and -x is ever used, which is unchanged when the
+// plane holds half of U60(x) + conj U60(-x) (the origin's real part and weight
+// are exactly that half).  One thread per half-volume voxel adds that sum
+// (fa, fb, wg; the weights are real) to acc, in float64 from the cube's
+// hi + lo, and into hi + lo by TwoSum in the compensated mode.
+struct TSym {
+    int c[12][3];   // row i of operator j: column c & 3, sign -1 when c & 4
+};
+
+__global__ void k_coset_sym(const float4* __restrict__ cube, const float4* __restrict__ cube_lo,
+                            float4* __restrict__ acc, float4* __restrict__ lo, int R, int dxy, int dz, TSym t) {
+    const long long n2 = (long long)dxy * dxy;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n2 * dz) return;
+    const int Z = (int)(id / n2), Y = (int)((id / dxy) % dxy), X = (int)(id % dxy);
+    const int x[3] = {X - R, Y - R, Z};
+    auto at = [&](int u0, int u1, int u2) -> double4 {
+        if (u0 < -R || u0 > R + 1 || u1 < -R || u1 > R + 1 || u2 < -R || u2 > R + 1) return make_double4(0, 0, 0, 0);
+        const long long i = ((long long)(u2 + R) * dxy + (u1 + R)) * dxy + (u0 + R);
+        const float4 h = cube[i];
+        double4 r = make_double4(h.x, h.y, h.z, 0.0);
+        if (cube_lo) {
+            const float4 l = cube_lo[i];
+            r.x += l.x;
+            r.y += l.y;
+            r.z += l.z;
+        }
+        return r;
+    };
+    double fa = 0.0, fb = 0.0, wg = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < 12; ++j) {
+        // u = g_j^T x: u[c_i] = s_i x[i]
+        int u[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int c = t.c[j][i];
+            u[c & 3] = (c & 4) ? -x[i] : x[i];
+        }
+        const double4 p = at(u[0], u[1], u[2]), m = at(-u[0], -u[1], -u[2]);
+        fa += p.x + m.x;
+        fb += p.y - m.y;
+        wg += p.z + m.z;
+    }
+    if (Z == 0) {
+        fa *= 0.5;
+        fb *= 0.5;
+        wg *= 0.5;
+    }
+    if (fa == 0.0 && fb == 0.0 && wg == 0.0) return;
+    float4 v = acc[id];
+    if (lo) {
+        float4 l = lo[id];
+        comp_add(v.x, l.x, fa);
+        comp_add(v.y, l.y, fb);
+        comp_add(v.z, l.z, wg);
+        lo[id] = l;
+    } else {
+        v.x = (float)((double)v.x + fa);
+        v.y = (float)((double)v.y + fb);
+        v.z = (float)((double)v.z + wg);
+    }
+    acc[id] = v;
 }
 
 // ---- host side -----------------------------------------------------------
@@ -928,6 +1029,96 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
     rc = brick_list(kBrickQ[qs][0], kBrickQ[qs][1], kBrickQ[qs][2], central, r.bricksq, r.nbricksq);
     if (rc != CIM_OK) return rc;
     r.qshape = qs;
+    // coset insert: the default for the full layout with the 60 icosahedral operators;
+    // CIM_INSERT=brick (the 60-plane queue kernel), brickrows or scatter select another
+    const char* em = getenv("CIM_INSERT");
+    r.cosets = layout == 0 && ops == nullptr && (!em || strcmp(em, "cosets") == 0) && maxR <= 512;
+    r.cube_dirty = false;
+    if (!r.cosets) {
+        r.cube.release();
+        r.cube_lo.release();
+    }
+    if (r.cosets) {
+        // T: the operators that are signed permutations; representatives: the first operator of
+        // each coset T c, in the table's order
+        std::vector<int> tj;
+        for (int o = 0; o < 60; ++o) {
+            const double* S = hops.data() + 9 * o;
+            bool perm = true;
+            for (int i = 0; i < 9; ++i) perm = perm && (std::fabs(S[i]) < 1e-9 || std::fabs(std::fabs(S[i]) - 1.0) < 1e-9);
+            if (perm) tj.push_back(o);
+        }
+        std::vector<int> reps;
+        std::vector<bool> covered(60, false);
+        for (int o = 0; o < 60 && tj.size() == 12; ++o) {
+            if (covered[o]) continue;
+            reps.push_back(o);
+            for (int j : tj) {
+                const double* G = hops.data() + 9 * j;
+                const double* C = hops.data() + 9 * o;
+                double P[9];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) P[3 * a + b] = G[3 * a] * C[b] + G[3 * a + 1] * C[3 + b] + G[3 * a + 2] * C[6 + b];
+                for (int q = 0; q < 60; ++q) {
+                    double dev = 0.0;
+                    for (int i = 0; i < 9; ++i) dev = std::max(dev, std::fabs(P[i] - hops[9 * q + i]));
+                    if (dev < 1e-6) covered[q] = true;
+                }
+            }
+        }
+        const bool all = std::count(covered.begin(), covered.end(), true) == 60;
+        if (tj.size() != 12 || reps.size() != 5 || !all)
+            return set_error(ctx, CIM_E_STATE, "icosahedral table: no 12-operator signed-permutation subgroup");
+        for (int j = 0; j < 12; ++j)
+            for (int i = 0; i < 3; ++i)
+                for (int c = 0; c < 3; ++c) {
+                    const double v = hops[9 * tj[j] + 3 * i + c];
+                    if (std::fabs(v) > 0.5) r.tsym[j][i] = c | (v < 0 ? 4 : 0);
+                }
+        std::vector<double> cops(45);
+        for (int i = 0; i < 5; ++i)
+            for (int e = 0; e < 9; ++e) cops[9 * i + e] = hops[9 * reps[i] + e];
+        CIM_ALLOC(r.cops, sizeof(double) * 45);
+        CIM_HIP(hipMemcpyAsync(r.cops.ptr, cops.data(), sizeof(double) * 45, hipMemcpyHostToDevice, ctx->stream));
+        const long long ncube = (long long)r.dxy * r.dxy * r.dxy;
+        CIM_ALLOC(r.cube, sizeof(float4) * ncube);
+        hipLaunchKernelGGL(k_fill_acc, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, ctx->stream,
+                           r.cube.as<float4>(), ncube, 0.f);
+        CIM_HIP(hipGetLastError());
+        if (comp) {
+            CIM_ALLOC(r.cube_lo, sizeof(float4) * ncube);
+            hipLaunchKernelGGL(k_fill_acc, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, ctx->stream,
+                               r.cube_lo.as<float4>(), ncube, 0.f);
+            CIM_HIP(hipGetLastError());
+        }
+        // bricks of the cube within reach, central first (16 x 16 x 8)
+        std::vector<std::pair<double, int>> order;
+        std::vector<int> bl;
+        const int bx = 16, by = 16, bz = 8;
+        auto nearest = [&](int a0, int n) {
+            const int lo = a0 - maxR - 1, hi = a0 - maxR + n;
+            return lo > 0 ? lo : (hi < 0 ? hi : 0);
+        };
+        for (int z = 0; z < r.dxy; z += bz)
+            for (int y = 0; y < r.dxy; y += by)
+                for (int x = 0; x < r.dxy; x += bx) {
+                    const double nx = nearest(x, bx), ny = nearest(y, by), nz = nearest(z, bz);
+                    const double d = std::sqrt(nx * nx + ny * ny + nz * nz);
+                    if (d <= rcut + 2.0) {
+                        order.push_back({d, (int)order.size()});
+                        bl.push_back(x);
+                        bl.push_back(y);
+                        bl.push_back(z);
+                    }
+                }
+        std::stable_sort(order.begin(), order.end());
+        std::vector<int> ol(bl.size());
+        for (size_t i = 0; i < order.size(); ++i)
+            for (int j = 0; j < 3; ++j) ol[3 * i + j] = bl[3 * order[i].second + j];
+        r.nbricksc = (int)order.size();
+        CIM_ALLOC(r.bricksc, sizeof(int) * std::max<size_t>(ol.size(), 3));
+        CIM_HIP(hipMemcpyAsync(r.bricksc.ptr, ol.data(), sizeof(int) * ol.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
     CIM_ALLOC(r.samples, sizeof(int2) * smp.size());
     CIM_HIP(hipMemcpyAsync(r.samples.ptr, smp.data(), sizeof(int2) * smp.size(), hipMemcpyHostToDevice, ctx->stream));
     CIM_HIP(hipStreamSynchronize(ctx->stream));
@@ -941,11 +1132,15 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
     if (n <= 0) return CIM_OK;
-    CIM_ALLOC(ctx->aux2, sizeof(float) * 9 * (size_t)n * r.nops);
+    const int total_vals = (2 * r.maxR + 1) * (r.maxR + 1);
+    // the coset kernel indexes the batch's values with 32 bits as the queue kernel does
+    const bool cosets = r.cosets && (long long)total_vals * n < (1ll << 32);
+    const int nplanes = cosets ? 5 : r.nops;
+    CIM_ALLOC(ctx->aux2, sizeof(float) * 9 * (size_t)n * nplanes);
     {
-        const int tot = n * r.nops;
+        const int tot = n * nplanes;
         hipLaunchKernelGGL(k_compose, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, parts_dev, n,
-                           r.ops.as<double>(), r.nops, ctx->aux2.as<float>());
+                           cosets ? r.cops.as<double>() : r.ops.as<double>(), nplanes, ctx->aux2.as<float>());
         CIM_HIP(hipGetLastError());
     }
     InsertGeom g;
@@ -988,7 +1183,18 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
         const bool rows = (mode && strcmp(mode, "brickrows") == 0) || (long long)total * n >= (1ll << 32) ||
                           r.maxR > 512;
         TimedLaunch tl(ctx, "insert");
-        if (rows) hipLaunchKernelGGL(k_insert_brick, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
+        if (cosets) {
+            // the 5 coset planes per particle into the cube (flushed by rec_flush_cosets)
+            ba.acc = r.cube.as<float4>();
+            ba.lo = r.comp ? r.cube_lo.as<float4>() : nullptr;
+            ba.bricks = r.bricksc.as<int>();
+            ba.dz = r.dxy;
+            ba.g.nops = 5;
+            hipLaunchKernelGGL((k_insert_brickq<16, 16, 8, 512, 1, true>), dim3(r.nbricksc), dim3(512), 0,
+                               ctx->stream, ba);
+            r.cube_dirty = true;
+            ctx->counters["insert_coset_particles"] += n;
+        } else if (rows) hipLaunchKernelGGL(k_insert_brick, dim3(r.nbricks), dim3(256), 0, ctx->stream, ba);
         else {
             ba.bricks = r.bricksq.as<int>();
             if (r.qshape == 0)
@@ -1010,9 +1216,39 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
     return CIM_OK;
 }
 
+// Add the coset insert's cube, symmetrised by T, into the half-volume sums
+// and clear it; a no-op when nothing was inserted since the last flush.
+int rec_flush_cosets(cim_ctx* ctx) {
+    RecState& r = ctx->rec;
+    if (!r.cosets || !r.cube_dirty) return CIM_OK;
+    TSym t;
+    for (int j = 0; j < 12; ++j)
+        for (int i = 0; i < 3; ++i) t.c[j][i] = r.tsym[j][i];
+    const long long nvox = (long long)r.dxy * r.dxy * r.dz;
+    {
+        TimedLaunch tl(ctx, "coset_sym");
+        hipLaunchKernelGGL(k_coset_sym, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream,
+                           r.cube.as<float4>(), r.comp ? r.cube_lo.as<float4>() : (const float4*)nullptr,
+                           r.acc.as<float4>(), r.comp ? r.lo.as<float4>() : (float4*)nullptr, r.maxR, r.dxy, r.dz, t);
+        CIM_HIP(hipGetLastError());
+    }
+    const long long ncube = (long long)r.dxy * r.dxy * r.dxy;
+    hipLaunchKernelGGL(k_fill_acc, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, ctx->stream,
+                       r.cube.as<float4>(), ncube, 0.f);
+    CIM_HIP(hipGetLastError());
+    if (r.comp) {
+        hipLaunchKernelGGL(k_fill_acc, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, ctx->stream,
+                           r.cube_lo.as<float4>(), ncube, 0.f);
+        CIM_HIP(hipGetLastError());
+    }
+    r.cube_dirty = false;
+    return CIM_OK;
+}
+
 int rec_tnf_io(cim_ctx* ctx, float* fa, float* fb, float* wg, int add) {
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
+    if (int rc = rec_flush_cosets(ctx)) return rc;
     const int t = 2 * r.maxR + 1;
     const long long total = (long long)t * t * (r.maxR + 1);
     CIM_ALLOC(ctx->out, sizeof(float) * 3 * total);
@@ -1065,6 +1301,7 @@ int fold_to_map_dev(cim_ctx* ctx, const float4* acc, const float4* lo, int dxy, 
 int rec_merge_dev(cim_ctx* ctx, float* out_dev, int nicosavg, float imgmask) {
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
+    if (int rc = rec_flush_cosets(ctx)) return rc;
     const long long nvox = (long long)r.box * r.box * r.box;
     CIM_ALLOC(ctx->aux, sizeof(float) * nvox);
     int rc = fold_to_map_dev(ctx, r.acc.as<float4>(), r.comp ? r.lo.as<float4>() : nullptr, r.dxy, r.box, r.maxR, 1,
@@ -1083,6 +1320,7 @@ int rec_merge_dev(cim_ctx* ctx, float* out_dev, int nicosavg, float imgmask) {
 int rec_finish_dev(cim_ctx* ctx, float* map_dev) {
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
+    if (int rc = rec_flush_cosets(ctx)) return rc;
     return fold_to_map_dev(ctx, r.acc.as<float4>(), r.comp ? r.lo.as<float4>() : nullptr, r.dxy, r.box, r.maxR, 0,
                            -1.f, -1.f, map_dev);
 }

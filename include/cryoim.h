@@ -172,7 +172,15 @@ int cim_fft3d_dev(cim_ctx* ctx, void* data, int n, int dir, int conj_out);
  * the compact layout, whose edge corners wrap as that array's flat index
  * does -- see cim_rec_get_tnf).  wg_init: initial weight (0.01 reconstruct.f90:28,
  * 0 substep).  ops: nops row-major 3x3 float64 symmetry operators, or NULL
- * with nops = 60 for the icosahedral group (icos2f_sym_mat order). */
+ * with nops = 60 for the icosahedral group (icos2f_sym_mat order).
+ * Full layout with ops = NULL (the icosahedral group) selects the coset
+ * insert: the 12 operators that are signed permutations map the lattice and
+ * the trilinear weights onto themselves, so each particle is inserted only
+ * under the 5 representatives of their cosets, into a full cube without the
+ * z fold, and that cube is symmetrised by the 12 into the sums when they
+ * are read (cim_rec_get_tnf, cim_rec_finish, cim_merge_map): 12x fewer
+ * corner updates for the same sums up to float rounding of the weights.
+ * CIM_INSERT=brick keeps the 60-plane brick gather. */
 int cim_rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init,
                   int nops, const double* ops);
 /* Full per-particle pipeline of reconstruct.f90:77-147: optional norm, CTF,
@@ -190,7 +198,12 @@ int cim_rec_insert_ft(cim_ctx* ctx, int n, const float* ft_re, const float* ft_i
  * lz slowest, hx fastest; reconstruct_substep.f90:148-155).  With the
  * compact layout, corners at x or y = maxR+1 land where the reference's flat
  * index puts them (next row / next plane, reconstruct_half.f90:127-130) and
- * corners past the last plane are dropped. */
+ * corners past the last plane are dropped.  With the full layout and the 60
+ * icosahedral operators (the coset insert, see cim_rec_begin) the lz = 0
+ * plane holds half of each voxel's Friedel combination F(x) + conj F(-x)
+ * instead of the reference's split by the side each sample came from; the
+ * fold (cim_rec_finish, merges) only ever reads that combination, so every
+ * map is the reference's.  Planes lz >= 1 are the reference's sums. */
 int cim_rec_get_tnf(cim_ctx* ctx, float* fa, float* fb, float* wg);
 /* Add tnf-order partial sums into the accumulator (merge of several
  * reconstruct_substep outputs, merge_reconstruct.f90:23-34). */

@@ -48,6 +48,9 @@ def _init(kind, n):
     _W["O"] = Oracle(64)
 
 
+CACHE = os.environ.get("CIM_SCALE_CACHE", os.path.join(tempfile.gettempdir(), "cim_scale_cache"))
+
+
 def _one(i):
     t0 = time.perf_counter()
     r = pipelines.oracle_search_one(_W["O"], _W["w"], i)
@@ -55,12 +58,26 @@ def _one(i):
 
 
 def oracle_decisions(kind, n, jobs):
+    """The oracle's decisions for particles 0..n-1; each one is also kept in
+    CACHE (per workload and particle count), so an interrupted run resumes."""
     out = np.zeros((n, 4))
     secs = np.zeros(n)
+    cdir = os.path.join(CACHE, f"{kind}_{n}")
+    os.makedirs(cdir, exist_ok=True)
+    todo = []
+    for i in range(n):
+        f = os.path.join(cdir, f"{i}.npy")
+        if os.path.exists(f):
+            v = np.load(f)
+            out[i], secs[i] = v[:4], v[4]
+        else:
+            todo.append(i)
+    print(f"{kind}: {n - len(todo)} of {n} particles cached in {cdir}", flush=True)
     with cf.ProcessPoolExecutor(max_workers=jobs, initializer=_init, initargs=(kind, n)) as ex:
-        for i, r, t in ex.map(_one, range(n)):
+        for i, r, t in ex.map(_one, todo):
             out[i] = r
             secs[i] = t
+            np.save(os.path.join(cdir, f"{i}.npy"), np.r_[r, t])
             if i % 50 == 0:
                 print(f"{kind} oracle particle {i}: {r} ({t:.1f} s)", flush=True)
     return out, secs

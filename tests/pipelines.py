@@ -284,3 +284,18 @@ def angle_steps(a, b, step):
     b = np.asarray(b, np.float64)[:, :3]
     d = np.abs(((a - b + 180.0) % 360.0) - 180.0)
     return d.max(axis=1) / step
+
+
+def shell_fsc(a, b, rmax):
+    """FSC per shell ir = int(r), r = 0..rmax, of two real maps (numpy, float64)."""
+    fa, fb = np.fft.rfftn(np.asarray(a, np.float64)), np.fft.rfftn(np.asarray(b, np.float64))
+    n = a.shape[0]
+    f0 = np.fft.fftfreq(n, d=1.0 / n)
+    f2 = np.fft.rfftfreq(n, d=1.0 / n)
+    r = np.sqrt(f0[:, None, None] ** 2 + f0[None, :, None] ** 2 + f2[None, None, :] ** 2).astype(np.int64).ravel()
+    keep = r <= rmax
+    r, fa, fb = r[keep], fa.ravel()[keep], fb.ravel()[keep]
+    num = np.bincount(r, weights=(fa * np.conj(fb)).real, minlength=rmax + 1)
+    da = np.bincount(r, weights=np.abs(fa) ** 2, minlength=rmax + 1)
+    db = np.bincount(r, weights=np.abs(fb) ** 2, minlength=rmax + 1)
+    return num / np.sqrt(np.maximum(da * db, 1e-300))

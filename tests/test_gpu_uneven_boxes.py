@@ -113,6 +113,26 @@ def test_insert_pipeline_matches_fp64_oracle_at_nyquist(ctx, O64, box):
         assert max(errs) < 1e-4
     for a, b in zip(got["plain"], got["rows"]):
         assert np.array_equal(a, b)
+    # the coset insert (full layout, the default there): half-volume sums for z >= 1 and the
+    # z = 0 plane after the fold's Friedel combination, against the same FP64 sums (rcut maxR-2
+    # keeps every corner off the compact array's wrapped edge, so compact and full sums agree)
+    _say(f"insert box {box}: coset insert, compensated, {n} particles")
+    ctx.rec_begin(box, maxR, layout=COMPENSATED, rcut=rcut, wg_init=0.0)
+    ctx.rec_insert(p, ps.images, _parts(ps, mask_first=True))
+    cs = ctx.rec_get_tnf(maxR)
+
+    def plane0(t):
+        fa, fb, wg = (np.asarray(v, np.float64)[0] for v in t)
+        return fa + fa[::-1, ::-1], fb - fb[::-1, ::-1], wg + wg[::-1, ::-1]
+
+    # (the folded plane is real up to rounding: the 2-fold about z maps x to -x there)
+    g0, o0 = plane0(cs), plane0(vol)
+    errs = [rel_l2(g[1:], o[1:]) for g, o in zip(cs, vol)] + [rel_l2(g0[0] + 1j * g0[1], o0[0] + 1j * o0[1]),
+                                                               rel_l2(g0[2], o0[2])]
+    _say(f"box {box} cosets: rel-L2 vs FP64 oracle z >= 1: fa {errs[0]:.2e} fb {errs[1]:.2e} wg {errs[2]:.2e}; "
+         f"folded z = 0: fa + i fb {errs[3]:.2e} wg {errs[4]:.2e}; |fb| there {np.abs(g0[1]).max():.1e} of "
+         f"|fa| {np.abs(g0[0]).max():.1e}")
+    assert max(errs) < 1e-4
 
 
 @pytest.mark.parametrize("box", [256, 384, 512])
