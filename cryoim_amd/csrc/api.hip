@@ -107,6 +107,12 @@ int cim_init(cim_ctx** out, int device) {
         delete ctx;
         return CIM_E_HIP;
     }
+    // this device's copy of the scratch array of cim_local_box_stack_noalloc
+    if (lbox_scratch_setup(ctx) != CIM_OK) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return CIM_E_HIP;
+    }
     *out = ctx;
     return CIM_OK;
 }

@@ -208,6 +208,8 @@ int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_ou
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
 // Fold {fa, fb, wg} (pitch dxy, z >= 0) -> complex volume -> inverse FFT ->
 // real map with checkerboard (inside radius rmask only when rmask >= 0).
+int lbox_scratch_setup(cim_ctx* ctx);
+int fft_plan_host(cim_ctx* ctx, int n, FftPlan& pl, std::vector<float2>& tw);
 int fold_to_map_dev(cim_ctx* ctx, const float4* acc, const float4* lo, int dxy, int box, int maxR, int int_radius,
                     float isg, float rmask, float* map_dev);
 

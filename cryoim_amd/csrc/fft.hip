@@ -65,14 +65,11 @@ __global__ void __launch_bounds__(256) k_fft_lines(float2* __restrict__ data, co
     }
 }
 
-int fft_plan(cim_ctx* ctx, int n, FftPlan** out) {
-    auto it = ctx->plans.find(n);
-    if (it != ctx->plans.end()) {
-        *out = &it->second;
-        return CIM_OK;
-    }
+// Factors (radix passes) and twiddles exp(-2 pi i k / n) of one length, on the
+// host; pl.tw is left for the caller to point at device twiddles.
+int fft_plan_host(cim_ctx* ctx, int n, FftPlan& pl, std::vector<float2>& tw) {
     if (n < 2 || n > 1024) return set_error(ctx, CIM_E_ARG, "FFT length %d outside [2, 1024]", n);
-    FftPlan pl;
+    pl = FftPlan();
     pl.n = n;
     int m = n;
     // fewest passes: radix 8 first (384 = 8 x 8 x 6: three passes through LDS)
@@ -83,11 +80,23 @@ int fft_plan(cim_ctx* ctx, int n, FftPlan** out) {
     while (m % 3 == 0) { pl.fac[pl.nfac++] = 3; m /= 3; }
     while (m % 5 == 0) { pl.fac[pl.nfac++] = 5; m /= 5; }
     if (m != 1) return set_error(ctx, CIM_E_ARG, "FFT length %d has a prime factor other than 2, 3, 5", n);
-    std::vector<float2> h(n);
+    tw.resize(n);
     for (int k = 0; k < n; ++k) {
         double a = -2.0 * M_PI * (double)k / (double)n;
-        h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+        tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
+    return CIM_OK;
+}
+
+int fft_plan(cim_ctx* ctx, int n, FftPlan** out) {
+    auto it = ctx->plans.find(n);
+    if (it != ctx->plans.end()) {
+        *out = &it->second;
+        return CIM_OK;
+    }
+    FftPlan pl;
+    std::vector<float2> h;
+    if (int rc = fft_plan_host(ctx, n, pl, h)) return rc;
     CIM_HIP(hipMalloc(&pl.tw, sizeof(float2) * n));
     CIM_HIP(hipMemcpy(pl.tw, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
     ctx->plans[n] = pl;

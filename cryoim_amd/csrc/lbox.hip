@@ -21,12 +21,39 @@
 // map of the image, which the per-box normalisation removes again, so it
 // is not run.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "cim_device.h"
 #include "cim_fft.h"
 
 namespace cim {
+
+// Working memory of cim_local_box_stack_noalloc, one copy per GPU for the
+// life of the process: the FFT twiddles, the operators, and per pass of
+// particles their angles and shifts, records, complex rows and shifted
+// images.  Host code uses the address hipGetSymbolAddress returns for the
+// current device (lbox_scratch_setup), never the array's name.
+constexpr size_t LBOX_SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[LBOX_SCRATCH_BYTES];
+constexpr int LBOX_SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[LBOX_SCRATCH_DEVICES] = {};
+
+// Called by cim_init with ctx->device current: resolves this device's copy of
+// the scratch array once.
+int lbox_scratch_setup(cim_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= LBOX_SCRATCH_DEVICES)
+        return set_error(ctx, CIM_E_ARG, "device %d: no scratch slot", ctx->device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[ctx->device]) return CIM_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(ctx, CIM_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = p;
+    return CIM_OK;
+}
 
 namespace {
 
@@ -341,6 +368,115 @@ int lbox_stack_dev(cim_ctx* ctx, int n, const float* images_dev, const float* pa
     return CIM_OK;
 }
 
+// The stack path with every working buffer in this device's scratch array:
+// particles in passes that fit it, each pass uploading its angles and
+// shifts, computing its records and cutting its boxes.  No device memory is
+// allocated.  Calls that overlap on one GPU share the array and race.
+int lbox_stack_noalloc(cim_ctx* ctx, int n, const float* images_dev, const float* parts, int box, int nops,
+                       const double* ops, const float* x0, int sub, float apix, float* boxes_dev, float* rec_host) {
+    int rc = lbox_check(ctx, n, parts, nops, ops, x0, box);
+    if (rc) return rc;
+    if (sub < 2 || sub > box) return set_error(ctx, CIM_E_ARG, "sub_FFTsize %d outside [2, box %d]", sub, box);
+    const double r0 = std::sqrt((double)x0[0] * x0[0] + (double)x0[1] * x0[1] + (double)x0[2] * x0[2]);
+    if (r0 > (double)box - 2.0)
+        return set_error(ctx, CIM_E_ARG, "|(local_x, local_y, local_z)| = %g reaches past the %d-pixel image", r0, box);
+    unsigned char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < LBOX_SCRATCH_DEVICES)
+            base = static_cast<unsigned char*>(g_scratch_ptr[ctx->device]);
+    }
+    if (!base) return set_error(ctx, CIM_E_STATE, "scratch of device %d not set up", ctx->device);
+    FftPlan pl;
+    std::vector<float2> tw;
+    if ((rc = fft_plan_host(ctx, box, pl, tw))) return rc;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const long long b2 = (long long)box * box;
+    const size_t fixed = al(sizeof(float2) * box) + al(sizeof(float) * 9 * (size_t)nops);
+    // per particle: angles and shifts, records, complex rows, shifted image (+ alignment slack)
+    const size_t per = sizeof(float) * (5 + (size_t)CIM_LBOX_REC * nops) + (sizeof(float2) + sizeof(float)) * b2;
+    if (fixed + per + 4 * 256 > LBOX_SCRATCH_BYTES)
+        return set_error(ctx, CIM_E_ARG,
+                         "box %d with %d operators needs %zu bytes per particle: more than the %zu-byte scratch", box,
+                         nops, fixed + per + 4 * 256, LBOX_SCRATCH_BYTES);
+    const int chunk = (int)std::max<long long>(1, std::min<long long>(std::max(n, 1),
+                                                   (long long)((LBOX_SCRATCH_BYTES - fixed - 4 * 256) / per)));
+    unsigned char* q = base;
+    float2* d_tw = reinterpret_cast<float2*>(q);
+    q += al(sizeof(float2) * box);
+    float* d_ops = reinterpret_cast<float*>(q);
+    q += al(sizeof(float) * 9 * (size_t)nops);
+    float* d_pr = reinterpret_cast<float*>(q);
+    q += al(sizeof(float) * 5 * (size_t)chunk);
+    float* d_rec = reinterpret_cast<float*>(q);
+    q += al(sizeof(float) * CIM_LBOX_REC * (size_t)chunk * nops);
+    float2* z = reinterpret_cast<float2*>(q);
+    q += al(sizeof(float2) * b2 * chunk);
+    float* rimg = reinterpret_cast<float*>(q);
+    q += al(sizeof(float) * b2 * chunk);
+    if ((size_t)(q - base) > LBOX_SCRATCH_BYTES)
+        return set_error(ctx, CIM_E_ARG, "scratch layout of %zu bytes exceeds %zu", (size_t)(q - base),
+                         LBOX_SCRATCH_BYTES);
+    LboxSetup st;
+    st.pr.resize(5 * (size_t)std::max(n, 1));
+    for (int i = 0; i < n; ++i) {
+        const float* r = parts + (size_t)CIM_PART_STRIDE * i;
+        st.pr[5 * i + 0] = r[CIM_P_THETA];
+        st.pr[5 * i + 1] = r[CIM_P_PHI];
+        st.pr[5 * i + 2] = r[CIM_P_OMEGA];
+        st.pr[5 * i + 3] = r[CIM_P_X];
+        st.pr[5 * i + 4] = r[CIM_P_Y];
+    }
+    st.ops.assign(ops, ops + 9 * (size_t)nops);
+    CIM_HIP(hipMemcpyAsync(d_tw, tw.data(), sizeof(float2) * box, hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipMemcpyAsync(d_ops, st.ops.data(), sizeof(float) * st.ops.size(), hipMemcpyHostToDevice, ctx->stream));
+    pl.tw = d_tw;
+    const FacList fl = fac_list(&pl);
+    const float twopi = 2.0f * (4.0f * atan2f(1.0f, 1.0f));
+    const int Lr = box >= 1024 ? 1 : (box >= 512 ? 2 : (box >= 256 ? 4 : (box >= 128 ? 8 : 16)));
+    const int Lc = box <= 128 ? 16 : (box <= 512 ? 8 : 4);
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+        const int nb = std::min(chunk, n - i0);
+        CIM_HIP(hipMemcpyAsync(d_pr, st.pr.data() + 5 * (size_t)i0, sizeof(float) * 5 * nb, hipMemcpyHostToDevice,
+                               ctx->stream));
+        LboxGeomArgs a{};
+        a.pr = d_pr;
+        a.ops = d_ops;
+        a.rec = d_rec;
+        a.n = nb;
+        a.nops = nops;
+        a.box = box;
+        a.stack = 1;
+        a.x0 = x0[0];
+        a.y0 = x0[1];
+        a.z0 = x0[2];
+        a.apix = apix;
+        a.d2r = 4.0f * atan2f(1.0f, 1.0f) / 180.0f;
+        a.r2d = 45.0f / atan2f(1.0f, 1.0f);
+        if ((rc = lbox_geom(ctx, a))) return rc;
+        const long long nlines = (long long)nb * box;
+        hipLaunchKernelGGL(k_lbox_shift<0>, dim3((unsigned)((nlines + Lr - 1) / Lr)), dim3(256),
+                           sizeof(float2) * 2 * Lr * box, ctx->stream, images_dev + b2 * i0, z, nullptr, d_tw, box, fl,
+                           nlines, Lr, d_pr, twopi);
+        CIM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lbox_shift<1>, dim3((unsigned)((nlines + Lc - 1) / Lc)), dim3(256),
+                           sizeof(float2) * 2 * Lc * box, ctx->stream, nullptr, z, rimg, d_tw, box, fl, nlines, Lc,
+                           d_pr, twopi);
+        CIM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lbox_cut, dim3((unsigned)(nb * nops)), dim3(256), 0, ctx->stream, rimg, d_rec, box, sub,
+                           nops, boxes_dev + (long long)sub * sub * nops * i0);
+        CIM_HIP(hipGetLastError());
+        if (rec_host)
+            CIM_HIP(hipMemcpyAsync(rec_host + (size_t)CIM_LBOX_REC * nops * i0, d_rec,
+                                   sizeof(float) * CIM_LBOX_REC * (size_t)nb * nops, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        // the next pass rewrites the angles, records and images of this one
+        CIM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    return CIM_OK;
+}
+
 }  // namespace
 
 }  // namespace cim
@@ -393,6 +529,15 @@ int cim_local_box_stack(cim_ctx* ctx, int n, const float* images, const float* p
     }
     CIM_HIP(hipStreamSynchronize(ctx->stream));
     return CIM_OK;
+}
+
+int cim_local_box_stack_noalloc(cim_ctx* ctx, int n, const void* images_dev, const float* parts, int box, int nops,
+                                const double* ops, const float* x0, int sub, float apix, void* boxes_dev, float* rec) {
+    if (!ctx) return CIM_E_ARG;
+    if (n > 0 && (!images_dev || !boxes_dev)) return set_error(ctx, CIM_E_ARG, "NULL images or boxes");
+    CIM_HIP(hipSetDevice(ctx->device));
+    return lbox_stack_noalloc(ctx, n, static_cast<const float*>(images_dev), parts, box, nops, ops, x0, sub, apix,
+                              static_cast<float*>(boxes_dev), rec);
 }
 
 int cim_local_box_stack_dev(cim_ctx* ctx, int n, const void* images_dev, const float* parts, int box, int nops,

@@ -145,6 +145,7 @@ def load() -> ctypes.CDLL:
         "cim_local_box_coords": (ip, [vp, ip, fp, ip, dp, fp, ip, ctypes.c_float, fp]),
         "cim_local_box_stack": (ip, [vp, ip, fp, fp, ip, ip, dp, fp, ip, ctypes.c_float, fp, fp]),
         "cim_local_box_stack_dev": (ip, [vp, ip, vp, fp, ip, ip, dp, fp, ip, ctypes.c_float, vp, fp]),
+        "cim_local_box_stack_noalloc": (ip, [vp, ip, vp, fp, ip, ip, dp, fp, ip, ctypes.c_float, vp, fp]),
         "cim_lref_model": (ip, [vp, P, fp]),
         "cim_center_model": (ip, [vp, P, fp]),
         "cim_center": (ip, [vp, P, ip, ip, fp, fp, fp]),
@@ -380,6 +381,24 @@ class Context:
                                           int(sub), float(apix), _fptr(out), _fptr(rec))
         self._check(rc, "cim_local_box_stack")
         return out, rec
+
+    def local_box_stack_noalloc(self, images, parts, ops, x0, sub: int, apix: float, boxes, box: int):
+        """cim_local_box_stack_noalloc: as the device form of local_box_stack,
+        with every working buffer in the library's static scratch array (no
+        device allocation).  images, boxes: device arrays with data_ptr()."""
+        parts = _f32(parts).reshape(-1, PART_STRIDE)
+        n = parts.shape[0]
+        o = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 9)
+        optr = o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        x = _f32(x0).reshape(3)
+        rec = np.zeros((n, o.shape[0], LBOX_REC), np.float32)
+        if boxes.numel() != n * o.shape[0] * sub * sub:
+            raise CimError("boxes must hold n*nops*sub^2 floats")
+        rc = self.lib.cim_local_box_stack_noalloc(self.h, n, ctypes.c_void_p(images.data_ptr()), _fptr(parts),
+                                                  int(box), o.shape[0], optr, _fptr(x), int(sub), float(apix),
+                                                  ctypes.c_void_p(boxes.data_ptr()), _fptr(rec))
+        self._check(rc, "cim_local_box_stack_noalloc")
+        return boxes, rec
 
     # -- search -------------------------------------------------------------
     def search_templates(self, params: CimParams, images, tort):

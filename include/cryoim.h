@@ -375,6 +375,20 @@ int cim_local_box_stack(cim_ctx* ctx, int n, const float* images, const float* p
 /* Same with images and boxes in device memory. */
 int cim_local_box_stack_dev(cim_ctx* ctx, int n, const void* images_dev, const float* parts, int box, int nops,
                             const double* ops, const float* x0, int sub, float apix, void* boxes_dev, float* rec);
+/* cim_local_box_stack_dev without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * working memory (twiddles, operators, per pass of particles their angles,
+ * records, complex rows and shifted images) lives in a 64 MiB array in device
+ * memory that every GPU the library runs on holds until the process ends;
+ * particles are taken in passes that fit it.  CIM_E_ARG when one particle's
+ * working set (12 box^2 + 32 nops + 20 bytes, plus the twiddles and
+ * operators) does not fit.  Nothing in the array needs clearing between
+ * calls: every pass writes what it reads.  Calls that overlap on one GPU
+ * (several contexts or threads) share the array and race.  Same results as
+ * cim_local_box_stack_dev (the same kernels). */
+int cim_local_box_stack_noalloc(cim_ctx* ctx, int n, const void* images_dev, const float* parts, int box, int nops,
+                                const double* ops, const float* x0, int sub, float apix, void* boxes_dev, float* rec);
 
 #ifdef __cplusplus
 }
