@@ -507,6 +507,161 @@ __global__ void __launch_bounds__(256) k_cl_table(const float2* __restrict__ ppo
     }
 }
 
+// The same tables on the matrix cores.  With g = 1 for an ungated sample
+// (0 when gated) and a = g |F|/2, the weight of a ray pair at one radius is
+// w = gT gP (aT + aP), so D = sum_r (aT gP + gT aP) is a product of a
+// 2 nr-column template matrix and a particle matrix.  The folded phase
+// difference has the cosine series |d| = pi/2 - (4/pi) sum_{k odd} cos(k d) / k^2
+// (d wrapped to [-pi, pi]); cut after NH odd harmonics and with
+// cos k(pT - pP) = cos kpT cos kpP + sin kpT sin kpP, N = pi/2 D + sum_r sum_k
+// c_k (aT gP + gT aP) cos k(pT - pP), c_k = -4 / (pi k^2), is a product with
+// 4 NH columns per radius.  Both run as v_mfma_f32_32x32x16_bf16 (bfloat16
+// features, float32 sums).  The truncated series ranks AU points by a
+// smoothed residual (NH = 1: by the weighted cosine of the phase
+// difference); the exact re-score of the best candidates decides.
+// One workgroup per 128 x 128 tile of (jT, jP), 4 waves of 64 x 64, each
+// 2 x 2 tiles of 32 x 32; per chunk of TMR radii the 256 threads write the
+// features of the tile's 128 template rows and 128 particle rays to LDS.
+typedef __bf16 cim_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 cim_bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 cim_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float cim_f32x16 __attribute__((ext_vector_type(16)));
+
+template <int NH>
+struct TabMfma {
+    static constexpr int RC = NH <= 2 ? 8 : 4;     // radii per LDS chunk
+    static constexpr int KH = RC * 4 * NH;         // harmonic features per chunk (a multiple of 32)
+    static constexpr int KD = 16;                  // D features per chunk (2 RC, zero-padded to 16)
+    static constexpr int PH = KH + 8;              // row pitch (bfloat16): 16 bytes of padding against bank conflicts
+    static constexpr int PD = KD + 8;
+};
+
+template <int NH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_cl_table_mfma(const float2* __restrict__ ppol, const float2* __restrict__ tpol,
+                                                       int tpol_per_particle, int T, int nr, int npsi,
+                                                       unsigned int* __restrict__ table) {
+    using C = TabMfma<NH>;
+    __shared__ __attribute__((aligned(16))) __bf16 sHT[128][C::PH], sHP[128][C::PH];
+    __shared__ __attribute__((aligned(16))) __bf16 sDT[128][C::PD], sDP[128][C::PD];
+    const int item = blockIdx.z, p = item / T, t = item - p * T;
+    const float2* P = ppol + (long long)p * nr * npsi;
+    const float2* Tm = tpol + (long long)(tpol_per_particle ? item : t) * nr * npsi;
+    const int jP0 = blockIdx.x * 128, jT0 = blockIdx.y * 128;
+    const int nrow = npsi / 2;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+    // this thread's template row (tid < 128) or particle ray
+    const bool tside = tid < 128;
+    const int jj = tside ? jT0 + tid : jP0 + tid - 128;
+    const bool live = tside ? jj < nrow : jj < npsi;
+    const float2* src = tside ? Tm : P;
+    __bf16* hrow = tside ? &sHT[tid][0] : &sHP[tid - 128][0];
+    __bf16* drow = tside ? &sDT[tid][0] : &sDP[tid - 128][0];
+    cim_f32x16 accH[2][2], accD[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) { accH[i][j][e] = 0.f; accD[i][j][e] = 0.f; }
+    // rays of the next chunk are loaded before the current chunk's products, so their
+    // latency overlaps the matrix work
+    float2 nxt[C::RC];
+#pragma unroll
+    for (int rr = 0; rr < C::RC; ++rr)
+        nxt[rr] = (live && rr < nr) ? src[(long long)rr * npsi + jj] : make_float2(0.f, GATED);
+    for (int r0 = 0; r0 < nr; r0 += C::RC) {
+#pragma unroll
+        for (int rr = 0; rr < C::RC; ++rr) {
+            const float2 v = nxt[rr];
+            const float g = v.y > 0.5f * GATED ? 1.f : 0.f;
+            const float a = g * v.y;
+            // D: template [a, g], particle [g, a]
+            cim_bf16x2 dv;
+            dv[0] = (__bf16)(tside ? a : g);
+            dv[1] = (__bf16)(tside ? g : a);
+            *reinterpret_cast<cim_bf16x2*>(drow + 2 * rr) = dv;
+            float c1, s1;
+            __sincosf(v.x, &s1, &c1);
+            const float c2 = c1 * c1 - s1 * s1, s2 = 2.f * c1 * s1;
+            float ck = c1, sk = s1;
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) {
+                const int k = 2 * kk + 1;
+                // template [a c, a s, g c, g s]; particle c_k [g c, g s, a c, a s]
+                const float f = tside ? 1.f : -1.2732395447351628f / (float)(k * k);
+                const float x0 = tside ? a : g, x1 = tside ? g : a;
+                cim_bf16x4 hv;
+                hv[0] = (__bf16)(f * x0 * ck);
+                hv[1] = (__bf16)(f * x0 * sk);
+                hv[2] = (__bf16)(f * x1 * ck);
+                hv[3] = (__bf16)(f * x1 * sk);
+                *reinterpret_cast<cim_bf16x4*>(hrow + rr * 4 * NH + 4 * kk) = hv;
+                const float cn = ck * c2 - sk * s2, sn = sk * c2 + ck * s2;
+                ck = cn;
+                sk = sn;
+            }
+        }
+#pragma unroll
+        for (int e = 2 * C::RC; e < C::KD; e += 2) *reinterpret_cast<cim_bf16x2*>(drow + e) = cim_bf16x2{};
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < C::RC; ++rr) {
+            const int r = r0 + C::RC + rr;
+            nxt[rr] = (live && r < nr) ? src[(long long)r * npsi + jj] : make_float2(0.f, GATED);
+        }
+        const int lr = lane & 31, lk = 8 * (lane >> 5);
+#pragma unroll
+        for (int kb = 0; kb < C::KH / 16; ++kb) {
+            cim_bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                fa[i] = *reinterpret_cast<const cim_bf16x8*>(&sHT[wr + 32 * i + lr][16 * kb + lk]);
+                fb[i] = *reinterpret_cast<const cim_bf16x8*>(&sHP[wc + 32 * i + lr][16 * kb + lk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    accH[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], accH[i][j], 0, 0, 0);
+        }
+        {
+            cim_bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                fa[i] = *reinterpret_cast<const cim_bf16x8*>(&sDT[wr + 32 * i + lr][lk]);
+                fb[i] = *reinterpret_cast<const cim_bf16x8*>(&sDP[wc + 32 * i + lr][lk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    accD[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], accD[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // 32 x 32 result tile: register e of lane l is row 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32
+    unsigned int* out = table + (long long)item * nrow * 2 * npsi;
+    const float half_pi = 1.5707963267948966f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int jp = jP0 + wc + 32 * j + (lane & 31);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int jt = jT0 + wr + 32 * i + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                if (jt < nrow && jp < npsi) {
+                    const float D = accD[i][j][e];
+                    const float N = fmaf(half_pi, D, accH[i][j][e]);
+                    const unsigned int v = bf16_bits(N) | (bf16_bits(D) << 16);
+                    out[(long long)jt * 2 * npsi + jp] = v;
+                    out[(long long)jt * 2 * npsi + jp + npsi] = v;
+                }
+            }
+        }
+}
+
 struct GatherArgs {
     const unsigned int* table;   // n x T x (npsi/2) x 2 npsi, {N, D} as bfloat16, rows repeated once
     const float* tang;        // T x 3
@@ -1024,6 +1179,37 @@ static int tune_shape(cim_ctx* ctx, SearchArgs a, int lds_with, int lds_without,
 
 // Global search of n particles already on the device (images_dev) with
 // host records parts (in/out).  pr_array: optional, for n == 1.
+constexpr int CIM_TABLE_NH_DEFAULT = 4;
+
+// The residual tables of nb particles x T templates: the MFMA kernel with nh
+// harmonics, or the exact VALU kernel (nh = 0).
+int launch_cl_table(cim_ctx* ctx, int nh, const float2* ppol, const float2* tpol, int tpol_per_particle, int nb, int T,
+                    int nr, int npsi, unsigned int* table) {
+    if (nh == 0) {
+        const unsigned nt = (unsigned)((npsi + 63) / 64), ntr = (unsigned)((npsi / 2 + 63) / 64);
+        hipLaunchKernelGGL(k_cl_table, dim3(nt, ntr, nb * T), dim3(256), 0, ctx->stream, ppol, tpol,
+                           tpol_per_particle, T, nr, npsi, table);
+    } else {
+        const dim3 grid((unsigned)((npsi + 127) / 128), (unsigned)((npsi / 2 + 127) / 128), (unsigned)(nb * T));
+        if (nh == 1)
+            hipLaunchKernelGGL(k_cl_table_mfma<1>, grid, dim3(256), 0, ctx->stream, ppol, tpol, tpol_per_particle, T,
+                               nr, npsi, table);
+        else if (nh == 2)
+            hipLaunchKernelGGL(k_cl_table_mfma<2>, grid, dim3(256), 0, ctx->stream, ppol, tpol, tpol_per_particle, T,
+                               nr, npsi, table);
+        else if (nh == 4)
+            hipLaunchKernelGGL(k_cl_table_mfma<4>, grid, dim3(256), 0, ctx->stream, ppol, tpol, tpol_per_particle, T,
+                               nr, npsi, table);
+        else if (nh == 8)
+            hipLaunchKernelGGL(k_cl_table_mfma<8>, grid, dim3(256), 0, ctx->stream, ppol, tpol, tpol_per_particle, T,
+                               nr, npsi, table);
+        else
+            return set_error(ctx, CIM_E_ARG, "CIM_TABLE_NH %d: use 0, 1, 2, 4 or 8", nh);
+    }
+    CIM_HIP(hipGetLastError());
+    return CIM_OK;
+}
+
 static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts,
                              float* pr_array) {
     SearchState& s = ctx->srch;
@@ -1085,6 +1271,10 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     const int npsi = g.No * sub;
     const float dpsi = p->searchstep / (float)sub;
     const int topk = std::min(std::min(p->search_topk > 0 ? p->search_topk : 32, npts), 256);
+    // residual tables on the matrix cores with NH odd harmonics of the folded phase difference
+    // (CIM_TABLE_NH = 1, 2, 4 or 8), or the exact residual on the VALU (0)
+    int table_nh = CIM_TABLE_NH_DEFAULT;
+    if (const char* eh = getenv("CIM_TABLE_NH")) table_nh = atoi(eh);
     if (tables && g.No > 2048) return set_error(ctx, CIM_E_ARG, "searchstep %g too fine for the tables", p->searchstep);
     // the polar grid must close on itself with rays 180 deg apart on it
     if (tables && (std::fabs(p->searchstep * (double)g.No - 360.0) > 1e-3 || (npsi & 1)))
@@ -1253,11 +1443,9 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             CIM_ALLOC(sb.table, sizeof(unsigned int) * (size_t)nb * T * (npsi / 2) * 2 * npsi);
             {
                 TimedLaunch tl(ctx, "cl_table");
-                const unsigned nt = (unsigned)((npsi + 63) / 64), ntr = (unsigned)((npsi / 2 + 63) / 64);
-                hipLaunchKernelGGL(k_cl_table, dim3(nt, ntr, nb * T), dim3(256), 0, ctx->stream,
-                                   sb.ppol.as<float2>(), sb.tpol.as<float2>(), p->apply_ctf ? 1 : 0, T, nr, npsi,
-                                   sb.table.as<unsigned int>());
-                CIM_HIP(hipGetLastError());
+                rc = launch_cl_table(ctx, table_nh, sb.ppol.as<float2>(), sb.tpol.as<float2>(), p->apply_ctf ? 1 : 0,
+                                     nb, T, nr, npsi, sb.table.as<unsigned int>());
+                if (rc) return rc;
             }
             GatherArgs ga{sb.table.as<unsigned int>(), sb.tang.as<float>(), sb.pts.as<float>(), sb.pbase.as<int>(),
                           sb.ops.as<float>(), T, 60, npts, npsi, sub, g.No, dpsi, nb, sb.outv.as<float>(),
@@ -1717,6 +1905,28 @@ static int refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* i
 using namespace cim;
 
 extern "C" {
+
+int cim_debug_cl_table(cim_ctx* ctx, int nh, int np, int T, int tpol_per_particle, int nr, int npsi, const float* ppol,
+                       const float* tpol, unsigned int* table) {
+    if (!ctx) return CIM_E_ARG;
+    if (np < 1 || T < 1 || nr < 1 || npsi < 2 || (npsi & 1) || !ppol || !tpol || !table)
+        return set_error(ctx, CIM_E_ARG, "cim_debug_cl_table: bad sizes or NULL arrays");
+    CIM_HIP(hipSetDevice(ctx->device));
+    const size_t pb = sizeof(float2) * (size_t)np * nr * npsi;
+    const size_t tb = sizeof(float2) * (size_t)(tpol_per_particle ? np * T : T) * nr * npsi;
+    const size_t ob = sizeof(unsigned int) * (size_t)np * T * (npsi / 2) * 2 * npsi;
+    CIM_ALLOC(ctx->img, pb);
+    CIM_ALLOC(ctx->aux, tb);
+    CIM_ALLOC(ctx->out, ob);
+    CIM_HIP(hipMemcpyAsync(ctx->img.ptr, ppol, pb, hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipMemcpyAsync(ctx->aux.ptr, tpol, tb, hipMemcpyHostToDevice, ctx->stream));
+    int rc = launch_cl_table(ctx, nh, ctx->img.as<float2>(), ctx->aux.as<float2>(), tpol_per_particle, np, T, nr, npsi,
+                             ctx->out.as<unsigned int>());
+    if (rc) return rc;
+    CIM_HIP(hipMemcpyAsync(table, ctx->out.ptr, ob, hipMemcpyDeviceToHost, ctx->stream));
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    return CIM_OK;
+}
 
 int cim_search_templates(cim_ctx* ctx, const cim_params* p, int T, const float* images, const float* tort) {
     if (!ctx) return CIM_E_ARG;

@@ -146,6 +146,7 @@ def load() -> ctypes.CDLL:
         "cim_local_box_stack": (ip, [vp, ip, fp, fp, ip, ip, dp, fp, ip, ctypes.c_float, fp, fp]),
         "cim_local_box_stack_dev": (ip, [vp, ip, vp, fp, ip, ip, dp, fp, ip, ctypes.c_float, vp, fp]),
         "cim_local_box_stack_noalloc": (ip, [vp, ip, vp, fp, ip, ip, dp, fp, ip, ctypes.c_float, vp, fp]),
+        "cim_debug_cl_table": (ip, [vp, ip, ip, ip, ip, ip, ip, fp, fp, ctypes.POINTER(ctypes.c_uint32)]),
         "cim_lref_model": (ip, [vp, P, fp]),
         "cim_center_model": (ip, [vp, P, fp]),
         "cim_center": (ip, [vp, P, ip, ip, fp, fp, fp]),
@@ -399,6 +400,22 @@ class Context:
                                                   ctypes.c_void_p(boxes.data_ptr()), _fptr(rec))
         self._check(rc, "cim_local_box_stack_noalloc")
         return boxes, rec
+
+    def debug_cl_table(self, nh: int, ppol, tpol, T: int, tpol_per_particle: bool = True):
+        """cim_debug_cl_table (include/cryoim_debug.h): residual tables from polar
+        rays ppol (np, nr, npsi, 2) and tpol; returns (N, D) as float32 arrays
+        (np*T, npsi/2, npsi) decoded from the bfloat16 pairs (first copy of
+        each row)."""
+        pp = _f32(ppol)
+        tp = _f32(tpol)
+        np_, nr, npsi = pp.shape[:3]
+        out = np.zeros((np_ * T, npsi // 2, 2 * npsi), np.uint32)
+        rc = self.lib.cim_debug_cl_table(self.h, int(nh), np_, int(T), int(bool(tpol_per_particle)), nr, npsi,
+                                         _fptr(pp), _fptr(tp), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        self._check(rc, "cim_debug_cl_table")
+        assert np.array_equal(out[:, :, :npsi], out[:, :, npsi:])
+        e = out[:, :, :npsi]
+        return (e << 16).view(np.float32), (e & 0xFFFF0000).view(np.float32)
 
     # -- search -------------------------------------------------------------
     def search_templates(self, params: CimParams, images, tort):
