@@ -63,6 +63,8 @@ COMPENSATED = 16          # CIM_REC_COMPENSATED
 FLOPS_PER_SAMPLE = 40
 # insert, per (sample, operator): 8 corners x read-modify-write of fa, fb, wg in f32
 INSERT_BYTES_PER_SAMPLE_OP = 8 * 3 * 8
+# where each particle's exact minimum ranked among its table candidates (counted by the library)
+RANK_KEYS = ("cl_rank_lt4", "cl_rank_lt8", "cl_rank_lt16", "cl_rank_lt24", "cl_rank_ge24")
 METRIC = "particles/s search+insert, box 384: chosen vs plain FP32 GPU at FSC>=0.999"
 
 
@@ -183,7 +185,7 @@ def gpu_run(args, tang, timgs, ps):
         kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_gather", "cl_rescore",
                                                    "cl_fallback", "insert")}
         cnt = {k: ctx.counter(k) for k in ("cl_search_samples", "insert_samples", "cl_fallback_particles",
-                                           "cl_table_pairs", "cl_gather_points")}
+                                           "cl_table_pairs", "cl_gather_points", *RANK_KEYS)}
         ctx.timing(False)
         return out, e0.elapsed_time(e1), wall, kt, cnt
 
@@ -229,7 +231,7 @@ def gpu_run(args, tang, timgs, ps):
                                                    "cl_fallback")}
         kt["insert"] = ctx_i.timing_get("insert")
         cnt = {k: ctx.counter(k) for k in ("cl_search_samples", "cl_fallback_particles", "cl_table_pairs",
-                                           "cl_gather_points")}
+                                           "cl_gather_points", *RANK_KEYS)}
         cnt["insert_samples"] = ctx_i.counter("insert_samples")
         for c in (ctx, ctx_i):
             c.timing(False)
@@ -681,6 +683,7 @@ def main():
                   "fsc_at_r190": round(float(fsc[-1]), 6),
                   "map_rel_l2_chosen_vs_plain": r["map_rel"],
                   "fallback_particles": int(r["cnt_c"]["cl_fallback_particles"]),
+                  "exact_minimum_rank": {k[8:]: int(r["cnt_c"][k]) for k in RANK_KEYS},
                   "chosen_kernels": kt,
                   "plain_search_avg_ms": round(r["kt_p"]["cl_search"][0], 3),
                   "search_angle_err_deg_median": float(np.median(r["ang_err"])),

@@ -488,9 +488,7 @@ __global__ void __launch_bounds__(256) k_cl_table(const float2* __restrict__ ppo
         }
         __syncthreads();
     }
-    // rows of 2 npsi entries, the second half repeating the first, so that the
-    // gather reads the npsi omega columns of a common line without a wrap
-    unsigned int* out = table + (long long)item * nrow * 2 * npsi;
+    unsigned int* out = table + (long long)item * nrow * npsi;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int jt = jT0 + ty * 4 + i;
@@ -500,8 +498,7 @@ __global__ void __launch_bounds__(256) k_cl_table(const float2* __restrict__ ppo
             const int jp = jP0 + tx * 4 + j;
             if (jp < npsi) {
                 const unsigned int e = bf16_bits(N[i][j]) | (bf16_bits(D[i][j]) << 16);
-                out[(long long)jt * 2 * npsi + jp] = e;
-                out[(long long)jt * 2 * npsi + jp + npsi] = e;
+                out[(long long)jt * npsi + jp] = e;
             }
         }
     }
@@ -641,7 +638,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         __syncthreads();
     }
     // 32 x 32 result tile: register e of lane l is row 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32
-    unsigned int* out = table + (long long)item * nrow * 2 * npsi;
+    unsigned int* out = table + (long long)item * nrow * npsi;
     const float half_pi = 1.5707963267948966f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -655,15 +652,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                     const float D = accD[i][j][e];
                     const float N = fmaf(half_pi, D, accH[i][j][e]);
                     const unsigned int v = bf16_bits(N) | (bf16_bits(D) << 16);
-                    out[(long long)jt * 2 * npsi + jp] = v;
-                    out[(long long)jt * 2 * npsi + jp + npsi] = v;
+                    out[(long long)jt * npsi + jp] = v;
                 }
             }
         }
 }
 
 struct GatherArgs {
-    const unsigned int* table;   // n x T x (npsi/2) x 2 npsi, {N, D} as bfloat16, rows repeated once
+    const unsigned int* table;   // n x T x (npsi/2) x npsi, {N, D} as bfloat16
     const float* tang;        // T x 3
     const float* pts;         // npts x 2
     const int* pbase;
@@ -717,18 +713,17 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     const int tid = threadIdx.x, bd = blockDim.x;
     const int nrow = a.npsi >> 1;
     float num[VPT], den[VPT];
-    // column of omega k on a line with zero-angle column j0: j0 - k sub (mod npsi), read in
-    // the repeated row as j0 + npsi - maxk + off with off = maxk - k sub in [0, maxk]
-    // (columns past Nomega read column j0 + npsi and are not used)
-    const int maxk = (a.Nomega - 1) * a.sub;
-    unsigned boff[VPT];   // byte offsets
+    // column of omega k on a line with zero-angle column j0: j0 - k sub, plus npsi when
+    // that is negative (columns past Nomega read column j0 and are not used)
+    int ks[VPT];
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
         num[v] = 0.f;
         den[v] = 0.f;
         const int k = tid + v * bd;
-        boff[v] = 4u * (unsigned)(maxk - (k < a.Nomega ? k * a.sub : 0));
+        ks[v] = k < a.Nomega ? k * a.sub : 0;
     }
+    const int npsi = a.npsi;
     float E1[9];
     euler_f(a.pts[2 * pt], a.pts[2 * pt + 1], 0.f, E1);
     for (int t = 0; t < a.T; ++t) {
@@ -751,24 +746,29 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
             const unsigned long long mask = __ballot(ok);
             if (ok) {
                 const int slot = __popcll(mask & ((1ull << tid) - 1ull));
-                srow[slot] = jt * 2 * a.npsi;
-                sj0[slot] = j0 + a.npsi - maxk;
+                srow[slot] = jt * a.npsi;
+                sj0[slot] = j0;
             }
             if (tid == 0) snv = __popcll(mask);
         }
         __syncthreads();
         const int nv = snv;
-        const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * 2 * a.npsi;
+        const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * a.npsi;
         constexpr int G = 8;
         int j = 0;
         for (; j + G <= nv; j += G) {
             unsigned int e[G][VPT];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                // the line's start is the same for the whole workgroup: a scalar base
-                const char* row = (const char*)(tab + __builtin_amdgcn_readfirstlane(srow[j + g] + sj0[j + g]));
+                // the line's row and zero-angle column are the same for the whole workgroup
+                const unsigned int* row = tab + __builtin_amdgcn_readfirstlane(srow[j + g]);
+                const int j0 = __builtin_amdgcn_readfirstlane(sj0[j + g]);
 #pragma unroll
-                for (int v = 0; v < VPT; ++v) e[g][v] = *(const unsigned int*)(row + boff[v]);
+                for (int v = 0; v < VPT; ++v) {
+                    int c = j0 - ks[v];
+                    c += c < 0 ? npsi : 0;
+                    e[g][v] = row[c];
+                }
             }
             // keep all G x VPT loads in flight before the first use
             __builtin_amdgcn_sched_barrier(0);
@@ -778,9 +778,14 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
                 for (int v = 0; v < VPT; ++v) acc_entry(e[g][v], num[v], den[v]);
         }
         for (; j < nv; ++j) {
-            const char* row = (const char*)(tab + __builtin_amdgcn_readfirstlane(srow[j] + sj0[j]));
+            const unsigned int* row = tab + __builtin_amdgcn_readfirstlane(srow[j]);
+            const int j0 = __builtin_amdgcn_readfirstlane(sj0[j]);
 #pragma unroll
-            for (int v = 0; v < VPT; ++v) acc_entry(*(const unsigned int*)(row + boff[v]), num[v], den[v]);
+            for (int v = 0; v < VPT; ++v) {
+                int c = j0 - ks[v];
+                c += c < 0 ? npsi : 0;
+                acc_entry(row[c], num[v], den[v]);
+            }
         }
         __syncthreads();
     }
@@ -1440,7 +1445,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                    (long long)nr, sb.tpol.as<float2>());
                 CIM_HIP(hipGetLastError());
             }
-            CIM_ALLOC(sb.table, sizeof(unsigned int) * (size_t)nb * T * (npsi / 2) * 2 * npsi);
+            CIM_ALLOC(sb.table, sizeof(unsigned int) * (size_t)nb * T * (npsi / 2) * npsi);
             {
                 TimedLaunch tl(ctx, "cl_table");
                 rc = launch_cl_table(ctx, table_nh, sb.ppol.as<float2>(), sb.tpol.as<float2>(), p->apply_ctf ? 1 : 0,
@@ -1571,6 +1576,9 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 int rank = topk;
                 for (int c = 0; c < topk; ++c)
                     if (hc[(size_t)i * topk + c] == pt) { rank = c; break; }
+                // where the exact minimum ranks among the table's candidates
+                ctx->counters[rank < 4 ? "cl_rank_lt4" : rank < 8 ? "cl_rank_lt8" : rank < 16 ? "cl_rank_lt16"
+                              : rank < 24 ? "cl_rank_lt24" : "cl_rank_ge24"] += 1;
                 bool edge = false;
                 if (a.kwin && rank < topk) {
                     // exact minimum within 2 columns of its window edge: a lower one may lie outside
@@ -1914,7 +1922,7 @@ int cim_debug_cl_table(cim_ctx* ctx, int nh, int np, int T, int tpol_per_particl
     CIM_HIP(hipSetDevice(ctx->device));
     const size_t pb = sizeof(float2) * (size_t)np * nr * npsi;
     const size_t tb = sizeof(float2) * (size_t)(tpol_per_particle ? np * T : T) * nr * npsi;
-    const size_t ob = sizeof(unsigned int) * (size_t)np * T * (npsi / 2) * 2 * npsi;
+    const size_t ob = sizeof(unsigned int) * (size_t)np * T * (npsi / 2) * npsi;
     CIM_ALLOC(ctx->img, pb);
     CIM_ALLOC(ctx->aux, tb);
     CIM_ALLOC(ctx->out, ob);

@@ -404,18 +404,15 @@ class Context:
     def debug_cl_table(self, nh: int, ppol, tpol, T: int, tpol_per_particle: bool = True):
         """cim_debug_cl_table (include/cryoim_debug.h): residual tables from polar
         rays ppol (np, nr, npsi, 2) and tpol; returns (N, D) as float32 arrays
-        (np*T, npsi/2, npsi) decoded from the bfloat16 pairs (first copy of
-        each row)."""
+        (np*T, npsi/2, npsi) decoded from the bfloat16 pairs."""
         pp = _f32(ppol)
         tp = _f32(tpol)
         np_, nr, npsi = pp.shape[:3]
-        out = np.zeros((np_ * T, npsi // 2, 2 * npsi), np.uint32)
+        out = np.zeros((np_ * T, npsi // 2, npsi), np.uint32)
         rc = self.lib.cim_debug_cl_table(self.h, int(nh), np_, int(T), int(bool(tpol_per_particle)), nr, npsi,
                                          _fptr(pp), _fptr(tp), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
         self._check(rc, "cim_debug_cl_table")
-        assert np.array_equal(out[:, :, :npsi], out[:, :, npsi:])
-        e = out[:, :, :npsi]
-        return (e << 16).view(np.float32), (e & 0xFFFF0000).view(np.float32)
+        return (out << 16).view(np.float32), (out & 0xFFFF0000).view(np.float32)
 
     # -- search -------------------------------------------------------------
     def search_templates(self, params: CimParams, images, tort):

@@ -17,8 +17,8 @@ extern "C" {
  * particles x T templates from polar rays given on the host:
  * ppol np x nr x npsi and tpol (tpol_per_particle ? np T : T) x nr x npsi
  * pairs {phase (radians), |F|/2 or -1e30 when gated}; table receives
- * np T x (npsi/2) x 2 npsi entries {N (low 16 bits), D (high 16 bits)} as
- * bfloat16, each row's npsi entries written twice.  nh = 0: the exact
+ * np T x (npsi/2) x npsi entries {N (low 16 bits), D (high 16 bits)} as
+ * bfloat16.  nh = 0: the exact
  * folded phase residual on the VALU; nh = 1, 2, 4, 8: its cosine series cut
  * after nh odd harmonics, on the matrix cores. */
 int cim_debug_cl_table(cim_ctx* ctx, int nh, int np, int T, int tpol_per_particle, int nr, int npsi, const float* ppol,
