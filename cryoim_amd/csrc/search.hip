@@ -60,6 +60,7 @@ __device__ __forceinline__ void crosslin_one(const float* E1, const float* E2, c
 // of two samples folded into [0, pi], which cal_crosslin_PR_omega.f90:57-60
 // computes from two atan2 calls.
 __device__ __forceinline__ float atan2_upper(float y, float x) {
+#pragma clang fp contract(on)
     const float ax = fabsf(x);
     const float mn = fminf(ax, y), mx = fmaxf(ax, y);
     const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
@@ -89,6 +90,7 @@ __device__ __forceinline__ float fold_phase(float d) {
 // elements whose index 0 is relative coordinate -(R+1).
 __device__ __forceinline__ float2 crop_bilin(const float2* __restrict__ crop, int Dc, int R, float x, float y,
                                              int ic) {
+#pragma clang fp contract(on)
     int lx = (int)x, ly = (int)y;
     float ax = x - (float)lx, ay = y - (float)ly;
     int cx = lx - ic + R + 1, cy = ly - ic + R + 1;
@@ -157,6 +159,70 @@ struct SearchArgs {
     int wincols;
     int wsplit;               // workgroups per window (grid.x = ncand * wsplit), each wincols / wsplit columns
 };
+
+// Direction (cos, sin) of omega column k on a common line at in-plane angle a1 (deg).
+__device__ __forceinline__ void column_dir(float a1, float step, int k, float& cs, float& sn) {
+#pragma clang fp contract(on)
+    const float d2r = 0.0174532925199432957692f;
+    const float om = (a1 - step * (float)k) * d2r;
+    cs = cosf(om);
+    sn = sinf(om);
+}
+
+// Bilinear sample position of column (cs, sn) at radius FR about the centre cf:
+// lower tap (lx, ly) and fractions.
+__device__ __forceinline__ void sample_pos(float FR, float cs, float sn, float cf, int& lx, int& ly, float& ax,
+                                           float& ay) {
+#pragma clang fp contract(on)
+    const float x = FR * cs + cf, y = FR * sn + cf;
+    lx = (int)x;
+    ly = (int)y;
+    ax = x - (float)lx;
+    ay = y - (float)ly;
+}
+
+// One sample of a column from its four particle taps against the template
+// line record tv = {re, im, |F| (-1 gated), particle Fcut f0}: amplitude gate,
+// folded phase residual and weight added to the column's sums (and, PAIR, to
+// those of the Friedel column with the conjugate sample).  Every kernel that
+// scores columns calls this, so a column's bits do not depend on the kernel.
+template <bool PAIR>
+__device__ __forceinline__ void score_sample(float2 q00, float2 q10, float2 q01, float2 q11, float ax, float ay,
+                                             float4 tv, float f0, float& num, float& den, float& num2, float& den2) {
+#pragma clang fp contract(on)
+    const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay);
+    const float w01 = (1.f - ax) * ay, w11 = ax * ay;
+    const float zx = w00 * q00.x + w10 * q10.x + w01 * q01.x + w11 * q11.x;
+    const float zy = w00 * q00.y + w10 * q10.y + w01 * q01.y + w11 * q11.y;
+    const float amp1 = __builtin_amdgcn_sqrtf(zx * zx + zy * zy);
+    if (amp1 >= f0) {
+        const float w = 0.5f * (amp1 + tv.z);
+        const float pxy = zx * tv.y, pyx = zy * tv.x, pxx = zx * tv.x, pyy = zy * tv.y;
+        num += atan2_upper(fabsf(pxy - pyx), pxx + pyy) * w;
+        den += w;
+        if (PAIR) {
+            // conjugate sample of column k + Nomega/2
+            num2 += atan2_upper(fabsf(pxy + pyx), pxx - pyy) * w;
+            den2 += w;
+        }
+    }
+}
+
+// Template line record of line j, radius index fr: the template x CTF crop
+// sampled along the common line at alpha2 (deg), gated by its Fcut.
+__device__ __forceinline__ float4 template_line(const float2* __restrict__ tc, const float* __restrict__ tf, int Dc,
+                                                int R, int minR, int fr, float alpha2, float cf, int ic, float fc0) {
+#pragma clang fp contract(on)
+    const float d2r = 0.0174532925199432957692f;
+    float4 res = make_float4(0.f, 0.f, -1.f, fc0);
+    const float FR = (float)(minR + fr);
+    const float ang = alpha2 * d2r;
+    float x = FR * cosf(ang) + cf, y = FR * sinf(ang) + cf;
+    float2 z = crop_bilin(tc, Dc, R, x, y, ic);
+    float amp2 = sqrtf(z.x * z.x + z.y * z.y);
+    if (amp2 >= tf[fr]) res = make_float4(z.x, z.y, amp2, fc0);
+    return res;
+}
 
 // Particle taps: MODE 0 reads the float2 disk staged in LDS, MODE 1 reads
 // the tap-paired crop (float4 {F(x,y), F(x+1,y)}) through the caches, two
@@ -233,16 +299,9 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
             for (int q = tid; q < jn * a.nr; q += bd) {
                 const int jj = q / a.nr, fr = q - jj * a.nr;
                 const int j = j0 + jj;
-                float4 res = make_float4(0.f, 0.f, -1.f, fc0[fr]);
-                if (alpha1[j] < 999.f && alpha2[j] < 999.f) {
-                    const float FR = (float)(a.minR + fr);
-                    const float ang = alpha2[j] * d2r;
-                    float x = FR * cosf(ang) + cf, y = FR * sinf(ang) + cf;
-                    float2 z = crop_bilin(tc, a.Dc, a.R, x, y, ic);
-                    float amp2 = sqrtf(z.x * z.x + z.y * z.y);
-                    if (amp2 >= tf[fr]) res = make_float4(z.x, z.y, amp2, fc0[fr]);
-                }
-                tline[q] = res;
+                tline[q] = (alpha1[j] < 999.f && alpha2[j] < 999.f)
+                               ? template_line(tc, tf, a.Dc, a.R, a.minR, fr, alpha2[j], cf, ic, fc0[fr])
+                               : make_float4(0.f, 0.f, -1.f, fc0[fr]);
             }
             __syncthreads();
             for (int jj = 0; jj < jn; ++jj) {
@@ -254,9 +313,7 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
                 for (int v = 0; v < VPT; ++v) {
                     int k = kbase + tid + v * bd;
                     if (k >= ncol) k -= ncol;
-                    const float om = (a1 - a.step * (float)k) * d2r;
-                    cs[v] = cosf(om);
-                    sn[v] = sinf(om);
+                    column_dir(a1, a.step, k, cs[v], sn[v]);
                 }
                 const float4* tl = tline + jj * a.nr;
                 for (int fr = 0; fr < a.nr; ++fr) {
@@ -268,9 +325,9 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
 #pragma unroll
                     for (int v = 0; v < VPT; ++v) {
                         if (tid + v * bd >= nact) continue;
-                        const float x = FR * cs[v] + cf, y = FR * sn[v] + cf;
-                        const int lx = (int)x, ly = (int)y;
-                        const float ax = x - (float)lx, ay = y - (float)ly;
+                        int lx, ly;
+                        float ax, ay;
+                        sample_pos(FR, cs[v], sn[v], cf, lx, ly, ax, ay);
                         float2 q00, q10, q01, q11;
                         if (MODE == 0) {
                             const int r = ly - ic + a.R + 1;
@@ -282,22 +339,7 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
                             q00 = make_float2(u.x, u.y); q10 = make_float2(u.z, u.w);
                             q01 = make_float2(w4.x, w4.y); q11 = make_float2(w4.z, w4.w);
                         }
-                        const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay);
-                        const float w01 = (1.f - ax) * ay, w11 = ax * ay;
-                        const float zx = w00 * q00.x + w10 * q10.x + w01 * q01.x + w11 * q11.x;
-                        const float zy = w00 * q00.y + w10 * q10.y + w01 * q01.y + w11 * q11.y;
-                        const float amp1 = __builtin_amdgcn_sqrtf(zx * zx + zy * zy);
-                        if (amp1 >= f0) {
-                            const float w = 0.5f * (amp1 + tv.z);
-                            const float pxy = zx * tv.y, pyx = zy * tv.x, pxx = zx * tv.x, pyy = zy * tv.y;
-                            num[v] += atan2_upper(fabsf(pxy - pyx), pxx + pyy) * w;
-                            den[v] += w;
-                            if (PAIR) {
-                                // conjugate sample of column k + Nomega/2
-                                num2[v] += atan2_upper(fabsf(pxy + pyx), pxx - pyy) * w;
-                                den2[v] += w;
-                            }
-                        }
+                        score_sample<PAIR>(q00, q10, q01, q11, ax, ay, tv, f0, num[v], den[v], num2[v], den2[v]);
                     }
                 }
             }
@@ -344,6 +386,119 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
         a.outv[oslot] = best;
         a.outi[oslot] = bidx;
         if (a.nsamples) atomicAdd(a.nsamples, pairs * (unsigned long long)a.Nomega);
+    }
+}
+
+// Exact re-score of the tabulated search's candidates, CPW candidates per
+// wave: lane l scores column kwin[c] + l % W of candidate c = l / W (and its
+// Friedel column), W = 64 / CPW columns around the omega of the candidate's
+// table minimum.  Each column takes the same terms in the same order through
+// the same arithmetic as in k_cl_search (template_line, sample_pos,
+// score_sample), so its bits equal the plain search's; the lanes of the
+// candidates differ only in which lines and radii they skip.  Particle taps
+// are read tap-paired through the caches.  The first minimum of each
+// candidate's columns goes to outv/outi[p * ncand + candidate].
+template <int CPW, bool PAIR>
+__global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
+    constexpr int W = 64 / CPW;
+    extern __shared__ float4 smem4[];
+    float4* tline = smem4;                                           // CPW x chunk x nr
+    float* alpha1 = reinterpret_cast<float*>(tline + (size_t)CPW * a.chunk * a.nr);   // CPW x 64
+    float* alpha2 = alpha1 + CPW * 64;
+    float* fc0 = alpha2 + CPW * 64;                                   // nr
+    const int lane = threadIdx.x, c = lane / W, col = lane - c * W;
+    const int p = blockIdx.y, g0 = blockIdx.x * CPW;
+    const bool act = g0 + c < a.ncand;
+    const int slot = act ? g0 + c : g0;
+    const long long cbase = (long long)p * a.ncand;
+    const int pt = a.cand[cbase + slot];
+    const float2* tcp = a.tcrop + (long long)p * a.tc_pstride;
+    const float* tfp = a.tfcut + (long long)p * a.tf_pstride;
+    const float4* pc4 = a.pcrop4 + (long long)p * a.Dc * a.Dc;
+    const int ic = a.box / 2;
+    const float cf = (float)ic;
+    const float d2r = 0.0174532925199432957692f;
+    const int ncol = PAIR ? a.Nomega / 2 : a.Nomega;
+    int k = a.kwin[cbase + slot] + col;
+    if (k >= ncol) k -= ncol;
+    for (int i = lane; i < a.nr; i += 64) fc0[i] = a.fcut0[(long long)p * a.nr + i];
+    float num = 0.f, den = 0.f, num2 = 0.f, den2 = 0.f;
+    unsigned long long pairs = 0;
+    for (int t = 0; t < a.T; ++t) {
+        const float2* tc = tcp + (long long)t * a.Dc * a.Dc;
+        const float* tf = tfp + (long long)t * a.nr;
+        __syncthreads();
+        for (int q = lane; q < CPW * a.nops; q += 64) {
+            const int cc = q / a.nops, j = q - cc * a.nops;
+            const int ptc = a.cand[cbase + min(g0 + cc, a.ncand - 1)];
+            float E1[9], E2[9];
+            euler_f(a.pts[2 * ptc], a.pts[2 * ptc + 1], 0.f, E1);
+            euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
+            crosslin_one(E1, E2, a.ops + 9 * j, &alpha1[cc * 64 + j], &alpha2[cc * 64 + j]);
+        }
+        __syncthreads();
+        for (int j0 = 0; j0 < a.nops; j0 += a.chunk) {
+            const int jn = min(a.chunk, a.nops - j0);
+            for (int q = lane; q < CPW * jn * a.nr; q += 64) {
+                const int cc = q / (jn * a.nr), rem = q - cc * jn * a.nr;
+                const int jj = rem / a.nr, fr = rem - jj * a.nr;
+                const int j = j0 + jj;
+                const float al1 = alpha1[cc * 64 + j], al2 = alpha2[cc * 64 + j];
+                tline[(cc * a.chunk + jj) * a.nr + fr] =
+                    (al1 < 999.f && al2 < 999.f) ? template_line(tc, tf, a.Dc, a.R, a.minR, fr, al2, cf, ic, fc0[fr])
+                                                 : make_float4(0.f, 0.f, -1.f, fc0[fr]);
+            }
+            __syncthreads();
+            for (int jj = 0; jj < jn; ++jj) {
+                const int j = j0 + jj;
+                const float a1 = alpha1[c * 64 + j];
+                if (!(a1 < 999.f && alpha2[c * 64 + j] < 999.f)) continue;
+                float cs, sn;
+                column_dir(a1, a.step, k, cs, sn);
+                const float4* tl = tline + (c * a.chunk + jj) * a.nr;
+                for (int fr = 0; fr < a.nr; ++fr) {
+                    const float4 tv = tl[fr];
+                    if (tv.z < 0.f) continue;
+                    ++pairs;
+                    const float FR = (float)(a.minR + fr);
+                    int lx, ly;
+                    float ax, ay;
+                    sample_pos(FR, cs, sn, cf, lx, ly, ax, ay);
+                    const float4* r0 = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
+                    const float4 u = r0[0], w4 = r0[a.Dc];
+                    score_sample<PAIR>(make_float2(u.x, u.y), make_float2(u.z, u.w), make_float2(w4.x, w4.y),
+                                       make_float2(w4.z, w4.w), ax, ay, tv, tv.w, num, den, num2, den2);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const float r2d = 57.2957795130823208768f;
+    float best = 3.0e38f;
+    int bidx = 0x7fffffff;
+#pragma unroll
+    for (int h = 0; h < (PAIR ? 2 : 1); ++h) {
+        const float nn = h ? num2 : num, dd = h ? den2 : den;
+        const float pr = dd > 1.0e-10f ? nn / dd * r2d : 999.f;
+        const int li = a.pbase[pt] + k + h * ncol;
+        if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
+    }
+    // first minimum over the W lanes of each candidate
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bidx, o, 64);
+        if (ov < best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+    }
+    if (col == 0 && act) {
+        a.outv[cbase + slot] = best;
+        a.outi[cbase + slot] = bidx;
+    }
+    if (a.nsamples) {
+        unsigned long long tot = act ? pairs * (PAIR ? 2ull : 1ull) : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+        if (lane == 0) atomicAdd(a.nsamples, tot);
     }
 }
 
@@ -1296,9 +1451,20 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     for (int i = 0; i < npts; ++i) base2pt[g.base[i]] = i;
     // exact re-scoring window: wincols columns per (paired) omega half around the table minimum
     const int ncolx = pair ? g.No / 2 : g.No;
-    int wincols = 64;
-    if (const char* ew = getenv("CIM_TABLE_WINDOW")) wincols = atoi(ew) > 0 ? ((atoi(ew) + 63) / 64) * 64 : ncolx;
-    if (wincols > ncolx) wincols = ncolx;
+    // CIM_RESCORE_CPW candidates (1, 2, 4 or 8) share a wave, each with 64 / CPW columns; CPW = 1
+    // scores windows of CIM_TABLE_WINDOW (64) columns, one wave per candidate
+    int rescore_cpw = 4;
+    if (const char* ec = getenv("CIM_RESCORE_CPW")) rescore_cpw = atoi(ec);
+    if (rescore_cpw != 1 && rescore_cpw != 2 && rescore_cpw != 4 && rescore_cpw != 8) rescore_cpw = 4;
+    int wincols = 64 / rescore_cpw;
+    if (const char* ew = getenv("CIM_TABLE_WINDOW")) {
+        wincols = atoi(ew) > 0 ? ((atoi(ew) + 63) / 64) * 64 : ncolx;
+        rescore_cpw = 1;
+    }
+    if (wincols > ncolx) {
+        wincols = ncolx;
+        rescore_cpw = 1;
+    }
 
     // the tables take 4 T npsi^2 bytes per particle (8.3 MB at T = 4, 0.5 deg): batches of
     // CIM_TABLE_BATCH (default 512, 4.2 GB) particles.  Large batches keep enough waves of
@@ -1519,12 +1685,23 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 // (the chunking does not change any column's accumulation order)
                 a.chunk = std::min(chunk, 6);
             }
-            nscore = topk * (a.kwin ? a.wsplit : 1);
+            nscore = topk * (a.kwin && rescore_cpw == 1 ? a.wsplit : 1);
         }
         {
             TimedLaunch tl(ctx, tables ? "cl_rescore" : "cl_search");
             // windows: one wave of single-column lanes in the same kernel variant
-            if (a.kwin) {
+            if (a.kwin && rescore_cpw > 1) {
+                a.chunk = std::min(chunk, 3);
+                const int lw = (int)(sizeof(float4) * (size_t)rescore_cpw * a.chunk * nr +
+                                     sizeof(float) * (2 * 64 * rescore_cpw + nr));
+                const dim3 grid((unsigned)((topk + rescore_cpw - 1) / rescore_cpw), (unsigned)nb);
+#define CIM_RESCORE(C, P) hipLaunchKernelGGL((k_cl_rescore<C, P>), grid, dim3(64), lw, ctx->stream, a)
+                if (rescore_cpw == 2) { if (sh.pair) CIM_RESCORE(2, true); else CIM_RESCORE(2, false); }
+                else if (rescore_cpw == 4) { if (sh.pair) CIM_RESCORE(4, true); else CIM_RESCORE(4, false); }
+                else { if (sh.pair) CIM_RESCORE(8, true); else CIM_RESCORE(8, false); }
+#undef CIM_RESCORE
+                CIM_HIP(hipGetLastError());
+            } else if (a.kwin) {
                 const int lw = (int)(misc + sizeof(float4) * (size_t)a.chunk * nr) +
                                (sh.lds ? (int)(sizeof(float2) * (size_t)disk_total) : 0);
                 rc = launch_search(ctx, a, nb, lw, sh.lds ? 0 : 1, sh.pair, 1, wincols / a.wsplit);
