@@ -1200,13 +1200,13 @@ static void make_disk(int R, std::vector<int>& rows, int& total) {
     }
 }
 
-static int prep_templates(cim_ctx* ctx, const cim_params* p, int T, const float* images, const float* tort) {
+// The template images (T x box^2, in ctx->img) masked at their own shifts,
+// transformed, centre-shifted and Fcut-fitted (load_template.f90:35-48).
+static int prep_templates_img(cim_ctx* ctx, const cim_params* p, int T, const float* tort) {
     SearchState& s = ctx->srch;
     const int box = p->box;
-    const long long N = (long long)box * box;
     const int nr = p->maxR - p->minR + 1;
-    CIM_ALLOC(ctx->img, sizeof(float) * N * T);
-    CIM_HIP(hipMemcpyAsync(ctx->img.ptr, images, sizeof(float) * N * T, hipMemcpyHostToDevice, ctx->stream));
+    const long long N = (long long)box * box;
     std::vector<float> parts((size_t)T * CIM_PART_STRIDE, 0.f);
     s.tang.assign(3 * T, 0.f);
     for (int t = 0; t < T; ++t) {
@@ -1237,6 +1237,79 @@ static int prep_templates(cim_ctx* ctx, const cim_params* p, int T, const float*
     s.minR = p->minR;
     s.maxR = p->maxR;
     s.ready = true;
+    return CIM_OK;
+}
+
+static int prep_templates(cim_ctx* ctx, const cim_params* p, int T, const float* images, const float* tort) {
+    const long long N = (long long)p->box * p->box;
+    CIM_ALLOC(ctx->img, sizeof(float) * N * T);
+    CIM_HIP(hipMemcpyAsync(ctx->img.ptr, images, sizeof(float) * N * T, hipMemcpyHostToDevice, ctx->stream));
+    return prep_templates_img(ctx, p, T, tort);
+}
+
+// project_template2d.f90:3-80 (and mode 0's project2d.f90): the real-space
+// projection of the nx^3 model at each template's orientation.  One thread
+// per (pixel, template) walks its voxel column: for each voxel strictly
+// inside radius nx/2 - 1, X = R (ix, iy, iz) - nx/2 + nx/2 and the trilinear
+// value of the model at X is added, iz ascending, in the reference's REAL*4
+// order of operations (no contraction).  R: T row-major euler_mat matrices.
+__global__ void k_project_template(const float* __restrict__ vol, int nx, const float* __restrict__ R,
+                                   float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long n2 = (long long)nx * nx;
+    const int t = blockIdx.y;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n2) return;
+    const int ix = (int)(idx % nx), iy = (int)(idx / nx), h = nx / 2;
+    const float* r = R + 9 * t;
+    const float x0 = (float)ix - (float)h, x1 = (float)iy - (float)h, lim = (float)(h - 1);
+    float s = 0.f;
+    for (int iz = 0; iz < nx; ++iz) {
+        const float x2 = (float)iz - (float)h;
+        if (sqrtf(x0 * x0 + x1 * x1 + x2 * x2) >= lim) continue;
+        const float X = r[0] * x0 + r[1] * x1 + r[2] * x2 + (float)h;
+        const float Y = r[3] * x0 + r[4] * x1 + r[5] * x2 + (float)h;
+        const float Z = r[6] * x0 + r[7] * x1 + r[8] * x2 + (float)h;
+        const int jx = (int)X, jy = (int)Y, jz = (int)Z;
+        const float xb = X - (float)jx, yb = Y - (float)jy, zb = Z - (float)jz;
+        const float* v = vol + (long long)jz * n2 + (long long)jy * nx + jx;
+        s = s + (1.f - xb) * (1.f - yb) * (1.f - zb) * v[0];
+        s = s + xb * (1.f - yb) * (1.f - zb) * v[1];
+        s = s + (1.f - xb) * yb * (1.f - zb) * v[nx];
+        s = s + xb * yb * (1.f - zb) * v[nx + 1];
+        s = s + (1.f - xb) * (1.f - yb) * zb * v[n2];
+        s = s + xb * (1.f - yb) * zb * v[n2 + 1];
+        s = s + (1.f - xb) * yb * zb * v[n2 + nx];
+        s = s + xb * yb * zb * v[n2 + nx + 1];
+    }
+    out[(long long)t * n2 + idx] = s;
+}
+
+// euler_mat.f90:3-21 in float32 on the host (R row-major).
+static void euler_host(float th0, float ph0, float om0, float* R) {
+    const float d2r = std::atan2(1.0f, 1.0f) / 45.0f;
+    const float th = th0 * d2r, ph = ph0 * d2r, om = om0 * d2r;
+    const float c1 = std::cos(ph), c2 = std::cos(th), c3 = std::cos(om);
+    const float s1 = std::sin(ph), s2 = std::sin(th), s3 = std::sin(om);
+    R[0] = c1 * c2 * c3 - s1 * s3; R[1] = -c1 * c2 * s3 - c3 * s1; R[2] = c1 * s2;
+    R[3] = c2 * c3 * s1 + c1 * s3; R[4] = -c2 * s1 * s3 + c1 * c3; R[5] = s1 * s2;
+    R[6] = -c3 * s2;               R[7] = s2 * s3;                 R[8] = c2;
+}
+
+// Project T templates from the model (host, box^3) into ctx->img.
+static int project_templates(cim_ctx* ctx, int box, int T, const float* model, const float* tort) {
+    const long long n3 = (long long)box * box * box, n2 = (long long)box * box;
+    std::vector<float> R(9 * (size_t)T);
+    for (int t = 0; t < T; ++t) euler_host(tort[5 * t], tort[5 * t + 1], tort[5 * t + 2], &R[9 * (size_t)t]);
+    CIM_ALLOC(ctx->aux, sizeof(float) * n3);
+    CIM_ALLOC(ctx->aux2, sizeof(float) * 9 * T);
+    CIM_ALLOC(ctx->img, sizeof(float) * n2 * T);
+    CIM_HIP(hipMemcpyAsync(ctx->aux.ptr, model, sizeof(float) * n3, hipMemcpyHostToDevice, ctx->stream));
+    CIM_HIP(hipMemcpyAsync(ctx->aux2.ptr, R.data(), sizeof(float) * 9 * T, hipMemcpyHostToDevice, ctx->stream));
+    TimedLaunch tl(ctx, "project");
+    hipLaunchKernelGGL(k_project_template, dim3((unsigned)((n2 + 255) / 256), T), dim3(256), 0, ctx->stream,
+                       ctx->aux.as<float>(), box, ctx->aux2.as<float>(), ctx->img.as<float>());
+    CIM_HIP(hipGetLastError());
     return CIM_OK;
 }
 
@@ -2109,6 +2182,32 @@ int cim_debug_cl_table(cim_ctx* ctx, int nh, int np, int T, int tpol_per_particl
                              ctx->out.as<unsigned int>());
     if (rc) return rc;
     CIM_HIP(hipMemcpyAsync(table, ctx->out.ptr, ob, hipMemcpyDeviceToHost, ctx->stream));
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    return CIM_OK;
+}
+
+int cim_search_templates_model(cim_ctx* ctx, const cim_params* p, int T, const float* model, const float* tort) {
+    if (!ctx) return CIM_E_ARG;
+    int rc = check_params_public(ctx, p);
+    if (rc) return rc;
+    if (T < 1 || !model || !tort) return set_error(ctx, CIM_E_ARG, "need T >= 1 templates and a model");
+    if (p->box < 4) return set_error(ctx, CIM_E_ARG, "box %d too small for a projection", p->box);
+    CIM_HIP(hipSetDevice(ctx->device));
+    if ((rc = project_templates(ctx, p->box, T, model, tort))) return rc;
+    return prep_templates_img(ctx, p, T, tort);
+}
+
+int cim_project_templates(cim_ctx* ctx, int box, int T, const float* model, const float* angles, float* out) {
+    if (!ctx) return CIM_E_ARG;
+    if (box < 4 || T < 1 || !model || !angles || !out) return set_error(ctx, CIM_E_ARG, "bad sizes or NULL arrays");
+    CIM_HIP(hipSetDevice(ctx->device));
+    std::vector<float> tort(5 * (size_t)T, 0.f);
+    for (int t = 0; t < T; ++t)
+        for (int k = 0; k < 3; ++k) tort[5 * (size_t)t + k] = angles[3 * (size_t)t + k];
+    int rc = project_templates(ctx, box, T, model, tort.data());
+    if (rc) return rc;
+    CIM_HIP(hipMemcpyAsync(out, ctx->img.ptr, sizeof(float) * (size_t)box * box * T, hipMemcpyDeviceToHost,
+                           ctx->stream));
     CIM_HIP(hipStreamSynchronize(ctx->stream));
     return CIM_OK;
 }

@@ -122,6 +122,23 @@ module cim_iface
             real(c_float) :: images(*), tort(*)
         end function
 
+        integer(c_int) function cim_search_templates_model(ctx, p, T, model, tort) &
+                bind(C, name='cim_search_templates_model')
+            import :: c_ptr, c_int, c_float, cim_params
+            type(c_ptr), value :: ctx
+            type(cim_params) :: p
+            integer(c_int), value :: T
+            real(c_float) :: model(*), tort(*)
+        end function
+
+        integer(c_int) function cim_project_templates(ctx, box, T, model, angles, out) &
+                bind(C, name='cim_project_templates')
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: box, T
+            real(c_float) :: model(*), angles(*), out(*)
+        end function
+
         integer(c_int) function cim_search_global(ctx, p, n, images, parts) bind(C, name='cim_search_global')
             import :: c_ptr, c_int, c_float, cim_params
             type(c_ptr), value :: ctx

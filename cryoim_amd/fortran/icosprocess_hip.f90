@@ -94,6 +94,12 @@ program icosprocess_hip
         stop 2
     end if
 
+    ! mode 0 (0main.f90:69-86): the template stack projected from model3d (project2d.f90)
+    if (mode == 0) then
+        call run_project0()
+        stop
+    end if
+
     ! initialvalue: box from the first stack, particle count from the ortfile
     open(21, file=trim(ortfile), status='old')
     read(21, *) stck0
@@ -160,7 +166,7 @@ program icosprocess_hip
     case (6)
         call run_local()
     case default
-        print *, 'icosprocess_hip implements modes 1, 3, 4, 5 and 6'
+        print *, 'icosprocess_hip implements modes 0, 1, 3, 4, 5 and 6'
         stop 2
     end select
     rc = cim_finalize(ctx)
@@ -471,7 +477,7 @@ contains
 
     subroutine run_search()
         integer, parameter :: B = 256
-        real(c_float), allocatable :: imgs(:), rec(:, :), timgs(:), tort(:, :)
+        real(c_float), allocatable :: imgs(:), rec(:, :), timgs(:), tort(:, :), model(:)
         integer :: i0, i1, i, k, u, pid
         real :: ox
         ! template bank (load_template.f90:18-33)
@@ -488,12 +494,24 @@ contains
             read(u, *) pid, tort(1, i), tort(2, i), tort(3, i), tort(4, i), tort(5, i)
         end do
         close(u)
-        open(newunit=u, file=trim(templatestck), form='unformatted', access='stream', status='old')
-        read(u, pos=1025) timgs
-        close(u)
         p%mask_on = merge(1, 0, imgmask_on == 'y')
-        rc = cim_search_templates(ctx, p, T, timgs, tort)
-        call cim_check(ctx, rc, 'cim_search_templates')
+        if (len_trim(templatestck) > 0) then
+            open(newunit=u, file=trim(templatestck), form='unformatted', access='stream', status='old')
+            read(u, pos=1025) timgs
+            close(u)
+            rc = cim_search_templates(ctx, p, T, timgs, tort)
+            call cim_check(ctx, rc, 'cim_search_templates')
+        else
+            ! load_template.f90:32-33: no templatestck, project the templates from model3d
+            ! (project_template2d.f90:21-26: header, then FFTsize**3 reals)
+            allocate(model(int(fftsize, 8)**3))
+            open(newunit=u, file=trim(model3d), form='unformatted', access='stream', status='old')
+            read(u, pos=1025) model
+            close(u)
+            rc = cim_search_templates_model(ctx, p, T, model, tort)
+            call cim_check(ctx, rc, 'cim_search_templates_model')
+            deallocate(model)
+        end if
         allocate(imgs(fftsize * fftsize * B), rec(CIM_PART_STRIDE, B))
         open(41, file=trim(newortfile))
         do i0 = first, last, B
@@ -521,6 +539,47 @@ contains
     ! icosprocess mode 3 (0main.f90:237-241): calc_center when model3d is
     ! given, calc_center_selfrot180 otherwise, then output_ort.f90:1-17
     ! (dtheta, dphi, domega stay 0 from load_original_ortcent.f90:7).
+    ! project2d.f90:2-86: the model3d header with nz = the template count, then one
+    ! real-space projection per templateort line
+    subroutine run_project0()
+        integer(c_int32_t) :: h(256)
+        real(c_float), allocatable :: model(:), ang(:, :), out(:)
+        integer :: u, i, T, pid, n
+        real :: th, ph, om
+        open(newunit=u, file=trim(model3d), form='unformatted', access='stream', status='old')
+        read(u) h
+        n = h(1)
+        allocate(model(int(n, 8)**3))
+        read(u) model
+        close(u)
+        open(newunit=u, file=trim(templateort), status='old')
+        T = 0
+        do
+            read(u, *, iostat=rc) pid, th, ph, om
+            if (rc /= 0) exit
+            T = T + 1
+        end do
+        rewind(u)
+        allocate(ang(3, T), out(int(n, 8) * n * T))
+        do i = 1, T
+            read(u, *) pid, ang(1, i), ang(2, i), ang(3, i)
+        end do
+        close(u)
+        rc = cim_init(ctx, 0)
+        if (rc /= CIM_OK) then
+            print *, 'no HIP device (cim_init returned', rc, ')'
+            stop 1
+        end if
+        rc = cim_project_templates(ctx, n, T, model, ang, out)
+        call cim_check(ctx, rc, 'cim_project_templates')
+        rc = cim_finalize(ctx)
+        h(3) = T
+        open(newunit=u, file=trim(templatestck), form='unformatted', access='stream', status='replace')
+        write(u) h
+        write(u) out
+        close(u)
+    end subroutine
+
     subroutine run_center()
         integer, parameter :: B = 1024
         real(c_float), allocatable :: imgs(:), rec(:, :), vol(:), cc(:)

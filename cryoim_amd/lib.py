@@ -146,6 +146,8 @@ def load() -> ctypes.CDLL:
         "cim_local_box_stack": (ip, [vp, ip, fp, fp, ip, ip, dp, fp, ip, ctypes.c_float, fp, fp]),
         "cim_local_box_stack_dev": (ip, [vp, ip, vp, fp, ip, ip, dp, fp, ip, ctypes.c_float, vp, fp]),
         "cim_local_box_stack_noalloc": (ip, [vp, ip, vp, fp, ip, ip, dp, fp, ip, ctypes.c_float, vp, fp]),
+        "cim_search_templates_model": (ip, [vp, P, ip, fp, fp]),
+        "cim_project_templates": (ip, [vp, ip, ip, fp, fp, fp]),
         "cim_debug_cl_table": (ip, [vp, ip, ip, ip, ip, ip, ip, fp, fp, ctypes.POINTER(ctypes.c_uint32)]),
         "cim_lref_model": (ip, [vp, P, fp]),
         "cim_center_model": (ip, [vp, P, fp]),
@@ -415,6 +417,23 @@ class Context:
         return (out << 16).view(np.float32), (out & 0xFFFF0000).view(np.float32)
 
     # -- search -------------------------------------------------------------
+    def search_templates_model(self, params: CimParams, model, tort):
+        """Template bank projected from the box^3 model (load_template.f90:32-33)."""
+        m = _f32(model)
+        to = _f32(tort).reshape(-1, 5)
+        self._check(self.lib.cim_search_templates_model(self.h, ctypes.byref(params), to.shape[0], _fptr(m),
+                                                        _fptr(to)), "cim_search_templates_model")
+
+    def project_templates(self, model, angles):
+        """Real-space projections (T, box, box) of the model (project_template2d.f90)."""
+        m = _f32(model)
+        ang = _f32(angles).reshape(-1, 3)
+        box = m.shape[-1]
+        out = np.zeros((ang.shape[0], box, box), np.float32)
+        self._check(self.lib.cim_project_templates(self.h, box, ang.shape[0], _fptr(m), _fptr(ang), _fptr(out)),
+                    "cim_project_templates")
+        return out
+
     def search_templates(self, params: CimParams, images, tort):
         im = _f32(images)
         to = _f32(tort).reshape(-1, 5)

@@ -280,6 +280,17 @@ int cim_fsc_map(cim_ctx* ctx, int box, int maxR, float imgmask, int variant, int
  * masked, checkerboarded, transformed and centre-shifted on the device. */
 int cim_search_templates(cim_ctx* ctx, const cim_params* p, int T, const float* images,
                          const float* tort);
+/* The same bank with the templates projected from a model instead
+ * (load_template.f90:32-33 when no templatestck is given):
+ * project_template2d.f90:3-80, the real-space projection of the box^3 map
+ * model at each template's angles (tort: T*5 theta phi omega x y) --
+ * per pixel the trilinear model values along its rotated voxel column,
+ * voxels strictly inside radius box/2 - 1 -- then the template-stack path. */
+int cim_search_templates_model(cim_ctx* ctx, const cim_params* p, int T, const float* model, const float* tort);
+/* The projections alone (icosprocess mode 0, project2d.f90:2-86, which
+ * writes them as the template stack; the same arithmetic as
+ * project_template2d.f90): out = T box^2 real images for angles T*3. */
+int cim_project_templates(cim_ctx* ctx, int box, int T, const float* model, const float* angles, float* out);
 /* Exhaustive common-lines phase-residual search over the AU grid
  * (ort_globesearch.f90:34-134).  parts in/out: theta, phi, omega and PR are
  * replaced by the search result; x, y are used as given. */

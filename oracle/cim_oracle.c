@@ -1314,6 +1314,43 @@ EXPORT void o_img2d_applyctf(real* a, int nx, const real* ctf)
     free(b);
 }
 
+/* project_template2d.f90:3-80: the template of one orientation projected in
+ * real space from an nx^3 model: for every voxel (ix, iy, iz) strictly inside
+ * radius nx/2 - 1 of the centre, X = R (ix, iy, iz) - nx/2 + nx/2 (R of
+ * euler_mat, row-major), the trilinear value of the model at X is added to
+ * pixel (ix, iy) -- the voxel's own column, iz ascending -- as
+ * slice + t1 + ... + t8 with ti = ((wx wy) wz) v.  out: nx^2. */
+EXPORT void o_project_template2d(const real* vol, int nx, const real* R, real* out)
+{
+    const int h = nx / 2;
+    const long n2 = (long)nx * nx;
+    for (long i = 0; i < n2; ++i) out[i] = 0;
+    for (int iz = 0; iz < nx; ++iz)
+        for (int iy = 0; iy < nx; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const real xp[3] = {(real)ix - (real)h, (real)iy - (real)h, (real)iz - (real)h};
+                const real d = (real)sqrt(xp[0] * xp[0] + xp[1] * xp[1] + xp[2] * xp[2]);
+                if (d >= (real)(h - 1)) continue;
+                real X[3];
+                for (int a = 0; a < 3; ++a) X[a] = R[3 * a] * xp[0] + R[3 * a + 1] * xp[1] + R[3 * a + 2] * xp[2];
+                for (int a = 0; a < 3; ++a) X[a] += (real)h;
+                const int jx = (int)X[0], jy = (int)X[1], jz = (int)X[2];
+                const real xb = X[0] - (real)jx, yb = X[1] - (real)jy, zb = X[2] - (real)jz;
+                const long b = (long)jz * n2 + (long)jy * nx + jx;
+                const real* v = vol;
+                real s = out[(long)iy * nx + ix];
+                s = s + ((real)1 - xb) * ((real)1 - yb) * ((real)1 - zb) * v[b];
+                s = s + xb * ((real)1 - yb) * ((real)1 - zb) * v[b + 1];
+                s = s + ((real)1 - xb) * yb * ((real)1 - zb) * v[b + nx];
+                s = s + xb * yb * ((real)1 - zb) * v[b + nx + 1];
+                s = s + ((real)1 - xb) * ((real)1 - yb) * zb * v[b + n2];
+                s = s + xb * ((real)1 - yb) * zb * v[b + n2 + 1];
+                s = s + ((real)1 - xb) * yb * zb * v[b + n2 + nx];
+                s = s + xb * yb * zb * v[b + n2 + nx + 1];
+                out[(long)iy * nx + ix] = s;
+            }
+}
+
 /* project2d_FT3dTo2d: icosprocess/SRC/project2d_FT3dTo2d.f90:2-94.  Central
  * slice of (fr, fi) by trilinear interpolation inside radius nx/2-2, then a
  * checkerboarded backward img2d_FFT; the real part is the projection. */

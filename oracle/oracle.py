@@ -364,6 +364,16 @@ class Oracle:
         self.L.o_img2d_applyctf(self._p(a), a.shape[-1], self._p(self._a(ctf)))
         return a
 
+    def project_template2d(self, vol, angles):
+        """project_template2d.f90:3-80: the real-space projection (nx, nx) of
+        the nx^3 model at orientation angles (theta, phi, omega)."""
+        v = self._a(vol)
+        n = v.shape[-1]
+        R = self._a(self.euler_mat(*angles)).reshape(9)
+        out = np.zeros((n, n), self.dt)
+        self.L.o_project_template2d(self._p(v), n, self._p(R), self._p(out))
+        return out
+
     def project2d_ft(self, fr, fi, th, ph, om):
         fr, fi = self._a(fr), self._a(fi)
         n = fr.shape[-1]

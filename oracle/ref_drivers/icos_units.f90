@@ -108,6 +108,29 @@ program icos_units
             call cal_crosslin_PR(r(1))
             write(12) r(1)
         end if
+    case ('project')
+        ! project_template2d.f90 through the module state load_template sets up:
+        ! FFTsize, totaltemplate, template(:) angles and the model3d file
+        read(11) nx, T
+        FFTsize = nx
+        totaltemplate = T
+        allocate(template(0:T), template2d(nx*nx, 2, 0:T))
+        do k = 1, T
+            read(11) th, ph, om
+            template(k)%theta = th
+            template(k)%phi = ph
+            template(k)%omega = om
+        end do
+        allocate(a(nx*nx*nx)); read(11) a
+        model3d = trim(fout) // '.model'
+        open(13, file=trim(model3d), form='unformatted', access='stream', status='replace')
+        write(13) (0, i = 1, 256)
+        write(13) a
+        close(13)
+        call project_template2d
+        do k = 1, T
+            write(12) template2d(:, 1, k)
+        end do
     case default
         print *, 'unknown case ', trim(what)
         stop 2

@@ -412,6 +412,45 @@ def program_vectors() -> dict:
     return g
 
 
+def projection_vectors() -> dict:
+    """project_template2d through the unit driver on the 48-pixel phantom map,
+    icosprocess mode 0 (project2d: the template stack) and mode 4 without a
+    templatestck (load_template projects the templates from model3d) on the
+    program_vectors data set."""
+    g = {}
+    box, apix, n = 48, 2.0, 30
+    ph = synth.make_phantom(box, radius=(13, 19), sigma=(1.2, 2.0), seed=7)
+    vol = synth.phantom_map(ph).astype(f4)
+    ang = np.array([[72.0, 8.0, 20.0], [80.0, -15.0, 140.0], [88.0, 3.0, 300.0], [33.3, 71.7, 189.1]], f4)
+    g["proj_model"], g["proj_angles"] = vol, ang
+    g["proj_out"] = _run_unit("icos_units", "project", _b(np.array([box, len(ang)], i4), ang, vol),
+                              len(ang) * box * box).reshape(len(ang), box, box)
+    ps = synth.make_particles(ph, n, seed=8, shift_max=1.5, snr=1.0, ctf=True, apix=apix, df_range=(1.0, 2.5))
+    tang = np.array([[72.0, 8.0, 20.0], [80.0, -15.0, 140.0], [88.0, 3.0, 300.0]])
+    exe = os.path.join(REF, "icosprocess")
+    with tempfile.TemporaryDirectory() as d:
+        formats.write_mrc(os.path.join(d, "stack.mrc"), ps.images)
+        formats.write_mrc(os.path.join(d, "model.mrc"), vol)
+        formats.write_template_ort(os.path.join(d, "tort.txt"), tang)
+        parts = [formats.Particle("stack.mrc", i + 1, *ps.angles[i], *ps.shifts[i], 0.0, i + 1, i + 1, ps.df[i, 0],
+                                  ps.df[i, 1], 0.0) for i in range(n)]
+        formats.write_ortfile(os.path.join(d, "ort.txt"), parts)
+
+        def run(args):
+            r = subprocess.run([exe, *args], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            return r.stdout
+
+        run(["mode=0", "model3d=model.mrc", "templateort=tort.txt", "templatestck=tstk.mrc"])
+        g["mode0_stack"] = formats.read_mrc(os.path.join(d, "tstk.mrc"))
+        run(["mode=4", "templateort=tort.txt", "model3d=model.mrc", "searchstep=6.0", "newortfile=m4.txt",
+             "apix=2.0", "maxRes=6.0", "minRes=100", "imgmask=40", "ortfile=ort.txt", "last=6"])
+        m4 = formats.read_ortfile900(os.path.join(d, "m4.txt"))
+        g["mode4_model_out"] = np.array([[p.theta, p.phi, p.omega, p.x, p.y, p.pr] for p in m4], f4)
+    return g
+
+
 def _read_fsc_txt(path) -> np.ndarray:
     """FSC.txt rows 'ir FSC sqrt(2FSC/(1+FSC))' (list-directed, NaN allowed)."""
     rows = []
@@ -698,7 +737,7 @@ def main():
             ("local", local_program_vectors), ("center", center_program_vectors),
             ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors),
             ("symc", symc_program_vectors), ("lrec", local_reconstruct_vectors),
-            ("lbox", local_box_vectors), ("symfiles", symfile_vectors)]
+            ("lbox", local_box_vectors), ("symfiles", symfile_vectors), ("proj", projection_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):

@@ -113,6 +113,35 @@ def test_search_modes(workdir, mode):
         assert abs(o.pr - r[5]) < 2e-3
 
 
+def test_mode0_template_stack(workdir):
+    """mode 0 (project2d.f90): the template stack projected from model3d."""
+    d, g = workdir
+    pj = np.load(os.path.join(ROOT, "tests", "golden", "proj.npz"))
+    formats.write_mrc(os.path.join(d, "model.mrc"), pj["proj_model"])
+    _run(d, "mode=0", "model3d=model.mrc", "templateort=tort.txt", "templatestck=tstk.mrc")
+    st = formats.read_mrc(os.path.join(d, "tstk.mrc"))
+    assert st.shape == pj["mode0_stack"].shape
+    err = rel_l2(st, pj["mode0_stack"])
+    print(f"mode 0 template stack vs reference: rel-L2 {err:.2e}")
+    assert err < 1e-6
+
+
+def test_mode4_with_model_templates(workdir):
+    """mode 4 without templatestck: load_template projects the templates from model3d."""
+    d, g = workdir
+    pj = np.load(os.path.join(ROOT, "tests", "golden", "proj.npz"))
+    formats.write_mrc(os.path.join(d, "model.mrc"), pj["proj_model"])
+    _run(d, "mode=4", "templateort=tort.txt", "model3d=model.mrc", "newortfile=m4model.txt", "searchstep=6.0",
+         "apix=2.0", "maxRes=6.0", "minRes=100", "imgmask=40", "ortfile=ort.txt", "last=6")
+    out = formats.read_ortfile900(os.path.join(d, "m4model.txt"))
+    ref = pj["mode4_model_out"]
+    assert len(out) == len(ref)
+    for o, r in zip(out, ref):
+        got = np.array([o.theta, o.phi, o.omega, o.x, o.y])
+        assert np.abs(got - r[:5]).max() < 2e-3, (got, r)
+        assert abs(o.pr - r[5]) < 2e-3
+
+
 @pytest.fixture(scope="module")
 def local_ort(workdir):
     d, g = workdir
