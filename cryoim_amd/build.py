@@ -5,9 +5,9 @@
 * ``oracle/liboracle{64,32}.so`` -- the C oracle (test infrastructure);
 * ``oracle/_ref/`` -- the reference's own Fortran programs, only where the
   reference sources are present (never on the GPU box);
-* ``cryoim_amd/bin/icosprocess_hip``, ``merge_icosprocess_subrec_hip`` and
-  ``local_reconstruct_hip`` -- the Fortran drivers over the C ABI,
-  when amdflang is available.
+* ``cryoim_amd/bin/icosprocess_hip``, ``merge_icosprocess_subrec_hip``,
+  ``local_reconstruct_hip``, ``local_box_hip`` and ``local_refine_hip`` -- the
+  Fortran drivers over the C ABI, when amdflang is available.
 """
 from __future__ import annotations
 
@@ -105,7 +105,8 @@ def build_fortran() -> bool:
             "-lamdhip64"]
     iface_o = os.path.join(modd, "cim_iface.o")
     _run([fc, "-O2", "-J", modd, "-c", srcs[0], "-o", iface_o])
-    for prog in ("icosprocess_hip", "merge_icosprocess_subrec_hip", "local_reconstruct_hip", "local_box_hip"):
+    for prog in ("icosprocess_hip", "merge_icosprocess_subrec_hip", "local_reconstruct_hip", "local_box_hip",
+                 "local_refine_hip"):
         src = os.path.join(fdir, prog + ".f90")
         if os.path.exists(src):
             _run([fc, "-O2", "-J", modd, "-o", os.path.join(bindir, prog), src, iface_o, *link])

@@ -313,6 +313,54 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
     }
 }
 
+// refine_particle_df.f90:100-135: at the sub-particle's local orientation
+// and shift, the template t = slice * phase ramp * shift2d * w_lp against the
+// particle for 401 defoci z_i = z0 + 4 (i - 200) A (df -/+ z_i by sign), the
+// phase residual of cal_PR_FT for each, and the first minimum (minloc).  One
+// workgroup per sub-particle; t staged in LDS once.  Records: local angles,
+// x, y in; CIM_P_Z holds z0 in and the chosen z out, CIM_P_PR the residual.
+__global__ void __launch_bounds__(256) k_lref_defocus(LrefArgs a, CtfConst cc, float sign) {
+    extern __shared__ float2 tz[];
+    __shared__ float red[2][16];
+    __shared__ float prs[16];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    const float2* g = a.g + (long long)p * a.nq;
+    const float* tf = a.tfac + (long long)p * a.nq;
+    float* rec = a.parts + (long long)p * CIM_PART_STRIDE;
+    float R[9];
+    euler_f(rec[CIM_P_THETA], rec[CIM_P_PHI], rec[CIM_P_OMEGA], R);
+    const float x0 = rec[CIM_P_X], y0 = rec[CIM_P_Y], z0 = rec[CIM_P_Z];
+    for (int q = tid; q < a.nq; q += blockDim.x) {
+        const int2 hk = a.disk[q];
+        float2 z = slice_at(a.model, a.n, R, hk.x, hk.y);
+        z = apply_shift(z, shift_phase(hk.x, hk.y, a.n, x0, y0, 0));
+        tz[q] = make_float2(z.x * tf[q], z.y * tf[q]);
+    }
+    __syncthreads();
+    float best = 0.f, bz = z0;
+    for (int i = 0; i <= 400; ++i) {
+        const float zi = z0 + (float)(i - 200) * 4.0f;
+        const float d1 = sign > 0.f ? rec[CIM_P_DF1] + zi : rec[CIM_P_DF1] - zi;
+        const float d2 = sign > 0.f ? rec[CIM_P_DF2] + zi : rec[CIM_P_DF2] - zi;
+        float num[1] = {0.f}, den[1] = {0.f};
+        for (int q = tid; q < a.nq; q += blockDim.x) {
+            const int2 hk = a.disk[q];
+            const float c = ctf_value(hk.x, hk.y, a.n, d1, d2, rec[CIM_P_ASTIG], cc);
+            const float2 t = tz[q];
+            pr_term(make_float2(t.x * c, t.y * c), g[q], num[0], den[0]);
+        }
+        block_reduce_pairs<1>(num, den, red, prs);
+        if (tid == 0 && (i == 0 || prs[0] < best)) {
+            best = prs[0];
+            bz = zi;
+        }
+    }
+    if (tid == 0) {
+        rec[CIM_P_PR] = best;
+        rec[CIM_P_Z] = bz;
+    }
+}
+
 // ---- symmetry operators of the local programs -------------------------------------
 
 // local_refine/SRC/icos2f_sym_matc.f90 generators, evaluated in float32 as
@@ -530,25 +578,16 @@ int cim_lref_model(cim_ctx* ctx, const cim_params* p, const float* vol) {
     return CIM_OK;
 }
 
-int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, float* parts,
-                    const char* sym, float convF, float maxres) {
-    if (!ctx) return CIM_E_ARG;
-    int rc = check_params_public(ctx, p);
-    if (rc) return rc;
-    if (!images || !parts) return set_error(ctx, CIM_E_ARG, "NULL images or parts");
-    if (!ctx->lref.ready || ctx->lref.box != p->box)
-        return set_error(ctx, CIM_E_STATE, "cim_lref_model has not been called for box %d", p->box);
-    if (box0 < p->box || box0 > 4096) return set_error(ctx, CIM_E_ARG, "image box %d < sub-particle box %d", box0, p->box);
-    if (!(maxres > 0.f)) return set_error(ctx, CIM_E_ARG, "maxres must be > 0");
-    if (n <= 0) return CIM_OK;
-    CIM_HIP(hipSetDevice(ctx->device));
-    std::vector<float> sops;
-    int nsym = 0;
-    std::string s(sym ? sym : "c1");
-    if (s != "c1" && s != "C1") {
-        nsym = local_symops(sym, sops);
-        if (nsym < 0) return set_error(ctx, CIM_E_ARG, "unknown symmetry '%s'", sym);
-    }
+}  // extern "C"
+
+namespace cim {
+
+// The particle side of refine_PR.f90:87-134 / refine_particle_df.f90:71-109 for
+// n sub-particles: whole-image norm, local box cut, mask at the local shift,
+// checkerboard, FFT (+Im), and on the disk of radius maxres the low-passed
+// values g and the template factor shift2d * CTF (apply_ctf) * w_lp.
+static int lref_prepare(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, const float* parts,
+                        float maxres, int apply_ctf, int* nq_out) {
     const int nb = p->box;
     const long long N0 = (long long)box0 * box0, Nb = (long long)nb * nb;
     // disk of pixels the phase residual reads (cal_PR_FT.f90:11-15, minRes = -1)
@@ -560,13 +599,9 @@ int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* image
     LrefState& L = ctx->lref;
     CIM_ALLOC(L.disk, sizeof(int2) * nq);
     CIM_HIP(hipMemcpyAsync(L.disk.ptr, disk.data(), sizeof(int2) * nq, hipMemcpyHostToDevice, ctx->stream));
-    if (nsym > 0) {
-        CIM_ALLOC(L.symops, sizeof(float) * 9 * nsym);
-        CIM_HIP(hipMemcpyAsync(L.symops.ptr, sops.data(), sizeof(float) * 9 * nsym, hipMemcpyHostToDevice, ctx->stream));
-    }
     CIM_ALLOC(ctx->img, sizeof(float) * N0 * n);
     CIM_HIP(hipMemcpyAsync(ctx->img.ptr, images, sizeof(float) * N0 * n, hipMemcpyHostToDevice, ctx->stream));
-    rc = upload_parts(ctx, n, parts);
+    int rc = upload_parts(ctx, n, parts);
     if (rc) return rc;
     CIM_ALLOC(ctx->aux, sizeof(float) * Nb * n);
     hipLaunchKernelGGL(k_cut_norm, dim3(n), dim3(1024), 0, ctx->stream, ctx->img.as<float>(), box0,
@@ -589,9 +624,51 @@ int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* image
     CIM_ALLOC(L.tfac, sizeof(float) * (size_t)nq * n);
     CtfConst cc = make_ctf_const(p);
     hipLaunchKernelGGL(k_lref_disk, dim3((nq + 255) / 256, n), dim3(256), 0, ctx->stream, ctx->ft.as<float2>(), nb,
-                       L.disk.as<int2>(), nq, ctx->parts.as<float>(), p->apply_ctf, cc, maxres, L.g.as<float2>(),
+                       L.disk.as<int2>(), nq, ctx->parts.as<float>(), apply_ctf, cc, maxres, L.g.as<float2>(),
                        L.tfac.as<float>());
     CIM_HIP(hipGetLastError());
+    *nq_out = nq;
+    return CIM_OK;
+}
+
+static int lref_check(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, const float* parts,
+                      float maxres) {
+    if (!ctx) return CIM_E_ARG;
+    int rc = check_params_public(ctx, p);
+    if (rc) return rc;
+    if (n > 0 && (!images || !parts)) return set_error(ctx, CIM_E_ARG, "NULL images or parts");
+    if (!ctx->lref.ready || ctx->lref.box != p->box)
+        return set_error(ctx, CIM_E_STATE, "cim_lref_model has not been called for box %d", p->box);
+    if (box0 < p->box || box0 > 4096) return set_error(ctx, CIM_E_ARG, "image box %d < sub-particle box %d", box0, p->box);
+    if (!(maxres > 0.f)) return set_error(ctx, CIM_E_ARG, "maxres must be > 0");
+    return CIM_OK;
+}
+
+}  // namespace cim
+
+extern "C" {
+
+int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, float* parts,
+                    const char* sym, float convF, float maxres) {
+    int rc = lref_check(ctx, p, n, images, box0, parts, maxres);
+    if (rc) return rc;
+    if (n <= 0) return CIM_OK;
+    CIM_HIP(hipSetDevice(ctx->device));
+    std::vector<float> sops;
+    int nsym = 0;
+    std::string s(sym ? sym : "c1");
+    if (s != "c1" && s != "C1") {
+        nsym = local_symops(sym, sops);
+        if (nsym < 0) return set_error(ctx, CIM_E_ARG, "unknown symmetry '%s'", sym);
+    }
+    LrefState& L = ctx->lref;
+    if (nsym > 0) {
+        CIM_ALLOC(L.symops, sizeof(float) * 9 * nsym);
+        CIM_HIP(hipMemcpyAsync(L.symops.ptr, sops.data(), sizeof(float) * 9 * nsym, hipMemcpyHostToDevice, ctx->stream));
+    }
+    int nq = 0;
+    if ((rc = lref_prepare(ctx, p, n, images, box0, parts, maxres, p->apply_ctf, &nq))) return rc;
+    const int nb = p->box;
     LrefArgs a{};
     a.model = L.model.as<float2>();
     a.n = nb;
@@ -618,6 +695,42 @@ int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* image
     for (int i = 0; i < n; ++i)
         for (int k : {CIM_P_THETA, CIM_P_PHI, CIM_P_OMEGA, CIM_P_X, CIM_P_Y, CIM_P_PR})
             parts[i * CIM_PART_STRIDE + k] = outp[i * CIM_PART_STRIDE + k];
+    ctx->counters["lref_particles"] += n;
+    return CIM_OK;
+}
+
+int cim_lref_defocus(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, float* parts,
+                     float sign, float maxres) {
+    int rc = lref_check(ctx, p, n, images, box0, parts, maxres);
+    if (rc) return rc;
+    if (!(sign == 1.f || sign == -1.f)) return set_error(ctx, CIM_E_ARG, "sign must be +1 or -1 (the sign key)");
+    if (n <= 0) return CIM_OK;
+    CIM_HIP(hipSetDevice(ctx->device));
+    int nq = 0;
+    if ((rc = lref_prepare(ctx, p, n, images, box0, parts, maxres, 0, &nq))) return rc;
+    if ((size_t)nq * sizeof(float2) > 64 * 1024)
+        return set_error(ctx, CIM_E_ARG, "maxres %g: %d disk pixels exceed the 64 KB template buffer", maxres, nq);
+    LrefState& L = ctx->lref;
+    LrefArgs a{};
+    a.model = L.model.as<float2>();
+    a.n = p->box;
+    a.disk = L.disk.as<int2>();
+    a.nq = nq;
+    a.g = L.g.as<float2>();
+    a.tfac = L.tfac.as<float>();
+    a.parts = ctx->parts.as<float>();
+    CtfConst cc = make_ctf_const(p);
+    {
+        TimedLaunch tl(ctx, "lref_defocus");
+        hipLaunchKernelGGL(k_lref_defocus, dim3(n), dim3(256), sizeof(float2) * nq, ctx->stream, a, cc, sign);
+        CIM_HIP(hipGetLastError());
+    }
+    std::vector<float> outp((size_t)n * CIM_PART_STRIDE);
+    CIM_HIP(hipMemcpyAsync(outp.data(), ctx->parts.ptr, sizeof(float) * CIM_PART_STRIDE * n, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i)
+        for (int k : {CIM_P_PR, CIM_P_Z}) parts[i * CIM_PART_STRIDE + k] = outp[i * CIM_PART_STRIDE + k];
     ctx->counters["lref_particles"] += n;
     return CIM_OK;
 }

@@ -19,6 +19,8 @@ module cim_iface
     ! 1-based positions of the record fields in rec(:, i)
     integer, parameter :: P_THETA = 1, P_PHI = 2, P_OMEGA = 3, P_X = 4, P_Y = 5, P_PR = 6
     integer, parameter :: P_DF1 = 7, P_DF2 = 8, P_ASTIG = 9, P_MASKX = 10, P_MASKY = 11, P_USE = 12
+    ! local_refine records: the sub-particle box centre and the height local_z0
+    integer, parameter :: P_BOXX = 10, P_BOXY = 11, P_Z = 12
 
     type, bind(C) :: cim_params
         integer(c_int) :: box = 0, minR = 0, maxR = 0, apply_ctf = 1, mask_on = 1, norm_on = 0
@@ -186,6 +188,16 @@ module cim_iface
             real(c_float) :: images(*), parts(*)
             character(kind=c_char) :: sym(*)
             real(c_float), value :: convF, maxres
+        end function
+
+        integer(c_int) function cim_lref_defocus(ctx, p, n, images, box0, parts, sign, maxres) &
+                bind(C, name='cim_lref_defocus')
+            import :: c_ptr, c_int, c_float, cim_params
+            type(c_ptr), value :: ctx
+            type(cim_params) :: p
+            integer(c_int), value :: n, box0
+            real(c_float) :: images(*), parts(*)
+            real(c_float), value :: sign, maxres
         end function
 
         integer(c_int) function cim_local_reconstruct_sym_ops(sym, ops, nops) &

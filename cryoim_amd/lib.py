@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcryoim_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "cryoim.h")
 
 PART_STRIDE = 12
+P_Z = 11          # local_refine records: local_z0 in, chosen z out (cim_lref_defocus)
 SEARCH_PLAIN, SEARCH_TABLES = 0, 1
 CENTER_MODEL, CENTER_SELFROT180 = 0, 1
 ICOSAVG_FULL, ICOSAVG_MERGE = 0, 1
@@ -154,6 +155,7 @@ def load() -> ctypes.CDLL:
         "cim_center": (ip, [vp, P, ip, ip, fp, fp, fp]),
         "cim_center_dev": (ip, [vp, P, ip, ip, vp, fp, fp]),
         "cim_lref_refine": (ip, [vp, P, ip, fp, ip, fp, ctypes.c_char_p, ctypes.c_float, ctypes.c_float]),
+        "cim_lref_defocus": (ip, [vp, P, ip, fp, ip, fp, ctypes.c_float, ctypes.c_float]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):
@@ -473,6 +475,16 @@ class Context:
         self._check(self.lib.cim_lref_refine(self.h, ctypes.byref(params), parts.shape[0], _fptr(im), box0,
                                              _fptr(parts), sym.encode(), float(convF), float(maxres)),
                     "cim_lref_refine")
+        return parts
+
+    def lref_defocus(self, params: CimParams, images, parts, sign: float = 1.0, maxres: float = 10.0) -> np.ndarray:
+        """refine_particle_df (cim_lref_defocus): records with the chosen z in
+        slot P_Z and its residual in P_PR."""
+        im = _f32(images)
+        parts = _f32(parts).reshape(-1, PART_STRIDE).copy()
+        box0 = im.shape[-1]
+        self._check(self.lib.cim_lref_defocus(self.h, ctypes.byref(params), parts.shape[0], _fptr(im), box0,
+                                              _fptr(parts), float(sign), float(maxres)), "cim_lref_defocus")
         return parts
 
     def center_model(self, params: CimParams, model):

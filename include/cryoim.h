@@ -24,7 +24,8 @@
  *   global search .... icosprocess/SRC/ort_globesearch.f90:34-134
  *   local refinement . icosprocess/SRC/ort_refinement.f90:31-124
  *   slice + PR ....... local_refine/SRC/refine_PR.f90:69-245, FFT3d_slice.f90,
- *                      cal_PR_FT.f90, cal_template_FT.f90
+ *                      cal_PR_FT.f90, cal_template_FT.f90, refine_PR_Zhigh.f90,
+ *                      refine_particle_df.f90
  *   half maps, FSC ... icosprocess/SRC/reconstruct_half.f90, reconstruct_FSC.f90
  *   tnf merges ....... merge_icosprocess_subrec/SRC/merge_reconstruct.f90,
  *                      merge_reconstruct_FSC.f90
@@ -130,6 +131,7 @@ typedef struct cim_params {
  * the full particle image (local_hx0, local_ky0 of the local ortfile). */
 #define CIM_P_BOXX 9
 #define CIM_P_BOXY 10
+#define CIM_P_Z 11        /* local_refine: local_z0 (A) in; cim_lref_defocus writes the chosen z */
 
 /* ---- context ---------------------------------------------------------- */
 int cim_init(cim_ctx** ctx, int device);
@@ -331,6 +333,19 @@ int cim_lref_model(cim_ctx* ctx, const cim_params* p, const float* vol);
  * the Gaussian low-pass exp(-r^2 / (2 maxres^2)). */
 int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, float* parts,
                     const char* sym, float convF, float maxres);
+/* refine_PR_Zhigh (local_refine/SRC/refine_PR_Zhigh.f90:3-283, Zhigh_df=y):
+ * cim_lref_refine with each record's df1, df2 replaced by df -/+ local_z0
+ * (the sign key) and apply_ctf = 1 -- the variant differs from refine_PR only
+ * in that CTF and in the output fields it writes.
+ * refine_particle_df (refine_particle_df.f90:3-177, particle_df=y): no
+ * orientation refinement; at each record's local angles and shift the
+ * central slice (phase-shifted, checkerboarded, low-passed) is compared by
+ * cal_PR_FT with the sub-particle under the CTF of 401 defoci
+ * df -/+ z_i, z_i = z0 + 4 (i - 200) A, z0 = CIM_P_Z (sign = +1 / -1 for the
+ * sign key '+' / '-'); the first minimum gives CIM_P_Z and CIM_P_PR.  Images,
+ * box0 and maxres as cim_lref_refine; needs cim_lref_model. */
+int cim_lref_defocus(cim_ctx* ctx, const cim_params* p, int n, const float* images, int box0, float* parts,
+                     float sign, float maxres);
 
 /* ---- centre finding (icosprocess mode 3) -------------------------------- */
 #define CIM_CENTER_MODEL 0        /* calc_center.f90:2-113 (model3d given) */

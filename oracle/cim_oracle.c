@@ -1222,6 +1222,46 @@ EXPORT real o_lref_refine(const real* fr, const real* fi, int nx, const real* a2
     return best;
 }
 
+/* One sub-particle of refine_particle_df (local_refine/SRC/refine_particle_df.f90:100-135):
+ * the slice at (th, ph, om), phase-shifted by (x0, y0), times shift2d and wlp,
+ * against the particle (a20, b20) under the CTF of the 401 defoci
+ * df -/+ z_i, z_i = z0 + (i - 200) 4 (sign +1 / -1), by cal_PR_FT; the first
+ * minimum.  out2 = {z, PR}; prs (may be NULL) receives all 401 residuals. */
+EXPORT void o_lref_defocus(const real* fr, const real* fi, int nx, const real* a20, const real* b20, const real* wlp,
+                           real th, real ph, real om, real x0, real y0, real z0, real df1, real df2, real astig,
+                           real apix, real vol, real cs, real ampwg, real sign, real maxres, real* out2, real* prs)
+{
+    long n2 = (long)nx * nx;
+    real* a1 = (real*)malloc(sizeof(real) * n2);
+    real* b1 = (real*)malloc(sizeof(real) * n2);
+    real* a10 = (real*)malloc(sizeof(real) * n2);
+    real* b10 = (real*)malloc(sizeof(real) * n2);
+    real* ctf = (real*)malloc(sizeof(real) * n2);
+    o_fft3d_slice(fr, fi, nx, th, ph, om, a1, b1);
+    o_ft_shift(a1, b1, nx, x0, y0, 0);
+    for (long q = 0; q < n2; ++q) {
+        real s = (((q / nx) + (q % nx)) & 1) ? (real)-1 : (real)1;
+        a1[q] = a1[q] * s * wlp[q];
+        b1[q] = b1[q] * s * wlp[q];
+    }
+    real best = 0, bz = z0;
+    for (int i = 0; i <= 400; ++i) {
+        real zi = z0 + (real)(i - 200) * (real)4.0;
+        real d1 = sign > 0 ? df1 + zi : df1 - zi, d2 = sign > 0 ? df2 + zi : df2 - zi;
+        o_getctf2d(nx, d1, d2, astig, apix, vol, cs, ampwg, (real)1.0, ctf);
+        for (long q = 0; q < n2; ++q) {
+            a10[q] = a1[q] * ctf[q];
+            b10[q] = b1[q] * ctf[q];
+        }
+        real pr = o_pr_ft(a10, b10, a20, b20, nx, (real)-1.0, maxres);
+        if (prs) prs[i] = pr;
+        if (i == 0 || pr < best) { best = pr; bz = zi; }
+    }
+    out2[0] = bz;
+    out2[1] = best;
+    free(a1); free(b1); free(a10); free(b10); free(ctf);
+}
+
 /* ------------------------------------------------------------------ */
 /* Centre finding (icosprocess mode 3)                                  */
 /* ------------------------------------------------------------------ */

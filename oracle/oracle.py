@@ -340,6 +340,18 @@ class Oracle:
                                   self._r(maxres), self._p(io5))
         return io5, pr
 
+    def lref_defocus(self, fr, fi, a20, b20, wlp, angles, x0, y0, z0, df1, df2, astig, apix, sign, maxres,
+                     vol=200.0, cs=2.7, ampwg=0.1):
+        """refine_particle_df for one sub-particle: (z, PR, the 401 residuals)."""
+        nx = a20.shape[-1]
+        out = np.zeros(2, self.dt)
+        prs = np.zeros(401, self.dt)
+        self.L.o_lref_defocus(self._p(self._a(fr)), self._p(self._a(fi)), nx, self._p(self._a(a20)),
+                              self._p(self._a(b20)), self._p(self._a(wlp)),
+                              *[self._r(v) for v in (*angles, x0, y0, z0, df1, df2, astig, apix, vol, cs, ampwg,
+                                                     sign, maxres)], self._p(out), self._p(prs))
+        return float(out[0]), float(out[1]), prs
+
     # ---- centre finding (icosprocess mode 3) -------------------------------
     def img3d_mask(self, vol, mask, origin=(0.0, 0.0, 0.0)):
         v = self._a(vol).copy()

@@ -234,6 +234,59 @@ def lref_program_vectors() -> dict:
     return g
 
 
+def lref_defocus_vectors() -> dict:
+    """The real local_refine with Zhigh_df=y (refine_PR_Zhigh: the CTF at
+    df + local_z0) and with particle_df=y (refine_particle_df: 401 defoci
+    local_z0 + 4 (i - 200) A scanned at the starting orientation), sign=+, on
+    sub-particles whose true defocus is the ortfile's df plus an offset z."""
+    g = {}
+    box, box0, apix, n = 32, 48, 2.0, 6
+    ph = synth.make_phantom(box, nblobs=8, radius=(6, 10), sigma=(1.2, 1.8), seed=27, sym=False)
+    model = synth.phantom_map(ph)
+    rng = np.random.default_rng(28)
+    ang = np.c_[rng.uniform(20, 160, n), rng.uniform(-180, 180, n), rng.uniform(0, 360, n)]
+    sh = rng.uniform(-1.0, 1.0, (n, 2))
+    proj = synth.project(ph, ang, sh, box=box)
+    df = rng.uniform(8000, 20000, n)              # the true defocus (A)
+    z = rng.uniform(-500, 500, n)                  # true height offset: the ortfile holds df - z
+    z0 = z + rng.uniform(-300, 300, n)             # the starting local_z0 of the scan
+    imgs = np.zeros((n, box0, box0), f4)
+    cen = rng.integers(18, 30, (n, 2))
+    for i in range(n):
+        c = synth.ctf2d(box, df[i], df[i], 0.0, apix, vol=200.0, df_scale=1.0)
+        p = synth.apply_ctf(proj[i], c) + 0.2 * rng.standard_normal((box, box))
+        ys, xs = (np.arange(box) + cen[i, 1] - box // 2) % box0, (np.arange(box) + cen[i, 0] - box // 2) % box0
+        imgs[i][np.ix_(ys, xs)] = p
+    start = ang + rng.uniform(-1.5, 1.5, (n, 3))
+    dfo = df - z
+    g.update(ldf_model=model, ldf_images=imgs, ldf_angles=ang.astype(f4), ldf_shifts=sh.astype(f4),
+             ldf_df=dfo.astype(f4), ldf_z=z.astype(f4), ldf_z0=z0.astype(f4), ldf_cen=cen.astype(i4),
+             ldf_start=start.astype(f4), ldf_box=np.array([box, box0], i4), ldf_apix=np.array([apix], f4))
+    exe = os.path.join(REF, "local_refine")
+    with tempfile.TemporaryDirectory() as d:
+        formats.write_mrc(os.path.join(d, "model.mrc"), model, is_map=True)
+        formats.write_mrc(os.path.join(d, "stk.mrc"), imgs)
+        with open(os.path.join(d, "lort.txt"), "w") as f:
+            for i in range(n):
+                f.write(f"stk.mrc {i + 1} 0 0 0 0 0 0 {i + 1} {i + 1} {dfo[i]:.3f} {dfo[i]:.3f} 0 {i + 1} "
+                        f"{cen[i, 0]} {cen[i, 1]} {start[i, 0]:.4f} {start[i, 1]:.4f} {start[i, 2]:.4f} "
+                        f"{sh[i, 0] * 0.5:.4f} {sh[i, 1] * 0.5:.4f} {z0[i]:.4f} 0\n")
+        for tag, extra in (("zhigh", ["Zhigh_df=y", "sign=+"]), ("pdf", ["particle_df=y", "sign=+"])):
+            r = subprocess.run([exe, "local_ort0=lort.txt", f"local_newort={tag}.txt", "model3d=model.mrc",
+                                f"apix={apix}", "maxRes=8", "Ncycle=3", "imgmask=24", *extra], cwd=d,
+                               stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout)
+            rows = []
+            with open(os.path.join(d, f"{tag}.txt")) as f:
+                for line in f:
+                    t = line[64:].split()
+                    # ... lth lph lom lx ly lz lPR (tokens 15-21)
+                    rows.append([float(v) for v in t[15:22]])
+            g[f"ldf_{tag}_out"] = np.array(rows, f4)
+    return g
+
+
 def local_reconstruct_vectors() -> dict:
     """The real local_reconstruct (built with the one line flang rejects
     removed) on 40 sub-particles: 32-pixel boxes cut from 48-pixel images at
@@ -737,7 +790,8 @@ def main():
             ("local", local_program_vectors), ("center", center_program_vectors),
             ("icosavg", icosavg_program_vectors), ("corrctf", corrctf_program_vectors),
             ("symc", symc_program_vectors), ("lrec", local_reconstruct_vectors),
-            ("lbox", local_box_vectors), ("symfiles", symfile_vectors), ("proj", projection_vectors)]
+            ("lbox", local_box_vectors), ("symfiles", symfile_vectors), ("proj", projection_vectors),
+            ("lref_df", lref_defocus_vectors)]
     for name, fn in jobs:
         # the seeded rng is shared by the first three: run them in order even when only one is wanted
         if args.only is not None and name not in args.only and name not in ("units", "pr", "lref_units"):
