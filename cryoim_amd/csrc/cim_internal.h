@@ -83,10 +83,14 @@ struct SearchBatch {
     DevBuf ppol, tpol, table, cand, kwin, cv, ci;
     // coarse-to-fine gather: coarse point list, 3x3 neighbours, per-particle coarse winners and fine lists
     DevBuf clist, nbr, ccand, ckwin, flist, fcnt;
+    // per (AU point, template): the valid common lines' table rows and zero-omega columns, and their count
+    DevBuf lines, lcount;
+    // coarse E4M3 tables and the D bounds of the particles and templates they are scaled by
+    DevBuf ctab, bound;
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
                           &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci, &clist, &nbr, &ccand, &ckwin,
-                          &flist, &fcnt})
+                          &flist, &fcnt, &lines, &lcount, &ctab, &bound})
             b->release();
     }
 };

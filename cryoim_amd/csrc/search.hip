@@ -814,13 +814,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 }
 
 struct GatherArgs {
-    const unsigned int* table;   // n x T x (npsi/2) x npsi, {N, D} as bfloat16
-    const float* tang;        // T x 3
-    const float* pts;         // npts x 2
+    // n x T x (nray/2) x nray entries: {N, D} as bfloat16 (unsigned int), or as E4M3
+    // (unsigned short) on the coarse grid
+    const void* table;
+    const unsigned int* lines;   // npts x T x 64: (table row << 16) | zero-omega column (k_cl_lines)
+    const int* lcount;           // npts x T: lines of each (AU point, template)
     const int* pbase;
-    const float* ops;         // nops x 9
-    int T, nops, npts, npsi, sub, Nomega;
-    float dpsi;
+    int T, npts;
+    int nray;                 // rays of the table's polar grid
+    int sub;                  // table columns per omega column
+    int ncol;                 // omega columns scored
+    int ostep;                // linear-index omega steps per scored column
     int n;                    // particles
     float* outv;              // n x npts: best table score of the AU point
     int* outi;                // its linear index
@@ -840,22 +844,117 @@ __device__ __forceinline__ void acc_entry(unsigned int e, float& num, float& den
     den += __uint_as_float(e & 0xffff0000u);
 }
 
+// The same for an E4M3 {N, D} pair of the coarse table (N the low byte).
+__device__ __forceinline__ void acc_entry(unsigned short e, float& num, float& den) {
+    const auto f = __builtin_amdgcn_cvt_pk_f32_fp8((int)e, false);
+    num += f[0];
+    den += f[1];
+}
+
 __device__ __forceinline__ int wrap_index(int j, int n) {
     j %= n;
     return j < 0 ? j + n : j;
 }
 
-// One workgroup per (AU point, particle), threads over omega: every template's
-// common lines are summed in registers and the first minimum over omega kept.
-// Workgroups are dealt to the 8 XCDs in turn (workgroup L on XCD L mod 8), so
-// L mod 8 picks the particle within a group of 8 and L / 8 walks its AU
-// points: each XCD reads the tables of one particle at a time.  The valid
-// common lines of a template are compacted in operator order, and the loads
-// of 8 of them are issued together.
-template <int VPT>
+// The common lines of every (AU point, template) pair on the tables' polar
+// grid, which do not depend on the particle: the valid lines of the 60
+// operators compacted in operator order, each as its table row (the
+// template ray, folded to the stored half: both rays of a pair turned by
+// 180 deg give the same entry) and the column of the particle ray at omega 0.
+// One wave per pair.
+__device__ __forceinline__ unsigned int line_entry(float a1, float a2, float dpsi, int npsi) {
+    const int nrow = npsi >> 1;
+    int jt = wrap_index((int)rintf(a2 / dpsi), npsi);
+    int j0 = wrap_index((int)rintf(a1 / dpsi), npsi);
+    if (jt >= nrow) {
+        jt -= nrow;
+        j0 += nrow;
+        if (j0 >= npsi) j0 -= npsi;
+    }
+    return ((unsigned int)jt << 16) | (unsigned int)j0;
+}
+
+__global__ void __launch_bounds__(64) k_cl_lines(const float* __restrict__ pts, const float* __restrict__ tang,
+                                                 const float* __restrict__ ops, int T, int nops, int npsi, float dpsi,
+                                                 unsigned int* __restrict__ lines, unsigned int* __restrict__ lines2,
+                                                 int* __restrict__ lcount) {
+    const int pt = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+    bool ok = false;
+    float a1 = 0.f, a2 = 0.f;
+    if (tid < nops) {
+        float E1[9], E2[9];
+        euler_f(pts[2 * pt], pts[2 * pt + 1], 0.f, E1);
+        euler_f(tang[3 * t], tang[3 * t + 1], tang[3 * t + 2], E2);
+        crosslin_one(E1, E2, ops + 9 * tid, &a1, &a2);
+        ok = a1 < 999.f && a2 < 999.f;
+    }
+    const unsigned long long mask = __ballot(ok);
+    const long long at = ((long long)pt * T + t) * 64 + __popcll(mask & ((1ull << tid) - 1ull));
+    if (ok) {
+        lines[at] = line_entry(a1, a2, dpsi, npsi);
+        // the same lines on the polar grid of twice the step (lines2 null: none)
+        if (lines2) lines2[at] = line_entry(a1, a2, 2.f * dpsi, npsi >> 1);
+    }
+    if (tid == 0) lcount[(long long)pt * T + t] = __popcll(mask);
+}
+
+// Upper bound of the D entries of a residual table with the polar rays of
+// one item: the sum over radii of the largest ungated |F|/2 on the ring
+// (D(jT, jP) = sum_r aT gP + gT aP is at most the template's bound plus the
+// particle's).
+__global__ void __launch_bounds__(256) k_ring_bound(const float2* __restrict__ pol, int nr, int npsi,
+                                                    float* __restrict__ out) {
+    __shared__ float red[4];
+    const int item = blockIdx.x, tid = threadIdx.x;
+    const float2* P = pol + (long long)item * nr * npsi;
+    float acc = 0.f;
+    for (int r = 0; r < nr; ++r) {
+        float m = 0.f;
+        for (int j = tid; j < npsi; j += 256) m = fmaxf(m, P[(long long)r * npsi + j].y);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((tid & 63) == 0) red[tid >> 6] = m;
+        __syncthreads();
+        if (tid == 0) acc += fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        __syncthreads();
+    }
+    if (tid == 0) out[item] = acc;
+}
+
+// The coarse tables: the entries of the even template and particle rays (the
+// residual tables of the polar grid of twice the step) as E4M3 {N, D}
+// pairs, 2 bytes each, scaled by 128 / (the particle's D bound plus the
+// largest of its templates'), so D <= 128 and N <= pi D stay below the
+// format's 448.  One scale per particle: a score sums the entries of all
+// its templates and is a ratio, so the scale cancels and is not stored.
+__global__ void __launch_bounds__(256) k_coarse_table(const unsigned int* __restrict__ table,
+                                                      const float* __restrict__ pbound,
+                                                      const float* __restrict__ tbound, int tb_per_particle, int T,
+                                                      int npsi, unsigned short* __restrict__ ctab) {
+    const int item = blockIdx.z, p = item / T, i = blockIdx.y;
+    const int nrow = npsi >> 1, n2 = npsi >> 1, nrow2 = npsi >> 2;
+    const int jc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jc >= n2) return;
+    float tb = 0.f;
+    for (int t = 0; t < T; ++t) tb = fmaxf(tb, tbound[tb_per_particle ? p * T + t : t]);
+    const float bound = pbound[p] + tb;
+    const float sc = bound > 0.f ? 128.f / bound : 0.f;
+    const unsigned int e = table[((long long)item * nrow + 2 * i) * npsi + 2 * jc];
+    const float N = fminf(__uint_as_float(e << 16) * sc, 448.f);
+    const float D = fminf(__uint_as_float(e & 0xffff0000u) * sc, 448.f);
+    ctab[((long long)item * nrow2 + i) * n2 + jc] = (unsigned short)__builtin_amdgcn_cvt_pk_fp8_f32(N, D, 0, false);
+}
+
+// One workgroup per (AU point, particle), threads over omega: every
+// template's common lines (k_cl_lines) are summed in registers and the
+// first minimum over omega kept.  Workgroups are dealt to the 8 XCDs in turn
+// (workgroup L on XCD L mod 8), so L mod 8 picks the particle within a group
+// of 8 and L / 8 walks its AU points: each XCD reads the tables of one
+// particle at a time.  The line lists are the same for the whole workgroup
+// (scalar loads), and the loads of 8 lines are issued together.  Entry:
+// unsigned int (bfloat16 pairs) or unsigned short (the coarse E4M3 pairs).
+template <int VPT, typename Entry>
 __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
-    __shared__ int srow[64], sj0[64];
-    __shared__ int snv;
     __shared__ float redv[4];
     __shared__ int redi[4];
     const long long L = blockIdx.x;
@@ -866,58 +965,31 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     if (a.pcnt && slot >= a.pcnt[p]) return;
     const int pt = a.plist ? a.plist[(long long)p * a.plstride + slot] : slot;
     const int tid = threadIdx.x, bd = blockDim.x;
-    const int nrow = a.npsi >> 1;
+    const int npsi = a.nray, nrow = npsi >> 1;
     float num[VPT], den[VPT];
     // column of omega k on a line with zero-angle column j0: j0 - k sub, plus npsi when
-    // that is negative (columns past Nomega read column j0 and are not used)
+    // that is negative (columns past ncol read column j0 and are not used)
     int ks[VPT];
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
         num[v] = 0.f;
         den[v] = 0.f;
         const int k = tid + v * bd;
-        ks[v] = k < a.Nomega ? k * a.sub : 0;
+        ks[v] = k < a.ncol ? k * a.sub : 0;
     }
-    const int npsi = a.npsi;
-    float E1[9];
-    euler_f(a.pts[2 * pt], a.pts[2 * pt + 1], 0.f, E1);
     for (int t = 0; t < a.T; ++t) {
-        if (tid < 64) {
-            bool ok = false;
-            int jt = 0, j0 = 0;
-            if (tid < a.nops) {
-                float E2[9], a1, a2;
-                euler_f(a.tang[3 * t], a.tang[3 * t + 1], a.tang[3 * t + 2], E2);
-                crosslin_one(E1, E2, a.ops + 9 * tid, &a1, &a2);
-                ok = a1 < 999.f && a2 < 999.f;
-                jt = wrap_index((int)rintf(a2 / a.dpsi), a.npsi);
-                j0 = wrap_index((int)rintf(a1 / a.dpsi), a.npsi);
-                if (jt >= nrow) {
-                    jt -= nrow;
-                    j0 += nrow;
-                    if (j0 >= a.npsi) j0 -= a.npsi;
-                }
-            }
-            const unsigned long long mask = __ballot(ok);
-            if (ok) {
-                const int slot = __popcll(mask & ((1ull << tid) - 1ull));
-                srow[slot] = jt * a.npsi;
-                sj0[slot] = j0;
-            }
-            if (tid == 0) snv = __popcll(mask);
-        }
-        __syncthreads();
-        const int nv = snv;
-        const unsigned int* tab = a.table + (long long)(p * a.T + t) * nrow * a.npsi;
+        const unsigned int* lp = a.lines + ((long long)pt * a.T + t) * 64;
+        const int nv = a.lcount[(long long)pt * a.T + t];
+        const Entry* tab = static_cast<const Entry*>(a.table) + (long long)(p * a.T + t) * nrow * npsi;
         constexpr int G = 8;
         int j = 0;
         for (; j + G <= nv; j += G) {
-            unsigned int e[G][VPT];
+            Entry e[G][VPT];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                // the line's row and zero-angle column are the same for the whole workgroup
-                const unsigned int* row = tab + __builtin_amdgcn_readfirstlane(srow[j + g]);
-                const int j0 = __builtin_amdgcn_readfirstlane(sj0[j + g]);
+                const unsigned int pk = lp[j + g];
+                const Entry* row = tab + (long long)(pk >> 16) * npsi;
+                const int j0 = (int)(pk & 0xffffu);
 #pragma unroll
                 for (int v = 0; v < VPT; ++v) {
                     int c = j0 - ks[v];
@@ -933,8 +1005,9 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
                 for (int v = 0; v < VPT; ++v) acc_entry(e[g][v], num[v], den[v]);
         }
         for (; j < nv; ++j) {
-            const unsigned int* row = tab + __builtin_amdgcn_readfirstlane(srow[j]);
-            const int j0 = __builtin_amdgcn_readfirstlane(sj0[j]);
+            const unsigned int pk = lp[j];
+            const Entry* row = tab + (long long)(pk >> 16) * npsi;
+            const int j0 = (int)(pk & 0xffffu);
 #pragma unroll
             for (int v = 0; v < VPT; ++v) {
                 int c = j0 - ks[v];
@@ -942,7 +1015,6 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
                 acc_entry(row[c], num[v], den[v]);
             }
         }
-        __syncthreads();
     }
     const float r2d = 57.2957795130823208768f;
     float best = 3.0e38f;
@@ -950,9 +1022,9 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
         const int k = tid + v * bd;
-        if (k >= a.Nomega) continue;
+        if (k >= a.ncol) continue;
         const float pr = den[v] > 1.0e-10f ? num[v] / den[v] * r2d : 999.f;
-        const int li = a.pbase[pt] + k;
+        const int li = a.pbase[pt] + k * a.ostep;
         if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
     }
 #pragma unroll
@@ -984,12 +1056,14 @@ __global__ void k_fill_scores(float* __restrict__ v, int* __restrict__ ix, long 
 }
 
 // Coarse-to-fine gather, second stage: the 3x3 grid neighbours of a
-// particle's M best coarse points that have no score yet, each once, as that
-// particle's fine list (one workgroup per particle; npts <= 8192).  Byte
-// flags in LDS are claimed with an atomic OR on their 32-bit word.
+// particle's M best coarse points that have no score yet (every one of them,
+// the coarse points included, when rescore_all: the coarse stage scored only
+// some omega columns), each once, as that particle's fine list (one
+// workgroup per particle; npts <= 8192).  Byte flags in LDS are claimed with
+// an atomic OR on their 32-bit word.
 __global__ void __launch_bounds__(256) k_expand(const int* __restrict__ ccand, int M, const int* __restrict__ nbr,
                                                 const float* __restrict__ v, int npts, int* __restrict__ flist,
-                                                int fstride, int* __restrict__ fcnt) {
+                                                int fstride, int* __restrict__ fcnt, int rescore_all) {
     __shared__ unsigned int seen[8192 / 4];
     __shared__ int cnt;
     const int p = blockIdx.x, tid = threadIdx.x;
@@ -999,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_expand(const int* __restrict__ ccand, i
     for (int e = tid; e < M * 9; e += blockDim.x) {
         const int c = ccand[(long long)p * M + e / 9];
         const int q = nbr[(long long)c * 9 + e % 9];
-        if (q < 0 || v[(long long)p * npts + q] < 1.0e38f) continue;
+        if (q < 0 || (!rescore_all && v[(long long)p * npts + q] < 1.0e38f)) continue;
         const unsigned int bit = 1u << (8 * (q & 3));
         if (atomicOr(&seen[q >> 2], bit) & bit) continue;
         flist[(long long)p * fstride + atomicAdd(&cnt, 1)] = q;
@@ -1554,11 +1628,27 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     int coarseM = 64;
     // (k_topk_scan keeps the chosen points in a 256-entry LDS array)
     if (const char* ec = getenv("CIM_TABLE_COARSE")) coarseM = std::max(0, std::min(256, atoi(ec)));
+    // the coarse stage searches the grid of twice the step in all three angles, from E4M3 tables of
+    // the polar grid of twice the step (CIM_TABLE_COARSE_GRID=0: the step's omega and tables); the
+    // fine stage then scores every omega column of the 3x3 neighbourhoods of the coarseM best, their
+    // centres included
+    bool coarse_grid = sub == 1 && npsi % 4 == 0;
+    if (const char* ek = getenv("CIM_TABLE_COARSE_GRID")) coarse_grid = coarse_grid && atoi(ek) != 0;
     std::vector<int> clist, nbrv;
     bool c2f = false;
     if (tables && coarseM > 0 && npts <= 8192) {
         coarse_lists(g, clist, nbrv);
         c2f = (int)clist.size() > coarseM && (int)clist.size() >= topk;
+    }
+    if (tables) {
+        // the common lines of every (AU point, template) on the polar grid, once per call
+        CIM_ALLOC(sb.lines, sizeof(unsigned int) * (size_t)npts * T * 64 * 2);
+        CIM_ALLOC(sb.lcount, sizeof(int) * (size_t)npts * T);
+        hipLaunchKernelGGL(k_cl_lines, dim3(npts, T), dim3(64), 0, ctx->stream, sb.pts.as<float>(), sb.tang.as<float>(),
+                           sb.ops.as<float>(), T, 60, npsi, dpsi, sb.lines.as<unsigned int>(),
+                           coarse_grid ? sb.lines.as<unsigned int>() + (size_t)npts * T * 64 : nullptr,
+                           sb.lcount.as<int>());
+        CIM_HIP(hipGetLastError());
     }
     if (c2f) {
         CIM_ALLOC(sb.clist, sizeof(int) * clist.size());
@@ -1691,18 +1781,42 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                      nb, T, nr, npsi, sb.table.as<unsigned int>());
                 if (rc) return rc;
             }
-            GatherArgs ga{sb.table.as<unsigned int>(), sb.tang.as<float>(), sb.pts.as<float>(), sb.pbase.as<int>(),
-                          sb.ops.as<float>(), T, 60, npts, npsi, sub, g.No, dpsi, nb, sb.outv.as<float>(),
-                          sb.outi.as<int>(), nullptr, 0, 0, nullptr};
-            auto gather = [&](int nlist) -> int {
-                const int vg = (g.No + 255) / 256;
+            const bool cgrid = c2f && coarse_grid;
+            if (cgrid) {
+                // coarse E4M3 tables of the even rays, scaled per particle by the D bounds
+                CIM_ALLOC(sb.bound, sizeof(float) * (size_t)(nb + titems));
+                CIM_ALLOC(sb.ctab, sizeof(unsigned short) * (size_t)nb * T * (npsi / 4) * (npsi / 2));
+                TimedLaunch tl(ctx, "cl_table");
+                hipLaunchKernelGGL(k_ring_bound, dim3(nb), dim3(256), 0, ctx->stream, sb.ppol.as<float2>(), nr, npsi,
+                                   sb.bound.as<float>());
+                CIM_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_ring_bound, dim3(titems), dim3(256), 0, ctx->stream, sb.tpol.as<float2>(), nr,
+                                   npsi, sb.bound.as<float>() + nb);
+                CIM_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_coarse_table, dim3((unsigned)((npsi / 2 + 255) / 256), npsi / 4, nb * T),
+                                   dim3(256), 0, ctx->stream, sb.table.as<unsigned int>(), sb.bound.as<float>(),
+                                   sb.bound.as<float>() + nb, p->apply_ctf ? 1 : 0, T, npsi,
+                                   sb.ctab.as<unsigned short>());
+                CIM_HIP(hipGetLastError());
+            }
+            GatherArgs ga{sb.table.ptr, sb.lines.as<unsigned int>(), sb.lcount.as<int>(), sb.pbase.as<int>(), T, npts,
+                          npsi, sub, g.No, 1, nb, sb.outv.as<float>(), sb.outi.as<int>(), nullptr, 0, 0, nullptr};
+            // one workgroup of 64 x ceil(ncol / (64 VPT)) threads per (AU point, particle)
+            auto gather = [&](int nlist, bool coarse) -> int {
+                const int vg = (ga.ncol + 255) / 256;
+                const int vpt = vg <= 1 ? 1 : vg == 2 ? 2 : vg == 3 ? 3 : vg == 4 ? 4 : vg <= 6 ? 6 : 8;
                 const dim3 gg((unsigned)(8LL * nlist * ((nb + 7) / 8)));
-                if (vg <= 1) hipLaunchKernelGGL(k_cl_gather<1>, gg, dim3(256), 0, ctx->stream, ga);
-                else if (vg == 2) hipLaunchKernelGGL(k_cl_gather<2>, gg, dim3(256), 0, ctx->stream, ga);
-                else if (vg == 3) hipLaunchKernelGGL(k_cl_gather<3>, gg, dim3(256), 0, ctx->stream, ga);
-                else if (vg == 4) hipLaunchKernelGGL(k_cl_gather<4>, gg, dim3(256), 0, ctx->stream, ga);
-                else if (vg <= 6) hipLaunchKernelGGL(k_cl_gather<6>, gg, dim3(256), 0, ctx->stream, ga);
-                else hipLaunchKernelGGL(k_cl_gather<8>, gg, dim3(256), 0, ctx->stream, ga);
+                const dim3 blk((unsigned)(64 * ((ga.ncol + 64 * vpt - 1) / (64 * vpt))));
+#define CIM_GATHER(V)                                                                                            \
+    if (coarse) hipLaunchKernelGGL((k_cl_gather<V, unsigned short>), gg, blk, 0, ctx->stream, ga);             \
+    else hipLaunchKernelGGL((k_cl_gather<V, unsigned int>), gg, blk, 0, ctx->stream, ga);
+                if (vpt == 1) { CIM_GATHER(1) }
+                else if (vpt == 2) { CIM_GATHER(2) }
+                else if (vpt == 3) { CIM_GATHER(3) }
+                else if (vpt == 4) { CIM_GATHER(4) }
+                else if (vpt == 6) { CIM_GATHER(6) }
+                else { CIM_GATHER(8) }
+#undef CIM_GATHER
                 CIM_HIP(hipGetLastError());
                 return CIM_OK;
             };
@@ -1717,21 +1831,33 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                     ga.plist = sb.clist.as<int>();
                     ga.plstride = 0;
                     ga.nlist = (int)clist.size();
-                    if ((rc = gather(ga.nlist))) return rc;
+                    if (cgrid) {
+                        // the grid of twice the step: half the rays, every other omega
+                        GatherArgs gc = ga;
+                        ga.table = sb.ctab.ptr;
+                        ga.lines = sb.lines.as<unsigned int>() + (size_t)npts * T * 64;
+                        ga.nray = npsi / 2;
+                        ga.ncol = g.No / 2;
+                        ga.ostep = 2;
+                        if ((rc = gather(ga.nlist, true))) return rc;
+                        ga = gc;
+                    } else if ((rc = gather(ga.nlist, false))) {
+                        return rc;
+                    }
                     hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0,
                                        ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(), npts, coarseM,
                                        sb.ccand.as<int>(), g.No, ncolx, wincols, sb.ckwin.as<int>());
                     CIM_HIP(hipGetLastError());
                     hipLaunchKernelGGL(k_expand, dim3(nb), dim3(256), 0, ctx->stream, sb.ccand.as<int>(), coarseM,
                                        sb.nbr.as<int>(), sb.outv.as<float>(), npts, sb.flist.as<int>(), coarseM * 9,
-                                       sb.fcnt.as<int>());
+                                       sb.fcnt.as<int>(), cgrid ? 1 : 0);
                     CIM_HIP(hipGetLastError());
                     ga.plist = sb.flist.as<int>();
                     ga.plstride = coarseM * 9;
                     ga.nlist = coarseM * 9;
                     ga.pcnt = sb.fcnt.as<int>();
-                    if ((rc = gather(ga.nlist))) return rc;
-                } else if ((rc = gather(npts))) {
+                    if ((rc = gather(ga.nlist, false))) return rc;
+                } else if ((rc = gather(npts, false))) {
                     return rc;
                 }
             }
@@ -1813,9 +1939,10 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         }
         CIM_HIP(hipStreamSynchronize(ctx->stream));
         if (c2f) {
-            long long scored = (long long)nb * (long long)clist.size();
+            long long scored = 0;
             for (int c : hfc) scored += c;
-            ctx->counters["cl_gather_points"] += scored * g.No;
+            ctx->counters["cl_gather_points"] +=
+                scored * g.No + (long long)nb * (long long)clist.size() * (coarse_grid ? g.No / 2 : g.No);
         }
         if (tables && fb_rank <= topk) {
             // Fallback: when the exact minimum sits deep in the candidate list, the tables ranked this

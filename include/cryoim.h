@@ -92,17 +92,20 @@ typedef struct cim_params {
  *                    over all pairs of rays on a polar grid of step
  *                    searchstep; (AU point, omega) pairs are scored from the
  *                    tables (common-line angles rounded to the grid) coarse
- *                    to fine: the points of the grid of twice the step plus
- *                    the AU border points first, then the unscored 3x3 grid
- *                    neighbours of each particle's 64 best of those
- *                    (CIM_TABLE_COARSE), so a point that is neither coarse
- *                    nor next to a good coarse point is never scored.  The
- *                    search_topk best AU points of each particle are scored
- *                    again exactly as CIM_SEARCH_PLAIN does, but only over
- *                    a window of 64 omega columns (plus their Friedel
- *                    pairs; CIM_TABLE_WINDOW) centred on the omega of the
- *                    table minimum, and the first minimum among them is
- *                    returned.  A particle whose exact minimum ranks in the
+ *                    to fine: first the grid of twice the step in all three
+ *                    angles (the AU points of even theta and phi index plus
+ *                    the AU border points, every other omega) from E4M3
+ *                    tables of the polar grid of twice the step, then every
+ *                    omega of the 3x3 grid neighbourhoods of each particle's
+ *                    64 best coarse points (CIM_TABLE_COARSE), so a point
+ *                    that is neither coarse nor next to a good coarse point
+ *                    is never scored at the step.  The search_topk best AU
+ *                    points of each particle are scored again exactly as
+ *                    CIM_SEARCH_PLAIN does, but only over a window of 16
+ *                    omega columns (plus their Friedel pairs;
+ *                    CIM_RESCORE_CPW, CIM_TABLE_WINDOW) centred on the omega
+ *                    of the table minimum, and the first minimum among them
+ *                    is returned.  A particle whose exact minimum ranks in the
  *                    last quarter of the candidates, or lies within 2
  *                    columns of its window's edge, is searched again
  *                    exhaustively (the plain search).  Equality with the
