@@ -53,16 +53,21 @@ sys.path.insert(0, ROOT)
 BOX, APIX, MAXRES, MINRES = 384, 1.5, 8.0, 300.0
 INSERT_MAXR = 190
 STEP = 0.5
+# MI355X published peaks: FP32 (vector and matrix alike), dense BF16 matrix, HBM3E
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 COMPENSATED = 16          # CIM_REC_COMPENSATED
-# arithmetic of one scored sample of cal_crosslin_PR_omega.f90:48-69:
-# 2 coordinate FMAs (4), bilinear weights (8) and two 4-tap blends (14),
-# |F|^2 + sqrt (4), gate (1), atan2 (1), phase difference and fold (3),
-# weight (2), two accumulations (3)
-FLOPS_PER_SAMPLE = 40
-# insert, per (sample, operator): 8 corners x read-modify-write of fa, fb, wg in f32
-INSERT_BYTES_PER_SAMPLE_OP = 8 * 3 * 8
+# FP32 operations of one call of the exact score's sample (search.hip score_sample, paired: one
+# interpolation feeds a column and its Friedel column, counted as two samples): position 6,
+# bilinear weights 6, two 4-tap blends 14, |F| 4 (gate-failing samples stop here: 30), weight 2,
+# four products 4, per column two sums, the folded phase difference (atan2_upper: reciprocal,
+# 7 FMAs of the polynomial and 5 more: 20) and the two accumulations 3 (25 each): 86
+FLOPS_PAIR_CALL_PASS = 86
+FLOPS_PAIR_CALL_GATED = 30
+# residual tables on the matrix cores: K features per (template ray, particle ray) entry and radius
+# (2 for D, 4 NH = 16 for N at NH = 4), 2 flops per multiply-add
+TABLE_FEATURES_PER_RADIUS = 2 + 4 * 4
 # where each particle's exact minimum ranked among its table candidates (counted by the library)
 RANK_KEYS = ("cl_rank_lt4", "cl_rank_lt8", "cl_rank_lt16", "cl_rank_lt24", "cl_rank_ge24")
 METRIC = "particles/s search+insert, box 384: chosen vs plain FP32 GPU at FSC>=0.999"
@@ -84,6 +89,9 @@ def _parse():
                          "lbox: only local_box's stack mode (12 vertex boxes of 96 pixels per particle)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--chosen-only", action="store_true",
+                    help="time only the chosen path, step after step (for counter runs under rocprofv3 --pmc): "
+                         "no pipelined leg, plain path, maps or centre stage")
     return ap.parse_args()
 
 
@@ -182,10 +190,12 @@ def gpu_run(args, tang, timgs, ps):
         ctx.sync()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_gather", "cl_rescore",
+        kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_coarse_table",
+                                                   "cl_gather", "cl_rescore",
                                                    "cl_fallback", "insert")}
         cnt = {k: ctx.counter(k) for k in ("cl_search_samples", "insert_samples", "cl_fallback_particles",
-                                           "cl_table_pairs", "cl_gather_points", *RANK_KEYS)}
+                                           "cl_table_pairs", "cl_gather_points", "cl_rescore_samples",
+                                           "cl_rescore_pass", *RANK_KEYS)}
         ctx.timing(False)
         return out, e0.elapsed_time(e1), wall, kt, cnt
 
@@ -227,16 +237,22 @@ def gpu_run(args, tang, timgs, ps):
             ctx_i.sync()
             torch.cuda.synchronize()
             wall = time.perf_counter() - t0
-        kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_gather", "cl_rescore",
+        kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_coarse_table",
+                                                   "cl_gather", "cl_rescore",
                                                    "cl_fallback")}
         kt["insert"] = ctx_i.timing_get("insert")
         cnt = {k: ctx.counter(k) for k in ("cl_search_samples", "cl_fallback_particles", "cl_table_pairs",
-                                           "cl_gather_points", *RANK_KEYS)}
+                                           "cl_gather_points", "cl_rescore_samples", "cl_rescore_pass", *RANK_KEYS)}
         cnt["insert_samples"] = ctx_i.counter("insert_samples")
         for c in (ctx, ctx_i):
             c.timing(False)
         return out, wall * 1e3, wall, kt, cnt
 
+    if args.chosen_only:
+        out_c, ev_c, wall_c, kt_c, cnt_c = timed(p_chosen, COMPENSATED, args.steps, args.warmup)
+        ctx.close()
+        ctx_i.close()
+        return dict(n=n, d=d, ev_c=ev_c, wall_c=wall_c, kt_c=kt_c, cnt_c=cnt_c)
     # chosen path, timed: pipelined (the value) and sequential (reported beside it)
     if args.no_pipeline:
         out_c, ev_c, wall_c, kt_c, cnt_c = timed(p_chosen, COMPENSATED, args.steps, args.warmup)
@@ -535,7 +551,7 @@ def _oracle32():
     return Oracle(32), pipelines
 
 
-PORT_COARSE = 3.0      # search grid step of the port sample (deg); scaled to STEP by the grid-work ratio
+PORT_COARSE = 1.5      # search grid step of the port sample (deg); scaled to STEP by the grid-work ratio
 
 
 def _port_main_job(job):
@@ -579,38 +595,55 @@ def cpu_baseline(args, tang, timgs, ps):
 
 
 def _roofline(r, args):
-    """Roofline of the kernel with the most time in a chosen step."""
+    """Roofline of the kernel with the most time per chosen step, from its
+    HIP-event durations over the timed steps and the library's own work
+    counters for the same steps: FP32 operations for the exact re-score,
+    BF16 matrix operations for the residual tables, table bytes for the
+    gather, sample bytes for the insert."""
     kt, cnt, n = r["kt_c"], r["cnt_c"], r["n"]
     tot = {k: v[0] * v[1] for k, v in kt.items() if v[1]}
     dom = max(tot, key=tot.get)
     ms, launches = kt[dom]
-    if dom == "insert":
-        per_launch = cnt["insert_samples"] / launches * 60 * INSERT_BYTES_PER_SAMPLE_OP * 2   # hi + lo
-        achieved = per_launch / (ms / 1e3) / 1e9
-        return dom, {"bound": "hbm", "kernel": "k_insert_brickq (60-fold trilinear insert, compensated FP32)",
-                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(per_launch),
-                     "bytes_rule": "inserted samples x 60 operators x 8 corners x read+write of fa, fb, wg (f32) "
-                                   "x 2 (hi and lo sums)", "avg_launch_ms": round(ms, 3), "launches": launches,
-                     "note": "frac above 1: the rule counts one HBM read-modify-write per corner update (an atomic "
-                             "scatter); the brick gather sums a batch in LDS and touches each accumulator voxel "
-                             "once, so its measured traffic is a small part of the rule's bytes and the kernel is "
-                             "limited by its VALU work and LDS atomics (and their latencies) instead"}
-    if dom in ("cl_rescore", "cl_search", "cl_fallback"):
-        samples = cnt["cl_search_samples"] / launches
-        achieved = samples * FLOPS_PER_SAMPLE / (ms / 1e3) / 1e12
-        return dom, {"bound": "mfma", "kernel": "k_cl_search (exact phase residual, FP32 VALU)",
+    sec = ms / 1e3
+    base = {"avg_launch_ms": round(ms, 3), "launches": launches, "traffic": None}
+    if dom == "cl_rescore":
+        calls = cnt["cl_rescore_samples"] / 2 / launches          # paired calls (two columns each)
+        passed = cnt["cl_rescore_pass"] / 2 / launches
+        flops = passed * FLOPS_PAIR_CALL_PASS + (calls - passed) * FLOPS_PAIR_CALL_GATED
+        achieved = flops / sec / 1e12
+        return dom, {"bound": "mfma", "kernel": "k_cl_rescore<4,true> (exact phase residual of the top-32 AU "
+                                                "points x 16 omega columns and their Friedel pairs, FP32)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "avg_launch_ms": round(ms, 3), "launches": launches}
-    # table gather: one 4-byte table entry per (AU point, omega, template, common line)
-    per_launch = cnt["cl_gather_points"] / launches * args.templates * 60 * 4
-    achieved = per_launch / (ms / 1e3) / 1e9
-    return dom, {"bound": "hbm", "kernel": "k_cl_gather (table reads, mostly L2 hits)",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), **base,
+                     "algorithmic_flops_per_launch": int(flops),
+                     "flops_rule": f"{FLOPS_PAIR_CALL_PASS} FP32 operations per paired sample call that passes the "
+                                   f"particle amplitude gate, {FLOPS_PAIR_CALL_GATED} per call that fails it "
+                                   f"(library counters: {int(calls)} calls, {passed / calls:.3f} passing, per launch)",
+                     "note": "FP32 vector arithmetic: the peak is MI355X's published FP32 figure (vector and "
+                             "matrix alike, counting packed FP32); this build compiles without packed FP32, so its "
+                             "issue limit is half the peak"}
+    if dom == "cl_table":
+        flops = cnt["cl_table_pairs"] / launches * TABLE_FEATURES_PER_RADIUS
+        achieved = flops / sec / 1e12
+        return dom, {"bound": "mfma", "kernel": "k_cl_table_mfma<4> (residual tables, v_mfma_f32_32x32x16_bf16)",
+                     "achieved": round(achieved, 1), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_BF16_TFLOPS, 4), **base, "algorithmic_flops_per_launch": int(flops)}
+    if dom == "cl_gather":
+        npsi = int(round(360.0 / STEP))
+        tb = n * args.templates * (npsi // 2) * npsi * 4 + n * args.templates * (npsi // 4) * (npsi // 2) * 2
+        achieved = tb / sec / 1e9
+        return dom, {"bound": "hbm", "kernel": "k_cl_gather (coarse E4M3 and fine bfloat16 table sums)",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), **base, "algorithmic_bytes_per_launch": int(tb),
+                     "bytes_rule": "every coarse and fine table entry of the batch read once",
+                     "note": "the sums read each table row once per (AU point, line) from the caches: the kernel "
+                             "is bound by cache-line throughput, not by HBM"}
+    per = cnt["insert_samples"] / launches * 12
+    achieved = per / sec / 1e9
+    return dom, {"bound": "hbm", "kernel": "k_insert_brickq (60-fold coset insert, compensated FP32)",
                  "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                 "avg_launch_ms": round(ms, 3), "launches": launches}
+                 "frac": round(achieved / PEAK_HBM_GBS, 4), **base, "algorithmic_bytes_per_launch": int(per),
+                 "bytes_rule": "each inserted sample's value and weight read once (12 bytes)"}
 
 
 def main():
@@ -628,22 +661,31 @@ def main():
     t_setup = time.perf_counter()
     ph, tang, timgs, ps = make_workload(args.batch, args.templates)
     cpu = None
-    if not args.no_cpu_baseline:       # host cores only, before any GPU use
+    if not (args.no_cpu_baseline or args.chosen_only):       # host cores only, before any GPU use
         try:
             cpu = cpu_baseline(args, tang, timgs, ps)
         except Exception as e:
             cpu = {"value": None, "unit": "particles/s", "cores": 0, "kind": "port", "sample": f"failed: {e}"}
     r = gpu_run(args, tang, timgs, ps)
-    cen = center_run(ph, ps, args.steps, args.warmup)
     n = r["n"]
     value = n * args.steps / (r["ev_c"] / 1e3)
-    plain = n / (r["ev_p"] / 1e3)
     dom, roof = _roofline(r, args)
+    if args.chosen_only:
+        print(json.dumps({"metric": METRIC + " (chosen path only, step after step)", "value": round(value, 3),
+                          "unit": "particles/s", "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(r["ev_c"] / args.steps, 3), "roofline": roof,
+                          "chosen_kernels": {k: {"avg_ms": round(v[0], 3), "launches": v[1]}
+                                             for k, v in r["kt_c"].items() if v[1]},
+                          "counters": {k: int(v) for k, v in r["cnt_c"].items()}}))
+        return
+    cen = center_run(ph, ps, args.steps, args.warmup)
+    plain = n / (r["ev_p"] / 1e3)
     tj = os.path.join(ROOT, "profiles", "hbm_traffic_box384.json")
     if os.path.exists(tj):   # rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated, per launch
         with open(tj) as f:
             tr = json.load(f)
-        key = {"insert": "k_insert_brickq", "cl_gather": "k_cl_gather"}.get(dom, "k_cl_search")
+        key = {"insert": "k_insert_brickq", "cl_gather": "k_cl_gather", "cl_table": "k_cl_table_mfma"}.get(
+            dom, "k_cl_rescore")
         if key in tr and tr[key].get("batch") == n:
             roof["traffic"] = int(tr[key]["hbm_bytes_per_launch"])
     d = r["d"]

@@ -87,10 +87,12 @@ struct SearchBatch {
     DevBuf lines, lcount;
     // coarse E4M3 tables and the D bounds of the particles and templates they are scaled by
     DevBuf ctab, bound;
+    // fine stage: the two omega windows of each listed point, the coarse second minima and their windows
+    DevBuf fwin, fwin2, outi2, ckwin2;
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
                           &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci, &clist, &nbr, &ccand, &ckwin,
-                          &flist, &fcnt, &lines, &lcount, &ctab, &bound})
+                          &flist, &fcnt, &lines, &lcount, &ctab, &bound, &fwin, &fwin2, &outi2, &ckwin2})
             b->release();
     }
 };

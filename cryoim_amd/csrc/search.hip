@@ -153,6 +153,8 @@ struct SearchArgs {
     int* outi;
     float* prfull;            // optional n x npts x Nomega
     unsigned long long* nsamples;   // optional: evaluated (line, radius, omega) samples
+    unsigned long long* npass;      // optional (k_cl_rescore): samples whose particle amplitude passed its gate
+    int nbatch;                     // k_cl_rescore: particles of the launch
     const int* cand;          // optional n x ncand AU point indices: score only these (grid.x = ncand)
     int ncand;
     const int* kwin;          // optional n x ncand first column of a window of wincols columns (mod ncol)
@@ -186,9 +188,11 @@ __device__ __forceinline__ void sample_pos(float FR, float cs, float sn, float c
 // folded phase residual and weight added to the column's sums (and, PAIR, to
 // those of the Friedel column with the conjugate sample).  Every kernel that
 // scores columns calls this, so a column's bits do not depend on the kernel.
+// npass counts the samples that pass the particle gate.
 template <bool PAIR>
 __device__ __forceinline__ void score_sample(float2 q00, float2 q10, float2 q01, float2 q11, float ax, float ay,
-                                             float4 tv, float f0, float& num, float& den, float& num2, float& den2) {
+                                             float4 tv, float f0, float& num, float& den, float& num2, float& den2,
+                                             unsigned int& npass) {
 #pragma clang fp contract(on)
     const float w00 = (1.f - ax) * (1.f - ay), w10 = ax * (1.f - ay);
     const float w01 = (1.f - ax) * ay, w11 = ax * ay;
@@ -196,6 +200,7 @@ __device__ __forceinline__ void score_sample(float2 q00, float2 q10, float2 q01,
     const float zy = w00 * q00.y + w10 * q10.y + w01 * q01.y + w11 * q11.y;
     const float amp1 = __builtin_amdgcn_sqrtf(zx * zx + zy * zy);
     if (amp1 >= f0) {
+        ++npass;
         const float w = 0.5f * (amp1 + tv.z);
         const float pxy = zx * tv.y, pyx = zy * tv.x, pxx = zx * tv.x, pyy = zy * tv.y;
         num += atan2_upper(fabsf(pxy - pyx), pxx + pyy) * w;
@@ -283,6 +288,7 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
 #pragma unroll
     for (int v = 0; v < VPT; ++v) { num[v] = 0.f; den[v] = 0.f; num2[v] = 0.f; den2[v] = 0.f; }
     unsigned long long pairs = 0;   // (line, radius) pairs that pass the template gate
+    unsigned int npass_unused = 0;
     __syncthreads();
 
     for (int t = 0; t < a.T; ++t) {
@@ -339,7 +345,8 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
                             q00 = make_float2(u.x, u.y); q10 = make_float2(u.z, u.w);
                             q01 = make_float2(w4.x, w4.y); q11 = make_float2(w4.z, w4.w);
                         }
-                        score_sample<PAIR>(q00, q10, q01, q11, ax, ay, tv, f0, num[v], den[v], num2[v], den2[v]);
+                        score_sample<PAIR>(q00, q10, q01, q11, ax, ay, tv, f0, num[v], den[v], num2[v], den2[v],
+                                           npass_unused);
                     }
                 }
             }
@@ -395,19 +402,29 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
 // table minimum.  Each column takes the same terms in the same order through
 // the same arithmetic as in k_cl_search (template_line, sample_pos,
 // score_sample), so its bits equal the plain search's; the lanes of the
-// candidates differ only in which lines and radii they skip.  Particle taps
-// are read tap-paired through the caches.  The first minimum of each
-// candidate's columns goes to outv/outi[p * ncand + candidate].
+// candidates differ only in which lines and radii they skip.  The template
+// line records that pass the template gate are compacted per (candidate,
+// line) in radius order, so each lane walks only its own candidate's
+// records.  Particle taps are read tap-paired through the caches; the
+// workgroups of one particle all go to one XCD (workgroup L runs on XCD
+// L mod 8: L mod 8 picks the particle within a group of 8, (L / 8) mod G the
+// candidate group), so its crops stay in that XCD's L2.  The first minimum of
+// each candidate's columns goes to outv/outi[p * ncand + candidate].
 template <int CPW, bool PAIR>
 __global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
     constexpr int W = 64 / CPW;
     extern __shared__ float4 smem4[];
-    float4* tline = smem4;                                           // CPW x chunk x nr
+    float4* tline = smem4;                                           // CPW x chunk x nr, compacted
     float* alpha1 = reinterpret_cast<float*>(tline + (size_t)CPW * a.chunk * a.nr);   // CPW x 64
     float* alpha2 = alpha1 + CPW * 64;
     float* fc0 = alpha2 + CPW * 64;                                   // nr
+    int* tcnt = reinterpret_cast<int*>(fc0 + a.nr);                  // CPW x chunk: records kept
+    unsigned short* tfr = reinterpret_cast<unsigned short*>(tcnt + CPW * a.chunk);   // their radius indices
     const int lane = threadIdx.x, c = lane / W, col = lane - c * W;
-    const int p = blockIdx.y, g0 = blockIdx.x * CPW;
+    const int G = (a.ncand + CPW - 1) / CPW;
+    const long long L = blockIdx.x;
+    const int p = (int)(L & 7) + 8 * (int)(L / (8LL * G)), g0 = (int)((L >> 3) % G) * CPW;
+    if (p >= a.nbatch) return;
     const bool act = g0 + c < a.ncand;
     const int slot = act ? g0 + c : g0;
     const long long cbase = (long long)p * a.ncand;
@@ -424,6 +441,7 @@ __global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
     for (int i = lane; i < a.nr; i += 64) fc0[i] = a.fcut0[(long long)p * a.nr + i];
     float num = 0.f, den = 0.f, num2 = 0.f, den2 = 0.f;
     unsigned long long pairs = 0;
+    unsigned int npass = 0;
     for (int t = 0; t < a.T; ++t) {
         const float2* tc = tcp + (long long)t * a.Dc * a.Dc;
         const float* tf = tfp + (long long)t * a.nr;
@@ -439,14 +457,30 @@ __global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
         __syncthreads();
         for (int j0 = 0; j0 < a.nops; j0 += a.chunk) {
             const int jn = min(a.chunk, a.nops - j0);
-            for (int q = lane; q < CPW * jn * a.nr; q += 64) {
-                const int cc = q / (jn * a.nr), rem = q - cc * jn * a.nr;
-                const int jj = rem / a.nr, fr = rem - jj * a.nr;
+            // one (candidate, line) row at a time: the lanes take radii, the records that
+            // pass the template gate are packed in radius order
+            for (int row = 0; row < CPW * jn; ++row) {
+                const int cc = row / jn, jj = row - cc * jn;
                 const int j = j0 + jj;
                 const float al1 = alpha1[cc * 64 + j], al2 = alpha2[cc * 64 + j];
-                tline[(cc * a.chunk + jj) * a.nr + fr] =
-                    (al1 < 999.f && al2 < 999.f) ? template_line(tc, tf, a.Dc, a.R, a.minR, fr, al2, cf, ic, fc0[fr])
-                                                 : make_float4(0.f, 0.f, -1.f, fc0[fr]);
+                const bool valid = al1 < 999.f && al2 < 999.f;
+                float4* dst = tline + (cc * a.chunk + jj) * a.nr;
+                unsigned short* dfr = tfr + (cc * a.chunk + jj) * a.nr;
+                int kept = 0;
+                for (int f0 = 0; f0 < a.nr; f0 += 64) {
+                    const int fr = f0 + lane;
+                    float4 tv = make_float4(0.f, 0.f, -1.f, 0.f);
+                    if (valid && fr < a.nr) tv = template_line(tc, tf, a.Dc, a.R, a.minR, fr, al2, cf, ic, fc0[fr]);
+                    const bool ok = tv.z >= 0.f;
+                    const unsigned long long m = __ballot(ok);
+                    if (ok) {
+                        const int at = kept + __popcll(m & ((1ull << lane) - 1ull));
+                        dst[at] = tv;
+                        dfr[at] = (unsigned short)fr;
+                    }
+                    kept += __popcll(m);
+                }
+                if (lane == 0) tcnt[cc * a.chunk + jj] = kept;
             }
             __syncthreads();
             for (int jj = 0; jj < jn; ++jj) {
@@ -456,18 +490,34 @@ __global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
                 float cs, sn;
                 column_dir(a1, a.step, k, cs, sn);
                 const float4* tl = tline + (c * a.chunk + jj) * a.nr;
-                for (int fr = 0; fr < a.nr; ++fr) {
-                    const float4 tv = tl[fr];
-                    if (tv.z < 0.f) continue;
-                    ++pairs;
-                    const float FR = (float)(a.minR + fr);
-                    int lx, ly;
-                    float ax, ay;
-                    sample_pos(FR, cs, sn, cf, lx, ly, ax, ay);
-                    const float4* r0 = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
-                    const float4 u = r0[0], w4 = r0[a.Dc];
-                    score_sample<PAIR>(make_float2(u.x, u.y), make_float2(u.z, u.w), make_float2(w4.x, w4.y),
-                                       make_float2(w4.z, w4.w), ax, ay, tv, tv.w, num, den, num2, den2);
+                const unsigned short* tr = tfr + (c * a.chunk + jj) * a.nr;
+                const int nk = tcnt[c * a.chunk + jj];
+                pairs += nk;
+                if (nk == 0) continue;
+                // software pipeline over pairs of records: the taps of one record are requested
+                // before the other is scored.  An odd count ends with a repeat of the last record
+                // whose gate is +inf, so it adds nothing.
+                int lx, ly;
+                float axA, ayA, axB, ayB;
+                sample_pos((float)(a.minR + (int)tr[0]), cs, sn, cf, lx, ly, axA, ayA);
+                const float4* rA = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
+                float4 uA = rA[0], wA = rA[a.Dc];
+                for (int i = 0; i < nk; i += 2) {
+                    const int ib = min(i + 1, nk - 1);
+                    sample_pos((float)(a.minR + (int)tr[ib]), cs, sn, cf, lx, ly, axB, ayB);
+                    const float4* rB = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
+                    const float4 uB = rB[0], wB = rB[a.Dc];
+                    const float4 tva = tl[i];
+                    score_sample<PAIR>(make_float2(uA.x, uA.y), make_float2(uA.z, uA.w), make_float2(wA.x, wA.y),
+                                       make_float2(wA.z, wA.w), axA, ayA, tva, tva.w, num, den, num2, den2, npass);
+                    sample_pos((float)(a.minR + (int)tr[min(i + 2, nk - 1)]), cs, sn, cf, lx, ly, axA, ayA);
+                    rA = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
+                    uA = rA[0];
+                    wA = rA[a.Dc];
+                    const float4 tvb = tl[ib];
+                    score_sample<PAIR>(make_float2(uB.x, uB.y), make_float2(uB.z, uB.w), make_float2(wB.x, wB.y),
+                                       make_float2(wB.z, wB.w), axB, ayB, tvb, i + 1 < nk ? tvb.w : INFINITY, num,
+                                       den, num2, den2, npass);
                 }
             }
             __syncthreads();
@@ -496,9 +546,16 @@ __global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
     }
     if (a.nsamples) {
         unsigned long long tot = act ? pairs * (PAIR ? 2ull : 1ull) : 0ull;
+        unsigned long long tp = act ? npass * (PAIR ? 2ull : 1ull) : 0ull;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
-        if (lane == 0) atomicAdd(a.nsamples, tot);
+        for (int o = 32; o > 0; o >>= 1) {
+            tot += __shfl_xor(tot, o, 64);
+            tp += __shfl_xor(tp, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(a.nsamples, tot);
+            if (a.npass) atomicAdd(a.npass, tp);
+        }
     }
 }
 
@@ -833,6 +890,15 @@ struct GatherArgs {
     const int* plist;
     int plstride, nlist;
     const int* pcnt;
+    // optional: with plist, the first omegas of a window per listed point: ncol columns in two
+    // halves, from pwin and from pwin2 (mod nomega); null: columns 0 .. ncol-1
+    const int* pwin;
+    const int* pwin2;
+    int nomega;
+    // optional: the linear index of each point's best omega at least excl omega steps from its
+    // first minimum (circularly)
+    int* outi2;
+    int excl;
 };
 
 // Add the {N, D} bfloat16 pair of a table entry to the sums: N the low half,
@@ -945,9 +1011,28 @@ __global__ void __launch_bounds__(256) k_coarse_table(const unsigned int* __rest
     ctab[((long long)item * nrow2 + i) * n2 + jc] = (unsigned short)__builtin_amdgcn_cvt_pk_fp8_f32(N, D, 0, false);
 }
 
+// First minimum (value, then linear index) over a workgroup of at most 4
+// waves; every thread returns with it.
+__device__ __forceinline__ void block_argmin(float& best, int& bidx, float* redv, int* redi) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bidx, o, 64);
+        if (ov < best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+    }
+    const int tid = threadIdx.x, nw = (int)(blockDim.x >> 6);
+    if ((tid & 63) == 0) { redv[tid >> 6] = best; redi[tid >> 6] = bidx; }
+    __syncthreads();
+    best = redv[0];
+    bidx = redi[0];
+    for (int i = 1; i < nw; ++i)
+        if (redv[i] < best || (redv[i] == best && redi[i] < bidx)) { best = redv[i]; bidx = redi[i]; }
+    __syncthreads();
+}
+
 // One workgroup per (AU point, particle), threads over omega: every
 // template's common lines (k_cl_lines) are summed in registers and the
-// first minimum over omega kept.  Workgroups are dealt to the 8 XCDs in turn
+// first minimum over omega kept (and, with outi2, a second one elsewhere).  Workgroups are dealt to the 8 XCDs in turn
 // (workgroup L on XCD L mod 8), so L mod 8 picks the particle within a group
 // of 8 and L / 8 walks its AU points: each XCD reads the tables of one
 // particle at a time.  The line lists are the same for the whole workgroup
@@ -966,6 +1051,9 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     const int pt = a.plist ? a.plist[(long long)p * a.plstride + slot] : slot;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int npsi = a.nray, nrow = npsi >> 1;
+    const int k0 = a.pwin ? a.pwin[(long long)p * a.plstride + slot] : 0;
+    const int half = a.pwin ? a.ncol >> 1 : a.ncol;
+    const int k1 = a.pwin ? a.pwin2[(long long)p * a.plstride + slot] : 0;
     float num[VPT], den[VPT];
     // column of omega k on a line with zero-angle column j0: j0 - k sub, plus npsi when
     // that is negative (columns past ncol read column j0 and are not used)
@@ -974,8 +1062,10 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     for (int v = 0; v < VPT; ++v) {
         num[v] = 0.f;
         den[v] = 0.f;
-        const int k = tid + v * bd;
-        ks[v] = k < a.ncol ? k * a.sub : 0;
+        const int i = tid + v * bd;
+        int k = i < half ? k0 + i : k1 + (i - half);
+        k -= k >= a.nomega ? a.nomega : 0;
+        ks[v] = i < a.ncol ? k * a.sub : 0;
     }
     for (int t = 0; t < a.T; ++t) {
         const unsigned int* lp = a.lines + ((long long)pt * a.T + t) * 64;
@@ -1021,26 +1111,39 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     int bidx = 0x7fffffff;
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
-        const int k = tid + v * bd;
-        if (k >= a.ncol) continue;
+        const int i = tid + v * bd;
+        if (i >= a.ncol) continue;
+        int k = i < half ? k0 + i : k1 + (i - half);
+        k -= k >= a.nomega ? a.nomega : 0;
         const float pr = den[v] > 1.0e-10f ? num[v] / den[v] * r2d : 999.f;
         const int li = a.pbase[pt] + k * a.ostep;
         if (pr < best || (pr == best && li < bidx)) { best = pr; bidx = li; }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const float ov = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bidx, o, 64);
-        if (ov < best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
-    }
-    if ((tid & 63) == 0) { redv[tid >> 6] = best; redi[tid >> 6] = bidx; }
-    __syncthreads();
+    block_argmin(best, bidx, redv, redi);
     if (tid == 0) {
-        for (int i = 1; i < (bd >> 6); ++i)
-            if (redv[i] < best || (redv[i] == best && redi[i] < bidx)) { best = redv[i]; bidx = redi[i]; }
         a.outv[(long long)p * a.npts + pt] = best;
         a.outi[(long long)p * a.npts + pt] = bidx;
     }
+    if (!a.outi2) return;
+    // the best omega at least excl steps from the first minimum (every thread holds it)
+    const int kb = bidx - a.pbase[pt];
+    float best2 = 3.0e38f;
+    int bidx2 = 0x7fffffff;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+        const int i = tid + v * bd;
+        if (i >= a.ncol) continue;
+        int k = i < half ? k0 + i : k1 + (i - half);
+        k -= k >= a.nomega ? a.nomega : 0;
+        int dk = abs(k * a.ostep - kb);
+        dk = min(dk, a.nomega - dk);
+        if (dk < a.excl) continue;
+        const float pr = den[v] > 1.0e-10f ? num[v] / den[v] * r2d : 999.f;
+        const int li = a.pbase[pt] + k * a.ostep;
+        if (pr < best2 || (pr == best2 && li < bidx2)) { best2 = pr; bidx2 = li; }
+    }
+    block_argmin(best2, bidx2, redv, redi);
+    if (tid == 0) a.outi2[(long long)p * a.npts + pt] = bidx2;
 }
 
 // The K AU points of lowest table score per particle (ties: lower linear
@@ -1057,29 +1160,57 @@ __global__ void k_fill_scores(float* __restrict__ v, int* __restrict__ ix, long 
 
 // Coarse-to-fine gather, second stage: the 3x3 grid neighbours of a
 // particle's M best coarse points that have no score yet (every one of them,
-// the coarse points included, when rescore_all: the coarse stage scored only
-// some omega columns), each once, as that particle's fine list (one
-// workgroup per particle; npts <= 8192).  Byte flags in LDS are claimed with
-// an atomic OR on their 32-bit word.
+// the coarse points included, when rescore_all: the coarse stage scored
+// another grid), each once, as that particle's fine list (one workgroup per
+// particle; npts <= 8192).  A point's parent is the best-ranked coarse point
+// that lists it (an atomic minimum of the entry index in LDS, so the choice
+// does not depend on timing); with fwin, the point takes its parent's omega
+// window cwin.  The order of the list is not fixed, and nothing depends on it.
 __global__ void __launch_bounds__(256) k_expand(const int* __restrict__ ccand, int M, const int* __restrict__ nbr,
                                                 const float* __restrict__ v, int npts, int* __restrict__ flist,
-                                                int fstride, int* __restrict__ fcnt, int rescore_all) {
-    __shared__ unsigned int seen[8192 / 4];
+                                                int fstride, int* __restrict__ fcnt, int rescore_all,
+                                                const int* __restrict__ cwin, int* __restrict__ fwin,
+                                                const int* __restrict__ cwin2, int* __restrict__ fwin2) {
+    __shared__ int owner[8192];
     __shared__ int cnt;
     const int p = blockIdx.x, tid = threadIdx.x;
-    for (int i = tid; i < (npts + 3) / 4; i += blockDim.x) seen[i] = 0u;
+    for (int i = tid; i < npts; i += blockDim.x) owner[i] = 0x7fffffff;
     if (tid == 0) cnt = 0;
     __syncthreads();
     for (int e = tid; e < M * 9; e += blockDim.x) {
         const int c = ccand[(long long)p * M + e / 9];
         const int q = nbr[(long long)c * 9 + e % 9];
         if (q < 0 || (!rescore_all && v[(long long)p * npts + q] < 1.0e38f)) continue;
-        const unsigned int bit = 1u << (8 * (q & 3));
-        if (atomicOr(&seen[q >> 2], bit) & bit) continue;
-        flist[(long long)p * fstride + atomicAdd(&cnt, 1)] = q;
+        atomicMin(&owner[q], e);
+    }
+    __syncthreads();
+    for (int e = tid; e < M * 9; e += blockDim.x) {
+        const int c = ccand[(long long)p * M + e / 9];
+        const int q = nbr[(long long)c * 9 + e % 9];
+        if (q < 0 || owner[q] != e) continue;
+        const int at = atomicAdd(&cnt, 1);
+        flist[(long long)p * fstride + at] = q;
+        if (fwin) {
+            fwin[(long long)p * fstride + at] = cwin[(long long)p * M + e / 9];
+            fwin2[(long long)p * fstride + at] = cwin2[(long long)p * M + e / 9];
+        }
     }
     __syncthreads();
     if (tid == 0) fcnt[p] = cnt;
+}
+
+// The first column of a window of w columns (of nomega) centred on the second
+// minimum of each of a particle's M coarse winners (180 deg from the first
+// when a point has none).
+__global__ void k_second_win(const int* __restrict__ ccand, int M, const int* __restrict__ ix2,
+                             const int* __restrict__ ix, int npts, int nomega, int w, int* __restrict__ kwin2) {
+    const int p = blockIdx.x;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        const long long q = (long long)p * npts + ccand[(long long)p * M + m];
+        const int i2 = ix2[q];
+        const int kb = i2 == 0x7fffffff ? ix[q] % nomega + nomega / 2 : i2 % nomega;
+        kwin2[(long long)p * M + m] = ((kb - w / 2) % nomega + nomega) % nomega;
+    }
 }
 
 // The K AU points of lowest table score per particle, any number of points:
@@ -1634,6 +1765,11 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     // centres included
     bool coarse_grid = sub == 1 && npsi % 4 == 0;
     if (const char* ek = getenv("CIM_TABLE_COARSE_GRID")) coarse_grid = coarse_grid && atoi(ek) != 0;
+    // with the coarse grid, the fine stage scores CIM_TABLE_FINE_WINDOW (128) omega columns of each
+    // point: half around its parent's coarse minimum, half around the parent's best omega at least
+    // a half window from it (a second minimum); 0: every omega
+    int fine_win = 128;
+    if (const char* ew = getenv("CIM_TABLE_FINE_WINDOW")) fine_win = std::max(0, atoi(ew)) & ~1;
     std::vector<int> clist, nbrv;
     bool c2f = false;
     if (tables && coarseM > 0 && npts <= 8192) {
@@ -1656,6 +1792,9 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         CIM_ALLOC(sb.ccand, sizeof(int) * (size_t)batch * coarseM);
         CIM_ALLOC(sb.ckwin, sizeof(int) * (size_t)batch * coarseM);
         CIM_ALLOC(sb.flist, sizeof(int) * (size_t)batch * coarseM * 9);
+        CIM_ALLOC(sb.fwin, sizeof(int) * (size_t)batch * coarseM * 9);
+        CIM_ALLOC(sb.fwin2, sizeof(int) * (size_t)batch * coarseM * 9);
+        CIM_ALLOC(sb.ckwin2, sizeof(int) * (size_t)batch * coarseM);
         CIM_ALLOC(sb.fcnt, sizeof(int) * (size_t)batch);
         CIM_HIP(hipMemcpyAsync(sb.clist.ptr, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -1747,9 +1886,11 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         a.outv = sb.outv.as<float>();
         a.outi = sb.outi.as<int>();
         a.prfull = pr_array ? sb.prfull.as<float>() : nullptr;
-        CIM_ALLOC(sb.counter, sizeof(unsigned long long));
-        CIM_HIP(hipMemsetAsync(sb.counter.ptr, 0, sizeof(unsigned long long), ctx->stream));
+        // sample counters: exhaustive launches, re-score gate passes, re-score samples
+        CIM_ALLOC(sb.counter, 3 * sizeof(unsigned long long));
+        CIM_HIP(hipMemsetAsync(sb.counter.ptr, 0, 3 * sizeof(unsigned long long), ctx->stream));
         a.nsamples = sb.counter.as<unsigned long long>();
+        a.npass = nullptr;
         SearchShape sh{};
         rc = tune_shape(ctx, a, lds_with, lds_without, lds_fits, g.No, pair, &sh);
         if (rc) return rc;
@@ -1786,7 +1927,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 // coarse E4M3 tables of the even rays, scaled per particle by the D bounds
                 CIM_ALLOC(sb.bound, sizeof(float) * (size_t)(nb + titems));
                 CIM_ALLOC(sb.ctab, sizeof(unsigned short) * (size_t)nb * T * (npsi / 4) * (npsi / 2));
-                TimedLaunch tl(ctx, "cl_table");
+                TimedLaunch tl(ctx, "cl_coarse_table");
                 hipLaunchKernelGGL(k_ring_bound, dim3(nb), dim3(256), 0, ctx->stream, sb.ppol.as<float2>(), nr, npsi,
                                    sb.bound.as<float>());
                 CIM_HIP(hipGetLastError());
@@ -1800,7 +1941,9 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 CIM_HIP(hipGetLastError());
             }
             GatherArgs ga{sb.table.ptr, sb.lines.as<unsigned int>(), sb.lcount.as<int>(), sb.pbase.as<int>(), T, npts,
-                          npsi, sub, g.No, 1, nb, sb.outv.as<float>(), sb.outi.as<int>(), nullptr, 0, 0, nullptr};
+                          npsi, sub, g.No, 1, nb, sb.outv.as<float>(), sb.outi.as<int>(), nullptr, 0, 0, nullptr,
+                          nullptr, nullptr, g.No, nullptr, 0};
+            const bool fwin = cgrid && fine_win > 0 && fine_win < g.No;
             // one workgroup of 64 x ceil(ncol / (64 VPT)) threads per (AU point, particle)
             auto gather = [&](int nlist, bool coarse) -> int {
                 const int vg = (ga.ncol + 255) / 256;
@@ -1839,23 +1982,44 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                         ga.nray = npsi / 2;
                         ga.ncol = g.No / 2;
                         ga.ostep = 2;
+                        if (fwin) {
+                            CIM_ALLOC(sb.outi2, sizeof(int) * (size_t)nb * npts);
+                            ga.outi2 = sb.outi2.as<int>();
+                            ga.excl = fine_win / 2;
+                        }
                         if ((rc = gather(ga.nlist, true))) return rc;
                         ga = gc;
                     } else if ((rc = gather(ga.nlist, false))) {
                         return rc;
                     }
+                    // the coarse winners, each with two windows of fine_win / 2 omega columns: around
+                    // its coarse minimum and around its best omega away from it
                     hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0,
                                        ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(), npts, coarseM,
-                                       sb.ccand.as<int>(), g.No, ncolx, wincols, sb.ckwin.as<int>());
+                                       sb.ccand.as<int>(), g.No, g.No, fwin ? fine_win / 2 : 0, sb.ckwin.as<int>());
                     CIM_HIP(hipGetLastError());
+                    if (fwin) {
+                        hipLaunchKernelGGL(k_second_win, dim3(nb), dim3(64), 0, ctx->stream, sb.ccand.as<int>(),
+                                           coarseM, sb.outi2.as<int>(), sb.outi.as<int>(), npts, g.No, fine_win / 2,
+                                           sb.ckwin2.as<int>());
+                        CIM_HIP(hipGetLastError());
+                        ga.outi2 = nullptr;
+                    }
                     hipLaunchKernelGGL(k_expand, dim3(nb), dim3(256), 0, ctx->stream, sb.ccand.as<int>(), coarseM,
                                        sb.nbr.as<int>(), sb.outv.as<float>(), npts, sb.flist.as<int>(), coarseM * 9,
-                                       sb.fcnt.as<int>(), cgrid ? 1 : 0);
+                                       sb.fcnt.as<int>(), cgrid ? 1 : 0, sb.ckwin.as<int>(),
+                                       fwin ? sb.fwin.as<int>() : nullptr, sb.ckwin2.as<int>(),
+                                       fwin ? sb.fwin2.as<int>() : nullptr);
                     CIM_HIP(hipGetLastError());
                     ga.plist = sb.flist.as<int>();
                     ga.plstride = coarseM * 9;
                     ga.nlist = coarseM * 9;
                     ga.pcnt = sb.fcnt.as<int>();
+                    if (fwin) {
+                        ga.pwin = sb.fwin.as<int>();
+                        ga.pwin2 = sb.fwin2.as<int>();
+                        ga.ncol = fine_win;
+                    }
                     if ((rc = gather(ga.nlist, false))) return rc;
                 } else if ((rc = gather(npts, false))) {
                     return rc;
@@ -1891,15 +2055,22 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             // windows: one wave of single-column lanes in the same kernel variant
             if (a.kwin && rescore_cpw > 1) {
                 a.chunk = std::min(chunk, 3);
+                a.nsamples = sb.counter.as<unsigned long long>() + 2;
+                a.npass = sb.counter.as<unsigned long long>() + 1;
                 const int lw = (int)(sizeof(float4) * (size_t)rescore_cpw * a.chunk * nr +
-                                     sizeof(float) * (2 * 64 * rescore_cpw + nr));
-                const dim3 grid((unsigned)((topk + rescore_cpw - 1) / rescore_cpw), (unsigned)nb);
+                                     sizeof(float) * (2 * 64 * rescore_cpw + nr) +
+                                     sizeof(int) * (size_t)rescore_cpw * a.chunk +
+                                     sizeof(unsigned short) * (size_t)rescore_cpw * a.chunk * nr);
+                const dim3 grid((unsigned)(((topk + rescore_cpw - 1) / rescore_cpw) * 8LL * ((nb + 7) / 8)));
+                a.nbatch = nb;
 #define CIM_RESCORE(C, P) hipLaunchKernelGGL((k_cl_rescore<C, P>), grid, dim3(64), lw, ctx->stream, a)
                 if (rescore_cpw == 2) { if (sh.pair) CIM_RESCORE(2, true); else CIM_RESCORE(2, false); }
                 else if (rescore_cpw == 4) { if (sh.pair) CIM_RESCORE(4, true); else CIM_RESCORE(4, false); }
                 else { if (sh.pair) CIM_RESCORE(8, true); else CIM_RESCORE(8, false); }
 #undef CIM_RESCORE
                 CIM_HIP(hipGetLastError());
+                a.nsamples = sb.counter.as<unsigned long long>();
+                a.npass = nullptr;
             } else if (a.kwin) {
                 const int lw = (int)(misc + sizeof(float4) * (size_t)a.chunk * nr) +
                                (sh.lds ? (int)(sizeof(float2) * (size_t)disk_total) : 0);
@@ -1941,8 +2112,10 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         if (c2f) {
             long long scored = 0;
             for (int c : hfc) scored += c;
-            ctx->counters["cl_gather_points"] +=
-                scored * g.No + (long long)nb * (long long)clist.size() * (coarse_grid ? g.No / 2 : g.No);
+            const bool fwin = coarse_grid && fine_win > 0 && fine_win < g.No;
+            ctx->counters["cl_gather_points"] += scored * (fwin ? fine_win : g.No) +
+                                                 (long long)nb * (long long)clist.size() *
+                                                     (coarse_grid ? g.No / 2 : g.No);
         }
         if (tables && fb_rank <= topk) {
             // Fallback: when the exact minimum sits deep in the candidate list, the tables ranked this
@@ -1993,10 +2166,12 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 ctx->counters["cl_fallback_particles"] += 1;
             }
         }
-        unsigned long long hcount = 0;
-        CIM_HIP(hipMemcpyAsync(&hcount, sb.counter.ptr, sizeof(hcount), hipMemcpyDeviceToHost, ctx->stream));
+        unsigned long long hcount[3] = {0, 0, 0};
+        CIM_HIP(hipMemcpyAsync(hcount, sb.counter.ptr, sizeof(hcount), hipMemcpyDeviceToHost, ctx->stream));
         CIM_HIP(hipStreamSynchronize(ctx->stream));
-        ctx->counters["cl_search_samples"] += (long long)hcount;
+        ctx->counters["cl_search_samples"] += (long long)(hcount[0] + hcount[2]);
+        ctx->counters["cl_rescore_samples"] += (long long)hcount[2];
+        ctx->counters["cl_rescore_pass"] += (long long)hcount[1];
         ctx->counters["cl_search_particles"] += nb;
         const int plane = g.No * (2 * g.Np + 1);
         for (int i = 0; i < nb; ++i) {
