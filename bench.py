@@ -84,9 +84,12 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="time the chosen path step after step (search, then insert) on one stream")
-    ap.add_argument("--stage", choices=("main", "center", "lbox"), default="main",
+    ap.add_argument("--stage", choices=("main", "center", "lbox", "missrate"), default="main",
                     help="center: only the centre finding of icosprocess mode 3 at box 384; "
-                         "lbox: only local_box's stack mode (12 vertex boxes of 96 pixels per particle)")
+                         "lbox: only local_box's stack mode (12 vertex boxes of 96 pixels per particle); "
+                         "missrate: chosen vs plain search on --particles distinct particles")
+    ap.add_argument("--particles", type=int, default=5120,
+                    help="missrate: distinct particles (generated on the GPU in batches of --batch)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--chosen-only", action="store_true",
@@ -334,6 +337,42 @@ def center_run(ph, ps, steps: int, warmup: int):
     return res
 
 
+def refine_run(tang, timgs, ps, steps: int, warmup: int):
+    """icosprocess mode 5 (ort_refinement: Ncycle 6 cycles of the 3^5 stencil
+    around the starting orientation and shift, deltaAngle and deltaTrans of
+    0calc_initialvalues) on the batch, started from the true orientations
+    rounded to the search grid.  Host images (cim_refine_local): the wall
+    time includes their copy to the device; the kernel time does not."""
+    from cryoim_amd import lib
+    d = derived()
+    n = len(ps.images)
+    ctx = lib.Context(0)
+    maxres = d["maxR"]
+    p = lib.make_params(BOX, d["minR"], d["maxR"], apply_ctf=True, mask_on=True, apix=APIX, imgmask=d["mask"],
+                        searchstep=STEP, ncycle=6, delta_angle=360.0 / (maxres * np.sqrt(5.0) * 3.14159),
+                        delta_trans=BOX / (maxres * np.sqrt(5.0)))
+    ctx.search_templates(p, timgs, np.c_[tang, np.zeros((len(tang), 2))])
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 0:3] = np.round(ps.angles / STEP) * STEP
+    P[:, 3:5] = np.round(ps.shifts)
+    P[:, 6:8] = ps.df
+    P[:, 11] = 1.0
+    for _ in range(warmup):
+        ctx.refine_local(p, ps.images, P)
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ctx.refine_local(p, ps.images, P)
+    wall = (time.perf_counter() - t0) / steps
+    kms, launches = ctx.timing_get("refine")
+    ctx.timing(False)
+    ctx.close()
+    return {"particles_per_s": round(n / wall, 1), "ms_per_step": round(wall * 1e3, 3),
+            "kernel_avg_ms": round(kms, 3), "kernel_particles_per_s": round(n / (kms / 1e3), 1),
+            "median_pr_deg": round(float(np.median(out[:, 5])), 4),
+            "median_shift_change_px": round(float(np.median(np.abs(out[:, 3:5] - P[:, 3:5]))), 3)}
+
+
 _PORT_STATE = {}
 
 
@@ -398,6 +437,110 @@ def center_main(args):
 
 LBOX_BATCH, LBOX_SUB, LBOX_Z = 256, 96, 140.0
 LBOX_METRIC = "particles/s local_box stack mode, box 384: 12 vertex sub-particles of 96^2 per particle"
+
+
+MISS_METRIC = "chosen vs plain global search on distinct particles, box 384: decisions that differ"
+
+
+def missrate_main(args):
+    """The chosen (tabulated) and the plain search on args.particles distinct
+    particles of the bench recipe, generated on the GPU batch by batch (seeds
+    1000 + batch); every particle's orientation and PR compared, fallbacks
+    and ranks counted, and the maps of both sets of orientations compared."""
+    import torch
+    from scipy.stats import beta
+    from cryoim_amd import geometry, lib, synth
+    d = derived()
+    B = args.batch
+    nbat = (args.particles + B - 1) // B
+    s = BOX / 128.0
+    ph = synth.make_phantom(BOX, nblobs=20, sigma=(2.0 * s, 4.0 * s), radius=(40.0 * s, 55.0 * s), seed=1)
+    tang = synth.random_au_orientations(args.templates, np.random.default_rng(2))
+    timgs = synth.project(ph, tang)
+    ctx = lib.Context(0)
+    common = dict(apply_ctf=True, mask_on=True, apix=APIX, imgmask=d["mask"], searchstep=STEP)
+    p_plain = lib.make_params(BOX, d["minR"], d["maxR"], search_method=lib.SEARCH_PLAIN, **common)
+    p_chosen = lib.make_params(BOX, d["minR"], d["maxR"], search_method=lib.SEARCH_TABLES, search_topk=args.topk,
+                               **common)
+    p_insert = lib.make_params(BOX, 0, INSERT_MAXR, apply_ctf=True, mask_on=True, norm_on=True, apix=APIX,
+                               imgmask=d["mask"])
+    ctx.search_templates(p_chosen, timgs, np.c_[tang, np.zeros((len(tang), 2))])
+    maps = [lib.Context(0), lib.Context(0)]
+    for m in maps:
+        m.rec_begin(BOX, INSERT_MAXR, layout=COMPENSATED, rcut=INSERT_MAXR, wg_init=0.01)
+    ctx.counter_reset()
+    t_c = t_p = 0.0
+    n = ident = within = 0
+    misses, err_c, err_p = [], [], []
+    fb = 0
+    for b in range(nbat):
+        m = min(B, args.particles - b * B)
+        ps = synth.make_particles_torch(ph, m, seed=1000 + b, shift_max=3.0, df_range=(0.8, 3.5), snr=0.05,
+                                        apix=APIX, mask_radius=BOX / 2 - 4)
+        torch.cuda.synchronize()
+        P = np.zeros((m, lib.PART_STRIDE), np.float32)
+        P[:, 3:5] = ps.shifts
+        P[:, 6:8] = ps.df
+        P[:, 9:11] = ps.shifts[0]
+        P[:, 11] = 1.0
+        t0 = time.perf_counter()
+        oc = ctx.search_global(p_chosen, ps.images, P)
+        t_c += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        op = ctx.search_global(p_plain, ps.images, P)
+        t_p += time.perf_counter() - t0
+        for mc, o in zip(maps, (oc, op)):
+            Q = P.copy()
+            Q[:, 0:3] = o[:, 0:3]
+            mc.rec_insert(p_insert, ps.images, Q)
+        same = np.all(oc[:, [0, 1, 2, 5]] == op[:, [0, 1, 2, 5]], axis=1)
+        dang = np.abs(((oc[:, 0:3] - op[:, 0:3] + 180) % 360) - 180).max(axis=1)
+        ident += int(same.sum())
+        within += int((dang <= STEP + 1e-4).sum())
+        for i in np.nonzero(~same)[0]:
+            misses.append({"batch": b, "index": int(i), "true": [round(float(v), 3) for v in ps.angles[i]],
+                           "chosen": [round(float(v), 4) for v in oc[i, [0, 1, 2, 5]]],
+                           "plain": [round(float(v), 4) for v in op[i, [0, 1, 2, 5]]],
+                           "sym_distance_deg": round(float(geometry.sym_distance_deg(oc[i:i + 1, 0:3],
+                                                                                      op[i:i + 1, 0:3])[0]), 3)})
+        # view errors against the true orientations, over the 60 symmetry-equivalent forms
+        err_c.append(geometry.sym_distance_deg(oc[:, 0:3], ps.angles))
+        err_p.append(geometry.sym_distance_deg(op[:, 0:3], ps.angles))
+        n += m
+        print(f"missrate batch {b + 1}/{nbat}: {n} particles, {n - ident} differ so far", file=sys.stderr, flush=True)
+        del ps
+    fb = ctx.counter("cl_fallback_particles")
+    ranks = {k[8:]: int(ctx.counter(k)) for k in RANK_KEYS}
+    vols = []
+    for mc in maps:
+        v = torch.empty((BOX, BOX, BOX), dtype=torch.float32, device="cuda:0")
+        mc.rec_finish(BOX, v)
+        mc.sync()
+        vols.append(v)
+    fsc = _fsc(vols[0], vols[1], INSERT_MAXR)
+    rel = float((torch.linalg.vector_norm(vols[0] - vols[1]) / torch.linalg.vector_norm(vols[1])).item())
+    for c in (ctx, *maps):
+        c.close()
+    k = n - ident
+    upper = float(beta.ppf(0.95, k + 1, n - k)) if k < n else 1.0
+    err_c, err_p = np.concatenate(err_c), np.concatenate(err_p)
+    print(json.dumps({"metric": MISS_METRIC, "particles": n, "batches": nbat, "identical": ident,
+                      "within_one_step": within, "differ": k, "differ_rate": k / n,
+                      "differ_rate_upper95": upper, "fallback_particles": int(fb), "exact_minimum_rank": ranks,
+                      "chosen_particles_per_s": round(n / t_c, 1), "plain_particles_per_s": round(n / t_p, 2),
+                      "fsc_min_to_r190": round(float(np.min(fsc[1:])), 6), "map_rel_l2_chosen_vs_plain": rel,
+                      "differ_sym_distance_max_deg": max([m_["sym_distance_deg"] for m_ in misses], default=0.0),
+                      "view_error_vs_truth_deg": {"chosen_median": float(np.median(err_c)),
+                                                  "plain_median": float(np.median(err_p)),
+                                                  "chosen_p99": float(np.percentile(err_c, 99)),
+                                                  "plain_p99": float(np.percentile(err_p, 99)),
+                                                  "chosen_max": float(err_c.max()), "plain_max": float(err_p.max()),
+                                                  "note": "smallest rotation to the true view over the 60 "
+                                                          "icosahedral equivalents"},
+                      "differing": misses[:20],
+                      "config": {"workload": f"box {BOX}, {STEP} deg AU grid x 720 omega, maxR {d['maxR']}, "
+                                             f"T={args.templates}, top-{args.topk}; particles of the bench recipe "
+                                             f"generated on the GPU (seeds 1000 + batch)", "batch": B}}))
 
 
 def vertex_ops():
@@ -566,31 +709,35 @@ def _port_main_job(job):
     ctf = O.ctf2d(BOX, df[0], df[1], 0.0, APIX)
     a, b, tf = pipelines.template_inputs(O, ta, tb, ctf, d["minR"], d["maxR"])
     _, (pa, pb), fc0 = pipelines.particle_inputs(O, img, *shift, d["mask"], d["minR"], d["maxR"])
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
     o4 = O.globesearch(pa, pb, fc0, a, b, tf, tang, d["minR"], d["maxR"], PORT_COARSE)
     t_search = time.perf_counter() - t0
     vol = O.new_volume(BOX, INSERT_MAXR, compact=True, wg_init=0.0)
     t0 = time.perf_counter()
     fa, fb = O.particle_ft(img, *shift, mask_r=d["mask"], norm=True)
     O.insert(vol, fa, fb, ctf, o4[:3], INSERT_MAXR, rcut=INSERT_MAXR - 2, compact=True)
-    return t_search, time.perf_counter() - t0
+    return t_prep, t_search, time.perf_counter() - t0
 
 
 def cpu_baseline(args, tang, timgs, ps):
     """The C oracle port (FP32, one process per core) on a bounded sample:
-    one particle per core searched on the PORT_COARSE grid (its time scaled
-    to the STEP grid by the exact ratio of scored AU-cell x omega points)
-    and inserted at INSERT_MAXR.  The reference's own rate is quoted from
+    one particle per core: its template and particle transforms, the
+    search's scoring loop on the PORT_COARSE grid (that time alone scaled to
+    the STEP grid by the exact ratio of scored AU-cell x omega points) and
+    the insert at INSERT_MAXR.  The reference's own rate is quoted from
     profiles/ beside it."""
     ncores = min(8, os.cpu_count() or 1)
     jobs = [(timgs, tang, ps.images[i], tuple(ps.shifts[i]), tuple(ps.df[i])) for i in range(ncores)]
     res, wall = _port_pool(_port_main_job, jobs, ncores)
     scale = _grid_work(STEP) / _grid_work(PORT_COARSE)
-    per = float(np.mean([ts * scale + ti for ts, ti in res]))      # seconds per particle on one core
+    per = float(np.mean([tp + ts * scale + ti for tp, ts, ti in res]))      # seconds per particle on one core
+    m = np.mean(res, axis=0)
     return {"value": ncores / per, "unit": "particles/s", "cores": ncores, "kind": "port",
-            "sample": (f"C oracle (FP32) in {ncores} processes, one particle each: global search on the "
-                       f"{PORT_COARSE} deg grid (mean {np.mean([r[0] for r in res]):.2f} s, x{scale:.0f} to the "
-                       f"{STEP} deg grid by AU-cell x omega count) + 60-fold insert at maxR {INSERT_MAXR} (mean "
-                       f"{np.mean([r[1] for r in res]):.2f} s); {wall:.1f} s wall"),
+            "sample": (f"C oracle (FP32) in {ncores} processes, one particle each: transforms (mean {m[0]:.2f} s) "
+                       f"+ the search's scoring on the {PORT_COARSE} deg grid (mean {m[1]:.2f} s, x{scale:.0f} to "
+                       f"the {STEP} deg grid by AU-cell x omega count) + 60-fold insert at maxR {INSERT_MAXR} "
+                       f"(mean {m[2]:.2f} s); {wall:.1f} s wall"),
             "reference_here": _quoted_reference("main")}
 
 
@@ -658,6 +805,9 @@ def main():
     if args.stage == "lbox":
         lbox_main(args)
         return
+    if args.stage == "missrate":
+        missrate_main(args)
+        return
     t_setup = time.perf_counter()
     ph, tang, timgs, ps = make_workload(args.batch, args.templates)
     cpu = None
@@ -679,15 +829,17 @@ def main():
                           "counters": {k: int(v) for k, v in r["cnt_c"].items()}}))
         return
     cen = center_run(ph, ps, args.steps, args.warmup)
+    ref5 = refine_run(tang, timgs, ps, min(args.steps, 3), 1)
     plain = n / (r["ev_p"] / 1e3)
     tj = os.path.join(ROOT, "profiles", "hbm_traffic_box384.json")
     if os.path.exists(tj):   # rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated, per launch
         with open(tj) as f:
             tr = json.load(f)
-        key = {"insert": "k_insert_brickq", "cl_gather": "k_cl_gather", "cl_table": "k_cl_table_mfma"}.get(
-            dom, "k_cl_rescore")
-        if key in tr and tr[key].get("batch") == n:
-            roof["traffic"] = int(tr[key]["hbm_bytes_per_launch"])
+        keys = {"insert": ["k_insert_brickq"], "cl_gather": ["k_cl_gather_coarse", "k_cl_gather_fine"],
+                "cl_table": ["k_cl_table_mfma"]}.get(dom, ["k_cl_rescore"])
+        if all(k in tr and tr[k].get("batch") == n for k in keys):
+            roof["traffic"] = int(sum(tr[k]["hbm_bytes_per_launch"] for k in keys))
+            roof["traffic_source"] = "profiles/hbm_traffic_box384.json (rocprofv3 FETCH_SIZE and WRITE_SIZE, copy-calibrated)"
     d = r["d"]
     fsc = r["fsc"]
     kt = {k: {"avg_ms": round(v[0], 3), "launches": v[1]} for k, v in r["kt_c"].items() if v[1]}
@@ -731,6 +883,7 @@ def main():
                   "search_angle_err_deg_median": float(np.median(r["ang_err"])),
                   "center_mode3": {k: {kk: v[kk] for kk in ("particles_per_s", "ms_per_step")}
                                    for k, v in cen.items()},
+                  "refine_mode5": ref5,
                   "wall_s": round(r["wall_c"], 3),
                   "setup_s": round(time.perf_counter() - t_setup, 1)},
     }

@@ -132,3 +132,23 @@ def au_grid_points(step: float):
             if in_au(z0 * np.cos(ph), z0 * np.sin(ph), z):
                 pts.append((it, ip))
     return pts
+
+
+def sym_distance_deg(a, b, ops: np.ndarray | None = None) -> np.ndarray:
+    """Smallest rotation angle (degrees) between orientations a and b (n, 3)
+    Euler angles, over the symmetry-equivalent forms of b: the model is
+    invariant under the operators, so R and R g (model frame) are the same
+    view for every operator g (60 icosahedral operators by default)."""
+    ops = icos_ops() if ops is None else ops
+    a = np.asarray(a, dtype=np.float64).reshape(-1, 3)
+    b = np.asarray(b, dtype=np.float64).reshape(-1, 3)
+    Ra = euler_mat(a[:, 0], a[:, 1], a[:, 2])
+    Rb = euler_mat(b[:, 0], b[:, 1], b[:, 2])
+    best = np.full(len(a), 180.0)
+    for g in ops:
+        for M in (Rb @ g, g @ Rb):
+            # trace of Ra^T M gives the angle of the relative rotation
+            tr = np.einsum("nij,nij->n", Ra, M)
+            ang = np.rad2deg(np.arccos(np.clip((tr - 1.0) / 2.0, -1.0, 1.0)))
+            best = np.minimum(best, ang)
+    return best

@@ -155,3 +155,52 @@ def make_particles(ph: Phantom, n: int, seed: int = 1, shift_max: float = 3.0,
             var = float(np.var(imgs[i][inside]))
             imgs[i] = imgs[i] + rng.normal(scale=np.sqrt(var / snr), size=(nx, nx)).astype(np.float32)
     return ParticleSet(imgs.astype(np.float32), ang, sh, dfs, astig)
+
+
+def make_particles_torch(ph: Phantom, n: int, seed: int = 1, shift_max: float = 3.0, df_range=(1.0, 3.0),
+                         snr: Optional[float] = 0.1, apix: float = 1.5, vol: float = 300.0, cs: float = 2.7,
+                         ampwg: float = 0.1, mask_radius: Optional[float] = None, device: str = "cuda:0"):
+    """The particles of make_particles (same recipe: AU orientations, shifts,
+    defoci from a numpy generator seeded with seed; projections, CTF and
+    noise) generated on a torch device, for workloads too large to render on
+    the host.  The noise comes from a torch generator seeded with seed, so
+    images differ from make_particles' in their noise only.  Returns a
+    ParticleSet whose images are a float32 (n, box, box) device tensor."""
+    import torch
+    rng = np.random.default_rng(seed)
+    ang = random_au_orientations(n, rng)
+    sh = rng.uniform(-shift_max, shift_max, size=(n, 2))
+    df = rng.uniform(df_range[0], df_range[1], size=n)
+    nx = ph.box
+    c = nx // 2
+    R = geometry.euler_mat(ang[:, 0], ang[:, 1], ang[:, 2])
+    uv = np.einsum("nji,bj->nbi", R, ph.centres)[..., :2] + sh[:, None, :]
+    dev = torch.device(device)
+    ax = torch.arange(nx, dtype=torch.float64, device=dev) - c
+    inv = torch.as_tensor(1.0 / (2 * ph.sigmas ** 2), device=dev)
+    w = torch.as_tensor(ph.amps * ph.sigmas * np.sqrt(2 * np.pi), device=dev)
+    uvt = torch.as_tensor(uv, device=dev)
+    # separable Gaussians: image = Gy^T diag(w) Gx, one product per particle
+    gx = torch.exp(-(ax[None, None, :] - uvt[:, :, 0:1]) ** 2 * inv[None, :, None]).float()
+    gy = (torch.exp(-(ax[None, None, :] - uvt[:, :, 1:2]) ** 2 * inv[None, :, None]) * w[None, :, None]).float()
+    imgs = torch.bmm(gy.transpose(1, 2).contiguous(), gx.contiguous())
+    del gx, gy
+    # CTF of getCTF2d per particle, on the uncentred FFT grid
+    f = torch.fft.fftfreq(nx, d=1.0, device=dev, dtype=torch.float64) * nx
+    ky, hx = torch.meshgrid(f, f, indexing="ij")
+    lam = 0.387818 / np.sqrt(vol * (1.0 + 0.978459e-3 * vol))
+    res2 = (hx ** 2 + ky ** 2) / (nx * apix) ** 2
+    dft = torch.as_tensor(df * 1.0e4, device=dev)[:, None, None]
+    phase = 2 * np.pi * (-cs * 1e7 * lam ** 3 * res2[None] ** 2 / 4 + dft * lam * res2[None] / 2)
+    ctf = (-(np.sqrt(1 - ampwg ** 2) * torch.sin(phase) + ampwg * torch.cos(phase))).float()
+    imgs = torch.fft.ifft2(torch.fft.fft2(imgs) * ctf).real.contiguous()
+    del ctf, phase
+    if snr is not None:
+        r = mask_radius if mask_radius is not None else nx / 2 - 2
+        yy, xx = torch.meshgrid(ax, ax, indexing="ij")
+        inside = (xx ** 2 + yy ** 2) <= r * r
+        var = imgs[:, inside].var(dim=1, unbiased=False)
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(seed))
+        imgs += torch.randn(imgs.shape, generator=g, device=dev) * torch.sqrt(var / snr)[:, None, None]
+    return ParticleSet(imgs.float().contiguous(), ang, sh, np.stack([df, df], axis=1), np.zeros(n))

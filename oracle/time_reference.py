@@ -127,9 +127,13 @@ def _ref_env():
 
 
 def _ref_baseline(exe, args, tang, timgs, ps, ncores):
+    """icosprocess mode 4 at two coarse search steps, one particle per
+    process: the time is fitted as a + b x (AU cells x omega) and b carried
+    to the bench's step (a: start-up, template and particle transforms); then
+    mode 1 subrec at the insert radius."""
     from cryoim_amd import formats
     d = bench.derived()
-    coarse = 4.0
+    coarse_steps = (4.0, 2.5)
     m = len(ps.images)
     with tempfile.TemporaryDirectory() as tmp:
         formats.write_mrc(os.path.join(tmp, "stk.mrc"), ps.images)
@@ -140,15 +144,17 @@ def _ref_baseline(exe, args, tang, timgs, ps, ncores):
         formats.write_ortfile(os.path.join(tmp, "ort.txt"), parts)
         common = [f"apix={APIX}", f"maxRes={MAXRES}", f"minRes={MINRES}", f"imgmask={d['mask'] * APIX}",
                   "ortfile=ort.txt"]
-        t0 = time.perf_counter()
-        procs = [subprocess.Popen([exe, "mode=4", "templateort=tort.txt", "templatestck=tmpl.mrc",
-                                   f"searchstep={coarse}", f"newortfile=o{i}.txt", f"first={i + 1}",
-                                   f"last={i + 1}", *common], cwd=tmp, stdout=subprocess.DEVNULL,
-                                  stderr=subprocess.DEVNULL, env=_ref_env()) for i in range(ncores)]
-        rcs = [p.wait() for p in procs]
-        t_search = time.perf_counter() - t0
-        if any(rcs):
-            raise RuntimeError(f"icosprocess mode 4 exit codes {rcs}")
+        t_search = {}
+        for coarse in coarse_steps:
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([exe, "mode=4", "templateort=tort.txt", "templatestck=tmpl.mrc",
+                                       f"searchstep={coarse}", f"newortfile=o{i}.txt", f"first={i + 1}",
+                                       f"last={i + 1}", *common], cwd=tmp, stdout=subprocess.DEVNULL,
+                                      stderr=subprocess.DEVNULL, env=_ref_env()) for i in range(ncores)]
+            rcs = [p.wait() for p in procs]
+            t_search[coarse] = time.perf_counter() - t0
+            if any(rcs):
+                raise RuntimeError(f"icosprocess mode 4 exit codes {rcs}")
         # insert: reconstruct_substep at maxR 190 (maxRes = apix*box/190), 2 particles per process
         maxres_ins = APIX * BOX / INSERT_MAXR
         nins = 2
@@ -166,14 +172,18 @@ def _ref_baseline(exe, args, tang, timgs, ps, ncores):
         t_ins = time.perf_counter() - t0
         if any(rcs):
             raise RuntimeError(f"icosprocess mode 1 exit codes {rcs}")
-    scale = bench._grid_work(STEP) / bench._grid_work(coarse)
-    per_particle_core = t_search * scale + t_ins / nins      # seconds per particle on one core
+    (s1, s2) = coarse_steps
+    w1, w2 = bench._grid_work(s1), bench._grid_work(s2)
+    b = (t_search[s2] - t_search[s1]) / (w2 - w1)              # seconds per AU cell x omega
+    a = t_search[s1] - b * w1
+    per_particle_core = a + b * bench._grid_work(STEP) + t_ins / nins      # seconds per particle on one core
     rate = ncores / per_particle_core
     return {"value": rate, "unit": "particles/s", "cores": ncores, "kind": "reference",
             "sample": (f"icosprocess (flang -O2, MKL FFTW) as {ncores} concurrent processes: mode 4 on 1 particle "
-                       f"each at searchstep {coarse} ({t_search:.1f} s, x{scale:.0f} to the {STEP} deg grid by "
-                       f"AU-cell x omega count) + mode 1 subrec at maxR {INSERT_MAXR} on {nins} particles each "
-                       f"({t_ins:.1f} s)")}
+                       f"each at searchstep {s1} ({t_search[s1]:.1f} s) and {s2} ({t_search[s2]:.1f} s), fitted as "
+                       f"{a:.2f} s + {b * 1e6:.2f} us per AU cell x omega and carried to the {STEP} deg grid "
+                       f"({bench._grid_work(STEP)} cells x omega); + mode 1 subrec at maxR {INSERT_MAXR} on {nins} "
+                       f"particles each ({t_ins:.1f} s)")}
 
 
 
