@@ -155,6 +155,10 @@ struct SearchArgs {
     unsigned long long* nsamples;   // optional: evaluated (line, radius, omega) samples
     unsigned long long* npass;      // optional (k_cl_rescore): samples whose particle amplitude passed its gate
     int nbatch;                     // k_cl_rescore: particles of the launch
+    // optional (k_cl_search): score the particles fb_list[s] for s < *fb_count, grid.y at a time,
+    // results of list entry s at outv/outi[s * gridDim.x + point]
+    const int* fb_list;
+    const int* fb_count;
     const int* cand;          // optional n x ncand AU point indices: score only these (grid.x = ncand)
     int ncand;
     const int* kwin;          // optional n x ncand first column of a window of wincols columns (mod ncol)
@@ -240,9 +244,7 @@ __device__ __forceinline__ float4 template_line(const float2* __restrict__ tc, c
 // the second sample is the conjugate of the first: one interpolation feeds
 // both columns.
 template <int VPT, int MODE, bool PAIR>
-__global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
-    extern __shared__ float4 smem4[];
-    char* sm = reinterpret_cast<char*>(smem4);
+__device__ __forceinline__ void cl_search_one(const SearchArgs& a, char* sm, const int p, const long long oslot) {
     float2* disk = reinterpret_cast<float2*>(sm);
     size_t off = MODE == 0 ? sizeof(float2) * (size_t)a.disk_total : 0;
     // template line record per (line, radius): re, im, |F| (-1 if gated out), particle Fcut
@@ -256,9 +258,8 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
     int* redi = reinterpret_cast<int*>(redv + 16);
 
     const int split = a.kwin ? a.wsplit : 1;
-    const int slot = blockIdx.x / split, part = blockIdx.x - slot * split, p = blockIdx.y;
+    const int slot = blockIdx.x / split, part = blockIdx.x - slot * split;
     const int pt = a.cand ? a.cand[(long long)p * a.ncand + slot] : slot;
-    const long long oslot = (long long)p * gridDim.x + blockIdx.x;
     const int tid = threadIdx.x, bd = blockDim.x;
     const float2* pc = a.pcrop + (long long)p * a.Dc * a.Dc;
     const float4* pc4 = a.pcrop4 + (long long)p * a.Dc * a.Dc;
@@ -394,6 +395,88 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
         a.outi[oslot] = bidx;
         if (a.nsamples) atomicAdd(a.nsamples, pairs * (unsigned long long)a.Nomega);
     }
+}
+
+template <int VPT, int MODE, bool PAIR>
+__global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
+    extern __shared__ float4 smem4[];
+    char* sm = reinterpret_cast<char*>(smem4);
+    if (!a.fb_list) {
+        cl_search_one<VPT, MODE, PAIR>(a, sm, blockIdx.y, (long long)blockIdx.y * gridDim.x + blockIdx.x);
+        return;
+    }
+    // listed particles, grid.y of them at a time (the count is known on the device only)
+    const int cnt = *a.fb_count;
+    for (int s = blockIdx.y; s < cnt; s += gridDim.y) {
+        if (s != (int)blockIdx.y) __syncthreads();
+        cl_search_one<VPT, MODE, PAIR>(a, sm, a.fb_list[s], (long long)s * gridDim.x + blockIdx.x);
+    }
+}
+
+// The exhaustive fallback decision of the tabulated search, per particle:
+// the rank of its exact minimum among its candidates (counted in rank[0..4]:
+// < 4, < 8, < 16, < 24, more) and, with windows, whether it lies within 2
+// columns of its window's edge.  A particle whose minimum ranks fb_rank or
+// deeper, or lies at an edge, is appended to list (count in *cnt, order
+// free: each listed particle's result goes back to its own index).
+__global__ void k_fb_select(const int* __restrict__ fi, const int* __restrict__ cand, const int* __restrict__ kwin,
+                            const int* __restrict__ pbase, int nb, int topk, int nomega, int ncol, int wincols,
+                            int fb_rank, unsigned long long* __restrict__ rank, int* __restrict__ list,
+                            int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int li = fi[i];
+    const int base = li - li % nomega;
+    int r = topk;
+    for (int c = 0; c < topk; ++c)
+        if (pbase[cand[(long long)i * topk + c]] == base) { r = c; break; }
+    atomicAdd(&rank[r < 4 ? 0 : r < 8 ? 1 : r < 16 ? 2 : r < 24 ? 3 : 4], 1ull);
+    bool edge = false;
+    if (kwin && r < topk) {
+        const int kb = (li % nomega) % ncol;
+        const int off = ((kb - kwin[(long long)i * topk + r]) % ncol + ncol) % ncol;
+        edge = off < 2 || off > wincols - 3;
+    }
+    if (r >= fb_rank || edge) list[atomicAdd(cnt, 1)] = i;
+}
+
+// Each listed particle's exhaustive minimum (first minimum by value, then
+// linear index, over its npts points) replaces its tabulated result.
+__global__ void k_fb_merge(const float* __restrict__ v, const int* __restrict__ ix, int npts,
+                           const int* __restrict__ list, const int* __restrict__ cnt, float* __restrict__ ov,
+                           int* __restrict__ oi) {
+    __shared__ float sv[4];
+    __shared__ int si[4];
+    const int s = blockIdx.x;
+    if (s >= *cnt) return;
+    float best = 3.0e38f;
+    int bidx = 0x7fffffff;
+    for (int i = threadIdx.x; i < npts; i += blockDim.x) {
+        const float x = v[(long long)s * npts + i];
+        const int k = ix[(long long)s * npts + i];
+        if (x < best || (x == best && k < bidx)) { best = x; bidx = k; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float xv = __shfl_xor(best, o, 64);
+        const int xi = __shfl_xor(bidx, o, 64);
+        if (xv < best || (xv == best && xi < bidx)) { best = xv; bidx = xi; }
+    }
+    if ((threadIdx.x & 63) == 0) { sv[threadIdx.x >> 6] = best; si[threadIdx.x >> 6] = bidx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+            if (sv[i] < best || (sv[i] == best && si[i] < bidx)) { best = sv[i]; bidx = si[i]; }
+        ov[list[s]] = best;
+        oi[list[s]] = bidx;
+    }
+}
+
+// Fill n 64-bit words (counters) with zero: a kernel, so that the work stays
+// capturable in a graph.
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // Exact re-score of the tabulated search's candidates, CPW candidates per
@@ -1519,6 +1602,9 @@ static int project_templates(cim_ctx* ctx, int box, int T, const float* model, c
 }
 
 
+// exhaustive fallback: listed particles scored concurrently by one launch
+constexpr int FB_CONCURRENT = 8;
+
 static int launch_search(cim_ctx* ctx, SearchArgs& a, int n, int lds_bytes, int mode, bool pair, int vpt, int bd) {
     dim3 grid(a.cand ? a.ncand * (a.kwin ? a.wsplit : 1) : a.npts, n);
 #define CIM_LAUNCH(V, M, P)                                                                                      \
@@ -1725,8 +1811,6 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         const int v = atoi(ef);
         fb_rank = v > 0 ? v : (v == 0 ? topk + 1 : 0);
     }
-    std::map<int, int> base2pt;
-    for (int i = 0; i < npts; ++i) base2pt[g.base[i]] = i;
     // exact re-scoring window: wincols columns per (paired) omega half around the table minimum
     const int ncolx = pair ? g.No / 2 : g.No;
     // CIM_RESCORE_CPW candidates (1, 2, 4 or 8) share a wave, each with 64 / CPW columns; CPW = 1
@@ -1886,9 +1970,11 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         a.outv = sb.outv.as<float>();
         a.outi = sb.outi.as<int>();
         a.prfull = pr_array ? sb.prfull.as<float>() : nullptr;
-        // sample counters: exhaustive launches, re-score gate passes, re-score samples
-        CIM_ALLOC(sb.counter, 3 * sizeof(unsigned long long));
-        CIM_HIP(hipMemsetAsync(sb.counter.ptr, 0, 3 * sizeof(unsigned long long), ctx->stream));
+        // counters: samples of exhaustive launches, re-score gate passes, re-score samples, the 5 rank
+        // bins of k_fb_select
+        CIM_ALLOC(sb.counter, 8 * sizeof(unsigned long long));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, sb.counter.as<unsigned long long>(), 8);
+        CIM_HIP(hipGetLastError());
         a.nsamples = sb.counter.as<unsigned long long>();
         a.npass = nullptr;
         SearchShape sh{};
@@ -2084,6 +2170,41 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         hipLaunchKernelGGL(k_argmin, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(),
                            nscore, sb.fv.as<float>(), sb.fi.as<int>());
         CIM_HIP(hipGetLastError());
+        if (tables && fb_rank <= topk) {
+            // Fallback: when the exact minimum sits deep in the candidate list (or at its window's edge),
+            // the tables ranked the particle poorly; score every AU point of it exactly instead (the
+            // plain search).  Decided and run on the device: the listed particles go through one
+            // exhaustive launch, FBS of them at a time.
+            CIM_ALLOC(sb.fbl, sizeof(int) * (size_t)(nb + 2));
+            int* fbcnt = sb.fbl.as<int>();
+            int* fblist = fbcnt + 2;
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream,
+                               reinterpret_cast<unsigned long long*>(fbcnt), 1);
+            hipLaunchKernelGGL(k_fb_select, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, sb.fi.as<int>(),
+                               sb.cand.as<int>(), a.kwin, sb.pbase.as<int>(), nb, topk, g.No, ncolx, wincols, fb_rank,
+                               sb.counter.as<unsigned long long>() + 3, fblist, fbcnt);
+            CIM_HIP(hipGetLastError());
+            SearchArgs a1 = a;
+            a1.cand = nullptr;
+            a1.ncand = 0;
+            a1.kwin = nullptr;      // every omega column of every AU point
+            a1.wsplit = 1;
+            a1.fb_list = fblist;
+            a1.fb_count = fbcnt;
+            CIM_ALLOC(sb.cv, sizeof(float) * (size_t)npts * nb);
+            CIM_ALLOC(sb.ci, sizeof(int) * (size_t)npts * nb);
+            a1.outv = sb.cv.as<float>();
+            a1.outi = sb.ci.as<int>();
+            {
+                TimedLaunch tl(ctx, "cl_fallback");
+                rc = launch_search(ctx, a1, std::min(nb, FB_CONCURRENT), sh.lds_bytes, sh.lds ? 0 : 1, sh.pair,
+                                   sh.vpt, bd_exact);
+                if (rc) return rc;
+            }
+            hipLaunchKernelGGL(k_fb_merge, dim3(nb), dim3(256), 0, ctx->stream, sb.cv.as<float>(), sb.ci.as<int>(),
+                               npts, fblist, fbcnt, sb.fv.as<float>(), sb.fi.as<int>());
+            CIM_HIP(hipGetLastError());
+        }
         std::vector<float> hv(nb);
         std::vector<int> hi(nb);
         CIM_HIP(hipMemcpyAsync(hv.data(), sb.fv.ptr, sizeof(float) * nb, hipMemcpyDeviceToHost, ctx->stream));
@@ -2099,15 +2220,9 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             CIM_HIP(hipMemcpyAsync(full.data(), sb.prfull.ptr, sizeof(float) * full.size(), hipMemcpyDeviceToHost,
                                    ctx->stream));
         }
-        std::vector<int> hc, hw;
-        if (tables) {
-            hc.resize((size_t)nb * topk);
-            hw.resize((size_t)nb * topk);
-            CIM_HIP(hipMemcpyAsync(hc.data(), sb.cand.ptr, sizeof(int) * hc.size(), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-            CIM_HIP(hipMemcpyAsync(hw.data(), sb.kwin.ptr, sizeof(int) * hw.size(), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        }
+        int hfb = 0;
+        if (tables && fb_rank <= topk)
+            CIM_HIP(hipMemcpyAsync(&hfb, sb.fbl.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         CIM_HIP(hipStreamSynchronize(ctx->stream));
         if (c2f) {
             long long scored = 0;
@@ -2117,61 +2232,18 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                                  (long long)nb * (long long)clist.size() *
                                                      (coarse_grid ? g.No / 2 : g.No);
         }
-        if (tables && fb_rank <= topk) {
-            // Fallback: when the exact minimum sits deep in the candidate list, the tables ranked this
-            // particle poorly; score every AU point exactly instead (the plain search for that particle).
-            for (int i = 0; i < nb; ++i) {
-                const int base = hi[i] - hi[i] % g.No;
-                const int pt = base2pt.count(base) ? base2pt[base] : -1;
-                int rank = topk;
-                for (int c = 0; c < topk; ++c)
-                    if (hc[(size_t)i * topk + c] == pt) { rank = c; break; }
-                // where the exact minimum ranks among the table's candidates
-                ctx->counters[rank < 4 ? "cl_rank_lt4" : rank < 8 ? "cl_rank_lt8" : rank < 16 ? "cl_rank_lt16"
-                              : rank < 24 ? "cl_rank_lt24" : "cl_rank_ge24"] += 1;
-                bool edge = false;
-                if (a.kwin && rank < topk) {
-                    // exact minimum within 2 columns of its window edge: a lower one may lie outside
-                    const int kb = (hi[i] % g.No) % ncolx;
-                    const int off = ((kb - hw[(size_t)i * topk + rank]) % ncolx + ncolx) % ncolx;
-                    edge = off < 2 || off > wincols - 3;
-                }
-                if (rank < fb_rank && !edge) continue;
-                SearchArgs a1 = a;
-                a1.cand = nullptr;
-                a1.ncand = 0;
-                a1.kwin = nullptr;      // every omega column of every AU point
-                a1.wsplit = 1;
-                a1.pcrop += (long long)i * Dc * Dc;
-                a1.pcrop4 += (long long)i * Dc * Dc;
-                a1.fcut0 += (long long)i * nr;
-                a1.tcrop += (long long)i * a.tc_pstride;
-                a1.tfcut += (long long)i * a.tf_pstride;
-                CIM_ALLOC(sb.cv, sizeof(float) * npts);
-                CIM_ALLOC(sb.ci, sizeof(int) * npts);
-                a1.outv = sb.cv.as<float>();
-                a1.outi = sb.ci.as<int>();
-                {
-                    TimedLaunch tl(ctx, "cl_fallback");
-                    rc = launch_search(ctx, a1, 1, sh.lds_bytes, sh.lds ? 0 : 1, sh.pair, sh.vpt, bd_exact);
-                    if (rc) return rc;
-                }
-                hipLaunchKernelGGL(k_argmin, dim3(1), dim3(256), 0, ctx->stream, sb.cv.as<float>(), sb.ci.as<int>(),
-                                   npts, sb.fv.as<float>() + i, sb.fi.as<int>() + i);
-                CIM_HIP(hipGetLastError());
-                CIM_HIP(hipMemcpyAsync(&hv[i], sb.fv.as<float>() + i, sizeof(float), hipMemcpyDeviceToHost,
-                                       ctx->stream));
-                CIM_HIP(hipMemcpyAsync(&hi[i], sb.fi.as<int>() + i, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-                CIM_HIP(hipStreamSynchronize(ctx->stream));
-                ctx->counters["cl_fallback_particles"] += 1;
-            }
-        }
-        unsigned long long hcount[3] = {0, 0, 0};
+        ctx->counters["cl_fallback_particles"] += hfb;
+        unsigned long long hcount[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         CIM_HIP(hipMemcpyAsync(hcount, sb.counter.ptr, sizeof(hcount), hipMemcpyDeviceToHost, ctx->stream));
         CIM_HIP(hipStreamSynchronize(ctx->stream));
         ctx->counters["cl_search_samples"] += (long long)(hcount[0] + hcount[2]);
         ctx->counters["cl_rescore_samples"] += (long long)hcount[2];
         ctx->counters["cl_rescore_pass"] += (long long)hcount[1];
+        if (tables && fb_rank <= topk) {
+            // where each particle's exact minimum ranked among the table's candidates
+            const char* bins[5] = {"cl_rank_lt4", "cl_rank_lt8", "cl_rank_lt16", "cl_rank_lt24", "cl_rank_ge24"};
+            for (int b = 0; b < 5; ++b) ctx->counters[bins[b]] += (long long)hcount[3 + b];
+        }
         ctx->counters["cl_search_particles"] += nb;
         const int plane = g.No * (2 * g.Np + 1);
         for (int i = 0; i < nb; ++i) {
