@@ -263,6 +263,22 @@ def gpu_run(args, tang, timgs, ps):
     else:
         out_c, ev_c, wall_c, kt_c, cnt_c = timed_pipelined(p_chosen, COMPENSATED, args.steps, args.warmup)
         _, ev_seq, _, _, _ = timed(p_chosen, COMPENSATED, args.steps, 0)
+    # the same step captured as one HIP graph (cim_step_capture) and replayed step after step
+    ctx.rec_begin(BOX, INSERT_MAXR, layout=COMPENSATED, rcut=INSERT_MAXR, wg_init=0.01)
+    parts_dev = torch.from_numpy(P.copy()).to("cuda:0")
+    ctx.step_capture(p_chosen, p_insert, images, parts_dev)
+    for _ in range(args.warmup):
+        ctx.step_replay()
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    g0.record(stream)
+    for _ in range(args.steps):
+        ctx.step_replay()
+    g1.record(stream)
+    torch.cuda.synchronize()
+    ev_graph = g0.elapsed_time(g1)
+    graph_same = int(np.all(parts_dev.cpu().numpy()[:, [0, 1, 2, 5]] == out_c[:, [0, 1, 2, 5]], axis=1).sum())
+    ctx.step_release()
     # plain path on the same particles: one timed step after one untimed
     out_p, ev_p, wall_p, kt_p, cnt_p = timed(p_plain, 0, 1, 1)
     # one map per path from one pass over the batch, and their FSC
@@ -286,7 +302,7 @@ def gpu_run(args, tang, timgs, ps):
     ang_err = np.abs(((out_p[:, 0:3] - ps.angles + 180) % 360) - 180)
     return dict(n=n, d=d, ev_c=ev_c, wall_c=wall_c, kt_c=kt_c, cnt_c=cnt_c, ev_p=ev_p, wall_p=wall_p, kt_p=kt_p,
                 cnt_p=cnt_p, identical=identical, within=within, fsc=fsc, map_rel=map_rel, ang_err=ang_err,
-                ev_seq=ev_seq)
+                ev_seq=ev_seq, ev_graph=ev_graph, graph_same=graph_same)
 
 
 CENTER_BATCH = 224       # one working chunk at box 384
@@ -869,6 +885,8 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"plain_particles_per_s": round(plain, 3),
                   "sequential_particles_per_s": round(n * args.steps / (r["ev_seq"] / 1e3), 3),
+                  "graph_particles_per_s": round(n * args.steps / (r["ev_graph"] / 1e3), 3),
+                  "graph_orientation_identical": r["graph_same"],
                   "chosen_over_plain": round(value / plain, 2),
                   "plain_ms_per_step": round(r["ev_p"], 3),
                   "orientation_identical": r["identical"], "orientation_within_one_step": r["within"],

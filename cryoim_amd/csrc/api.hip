@@ -117,10 +117,13 @@ int cim_init(cim_ctx** out, int device) {
     return CIM_OK;
 }
 
+static void step_release(cim_ctx* ctx);
+
 int cim_finalize(cim_ctx* ctx) {
     if (!ctx) return CIM_E_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    step_release(ctx);
     for (auto& kv : ctx->plans) (void)hipFree(kv.second.tw);
     ctx->rec.acc.release();
     ctx->rec.lo.release();
@@ -295,6 +298,88 @@ static int rec_insert_images_dev(cim_ctx* ctx, const cim_params* p, int n, const
     rc = rec_insert_ft_dev(ctx, nu, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p);
     if (rc) return rc;
     CIM_HIP(hipStreamSynchronize(ctx->stream));
+    return CIM_OK;
+}
+
+// The insert of n device records (every record, CIM_P_USE not read), on the
+// stream with no host synchronisation.
+static int rec_insert_records_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev,
+                                  const float* parts_dev) {
+    int rc = check_params(ctx, p);
+    if (rc) return rc;
+    if (!ctx->rec.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
+    if (p->box != ctx->rec.box) return set_error(ctx, CIM_E_ARG, "box %d differs from cim_rec_begin box %d", p->box,
+                                                 ctx->rec.box);
+    const long long N = (long long)p->box * p->box;
+    CIM_ALLOC(ctx->parts, sizeof(float) * CIM_PART_STRIDE * (size_t)n);
+    CIM_HIP(hipMemcpyAsync(ctx->parts.ptr, parts_dev, sizeof(float) * CIM_PART_STRIDE * (size_t)n,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
+    rc = preprocess_images(ctx, p, n, images_dev, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on, p->mask_on,
+                           p->conj_out);
+    if (rc) return rc;
+    rc = centre_shift(ctx, ctx->ft.as<float2>(), p->box, n, ctx->parts.as<float>(), 1);
+    if (rc) return rc;
+    return rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p);
+}
+
+static void step_release(cim_ctx* ctx) {
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->gexec = nullptr;
+    ctx->graph = nullptr;
+}
+
+int cim_step_capture(cim_ctx* ctx, const cim_params* psearch, const cim_params* pinsert, int n,
+                     const void* images_dev, float* parts_dev) {
+    if (!ctx) return CIM_E_ARG;
+    if (!psearch || !pinsert || !images_dev || !parts_dev || n <= 0)
+        return set_error(ctx, CIM_E_ARG, "cim_step_capture: NULL argument or n <= 0");
+    CIM_HIP(hipSetDevice(ctx->device));
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    step_release(ctx);
+    // timing events are not recorded into the graph
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        ctx->timing = timing;
+        return set_error(ctx, CIM_E_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    int rc = search_records_dev(ctx, psearch, n, (const float*)images_dev, parts_dev);
+    if (!rc) rc = rec_insert_records_dev(ctx, pinsert, n, (const float*)images_dev, parts_dev);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->timing = timing;
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : set_error(ctx, CIM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        ctx->gexec = nullptr;
+        return set_error(ctx, CIM_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    ctx->graph = g;
+    return CIM_OK;
+}
+
+int cim_step_replay(cim_ctx* ctx) {
+    if (!ctx) return CIM_E_ARG;
+    if (!ctx->gexec) return set_error(ctx, CIM_E_STATE, "cim_step_capture has not been called");
+    CIM_HIP(hipSetDevice(ctx->device));
+    CIM_HIP(hipGraphLaunch(ctx->gexec, ctx->stream));
+    CIM_HIP(hipStreamSynchronize(ctx->stream));
+    // the coset insert's cube now holds samples to fold into the volume
+    if (ctx->rec.cosets) ctx->rec.cube_dirty = true;
+    return CIM_OK;
+}
+
+int cim_step_release(cim_ctx* ctx) {
+    if (!ctx) return CIM_E_ARG;
+    CIM_HIP(hipSetDevice(ctx->device));
+    step_release(ctx);
     return CIM_OK;
 }
 

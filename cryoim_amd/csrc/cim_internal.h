@@ -107,6 +107,8 @@ struct SearchState {
     DevBuf tfcut;      // float [T][nr] Fcut fitted without CTF (load_template.f90:44-47)
     std::vector<float> tang;   // T*3 template angles
     SearchBatch wk;
+    long long version = 0;     // template banks set up so far
+    std::string step_sig;      // the parameters wk's constant tables were last set up for
 };
 
 // Resident model transform and per-batch buffers of local_refine.
@@ -177,6 +179,9 @@ struct cim_ctx {
     std::map<std::string, int> tuned;
     // work counters (evaluated samples, inserted samples, ...)
     std::map<std::string, long long> counters;
+    // captured search + insert step (cim_step_capture)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace cim {
@@ -211,6 +216,10 @@ struct TimedLaunch {
 
 // FFT (fft.hip)
 int fft_plan(cim_ctx* ctx, int n, FftPlan** out);
+// Tabulated global search of n device records on the stream, with no host
+// synchronisation (search.hip); needs a host-records call with the same
+// parameters and count first.
+int search_records_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts_dev);
 // In-place transforms of interleaved complex data on ctx->stream.
 int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_out);
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);

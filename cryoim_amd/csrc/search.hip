@@ -472,6 +472,31 @@ __global__ void k_fb_merge(const float* __restrict__ v, const int* __restrict__ 
     }
 }
 
+// Device records in: n records copied into the search's working copy, each
+// masked around its own centre (ort_globesearch.f90:50).
+__global__ void k_records_in(const float* __restrict__ src, int n, float* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * CIM_PART_STRIDE) return;
+    const int k = i % CIM_PART_STRIDE;
+    const int b = i - k;
+    dst[i] = k == CIM_P_MASKX ? src[b + CIM_P_X] : k == CIM_P_MASKY ? src[b + CIM_P_Y] : src[i];
+}
+
+// Device records out: the search's result (first minimum's linear index and
+// PR) as theta, phi, omega and PR of each record.
+__global__ void k_records_out(const float* __restrict__ fv, const int* __restrict__ fi, int n, int plane, int nomega,
+                              int np, float step, float* __restrict__ parts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int idx = fi[i];
+    const int it = idx / plane, ip = (idx % plane) / nomega - np, io = idx % nomega;
+    float* q = parts + (long long)i * CIM_PART_STRIDE;
+    q[CIM_P_THETA] = 69.0f + step * (float)it;
+    q[CIM_P_PHI] = step * (float)ip;
+    q[CIM_P_OMEGA] = step * (float)io;
+    q[CIM_P_PR] = fv[i];
+}
+
 // Fill n 64-bit words (counters) with zero: a kernel, so that the work stays
 // capturable in a graph.
 __global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
@@ -1525,6 +1550,7 @@ static int prep_templates_img(cim_ctx* ctx, const cim_params* p, int T, const fl
     s.minR = p->minR;
     s.maxR = p->maxR;
     s.ready = true;
+    s.version += 1;
     return CIM_OK;
 }
 
@@ -1734,9 +1760,15 @@ int launch_cl_table(cim_ctx* ctx, int nh, const float2* ppol, const float2* tpol
     return CIM_OK;
 }
 
+// parts_dev: device records (n x CIM_PART_STRIDE) read and written on the
+// stream with no host synchronisation, copy or allocation, so that the call
+// can be captured in a graph; only for the tabulated search of one batch,
+// after a host-records call with the same parameters has set the constant
+// tables up.  parts (host records) is then unused.
 static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts,
-                             float* pr_array) {
+                             float* pr_array, float* parts_dev = nullptr) {
     SearchState& s = ctx->srch;
+    const bool devmode = parts_dev != nullptr;
     if (!s.ready) return set_error(ctx, CIM_E_STATE, "cim_search_templates has not been called");
     if (p->box != s.box || p->minR != s.minR || p->maxR != s.maxR)
         return set_error(ctx, CIM_E_ARG, "box/minR/maxR differ from the template bank");
@@ -1774,17 +1806,31 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         for (int i = 0; i < 540; ++i) ops[i] = (float)d[i];
     }
     SearchBatch& sb = ctx->srch.wk;
-    CIM_ALLOC(sb.ops, sizeof(float) * 540);
-    CIM_ALLOC(sb.rows, sizeof(int) * rows.size());
-    CIM_ALLOC(sb.pts, sizeof(float) * g.pts.size());
-    CIM_ALLOC(sb.pbase, sizeof(int) * g.base.size());
-    CIM_ALLOC(sb.tang, sizeof(float) * 3 * T);
-    CIM_HIP(hipMemcpyAsync(sb.ops.ptr, ops.data(), sizeof(float) * 540, hipMemcpyHostToDevice, ctx->stream));
-    CIM_HIP(hipMemcpyAsync(sb.rows.ptr, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, ctx->stream));
-    CIM_HIP(hipMemcpyAsync(sb.pts.ptr, g.pts.data(), sizeof(float) * g.pts.size(), hipMemcpyHostToDevice, ctx->stream));
-    CIM_HIP(hipMemcpyAsync(sb.pbase.ptr, g.base.data(), sizeof(int) * g.base.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
-    CIM_HIP(hipMemcpyAsync(sb.tang.ptr, s.tang.data(), sizeof(float) * 3 * T, hipMemcpyHostToDevice, ctx->stream));
+    // the constant tables depend on these; a device-records call needs them set up already
+    char sig[160];
+    snprintf(sig, sizeof(sig), "%d:%d:%d:%g:%d:%d:%d:%d:%lld", box, p->minR, R, (double)p->searchstep, T,
+             p->search_method, p->search_topk, n, (long long)s.version);
+    if (devmode) {
+        if (ctx->srch.step_sig != sig)
+            return set_error(ctx, CIM_E_STATE, "device-records search: call cim_search_global_dev with host records "
+                                               "and the same parameters and count first");
+        if (pr_array || p->search_method != CIM_SEARCH_TABLES)
+            return set_error(ctx, CIM_E_ARG, "device-records search: tabulated search only");
+    } else {
+        CIM_ALLOC(sb.ops, sizeof(float) * 540);
+        CIM_ALLOC(sb.rows, sizeof(int) * rows.size());
+        CIM_ALLOC(sb.pts, sizeof(float) * g.pts.size());
+        CIM_ALLOC(sb.pbase, sizeof(int) * g.base.size());
+        CIM_ALLOC(sb.tang, sizeof(float) * 3 * T);
+        CIM_HIP(hipMemcpyAsync(sb.ops.ptr, ops.data(), sizeof(float) * 540, hipMemcpyHostToDevice, ctx->stream));
+        CIM_HIP(hipMemcpyAsync(sb.rows.ptr, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        CIM_HIP(hipMemcpyAsync(sb.pts.ptr, g.pts.data(), sizeof(float) * g.pts.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        CIM_HIP(hipMemcpyAsync(sb.pbase.ptr, g.base.data(), sizeof(int) * g.base.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        CIM_HIP(hipMemcpyAsync(sb.tang.ptr, s.tang.data(), sizeof(float) * 3 * T, hipMemcpyHostToDevice, ctx->stream));
+    }
 
     // tabulated search (CIM_SEARCH_TABLES): polar grid of searchstep / sub
     const bool tables = p->search_method == CIM_SEARCH_TABLES && !pr_array;
@@ -1880,20 +1926,33 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         CIM_ALLOC(sb.fwin2, sizeof(int) * (size_t)batch * coarseM * 9);
         CIM_ALLOC(sb.ckwin2, sizeof(int) * (size_t)batch * coarseM);
         CIM_ALLOC(sb.fcnt, sizeof(int) * (size_t)batch);
-        CIM_HIP(hipMemcpyAsync(sb.clist.ptr, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
-        CIM_HIP(hipMemcpyAsync(sb.nbr.ptr, nbrv.data(), sizeof(int) * nbrv.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (!devmode) {
+            CIM_HIP(hipMemcpyAsync(sb.clist.ptr, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            CIM_HIP(hipMemcpyAsync(sb.nbr.ptr, nbrv.data(), sizeof(int) * nbrv.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        }
     }
+    if (devmode && n > batch)
+        return set_error(ctx, CIM_E_ARG, "device-records search: %d particles exceed one batch (%d)", n, batch);
     for (int b0 = 0; b0 < n; b0 += batch) {
         const int nb = std::min(batch, n - b0);
         // records: mask around each particle's own centre (ort_globesearch.f90:50)
-        std::vector<float> pr(parts + (size_t)b0 * CIM_PART_STRIDE, parts + (size_t)(b0 + nb) * CIM_PART_STRIDE);
-        for (int i = 0; i < nb; ++i) {
-            pr[i * CIM_PART_STRIDE + CIM_P_MASKX] = pr[i * CIM_PART_STRIDE + CIM_P_X];
-            pr[i * CIM_PART_STRIDE + CIM_P_MASKY] = pr[i * CIM_PART_STRIDE + CIM_P_Y];
+        int rc = CIM_OK;
+        if (devmode) {
+            CIM_ALLOC(ctx->parts, sizeof(float) * CIM_PART_STRIDE * (size_t)nb);
+            hipLaunchKernelGGL(k_records_in, dim3((nb * CIM_PART_STRIDE + 255) / 256), dim3(256), 0, ctx->stream,
+                               parts_dev + (size_t)b0 * CIM_PART_STRIDE, nb, ctx->parts.as<float>());
+            CIM_HIP(hipGetLastError());
+        } else {
+            std::vector<float> pr(parts + (size_t)b0 * CIM_PART_STRIDE, parts + (size_t)(b0 + nb) * CIM_PART_STRIDE);
+            for (int i = 0; i < nb; ++i) {
+                pr[i * CIM_PART_STRIDE + CIM_P_MASKX] = pr[i * CIM_PART_STRIDE + CIM_P_X];
+                pr[i * CIM_PART_STRIDE + CIM_P_MASKY] = pr[i * CIM_PART_STRIDE + CIM_P_Y];
+            }
+            rc = upload_parts(ctx, nb, pr.data());
+            if (rc) return rc;
         }
-        int rc = upload_parts(ctx, nb, pr.data());
-        if (rc) return rc;
         CIM_ALLOC(ctx->ft, sizeof(float2) * N * nb);
         float2* ft = ctx->ft.as<float2>();
         // the global search masks unconditionally (ort_globesearch.f90:50, unlike mode 5)
@@ -2205,6 +2264,14 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                npts, fblist, fbcnt, sb.fv.as<float>(), sb.fi.as<int>());
             CIM_HIP(hipGetLastError());
         }
+        const int plane = g.No * (2 * g.Np + 1);
+        if (devmode) {
+            hipLaunchKernelGGL(k_records_out, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, sb.fv.as<float>(),
+                               sb.fi.as<int>(), nb, plane, g.No, g.Np, p->searchstep,
+                               parts_dev + (size_t)b0 * CIM_PART_STRIDE);
+            CIM_HIP(hipGetLastError());
+            continue;
+        }
         std::vector<float> hv(nb);
         std::vector<int> hi(nb);
         CIM_HIP(hipMemcpyAsync(hv.data(), sb.fv.ptr, sizeof(float) * nb, hipMemcpyDeviceToHost, ctx->stream));
@@ -2245,7 +2312,6 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             for (int b = 0; b < 5; ++b) ctx->counters[bins[b]] += (long long)hcount[3 + b];
         }
         ctx->counters["cl_search_particles"] += nb;
-        const int plane = g.No * (2 * g.Np + 1);
         for (int i = 0; i < nb; ++i) {
             float* q = parts + (size_t)(b0 + i) * CIM_PART_STRIDE;
             const int idx = hi[i];
@@ -2262,7 +2328,12 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 for (int k = 0; k < g.No; ++k) pr_array[g.base[pt] + k] = full[(size_t)pt * g.No + k];
         }
     }
+    if (!devmode) ctx->srch.step_sig = sig;
     return CIM_OK;
+}
+
+int search_records_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts_dev) {
+    return search_global_dev(ctx, p, n, images_dev, nullptr, nullptr, parts_dev);
 }
 
 // ---- local refinement (mode 5) ------------------------------------------------

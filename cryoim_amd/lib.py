@@ -137,6 +137,9 @@ def load() -> ctypes.CDLL:
         "cim_search_templates": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global": (ip, [vp, P, ip, fp, fp]),
         "cim_search_global_dev": (ip, [vp, P, ip, vp, fp]),
+        "cim_step_capture": (ip, [vp, P, P, ip, vp, vp]),
+        "cim_step_replay": (ip, [vp]),
+        "cim_step_release": (ip, [vp]),
         "cim_search_pr_array": (ip, [vp, P, fp, fp, fp]),
         "cim_refine_local": (ip, [vp, P, ip, fp, fp]),
         "cim_sym_ops": (ip, [ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int)]),
@@ -453,6 +456,21 @@ class Context:
             rc = self.lib.cim_search_global(self.h, ctypes.byref(params), n, _fptr(im), _fptr(parts))
         self._check(rc, "cim_search_global")
         return parts
+
+    def step_capture(self, psearch: CimParams, pinsert: CimParams, images, parts_dev):
+        """cim_step_capture: search + insert of the device records parts_dev
+        (torch float32 (n, PART_STRIDE) on the GPU) and device images,
+        captured as one HIP graph."""
+        n = parts_dev.shape[0]
+        self._check(self.lib.cim_step_capture(self.h, ctypes.byref(psearch), ctypes.byref(pinsert), n,
+                                              ctypes.c_void_p(images.data_ptr()),
+                                              ctypes.c_void_p(parts_dev.data_ptr())), "cim_step_capture")
+
+    def step_replay(self):
+        self._check(self.lib.cim_step_replay(self.h), "cim_step_replay")
+
+    def step_release(self):
+        self._check(self.lib.cim_step_release(self.h), "cim_step_release")
 
     def search_pr_array(self, params: CimParams, image, part, shape) -> np.ndarray:
         im = _f32(image)

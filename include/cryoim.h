@@ -302,6 +302,24 @@ int cim_project_templates(cim_ctx* ctx, int box, int T, const float* model, cons
 int cim_search_global(cim_ctx* ctx, const cim_params* p, int n, const float* images, float* parts);
 int cim_search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const void* images_dev,
                           float* parts);
+/* One search + insert step captured as a HIP graph and replayed (no
+ * counterpart in the reference, whose loops are host programs; for callers
+ * that run the same batch shape step after step and want the step as one
+ * launch with no host work inside it).  n device records parts_dev
+ * (n x CIM_PART_STRIDE floats) and n box^2 device images: the step runs the
+ * tabulated global search (psearch, CIM_SEARCH_TABLES, one batch) with its
+ * fallback on the device, writes theta, phi, omega and PR into parts_dev,
+ * then inserts every record (CIM_P_USE is not read) with pinsert into the
+ * volume of cim_rec_begin.  Needs, before capture: cim_search_templates, one
+ * cim_search_global / _dev call with the same psearch and n (it sets up the
+ * search's constant tables) and cim_rec_begin.  cim_step_replay launches the
+ * graph and waits for it; refill images_dev and parts_dev between replays.
+ * The graph holds the context's buffers: release it before calls that use
+ * other sizes on the context.  Replays do not update the work counters. */
+int cim_step_capture(cim_ctx* ctx, const cim_params* psearch, const cim_params* pinsert, int n,
+                     const void* images_dev, float* parts_dev);
+int cim_step_replay(cim_ctx* ctx);
+int cim_step_release(cim_ctx* ctx);
 /* Full phase-residual map of one particle (Nomega x (2Nphi+1) x (Ntheta+1),
  * omega fastest, 1000 outside the AU) -- the PR_array of ort_globesearch. */
 int cim_search_pr_array(cim_ctx* ctx, const cim_params* p, const float* image, const float* part,
