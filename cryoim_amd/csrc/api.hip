@@ -124,6 +124,7 @@ int cim_finalize(cim_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     step_release(ctx);
+    ctx->cnn.release();
     for (auto& kv : ctx->plans) (void)hipFree(kv.second.tw);
     ctx->rec.acc.release();
     ctx->rec.lo.release();
@@ -363,6 +364,15 @@ int cim_step_capture(cim_ctx* ctx, const cim_params* psearch, const cim_params* 
     }
     ctx->graph = g;
     return CIM_OK;
+}
+
+int cim_search_candidates_dev(cim_ctx* ctx, const cim_params* p, int n, const void* images_dev, float* parts,
+                              int ncand, const int* cand, const int* omega) {
+    if (!ctx) return CIM_E_ARG;
+    if (!images_dev || !parts || !cand || !omega || ncand < 1 || ncand > 256)
+        return set_error(ctx, CIM_E_ARG, "cim_search_candidates_dev: NULL array or ncand %d outside 1..256", ncand);
+    CIM_HIP(hipSetDevice(ctx->device));
+    return search_candidates_dev(ctx, p, n, (const float*)images_dev, parts, ncand, cand, omega);
 }
 
 int cim_step_replay(cim_ctx* ctx) {

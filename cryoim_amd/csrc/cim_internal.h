@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <map>
@@ -134,6 +135,16 @@ struct CenterState {
     }
 };
 
+// Proposal network (cnn.hip): parameters, the activations of the last
+// forward pass (kept for the backward pass), patches, gradients.
+struct CnnState {
+    int batch = 0;
+    DevBuf params, act, cols, grads, dy, dx;
+    void release() {
+        for (DevBuf* b : {&params, &act, &cols, &grads, &dy, &dx}) b->release();
+    }
+};
+
 // Device buffers of local_box: angles and shifts, operators, per-(particle,
 // operator) records, row-shifted complex images, shifted real images.
 struct LboxState {
@@ -160,6 +171,7 @@ struct cim_ctx {
     cim::LrefState lref;
     cim::CenterState cen;
     cim::LboxState lbox;
+    cim::CnnState cnn;
     // scratch buffers reused across calls
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
     // half-map FSC: input tnf pairs, normalised copies, merged sums, shell sums, weights
@@ -220,6 +232,9 @@ int fft_plan(cim_ctx* ctx, int n, FftPlan** out);
 // synchronisation (search.hip); needs a host-records call with the same
 // parameters and count first.
 int search_records_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts_dev);
+// The exact score of given (AU point, omega window) candidates per particle (search.hip).
+int search_candidates_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts,
+                          int ncand, const int* cand, const int* omega);
 // In-place transforms of interleaved complex data on ctx->stream.
 int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_out);
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);

@@ -1765,8 +1765,12 @@ int launch_cl_table(cim_ctx* ctx, int nh, const float2* ppol, const float2* tpol
 // can be captured in a graph; only for the tabulated search of one batch,
 // after a host-records call with the same parameters has set the constant
 // tables up.  parts (host records) is then unused.
+// ucand, uomega (host, n x ucn): score only these AU points, each over a
+// window of omega columns centred on the given column (the tables are not
+// built and nothing falls back).
 static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts,
-                             float* pr_array, float* parts_dev = nullptr) {
+                             float* pr_array, float* parts_dev = nullptr, const int* ucand = nullptr,
+                             const int* uomega = nullptr, int ucn = 0) {
     SearchState& s = ctx->srch;
     const bool devmode = parts_dev != nullptr;
     if (!s.ready) return set_error(ctx, CIM_E_STATE, "cim_search_templates has not been called");
@@ -1833,7 +1837,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
     }
 
     // tabulated search (CIM_SEARCH_TABLES): polar grid of searchstep / sub
-    const bool tables = p->search_method == CIM_SEARCH_TABLES && !pr_array;
+    const bool tables = p->search_method == CIM_SEARCH_TABLES && !pr_array && !ucand;
     if (p->search_method != CIM_SEARCH_PLAIN && p->search_method != CIM_SEARCH_TABLES)
         return set_error(ctx, CIM_E_ARG, "search_method %d unknown", p->search_method);
     int sub = 1;
@@ -2195,8 +2199,34 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             }
             nscore = topk * (a.kwin && rescore_cpw == 1 ? a.wsplit : 1);
         }
+        if (ucand) {
+            // the caller's candidates: AU point, and the omega column its window is centred on
+            std::vector<int> hc((size_t)nb * ucn), hw((size_t)nb * ucn);
+            for (size_t i = 0; i < hc.size(); ++i) {
+                const int q = ucand[(size_t)b0 * ucn + i], kc = uomega[(size_t)b0 * ucn + i];
+                if (q < 0 || q >= npts || kc < 0 || kc >= g.No)
+                    return set_error(ctx, CIM_E_ARG, "candidate %zu: AU point %d (of %d) or omega %d (of %d)", i, q,
+                                     npts, kc, g.No);
+                hc[i] = q;
+                hw[i] = ((kc % ncolx - wincols / 2) % ncolx + ncolx) % ncolx;
+            }
+            CIM_ALLOC(sb.cand, sizeof(int) * hc.size());
+            CIM_ALLOC(sb.kwin, sizeof(int) * hw.size());
+            CIM_HIP(hipMemcpyAsync(sb.cand.ptr, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            CIM_HIP(hipMemcpyAsync(sb.kwin.ptr, hw.data(), sizeof(int) * hw.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            a.cand = sb.cand.as<int>();
+            a.ncand = ucn;
+            a.prfull = nullptr;
+            a.kwin = sb.kwin.as<int>();
+            a.wincols = wincols;
+            a.wsplit = 1;
+            a.chunk = std::min(chunk, 6);
+            nscore = ucn;
+        }
         {
-            TimedLaunch tl(ctx, tables ? "cl_rescore" : "cl_search");
+            TimedLaunch tl(ctx, tables || ucand ? "cl_rescore" : "cl_search");
             // windows: one wave of single-column lanes in the same kernel variant
             if (a.kwin && rescore_cpw > 1) {
                 a.chunk = std::min(chunk, 3);
@@ -2206,7 +2236,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                      sizeof(float) * (2 * 64 * rescore_cpw + nr) +
                                      sizeof(int) * (size_t)rescore_cpw * a.chunk +
                                      sizeof(unsigned short) * (size_t)rescore_cpw * a.chunk * nr);
-                const dim3 grid((unsigned)(((topk + rescore_cpw - 1) / rescore_cpw) * 8LL * ((nb + 7) / 8)));
+                const dim3 grid((unsigned)(((a.ncand + rescore_cpw - 1) / rescore_cpw) * 8LL * ((nb + 7) / 8)));
                 a.nbatch = nb;
 #define CIM_RESCORE(C, P) hipLaunchKernelGGL((k_cl_rescore<C, P>), grid, dim3(64), lw, ctx->stream, a)
                 if (rescore_cpw == 2) { if (sh.pair) CIM_RESCORE(2, true); else CIM_RESCORE(2, false); }
@@ -2328,12 +2358,18 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 for (int k = 0; k < g.No; ++k) pr_array[g.base[pt] + k] = full[(size_t)pt * g.No + k];
         }
     }
-    if (!devmode) ctx->srch.step_sig = sig;
+    // the device-records search reuses the constant tables of the last tabulated host-records call
+    if (!devmode) ctx->srch.step_sig = tables ? std::string(sig) : std::string();
     return CIM_OK;
 }
 
 int search_records_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts_dev) {
     return search_global_dev(ctx, p, n, images_dev, nullptr, nullptr, parts_dev);
+}
+
+int search_candidates_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, float* parts,
+                          int ncand, const int* cand, const int* omega) {
+    return search_global_dev(ctx, p, n, images_dev, parts, nullptr, nullptr, cand, omega, ncand);
 }
 
 // ---- local refinement (mode 5) ------------------------------------------------

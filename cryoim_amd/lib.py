@@ -17,6 +17,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cryoim.h")
 
 PART_STRIDE = 12
 P_Z = 11          # local_refine records: local_z0 in, chosen z out (cim_lref_defocus)
+CNN_HYP, CNN_OUT = 4, 28   # proposal network: hypotheses, outputs (7 per hypothesis)
 SEARCH_PLAIN, SEARCH_TABLES = 0, 1
 CENTER_MODEL, CENTER_SELFROT180 = 0, 1
 ICOSAVG_FULL, ICOSAVG_MERGE = 0, 1
@@ -159,6 +160,12 @@ def load() -> ctypes.CDLL:
         "cim_center_dev": (ip, [vp, P, ip, ip, vp, fp, fp]),
         "cim_lref_refine": (ip, [vp, P, ip, fp, ip, fp, ctypes.c_char_p, ctypes.c_float, ctypes.c_float]),
         "cim_lref_defocus": (ip, [vp, P, ip, fp, ip, fp, ctypes.c_float, ctypes.c_float]),
+        "cim_search_candidates_dev": (ip, [vp, P, ip, vp, fp, ip, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]),
+        "cim_cnn_nparams": (ip, []),
+        "cim_debug_cnn_act": (ip, [vp, ip, fp]),
+        "cim_cnn_forward": (ip, [vp, ip, vp, fp, vp]),
+        "cim_cnn_backward": (ip, [vp, vp, fp]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):
@@ -185,7 +192,10 @@ def _f32(a) -> np.ndarray:
 
 
 class Context:
-    """One cim_ctx on one device; all calls are synchronous."""
+    """One cim_ctx on one device; all calls are synchronous.  A process that
+    also uses torch on the GPU initialises torch's device first
+    (torch.cuda.init()): torch carries its own copy of the HIP runtime, and
+    the second runtime to reach the GPU finds none."""
 
     def __init__(self, device: int = 0):
         self.lib = load()
@@ -456,6 +466,43 @@ class Context:
             rc = self.lib.cim_search_global(self.h, ctypes.byref(params), n, _fptr(im), _fptr(parts))
         self._check(rc, "cim_search_global")
         return parts
+
+    # -- learned proposals -------------------------------------------------------
+    def search_candidates(self, params: CimParams, images, parts: np.ndarray, cand: np.ndarray,
+                          omega: np.ndarray) -> np.ndarray:
+        """cim_search_candidates_dev: exact scores of the (n, K) candidate AU
+        points, each over an omega window centred on omega[i, k]."""
+        parts = _f32(parts).reshape(-1, PART_STRIDE).copy()
+        n = parts.shape[0]
+        c = np.ascontiguousarray(cand, np.int32).reshape(n, -1)
+        o = np.ascontiguousarray(omega, np.int32).reshape(n, -1)
+        ip_ = ctypes.POINTER(ctypes.c_int)
+        self._check(self.lib.cim_search_candidates_dev(self.h, ctypes.byref(params), n,
+                                                       ctypes.c_void_p(images.data_ptr()), _fptr(parts), c.shape[1],
+                                                       c.ctypes.data_as(ip_), o.ctypes.data_as(ip_)),
+                    "cim_search_candidates_dev")
+        return parts
+
+    def cnn_forward(self, images, params: np.ndarray, out):
+        """cim_cnn_forward: images (B, 128, 128) float32 and out (B, CNN_OUT)
+        float32, both device tensors; params host float32."""
+        prm = np.ascontiguousarray(params, np.float32)
+        self._check(self.lib.cim_cnn_forward(self.h, int(images.shape[0]), ctypes.c_void_p(images.data_ptr()),
+                                             _fptr(prm), ctypes.c_void_p(out.data_ptr())), "cim_cnn_forward")
+
+    def debug_cnn_act(self, layer: int, batch: int) -> np.ndarray:
+        """cim_debug_cnn_act (include/cryoim_debug.h): the kept activations of layer."""
+        h = 128 >> layer
+        out = np.empty((batch, h, h, 1 if layer == 0 else 64), np.float32)
+        self._check(self.lib.cim_debug_cnn_act(self.h, int(layer), _fptr(out)), "cim_debug_cnn_act")
+        return out
+
+    def cnn_backward(self, dout) -> np.ndarray:
+        """cim_cnn_backward: dout (B, CNN_OUT) device tensor; returns the
+        parameter gradient (float32, host)."""
+        g = np.empty(self.lib.cim_cnn_nparams(), np.float32)
+        self._check(self.lib.cim_cnn_backward(self.h, ctypes.c_void_p(dout.data_ptr()), _fptr(g)), "cim_cnn_backward")
+        return g
 
     def step_capture(self, psearch: CimParams, pinsert: CimParams, images, parts_dev):
         """cim_step_capture: search + insert of the device records parts_dev

@@ -320,6 +320,35 @@ int cim_step_capture(cim_ctx* ctx, const cim_params* psearch, const cim_params* 
                      const void* images_dev, float* parts_dev);
 int cim_step_replay(cim_ctx* ctx);
 int cim_step_release(cim_ctx* ctx);
+/* ---- learned proposals (no counterpart in the reference) --------------
+ * The proposal network of the learned search: six 3x3 convolutions of 64
+ * channels, stride 2, ReLU (128^2 -> 2^2), then a linear head of
+ * CIM_CNN_OUT outputs: per hypothesis h < CIM_CNN_HYP, outputs 7h .. 7h+5
+ * are the first two columns of a rotation (column-major) and 7h+6 its
+ * confidence logit.  Products on the matrix cores in bfloat16 with float32
+ * sums.  params: cim_cnn_nparams() floats, per layer W [ky][kx][cin][cout]
+ * then the 64 biases, then the head W [256][CIM_CNN_OUT] (features in
+ * (y, x, channel) order) and its biases.  cim_cnn_forward takes B device
+ * images of 128 x 128 float32 and writes B x CIM_CNN_OUT device outputs,
+ * keeping the activations for cim_cnn_backward, which takes the device
+ * gradient of the loss with respect to those outputs and writes the
+ * parameter gradient to the host. */
+/* The exact score (the plain search's arithmetic, bit for bit) of ncand
+ * given candidates per particle: cand (n x ncand) AU point indices in the
+ * search grid's order (theta index outer; per phi index ip >= 0 the point
+ * at +ip, then at -ip when ip > 0) and omega (n x ncand) the omega column
+ * (0 .. Nomega-1) each window of 16 columns (and their Friedel columns) is
+ * centred on.  parts in/out as cim_search_global: the first minimum over
+ * the candidates.  For searches whose candidates come from elsewhere
+ * (learned proposals); ncand <= 256. */
+int cim_search_candidates_dev(cim_ctx* ctx, const cim_params* p, int n, const void* images_dev, float* parts,
+                              int ncand, const int* cand, const int* omega);
+#define CIM_CNN_HYP 4
+#define CIM_CNN_OUT 28
+int cim_cnn_nparams(void);
+int cim_cnn_forward(cim_ctx* ctx, int B, const void* images_dev, const float* params, void* out_dev);
+int cim_cnn_backward(cim_ctx* ctx, const void* dout_dev, float* grads);
+
 /* Full phase-residual map of one particle (Nomega x (2Nphi+1) x (Ntheta+1),
  * omega fastest, 1000 outside the AU) -- the PR_array of ort_globesearch. */
 int cim_search_pr_array(cim_ctx* ctx, const cim_params* p, const float* image, const float* part,

@@ -24,6 +24,11 @@ extern "C" {
 int cim_debug_cl_table(cim_ctx* ctx, int nh, int np, int T, int tpol_per_particle, int nr, int npsi, const float* ppol,
                        const float* tpol, unsigned int* table);
 
+/* The activations the last cim_cnn_forward kept: layer 0 the B input images
+ * (128 x 128), layer l = 1..6 the output of convolution l (NHWC, B x
+ * (128 >> l)^2 x 64 floats). */
+int cim_debug_cnn_act(cim_ctx* ctx, int layer, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,13 @@ def prog():
 @pytest.fixture(scope="session")
 def ctx():
     from cryoim_amd import lib
+    # torch carries its own copy of the HIP runtime: it must reach the GPU
+    # before the library's runtime does, or it finds none
+    try:
+        import torch
+        torch.cuda.init()
+    except Exception:
+        pass
     c = lib.Context(0)
     yield c
     c.close()
