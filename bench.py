@@ -774,7 +774,7 @@ def _roofline(r, args):
         passed = cnt["cl_rescore_pass"] / 2 / launches
         flops = passed * FLOPS_PAIR_CALL_PASS + (calls - passed) * FLOPS_PAIR_CALL_GATED
         achieved = flops / sec / 1e12
-        return dom, {"bound": "mfma", "kernel": "k_cl_rescore<4,true> (exact phase residual of the top-32 AU "
+        return dom, {"bound": "valu", "kernel": "k_cl_rescore<4,true> (exact phase residual of the top-32 AU "
                                                 "points x 16 omega columns and their Friedel pairs, FP32)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), **base,
@@ -782,9 +782,9 @@ def _roofline(r, args):
                      "flops_rule": f"{FLOPS_PAIR_CALL_PASS} FP32 operations per paired sample call that passes the "
                                    f"particle amplitude gate, {FLOPS_PAIR_CALL_GATED} per call that fails it "
                                    f"(library counters: {int(calls)} calls, {passed / calls:.3f} passing, per launch)",
-                     "note": "FP32 vector arithmetic: the peak is MI355X's published FP32 figure (vector and "
-                             "matrix alike, counting packed FP32); this build compiles without packed FP32, so its "
-                             "issue limit is half the peak"}
+                     "note": "FP32 vector arithmetic (no matrix instructions): the peak is MI355X's published FP32 "
+                             "figure (vector and matrix alike, counting packed FP32); this build compiles without "
+                             "packed FP32, so its issue limit is half the peak"}
     if dom == "cl_table":
         flops = cnt["cl_table_pairs"] / launches * TABLE_FEATURES_PER_RADIUS
         achieved = flops / sec / 1e12
