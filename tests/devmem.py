@@ -1,7 +1,8 @@
 """Device arrays for the *_dev entry points, allocated with hipMalloc through
-ctypes (in-process; torch is not initialised next to the library here)."""
+ctypes.  The HIP functions are looked up through the library's own handle
+(its dependency tree holds the HIP runtime it runs on), so the arrays come
+from that runtime and no second copy of it is opened next to torch's."""
 import ctypes
-import os
 
 import numpy as np
 
@@ -11,8 +12,8 @@ _HIP = None
 def _hip():
     global _HIP
     if _HIP is None:
-        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-        h = ctypes.CDLL(os.path.join(rocm, "lib", "libamdhip64.so"))
+        from cryoim_amd import lib
+        h = lib.load()
         h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         h.hipFree.argtypes = [ctypes.c_void_p]
         h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
