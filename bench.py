@@ -69,7 +69,7 @@ FLOPS_PAIR_CALL_GATED = 30
 # (2 for D, 4 NH = 16 for N at NH = 4), 2 flops per multiply-add
 TABLE_FEATURES_PER_RADIUS = 2 + 4 * 4
 # where each particle's exact minimum ranked among its table candidates (counted by the library)
-RANK_KEYS = ("cl_rank_lt4", "cl_rank_lt8", "cl_rank_lt16", "cl_rank_lt24", "cl_rank_ge24")
+RANK_KEYS = ("cl_rank_lt4", "cl_rank_lt8", "cl_rank_lt16", "cl_rank_lt24", "cl_rank_ge24", "cl_rank_second")
 METRIC = "particles/s search+insert, box 384: chosen vs plain FP32 GPU at FSC>=0.999"
 
 
@@ -84,12 +84,14 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="time the chosen path step after step (search, then insert) on one stream")
-    ap.add_argument("--stage", choices=("main", "center", "lbox", "missrate"), default="main",
+    ap.add_argument("--stage", choices=("main", "center", "lbox", "missrate", "c1"), default="main",
                     help="center: only the centre finding of icosprocess mode 3 at box 384; "
+                         "c1: the box-128 workload (1 deg grid, 2,000 particles per step), chosen vs plain; "
                          "lbox: only local_box's stack mode (12 vertex boxes of 96 pixels per particle); "
                          "missrate: chosen vs plain search on --particles distinct particles")
     ap.add_argument("--particles", type=int, default=5120,
-                    help="missrate: distinct particles (generated on the GPU in batches of --batch)")
+                    help="missrate: distinct particles (generated on the GPU in batches of --batch); "
+                         "c1: particles per step (default 2000)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--chosen-only", action="store_true",
@@ -448,6 +450,164 @@ def center_main(args):
                          "launches": fft["launches"]},
             "cpu_baseline": cpu,
             "extra": r}
+    print(json.dumps(line))
+
+
+C1_N, C1_STEP, C1_PORT_STEP, C1_SECOND = 2000, 1.0, 2.0, 8
+C1_METRIC = ("particles/s search+insert, box 128 (20-blob phantom, 1 deg AU grid x 360 omega, maxR 24, T=4; "
+             "60-fold insert at maxR 62): chosen vs plain FP32 GPU")
+
+
+def _port_c1_job(i):
+    """Particle i of the box-128 workload through the C oracle (FP32):
+    ort_globesearch on the C1_PORT_STEP grid, then the 60-fold insert at
+    the workload's insert radius."""
+    O, pipelines = _oracle32()
+    w = _PORT_STATE["w"]
+    d, box, apix, ps = w["d"], w["box"], w["apix"], w["ps"]
+    t0 = time.perf_counter()
+    ta, tb = pipelines.templates(O, w["timgs"], np.zeros((len(w["tang"]), 2)), d["mask"])
+    ctf = O.ctf2d(box, ps.df[i, 0], ps.df[i, 1], 0.0, apix)
+    a, b, tf = pipelines.template_inputs(O, ta, tb, ctf, d["minR"], d["maxR"])
+    _, (pa, pb), fc0 = pipelines.particle_inputs(O, ps.images[i], *ps.shifts[i], d["mask"], d["minR"], d["maxR"])
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o4 = O.globesearch(pa, pb, fc0, a, b, tf, w["tang"], d["minR"], d["maxR"], C1_PORT_STEP)
+    t_search = time.perf_counter() - t0
+    maxR = w["insert_maxR"]
+    vol = O.new_volume(box, maxR, compact=True, wg_init=0.0)
+    t0 = time.perf_counter()
+    fa, fb = O.particle_ft(ps.images[i], *ps.shifts[i], mask_r=d["mask"], norm=True)
+    O.insert(vol, fa, fb, ctf, o4[:3], maxR, rcut=maxR - 2, compact=True)
+    return t_prep, t_search, time.perf_counter() - t0
+
+
+def c1_cpu_baseline(w):
+    """The C oracle port (FP32) on the host cores, one particle per process:
+    transforms, the search's scoring loop on the C1_PORT_STEP grid scaled to
+    the C1_STEP grid by the ratio of scored AU-cell x omega points, and the
+    insert.  The reference's own rate is quoted from profiles/."""
+    ncores = min(8, os.cpu_count() or 1)
+    _PORT_STATE.update(w=w)
+    res, wall = _port_pool(_port_c1_job, list(range(ncores)), ncores)
+    _PORT_STATE.clear()
+    scale = _grid_work(C1_STEP) / _grid_work(C1_PORT_STEP)
+    per = float(np.mean([tp + ts * scale + ti for tp, ts, ti in res]))
+    m = np.mean(res, axis=0)
+    return {"value": ncores / per, "unit": "particles/s", "cores": ncores, "kind": "port",
+            "sample": (f"C oracle (FP32) in {ncores} processes, one particle each: transforms (mean {m[0]:.2f} s) "
+                       f"+ the search's scoring on the {C1_PORT_STEP} deg grid (mean {m[1]:.2f} s, x{scale:.1f} to "
+                       f"the {C1_STEP} deg grid by AU-cell x omega count) + 60-fold insert at maxR "
+                       f"{w['insert_maxR']} (mean {m[2]:.3f} s); {wall:.1f} s wall"),
+            "reference_here": _quoted_reference("c1")}
+
+
+def c1_main(args):
+    """The box-128 workload (the 20-blob phantom at 128^3, C1_N particles at
+    uniform AU orientations, +-3 px shifts, defocus U(1, 3) um, SNR 0.1):
+    one step is the whole particle set through the global search on the
+    1 deg grid and the 60-fold insert at Nyquist-2, chosen path against
+    plain path, and the map of each."""
+    import torch
+    from cryoim_amd import lib
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pipelines
+    n = args.particles if args.particles != 5120 else C1_N
+    w = pipelines.config1_workload(n)
+    cpu = None if args.no_cpu_baseline else c1_cpu_baseline(w)          # host cores, before any GPU use
+    box, apix, d, ps = w["box"], w["apix"], w["d"], w["ps"]
+    maxR = w["insert_maxR"]
+    ctx = lib.Context(0)
+    common = dict(apply_ctf=True, mask_on=True, apix=apix, imgmask=d["mask"], searchstep=C1_STEP)
+    p_plain = lib.make_params(box, d["minR"], d["maxR"], search_method=lib.SEARCH_PLAIN, **common)
+    # at maxR 24 the table ranking is coarser: 8 second windows per particle (measured on the
+    # 2,000 particles: 1,987 identical to the plain search without them, 1,995 with them)
+    p_chosen = lib.make_params(box, d["minR"], d["maxR"], search_method=lib.SEARCH_TABLES, search_topk=args.topk,
+                               search_second=C1_SECOND, **common)
+    p_insert = lib.make_params(box, 0, maxR, apply_ctf=True, mask_on=True, norm_on=True, apix=apix,
+                               imgmask=d["mask"])
+    T = len(w["tang"])
+    ctx.search_templates(p_chosen, w["timgs"], np.c_[w["tang"], np.zeros((T, 2))])
+    images = torch.from_numpy(ps.images).to("cuda:0")
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 3:5] = ps.shifts
+    P[:, 6:8] = ps.df
+    P[:, 9:11] = ps.shifts[0]          # reconstruct.f90:92 masks around the first particle
+    P[:, 11] = 1.0
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    def run(psearch, layout, steps, warmup):
+        ctx.rec_begin(box, maxR, layout=layout, rcut=maxR, wg_init=0.01)
+        out = None
+        for k in range(warmup + steps):
+            if k == warmup:
+                ctx.sync()
+                ctx.timing(True)
+                ctx.counter_reset()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record(stream)
+            out = ctx.search_global(psearch, images, P)
+            Q = P.copy()
+            Q[:, 0:3] = out[:, 0:3]
+            ctx.rec_insert(p_insert, images, Q)
+        e1.record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        kt = {lab: ctx.timing_get(lab) for lab in ("cl_search", "cl_polar", "cl_table", "cl_coarse_table",
+                                                   "cl_gather", "cl_rescore", "cl_fallback", "insert")}
+        cnt = {k: ctx.counter(k) for k in ("cl_fallback_particles", "cl_rescore_samples", "cl_rescore_pass",
+                                           "cl_table_pairs", "insert_samples", *RANK_KEYS)}
+        ctx.timing(False)
+        return out, e0.elapsed_time(e1), kt, cnt
+
+    def one_map(out, layout):
+        ctx.rec_begin(box, maxR, layout=layout, rcut=maxR, wg_init=0.01)
+        Q = P.copy()
+        Q[:, 0:3] = out[:, 0:3]
+        ctx.rec_insert(p_insert, images, Q)
+        m = torch.empty((box, box, box), dtype=torch.float32, device="cuda:0")
+        ctx.rec_finish(box, m)
+        ctx.sync()
+        return m
+
+    out_c, ms_c, kt_c, cnt_c = run(p_chosen, COMPENSATED, args.steps, args.warmup)
+    out_p, ms_p, kt_p, _ = run(p_plain, 0, 1, 1)
+    map_c, map_p = one_map(out_c, COMPENSATED), one_map(out_p, 0)
+    ctx.close()
+    fsc = _fsc(map_c, map_p, maxR)
+    rel = float((torch.linalg.vector_norm(map_c - map_p) / torch.linalg.vector_norm(map_p)).item())
+    from cryoim_amd import geometry
+    eq = np.all(out_c[:, [0, 1, 2, 5]] == out_p[:, [0, 1, 2, 5]], axis=1)
+    same = int(eq.sum())
+    dang = np.abs(((out_c[:, 0:3] - out_p[:, 0:3] + 180) % 360) - 180).max(axis=1)
+    sym = geometry.sym_distance_deg(out_c[:, 0:3], out_p[:, 0:3])
+    differing = [{"index": int(i), "true": [round(float(v), 3) for v in ps.angles[i]],
+                  "chosen": [round(float(v), 4) for v in out_c[i, [0, 1, 2, 5]]],
+                  "plain": [round(float(v), 4) for v in out_p[i, [0, 1, 2, 5]]],
+                  "sym_distance_deg": round(float(sym[i]), 3)} for i in np.nonzero(~eq)[0]]
+    value = n * args.steps / (ms_c / 1e3)
+    plain = n / (ms_p / 1e3)
+    kt = {k: {"avg_ms": round(v[0], 3), "launches": v[1]} for k, v in kt_c.items() if v[1]}
+    line = {"metric": C1_METRIC, "value": round(value, 1), "unit": "particles/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_c / args.steps, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded 20-blob 60-fold phantom at 128^3, projections with CTF, SNR 0.1)",
+            "config": {"workload": f"{n} particles of box {box} per step: global common-lines search (1 deg AU grid x "
+                                   f"360 omega, maxR {d['maxR']}, T={T}; chosen: tabulated ranking, top-{args.topk} "
+                                   f"+ {C1_SECOND} second windows exact re-score) + 60-fold insert at maxR {maxR} "
+                                   f"(chosen: compensated FP32)",
+                       "box": box, "batch": n, "templates": T, "search_maxR": d["maxR"], "insert_maxR": maxR,
+                       "searchstep": C1_STEP, "apix": apix, "topk": args.topk, "second_windows": C1_SECOND},
+            "cpu_baseline": cpu,
+            "extra": {"plain_particles_per_s": round(plain, 1), "chosen_over_plain": round(value / plain, 2),
+                      "orientation_identical": same, "orientation_within_one_step": int((dang <= C1_STEP + 1e-4).sum()),
+                      "particles_compared": n, "fallback_particles": int(cnt_c["cl_fallback_particles"]),
+                      "fsc_min_to_nyquist_minus_2": round(float(np.min(fsc[1:])), 6), "map_rel_l2_chosen_vs_plain": rel,
+                      "orientation_within_one_step_by_view": int((sym <= C1_STEP + 1e-4).sum()),
+                      "exact_minimum_rank": {k[8:]: int(cnt_c[k]) for k in RANK_KEYS},
+                      "chosen_kernels": kt, "plain_search_avg_ms": round(kt_p["cl_search"][0], 3),
+                      "differing": differing[:40]}}
     print(json.dumps(line))
 
 
@@ -823,6 +983,9 @@ def main():
         return
     if args.stage == "missrate":
         missrate_main(args)
+        return
+    if args.stage == "c1":
+        c1_main(args)
         return
     t_setup = time.perf_counter()
     ph, tang, timgs, ps = make_workload(args.batch, args.templates)

@@ -89,13 +89,13 @@ struct SearchBatch {
     // coarse E4M3 tables and the D bounds of the particles and templates they are scaled by
     DevBuf ctab, bound;
     // fine stage: the two omega windows of each listed point, the coarse second minima and their windows
-    DevBuf fwin, fwin2, outi2, ckwin2;
+    DevBuf fwin, fwin2, outi2, ckwin2, outv2;
     // exhaustive fallback: count and list of the particles it scores
     DevBuf fbl;
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
                           &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci, &clist, &nbr, &ccand, &ckwin,
-                          &flist, &fcnt, &lines, &lcount, &ctab, &bound, &fwin, &fwin2, &outi2, &ckwin2, &fbl})
+                          &flist, &fcnt, &lines, &lcount, &ctab, &bound, &fwin, &fwin2, &outi2, &ckwin2, &outv2, &fbl})
             b->release();
     }
 };

@@ -419,25 +419,36 @@ __global__ void __launch_bounds__(1024) k_cl_search(SearchArgs a) {
 // columns of its window's edge.  A particle whose minimum ranks fb_rank or
 // deeper, or lies at an edge, is appended to list (count in *cnt, order
 // free: each listed particle's result goes back to its own index).
+//
+// The candidates are ncand per particle: the topk AU points of the table
+// ranking, then (ncand > topk) second windows -- AU points whose table score
+// has another minimum away from the first, re-scored around it.  The exact
+// minimum belongs to the first candidate of its AU point whose window holds
+// its column; a minimum found in a second window is counted in rank[5] and
+// sends the particle to the fallback only from its window's edge.
 __global__ void k_fb_select(const int* __restrict__ fi, const int* __restrict__ cand, const int* __restrict__ kwin,
-                            const int* __restrict__ pbase, int nb, int topk, int nomega, int ncol, int wincols,
-                            int fb_rank, unsigned long long* __restrict__ rank, int* __restrict__ list,
+                            const int* __restrict__ pbase, int nb, int topk, int ncand, int nomega, int ncol,
+                            int wincols, int fb_rank, unsigned long long* __restrict__ rank, int* __restrict__ list,
                             int* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb) return;
     const int li = fi[i];
     const int base = li - li % nomega;
-    int r = topk;
-    for (int c = 0; c < topk; ++c)
-        if (pbase[cand[(long long)i * topk + c]] == base) { r = c; break; }
-    atomicAdd(&rank[r < 4 ? 0 : r < 8 ? 1 : r < 16 ? 2 : r < 24 ? 3 : 4], 1ull);
-    bool edge = false;
-    if (kwin && r < topk) {
-        const int kb = (li % nomega) % ncol;
-        const int off = ((kb - kwin[(long long)i * topk + r]) % ncol + ncol) % ncol;
-        edge = off < 2 || off > wincols - 3;
+    const int kb = (li % nomega) % ncol;
+    int r = ncand, off = 0;
+    for (int c = 0; c < ncand; ++c) {
+        if (pbase[cand[(long long)i * ncand + c]] != base) continue;
+        if (kwin) {
+            const int o = ((kb - kwin[(long long)i * ncand + c]) % ncol + ncol) % ncol;
+            if (o >= wincols) continue;
+            off = o;
+        }
+        r = c;
+        break;
     }
-    if (r >= fb_rank || edge) list[atomicAdd(cnt, 1)] = i;
+    atomicAdd(&rank[r < 4 ? 0 : r < 8 ? 1 : r < 16 ? 2 : r < 24 ? 3 : r < topk ? 4 : r < ncand ? 5 : 4], 1ull);
+    const bool edge = kwin && r < ncand && (off < 2 || off > wincols - 3);
+    if ((r < topk && r >= fb_rank) || r >= ncand || edge) list[atomicAdd(cnt, 1)] = i;
 }
 
 // Each listed particle's exhaustive minimum (first minimum by value, then
@@ -1004,9 +1015,11 @@ struct GatherArgs {
     const int* pwin2;
     int nomega;
     // optional: the linear index of each point's best omega at least excl omega steps from its
-    // first minimum (circularly)
+    // first minimum (circularly, modulo exmod steps; 0: nomega), and with outv2 its score
     int* outi2;
     int excl;
+    float* outv2;
+    int exmod;
 };
 
 // Add the {N, D} bfloat16 pair of a table entry to the sums: N the low half,
@@ -1235,6 +1248,7 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
     if (!a.outi2) return;
     // the best omega at least excl steps from the first minimum (every thread holds it)
     const int kb = bidx - a.pbase[pt];
+    const int em = a.exmod ? a.exmod : a.nomega;
     float best2 = 3.0e38f;
     int bidx2 = 0x7fffffff;
 #pragma unroll
@@ -1243,15 +1257,18 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
         if (i >= a.ncol) continue;
         int k = i < half ? k0 + i : k1 + (i - half);
         k -= k >= a.nomega ? a.nomega : 0;
-        int dk = abs(k * a.ostep - kb);
-        dk = min(dk, a.nomega - dk);
+        int dk = abs(k * a.ostep - kb) % em;
+        dk = min(dk, em - dk);
         if (dk < a.excl) continue;
         const float pr = den[v] > 1.0e-10f ? num[v] / den[v] * r2d : 999.f;
         const int li = a.pbase[pt] + k * a.ostep;
         if (pr < best2 || (pr == best2 && li < bidx2)) { best2 = pr; bidx2 = li; }
     }
     block_argmin(best2, bidx2, redv, redi);
-    if (tid == 0) a.outi2[(long long)p * a.npts + pt] = bidx2;
+    if (tid == 0) {
+        a.outi2[(long long)p * a.npts + pt] = bidx2;
+        if (a.outv2) a.outv2[(long long)p * a.npts + pt] = best2;
+    }
 }
 
 // The K AU points of lowest table score per particle (ties: lower linear
@@ -1322,9 +1339,11 @@ __global__ void k_second_win(const int* __restrict__ ccand, int M, const int* __
 }
 
 // The K AU points of lowest table score per particle, any number of points:
-// K rounds, each scanning the points not chosen yet.
+// K rounds, each scanning the points not chosen yet.  Candidate kk of
+// particle p goes to cand / kwin[p * cstride + coff + kk].
 __global__ void k_topk_scan(const float* __restrict__ v, const int* __restrict__ ix, int npts, int K,
-                            int* __restrict__ cand, int Nomega, int ncol, int wincols, int* __restrict__ kwin) {
+                            int* __restrict__ cand, int Nomega, int ncol, int wincols, int* __restrict__ kwin,
+                            int cstride, int coff) {
     __shared__ int chosen[256];
     __shared__ float sv[16];
     __shared__ int si[16], sp[16];
@@ -1353,9 +1372,9 @@ __global__ void k_topk_scan(const float* __restrict__ v, const int* __restrict__
             for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
                 if (sv[i] < best || (sv[i] == best && si[i] < bidx)) { best = sv[i]; bidx = si[i]; bpt = sp[i]; }
             chosen[kk] = bpt < 0 ? 0 : bpt;
-            cand[(long long)p * K + kk] = chosen[kk];
+            cand[(long long)p * cstride + coff + kk] = chosen[kk];
             const int kb = (bidx == 0x7fffffff ? 0 : bidx % Nomega) % ncol;
-            kwin[(long long)p * K + kk] = ((kb - wincols / 2) % ncol + ncol) % ncol;
+            kwin[(long long)p * cstride + coff + kk] = ((kb - wincols / 2) % ncol + ncol) % ncol;
         }
         __syncthreads();
     }
@@ -1367,7 +1386,7 @@ __global__ void k_topk_scan(const float* __restrict__ v, const int* __restrict__
 constexpr int TOPK_PER = 16;   // npts <= 4096 (else k_topk_scan)
 __global__ void __launch_bounds__(256) k_topk(const float* __restrict__ v, const int* __restrict__ ix, int npts, int K,
                                               int* __restrict__ cand, int Nomega, int ncol, int wincols,
-                                              int* __restrict__ kwin) {
+                                              int* __restrict__ kwin, int cstride, int coff) {
     __shared__ float sv[4];
     __shared__ int si[4], sp[4];
     __shared__ int win_pt, win_li;
@@ -1405,9 +1424,9 @@ __global__ void __launch_bounds__(256) k_topk(const float* __restrict__ v, const
             for (int w = 1; w < 4; ++w)
                 if (sv[w] < best || (sv[w] == best && si[w] < bidx)) { best = sv[w]; bidx = si[w]; bpt = sp[w]; }
             const int ch = bpt < 0 ? 0 : bpt;
-            cand[(long long)p * K + kk] = ch;
+            cand[(long long)p * cstride + coff + kk] = ch;
             const int kb = (bidx == 0x7fffffff ? 0 : bidx % Nomega) % ncol;
-            kwin[(long long)p * K + kk] = ((kb - wincols / 2) % ncol + ncol) % ncol;
+            kwin[(long long)p * cstride + coff + kk] = ((kb - wincols / 2) % ncol + ncol) % ncol;
             win_pt = bpt;
         }
         __syncthreads();
@@ -1877,6 +1896,12 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         wincols = ncolx;
         rescore_cpw = 1;
     }
+    // second windows: besides the topk AU points, the nsecond best second minima of the table
+    // scores over omega (at least a window from the point's first minimum) are re-scored too
+    // (search_second; CIM_TABLE_SECOND=<S> overrides it)
+    int nsecond = p->search_second;
+    if (const char* e2 = getenv("CIM_TABLE_SECOND")) nsecond = atoi(e2);
+    nsecond = tables && wincols < ncolx ? std::max(0, std::min(std::min(nsecond, npts), 256 - topk)) : 0;
 
     // the tables take 4 T npsi^2 bytes per particle (8.3 MB at T = 4, 0.5 deg): batches of
     // CIM_TABLE_BATCH (default 512, 4.2 GB) particles.  Large batches keep enough waves of
@@ -2035,8 +2060,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         a.prfull = pr_array ? sb.prfull.as<float>() : nullptr;
         // counters: samples of exhaustive launches, re-score gate passes, re-score samples, the 5 rank
         // bins of k_fb_select
-        CIM_ALLOC(sb.counter, 8 * sizeof(unsigned long long));
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, sb.counter.as<unsigned long long>(), 8);
+        CIM_ALLOC(sb.counter, 9 * sizeof(unsigned long long));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, sb.counter.as<unsigned long long>(), 9);
         CIM_HIP(hipGetLastError());
         a.nsamples = sb.counter.as<unsigned long long>();
         a.npass = nullptr;
@@ -2112,6 +2137,21 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 CIM_HIP(hipGetLastError());
                 return CIM_OK;
             };
+            // second windows: the fine scores also keep each point's best omega at least a window
+            // from its first minimum (in the re-score's column space), with its score
+            auto second_minima = [&]() -> int {
+                const long long ns = (long long)nb * npts;
+                CIM_ALLOC(sb.outi2, sizeof(int) * (size_t)ns);
+                CIM_ALLOC(sb.outv2, sizeof(float) * (size_t)ns);
+                hipLaunchKernelGGL(k_fill_scores, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   sb.outv2.as<float>(), sb.outi2.as<int>(), ns);
+                CIM_HIP(hipGetLastError());
+                ga.outi2 = sb.outi2.as<int>();
+                ga.outv2 = sb.outv2.as<float>();
+                ga.excl = wincols;
+                ga.exmod = ncolx;
+                return CIM_OK;
+            };
             {
                 TimedLaunch tl(ctx, "cl_gather");
                 if (c2f) {
@@ -2145,7 +2185,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                     // its coarse minimum and around its best omega away from it
                     hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0,
                                        ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(), npts, coarseM,
-                                       sb.ccand.as<int>(), g.No, g.No, fwin ? fine_win / 2 : 0, sb.ckwin.as<int>());
+                                       sb.ccand.as<int>(), g.No, g.No, fwin ? fine_win / 2 : 0, sb.ckwin.as<int>(),
+                                       coarseM, 0);
                     CIM_HIP(hipGetLastError());
                     if (fwin) {
                         hipLaunchKernelGGL(k_second_win, dim3(nb), dim3(64), 0, ctx->stream, sb.ccand.as<int>(),
@@ -2169,20 +2210,33 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                         ga.pwin2 = sb.fwin2.as<int>();
                         ga.ncol = fine_win;
                     }
+                    if (nsecond && (rc = second_minima())) return rc;
                     if ((rc = gather(ga.nlist, false))) return rc;
-                } else if ((rc = gather(npts, false))) {
-                    return rc;
+                } else {
+                    if (nsecond && (rc = second_minima())) return rc;
+                    if ((rc = gather(npts, false))) return rc;
                 }
             }
-            CIM_ALLOC(sb.cand, sizeof(int) * (size_t)nb * topk);
-            CIM_ALLOC(sb.kwin, sizeof(int) * (size_t)nb * topk);
-            hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0, ctx->stream, sb.outv.as<float>(), sb.outi.as<int>(),
-                               npts, topk, sb.cand.as<int>(), g.No, ncolx, wincols, sb.kwin.as<int>());
+            const int ncand = topk + nsecond;
+            CIM_ALLOC(sb.cand, sizeof(int) * (size_t)nb * ncand);
+            CIM_ALLOC(sb.kwin, sizeof(int) * (size_t)nb * ncand);
+            hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0, ctx->stream,
+                               sb.outv.as<float>(), sb.outi.as<int>(), npts, topk, sb.cand.as<int>(), g.No, ncolx,
+                               wincols, sb.kwin.as<int>(), ncand, 0);
             CIM_HIP(hipGetLastError());
+            if (nsecond) {
+                // the nsecond best second minima, after the topk first ones
+                hipLaunchKernelGGL(npts <= 256 * TOPK_PER ? k_topk : k_topk_scan, dim3(nb), dim3(256), 0,
+                                   ctx->stream, sb.outv2.as<float>(), sb.outi2.as<int>(), npts, nsecond,
+                                   sb.cand.as<int>(), g.No, ncolx, wincols, sb.kwin.as<int>(), ncand, topk);
+                CIM_HIP(hipGetLastError());
+                ga.outi2 = nullptr;
+                ga.outv2 = nullptr;
+            }
             ctx->counters["cl_table_pairs"] += (long long)nb * T * npsi * npsi * nr;
             if (!c2f) ctx->counters["cl_gather_points"] += (long long)nb * npts * g.No;
             a.cand = sb.cand.as<int>();
-            a.ncand = topk;
+            a.ncand = ncand;
             a.prfull = nullptr;
             if (wincols < ncolx) {
                 a.kwin = sb.kwin.as<int>();
@@ -2197,7 +2251,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                 // (the chunking does not change any column's accumulation order)
                 a.chunk = std::min(chunk, 6);
             }
-            nscore = topk * (a.kwin && rescore_cpw == 1 ? a.wsplit : 1);
+            nscore = ncand * (a.kwin && rescore_cpw == 1 ? a.wsplit : 1);
         }
         if (ucand) {
             // the caller's candidates: AU point, and the omega column its window is centred on
@@ -2229,7 +2283,14 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             TimedLaunch tl(ctx, tables || ucand ? "cl_rescore" : "cl_search");
             // windows: one wave of single-column lanes in the same kernel variant
             if (a.kwin && rescore_cpw > 1) {
-                a.chunk = std::min(chunk, 3);
+                // lines whose template records are staged per pass (CIM_RESCORE_CHUNK; the
+                // chunking does not change any column's order of terms): the LDS per one-wave
+                // workgroup sets how many of them stay resident.  One line: 7.6 KB at maxR 72, so
+                // 4 waves per SIMD (a batch of 512 is 16 workgroups per CU, all resident at once);
+                // measured at box 384: 6.4 ms per 512 particles against 7.9 (two lines) and 9.0 (three)
+                int rchunk = 1;
+                if (const char* er = getenv("CIM_RESCORE_CHUNK")) rchunk = std::max(1, std::min(6, atoi(er)));
+                a.chunk = std::min(chunk, rchunk);
                 a.nsamples = sb.counter.as<unsigned long long>() + 2;
                 a.npass = sb.counter.as<unsigned long long>() + 1;
                 const int lw = (int)(sizeof(float4) * (size_t)rescore_cpw * a.chunk * nr +
@@ -2270,8 +2331,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream,
                                reinterpret_cast<unsigned long long*>(fbcnt), 1);
             hipLaunchKernelGGL(k_fb_select, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, sb.fi.as<int>(),
-                               sb.cand.as<int>(), a.kwin, sb.pbase.as<int>(), nb, topk, g.No, ncolx, wincols, fb_rank,
-                               sb.counter.as<unsigned long long>() + 3, fblist, fbcnt);
+                               sb.cand.as<int>(), a.kwin, sb.pbase.as<int>(), nb, topk, a.ncand, g.No, ncolx, wincols,
+                               fb_rank, sb.counter.as<unsigned long long>() + 3, fblist, fbcnt);
             CIM_HIP(hipGetLastError());
             SearchArgs a1 = a;
             a1.cand = nullptr;
@@ -2330,7 +2391,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                                      (coarse_grid ? g.No / 2 : g.No);
         }
         ctx->counters["cl_fallback_particles"] += hfb;
-        unsigned long long hcount[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        unsigned long long hcount[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         CIM_HIP(hipMemcpyAsync(hcount, sb.counter.ptr, sizeof(hcount), hipMemcpyDeviceToHost, ctx->stream));
         CIM_HIP(hipStreamSynchronize(ctx->stream));
         ctx->counters["cl_search_samples"] += (long long)(hcount[0] + hcount[2]);
@@ -2338,8 +2399,9 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         ctx->counters["cl_rescore_pass"] += (long long)hcount[1];
         if (tables && fb_rank <= topk) {
             // where each particle's exact minimum ranked among the table's candidates
-            const char* bins[5] = {"cl_rank_lt4", "cl_rank_lt8", "cl_rank_lt16", "cl_rank_lt24", "cl_rank_ge24"};
-            for (int b = 0; b < 5; ++b) ctx->counters[bins[b]] += (long long)hcount[3 + b];
+            const char* bins[6] = {"cl_rank_lt4", "cl_rank_lt8", "cl_rank_lt16", "cl_rank_lt24", "cl_rank_ge24",
+                                   "cl_rank_second"};
+            for (int b = 0; b < 6; ++b) ctx->counters[bins[b]] += (long long)hcount[3 + b];
         }
         ctx->counters["cl_search_particles"] += nb;
         for (int i = 0; i < nb; ++i) {

@@ -25,8 +25,8 @@ module cim_iface
     type, bind(C) :: cim_params
         integer(c_int) :: box = 0, minR = 0, maxR = 0, apply_ctf = 1, mask_on = 1, norm_on = 0
         integer(c_int) :: ncycle = 6, conj_out = 1
-        integer(c_int) :: search_method = 0, search_topk = 0, corr_cycles = 0
-        integer(c_int) :: reserved_i(5) = 0
+        integer(c_int) :: search_method = 0, search_topk = 0, corr_cycles = 0, search_second = 0
+        integer(c_int) :: reserved_i(4) = 0
         real(c_float) :: apix = 1.0, imgmask = 0.0, vol = 300.0, cs = 2.7, ampwg = 0.1, df_scale = 1.0e4
         real(c_float) :: searchstep = 2.0, delta_angle = 0.0, delta_trans = 0.0, corr_step = 0.0
         real(c_float) :: reserved_f(6) = 0.0

@@ -60,7 +60,8 @@ class CimParams(ctypes.Structure):
         ("box", ctypes.c_int), ("minR", ctypes.c_int), ("maxR", ctypes.c_int),
         ("apply_ctf", ctypes.c_int), ("mask_on", ctypes.c_int), ("norm_on", ctypes.c_int),
         ("ncycle", ctypes.c_int), ("conj_out", ctypes.c_int), ("search_method", ctypes.c_int),
-        ("search_topk", ctypes.c_int), ("corr_cycles", ctypes.c_int), ("reserved_i", ctypes.c_int * 5),
+        ("search_topk", ctypes.c_int), ("corr_cycles", ctypes.c_int), ("search_second", ctypes.c_int),
+        ("reserved_i", ctypes.c_int * 4),
         ("apix", ctypes.c_float), ("imgmask", ctypes.c_float), ("vol", ctypes.c_float),
         ("cs", ctypes.c_float), ("ampwg", ctypes.c_float), ("df_scale", ctypes.c_float),
         ("searchstep", ctypes.c_float), ("delta_angle", ctypes.c_float),
@@ -71,7 +72,7 @@ class CimParams(ctypes.Structure):
 def make_params(box, minR=0, maxR=None, apply_ctf=True, mask_on=True, norm_on=False, ncycle=6,
                 conj_out=True, apix=1.0, imgmask=None, vol=300.0, cs=2.7, ampwg=0.1, df_scale=1.0e4,
                 searchstep=2.0, delta_angle=0.0, delta_trans=0.0, search_method=0, search_topk=0,
-                corr_step=0.0, corr_cycles=0) -> CimParams:
+                corr_step=0.0, corr_cycles=0, search_second=0) -> CimParams:
     p = CimParams()
     p.box = int(box)
     p.minR = int(minR)
@@ -88,6 +89,7 @@ def make_params(box, minR=0, maxR=None, apply_ctf=True, mask_on=True, norm_on=Fa
     p.delta_angle, p.delta_trans = float(delta_angle), float(delta_trans)
     p.search_method = int(search_method)
     p.search_topk = int(search_topk)
+    p.search_second = int(search_second)
     p.corr_step, p.corr_cycles = float(corr_step), int(corr_cycles)
     return p
 

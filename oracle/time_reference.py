@@ -9,7 +9,7 @@ Each program runs as N concurrent single-threaded processes (MKL and OpenMP
 on one thread each) over disjoint particle ranges: the reference's only
 parallel mode.
 
-Usage: python oracle/time_reference.py [--stages main center lbox]
+Usage: python oracle/time_reference.py [--stages main center lbox c1]
 """
 from __future__ import annotations
 
@@ -188,6 +188,71 @@ def _ref_baseline(exe, args, tang, timgs, ps, ncores):
 
 
 
+def _run_ref(exe, argv_of, ncores, tmp):
+    """ncores concurrent reference processes (argv_of(i) each); wall seconds."""
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen([exe, *argv_of(i)], cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                              env=_ref_env()) for i in range(ncores)]
+    rcs = [pr.wait() for pr in procs]
+    if any(rcs):
+        raise RuntimeError(f"{os.path.basename(exe)} exit codes {rcs}")
+    return time.perf_counter() - t0
+
+
+def c1_baseline(ncores):
+    """The box-128 workload of bench.py --stage c1 (tests/pipelines.py
+    config1_workload) through the reference icosprocess on the host cores:
+    mode 3 (calc_center with model3d), mode 4 on the 1 deg grid and mode 1
+    (subrec) at maxR 62.  Per-particle costs are differences of runs with
+    K and 1 particles per process, which removes start-up, template and
+    model transforms."""
+    from cryoim_amd import formats, synth
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pipelines
+    exe = os.path.join(REF, "icosprocess")
+    w = pipelines.config1_workload(64)
+    box, apix, d, ps, tang = w["box"], w["apix"], w["d"], w["ps"], w["tang"]
+    m = len(ps.images)
+    K4, K3, K1 = 2, 41, 41
+    common = [f"apix={apix}", "maxRes=8.0", "minRes=300.0", f"imgmask={d['mask'] * apix}"]
+    t = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        formats.write_mrc(os.path.join(tmp, "stk.mrc"), ps.images)
+        formats.write_mrc(os.path.join(tmp, "tmpl.mrc"), w["timgs"])
+        formats.write_mrc(os.path.join(tmp, "model.mrc"), synth.phantom_map(w["phantom"]))
+        formats.write_template_ort(os.path.join(tmp, "tort.txt"), tang)
+        K = max(K4, K3, K1)
+        parts = [formats.Particle("stk.mrc", (i % m) + 1, *ps.angles[i % m], *ps.shifts[i % m], 0.0, i + 1,
+                                  (i % m) + 1, ps.df[i % m, 0], ps.df[i % m, 1], 0.0) for i in range(K * ncores)]
+        formats.write_ortfile(os.path.join(tmp, "ort.txt"), parts)
+        for k in (1, K4):
+            t[("4", k)] = _run_ref(exe, lambda i, k=k: ["mode=4", "templateort=tort.txt", "templatestck=tmpl.mrc",
+                                                        "searchstep=1.0", f"newortfile=o{i}.txt", f"first={i * K + 1}",
+                                                        f"last={i * K + k}", "ortfile=ort.txt", *common], ncores, tmp)
+        for k in (1, K3):
+            t[("3", k)] = _run_ref(exe, lambda i, k=k: ["mode=3", "model3d=model.mrc", f"newortfile=c{i}.txt",
+                                                        f"first={i * K + 1}", f"last={i * K + k}", "ortfile=ort.txt",
+                                                        *common], ncores, tmp)
+        maxres_ins = apix * box / w["insert_maxR"]
+        for k in (1, K1):
+            t[("1", k)] = _run_ref(exe, lambda i, k=k: ["mode=1", "subrec=y", f"result3d=s{i}.tnf",
+                                                        f"first={i * K + 1}", f"last={i * K + k}", f"apix={apix}",
+                                                        f"maxRes={maxres_ins}", f"imgmask={d['mask'] * apix}",
+                                                        "ortfile=ort.txt"], ncores, tmp)
+    s4 = (t[("4", K4)] - t[("4", 1)]) / (K4 - 1)
+    s3 = (t[("3", K3)] - t[("3", 1)]) / (K3 - 1)
+    s1 = (t[("1", K1)] - t[("1", 1)]) / (K1 - 1)
+    stages = {"mode3_particles_per_s": round(ncores / s3, 3), "mode4_particles_per_s": round(ncores / s4, 4),
+              "mode1_particles_per_s": round(ncores / s1, 2)}
+    return {"value": ncores / (s4 + s1), "unit": "particles/s", "cores": ncores, "kind": "reference",
+            "stages": stages,
+            "sample": (f"icosprocess (flang -O2, MKL FFTW) as {ncores} concurrent processes on the box-128 workload "
+                       f"(T={len(tang)}): mode 4 on the 1 deg grid, {K4} particles each ({t[('4', K4)]:.1f} s) minus "
+                       f"1 each ({t[('4', 1)]:.1f} s); mode 1 subrec at maxR {w['insert_maxR']}, {K1} particles each "
+                       f"({t[('1', K1)]:.1f} s) minus 1 ({t[('1', 1)]:.1f} s); value = search + insert.  Mode 3 "
+                       f"(calc_center, model3d) apart: {K3} each ({t[('3', K3)]:.1f} s) minus 1 ({t[('3', 1)]:.1f} s)")}
+
+
 def _host():
     cpu = "unknown"
     try:
@@ -221,6 +286,10 @@ def main():
         r = center_cpu_baseline(ph, ps)
         res["center"] = dict(r, date=stamp, host=_host())
         print("center", res["center"], flush=True)
+    if "c1" in args.stages:
+        r = c1_baseline(ncores)
+        res["c1"] = dict(r, date=stamp, host=_host())
+        print("c1", res["c1"], flush=True)
     if "lbox" in args.stages:
         r = lbox_cpu_baseline(ps)
         res["lbox"] = dict(r, date=stamp, host=_host())

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from cryoim_amd import geometry, lib, synth
+import pipelines
 
 pytestmark = pytest.mark.gpu
 
@@ -227,6 +228,56 @@ def test_tabulated_search_fallback_is_plain(ctx, monkeypatch, step):
     tab, _ = _search_pair(ctx, lib.SEARCH_TABLES, 2, n=8, step=step)
     assert ctx.counter("cl_fallback_particles") == 8
     assert np.array_equal(tab[:, [0, 1, 2, 5]], plain[:, [0, 1, 2, 5]])
+
+
+def test_tabulated_search_second_windows(ctx, monkeypatch):
+    """Second windows (search_second) add candidates after the topk AU points:
+    each re-scored column is the plain search's column bit for bit, so with the
+    fallback off a particle's result can only move to a lower PR, every
+    particle identical to the plain search without them stays identical, and
+    every result's PR is at least the plain minimum.  On 2,000 particles of the
+    box-128 recipe (1 deg grid, maxR 24) some exact minima lie in a second
+    window."""
+    import torch
+    from cryoim_amd import synth
+    box, apix, step = 128, 1.5, 1.0
+    ph = synth.make_phantom(box, nblobs=20, sigma=(2.0, 4.0), radius=(40.0, 55.0), seed=101)
+    tang = synth.random_au_orientations(4, np.random.default_rng(102))
+    timgs = synth.project(ph, tang)
+    ps = synth.make_particles_torch(ph, 2000, seed=103, shift_max=3.0, df_range=(1.0, 3.0), snr=0.1, apix=apix,
+                                    mask_radius=box / 2 - 4)
+    d = pipelines.derived(box, apix, maxRes=8.0, minRes=300.0, imgmask_A=(box / 2 - 4) * apix)
+    P = np.zeros((2000, lib.PART_STRIDE), np.float32)
+    P[:, 3:5] = ps.shifts
+    P[:, 6:8] = ps.df
+    P[:, 11] = 1.0
+    out = {}
+    # fallback off (results can only improve with more windows), then on (ranks counted)
+    for name, method, S, fb in (("plain", lib.SEARCH_PLAIN, 0, "0"), ("s0", lib.SEARCH_TABLES, 0, "0"),
+                                ("s8", lib.SEARCH_TABLES, 8, "0"), ("s8fb", lib.SEARCH_TABLES, 8, None)):
+        if fb is None:
+            monkeypatch.delenv("CIM_TABLE_FALLBACK", raising=False)
+        else:
+            monkeypatch.setenv("CIM_TABLE_FALLBACK", fb)
+        p = lib.make_params(box, d["minR"], d["maxR"], apply_ctf=True, mask_on=True, apix=apix, imgmask=d["mask"],
+                            searchstep=step, search_method=method, search_second=S)
+        ctx.search_templates(p, timgs, np.c_[tang, np.zeros((4, 2))])
+        ctx.counter_reset()
+        out[name] = ctx.search_global(p, ps.images, P)
+        out[name, "second"] = ctx.counter("cl_rank_second")
+        torch.cuda.synchronize()
+    plain, s0, s8 = out["plain"], out["s0"], out["s8"]
+    same = {k: np.all(out[k][:, [0, 1, 2, 5]] == plain[:, [0, 1, 2, 5]], axis=1) for k in ("s0", "s8", "s8fb")}
+    moved = ~np.all(s8[:, [0, 1, 2, 5]] == s0[:, [0, 1, 2, 5]], axis=1)
+    print(f"identical to plain: {same['s0'].sum()} without, {same['s8'].sum()} with second windows, "
+          f"{same['s8fb'].sum()} with the fallback too; {moved.sum()} results moved; "
+          f"{out['s8fb', 'second']} minima found in a second window")
+    assert np.all(s8[:, 5] <= s0[:, 5])
+    assert np.all(same["s8"][same["s0"]])
+    assert np.all(s8[:, 5] >= plain[:, 5])
+    assert moved.sum() > 0 and same["s8"].sum() > same["s0"].sum()
+    assert out["s8fb", "second"] > 0
+    assert same["s8fb"].sum() >= same["s8"].sum()
 
 
 def test_tabulated_search_needs_closed_grid(ctx):
