@@ -261,10 +261,11 @@ __global__ void k_tnf_io(float4* __restrict__ acc, const float4* __restrict__ lo
     auto cell = [&](long long i) {
         float4 e = acc[i];
         if (lo) {
+            // one float add of hi and lo is their exact sum rounded once
             const float4 l = lo[i];
-            e.x = (float)((double)e.x + l.x);
-            e.y = (float)((double)e.y + l.y);
-            e.z = (float)((double)e.z + l.z);
+            e.x = __fadd_rn(e.x, l.x);
+            e.y = __fadd_rn(e.y, l.y);
+            e.z = __fadd_rn(e.z, l.z);
         }
         return e;
     };
@@ -375,12 +376,33 @@ __device__ __forceinline__ int brick_cap(const BrickArgs& a) {
 }
 
 // x (|x| < 2^51) rounded to the nearest integer, ties to even, as a 64-bit
-// two's complement value: adding 1.5 * 2^52 in double rounds x to an integer
-// in the low mantissa bits (the round-to-nearest of the add), and the
-// difference of the bit patterns is that integer.
+// two's complement value, in float32 and integer arithmetic: x = xh 2^24 + xr
+// with xh = rint(x 2^-24) and xr = x - xh 2^24, which the fused multiply-add
+// gives exactly (|xr| <= 2^23 and its bits lie within x's); xh 2^24 is an even
+// integer, so rint(x) = xh 2^24 + rint(xr), ties included.
 __device__ __forceinline__ unsigned long long fix64(float x) {
-    const double magic = 6755399441055744.0;
-    return (unsigned long long)(__double_as_longlong((double)x + magic) - __double_as_longlong(magic));
+    const float xh = __builtin_rintf(x * 0x1p-24f);
+    const float xr = __builtin_fmaf(-xh, 0x1p24f, x);
+    const long long h = (long long)(int)xh;          // |xh| < 2^27
+    return (unsigned long long)(h * (1ll << 24) + (long long)(int)__builtin_rintf(xr));
+}
+
+// s + c += x: TwoSum (Knuth) adds x to s and the add's rounding error to c.
+__device__ __forceinline__ void two_sum_acc(float& s, float& c, float x) {
+    const float t = __fadd_rn(s, x);
+    const float xp = __fsub_rn(t, s);
+    const float err = __fadd_rn(__fsub_rn(s, __fsub_rn(t, xp)), __fsub_rn(x, xp));
+    s = t;
+    c = __fadd_rn(c, err);
+}
+
+// A batch sum S of fixed-point terms (|S| < 2^63) times 2^-cap as three exact
+// float32 parts, largest first: S = c2 2^42 + c1 2^21 + c0 with 0 <= c0, c1 < 2^21
+// and |c2| < 2^21; sc = {2^(42-cap), 2^(21-cap), 2^-cap}.
+__device__ __forceinline__ void fix_parts(long long S, const float* sc, float* part) {
+    part[0] = (float)(int)(S >> 42) * sc[0];
+    part[1] = (float)(int)((S >> 21) & 0x1fffff) * sc[1];
+    part[2] = (float)(int)(S & 0x1fffff) * sc[2];
 }
 
 constexpr int BRICK_NV = BRICK * BRICK * BRICK;
@@ -391,15 +413,28 @@ constexpr int BRICK_NV = BRICK * BRICK * BRICK;
 // whatever the plane's orientation.
 __device__ __forceinline__ int brick_slot(int i) { return (i & ~31) | ((i + 5 * (i >> 5)) & 31); }
 
-// hi + lo += d: TwoSum (Knuth) keeps the rounding error of the float add in lo,
-// together with the part of d below float precision.
-__device__ __forceinline__ void comp_add(float& hi, float& lo, double d) {
-    const float y = (float)d;
-    const float s = __fadd_rn(hi, y);
-    const float yy = __fsub_rn(s, hi);
-    const float err = __fadd_rn(__fsub_rn(hi, __fsub_rn(s, yy)), __fsub_rn(y, yy));
-    lo = __fadd_rn(lo, __fadd_rn(err, (float)(d - (double)y)));
-    hi = s;
+// hi + lo += the batch sum S 2^-cap: its three exact parts (fix_parts) added
+// largest first by TwoSum, the rounding errors into lo (compensated); or the
+// parts' sum added to hi (plain).
+__device__ __forceinline__ void flush_add(float& hi, float* lo, long long S, const float* sc) {
+    float part[3];
+    fix_parts(S, sc, part);
+    if (lo) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) two_sum_acc(hi, *lo, part[k]);
+    } else {
+        float s = 0.f, c = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) two_sum_acc(s, c, part[k]);
+        hi = __fadd_rn(hi, __fadd_rn(s, c));
+    }
+}
+
+// The fixed-point scales of a flush: {2^(42-cap), 2^(21-cap), 2^-cap}.
+__device__ __forceinline__ void flush_scales(int cap, float* sc) {
+    sc[0] = ldexpf(1.f, 42 - cap);
+    sc[1] = ldexpf(1.f, 21 - cap);
+    sc[2] = ldexpf(1.f, -cap);
 }
 
 // One read-modify-write of the brick's exact batch sums into HBM (brick of
@@ -407,12 +442,14 @@ __device__ __forceinline__ void comp_add(float& hi, float& lo, double d) {
 // voxel i at LDS slot brick_slot(i)).
 template <int BX, int BY, int BZ, bool SWZ>
 __device__ void brick_flush(const BrickArgs& a, const unsigned long long* sacc, int bx0, int by0, int bz0,
-                            double inv) {
+                            int cap) {
     constexpr int NV = BX * BY * BZ;
     static_assert(!SWZ || NV % 32 == 0, "slot rotation needs whole blocks of 32");
     const InsertGeom& g = a.g;
     const int tid = threadIdx.x;
     const long long dxy = g.dxy;
+    float sc[3];
+    flush_scales(cap, sc);
     for (int i = tid; i < NV; i += blockDim.x) {
         const int cx = i % BX, cy = (i / BX) % BY, cz = i / (BX * BY);
         const int X = bx0 + cx, Y = by0 + cy, Z = bz0 + cz;
@@ -420,21 +457,19 @@ __device__ void brick_flush(const BrickArgs& a, const unsigned long long* sacc, 
         const int si = SWZ ? brick_slot(i) : i;
         const long long sa = (long long)sacc[si], sb = (long long)sacc[NV + si], sw = (long long)sacc[2 * NV + si];
         if ((sa | sb | sw) == 0) continue;
-        const double da = (double)sa * inv, db = (double)sb * inv, dw = (double)sw * inv;
         const long long vi = (long long)Z * dxy * dxy + (long long)Y * dxy + X;
         float4 v = a.acc[vi];
         if (a.lo) {
-            // compensated: hi + lo carries the sum; TwoSum (Knuth) keeps the rounding error of each add,
-            // and the part of the exact batch sum below float precision goes to lo as well
+            // compensated: hi + lo carries the sum; TwoSum keeps the rounding error of each add
             float4 l = a.lo[vi];
-            comp_add(v.x, l.x, da);
-            comp_add(v.y, l.y, db);
-            comp_add(v.z, l.z, dw);
+            flush_add(v.x, &l.x, sa, sc);
+            flush_add(v.y, &l.y, sb, sc);
+            flush_add(v.z, &l.z, sw, sc);
             a.lo[vi] = l;
         } else {
-            v.x = (float)((double)v.x + da);
-            v.y = (float)((double)v.y + db);
-            v.z = (float)((double)v.z + dw);
+            flush_add(v.x, nullptr, sa, sc);
+            flush_add(v.y, nullptr, sb, sc);
+            flush_add(v.z, nullptr, sw, sc);
         }
         a.acc[vi] = v;
     }
@@ -454,7 +489,6 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
     for (int i = tid; i < 3 * NV; i += blockDim.x) sacc[i] = 0ull;
     const int cap = brick_cap(a);
     const float scale = ldexpf(1.f, cap);
-    const double inv = ldexp(1.0, -cap);
     // positions (before the index offsets) whose corners can reach the brick:
     // index gx covers x in [gx - R - 1, gx - R + 1), likewise y; z in [gz - 1, gz + 1)
     const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - 1)};
@@ -569,7 +603,7 @@ __global__ void __launch_bounds__(256) k_insert_brick(BrickArgs a) {
         }
         __syncthreads();
     }
-    brick_flush<BRICK, BRICK, BRICK, false>(a, sacc, bx0, by0, bz0, inv);
+    brick_flush<BRICK, BRICK, BRICK, false>(a, sacc, bx0, by0, bz0, cap);
 }
 
 // The same brick gather with the work of a wave packed into full lanes.
@@ -611,7 +645,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
     for (int i = tid; i <= 2 * R; i += NT) shx[i] = a.hxmax[i];
     const int cap = brick_cap(a);
     const float scale = ldexpf(1.f, cap);
-    const double inv = ldexp(1.0, -cap);
     const int zsh = CUBE ? R : 0;
     const float lo[3] = {(float)(bx0 - R - 1), (float)(by0 - R - 1), (float)(bz0 - zsh - 1)};
     const float hi[3] = {(float)(bx0 - R + BX), (float)(by0 - R + BY), (float)(bz0 - zsh + BZ)};
@@ -767,7 +800,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) 
     }
     if (qn > 0) drain(qn);
     __syncthreads();
-    brick_flush<BX, BY, BZ, true>(a, sacc, bx0, by0, bz0, inv);
+    brick_flush<BX, BY, BZ, true>(a, sacc, bx0, by0, bz0, cap);
 }
 
 // ---- coset insert: symmetrisation of the cube into the half-volume sums -------
@@ -795,20 +828,23 @@ __global__ void k_coset_sym(const float4* __restrict__ cube, const float4* __res
     if (id >= n2 * dz) return;
     const int Z = (int)(id / n2), Y = (int)((id / dxy) % dxy), X = (int)(id % dxy);
     const int x[3] = {X - R, Y - R, Z};
-    auto at = [&](int u0, int u1, int u2) -> double4 {
-        if (u0 < -R || u0 > R + 1 || u1 < -R || u1 > R + 1 || u2 < -R || u2 > R + 1) return make_double4(0, 0, 0, 0);
+    // the 12 permuted values of x and of -x, summed with TwoSum (s) and their errors and the
+    // cube's lo parts (c), in float32
+    float fa = 0.f, fac = 0.f, fb = 0.f, fbc = 0.f, wg = 0.f, wgc = 0.f;
+    auto add = [&](int u0, int u1, int u2, float sgn_b) {
+        if (u0 < -R || u0 > R + 1 || u1 < -R || u1 > R + 1 || u2 < -R || u2 > R + 1) return;
         const long long i = ((long long)(u2 + R) * dxy + (u1 + R)) * dxy + (u0 + R);
         const float4 h = cube[i];
-        double4 r = make_double4(h.x, h.y, h.z, 0.0);
+        two_sum_acc(fa, fac, h.x);
+        two_sum_acc(fb, fbc, sgn_b * h.y);
+        two_sum_acc(wg, wgc, h.z);
         if (cube_lo) {
             const float4 l = cube_lo[i];
-            r.x += l.x;
-            r.y += l.y;
-            r.z += l.z;
+            fac = __fadd_rn(fac, l.x);
+            fbc = __fadd_rn(fbc, sgn_b * l.y);
+            wgc = __fadd_rn(wgc, l.z);
         }
-        return r;
     };
-    double fa = 0.0, fb = 0.0, wg = 0.0;
 #pragma unroll 1
     for (int j = 0; j < 12; ++j) {
         // u = g_j^T x: u[c_i] = s_i x[i]
@@ -818,28 +854,29 @@ __global__ void k_coset_sym(const float4* __restrict__ cube, const float4* __res
             const int c = t.c[j][i];
             u[c & 3] = (c & 4) ? -x[i] : x[i];
         }
-        const double4 p = at(u[0], u[1], u[2]), m = at(-u[0], -u[1], -u[2]);
-        fa += p.x + m.x;
-        fb += p.y - m.y;
-        wg += p.z + m.z;
+        add(u[0], u[1], u[2], 1.f);
+        add(-u[0], -u[1], -u[2], -1.f);
     }
     if (Z == 0) {
-        fa *= 0.5;
-        fb *= 0.5;
-        wg *= 0.5;
+        fa *= 0.5f; fac *= 0.5f;
+        fb *= 0.5f; fbc *= 0.5f;
+        wg *= 0.5f; wgc *= 0.5f;
     }
-    if (fa == 0.0 && fb == 0.0 && wg == 0.0) return;
+    if (fa == 0.f && fb == 0.f && wg == 0.f && fac == 0.f && fbc == 0.f && wgc == 0.f) return;
     float4 v = acc[id];
     if (lo) {
         float4 l = lo[id];
-        comp_add(v.x, l.x, fa);
-        comp_add(v.y, l.y, fb);
-        comp_add(v.z, l.z, wg);
+        two_sum_acc(v.x, l.x, fa);
+        two_sum_acc(v.y, l.y, fb);
+        two_sum_acc(v.z, l.z, wg);
+        l.x = __fadd_rn(l.x, fac);
+        l.y = __fadd_rn(l.y, fbc);
+        l.z = __fadd_rn(l.z, wgc);
         lo[id] = l;
     } else {
-        v.x = (float)((double)v.x + fa);
-        v.y = (float)((double)v.y + fb);
-        v.z = (float)((double)v.z + wg);
+        v.x = __fadd_rn(v.x, __fadd_rn(fa, fac));
+        v.y = __fadd_rn(v.y, __fadd_rn(fb, fbc));
+        v.z = __fadd_rn(v.z, __fadd_rn(wg, wgc));
     }
     acc[id] = v;
 }
