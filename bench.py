@@ -961,12 +961,26 @@ def _roofline(r, args):
                      "bytes_rule": "every coarse and fine table entry of the batch read once",
                      "note": "the sums read each table row once per (AU point, line) from the caches: the kernel "
                              "is bound by cache-line throughput, not by HBM"}
-    per = cnt["insert_samples"] / launches * 12
+    # the least HBM traffic of a launch: each particle's staged sample values read once
+    # ({a ctf, b ctf, ctf^2, in}, 16 B on the (2R+1)(R+1) half-plane grid) and the accumulator's
+    # hi and lo cells within radius R+1 of the origin read and written once (16 B each)
+    R = INSERT_MAXR
+    g = np.arange(-R - 1, R + 2)
+    sphere = int(np.count_nonzero(g[:, None, None] ** 2 + g[None, :, None] ** 2 + g[None, None, :] ** 2
+                                  <= (R + 1) ** 2))
+    per = n * (2 * R + 1) * (R + 1) * 16 + 2 * 2 * 16 * sphere
     achieved = per / sec / 1e9
     return dom, {"bound": "hbm", "kernel": "k_insert_brickq (60-fold coset insert, compensated FP32)",
                  "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                  "frac": round(achieved / PEAK_HBM_GBS, 4), **base, "algorithmic_bytes_per_launch": int(per),
-                 "bytes_rule": "each inserted sample's value and weight read once (12 bytes)"}
+                 "bytes_rule": (f"sample values read once ({n} particles x (2R+1)(R+1) x 16 B) + the accumulator's "
+                                f"hi and lo cells within radius R+1 ({sphere} per map) read and written once, "
+                                f"16 B each; R = {R}"),
+                 "note": ("the kernel sums each batch in LDS (64-bit fixed-point atomics, 3 per corner of 5 coset "
+                          "planes per particle) and touches each accumulator cell once per launch, so HBM does not "
+                          "bound it: LDS atomics (and their bank conflicts) and the point enumeration's VALU work "
+                          "do (profiles/sq_counters_box384.md); in the pipelined steps it also shares the chip "
+                          "with the next batch's search")}
 
 
 def main():

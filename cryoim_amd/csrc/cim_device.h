@@ -6,12 +6,67 @@
 
 namespace cim {
 
+// Float-float numbers (hi + lo, |lo| <= ulp(hi) / 2): about 48 significant bits
+// from float32 arithmetic and fused multiply-adds, for the few per-pixel
+// phases whose arguments reach hundreds of radians.
+struct ff32 {
+    float h, l;
+};
+
+__host__ __device__ inline ff32 ff_from_double(double d) {
+    const float h = (float)d;
+    return ff32{h, (float)(d - (double)h)};
+}
+
+__device__ __forceinline__ ff32 ff_quick(float s, float e) {
+    const float t = __fadd_rn(s, e);
+    return ff32{t, __fsub_rn(e, __fsub_rn(t, s))};
+}
+
+// a + b exactly as a float-float (TwoSum)
+__device__ __forceinline__ ff32 ff_two_sum(float a, float b) {
+    const float s = __fadd_rn(a, b);
+    const float bp = __fsub_rn(s, a);
+    return ff32{s, __fadd_rn(__fsub_rn(a, __fsub_rn(s, bp)), __fsub_rn(b, bp))};
+}
+
+// a * b exactly as a float-float (TwoProd by fused multiply-add)
+__device__ __forceinline__ ff32 ff_two_prod(float a, float b) {
+    const float p = __fmul_rn(a, b);
+    return ff32{p, __builtin_fmaf(a, b, -p)};
+}
+
+__device__ __forceinline__ ff32 ff_mul(ff32 x, ff32 y) {
+    const ff32 p = ff_two_prod(x.h, y.h);
+    return ff_quick(p.h, __fadd_rn(p.l, __fadd_rn(__fmul_rn(x.h, y.l), __fmul_rn(x.l, y.h))));
+}
+
+__device__ __forceinline__ ff32 ff_add(ff32 x, ff32 y) {
+    const ff32 s = ff_two_sum(x.h, y.h);
+    return ff_quick(s.h, __fadd_rn(s.l, __fadd_rn(x.l, y.l)));
+}
+
+// 1 / m for an integer m < 2^24, as a float-float (one Newton step)
+__device__ __forceinline__ ff32 ff_recip_int(int m) {
+    const float mf = (float)m;
+    const float q = 1.0f / mf;
+    return ff32{q, __fmul_rn(__builtin_fmaf(-q, mf, 1.0f), q)};
+}
+
+// x (float-float, |x| < 2^20) reduced to [-pi, pi] modulo 2 pi, as a float
+__device__ __forceinline__ float ff_reduce_2pi(ff32 x) {
+    const float two_pi_h = 6.28318548202514648f, two_pi_l = -1.74845553e-7f;   // 2 pi = h + l
+    const float k = __builtin_rintf(x.h * 0.159154943091895336f);
+    // x.h - k two_pi_h is exact: its bits lie between ulp(x.h) and 2^3
+    const float r = __builtin_fmaf(-k, two_pi_h, x.h);
+    return __fadd_rn(__fsub_rn(r, __fmul_rn(k, two_pi_l)), x.l);
+}
+
 struct CtfConst {
-    double lam;      // electron wavelength (A)
-    double cs;       // Cs in A
+    ff32 a;          // pi lambda / apix^2: the defocus term's factor (phase = a df s - b s^2, s = n / nx^2)
+    ff32 b;          // (pi / 2) Cs lambda^3 / apix^4
     float ampwg;     // amplitude contrast
     float sq;        // sqrt(1 - ampwg^2)
-    float apix;
     float df_scale;  // 1e4 (um) or 1 (A)
     int corr_cycles; // defocus series: C steps each side (0: one defocus)
     float corr_step; // step of the series in the df unit
@@ -19,35 +74,49 @@ struct CtfConst {
 
 inline CtfConst make_ctf_const(const cim_params* p) {
     CtfConst c;
-    // getCTF2d.f90:12-13
-    c.lam = 0.387818 / std::sqrt((double)p->vol * (1.0 + 0.978459e-3 * (double)p->vol));
-    c.cs = (double)p->cs * 1.0e7;
+    // getCTF2d.f90:12-13 (wavelength, Cs in A), folded with the pixel size on the host in float64
+    const double lam = 0.387818 / std::sqrt((double)p->vol * (1.0 + 0.978459e-3 * (double)p->vol));
+    const double cs = (double)p->cs * 1.0e7;
+    const double pi = 3.14159265358979323846, ap2 = (double)p->apix * (double)p->apix;
+    c.a = ff_from_double(pi * lam / ap2);
+    c.b = ff_from_double(0.5 * pi * cs * lam * lam * lam / (ap2 * ap2));
     c.ampwg = p->ampwg;
     c.sq = (float)std::sqrt(1.0 - (double)p->ampwg * p->ampwg);
-    c.apix = p->apix;
     c.df_scale = p->df_scale;
     c.corr_cycles = p->corr_cycles;
     c.corr_step = p->corr_step;
     return c;
 }
 
-// getCTF2d (icosprocess/SRC/getCTF2d.f90:17-27) at centred frequency
-// (hx, ky); the phase is evaluated in float64 so that its ~100 rad
-// argument keeps full precision.
+// getCTF2d (icosprocess/SRC/getCTF2d.f90:17-27) at centred frequency (hx, ky),
+// in float32 and float-float arithmetic.  With n = hx^2 + ky^2, s = n / nx^2
+// (the squared frequency times apix^2) and phi = atan2(ky, hx), the
+// reference's defocus df1 cos^2(phi - ast) + df2 sin^2(phi - ast) is
+// dfm + dfd cos 2(phi - ast), and n cos 2(phi - ast) = (hx^2 - ky^2) cos 2ast
+// + 2 hx ky sin 2ast, so no angle of the pixel is formed; the phase
+// a (df s) - b s^2 (about 100 rad at high frequency) is carried as a
+// float-float and reduced modulo 2 pi before its sine and cosine.  Its
+// error is about 2e-7 rad (3e-6 rad with astigmatism, from cos 2ast and
+// sin 2ast in float32).
 __device__ __forceinline__ float ctf_value(int hx, int ky, int nx, float df1, float df2, float astig_deg,
                                            const CtfConst& c) {
-    const double PI = 3.14159265358979323846;
-    double ph = atan2((double)ky, (double)hx);
-    if (ph < 0) ph += 2.0 * PI;
-    ph -= (double)astig_deg * (PI / 180.0);
-    double cp = cos(ph), sp = sin(ph);
-    double df = (double)df1 * c.df_scale * cp * cp + (double)df2 * c.df_scale * sp * sp;
-    double res = sqrt((double)(hx * hx + ky * ky)) / (double)nx / (double)c.apix;
-    double r2 = res * res;
-    double phase = 2.0 * PI * (-c.cs * c.lam * c.lam * c.lam * r2 * r2 / 4.0 + df * c.lam * r2 / 2.0);
-    double s, co;
-    sincos(phase, &s, &co);
-    return (float)(-((double)c.sq * s + (double)c.ampwg * co));
+    const int n = hx * hx + ky * ky;
+    if (n == 0) return -c.ampwg;     // phase 0: -(sq sin 0 + ampwg cos 0)
+    const ff32 inv = ff_recip_int(nx * nx);
+    const ff32 u = ff_mul(ff32{(float)n, 0.f}, inv);
+    // dfm = (df1 + df2) / 2 and dfd = (df1 - df2) / 2 in A, as float-floats
+    ff32 dfm = ff_two_sum(df1, df2), dfd = ff_two_sum(df1, -df2);
+    dfm = ff_mul(ff32{0.5f * dfm.h, 0.5f * dfm.l}, ff32{c.df_scale, 0.f});
+    dfd = ff_mul(ff32{0.5f * dfd.h, 0.5f * dfd.l}, ff32{c.df_scale, 0.f});
+    float s2a = 0.f, c2a = 1.f;
+    if (astig_deg != 0.f) sincosf(astig_deg * 0.0349065850398865915f, &s2a, &c2a);
+    const ff32 q = ff_add(ff_two_prod((float)(hx * hx - ky * ky), c2a), ff_two_prod((float)(2 * hx * ky), s2a));
+    const ff32 dfs = ff_add(ff_mul(dfm, u), ff_mul(ff_mul(dfd, q), inv));
+    const ff32 ph1 = ff_mul(c.a, dfs), ph2 = ff_mul(c.b, ff_mul(u, u));
+    const float ph = ff_reduce_2pi(ff_add(ph1, ff32{-ph2.h, -ph2.l}));
+    float sn, co;
+    sincosf(ph, &sn, &co);
+    return -(c.sq * sn + c.ampwg * co);
 }
 
 // CTF factors of one inserted sample: fa, fb take ca and wg takes cw.
@@ -75,15 +144,16 @@ __device__ __forceinline__ void ctf_factors(int hx, int ky, int nx, const float*
 }
 
 // exp(-i * 2 pi * ((x0 + s*nx/2) hx + (y0 + s*nx/2) ky) / nx) with s = recentre.
-// The nx/2 part contributes exactly (-1)^(hx+ky); the rest is reduced in
-// float64 before sincospi so the large-argument rounding of the float
-// formula in img2d_FT_centshift.f90:11 does not enter.
+// The nx/2 part contributes exactly (-1)^(hx+ky); the rest, (x0 hx + y0 ky) / nx
+// turns, is formed as a float-float (exact products, one rounding of about
+// 2^-48 relative) and reduced to [-1/2, 1/2] turn before sincospi, so the
+// large-argument rounding of the float formula in img2d_FT_centshift.f90:11
+// does not enter (error about 1e-7 rad).
 __device__ __forceinline__ float2 shift_phase(int hx, int ky, int nx, float x0, float y0, int recentre) {
-    double t = ((double)x0 * hx + (double)y0 * ky) / (double)nx;   // turns
-    t -= floor(t);
-    double s, c;
-    sincospi(2.0 * t, &s, &c);
-    float fc = (float)c, fs = (float)s;
+    const ff32 t = ff_mul(ff_add(ff_two_prod(x0, (float)hx), ff_two_prod(y0, (float)ky)), ff_recip_int(nx));
+    const float f = __fadd_rn(__fsub_rn(t.h, __builtin_rintf(t.h)), t.l);   // turns, |f| <= 1/2 + tiny
+    float fs, fc;
+    sincospif(2.0f * f, &fs, &fc);
     // even nx: (nx/2)(hx+ky)/nx = (hx+ky)/2 turns, an odd sum flips the sign
     if (recentre && ((hx + ky) & 1)) {
         fc = -fc;
