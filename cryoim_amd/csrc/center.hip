@@ -33,32 +33,38 @@ __device__ __forceinline__ float edge_cos(float ra, float R1) {
 
 // ---- model: img3d_mask, checkerboard, forward 3D FFT, checkerboard ------
 
-// Shell sum and count of img3d_mask.f90:26-34 (float64).
+// Shell sum and count of img3d_mask.f90:26-34: each block's compensated
+// float32 partial (sum, count) goes to part[4 blockIdx.x ..]; k_sum_partials
+// adds them in block order.
 __global__ void __launch_bounds__(256) k_mask3d_sum(const float* __restrict__ v, int n, float R1, float R2,
-                                                    double* __restrict__ acc) {
-    __shared__ double sh[4];
+                                                    float* __restrict__ part) {
+    __shared__ ff32 sh[4];
     const long long n2 = (long long)n * n, N = n2 * n;
     const float c = (float)(n / 2);
-    double s = 0.0, cnt = 0.0;
+    ff32 s{0.f, 0.f};
+    int cnt = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
         const float iz = (float)(i / n2), iy = (float)((i / n) % n), ix = (float)(i % n);
         const float zz = (-c + iz) * (-c + iz), yy = (-c + iy) * (-c + iy), xx = (-c + ix) * (-c + ix);
         const float ra = sqrtf(xx + yy + zz);
         if (ra > R1 && ra <= R2) {
-            s += v[i];
-            cnt += 1.0;
+            s = ff_add(s, ff32{v[i], 0.f});
+            ++cnt;
         }
     }
-    s = block_sum(s, sh);
-    cnt = block_sum(cnt, sh);
+    s = block_sum_ff(s, sh);
+    const ff32 ct = block_sum_ff(ff32{(float)cnt, 0.f}, sh);
     if (threadIdx.x == 0) {
-        atomicAdd(&acc[0], s);
-        atomicAdd(&acc[1], cnt);
+        float* q = part + 4 * (long long)blockIdx.x;
+        q[0] = s.h;
+        q[1] = s.l;
+        q[2] = ct.h;
+        q[3] = ct.l;
     }
 }
 
 // img3d_mask.f90:36-49 (when R2 > 0), then (-1)^(x+y+z), as complex (v, 0).
-__global__ void k_mask3d_apply(const float* __restrict__ v, int n, float R1, float R2, const double* __restrict__ acc,
+__global__ void k_mask3d_apply(const float* __restrict__ v, int n, float R1, float R2, const float* __restrict__ acc,
                                float2* __restrict__ out) {
     const long long n2 = (long long)n * n, N = n2 * n;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,12 +75,13 @@ __global__ void k_mask3d_apply(const float* __restrict__ v, int n, float R1, flo
         const float c = (float)(n / 2);
         const float zz = (-c + iz) * (-c + iz), yy = (-c + iy) * (-c + iy), xx = (-c + ix) * (-c + ix);
         const float ra = sqrtf(xx + yy + zz);
-        const double summ = acc[1] != 0.0 ? acc[0] / (double)(float)acc[1] : 0.0;
+        // acc: the shell's sum (hi, lo) and count (hi, lo); the count enters as a float, as there
+        const float summ = acc[2] != 0.f ? __fdiv_rn(__fadd_rn(acc[0], acc[1]), acc[2]) : 0.f;
         if (ra > R1 && ra <= R2) {
             const float e = edge_cos(ra, R1);
-            d = (float)((double)(d * e) + (double)(1.0f - e) * summ);
+            d = __builtin_fmaf(1.0f - e, summ, d * e);
         }
-        if (ra > R2) d = (float)summ;
+        if (ra > R2) d = summ;
     }
     if ((ix + iy + iz) & 1) d = -d;
     out[i] = make_float2(d, 0.f);
@@ -117,7 +124,7 @@ struct MaskGeom {
 // the input (ring mean, then mean, then deviation, then the write), so the
 // stage works in place on cplx.
 __global__ void __launch_bounds__(1024) k_stage(StageArgs a) {
-    __shared__ double sh[16];
+    __shared__ ff32 sh[16];
     const int box = a.box;
     const long long N = (long long)box * box;
     const long long base = (long long)blockIdx.x * N;
@@ -129,7 +136,7 @@ __global__ void __launch_bounds__(1024) k_stage(StageArgs a) {
     };
     // img2d_mask.f90:11-34
     MaskGeom g{0.f, 0.f, 0.f, 0.f};
-    double summ = 0.0;
+    float summ = 0.f;
     if (a.mask_mode) {
         g.R1 = a.mask_r - (float)(kHalfw / 2);
         if (g.R1 <= 0.f) g.R1 = 0.f;
@@ -138,17 +145,18 @@ __global__ void __launch_bounds__(1024) k_stage(StageArgs a) {
         const float ox = a.mask_mode == 1 ? pr[CIM_P_X] : 0.f, oy = a.mask_mode == 1 ? pr[CIM_P_Y] : 0.f;
         g.x0 = ox + (float)(box / 2);
         g.y0 = oy + (float)(box / 2);
-        double s = 0.0, c = 0.0;
+        ff32 s{0.f, 0.f};
+        int c = 0;
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
             const float ra = g.ra((int)(i % box), (int)(i / box));
             if (ra > g.R1 && ra <= g.R2) {
-                s += value(i);
-                c += 1.0;
+                s = ff_add(s, ff32{value(i), 0.f});
+                ++c;
             }
         }
-        s = block_sum(s, sh);
-        c = block_sum(c, sh);
-        summ = c != 0.0 ? s / (double)(float)c : 0.0;
+        s = block_sum_ff(s, sh);
+        const ff32 ct = block_sum_ff(ff32{(float)c, 0.f}, sh);
+        summ = ct.h != 0.f ? __fdiv_rn(__fadd_rn(s.h, s.l), ct.h) : 0.f;
     }
     auto masked = [&](long long i) -> float {
         float v = value(i);
@@ -156,24 +164,27 @@ __global__ void __launch_bounds__(1024) k_stage(StageArgs a) {
             const float ra = g.ra((int)(i % box), (int)(i / box));
             if (ra > g.R1 && ra <= g.R2) {
                 const float e = edge_cos(ra, g.R1);
-                v = (float)((double)(v * e) + (double)(1.0f - e) * summ);
+                v = __builtin_fmaf(1.0f - e, summ, v * e);
             }
-            if (ra > g.R2) v = (float)summ;
+            if (ra > g.R2) v = summ;
         }
         return v;
     };
-    // img2d_norm.f90:1-22 (sums in float64)
+    // img2d_norm.f90:1-22 (compensated float32 sums)
     float avg = 0.f, dev = 1.f;
     if (a.do_norm) {
-        double s = 0.0;
-        for (long long i = threadIdx.x; i < N; i += blockDim.x) s += masked(i);
-        avg = (float)(block_sum(s, sh) / (double)N);
-        double q = 0.0;
+        ff32 s{0.f, 0.f};
+        for (long long i = threadIdx.x; i < N; i += blockDim.x) s = ff_add(s, ff32{masked(i), 0.f});
+        s = block_sum_ff(s, sh);
+        const ff32 m = ff_mul(s, ff_recip_int((int)N));
+        avg = __fadd_rn(m.h, m.l);
+        ff32 q{0.f, 0.f};
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
-            const double d = (double)(masked(i) - avg);
-            q += d * d;
+            const float d = masked(i) - avg;
+            q = ff_add(q, ff_two_prod(d, d));
         }
-        dev = (float)sqrt(block_sum(q, sh) / (double)N);
+        q = block_sum_ff(q, sh);
+        dev = sqrtf(ff_mul(q, ff_recip_int((int)N)).h);
     }
     __syncthreads();   // every pass has read the input before it is overwritten
     for (long long i = threadIdx.x; i < N; i += blockDim.x) {
@@ -534,9 +545,10 @@ int cim_center_model(cim_ctx* ctx, const cim_params* p, const float* model) {
     CIM_ALLOC(ctx->img, sizeof(float) * N);
     CIM_HIP(hipMemcpyAsync(ctx->img.ptr, model, sizeof(float) * N, hipMemcpyHostToDevice, ctx->stream));
     CIM_ALLOC(ctx->cen.model, sizeof(float2) * N);
-    CIM_ALLOC(ctx->cen.acc, sizeof(double) * 2);
-    const double zero[2] = {0.0, 0.0};
-    CIM_HIP(hipMemcpyAsync(ctx->cen.acc.ptr, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+    // shell sum partials of k_mask3d_sum's 1024 blocks, then their total (sum hi, lo, count hi, lo)
+    CIM_ALLOC(ctx->cen.acc, sizeof(float) * 4 * (1024 + 1));
+    CIM_HIP(hipMemsetAsync(ctx->cen.acc.ptr, 0, sizeof(float) * 4 * (1024 + 1), ctx->stream));
+    float* acc_total = ctx->cen.acc.as<float>() + 4 * 1024;
     // img3d_mask.f90:11-17 radii; R2 = 0 switches the mask off
     float R1 = 0.f, R2 = 0.f;
     if (p->mask_on) {
@@ -545,12 +557,15 @@ int cim_center_model(cim_ctx* ctx, const cim_params* p, const float* model) {
         R2 = p->imgmask + (float)(kHalfw / 2);
         if (R2 > (float)(n / 2)) R2 = (float)(n / 2);
         hipLaunchKernelGGL(k_mask3d_sum, dim3(1024), dim3(256), 0, ctx->stream, ctx->img.as<float>(), n, R1, R2,
-                           ctx->cen.acc.as<double>());
+                           ctx->cen.acc.as<float>());
+        CIM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->cen.acc.as<float>(), 1024,
+                           acc_total);
         CIM_HIP(hipGetLastError());
     }
     const unsigned nb = (unsigned)((N + 255) / 256);
     hipLaunchKernelGGL(k_mask3d_apply, dim3(nb), dim3(256), 0, ctx->stream, ctx->img.as<float>(), n, R1, R2,
-                       ctx->cen.acc.as<double>(), ctx->cen.model.as<float2>());
+                       acc_total, ctx->cen.model.as<float2>());
     CIM_HIP(hipGetLastError());
     int rc = fft3d_dev(ctx, ctx->cen.model.as<float2>(), n, 1, 0);
     if (rc) return rc;

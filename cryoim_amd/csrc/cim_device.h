@@ -197,4 +197,47 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return r;
 }
 
+// Block-wide compensated sum of each thread's float-float partial (blockDim a
+// multiple of 64, <= 1024; sh holds 16 entries): float32 arithmetic only, in a
+// fixed order (an xor tree over the lanes -- ff_add is commutative, so every
+// lane holds the same value -- then the waves in order), so the result does
+// not depend on timing.  Every thread returns the total.
+__device__ __forceinline__ ff32 block_sum_ff(ff32 v, ff32* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = ff_add(v, ff32{__shfl_xor(v.h, o, 64), __shfl_xor(v.l, o, 64)});
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    ff32 r = sh[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 1; i < nw; ++i) r = ff_add(r, sh[i]);
+    return r;
+}
+
+// Sum of nblk partials {sum hi, lo, count hi, lo} (4 floats each, written by
+// one block each) in a fixed order: out = {sum hi, lo, count hi, lo}.  One
+// block of 256 threads.  (static: each translation unit launches its own copy)
+static __global__ void __launch_bounds__(256) k_sum_partials(const float* __restrict__ part, int nblk,
+                                                      float* __restrict__ out) {
+    __shared__ ff32 sh[4];
+    ff32 s{0.f, 0.f}, c{0.f, 0.f};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        s = ff_add(s, ff32{part[4 * b], part[4 * b + 1]});
+        c = ff_add(c, ff32{part[4 * b + 2], part[4 * b + 3]});
+    }
+    s = block_sum_ff(s, sh);
+    c = block_sum_ff(c, sh);
+    if (threadIdx.x == 0) {
+        out[0] = s.h;
+        out[1] = s.l;
+        out[2] = c.h;
+        out[3] = c.l;
+    }
+}
+
+
+// x - a for a float-float a, rounded to float
+__device__ __forceinline__ float ff_sub_from(float x, ff32 a) { return __fsub_rn(__fsub_rn(x, a.h), a.l); }
+
 }  // namespace cim

@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(256) k_lbox_shift(const float* __restrict__ im
 // normalised values (three passes; the box stays in L2).
 __global__ void __launch_bounds__(256) k_lbox_cut(const float* __restrict__ x, const float* __restrict__ rec,
                                                   int box, int sub, int nops, float* __restrict__ out) {
-    __shared__ double sh[4];
+    __shared__ ff32 sh[4];
     const int t = blockIdx.x;
     const int i = t / nops;
     const float* o = rec + (long long)CIM_LBOX_REC * t;
@@ -226,7 +226,8 @@ __global__ void __launch_bounds__(256) k_lbox_cut(const float* __restrict__ x, c
         const float* row = img + (long long)wrap(ky0 + r) * box;
         for (int c = tx; c < sub; c += 64) s += row[wrap(hx0 + c)];
     }
-    const float avg = (float)(block_sum((double)s, sh) / ((double)sub * sub));
+    const ff32 sa = ff_mul(block_sum_ff(ff32{s, 0.f}, sh), ff_recip_int(sub * sub));
+    const float avg = __fadd_rn(sa.h, sa.l);
     float d = 0.f;
     for (int r = ty; r < sub; r += ny) {
         const float* row = img + (long long)wrap(ky0 + r) * box;
@@ -235,7 +236,7 @@ __global__ void __launch_bounds__(256) k_lbox_cut(const float* __restrict__ x, c
             d += v * v;
         }
     }
-    const float dev = (float)sqrt(block_sum((double)d, sh) / ((double)sub * sub));
+    const float dev = sqrtf(ff_mul(block_sum_ff(ff32{d, 0.f}, sh), ff_recip_int(sub * sub)).h);
     for (int r = ty; r < sub; r += ny) {
         const float* row = img + (long long)wrap(ky0 + r) * box;
         for (int c = tx; c < sub; c += 64) dst[(long long)r * sub + c] = (row[wrap(hx0 + c)] - avg) / dev;

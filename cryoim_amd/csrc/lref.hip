@@ -23,28 +23,43 @@ int check_params_public(cim_ctx* ctx, const cim_params* p);
 
 // ---- model preparation: softmask (img3d_softmask.f90), checkerboard, FFT ----
 
-__global__ void k_softmask_sum(const float* __restrict__ v, int n, float inn, float out, double* __restrict__ acc) {
+// Shell sum and count of img3d_softmask.f90: each block's compensated float32
+// partial (sum, count) goes to part[4 blockIdx.x ..]; k_sum_partials adds them.
+__global__ void __launch_bounds__(256) k_softmask_sum(const float* __restrict__ v, int n, float inn, float out,
+                                                      float* __restrict__ part) {
+    __shared__ ff32 sh[4];
     const long long N = (long long)n * n * n, n2 = (long long)n * n;
-    double s = 0.0, c = 0.0;
+    ff32 s{0.f, 0.f};
+    int c = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
         const int iz = (int)(i / n2) - n / 2, iy = (int)((i / n) % n) - n / 2, ix = (int)(i % n) - n / 2;
         const float r = sqrtf((float)(ix * ix + iy * iy + iz * iz));
-        if (r > inn && r <= out) { s += v[i]; c += 1.0; }
+        if (r > inn && r <= out) { s = ff_add(s, ff32{v[i], 0.f}); ++c; }
     }
-    s = wave_sum(s);
-    c = wave_sum(c);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&acc[0], s);
-        atomicAdd(&acc[1], c);
+    s = block_sum_ff(s, sh);
+    const ff32 ct = block_sum_ff(ff32{(float)c, 0.f}, sh);
+    if (threadIdx.x == 0) {
+        float* q = part + 4 * (long long)blockIdx.x;
+        q[0] = s.h;
+        q[1] = s.l;
+        q[2] = ct.h;
+        q[3] = ct.l;
     }
 }
 
 __global__ void k_softmask_apply(const float* __restrict__ v, int n, float inn, float out,
-                                 const double* __restrict__ acc, float2* __restrict__ dst) {
+                                 const float* __restrict__ acc, float2* __restrict__ dst) {
     const long long N = (long long)n * n * n, n2 = (long long)n * n;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const float summ = acc[1] != 0.0 ? (float)(acc[0] / acc[1]) : 0.f;
+    // acc: the shell's sum (hi, lo) and count (hi, lo): their ratio in float-float, rounded
+    float summ = 0.f;
+    if (acc[2] != 0.f) {
+        const float q = __fdiv_rn(acc[0], acc[2]);
+        // one correction step: (sum - q count) / count
+        const ff32 qc = ff_mul(ff32{q, 0.f}, ff32{acc[2], acc[3]});
+        summ = __fadd_rn(q, __fdiv_rn(__fadd_rn(__fsub_rn(acc[0], qc.h), __fsub_rn(acc[1], qc.l)), acc[2]));
+    }
     const int Z = (int)(i / n2), Y = (int)((i / n) % n), X = (int)(i % n);
     const int iz = Z - n / 2, iy = Y - n / 2, ix = X - n / 2;
     const float r = sqrtf((float)(ix * ix + iy * iy + iz * iz));
@@ -72,19 +87,20 @@ __global__ void k_checker3d(float2* __restrict__ v, int n) {
 // with wrap-around (refine_PR.f90:87-126, img2d_local_box.f90:6-17).
 __global__ void __launch_bounds__(1024) k_cut_norm(const float* __restrict__ imgs, int n0, const float* __restrict__ parts,
                                                    int n, float* __restrict__ out) {
-    __shared__ double sh[32];
+    __shared__ ff32 sh[16];
     const long long N0 = (long long)n0 * n0;
     const float* img = imgs + (long long)blockIdx.x * N0;
     const float* pr = parts + (long long)blockIdx.x * CIM_PART_STRIDE;
-    double s = 0.0;
-    for (long long i = threadIdx.x; i < N0; i += blockDim.x) s += img[i];
-    const double avg = block_sum(s, sh) / (double)N0;
-    double q = 0.0;
+    // mean and rms in compensated float32
+    ff32 s{0.f, 0.f};
+    for (long long i = threadIdx.x; i < N0; i += blockDim.x) s = ff_add(s, ff32{img[i], 0.f});
+    const ff32 avg = ff_mul(block_sum_ff(s, sh), ff_recip_int((int)N0));
+    ff32 q{0.f, 0.f};
     for (long long i = threadIdx.x; i < N0; i += blockDim.x) {
-        const double d = (double)img[i] - avg;
-        q += d * d;
+        const float d = ff_sub_from(img[i], avg);
+        q = ff_add(q, ff_two_prod(d, d));
     }
-    const double inv = 1.0 / sqrt(block_sum(q, sh) / (double)N0);
+    const float inv = 1.0f / sqrtf(ff_mul(block_sum_ff(q, sh), ff_recip_int((int)N0)).h);
     const int ix0 = (int)pr[CIM_P_BOXX], iy0 = (int)pr[CIM_P_BOXY];
     float* o = out + (long long)blockIdx.x * n * n;
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
@@ -93,7 +109,7 @@ __global__ void __launch_bounds__(1024) k_cut_norm(const float* __restrict__ img
         if (hx >= n0) hx -= n0;
         if (ky < 0) ky += n0;
         if (ky >= n0) ky -= n0;
-        o[i] = (float)(((double)img[(long long)ky * n0 + hx] - avg) * inv);
+        o[i] = ff_sub_from(img[(long long)ky * n0 + hx], avg) * inv;
     }
 }
 
@@ -556,16 +572,19 @@ int cim_lref_model(cim_ctx* ctx, const cim_params* p, const float* vol) {
     CIM_ALLOC(ctx->img, sizeof(float) * N);
     CIM_HIP(hipMemcpyAsync(ctx->img.ptr, vol, sizeof(float) * N, hipMemcpyHostToDevice, ctx->stream));
     CIM_ALLOC(ctx->lref.model, sizeof(float2) * N);
-    CIM_ALLOC(ctx->lref.acc, sizeof(double) * 2);
-    CIM_HIP(hipMemsetAsync(ctx->lref.acc.ptr, 0, sizeof(double) * 2, ctx->stream));
+    // shell sum partials of k_softmask_sum's 1024 blocks, then their total (sum hi, lo, count hi, lo)
+    CIM_ALLOC(ctx->lref.acc, sizeof(float) * 4 * (1024 + 1));
+    float* acc_total = ctx->lref.acc.as<float>() + 4 * 1024;
     float inn = p->imgmask - 3.f, out = p->imgmask + 3.f;
     if (inn <= 0.f) inn = 0.f;
     if (out > (float)(n / 2)) out = (float)(n / 2);
     hipLaunchKernelGGL(k_softmask_sum, dim3(1024), dim3(256), 0, ctx->stream, ctx->img.as<float>(), n, inn, out,
-                       ctx->lref.acc.as<double>());
+                       ctx->lref.acc.as<float>());
+    CIM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->lref.acc.as<float>(), 1024, acc_total);
     CIM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_softmask_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream,
-                       ctx->img.as<float>(), n, inn, out, ctx->lref.acc.as<double>(), ctx->lref.model.as<float2>());
+                       ctx->img.as<float>(), n, inn, out, acc_total, ctx->lref.model.as<float2>());
     CIM_HIP(hipGetLastError());
     int rc = fft3d_dev(ctx, ctx->lref.model.as<float2>(), n, 1, 0);
     if (rc) return rc;

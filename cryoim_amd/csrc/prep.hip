@@ -42,26 +42,31 @@ int deinterleave(cim_ctx* ctx, const float2* in, float* re, float* im, long long
 
 // [norm] -> [mask] -> [norm] -> checkerboard, real image in, complex (v, 0)
 // out.  do_norm = 1 normalises before the mask (reconstruct.f90:79-92),
-// 2 after it (local_reconstruct/SRC/reconstruct_substep.f90:93-94).
+// 2 after it (local_reconstruct/SRC/reconstruct_substep.f90:93-94).  Means,
+// deviations and the ring mean are compensated float32 sums (block_sum_ff).
 __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ imgs, const float* __restrict__ parts,
                                                     float2* __restrict__ out, int box, int do_norm, int do_mask,
                                                     float mask_r) {
-    __shared__ double sh[32];
+    __shared__ ff32 sh[16];
     const long long N = (long long)box * box;
     const float* img = imgs + (long long)blockIdx.x * N;
     float2* o = out + (long long)blockIdx.x * N;
     const float* pr = parts + (long long)blockIdx.x * CIM_PART_STRIDE;
-    double avg = 0.0, inv_dev = 1.0;
+    const float invN = 1.0f / (float)N;
+    ff32 avg{0.f, 0.f};
+    float inv_dev = 1.f;
     if (do_norm == 1) {
-        double s = 0.0;
-        for (long long i = threadIdx.x; i < N; i += blockDim.x) s += img[i];
-        avg = block_sum(s, sh) / (double)N;
-        double q = 0.0;
+        ff32 s{0.f, 0.f};
+        for (long long i = threadIdx.x; i < N; i += blockDim.x) s = ff_add(s, ff32{img[i], 0.f});
+        s = block_sum_ff(s, sh);
+        avg = ff_mul(s, ff_recip_int((int)N));
+        ff32 q{0.f, 0.f};
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
-            double d = (double)img[i] - avg;
-            q += d * d;
+            const float d = ff_sub_from(img[i], avg);
+            q = ff_add(q, ff_two_prod(d, d));
         }
-        inv_dev = 1.0 / sqrt(block_sum(q, sh) / (double)N);
+        q = block_sum_ff(q, sh);
+        inv_dev = 1.0f / sqrtf(q.h * invN);
     }
     float R1 = 0.f, R2 = 0.f, ring_mean = 0.f;
     float x0 = 0.f, y0 = 0.f;
@@ -73,24 +78,25 @@ __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ im
         if (R2 > (float)(box / 2)) R2 = (float)(box / 2);
         x0 = pr[CIM_P_MASKX] + (float)(box / 2);
         y0 = pr[CIM_P_MASKY] + (float)(box / 2);
-        double s = 0.0, c = 0.0;
+        ff32 s{0.f, 0.f};
+        int c = 0;
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
             int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
             float dy = -y0 + (float)iy, dx = -x0 + (float)ix;
             float ra = sqrtf(dy * dy + dx * dx);
             if (ra > R1 && ra <= R2) {
-                s += ((double)img[i] - avg) * inv_dev;
-                c += 1.0;
+                s = ff_add(s, ff_two_prod(ff_sub_from(img[i], avg), inv_dev));
+                ++c;
             }
         }
-        double ssum = block_sum(s, sh);
-        double cnt = block_sum(c, sh);
-        ring_mean = cnt != 0.0 ? (float)(ssum / cnt) : 0.f;
+        s = block_sum_ff(s, sh);
+        const ff32 cnt = block_sum_ff(ff32{(float)c, 0.f}, sh);
+        ring_mean = cnt.h != 0.f ? __fdiv_rn(s.h, cnt.h) : 0.f;
     }
     const float pi = 3.14159265358979323846f;
     auto masked = [&](long long i) -> float {
         int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
-        float v = (float)(((double)img[i] - avg) * inv_dev);
+        float v = ff_sub_from(img[i], avg) * inv_dev;
         if (do_mask) {
             float dy = -y0 + (float)iy, dx = -x0 + (float)ix;
             float ra = sqrtf(dy * dy + dx * dx);
@@ -102,22 +108,25 @@ __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ im
         }
         return v;
     };
-    double avg2 = 0.0, inv_dev2 = 1.0;
+    ff32 avg2{0.f, 0.f};
+    float inv_dev2 = 1.f;
     if (do_norm == 2) {
-        double s = 0.0;
-        for (long long i = threadIdx.x; i < N; i += blockDim.x) s += masked(i);
-        avg2 = block_sum(s, sh) / (double)N;
-        double q = 0.0;
+        ff32 s{0.f, 0.f};
+        for (long long i = threadIdx.x; i < N; i += blockDim.x) s = ff_add(s, ff32{masked(i), 0.f});
+        s = block_sum_ff(s, sh);
+        avg2 = ff_mul(s, ff_recip_int((int)N));
+        ff32 q{0.f, 0.f};
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
-            double d = (double)masked(i) - avg2;
-            q += d * d;
+            const float d = ff_sub_from(masked(i), avg2);
+            q = ff_add(q, ff_two_prod(d, d));
         }
-        inv_dev2 = 1.0 / sqrt(block_sum(q, sh) / (double)N);
+        q = block_sum_ff(q, sh);
+        inv_dev2 = 1.0f / sqrtf(q.h * invN);
     }
     for (long long i = threadIdx.x; i < N; i += blockDim.x) {
         int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
         float v = masked(i);
-        if (do_norm == 2) v = (float)(((double)v - avg2) * inv_dev2);
+        if (do_norm == 2) v = ff_sub_from(v, avg2) * inv_dev2;
         if ((ix + iy) & 1) v = -v;
         o[i] = make_float2(v, 0.f);
     }
@@ -158,52 +167,66 @@ int centre_shift(cim_ctx* ctx, float2* ft, int box, int n, const float* parts_de
 
 // FT_amp_expfit (FT_amp_expfit.f90:2-47) of nitems centred transforms:
 // item i is at ft + i*stride with rows of ld elements and the zero
-// frequency at (cidx, cidx).
+// frequency at (cidx, cidx).  Thread k owns ring minR + k: it walks the
+// half-plane rows ky >= 0 and, in each, the hx interval that can round to
+// its radius (int(sqrt(hx^2 + ky^2) + 0.5), tested exactly as the reference
+// bins), summing |F| and the radii in compensated float32 in a fixed order;
+// the least-squares line through (radius, ln mean |F|) is solved in float32.
 __global__ void __launch_bounds__(256) k_expfit(const float2* __restrict__ ft, long long stride, int ld, int cidx,
                                                 int minR, int maxR, float* __restrict__ fcut) {
-    extern __shared__ double acc[];   // Favg[nr], resLN[nr], Num[nr]
+    extern __shared__ float racc[];   // per ring: mean radius, ln mean |F| (or 0)
     const int nr = maxR - minR + 1;
-    double* Favg = acc;
-    double* resLN = acc + nr;
-    double* Num = acc + 2 * nr;
-    for (int k = threadIdx.x; k < 3 * nr; k += blockDim.x) acc[k] = 0.0;
-    __syncthreads();
+    float* resLN = racc;
+    float* lnF = racc + nr;
     const int item = blockIdx.x;
     const float2* f = ft + (long long)item * stride;
-    const int w = 2 * maxR + 1;
-    const int total = (maxR + 1) * w;
-    for (int q = threadIdx.x; q < total; q += blockDim.x) {
-        int ky = q / w, hx = q % w - maxR;
-        float rad = sqrtf((float)(hx * hx + ky * ky));
-        int ir = (int)(rad + 0.5f);
-        if (ir >= minR && ir <= maxR) {
-            float2 v = f[(long long)(ky + cidx) * ld + (hx + cidx)];
-            double amp = sqrt((double)v.x * v.x + (double)v.y * v.y);
-            atomicAdd(&Favg[ir - minR], amp);
-            atomicAdd(&resLN[ir - minR], (double)rad);
-            atomicAdd(&Num[ir - minR], 1.0);
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+        const int ir = minR + k;
+        ff32 famp{0.f, 0.f}, frad{0.f, 0.f};
+        int num = 0;
+        const float rlo = fmaxf((float)ir - 0.5f, 0.f), rhi = (float)ir + 0.5f;
+        for (int ky = 0; ky <= min(ir + 1, maxR); ++ky) {
+            // |hx| with hx^2 + ky^2 in [rlo^2, rhi^2), widened by one step on each side
+            const float a2 = rlo * rlo - (float)(ky * ky), b2 = rhi * rhi - (float)(ky * ky);
+            if (b2 < 0.f) break;
+            const int h0 = a2 > 0.f ? max((int)sqrtf(a2) - 1, 0) : 0;
+            const int h1 = min((int)sqrtf(b2) + 1, maxR);
+            for (int side = 0; side < 2; ++side) {
+                for (int ah = h0; ah <= h1; ++ah) {
+                    const int hx = side ? -ah : ah;
+                    if (side && ah == 0) continue;
+                    const float rad = sqrtf((float)(hx * hx + ky * ky));
+                    if ((int)(rad + 0.5f) != ir) continue;
+                    const float2 v = f[(long long)(ky + cidx) * ld + (hx + cidx)];
+                    famp = ff_add(famp, ff32{sqrtf(__builtin_fmaf(v.x, v.x, v.y * v.y)), 0.f});
+                    frad = ff_add(frad, ff32{rad, 0.f});
+                    ++num;
+                }
+            }
         }
+        float r = frad.h, F = 0.f;
+        if (num > 0 && famp.h > 0.f) {
+            r = __fdiv_rn(frad.h, (float)num);
+            F = logf(__fdiv_rn(famp.h, (float)num));
+        }
+        resLN[k] = r;
+        lnF[k] = F;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a11 = 0, a12 = 0, a22 = 0, b1 = 0, b2 = 0;
+        ff32 a11{0.f, 0.f}, a12{0.f, 0.f}, b1{0.f, 0.f}, b2{0.f, 0.f};
         for (int k = 0; k < nr; ++k) {
-            double r = resLN[k], F = 0.0;
-            if (Num[k] > 0 && Favg[k] > 0) {
-                r = resLN[k] / Num[k];
-                F = log(Favg[k] / Num[k]);
-            }
-            resLN[k] = r;
-            a11 += r * r;
-            a12 += r;
-            a22 += 1.0;
-            b1 += r * F;
-            b2 += F;
+            const float r = resLN[k], F = lnF[k];
+            a11 = ff_add(a11, ff_two_prod(r, r));
+            a12 = ff_add(a12, ff32{r, 0.f});
+            b1 = ff_add(b1, ff_two_prod(r, F));
+            b2 = ff_add(b2, ff32{F, 0.f});
         }
-        double x2 = (b2 - b1 * a12 / a11) / (a22 - a12 * a12 / a11);
-        double x1 = (b1 - a12 * x2) / a11;
+        const float a22 = (float)nr;
+        const float x2 = (b2.h - b1.h * a12.h / a11.h) / (a22 - a12.h * a12.h / a11.h);
+        const float x1 = (b1.h - a12.h * x2) / a11.h;
         float* out = fcut + (long long)item * nr;
-        for (int k = 0; k < nr; ++k) out[k] = (float)(exp(resLN[k] * x1 + x2) * 0.8);
+        for (int k = 0; k < nr; ++k) out[k] = expf(resLN[k] * x1 + x2) * 0.8f;
     }
 }
 
@@ -212,7 +235,7 @@ int amp_expfit_general(cim_ctx* ctx, const float2* ft, long long stride, int ld,
     if (nitems <= 0) return CIM_OK;
     const int nr = maxR - minR + 1;
     TimedLaunch tl(ctx, "expfit");
-    hipLaunchKernelGGL(k_expfit, dim3(nitems), dim3(256), sizeof(double) * 3 * nr, ctx->stream, ft, stride, ld, cidx,
+    hipLaunchKernelGGL(k_expfit, dim3(nitems), dim3(256), sizeof(float) * 2 * nr, ctx->stream, ft, stride, ld, cidx,
                        minR, maxR, fcut_dev);
     CIM_HIP(hipGetLastError());
     return CIM_OK;
