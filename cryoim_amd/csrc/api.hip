@@ -58,8 +58,8 @@ void resolve_timing(cim_ctx* c) {
 }
 
 int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init, int nops, const double* ops);
-int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_dev, const float* ctf_img_dev,
-                      int apply_ctf, const cim_params* p);
+int rec_insert_ft_dev(cim_ctx* ctx, int n, float2* ft, const float* parts_dev, const float* ctf_img_dev,
+                      int apply_ctf, const cim_params* p, int shift);
 int rec_tnf_io(cim_ctx* ctx, float* fa, float* fb, float* wg, int add);
 int rec_finish_dev(cim_ctx* ctx, float* map_dev);
 
@@ -294,9 +294,8 @@ static int rec_insert_images_dev(cim_ctx* ctx, const cim_params* p, int n, const
     rc = preprocess_images(ctx, p, nu, src, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on, p->mask_on,
                            p->conj_out);
     if (rc) return rc;
-    rc = centre_shift(ctx, ctx->ft.as<float2>(), p->box, nu, ctx->parts.as<float>(), 1);
-    if (rc) return rc;
-    rc = rec_insert_ft_dev(ctx, nu, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p);
+    // the centre shift is applied where the insert stages its sample values
+    rc = rec_insert_ft_dev(ctx, nu, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1);
     if (rc) return rc;
     CIM_HIP(hipStreamSynchronize(ctx->stream));
     return CIM_OK;
@@ -319,9 +318,7 @@ static int rec_insert_records_dev(cim_ctx* ctx, const cim_params* p, int n, cons
     rc = preprocess_images(ctx, p, n, images_dev, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on, p->mask_on,
                            p->conj_out);
     if (rc) return rc;
-    rc = centre_shift(ctx, ctx->ft.as<float2>(), p->box, n, ctx->parts.as<float>(), 1);
-    if (rc) return rc;
-    return rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p);
+    return rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1);
 }
 
 static void step_release(cim_ctx* ctx) {
@@ -438,7 +435,7 @@ int cim_rec_insert_ft(cim_ctx* ctx, int n, const float* ft_re, const float* ft_i
     CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
     rc = interleave(ctx, d, d + N * n, ctx->ft.as<float2>(), N * n);
     if (rc) return rc;
-    rc = rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), dctf, 0, nullptr);
+    rc = rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), dctf, 0, nullptr, 0);
     if (rc) return rc;
     CIM_HIP(hipStreamSynchronize(ctx->stream));
     return CIM_OK;

@@ -303,12 +303,15 @@ constexpr int BRICK = 8;
 // Per-particle sample values {a*ca, b*ca, cw} (ctf_factors) on the (hx, ky) half
 // plane grid hx in [0, maxR], ky in [-maxR, maxR].
 // A sample is inserted iff hx <= hxmax[ky + maxR] (the host's radius test).
+// shift: the transform is not centre-shifted yet; each sample takes the phase
+// of img2d_FT_centshift here (shift_phase, recentred, as k_centshift would).
 // maxbits collects the largest |value| of the batch (float bits; non-negative
 // floats order as unsigned integers).
 __global__ void __launch_bounds__(256) k_insert_vals(const float2* __restrict__ ft, int box, int maxR,
                                                      const int* __restrict__ hxmax, const float* __restrict__ parts,
                                                      const float* __restrict__ ctf_img, int apply_ctf, CtfConst cc,
-                                                     float4* __restrict__ vals, unsigned int* __restrict__ maxbits) {
+                                                     int shift, float4* __restrict__ vals,
+                                                     unsigned int* __restrict__ maxbits) {
     __shared__ float red[4];
     const int p = blockIdx.y;
     const int w = maxR + 1, total = (2 * maxR + 1) * w;
@@ -319,7 +322,11 @@ __global__ void __launch_bounds__(256) k_insert_vals(const float2* __restrict__ 
         if (hx <= hxmax[ky + maxR]) {
             const long long N = (long long)box * box;
             const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
-            const float2 v = ft[(long long)p * N + i2];
+            float2 v = ft[(long long)p * N + i2];
+            if (shift) {
+                const float* pr = parts + (long long)p * CIM_PART_STRIDE;
+                v = apply_shift(v, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
+            }
             float ca = 1.f, cw = 1.f;
             if (ctf_img) {
                 ca = ctf_img[(long long)p * N + i2];
@@ -1164,8 +1171,12 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
 }
 
 // Insert n particles whose centred, centre-shifted FTs are in ft (device).
-int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_dev, const float* ctf_img_dev,
-                      int apply_ctf, const cim_params* p) {
+int centre_shift(cim_ctx* ctx, float2* ft, int box, int n, const float* parts_dev, int recentre);
+
+// shift: ft is not centre-shifted yet (the value staging applies it; the
+// scatter kernel, which reads ft itself, gets it applied first)
+int rec_insert_ft_dev(cim_ctx* ctx, int n, float2* ft, const float* parts_dev, const float* ctf_img_dev,
+                      int apply_ctf, const cim_params* p, int shift) {
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
     if (n <= 0) return CIM_OK;
@@ -1191,6 +1202,10 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
     if (p) cc = make_ctf_const(p);
     const char* mode = getenv("CIM_INSERT");
     if (mode && strcmp(mode, "scatter") == 0 && !r.comp) {
+        if (shift) {
+            const int rc = centre_shift(ctx, ft, r.box, n, parts_dev, 1);
+            if (rc) return rc;
+        }
         TimedLaunch tl(ctx, "insert");
         hipLaunchKernelGGL(k_insert, dim3((r.nsamples + 255) / 256, n), dim3(256), 0, ctx->stream, r.acc.as<float4>(),
                            ft, r.samples.as<int2>(), r.nsamples, ctx->aux2.as<float>(), parts_dev, ctf_img_dev,
@@ -1202,7 +1217,7 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, const float2* ft, const float* parts_
         CIM_ALLOC(r.maxv, sizeof(unsigned int));
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, ctx->stream, r.maxv.as<unsigned int>(), 0u);
         hipLaunchKernelGGL(k_insert_vals, dim3((total + 255) / 256, n), dim3(256), 0, ctx->stream, ft, r.box, r.maxR,
-                           r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, r.vals.as<float4>(),
+                           r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, shift, r.vals.as<float4>(),
                            r.maxv.as<unsigned int>());
         CIM_HIP(hipGetLastError());
         BrickArgs ba;

@@ -105,9 +105,12 @@ __device__ __forceinline__ float2 crop_bilin(const float2* __restrict__ crop, in
 
 // Crop rel [-(R+1), R+2] around the centre of n centred FTs, optionally
 // multiplied by the CTF of particle record (item / T) (template x CTF case:
-// item = p*T + t, source = src[t]).
+// item = p*T + t, source = src[t]), or (shift_parts) centre-shifted by the
+// phase of img2d_FT_centshift of record item, as k_centshift would (the same
+// shift_phase, so the same bits) but on the crop's pixels only.
 __global__ void k_crop(const float2* __restrict__ src, long long src_stride, int box, int R, int Dc,
-                       float2* __restrict__ dst, int T, const float* __restrict__ ctf_parts, CtfConst cc, int use_ctf) {
+                       float2* __restrict__ dst, int T, const float* __restrict__ ctf_parts, CtfConst cc, int use_ctf,
+                       const float* __restrict__ shift_parts) {
     const int item = blockIdx.y;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Dc * Dc) return;
@@ -119,6 +122,10 @@ __global__ void k_crop(const float2* __restrict__ src, long long src_stride, int
     const float2* s = src + (long long)(use_ctf ? t : item) * src_stride;
     float2 v = make_float2(0.f, 0.f);
     if (ax >= 0 && ax < box && ay >= 0 && ay < box) v = s[(long long)ay * box + ax];
+    if (shift_parts) {
+        const float* pr = shift_parts + (long long)item * CIM_PART_STRIDE;
+        v = apply_shift(v, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
+    }
     if (use_ctf) {
         const float* pr = ctf_parts + (long long)p * CIM_PART_STRIDE;
         float cv = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
@@ -1990,11 +1997,10 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         CIM_ALLOC(sb.fcut0, sizeof(float) * nr * nb);
         rc = amp_expfit(ctx, ft, box, nb, p->minR, R, sb.fcut0.as<float>());
         if (rc) return rc;
-        rc = centre_shift(ctx, ft, box, nb, ctx->parts.as<float>(), 1);
-        if (rc) return rc;
+        // the centre shift is applied to the crop's pixels only
         CIM_ALLOC(sb.pcrop, sizeof(float2) * (size_t)Dc * Dc * nb);
         hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream, ft, N, box, R, Dc,
-                           sb.pcrop.as<float2>(), 1, (const float*)nullptr, cc, 0);
+                           sb.pcrop.as<float2>(), 1, (const float*)nullptr, cc, 0, ctx->parts.as<float>());
         CIM_HIP(hipGetLastError());
         CIM_ALLOC(sb.pcrop4, sizeof(float4) * (size_t)Dc * Dc * nb);
         {
@@ -2008,7 +2014,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * nb * T);
             hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb * T), dim3(256), 0, ctx->stream,
                                s.tft.as<float2>(), N, box, R, Dc, sb.tcrop.as<float2>(), T, ctx->parts.as<float>(), cc,
-                               1);
+                               1, (const float*)nullptr);
             CIM_HIP(hipGetLastError());
             CIM_ALLOC(sb.tfcut, sizeof(float) * nr * nb * T);
             rc = amp_expfit_general(ctx, sb.tcrop.as<float2>(), (long long)Dc * Dc, Dc, R + 1, nb * T, p->minR, R,
@@ -2019,7 +2025,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         } else {
             CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * T);
             hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, T), dim3(256), 0, ctx->stream, s.tft.as<float2>(), N,
-                               box, R, Dc, sb.tcrop.as<float2>(), 1, (const float*)nullptr, cc, 0);
+                               box, R, Dc, sb.tcrop.as<float2>(), 1, (const float*)nullptr, cc, 0,
+                               (const float*)nullptr);
             CIM_HIP(hipGetLastError());
             CIM_ALLOC(sb.tfcut, sizeof(float) * nr * T);
             CIM_HIP(hipMemcpyAsync(sb.tfcut.ptr, s.tfcut.ptr, sizeof(float) * nr * T, hipMemcpyDeviceToDevice,
@@ -2643,7 +2650,7 @@ static int refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* i
     if (p->apply_ctf) {
         CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * n * T);
         hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, n * T), dim3(256), 0, ctx->stream, s.tft.as<float2>(), N,
-                           box, R, Dc, sb.tcrop.as<float2>(), T, ctx->parts.as<float>(), cc, 1);
+                           box, R, Dc, sb.tcrop.as<float2>(), T, ctx->parts.as<float>(), cc, 1, (const float*)nullptr);
         CIM_HIP(hipGetLastError());
         CIM_ALLOC(sb.tfcut, sizeof(float) * nr * n * T);
         rc = amp_expfit_general(ctx, sb.tcrop.as<float2>(), (long long)Dc * Dc, Dc, R + 1, n * T, p->minR, R,
