@@ -190,8 +190,10 @@ __device__ __forceinline__ void sample_pos(float FR, float cs, float sn, float c
     const float x = FR * cs + cf, y = FR * sn + cf;
     lx = (int)x;
     ly = (int)y;
-    ax = x - (float)lx;
-    ay = y - (float)ly;
+    // x, y > 0 (the crop's centre lies more than the radius inside): x - (int)x
+    // = x - floor(x), exact, which v_fract_f32 returns in one instruction
+    ax = __builtin_amdgcn_fractf(x);
+    ay = __builtin_amdgcn_fractf(y);
 }
 
 // One sample of a column from its four particle taps against the template
@@ -561,6 +563,8 @@ __global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
     const int ic = a.box / 2;
     const float cf = (float)ic;
     const float d2r = 0.0174532925199432957692f;
+    const int c0 = a.R + 1 - ic;                    // crop index of centred frequency 0, minus ic
+    const unsigned rowb = (unsigned)a.Dc << 4;      // bytes per crop row
     const int ncol = PAIR ? a.Nomega / 2 : a.Nomega;
     int k = a.kwin[cbase + slot] + col;
     if (k >= ncol) k -= ncol;
@@ -625,21 +629,27 @@ __global__ void __launch_bounds__(64) k_cl_rescore(SearchArgs a) {
                 // whose gate is +inf, so it adds nothing.
                 int lx, ly;
                 float axA, ayA, axB, ayB;
+                // taps at a 32-bit byte offset from the particle's crop (under 2^32 bytes), so the loads
+                // take a scalar base and a vector offset
+                auto taps = [&](int tx, int ty, float4& u, float4& w) {
+                    const unsigned off = (__umul24((unsigned)(ty + c0), (unsigned)a.Dc) + (unsigned)(tx + c0)) << 4;
+                    const char* b = reinterpret_cast<const char*>(pc4);
+                    u = *reinterpret_cast<const float4*>(b + off);
+                    w = *reinterpret_cast<const float4*>(b + (off + rowb));
+                };
                 sample_pos((float)(a.minR + (int)tr[0]), cs, sn, cf, lx, ly, axA, ayA);
-                const float4* rA = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
-                float4 uA = rA[0], wA = rA[a.Dc];
+                float4 uA, wA;
+                taps(lx, ly, uA, wA);
                 for (int i = 0; i < nk; i += 2) {
                     const int ib = min(i + 1, nk - 1);
                     sample_pos((float)(a.minR + (int)tr[ib]), cs, sn, cf, lx, ly, axB, ayB);
-                    const float4* rB = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
-                    const float4 uB = rB[0], wB = rB[a.Dc];
+                    float4 uB, wB;
+                    taps(lx, ly, uB, wB);
                     const float4 tva = tl[i];
                     score_sample<PAIR>(make_float2(uA.x, uA.y), make_float2(uA.z, uA.w), make_float2(wA.x, wA.y),
                                        make_float2(wA.z, wA.w), axA, ayA, tva, tva.w, num, den, num2, den2, npass);
                     sample_pos((float)(a.minR + (int)tr[min(i + 2, nk - 1)]), cs, sn, cf, lx, ly, axA, ayA);
-                    rA = pc4 + (long long)(ly - ic + a.R + 1) * a.Dc + (lx - ic + a.R + 1);
-                    uA = rA[0];
-                    wA = rA[a.Dc];
+                    taps(lx, ly, uA, wA);
                     const float4 tvb = tl[ib];
                     score_sample<PAIR>(make_float2(uB.x, uB.y), make_float2(uB.z, uB.w), make_float2(wB.x, wB.y),
                                        make_float2(wB.z, wB.w), axB, ayB, tvb, i + 1 < nk ? tvb.w : INFINITY, num,
