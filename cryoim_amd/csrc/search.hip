@@ -1209,6 +1209,13 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
         const unsigned int* lp = a.lines + ((long long)pt * a.T + t) * 64;
         const int nv = a.lcount[(long long)pt * a.T + t];
         const Entry* tab = static_cast<const Entry*>(a.table) + (long long)(p * a.T + t) * nrow * npsi;
+        // entry of column k on the line: row base (scalar) + a 32-bit byte offset (vector)
+        auto entry = [&](unsigned int pk, int v) -> Entry {
+            const char* row = reinterpret_cast<const char*>(tab + (long long)(pk >> 16) * npsi);
+            int c = (int)(pk & 0xffffu) - ks[v];
+            c += c < 0 ? npsi : 0;
+            return *reinterpret_cast<const Entry*>(row + (unsigned)c * (unsigned)sizeof(Entry));
+        };
         constexpr int G = 8;
         int j = 0;
         for (; j + G <= nv; j += G) {
@@ -1216,14 +1223,8 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const unsigned int pk = lp[j + g];
-                const Entry* row = tab + (long long)(pk >> 16) * npsi;
-                const int j0 = (int)(pk & 0xffffu);
 #pragma unroll
-                for (int v = 0; v < VPT; ++v) {
-                    int c = j0 - ks[v];
-                    c += c < 0 ? npsi : 0;
-                    e[g][v] = row[c];
-                }
+                for (int v = 0; v < VPT; ++v) e[g][v] = entry(pk, v);
             }
             // keep all G x VPT loads in flight before the first use
             __builtin_amdgcn_sched_barrier(0);
@@ -1234,14 +1235,8 @@ __global__ void __launch_bounds__(256) k_cl_gather(GatherArgs a) {
         }
         for (; j < nv; ++j) {
             const unsigned int pk = lp[j];
-            const Entry* row = tab + (long long)(pk >> 16) * npsi;
-            const int j0 = (int)(pk & 0xffffu);
 #pragma unroll
-            for (int v = 0; v < VPT; ++v) {
-                int c = j0 - ks[v];
-                c += c < 0 ? npsi : 0;
-                acc_entry(row[c], num[v], den[v]);
-            }
+            for (int v = 0; v < VPT; ++v) acc_entry(entry(pk, v), num[v], den[v]);
         }
     }
     const float r2d = 57.2957795130823208768f;
