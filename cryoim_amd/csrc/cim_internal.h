@@ -79,7 +79,7 @@ struct RecState {
 
 // Per-batch working set of the searches.
 struct SearchBatch {
-    DevBuf pcrop, pcrop4, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter;
+    DevBuf pcrop, pcrop4, fcut0, tcrop, tfcut, outv, outi, prfull, ops, rows, pts, pbase, tang, fv, fi, counter, ftrow;
     // tabulated search: polar rays of particles / templates, residual tables, candidates
     DevBuf ppol, tpol, table, cand, kwin, cv, ci;
     // coarse-to-fine gather: coarse point list, 3x3 neighbours, per-particle coarse winners and fine lists
@@ -95,7 +95,7 @@ struct SearchBatch {
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
                           &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci, &clist, &nbr, &ccand, &ckwin,
-                          &flist, &fcnt, &lines, &lcount, &ctab, &bound, &fwin, &fwin2, &outi2, &ckwin2, &outv2, &fbl})
+                          &flist, &fcnt, &lines, &lcount, &ctab, &bound, &fwin, &fwin2, &outi2, &ckwin2, &outv2, &fbl, &ftrow})
             b->release();
     }
 };
@@ -237,6 +237,8 @@ int search_candidates_dev(cim_ctx* ctx, const cim_params* p, int n, const float*
                           int ncand, const int* cand, const int* omega);
 // In-place transforms of interleaved complex data on ctx->stream.
 int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_out);
+int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int o0, int Dc, int conj_out,
+                   const float* parts, float2* tmp, float2* crop);
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
 // Fold {fa, fb, wg} (pitch dxy, z >= 0) -> complex volume -> inverse FFT ->
 // real map with checkerboard (inside radius rmask only when rmask >= 0).
@@ -257,6 +259,8 @@ int deinterleave(cim_ctx* ctx, const float2* in, float* re, float* im, long long
 
 // Per-particle preprocessing to a centred FT (prep.hip).
 // mode_norm: normalise first; mask centre from parts[CIM_P_MASKX/Y].
+int prep_images_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, const float* parts_dev,
+                    float2* ft_out, int do_norm, int do_mask);
 int preprocess_images(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev,
                       const float* parts_dev, float2* ft_out, int do_norm, int do_mask,
                       int conj_out);

@@ -10,6 +10,7 @@
 
 #include "cim_internal.h"
 #include "cim_fft.h"
+#include "cim_device.h"
 
 namespace cim {
 
@@ -130,6 +131,104 @@ int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_ou
     rc = lines_pass(ctx, pl, data, (long long)batch * n, 1, 0, n, 1, dir, 1.f, 0);   // rows
     if (rc) return rc;
     return lines_pass(ctx, pl, data, (long long)batch * n, n, 1, n2, n, dir, scale, conj_out);  // columns
+}
+
+// ---- forward 2D transform pruned to a centred crop ---------------------------
+//
+// The transform of each n x n image is needed only on the crop of Dc columns
+// and rows starting at index o0 (the centred frequencies o0 - n/2 ..
+// o0 + Dc - 1 - n/2).  The row pass transforms every row but keeps the Dc
+// outputs of the crop's columns (rows x Dc, to tmp); the column pass
+// transforms those Dc columns and keeps the crop's Dc rows, scaled (1/n^2,
+// conjugated when asked) and, with parts, multiplied by the centre-shift
+// phase of each image's record (shift_phase, as k_centshift).  Every kept
+// value goes through the same passes and arithmetic as in fft2d_dev, so it
+// has the same bits.
+
+// rows: line l = image l / n, row l % n of data; outputs o0 .. o0 + Dc - 1 to tmp[l * Dc ..]
+__global__ void __launch_bounds__(256) k_fft_rows_crop(const float2* __restrict__ data, const float2* __restrict__ tw,
+                                                       int n, FacList fl, long long nlines, int L, int o0, int Dc,
+                                                       float2* __restrict__ tmp) {
+    extern __shared__ float2 lds[];
+    float2* b0 = lds;
+    float2* b1 = lds + (size_t)L * n;
+    const long long l0 = (long long)blockIdx.x * L;
+    const int nl = (int)((nlines - l0) < L ? (nlines - l0) : L);
+    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) b0[idx] = data[l0 * n + idx];
+    for (int idx = nl * n + threadIdx.x; idx < L * n; idx += blockDim.x) b0[idx] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const float2* in = lds_fft(b0, b1, n, L, fl, tw, -1.f);
+    for (int idx = threadIdx.x; idx < nl * Dc; idx += blockDim.x) {
+        const int li = idx / Dc, e = idx - li * Dc;
+        tmp[(l0 + li) * Dc + e] = in[li * n + o0 + e];
+    }
+}
+
+// columns: line l = image l / Dc, column l % Dc of tmp (elements Dc apart);
+// outputs o0 .. o0 + Dc - 1 to crop[image][row][column]
+__global__ void __launch_bounds__(256) k_fft_cols_crop(const float2* __restrict__ tmp, const float2* __restrict__ tw,
+                                                       int n, FacList fl, long long nlines, int L, int o0, int Dc,
+                                                       float scale, int conj, const float* __restrict__ parts,
+                                                       float2* __restrict__ crop) {
+    extern __shared__ float2 lds[];
+    float2* b0 = lds;
+    float2* b1 = lds + (size_t)L * n;
+    const long long l0 = (long long)blockIdx.x * L;
+    const int nl = (int)((nlines - l0) < L ? (nlines - l0) : L);
+    const int lgL = __ffs(L) - 1;
+    for (int idx = threadIdx.x; idx < L * n; idx += blockDim.x) {
+        const int e = idx >> lgL, li = idx & (L - 1);
+        if (li < nl) {
+            const long long l = l0 + li, img = l / Dc;
+            b0[li * n + e] = tmp[(img * n + e) * Dc + (l - img * Dc)];
+        } else {
+            b0[li * n + e] = make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    const float2* in = lds_fft(b0, b1, n, L, fl, tw, -1.f);
+    const float cs = conj ? -scale : scale;
+    const int c = n / 2;
+    for (int idx = threadIdx.x; idx < L * Dc; idx += blockDim.x) {
+        const int e = idx >> lgL, li = idx & (L - 1);
+        if (li >= nl) continue;
+        const long long l = l0 + li, img = l / Dc;
+        const int col = (int)(l - img * Dc);
+        const float2 v = in[li * n + o0 + e];
+        float2 o = make_float2(v.x * scale, v.y * cs);
+        if (parts) {
+            const float* pr = parts + img * CIM_PART_STRIDE;
+            o = apply_shift(o, shift_phase(o0 + col - c, o0 + e - c, n, pr[CIM_P_X], pr[CIM_P_Y], 1));
+        }
+        crop[(img * Dc + e) * Dc + col] = o;
+    }
+}
+
+// Forward 2D transform of batch images (data, n x n each, left unchanged) on
+// the crop of Dc x Dc centred frequencies from index o0 (crop[image][ky][hx],
+// ky, hx = o0 - n/2 + index), centre-shifted by the records parts when given.
+// tmp: batch * n * Dc float2.
+int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int o0, int Dc, int conj_out,
+                   const float* parts, float2* tmp, float2* crop) {
+    if (o0 < 0 || Dc <= 0 || o0 + Dc > n) return set_error(ctx, CIM_E_ARG, "crop %d+%d outside %d", o0, Dc, n);
+    FftPlan* pl = nullptr;
+    int rc = fft_plan(ctx, n, &pl);
+    if (rc) return rc;
+    TimedLaunch tl(ctx, "fft2d");
+    const FacList fl = fac_list(pl);
+    const int Lr = n >= 1024 ? 1 : (n >= 512 ? 2 : (n >= 256 ? 4 : (n >= 128 ? 8 : 16)));
+    const long long nrows = (long long)batch * n;
+    hipLaunchKernelGGL(k_fft_rows_crop, dim3((unsigned)((nrows + Lr - 1) / Lr)), dim3(256),
+                       sizeof(float2) * 2 * (size_t)Lr * n, ctx->stream, data, pl->tw, n, fl, nrows, Lr, o0, Dc, tmp);
+    CIM_HIP(hipGetLastError());
+    const int Lc = n <= 128 ? 16 : (n <= 512 ? 8 : 4);
+    const long long ncols = (long long)batch * Dc;
+    const float scale = (float)(1.0 / ((double)n * n));
+    hipLaunchKernelGGL(k_fft_cols_crop, dim3((unsigned)((ncols + Lc - 1) / Lc)), dim3(256),
+                       sizeof(float2) * 2 * (size_t)Lc * n, ctx->stream, tmp, pl->tw, n, fl, ncols, Lc, o0, Dc, scale,
+                       conj_out, parts, crop);
+    CIM_HIP(hipGetLastError());
+    return CIM_OK;
 }
 
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out) {

@@ -132,15 +132,22 @@ __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ im
     }
 }
 
+// [norm] -> [mask] -> [norm] -> checkerboard of n images into ft_out (complex,
+// before the transform)
+int prep_images_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, const float* parts_dev,
+                    float2* ft_out, int do_norm, int do_mask) {
+    if (n <= 0) return CIM_OK;
+    TimedLaunch tl(ctx, "prep");
+    hipLaunchKernelGGL(k_prep_real, dim3(n), dim3(1024), 0, ctx->stream, images_dev, parts_dev, ft_out, p->box,
+                       do_norm, do_mask, p->imgmask);
+    CIM_HIP(hipGetLastError());
+    return CIM_OK;
+}
+
 int preprocess_images(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, const float* parts_dev,
                       float2* ft_out, int do_norm, int do_mask, int conj_out) {
     if (n <= 0) return CIM_OK;
-    {
-        TimedLaunch tl(ctx, "prep");
-        hipLaunchKernelGGL(k_prep_real, dim3(n), dim3(1024), 0, ctx->stream, images_dev, parts_dev, ft_out, p->box,
-                           do_norm, do_mask, p->imgmask);
-        CIM_HIP(hipGetLastError());
-    }
+    if (int rc = prep_images_dev(ctx, p, n, images_dev, parts_dev, ft_out, do_norm, do_mask)) return rc;
     return fft2d_dev(ctx, ft_out, p->box, n, 1, conj_out);
 }
 

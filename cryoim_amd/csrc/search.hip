@@ -135,6 +135,19 @@ __global__ void k_crop(const float2* __restrict__ src, long long src_stride, int
     dst[(long long)item * Dc * Dc + q] = v;
 }
 
+// The centre shift of img2d_FT_centshift applied in place to n crops of
+// Dc x Dc centred frequencies (crop index i <-> frequency i - R - 1), as
+// k_centshift would on the full transform (the same shift_phase).
+__global__ void k_crop_shift(float2* __restrict__ crop, int box, int R, int Dc, const float* __restrict__ parts) {
+    const int item = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Dc * Dc) return;
+    const int hx = q % Dc - R - 1, ky = q / Dc - R - 1;
+    const float* pr = parts + (long long)item * CIM_PART_STRIDE;
+    float2* f = crop + (long long)item * Dc * Dc + q;
+    *f = apply_shift(*f, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
+}
+
 // ---- global search kernel -------------------------------------------------
 
 struct SearchArgs {
@@ -1996,17 +2009,33 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         }
         CIM_ALLOC(ctx->ft, sizeof(float2) * N * nb);
         float2* ft = ctx->ft.as<float2>();
-        // the global search masks unconditionally (ort_globesearch.f90:50, unlike mode 5)
-        rc = preprocess_images(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), ft, 0, 1, 1);
-        if (rc) return rc;
         CIM_ALLOC(sb.fcut0, sizeof(float) * nr * nb);
-        rc = amp_expfit(ctx, ft, box, nb, p->minR, R, sb.fcut0.as<float>());
-        if (rc) return rc;
-        // the centre shift is applied to the crop's pixels only
         CIM_ALLOC(sb.pcrop, sizeof(float2) * (size_t)Dc * Dc * nb);
-        hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream, ft, N, box, R, Dc,
-                           sb.pcrop.as<float2>(), 1, (const float*)nullptr, cc, 0, ctx->parts.as<float>());
-        CIM_HIP(hipGetLastError());
+        // the global search masks unconditionally (ort_globesearch.f90:50, unlike mode 5)
+        const int o0 = box / 2 - R - 1;
+        if (o0 >= 0 && o0 + Dc <= box) {
+            // the transform is needed on the crop only: rows pruned to its columns, then its columns
+            rc = prep_images_dev(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), ft, 0, 1);
+            if (rc) return rc;
+            CIM_ALLOC(sb.ftrow, sizeof(float2) * (size_t)nb * box * Dc);
+            rc = fft2d_crop_dev(ctx, ft, box, nb, o0, Dc, 1, nullptr, sb.ftrow.as<float2>(), sb.pcrop.as<float2>());
+            if (rc) return rc;
+            rc = amp_expfit_general(ctx, sb.pcrop.as<float2>(), (long long)Dc * Dc, Dc, R + 1, nb, p->minR, R,
+                                    sb.fcut0.as<float>());
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_crop_shift, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream,
+                               sb.pcrop.as<float2>(), box, R, Dc, ctx->parts.as<float>());
+            CIM_HIP(hipGetLastError());
+        } else {
+            rc = preprocess_images(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), ft, 0, 1, 1);
+            if (rc) return rc;
+            rc = amp_expfit(ctx, ft, box, nb, p->minR, R, sb.fcut0.as<float>());
+            if (rc) return rc;
+            // the centre shift is applied to the crop's pixels only
+            hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream, ft, N, box, R, Dc,
+                               sb.pcrop.as<float2>(), 1, (const float*)nullptr, cc, 0, ctx->parts.as<float>());
+            CIM_HIP(hipGetLastError());
+        }
         CIM_ALLOC(sb.pcrop4, sizeof(float4) * (size_t)Dc * Dc * nb);
         {
             const long long tot = (long long)Dc * Dc * nb;
