@@ -59,7 +59,8 @@ void resolve_timing(cim_ctx* c) {
 
 int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init, int nops, const double* ops);
 int rec_insert_ft_dev(cim_ctx* ctx, int n, float2* ft, const float* parts_dev, const float* ctf_img_dev,
-                      int apply_ctf, const cim_params* p, int shift);
+                      int apply_ctf, const cim_params* p, int shift,
+                      const FtView* view = nullptr);
 int rec_tnf_io(cim_ctx* ctx, float* fa, float* fb, float* wg, int add);
 int rec_finish_dev(cim_ctx* ctx, float* map_dev);
 
@@ -145,8 +146,8 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->lref.release();
     ctx->cen.release();
     ctx->lbox.release();
-    for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->aux, &ctx->aux2, &ctx->parts, &ctx->out, &ctx->fsc_in,
-                      &ctx->fsc_nrm, &ctx->fsc_acc, &ctx->fsc_sums, &ctx->fsc_wt, &ctx->avg_ops, &ctx->avg_acc})
+    for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->ftrow, &ctx->ftcrop, &ctx->aux, &ctx->aux2, &ctx->parts,
+                      &ctx->out, &ctx->fsc_in, &ctx->fsc_nrm, &ctx->fsc_acc, &ctx->fsc_sums, &ctx->fsc_wt, &ctx->avg_ops, &ctx->avg_acc})
         b->release();
     resolve_timing(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
@@ -254,6 +255,35 @@ int cim_fft3d_dev(cim_ctx* ctx, void* data, int n, int dir, int conj_out) {
 
 // ---- reconstruction ----------------------------------------------------------
 
+// Preparation, transform and insert of n device images whose records are in
+// ctx->parts.  The staging reads only the half plane 0 <= hx <= maxR,
+// |ky| <= maxR, so the transform is pruned to it (rows keep those columns, the
+// columns pass keeps those rows: the same values as the whole transform);
+// the scatter kernel and radii past the box take whole transforms.  The
+// centre shift is applied where the values are staged.
+static int insert_images_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev) {
+    const int box = p->box, R = ctx->rec.maxR, c = box / 2;
+    const long long N = (long long)box * box;
+    CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
+    if (R <= c - 1 && !insert_scatter(ctx)) {
+        int rc = prep_images_dev(ctx, p, n, images_dev, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on,
+                                 p->mask_on);
+        if (rc) return rc;
+        const int Dx = R + 1, Dy = 2 * R + 1;
+        CIM_ALLOC(ctx->ftrow, sizeof(float2) * (size_t)n * box * Dx);
+        CIM_ALLOC(ctx->ftcrop, sizeof(float2) * (size_t)n * Dx * Dy);
+        rc = fft2d_crop_dev(ctx, ctx->ft.as<float2>(), box, n, c, Dx, c - R, Dy, p->conj_out, nullptr,
+                            ctx->ftrow.as<float2>(), ctx->ftcrop.as<float2>());
+        if (rc) return rc;
+        const FtView v{ctx->ftcrop.as<float2>(), (long long)Dx * Dy, Dx, 0, R};
+        return rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1, &v);
+    }
+    int rc = preprocess_images(ctx, p, n, images_dev, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on,
+                               p->mask_on, p->conj_out);
+    if (rc) return rc;
+    return rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1);
+}
+
 int cim_rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_init, int nops,
                   const double* ops) {
     if (!ctx) return CIM_E_ARG;
@@ -290,12 +320,7 @@ static int rec_insert_images_dev(cim_ctx* ctx, const cim_params* p, int n, const
                                    hipMemcpyDeviceToDevice, ctx->stream));
         src = ctx->aux.as<float>();
     }
-    CIM_ALLOC(ctx->ft, sizeof(float2) * N * nu);
-    rc = preprocess_images(ctx, p, nu, src, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on, p->mask_on,
-                           p->conj_out);
-    if (rc) return rc;
-    // the centre shift is applied where the insert stages its sample values
-    rc = rec_insert_ft_dev(ctx, nu, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1);
+    rc = insert_images_dev(ctx, p, nu, src);
     if (rc) return rc;
     CIM_HIP(hipStreamSynchronize(ctx->stream));
     return CIM_OK;
@@ -314,11 +339,7 @@ static int rec_insert_records_dev(cim_ctx* ctx, const cim_params* p, int n, cons
     CIM_ALLOC(ctx->parts, sizeof(float) * CIM_PART_STRIDE * (size_t)n);
     CIM_HIP(hipMemcpyAsync(ctx->parts.ptr, parts_dev, sizeof(float) * CIM_PART_STRIDE * (size_t)n,
                            hipMemcpyDeviceToDevice, ctx->stream));
-    CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
-    rc = preprocess_images(ctx, p, n, images_dev, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on, p->mask_on,
-                           p->conj_out);
-    if (rc) return rc;
-    return rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1);
+    return insert_images_dev(ctx, p, n, images_dev);
 }
 
 static void step_release(cim_ctx* ctx) {

@@ -174,6 +174,8 @@ struct cim_ctx {
     cim::CnnState cnn;
     // scratch buffers reused across calls
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
+    // the insert's transform pruned to the half plane it reads: row-pass columns, then the crop
+    cim::DevBuf ftrow, ftcrop;
     // half-map FSC: input tnf pairs, normalised copies, merged sums, shell sums, weights
     cim::DevBuf fsc_in, fsc_nrm, fsc_acc, fsc_sums, fsc_wt;
     // real-space icosahedral average: operators and the sum volume
@@ -237,8 +239,16 @@ int search_candidates_dev(cim_ctx* ctx, const cim_params* p, int n, const float*
                           int ncand, const int* cand, const int* omega);
 // In-place transforms of interleaved complex data on ctx->stream.
 int fft2d_dev(cim_ctx* ctx, float2* data, int n, int batch, int dir, int conj_out);
-int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int o0, int Dc, int conj_out,
-                   const float* parts, float2* tmp, float2* crop);
+// A batch of centred transforms (or crops of them): image i's value at centred
+// frequency (hx, ky) is p[i * stride + (ky + oy) * ld + (hx + ox)].
+struct FtView {
+    const float2* p;
+    long long stride;
+    int ld, ox, oy;
+};
+bool insert_scatter(const cim_ctx* ctx);
+int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int ox0, int Dx, int oy0, int Dy,
+                   int conj_out, const float* parts, float2* tmp, float2* crop);
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
 // Fold {fa, fb, wg} (pitch dxy, z >= 0) -> complex volume -> inverse FFT ->
 // real map with checkerboard (inside radius rmask only when rmask >= 0).

@@ -165,11 +165,12 @@ __global__ void __launch_bounds__(256) k_fft_rows_crop(const float2* __restrict_
 }
 
 // columns: line l = image l / Dc, column l % Dc of tmp (elements Dc apart);
-// outputs o0 .. o0 + Dc - 1 to crop[image][row][column]
+// outputs oy0 .. oy0 + Dy - 1 to crop[image][row][column] (the columns are those
+// from index ox0 of the row pass)
 __global__ void __launch_bounds__(256) k_fft_cols_crop(const float2* __restrict__ tmp, const float2* __restrict__ tw,
-                                                       int n, FacList fl, long long nlines, int L, int o0, int Dc,
-                                                       float scale, int conj, const float* __restrict__ parts,
-                                                       float2* __restrict__ crop) {
+                                                       int n, FacList fl, long long nlines, int L, int ox0, int Dc,
+                                                       int oy0, int Dy, float scale, int conj,
+                                                       const float* __restrict__ parts, float2* __restrict__ crop) {
     extern __shared__ float2 lds[];
     float2* b0 = lds;
     float2* b1 = lds + (size_t)L * n;
@@ -189,28 +190,29 @@ __global__ void __launch_bounds__(256) k_fft_cols_crop(const float2* __restrict_
     const float2* in = lds_fft(b0, b1, n, L, fl, tw, -1.f);
     const float cs = conj ? -scale : scale;
     const int c = n / 2;
-    for (int idx = threadIdx.x; idx < L * Dc; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < L * Dy; idx += blockDim.x) {
         const int e = idx >> lgL, li = idx & (L - 1);
         if (li >= nl) continue;
         const long long l = l0 + li, img = l / Dc;
         const int col = (int)(l - img * Dc);
-        const float2 v = in[li * n + o0 + e];
+        const float2 v = in[li * n + oy0 + e];
         float2 o = make_float2(v.x * scale, v.y * cs);
         if (parts) {
             const float* pr = parts + img * CIM_PART_STRIDE;
-            o = apply_shift(o, shift_phase(o0 + col - c, o0 + e - c, n, pr[CIM_P_X], pr[CIM_P_Y], 1));
+            o = apply_shift(o, shift_phase(ox0 + col - c, oy0 + e - c, n, pr[CIM_P_X], pr[CIM_P_Y], 1));
         }
-        crop[(img * Dc + e) * Dc + col] = o;
+        crop[(img * Dy + e) * Dc + col] = o;
     }
 }
 
 // Forward 2D transform of batch images (data, n x n each, left unchanged) on
-// the crop of Dc x Dc centred frequencies from index o0 (crop[image][ky][hx],
-// ky, hx = o0 - n/2 + index), centre-shifted by the records parts when given.
-// tmp: batch * n * Dc float2.
-int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int o0, int Dc, int conj_out,
-                   const float* parts, float2* tmp, float2* crop) {
-    if (o0 < 0 || Dc <= 0 || o0 + Dc > n) return set_error(ctx, CIM_E_ARG, "crop %d+%d outside %d", o0, Dc, n);
+// the crop of columns ox0 .. ox0 + Dx - 1 and rows oy0 .. oy0 + Dy - 1
+// (crop[image][row][column], frequency = index - n/2), centre-shifted by the
+// records parts when given.  tmp: batch * n * Dx float2.
+int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int ox0, int Dx, int oy0, int Dy,
+                   int conj_out, const float* parts, float2* tmp, float2* crop) {
+    if (ox0 < 0 || Dx <= 0 || ox0 + Dx > n || oy0 < 0 || Dy <= 0 || oy0 + Dy > n)
+        return set_error(ctx, CIM_E_ARG, "crop columns %d+%d, rows %d+%d outside %d", ox0, Dx, oy0, Dy, n);
     FftPlan* pl = nullptr;
     int rc = fft_plan(ctx, n, &pl);
     if (rc) return rc;
@@ -219,14 +221,14 @@ int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int o0, i
     const int Lr = n >= 1024 ? 1 : (n >= 512 ? 2 : (n >= 256 ? 4 : (n >= 128 ? 8 : 16)));
     const long long nrows = (long long)batch * n;
     hipLaunchKernelGGL(k_fft_rows_crop, dim3((unsigned)((nrows + Lr - 1) / Lr)), dim3(256),
-                       sizeof(float2) * 2 * (size_t)Lr * n, ctx->stream, data, pl->tw, n, fl, nrows, Lr, o0, Dc, tmp);
+                       sizeof(float2) * 2 * (size_t)Lr * n, ctx->stream, data, pl->tw, n, fl, nrows, Lr, ox0, Dx, tmp);
     CIM_HIP(hipGetLastError());
     const int Lc = n <= 128 ? 16 : (n <= 512 ? 8 : 4);
-    const long long ncols = (long long)batch * Dc;
+    const long long ncols = (long long)batch * Dx;
     const float scale = (float)(1.0 / ((double)n * n));
     hipLaunchKernelGGL(k_fft_cols_crop, dim3((unsigned)((ncols + Lc - 1) / Lc)), dim3(256),
-                       sizeof(float2) * 2 * (size_t)Lc * n, ctx->stream, tmp, pl->tw, n, fl, ncols, Lc, o0, Dc, scale,
-                       conj_out, parts, crop);
+                       sizeof(float2) * 2 * (size_t)Lc * n, ctx->stream, tmp, pl->tw, n, fl, ncols, Lc, ox0, Dx, oy0,
+                       Dy, scale, conj_out, parts, crop);
     CIM_HIP(hipGetLastError());
     return CIM_OK;
 }

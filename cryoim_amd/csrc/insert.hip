@@ -307,7 +307,7 @@ constexpr int BRICK = 8;
 // of img2d_FT_centshift here (shift_phase, recentred, as k_centshift would).
 // maxbits collects the largest |value| of the batch (float bits; non-negative
 // floats order as unsigned integers).
-__global__ void __launch_bounds__(256) k_insert_vals(const float2* __restrict__ ft, int box, int maxR,
+__global__ void __launch_bounds__(256) k_insert_vals(FtView ft, int box, int maxR,
                                                      const int* __restrict__ hxmax, const float* __restrict__ parts,
                                                      const float* __restrict__ ctf_img, int apply_ctf, CtfConst cc,
                                                      int shift, float4* __restrict__ vals,
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(256) k_insert_vals(const float2* __restrict__ 
         if (hx <= hxmax[ky + maxR]) {
             const long long N = (long long)box * box;
             const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
-            float2 v = ft[(long long)p * N + i2];
+            float2 v = ft.p[(long long)p * ft.stride + (long long)(ky + ft.oy) * ft.ld + (hx + ft.ox)];
             if (shift) {
                 const float* pr = parts + (long long)p * CIM_PART_STRIDE;
                 v = apply_shift(v, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
@@ -1173,13 +1173,23 @@ int rec_begin(cim_ctx* ctx, int box, int maxR, int layout, float rcut, float wg_
 // Insert n particles whose centred, centre-shifted FTs are in ft (device).
 int centre_shift(cim_ctx* ctx, float2* ft, int box, int n, const float* parts_dev, int recentre);
 
+// The atomic scatter kernel is selected (CIM_INSERT=scatter, plain sums only).
+bool insert_scatter(const cim_ctx* ctx) {
+    const char* mode = getenv("CIM_INSERT");
+    return mode && strcmp(mode, "scatter") == 0 && !ctx->rec.comp;
+}
+
 // shift: ft is not centre-shifted yet (the value staging applies it; the
-// scatter kernel, which reads ft itself, gets it applied first)
+// scatter kernel, which reads ft itself, gets it applied first).  view: where
+// the staging reads the transform (a half-plane crop); null: ft, whole
+// box x box transforms.
 int rec_insert_ft_dev(cim_ctx* ctx, int n, float2* ft, const float* parts_dev, const float* ctf_img_dev,
-                      int apply_ctf, const cim_params* p, int shift) {
+                      int apply_ctf, const cim_params* p, int shift, const FtView* view) {
     RecState& r = ctx->rec;
     if (!r.active) return set_error(ctx, CIM_E_STATE, "cim_rec_begin has not been called");
     if (n <= 0) return CIM_OK;
+    const FtView whole{ft, (long long)r.box * r.box, r.box, r.box / 2, r.box / 2};
+    const FtView fv = view ? *view : whole;
     const int total_vals = (2 * r.maxR + 1) * (r.maxR + 1);
     // the coset kernel indexes the batch's values with 32 bits as the queue kernel does
     const bool cosets = r.cosets && (long long)total_vals * n < (1ll << 32);
@@ -1201,7 +1211,8 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, float2* ft, const float* parts_dev, c
     CtfConst cc{};
     if (p) cc = make_ctf_const(p);
     const char* mode = getenv("CIM_INSERT");
-    if (mode && strcmp(mode, "scatter") == 0 && !r.comp) {
+    if (insert_scatter(ctx)) {
+        if (view) return set_error(ctx, CIM_E_STATE, "the scatter insert reads whole transforms");
         if (shift) {
             const int rc = centre_shift(ctx, ft, r.box, n, parts_dev, 1);
             if (rc) return rc;
@@ -1216,7 +1227,7 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, float2* ft, const float* parts_dev, c
         CIM_ALLOC(r.vals, sizeof(float4) * (size_t)total * n);
         CIM_ALLOC(r.maxv, sizeof(unsigned int));
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, ctx->stream, r.maxv.as<unsigned int>(), 0u);
-        hipLaunchKernelGGL(k_insert_vals, dim3((total + 255) / 256, n), dim3(256), 0, ctx->stream, ft, r.box, r.maxR,
+        hipLaunchKernelGGL(k_insert_vals, dim3((total + 255) / 256, n), dim3(256), 0, ctx->stream, fv, r.box, r.maxR,
                            r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, shift, r.vals.as<float4>(),
                            r.maxv.as<unsigned int>());
         CIM_HIP(hipGetLastError());

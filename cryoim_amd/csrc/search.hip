@@ -2018,7 +2018,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             rc = prep_images_dev(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), ft, 0, 1);
             if (rc) return rc;
             CIM_ALLOC(sb.ftrow, sizeof(float2) * (size_t)nb * box * Dc);
-            rc = fft2d_crop_dev(ctx, ft, box, nb, o0, Dc, 1, nullptr, sb.ftrow.as<float2>(), sb.pcrop.as<float2>());
+            rc = fft2d_crop_dev(ctx, ft, box, nb, o0, Dc, o0, Dc, 1, nullptr, sb.ftrow.as<float2>(),
+                                sb.pcrop.as<float2>());
             if (rc) return rc;
             rc = amp_expfit_general(ctx, sb.pcrop.as<float2>(), (long long)Dc * Dc, Dc, R + 1, nb, p->minR, R,
                                     sb.fcut0.as<float>());
