@@ -261,10 +261,10 @@ def gpu_run(args, tang, timgs, ps):
     # chosen path, timed: pipelined (the value) and sequential (reported beside it)
     if args.no_pipeline:
         out_c, ev_c, wall_c, kt_c, cnt_c = timed(p_chosen, COMPENSATED, args.steps, args.warmup)
-        ev_seq = ev_c
+        ev_seq, kt_seq = ev_c, kt_c
     else:
         out_c, ev_c, wall_c, kt_c, cnt_c = timed_pipelined(p_chosen, COMPENSATED, args.steps, args.warmup)
-        _, ev_seq, _, _, _ = timed(p_chosen, COMPENSATED, args.steps, 0)
+        _, ev_seq, _, kt_seq, _ = timed(p_chosen, COMPENSATED, args.steps, 0)
     # the same step captured as one HIP graph (cim_step_capture) and replayed step after step
     ctx.rec_begin(BOX, INSERT_MAXR, layout=COMPENSATED, rcut=INSERT_MAXR, wg_init=0.01)
     parts_dev = torch.from_numpy(P.copy()).to("cuda:0")
@@ -304,7 +304,7 @@ def gpu_run(args, tang, timgs, ps):
     ang_err = np.abs(((out_p[:, 0:3] - ps.angles + 180) % 360) - 180)
     return dict(n=n, d=d, ev_c=ev_c, wall_c=wall_c, kt_c=kt_c, cnt_c=cnt_c, ev_p=ev_p, wall_p=wall_p, kt_p=kt_p,
                 cnt_p=cnt_p, identical=identical, within=within, fsc=fsc, map_rel=map_rel, ang_err=ang_err,
-                ev_seq=ev_seq, ev_graph=ev_graph, graph_same=graph_same)
+                ev_seq=ev_seq, kt_seq=kt_seq, ev_graph=ev_graph, graph_same=graph_same)
 
 
 CENTER_BATCH = 224       # one working chunk at box 384
@@ -919,19 +919,22 @@ def cpu_baseline(args, tang, timgs, ps):
 
 def _roofline(r, args):
     """Roofline of the kernel with the most time per chosen step, from its
-    HIP-event durations over the timed steps and the library's own work
-    counters for the same steps: FP32 operations for the exact re-score,
-    BF16 matrix operations for the residual tables, table bytes for the
-    gather, sample bytes for the insert."""
-    kt, cnt, n = r["kt_c"], r["cnt_c"], r["n"]
+    HIP-event durations over the sequential timed steps (each kernel alone on
+    the chip; in the pipelined steps the insert shares it with the next
+    batch's search) and the library's own work counters: FP32 operations for
+    the exact re-score, BF16 matrix operations for the residual tables, table
+    bytes for the gather, sample bytes for the insert."""
+    kt, cnt, n = r.get("kt_seq") or r["kt_c"], r["cnt_c"], r["n"]
     tot = {k: v[0] * v[1] for k, v in kt.items() if v[1]}
     dom = max(tot, key=tot.get)
     ms, launches = kt[dom]
+    lc = r["kt_c"][dom][1]          # launches the work counters cover
     sec = ms / 1e3
-    base = {"avg_launch_ms": round(ms, 3), "launches": launches, "traffic": None}
+    base = {"avg_launch_ms": round(ms, 3), "launches": launches,
+            "timing": "HIP events on the library's stream, sequential timed steps", "traffic": None}
     if dom == "cl_rescore":
-        calls = cnt["cl_rescore_samples"] / 2 / launches          # paired calls (two columns each)
-        passed = cnt["cl_rescore_pass"] / 2 / launches
+        calls = cnt["cl_rescore_samples"] / 2 / lc          # paired calls (two columns each)
+        passed = cnt["cl_rescore_pass"] / 2 / lc
         flops = passed * FLOPS_PAIR_CALL_PASS + (calls - passed) * FLOPS_PAIR_CALL_GATED
         achieved = flops / sec / 1e12
         return dom, {"bound": "valu", "kernel": "k_cl_rescore<4,true> (exact phase residual of the top-32 AU "
@@ -946,7 +949,7 @@ def _roofline(r, args):
                              "figure (vector and matrix alike, counting packed FP32); this build compiles without "
                              "packed FP32, so its issue limit is half the peak"}
     if dom == "cl_table":
-        flops = cnt["cl_table_pairs"] / launches * TABLE_FEATURES_PER_RADIUS
+        flops = cnt["cl_table_pairs"] / lc * TABLE_FEATURES_PER_RADIUS
         achieved = flops / sec / 1e12
         return dom, {"bound": "mfma", "kernel": "k_cl_table_mfma<4> (residual tables, v_mfma_f32_32x32x16_bf16)",
                      "achieved": round(achieved, 1), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
@@ -981,6 +984,38 @@ def _roofline(r, args):
                           "bound it: LDS atomics (and their bank conflicts) and the point enumeration's VALU work "
                           "do (profiles/sq_counters_box384.md); in the pipelined steps it also shares the chip "
                           "with the next batch's search")}
+
+
+CLOCK_GHZ = 2.4            # MI355X peak engine clock
+SIMDS, CUS = 1024, 256
+
+
+def _limits(keys, roof, tr, n):
+    """Utilisation of each unit that can bound the roofline kernel: its
+    per-launch SQ counts (profiles/sq_counters_box384.json, rocprofv3 --pmc,
+    same batch) over this run's live launch time.  A wave64 VALU instruction
+    holds its SIMD for 4 cycles, so valu_busy = 4 ACTIVE_INST_VALU / SIMD
+    cycles; MFMA busy cycles are counted per SIMD, LDS instructions and bank
+    conflict cycles per CU."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_box384.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        sq = json.load(f)
+    if not all(k in sq and sq[k].get("batch") == n for k in keys):
+        return None
+    def tot(c):
+        return sum(sq[k]["per_launch"].get(c, 0.0) for k in keys)
+    cyc = CLOCK_GHZ * 1e9 * roof["avg_launch_ms"] / 1e3
+    out = {"valu_busy": round(4 * tot("SQ_ACTIVE_INST_VALU") / (SIMDS * cyc), 3),
+           "mfma_busy": round(tot("SQ_VALU_MFMA_BUSY_CYCLES") / (SIMDS * cyc), 3),
+           "lds_instr_per_cu_cycle": round(tot("SQ_INSTS_LDS") / (CUS * cyc), 4),
+           "lds_bank_conflict_frac": round(tot("SQ_LDS_BANK_CONFLICT") / (CUS * cyc), 3)}
+    if roof.get("traffic"):
+        out["hbm_frac"] = round(roof["traffic"] / (roof["avg_launch_ms"] / 1e3) / (PEAK_HBM_GBS * 1e9), 4)
+    out["rule"] = (f"counts per launch from profiles/sq_counters_box384.json over {CLOCK_GHZ} GHz x the live launch "
+                   f"time; {SIMDS} SIMDs, {CUS} CUs")
+    return out
 
 
 def main():
@@ -1033,6 +1068,7 @@ def main():
         if all(k in tr and tr[k].get("batch") == n for k in keys):
             roof["traffic"] = int(sum(tr[k]["hbm_bytes_per_launch"] for k in keys))
             roof["traffic_source"] = "profiles/hbm_traffic_box384.json (rocprofv3 FETCH_SIZE and WRITE_SIZE, copy-calibrated)"
+        roof["limits"] = _limits(keys, roof, tr, n)
     d = r["d"]
     fsc = r["fsc"]
     kt = {k: {"avg_ms": round(v[0], 3), "launches": v[1]} for k, v in r["kt_c"].items() if v[1]}

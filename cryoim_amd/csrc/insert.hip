@@ -305,8 +305,13 @@ constexpr int BRICK = 8;
 // A sample is inserted iff hx <= hxmax[ky + maxR] (the host's radius test).
 // shift: the transform is not centre-shifted yet; each sample takes the phase
 // of img2d_FT_centshift here (shift_phase, recentred, as k_centshift would).
-// maxbits collects the largest |value| of the batch (float bits; non-negative
-// floats order as unsigned integers).
+// Each thread stages VALS_PER_THREAD samples, 256 apart.  maxbits holds one
+// slot per particle (slot p at maxbits[MAXV_STRIDE * (p + 1)], each in its own
+// 128-byte line, so the blocks' maxima do not queue on one address): slot p
+// collects the largest |value| of particle p (float bits; non-negative floats
+// order as unsigned integers), and k_max_slots folds the slots into maxbits[0].
+constexpr int VALS_PER_THREAD = 4;
+constexpr int MAXV_STRIDE = 32;
 __global__ void __launch_bounds__(256) k_insert_vals(FtView ft, int box, int maxR,
                                                      const int* __restrict__ hxmax, const float* __restrict__ parts,
                                                      const float* __restrict__ ctf_img, int apply_ctf, CtfConst cc,
@@ -315,41 +320,58 @@ __global__ void __launch_bounds__(256) k_insert_vals(FtView ft, int box, int max
     __shared__ float red[4];
     const int p = blockIdx.y;
     const int w = maxR + 1, total = (2 * maxR + 1) * w;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (q < total) {
+    const float* pr = parts + (long long)p * CIM_PART_STRIDE;
+    float mx = 0.f;
+#pragma unroll
+    for (int k = 0; k < VALS_PER_THREAD; ++k) {
+        const int q = (blockIdx.x * VALS_PER_THREAD + k) * 256 + threadIdx.x;
+        if (q >= total) break;
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         const int hx = q % w, ky = q / w - maxR;
         if (hx <= hxmax[ky + maxR]) {
-            const long long N = (long long)box * box;
-            const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
             float2 v = ft.p[(long long)p * ft.stride + (long long)(ky + ft.oy) * ft.ld + (hx + ft.ox)];
-            if (shift) {
-                const float* pr = parts + (long long)p * CIM_PART_STRIDE;
-                v = apply_shift(v, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
-            }
+            if (shift) v = apply_shift(v, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
             float ca = 1.f, cw = 1.f;
             if (ctf_img) {
+                const long long N = (long long)box * box;
+                const long long i2 = (long long)(ky + box / 2) * box + (hx + box / 2);
                 ca = ctf_img[(long long)p * N + i2];
                 cw = ca * ca;
             } else if (apply_ctf) {
-                ctf_factors(hx, ky, box, parts + (long long)p * CIM_PART_STRIDE, cc, ca, cw);
+                ctf_factors(hx, ky, box, pr, cc, ca, cw);
             }
             out = make_float4(v.x * ca, v.y * ca, cw, 1.f);
         }
         vals[(long long)p * total + q] = out;
+        mx = fmaxf(mx, fmaxf(fabsf(out.x), fmaxf(fabsf(out.y), fabsf(out.z))));
     }
-    float mx = fmaxf(fabsf(out.x), fmaxf(fabsf(out.y), fabsf(out.z)));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx = fmaxf(mx, red[i]);
-        atomicMax(maxbits, __float_as_uint(mx));
+        atomicMax(maxbits + MAXV_STRIDE * (p + 1), __float_as_uint(mx));
     }
 }
 
-__global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
+// Clears the n particle slots of maxbits.
+__global__ void k_clear_slots(unsigned int* maxbits, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) maxbits[MAXV_STRIDE * (p + 1)] = 0u;
+}
+
+// maxbits[0] = the largest of the n particle slots (one workgroup of 256).
+__global__ void __launch_bounds__(256) k_max_slots(unsigned int* maxbits, int n) {
+    __shared__ unsigned int red[4];
+    unsigned int m = 0u;
+    for (int p = threadIdx.x; p < n; p += 256) m = max(m, maxbits[MAXV_STRIDE * (p + 1)]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) maxbits[0] = max(max(red[0], red[1]), max(red[2], red[3]));
+}
 
 struct BrickArgs {
     float4* acc;
@@ -1225,11 +1247,13 @@ int rec_insert_ft_dev(cim_ctx* ctx, int n, float2* ft, const float* parts_dev, c
     } else {
         const int total = (2 * r.maxR + 1) * (r.maxR + 1);
         CIM_ALLOC(r.vals, sizeof(float4) * (size_t)total * n);
-        CIM_ALLOC(r.maxv, sizeof(unsigned int));
-        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, ctx->stream, r.maxv.as<unsigned int>(), 0u);
-        hipLaunchKernelGGL(k_insert_vals, dim3((total + 255) / 256, n), dim3(256), 0, ctx->stream, fv, r.box, r.maxR,
-                           r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, shift, r.vals.as<float4>(),
-                           r.maxv.as<unsigned int>());
+        CIM_ALLOC(r.maxv, sizeof(unsigned int) * MAXV_STRIDE * ((size_t)n + 1));
+        hipLaunchKernelGGL(k_clear_slots, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, r.maxv.as<unsigned int>(), n);
+        const int per_block = 256 * VALS_PER_THREAD;
+        hipLaunchKernelGGL(k_insert_vals, dim3((total + per_block - 1) / per_block, n), dim3(256), 0, ctx->stream, fv,
+                           r.box, r.maxR, r.hxmax.as<int>(), parts_dev, ctf_img_dev, apply_ctf, cc, shift,
+                           r.vals.as<float4>(), r.maxv.as<unsigned int>());
+        hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(256), 0, ctx->stream, r.maxv.as<unsigned int>(), n);
         CIM_HIP(hipGetLastError());
         BrickArgs ba;
         ba.acc = r.acc.as<float4>();

@@ -174,50 +174,70 @@ int centre_shift(cim_ctx* ctx, float2* ft, int box, int n, const float* parts_de
 
 // FT_amp_expfit (FT_amp_expfit.f90:2-47) of nitems centred transforms:
 // item i is at ft + i*stride with rows of ld elements and the zero
-// frequency at (cidx, cidx).  Thread k owns ring minR + k: it walks the
-// half-plane rows ky >= 0 and, in each, the hx interval that can round to
-// its radius (int(sqrt(hx^2 + ky^2) + 0.5), tested exactly as the reference
-// bins), summing |F| and the radii in compensated float32 in a fixed order;
-// the least-squares line through (radius, ln mean |F|) is solved in float32.
+// frequency at (cidx, cidx).  tpr consecutive threads (a power of two, at
+// most 8, so a group sits in one wave) own ring minR + k: thread j of the
+// group walks the half-plane rows ky >= 0 with ky = j mod tpr and, in each,
+// the hx interval that can round to the ring's radius (int(sqrt(hx^2 +
+// ky^2) + 0.5), tested exactly as the reference bins), summing |F| and the
+// radii in compensated float32; the group's partial sums are combined in a
+// fixed tree order, and the least-squares line through (radius, ln mean
+// |F|) is solved in float32.
 __global__ void __launch_bounds__(256) k_expfit(const float2* __restrict__ ft, long long stride, int ld, int cidx,
-                                                int minR, int maxR, float* __restrict__ fcut) {
+                                                int minR, int maxR, int tpr, float* __restrict__ fcut) {
     extern __shared__ float racc[];   // per ring: mean radius, ln mean |F| (or 0)
+    __shared__ float fit[2];          // slope and intercept of the line
     const int nr = maxR - minR + 1;
     float* resLN = racc;
     float* lnF = racc + nr;
     const int item = blockIdx.x;
     const float2* f = ft + (long long)item * stride;
-    for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+    const int j = threadIdx.x & (tpr - 1), rings_per_pass = blockDim.x / tpr;
+    for (int k0 = 0; k0 < nr; k0 += rings_per_pass) {
+        const int k = k0 + threadIdx.x / tpr;
         const int ir = minR + k;
         ff32 famp{0.f, 0.f}, frad{0.f, 0.f};
         int num = 0;
-        const float rlo = fmaxf((float)ir - 0.5f, 0.f), rhi = (float)ir + 0.5f;
-        for (int ky = 0; ky <= min(ir + 1, maxR); ++ky) {
-            // |hx| with hx^2 + ky^2 in [rlo^2, rhi^2), widened by one step on each side
-            const float a2 = rlo * rlo - (float)(ky * ky), b2 = rhi * rhi - (float)(ky * ky);
-            if (b2 < 0.f) break;
-            const int h0 = a2 > 0.f ? max((int)sqrtf(a2) - 1, 0) : 0;
-            const int h1 = min((int)sqrtf(b2) + 1, maxR);
-            for (int side = 0; side < 2; ++side) {
-                for (int ah = h0; ah <= h1; ++ah) {
-                    const int hx = side ? -ah : ah;
-                    if (side && ah == 0) continue;
-                    const float rad = sqrtf((float)(hx * hx + ky * ky));
-                    if ((int)(rad + 0.5f) != ir) continue;
-                    const float2 v = f[(long long)(ky + cidx) * ld + (hx + cidx)];
-                    famp = ff_add(famp, ff32{sqrtf(__builtin_fmaf(v.x, v.x, v.y * v.y)), 0.f});
-                    frad = ff_add(frad, ff32{rad, 0.f});
-                    ++num;
+        if (k < nr) {
+            const float rlo = fmaxf((float)ir - 0.5f, 0.f), rhi = (float)ir + 0.5f;
+            for (int ky = j; ky <= min(ir + 1, maxR); ky += tpr) {
+                // |hx| with hx^2 + ky^2 in [rlo^2, rhi^2), widened by one step on each side
+                const float a2 = rlo * rlo - (float)(ky * ky), b2 = rhi * rhi - (float)(ky * ky);
+                if (b2 < 0.f) break;
+                const int h0 = a2 > 0.f ? max((int)sqrtf(a2) - 1, 0) : 0;
+                const int h1 = min((int)sqrtf(b2) + 1, maxR);
+                const float2* row = f + (long long)(ky + cidx) * ld + cidx;
+                for (int side = 0; side < 2; ++side) {
+                    for (int ah = h0; ah <= h1; ++ah) {
+                        const int hx = side ? -ah : ah;
+                        if (side && ah == 0) continue;
+                        const float rad = sqrtf((float)(hx * hx + ky * ky));
+                        if ((int)(rad + 0.5f) != ir) continue;
+                        const float2 v = row[hx];
+                        famp = ff_add(famp, ff32{sqrtf(__builtin_fmaf(v.x, v.x, v.y * v.y)), 0.f});
+                        frad = ff_add(frad, ff32{rad, 0.f});
+                        ++num;
+                    }
                 }
             }
         }
-        float r = frad.h, F = 0.f;
-        if (num > 0 && famp.h > 0.f) {
-            r = __fdiv_rn(frad.h, (float)num);
-            F = logf(__fdiv_rn(famp.h, (float)num));
+        // group sums: after the step with offset o, thread j holds threads j .. j + 2o - 1
+        for (int o = 1; o < tpr; o <<= 1) {
+            const ff32 pa{__shfl_down(famp.h, o, 64), __shfl_down(famp.l, o, 64)};
+            const ff32 pr{__shfl_down(frad.h, o, 64), __shfl_down(frad.l, o, 64)};
+            const int pn = __shfl_down(num, o, 64);
+            famp = ff_add(famp, pa);
+            frad = ff_add(frad, pr);
+            num += pn;
         }
-        resLN[k] = r;
-        lnF[k] = F;
+        if (k < nr && j == 0) {
+            float r = frad.h, F = 0.f;
+            if (num > 0 && famp.h > 0.f) {
+                r = __fdiv_rn(frad.h, (float)num);
+                F = logf(__fdiv_rn(famp.h, (float)num));
+            }
+            resLN[k] = r;
+            lnF[k] = F;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -232,9 +252,12 @@ __global__ void __launch_bounds__(256) k_expfit(const float2* __restrict__ ft, l
         const float a22 = (float)nr;
         const float x2 = (b2.h - b1.h * a12.h / a11.h) / (a22 - a12.h * a12.h / a11.h);
         const float x1 = (b1.h - a12.h * x2) / a11.h;
-        float* out = fcut + (long long)item * nr;
-        for (int k = 0; k < nr; ++k) out[k] = expf(resLN[k] * x1 + x2) * 0.8f;
+        fit[0] = x1;
+        fit[1] = x2;
     }
+    __syncthreads();
+    float* out = fcut + (long long)item * nr;
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) out[k] = expf(resLN[k] * fit[0] + fit[1]) * 0.8f;
 }
 
 int amp_expfit_general(cim_ctx* ctx, const float2* ft, long long stride, int ld, int cidx, int nitems, int minR,
@@ -242,8 +265,11 @@ int amp_expfit_general(cim_ctx* ctx, const float2* ft, long long stride, int ld,
     if (nitems <= 0) return CIM_OK;
     const int nr = maxR - minR + 1;
     TimedLaunch tl(ctx, "expfit");
+    // threads per ring: 256 threads over the rings, a power of two from 1 to 8
+    int tpr = 8;
+    while (tpr > 1 && tpr * nr > 256) tpr >>= 1;
     hipLaunchKernelGGL(k_expfit, dim3(nitems), dim3(256), sizeof(float) * 2 * nr, ctx->stream, ft, stride, ld, cidx,
-                       minR, maxR, fcut_dev);
+                       minR, maxR, tpr, fcut_dev);
     CIM_HIP(hipGetLastError());
     return CIM_OK;
 }
