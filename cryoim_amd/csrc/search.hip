@@ -18,6 +18,7 @@
 // centre shift applied to the taps.
 #include <cmath>
 #include <map>
+#include <type_traits>
 
 #include "cim_device.h"
 
@@ -928,7 +929,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 #pragma unroll
     for (int rr = 0; rr < C::RC; ++rr)
         nxt[rr] = (live && rr < nr) ? src[(long long)rr * npsi + jj] : make_float2(0.f, GATED);
-    for (int r0 = 0; r0 < nr; r0 += C::RC) {
+    // waves 0-1 hold template rows, waves 2-3 particle rays: the side is uniform per
+    // wave, so each wave runs only its own side's feature code
+    const bool tside_w = __builtin_amdgcn_readfirstlane(tid) < 128;
+    auto features = [&](auto side) {
+        constexpr bool TS = decltype(side)::value;
 #pragma unroll
         for (int rr = 0; rr < C::RC; ++rr) {
             const float2 v = nxt[rr];
@@ -936,8 +941,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             const float a = g * v.y;
             // D: template [a, g], particle [g, a]
             cim_bf16x2 dv;
-            dv[0] = (__bf16)(tside ? a : g);
-            dv[1] = (__bf16)(tside ? g : a);
+            dv[0] = (__bf16)(TS ? a : g);
+            dv[1] = (__bf16)(TS ? g : a);
             *reinterpret_cast<cim_bf16x2*>(drow + 2 * rr) = dv;
             float c1, s1;
             __sincosf(v.x, &s1, &c1);
@@ -947,19 +952,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             for (int kk = 0; kk < NH; ++kk) {
                 const int k = 2 * kk + 1;
                 // template [a c, a s, g c, g s]; particle c_k [g c, g s, a c, a s]
-                const float f = tside ? 1.f : -1.2732395447351628f / (float)(k * k);
-                const float x0 = tside ? a : g, x1 = tside ? g : a;
+                // (f x0 and f x1 first, as the products (f x0) c associate; 1 a = a)
+                const float f = -1.2732395447351628f / (float)(k * k);
+                const float f0 = TS ? a : f * g, f1 = TS ? g : f * a;
                 cim_bf16x4 hv;
-                hv[0] = (__bf16)(f * x0 * ck);
-                hv[1] = (__bf16)(f * x0 * sk);
-                hv[2] = (__bf16)(f * x1 * ck);
-                hv[3] = (__bf16)(f * x1 * sk);
+                hv[0] = (__bf16)(f0 * ck);
+                hv[1] = (__bf16)(f0 * sk);
+                hv[2] = (__bf16)(f1 * ck);
+                hv[3] = (__bf16)(f1 * sk);
                 *reinterpret_cast<cim_bf16x4*>(hrow + rr * 4 * NH + 4 * kk) = hv;
                 const float cn = ck * c2 - sk * s2, sn = sk * c2 + ck * s2;
                 ck = cn;
                 sk = sn;
             }
         }
+    };
+    for (int r0 = 0; r0 < nr; r0 += C::RC) {
+        if (tside_w) features(std::true_type{});
+        else features(std::false_type{});
 #pragma unroll
         for (int e = 2 * C::RC; e < C::KD; e += 2) *reinterpret_cast<cim_bf16x2*>(drow + e) = cim_bf16x2{};
         __syncthreads();
@@ -1119,20 +1129,28 @@ __global__ void __launch_bounds__(64) k_cl_lines(const float* __restrict__ pts, 
 // one item: the sum over radii of the largest ungated |F|/2 on the ring
 // (D(jT, jP) = sum_r aT gP + gT aP is at most the template's bound plus the
 // particle's).
+// Each wave takes rings in turn (its lanes over the rays); the ring maxima
+// are then added in ring order.
 __global__ void __launch_bounds__(256) k_ring_bound(const float2* __restrict__ pol, int nr, int npsi,
                                                     float* __restrict__ out) {
-    __shared__ float red[4];
-    const int item = blockIdx.x, tid = threadIdx.x;
+    constexpr int RCH = 256;
+    __shared__ float rmax[RCH];
+    const int item = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const float2* P = pol + (long long)item * nr * npsi;
     float acc = 0.f;
-    for (int r = 0; r < nr; ++r) {
-        float m = 0.f;
-        for (int j = tid; j < npsi; j += 256) m = fmaxf(m, P[(long long)r * npsi + j].y);
+    for (int r0 = 0; r0 < nr; r0 += RCH) {
+        const int nc = min(RCH, nr - r0);
+        for (int rr = w; rr < nc; rr += 4) {
+            const float2* row = P + (long long)(r0 + rr) * npsi;
+            float m = 0.f;
+            for (int j = lane; j < npsi; j += 64) m = fmaxf(m, row[j].y);
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        if ((tid & 63) == 0) red[tid >> 6] = m;
+            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+            if (lane == 0) rmax[rr] = m;
+        }
         __syncthreads();
-        if (tid == 0) acc += fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        if (tid == 0)
+            for (int rr = 0; rr < nc; ++rr) acc += rmax[rr];
         __syncthreads();
     }
     if (tid == 0) out[item] = acc;
