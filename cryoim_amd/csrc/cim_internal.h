@@ -86,8 +86,9 @@ struct SearchBatch {
     DevBuf clist, nbr, ccand, ckwin, flist, fcnt;
     // per (AU point, template): the valid common lines' table rows and zero-omega columns, and their count
     DevBuf lines, lcount;
-    // coarse E4M3 tables and the D bounds of the particles and templates they are scaled by
-    DevBuf ctab, bound;
+    // coarse E4M3 tables and the D bounds of the particles and templates they are scaled by (sums
+    // of the ring maxima rmax); directions of the polar rays
+    DevBuf ctab, bound, rmax, rays;
     // fine stage: the two omega windows of each listed point, the coarse second minima and their windows
     DevBuf fwin, fwin2, outi2, ckwin2, outv2;
     // exhaustive fallback: count and list of the particles it scores
@@ -95,7 +96,7 @@ struct SearchBatch {
     void release() {
         for (DevBuf* b : {&pcrop, &pcrop4, &fcut0, &tcrop, &tfcut, &outv, &outi, &prfull, &ops, &rows, &pts, &pbase, &tang,
                           &fv, &fi, &counter, &ppol, &tpol, &table, &cand, &kwin, &cv, &ci, &clist, &nbr, &ccand, &ckwin,
-                          &flist, &fcnt, &lines, &lcount, &ctab, &bound, &fwin, &fwin2, &outi2, &ckwin2, &outv2, &fbl, &ftrow})
+                          &flist, &fcnt, &lines, &lcount, &ctab, &bound, &rmax, &rays, &fwin, &fwin2, &outi2, &ckwin2, &outv2, &fbl, &ftrow})
             b->release();
     }
 };

@@ -768,25 +768,48 @@ __global__ void k_argmin(const float* __restrict__ v, const int* __restrict__ ix
 constexpr float TWO_PI_F = 6.28318530717958647692f;
 constexpr float GATED = -1.0e30f;
 
-// Polar rays: out[item][fr][j] = {phase, |F|/2 or GATED}, j * dpsi degrees,
-// radius minR + fr, bilinear on the crop as the exact kernel.
-__global__ void k_polar(const float2* __restrict__ crops, long long crop_stride, int Dc, int R, int box, int minR,
-                        int nr, int npsi, float dpsi, const float* __restrict__ gate, long long gate_stride,
-                        float2* __restrict__ out) {
-    const int item = blockIdx.y;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nr * npsi) return;
-    const int fr = q / npsi, j = q - fr * npsi;
-    const float d2r = 0.0174532925199432957692f;
+// Polar rays: out[item][fr][j] = {phase, |F|/2 or GATED}, ray j in direction
+// ray[j] = {cos, sin}(j dpsi deg), radius minR + fr, bilinear on the crop as
+// the exact kernel.  One workgroup per (ring, item); with rmax, it also
+// leaves the ring's largest ungated |F|/2 (0 when none) in rmax[item][fr].
+__global__ void __launch_bounds__(256) k_polar(const float2* __restrict__ crops, long long crop_stride, int Dc,
+                                               int R, int box, int minR, int nr, int npsi,
+                                               const float2* __restrict__ ray, const float* __restrict__ gate,
+                                               long long gate_stride, float2* __restrict__ out,
+                                               float* __restrict__ rmax) {
+    __shared__ float red[4];
+    const int item = blockIdx.y, fr = blockIdx.x;
     const int ic = box / 2;
     const float cf = (float)ic;
     const float FR = (float)(minR + fr);
+    const float g = gate[(long long)item * gate_stride + fr];
+    const float2* crop = crops + (long long)item * crop_stride;
+    float2* o = out + ((long long)item * nr + fr) * npsi;
+    float m = 0.f;
+    for (int j = threadIdx.x; j < npsi; j += blockDim.x) {
+        const float2 cs = ray[j];
+        const float x = FR * cs.x + cf, y = FR * cs.y + cf;
+        const float2 z = crop_bilin(crop, Dc, R, x, y, ic);
+        const float amp = sqrtf(z.x * z.x + z.y * z.y);
+        const float h = amp >= g ? 0.5f * amp : GATED;
+        o[j] = make_float2(atan2f(z.y, z.x), h);
+        m = fmaxf(m, h);
+    }
+    if (!rmax) return;
+#pragma unroll
+    for (int k = 32; k > 0; k >>= 1) m = fmaxf(m, __shfl_xor(m, k, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) rmax[(long long)item * nr + fr] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// Direction {cos, sin} of ray j of the polar grid (angle j dpsi deg).
+__global__ void k_rays(int npsi, float dpsi, float2* __restrict__ ray) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npsi) return;
+    const float d2r = 0.0174532925199432957692f;
     const float ang = ((float)j * dpsi) * d2r;
-    const float x = FR * cosf(ang) + cf, y = FR * sinf(ang) + cf;
-    const float2 z = crop_bilin(crops + (long long)item * crop_stride, Dc, R, x, y, ic);
-    const float amp = sqrtf(z.x * z.x + z.y * z.y);
-    const float h = amp >= gate[(long long)item * gate_stride + fr] ? 0.5f * amp : GATED;
-    out[(long long)item * nr * npsi + q] = make_float2(atan2f(z.y, z.x), h);
+    ray[j] = make_float2(cosf(ang), sinf(ang));
 }
 
 // bfloat16 of a finite non-negative float, rounded to nearest even.
@@ -1128,32 +1151,15 @@ __global__ void __launch_bounds__(64) k_cl_lines(const float* __restrict__ pts, 
 // Upper bound of the D entries of a residual table with the polar rays of
 // one item: the sum over radii of the largest ungated |F|/2 on the ring
 // (D(jT, jP) = sum_r aT gP + gT aP is at most the template's bound plus the
-// particle's).
-// Each wave takes rings in turn (its lanes over the rays); the ring maxima
-// are then added in ring order.
-__global__ void __launch_bounds__(256) k_ring_bound(const float2* __restrict__ pol, int nr, int npsi,
-                                                    float* __restrict__ out) {
-    constexpr int RCH = 256;
-    __shared__ float rmax[RCH];
-    const int item = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const float2* P = pol + (long long)item * nr * npsi;
+// particle's).  k_polar leaves each ring's largest value (or 0) in rmax;
+// one thread per item adds them in ring order.
+__global__ void k_ring_sum(const float* __restrict__ rmax, int nr, int nitems, float* __restrict__ out) {
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const float* m = rmax + (long long)item * nr;
     float acc = 0.f;
-    for (int r0 = 0; r0 < nr; r0 += RCH) {
-        const int nc = min(RCH, nr - r0);
-        for (int rr = w; rr < nc; rr += 4) {
-            const float2* row = P + (long long)(r0 + rr) * npsi;
-            float m = 0.f;
-            for (int j = lane; j < npsi; j += 64) m = fmaxf(m, row[j].y);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-            if (lane == 0) rmax[rr] = m;
-        }
-        __syncthreads();
-        if (tid == 0)
-            for (int rr = 0; rr < nc; ++rr) acc += rmax[rr];
-        __syncthreads();
-    }
-    if (tid == 0) out[item] = acc;
+    for (int r = 0; r < nr; ++r) acc += m[r];
+    out[item] = acc;
 }
 
 // The coarse tables: the entries of the even template and particle rays (the
@@ -2137,16 +2143,27 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             const int titems = p->apply_ctf ? nb * T : T;
             CIM_ALLOC(sb.ppol, sizeof(float2) * (size_t)nb * nr * npsi);
             CIM_ALLOC(sb.tpol, sizeof(float2) * (size_t)titems * nr * npsi);
+            const bool cgrid = c2f && coarse_grid;
+            // ring maxima of the particle and template rays (the coarse tables' D bounds)
+            float* rmax = nullptr;
+            if (cgrid) {
+                CIM_ALLOC(sb.rmax, sizeof(float) * (size_t)(nb + titems) * nr);
+                rmax = sb.rmax.as<float>();
+            }
             {
                 TimedLaunch tl(ctx, "cl_polar");
-                const unsigned gx = (unsigned)((nr * npsi + 255) / 256);
-                hipLaunchKernelGGL(k_polar, dim3(gx, nb), dim3(256), 0, ctx->stream, sb.pcrop.as<float2>(),
-                                   (long long)Dc * Dc, Dc, R, box, p->minR, nr, npsi, dpsi, sb.fcut0.as<float>(),
-                                   (long long)nr, sb.ppol.as<float2>());
+                CIM_ALLOC(sb.rays, sizeof(float2) * (size_t)npsi);
+                hipLaunchKernelGGL(k_rays, dim3((npsi + 255) / 256), dim3(256), 0, ctx->stream, npsi, dpsi,
+                                   sb.rays.as<float2>());
                 CIM_HIP(hipGetLastError());
-                hipLaunchKernelGGL(k_polar, dim3(gx, titems), dim3(256), 0, ctx->stream, sb.tcrop.as<float2>(),
-                                   (long long)Dc * Dc, Dc, R, box, p->minR, nr, npsi, dpsi, sb.tfcut.as<float>(),
-                                   (long long)nr, sb.tpol.as<float2>());
+                hipLaunchKernelGGL(k_polar, dim3(nr, nb), dim3(256), 0, ctx->stream, sb.pcrop.as<float2>(),
+                                   (long long)Dc * Dc, Dc, R, box, p->minR, nr, npsi, sb.rays.as<float2>(),
+                                   sb.fcut0.as<float>(), (long long)nr, sb.ppol.as<float2>(), rmax);
+                CIM_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_polar, dim3(nr, titems), dim3(256), 0, ctx->stream, sb.tcrop.as<float2>(),
+                                   (long long)Dc * Dc, Dc, R, box, p->minR, nr, npsi, sb.rays.as<float2>(),
+                                   sb.tfcut.as<float>(), (long long)nr, sb.tpol.as<float2>(),
+                                   rmax ? rmax + (size_t)nb * nr : nullptr);
                 CIM_HIP(hipGetLastError());
             }
             CIM_ALLOC(sb.table, sizeof(unsigned int) * (size_t)nb * T * (npsi / 2) * npsi);
@@ -2156,17 +2173,13 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                      nb, T, nr, npsi, sb.table.as<unsigned int>());
                 if (rc) return rc;
             }
-            const bool cgrid = c2f && coarse_grid;
             if (cgrid) {
                 // coarse E4M3 tables of the even rays, scaled per particle by the D bounds
                 CIM_ALLOC(sb.bound, sizeof(float) * (size_t)(nb + titems));
                 CIM_ALLOC(sb.ctab, sizeof(unsigned short) * (size_t)nb * T * (npsi / 4) * (npsi / 2));
                 TimedLaunch tl(ctx, "cl_coarse_table");
-                hipLaunchKernelGGL(k_ring_bound, dim3(nb), dim3(256), 0, ctx->stream, sb.ppol.as<float2>(), nr, npsi,
-                                   sb.bound.as<float>());
-                CIM_HIP(hipGetLastError());
-                hipLaunchKernelGGL(k_ring_bound, dim3(titems), dim3(256), 0, ctx->stream, sb.tpol.as<float2>(), nr,
-                                   npsi, sb.bound.as<float>() + nb);
+                hipLaunchKernelGGL(k_ring_sum, dim3((nb + titems + 255) / 256), dim3(256), 0, ctx->stream,
+                                   sb.rmax.as<float>(), nr, nb + titems, sb.bound.as<float>());
                 CIM_HIP(hipGetLastError());
                 hipLaunchKernelGGL(k_coarse_table, dim3((unsigned)((npsi / 2 + 255) / 256), npsi / 4, nb * T),
                                    dim3(256), 0, ctx->stream, sb.table.as<unsigned int>(), sb.bound.as<float>(),
