@@ -105,35 +105,45 @@ __device__ __forceinline__ float2 crop_bilin(const float2* __restrict__ crop, in
 // ---- crops ----------------------------------------------------------------
 
 // Crop rel [-(R+1), R+2] around the centre of n centred FTs, optionally
-// multiplied by the CTF of particle record (item / T) (template x CTF case:
-// item = p*T + t, source = src[t]), or (shift_parts) centre-shifted by the
-// phase of img2d_FT_centshift of record item, as k_centshift would (the same
-// shift_phase, so the same bits) but on the crop's pixels only.
+// (shift_parts) centre-shifted by the phase of img2d_FT_centshift of record
+// item, as k_centshift would (the same shift_phase, so the same bits) but on
+// the crop's pixels only.  Without use_ctf, workgroup row y is item y and
+// reads src[y].  With use_ctf (template x CTF), row y is particle record y:
+// its CTF is evaluated once per pixel and multiplies the T template sources
+// src[0 .. T-1] into items y*T + t.
 __global__ void k_crop(const float2* __restrict__ src, long long src_stride, int box, int R, int Dc,
                        float2* __restrict__ dst, int T, const float* __restrict__ ctf_parts, CtfConst cc, int use_ctf,
                        const float* __restrict__ shift_parts) {
-    const int item = blockIdx.y;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Dc * Dc) return;
     const int cy = q / Dc, cx = q % Dc;
     const int c = box / 2;
     const int hx = cx - R - 1, ky = cy - R - 1;
     const int ax = hx + c, ay = ky + c;
-    const int t = item % T, p = item / T;
-    const float2* s = src + (long long)(use_ctf ? t : item) * src_stride;
-    float2 v = make_float2(0.f, 0.f);
-    if (ax >= 0 && ax < box && ay >= 0 && ay < box) v = s[(long long)ay * box + ax];
-    if (shift_parts) {
+    const bool inside = ax >= 0 && ax < box && ay >= 0 && ay < box;
+    const long long si = (long long)ay * box + ax;
+    auto shifted = [&](float2 v, int item) {
+        if (!shift_parts) return v;
         const float* pr = shift_parts + (long long)item * CIM_PART_STRIDE;
-        v = apply_shift(v, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
+        return apply_shift(v, shift_phase(hx, ky, box, pr[CIM_P_X], pr[CIM_P_Y], 1));
+    };
+    if (!use_ctf) {
+        const int item = blockIdx.y;
+        float2 v = inside ? src[(long long)item * src_stride + si] : make_float2(0.f, 0.f);
+        dst[(long long)item * Dc * Dc + q] = shifted(v, item);
+        return;
     }
-    if (use_ctf) {
-        const float* pr = ctf_parts + (long long)p * CIM_PART_STRIDE;
-        float cv = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+    const int p = blockIdx.y;
+    const float* pr = ctf_parts + (long long)p * CIM_PART_STRIDE;
+    const float cv = ctf_value(hx, ky, box, pr[CIM_P_DF1], pr[CIM_P_DF2], pr[CIM_P_ASTIG], cc);
+    for (int t = 0; t < T; ++t) {
+        const int item = p * T + t;
+        float2 v = inside ? src[(long long)t * src_stride + si] : make_float2(0.f, 0.f);
+        v = shifted(v, item);
         v.x *= cv;
         v.y *= cv;
+        dst[(long long)item * Dc * Dc + q] = v;
     }
-    dst[(long long)item * Dc * Dc + q] = v;
 }
 
 // The centre shift of img2d_FT_centshift applied in place to n crops of
@@ -2071,7 +2081,7 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
         SearchArgs a{};
         if (p->apply_ctf) {
             CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * nb * T);
-            hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb * T), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream,
                                s.tft.as<float2>(), N, box, R, Dc, sb.tcrop.as<float2>(), T, ctx->parts.as<float>(), cc,
                                1, (const float*)nullptr);
             CIM_HIP(hipGetLastError());
@@ -2715,7 +2725,7 @@ static int refine_local(cim_ctx* ctx, const cim_params* p, int n, const float* i
     const int Dc = 2 * R + 4;
     if (p->apply_ctf) {
         CIM_ALLOC(sb.tcrop, sizeof(float2) * (size_t)Dc * Dc * n * T);
-        hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, n * T), dim3(256), 0, ctx->stream, s.tft.as<float2>(), N,
+        hipLaunchKernelGGL(k_crop, dim3((Dc * Dc + 255) / 256, n), dim3(256), 0, ctx->stream, s.tft.as<float2>(), N,
                            box, R, Dc, sb.tcrop.as<float2>(), T, ctx->parts.as<float>(), cc, 1, (const float*)nullptr);
         CIM_HIP(hipGetLastError());
         CIM_ALLOC(sb.tfcut, sizeof(float) * nr * n * T);
