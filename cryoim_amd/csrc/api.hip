@@ -146,7 +146,7 @@ int cim_finalize(cim_ctx* ctx) {
     ctx->lref.release();
     ctx->cen.release();
     ctx->lbox.release();
-    for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->ftrow, &ctx->ftcrop, &ctx->aux, &ctx->aux2, &ctx->parts,
+    for (DevBuf* b : {&ctx->img, &ctx->ft, &ctx->ft2, &ctx->ftrow, &ctx->ftcrop, &ctx->prepst, &ctx->aux, &ctx->aux2, &ctx->parts,
                       &ctx->out, &ctx->fsc_in, &ctx->fsc_nrm, &ctx->fsc_acc, &ctx->fsc_sums, &ctx->fsc_wt, &ctx->avg_ops, &ctx->avg_acc})
         b->release();
     resolve_timing(ctx);
@@ -264,8 +264,9 @@ int cim_fft3d_dev(cim_ctx* ctx, void* data, int n, int dir, int conj_out) {
 static int insert_images_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev) {
     const int box = p->box, R = ctx->rec.maxR, c = box / 2;
     const long long N = (long long)box * box;
-    CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
-    if (R <= c - 1 && !insert_scatter(ctx)) {
+    if (R <= c - 1 && !insert_scatter(ctx) && !prep_fused()) {
+        // the same with a prepared copy of the images (CIM_PREP_FUSED=0)
+        CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
         int rc = prep_images_dev(ctx, p, n, images_dev, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on,
                                  p->mask_on);
         if (rc) return rc;
@@ -276,8 +277,25 @@ static int insert_images_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                             ctx->ftrow.as<float2>(), ctx->ftcrop.as<float2>());
         if (rc) return rc;
         const FtView v{ctx->ftcrop.as<float2>(), (long long)Dx * Dy, Dx, 0, R};
-        return rec_insert_ft_dev(ctx, n, ctx->ft.as<float2>(), ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1, &v);
+        return rec_insert_ft_dev(ctx, n, nullptr, ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1, &v);
     }
+    if (R <= c - 1 && !insert_scatter(ctx)) {
+        // the images are prepared as the row pass reads them, from their sums
+        CIM_ALLOC(ctx->prepst, sizeof(PrepStats) * (size_t)n);
+        int rc = prep_stats_dev(ctx, p, n, images_dev, ctx->parts.as<float>(), p->norm_on, p->mask_on,
+                                ctx->prepst.as<PrepStats>());
+        if (rc) return rc;
+        const int Dx = R + 1, Dy = 2 * R + 1;
+        CIM_ALLOC(ctx->ftrow, sizeof(float2) * (size_t)n * box * Dx);
+        CIM_ALLOC(ctx->ftcrop, sizeof(float2) * (size_t)n * Dx * Dy);
+        rc = fft2d_crop_real_dev(ctx, images_dev, ctx->prepst.as<PrepStats>(), p->norm_on, p->mask_on, p->imgmask,
+                                 box, n, c, Dx, c - R, Dy, p->conj_out, nullptr, ctx->ftrow.as<float2>(),
+                                 ctx->ftcrop.as<float2>());
+        if (rc) return rc;
+        const FtView v{ctx->ftcrop.as<float2>(), (long long)Dx * Dy, Dx, 0, R};
+        return rec_insert_ft_dev(ctx, n, nullptr, ctx->parts.as<float>(), nullptr, p->apply_ctf, p, 1, &v);
+    }
+    CIM_ALLOC(ctx->ft, sizeof(float2) * N * n);
     int rc = preprocess_images(ctx, p, n, images_dev, ctx->parts.as<float>(), ctx->ft.as<float2>(), p->norm_on,
                                p->mask_on, p->conj_out);
     if (rc) return rc;

@@ -240,4 +240,60 @@ static __global__ void __launch_bounds__(256) k_sum_partials(const float* __rest
 // x - a for a float-float a, rounded to float
 __device__ __forceinline__ float ff_sub_from(float x, ff32 a) { return __fsub_rn(__fsub_rn(x, a.h), a.l); }
 
+// Image preparation ([norm] -> [mask] -> [norm] -> checkerboard) of one pixel
+// from its image's sums (k_prep_real): the same arithmetic wherever a value is
+// formed, so the preparation pass and a transform that forms the values as it
+// reads the image give the same bits.
+struct PrepStats {
+    ff32 avg;         // mean before the mask (0 unless do_norm == 1)
+    float inv_dev;    // 1 / deviation before the mask (1 unless do_norm == 1)
+    float ring_mean;  // mean over the soft edge of the mask
+    ff32 avg2;        // mean after the mask (0 unless do_norm == 2)
+    float inv_dev2;   // 1 / deviation after the mask (1 unless do_norm == 2)
+    float x0, y0;     // mask centre (pixels)
+};
+struct PrepCfg {
+    int do_norm, do_mask;
+    float R1, R2;     // soft edge of the mask: (R1, R2]
+};
+constexpr int PREP_HALFW = 6;   // width of the soft edge
+
+__host__ __device__ inline PrepCfg prep_cfg(int box, int do_norm, int do_mask, float mask_r) {
+    PrepCfg c{do_norm, do_mask, 0.f, 0.f};
+    if (do_mask) {
+        c.R1 = mask_r - (float)(PREP_HALFW / 2);
+        if (c.R1 <= 0.f) c.R1 = 0.f;
+        c.R2 = mask_r + (float)(PREP_HALFW / 2);
+        if (c.R2 > (float)(box / 2)) c.R2 = (float)(box / 2);
+    }
+    return c;
+}
+
+// [norm] -> [mask] of pixel (ix, iy) with value x.  Contraction is pinned to
+// each expression, so the value does not depend on the kernel it is inlined in.
+__device__ __forceinline__ float prep_masked(float x, int ix, int iy, const PrepStats& st, const PrepCfg& c) {
+#pragma clang fp contract(on)
+    const float pi = 3.14159265358979323846f;
+    float v = ff_sub_from(x, st.avg) * st.inv_dev;
+    if (c.do_mask) {
+        float dy = -st.y0 + (float)iy, dx = -st.x0 + (float)ix;
+        float ra = sqrtf(dy * dy + dx * dx);
+        if (ra > c.R1 && ra <= c.R2) {
+            float edge = (1.0f + cosf(pi * (ra - c.R1) / (float)PREP_HALFW)) / 2.0f;
+            v = v * edge + (1.0f - edge) * st.ring_mean;
+        }
+        if (ra > c.R2) v = st.ring_mean;
+    }
+    return v;
+}
+
+// the whole preparation of pixel (ix, iy): masked, [norm], checkerboard sign
+__device__ __forceinline__ float prep_value(float x, int ix, int iy, const PrepStats& st, const PrepCfg& c) {
+#pragma clang fp contract(on)
+    float v = prep_masked(x, ix, iy, st, c);
+    if (c.do_norm == 2) v = ff_sub_from(v, st.avg2) * st.inv_dev2;
+    if ((ix + iy) & 1) v = -v;
+    return v;
+}
+
 }  // namespace cim

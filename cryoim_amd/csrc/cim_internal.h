@@ -177,6 +177,7 @@ struct cim_ctx {
     cim::DevBuf img, ft, ft2, aux, aux2, parts, out;
     // the insert's transform pruned to the half plane it reads: row-pass columns, then the crop
     cim::DevBuf ftrow, ftcrop;
+    cim::DevBuf prepst;   // per-image preparation sums (PrepStats) of the pruned transforms
     // half-map FSC: input tnf pairs, normalised copies, merged sums, shell sums, weights
     cim::DevBuf fsc_in, fsc_nrm, fsc_acc, fsc_sums, fsc_wt;
     // real-space icosahedral average: operators and the sum volume
@@ -250,6 +251,15 @@ struct FtView {
 bool insert_scatter(const cim_ctx* ctx);
 int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int ox0, int Dx, int oy0, int Dy,
                    int conj_out, const float* parts, float2* tmp, float2* crop);
+// The same transform of batch real images prepared as it reads them, from
+// their sums (prep_stats_dev; the same values as prep_images_dev's).
+struct PrepStats;
+// The pruned transforms prepare the images as the row pass reads them unless
+// CIM_PREP_FUSED=0 (then from a prepared copy, the same bits).
+bool prep_fused();
+int fft2d_crop_real_dev(cim_ctx* ctx, const float* imgs, const PrepStats* stats, int do_norm, int do_mask,
+                        float mask_r, int n, int batch, int ox0, int Dx, int oy0, int Dy, int conj_out,
+                        const float* parts, float2* tmp, float2* crop);
 int fft3d_dev(cim_ctx* ctx, float2* data, int n, int dir, int conj_out);
 // Fold {fa, fb, wg} (pitch dxy, z >= 0) -> complex volume -> inverse FFT ->
 // real map with checkerboard (inside radius rmask only when rmask >= 0).
@@ -272,6 +282,8 @@ int deinterleave(cim_ctx* ctx, const float2* in, float* re, float* im, long long
 // mode_norm: normalise first; mask centre from parts[CIM_P_MASKX/Y].
 int prep_images_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, const float* parts_dev,
                     float2* ft_out, int do_norm, int do_mask);
+int prep_stats_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, const float* parts_dev,
+                   int do_norm, int do_mask, PrepStats* stats);
 int preprocess_images(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev,
                       const float* parts_dev, float2* ft_out, int do_norm, int do_mask,
                       int conj_out);

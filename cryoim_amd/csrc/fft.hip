@@ -164,6 +164,39 @@ __global__ void __launch_bounds__(256) k_fft_rows_crop(const float2* __restrict_
     }
 }
 
+// The same row pass over real images, each value prepared as it is read
+// (prep_value from its image's sums): no prepared copy of the images.
+__global__ void __launch_bounds__(256) k_fft_rows_crop_real(const float* __restrict__ imgs,
+                                                            const PrepStats* __restrict__ stats, PrepCfg cfg,
+                                                            const float2* __restrict__ tw, int n, FacList fl,
+                                                            long long nlines, int L, int o0, int Dc,
+                                                            float2* __restrict__ tmp) {
+    extern __shared__ float2 lds[];
+    float2* b0 = lds;
+    float2* b1 = lds + (size_t)L * n;
+    const long long l0 = (long long)blockIdx.x * L;
+    const int nl = (int)((nlines - l0) < L ? (nlines - l0) : L);
+    for (int li = 0; li < L; ++li) {
+        const long long l = l0 + li;
+        if (li < nl) {
+            const long long img = l / n;
+            const int iy = (int)(l - img * n);
+            const PrepStats st = stats[img];
+            const float* row = imgs + l * n;
+            for (int ix = threadIdx.x; ix < n; ix += blockDim.x)
+                b0[li * n + ix] = make_float2(prep_value(row[ix], ix, iy, st, cfg), 0.f);
+        } else {
+            for (int ix = threadIdx.x; ix < n; ix += blockDim.x) b0[li * n + ix] = make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    const float2* in = lds_fft(b0, b1, n, L, fl, tw, -1.f);
+    for (int idx = threadIdx.x; idx < nl * Dc; idx += blockDim.x) {
+        const int li = idx / Dc, e = idx - li * Dc;
+        tmp[(l0 + li) * Dc + e] = in[li * n + o0 + e];
+    }
+}
+
 // columns: line l = image l / Dc, column l % Dc of tmp (elements Dc apart);
 // outputs oy0 .. oy0 + Dy - 1 to crop[image][row][column] (the columns are those
 // from index ox0 of the row pass)
@@ -222,6 +255,34 @@ int fft2d_crop_dev(cim_ctx* ctx, const float2* data, int n, int batch, int ox0, 
     const long long nrows = (long long)batch * n;
     hipLaunchKernelGGL(k_fft_rows_crop, dim3((unsigned)((nrows + Lr - 1) / Lr)), dim3(256),
                        sizeof(float2) * 2 * (size_t)Lr * n, ctx->stream, data, pl->tw, n, fl, nrows, Lr, ox0, Dx, tmp);
+    CIM_HIP(hipGetLastError());
+    const int Lc = n <= 128 ? 16 : (n <= 512 ? 8 : 4);
+    const long long ncols = (long long)batch * Dx;
+    const float scale = (float)(1.0 / ((double)n * n));
+    hipLaunchKernelGGL(k_fft_cols_crop, dim3((unsigned)((ncols + Lc - 1) / Lc)), dim3(256),
+                       sizeof(float2) * 2 * (size_t)Lc * n, ctx->stream, tmp, pl->tw, n, fl, ncols, Lc, ox0, Dx, oy0,
+                       Dy, scale, conj_out, parts, crop);
+    CIM_HIP(hipGetLastError());
+    return CIM_OK;
+}
+
+int fft2d_crop_real_dev(cim_ctx* ctx, const float* imgs, const PrepStats* stats, int do_norm, int do_mask,
+                        float mask_r, int n, int batch, int ox0, int Dx, int oy0, int Dy, int conj_out,
+                        const float* parts, float2* tmp, float2* crop) {
+    if (batch <= 0) return CIM_OK;
+    if (ox0 < 0 || Dx < 1 || ox0 + Dx > n || oy0 < 0 || Dy < 1 || oy0 + Dy > n)
+        return set_error(ctx, CIM_E_ARG, "fft2d crop [%d, %d) x [%d, %d) outside %d", ox0, ox0 + Dx, oy0, oy0 + Dy, n);
+    FftPlan* pl = nullptr;
+    int rc = fft_plan(ctx, n, &pl);
+    if (rc) return rc;
+    TimedLaunch tl(ctx, "fft2d");
+    const FacList fl = fac_list(pl);
+    const PrepCfg cfg = prep_cfg(n, do_norm, do_mask, mask_r);
+    const int Lr = n >= 1024 ? 1 : (n >= 512 ? 2 : (n >= 256 ? 4 : (n >= 128 ? 8 : 16)));
+    const long long nrows = (long long)batch * n;
+    hipLaunchKernelGGL(k_fft_rows_crop_real, dim3((unsigned)((nrows + Lr - 1) / Lr)), dim3(256),
+                       sizeof(float2) * 2 * (size_t)Lr * n, ctx->stream, imgs, stats, cfg, pl->tw, n, fl, nrows, Lr,
+                       ox0, Dx, tmp);
     CIM_HIP(hipGetLastError());
     const int Lc = n <= 128 ? 16 : (n <= 512 ? 8 : 4);
     const long long ncols = (long long)batch * Dx;

@@ -5,6 +5,7 @@
 // the ring-amplitude fit FT_amp_expfit.f90:2-47.  One workgroup per image
 // for the reductions; sums are carried in float64.
 #include <cmath>
+#include <cstdlib>
 
 #include "cim_device.h"
 
@@ -44,91 +45,80 @@ int deinterleave(cim_ctx* ctx, const float2* in, float* re, float* im, long long
 // out.  do_norm = 1 normalises before the mask (reconstruct.f90:79-92),
 // 2 after it (local_reconstruct/SRC/reconstruct_substep.f90:93-94).  Means,
 // deviations and the ring mean are compensated float32 sums (block_sum_ff).
+// With stats, the kernel stops after the sums and leaves them there (the
+// row transform then forms each value as it reads the image, prep_value).
 __global__ void __launch_bounds__(1024) k_prep_real(const float* __restrict__ imgs, const float* __restrict__ parts,
                                                     float2* __restrict__ out, int box, int do_norm, int do_mask,
-                                                    float mask_r) {
+                                                    float mask_r, PrepStats* __restrict__ stats) {
     __shared__ ff32 sh[16];
     const long long N = (long long)box * box;
     const float* img = imgs + (long long)blockIdx.x * N;
-    float2* o = out + (long long)blockIdx.x * N;
     const float* pr = parts + (long long)blockIdx.x * CIM_PART_STRIDE;
     const float invN = 1.0f / (float)N;
-    ff32 avg{0.f, 0.f};
-    float inv_dev = 1.f;
+    const PrepCfg cfg = prep_cfg(box, do_norm, do_mask, mask_r);
+    PrepStats st{};
+    st.avg = ff32{0.f, 0.f};
+    st.inv_dev = 1.f;
+    st.avg2 = ff32{0.f, 0.f};
+    st.inv_dev2 = 1.f;
     if (do_norm == 1) {
         ff32 s{0.f, 0.f};
         for (long long i = threadIdx.x; i < N; i += blockDim.x) s = ff_add(s, ff32{img[i], 0.f});
         s = block_sum_ff(s, sh);
-        avg = ff_mul(s, ff_recip_int((int)N));
+        st.avg = ff_mul(s, ff_recip_int((int)N));
         ff32 q{0.f, 0.f};
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
-            const float d = ff_sub_from(img[i], avg);
+            const float d = ff_sub_from(img[i], st.avg);
             q = ff_add(q, ff_two_prod(d, d));
         }
         q = block_sum_ff(q, sh);
-        inv_dev = 1.0f / sqrtf(q.h * invN);
+        st.inv_dev = 1.0f / sqrtf(q.h * invN);
     }
-    float R1 = 0.f, R2 = 0.f, ring_mean = 0.f;
-    float x0 = 0.f, y0 = 0.f;
-    const int halfw = 6;
+    st.ring_mean = 0.f;
+    st.x0 = st.y0 = 0.f;
     if (do_mask) {
-        R1 = mask_r - (float)(halfw / 2);
-        if (R1 <= 0.f) R1 = 0.f;
-        R2 = mask_r + (float)(halfw / 2);
-        if (R2 > (float)(box / 2)) R2 = (float)(box / 2);
-        x0 = pr[CIM_P_MASKX] + (float)(box / 2);
-        y0 = pr[CIM_P_MASKY] + (float)(box / 2);
+        st.x0 = pr[CIM_P_MASKX] + (float)(box / 2);
+        st.y0 = pr[CIM_P_MASKY] + (float)(box / 2);
         ff32 s{0.f, 0.f};
         int c = 0;
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
             int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
-            float dy = -y0 + (float)iy, dx = -x0 + (float)ix;
+            float dy = -st.y0 + (float)iy, dx = -st.x0 + (float)ix;
             float ra = sqrtf(dy * dy + dx * dx);
-            if (ra > R1 && ra <= R2) {
-                s = ff_add(s, ff_two_prod(ff_sub_from(img[i], avg), inv_dev));
+            if (ra > cfg.R1 && ra <= cfg.R2) {
+                s = ff_add(s, ff_two_prod(ff_sub_from(img[i], st.avg), st.inv_dev));
                 ++c;
             }
         }
         s = block_sum_ff(s, sh);
         const ff32 cnt = block_sum_ff(ff32{(float)c, 0.f}, sh);
-        ring_mean = cnt.h != 0.f ? __fdiv_rn(s.h, cnt.h) : 0.f;
+        st.ring_mean = cnt.h != 0.f ? __fdiv_rn(s.h, cnt.h) : 0.f;
     }
-    const float pi = 3.14159265358979323846f;
-    auto masked = [&](long long i) -> float {
-        int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
-        float v = ff_sub_from(img[i], avg) * inv_dev;
-        if (do_mask) {
-            float dy = -y0 + (float)iy, dx = -x0 + (float)ix;
-            float ra = sqrtf(dy * dy + dx * dx);
-            if (ra > R1 && ra <= R2) {
-                float edge = (1.0f + cosf(pi * (ra - R1) / (float)halfw)) / 2.0f;
-                v = v * edge + (1.0f - edge) * ring_mean;
-            }
-            if (ra > R2) v = ring_mean;
-        }
-        return v;
-    };
-    ff32 avg2{0.f, 0.f};
-    float inv_dev2 = 1.f;
     if (do_norm == 2) {
         ff32 s{0.f, 0.f};
-        for (long long i = threadIdx.x; i < N; i += blockDim.x) s = ff_add(s, ff32{masked(i), 0.f});
+        for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+            const int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
+            s = ff_add(s, ff32{prep_masked(img[i], ix, iy, st, cfg), 0.f});
+        }
         s = block_sum_ff(s, sh);
-        avg2 = ff_mul(s, ff_recip_int((int)N));
+        st.avg2 = ff_mul(s, ff_recip_int((int)N));
         ff32 q{0.f, 0.f};
         for (long long i = threadIdx.x; i < N; i += blockDim.x) {
-            const float d = ff_sub_from(masked(i), avg2);
+            const int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
+            const float d = ff_sub_from(prep_masked(img[i], ix, iy, st, cfg), st.avg2);
             q = ff_add(q, ff_two_prod(d, d));
         }
         q = block_sum_ff(q, sh);
-        inv_dev2 = 1.0f / sqrtf(q.h * invN);
+        st.inv_dev2 = 1.0f / sqrtf(q.h * invN);
     }
+    if (stats) {
+        if (threadIdx.x == 0) stats[blockIdx.x] = st;
+        return;
+    }
+    float2* o = out + (long long)blockIdx.x * N;
     for (long long i = threadIdx.x; i < N; i += blockDim.x) {
-        int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
-        float v = masked(i);
-        if (do_norm == 2) v = ff_sub_from(v, avg2) * inv_dev2;
-        if ((ix + iy) & 1) v = -v;
-        o[i] = make_float2(v, 0.f);
+        const int iy = (int)(i / box), ix = (int)(i - (long long)iy * box);
+        o[i] = make_float2(prep_value(img[i], ix, iy, st, cfg), 0.f);
     }
 }
 
@@ -139,7 +129,24 @@ int prep_images_dev(cim_ctx* ctx, const cim_params* p, int n, const float* image
     if (n <= 0) return CIM_OK;
     TimedLaunch tl(ctx, "prep");
     hipLaunchKernelGGL(k_prep_real, dim3(n), dim3(1024), 0, ctx->stream, images_dev, parts_dev, ft_out, p->box,
-                       do_norm, do_mask, p->imgmask);
+                       do_norm, do_mask, p->imgmask, (PrepStats*)nullptr);
+    CIM_HIP(hipGetLastError());
+    return CIM_OK;
+}
+
+bool prep_fused() {
+    const char* e = getenv("CIM_PREP_FUSED");
+    return !e || atoi(e) != 0;
+}
+
+// The sums of the same preparation, per image, for a transform that forms
+// the values as it reads the images (fft2d_crop_real_dev).
+int prep_stats_dev(cim_ctx* ctx, const cim_params* p, int n, const float* images_dev, const float* parts_dev,
+                   int do_norm, int do_mask, PrepStats* stats) {
+    if (n <= 0) return CIM_OK;
+    TimedLaunch tl(ctx, "prep");
+    hipLaunchKernelGGL(k_prep_real, dim3(n), dim3(1024), 0, ctx->stream, images_dev, parts_dev, (float2*)nullptr,
+                       p->box, do_norm, do_mask, p->imgmask, stats);
     CIM_HIP(hipGetLastError());
     return CIM_OK;
 }

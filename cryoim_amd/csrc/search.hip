@@ -2041,14 +2041,32 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
             rc = upload_parts(ctx, nb, pr.data());
             if (rc) return rc;
         }
-        CIM_ALLOC(ctx->ft, sizeof(float2) * N * nb);
-        float2* ft = ctx->ft.as<float2>();
         CIM_ALLOC(sb.fcut0, sizeof(float) * nr * nb);
         CIM_ALLOC(sb.pcrop, sizeof(float2) * (size_t)Dc * Dc * nb);
         // the global search masks unconditionally (ort_globesearch.f90:50, unlike mode 5)
         const int o0 = box / 2 - R - 1;
-        if (o0 >= 0 && o0 + Dc <= box) {
-            // the transform is needed on the crop only: rows pruned to its columns, then its columns
+        if (o0 >= 0 && o0 + Dc <= box && prep_fused()) {
+            // the transform is needed on the crop only: rows pruned to its columns, then its columns;
+            // the images are prepared as the row pass reads them, from their sums
+            CIM_ALLOC(ctx->prepst, sizeof(PrepStats) * (size_t)nb);
+            rc = prep_stats_dev(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), 0, 1,
+                                ctx->prepst.as<PrepStats>());
+            if (rc) return rc;
+            CIM_ALLOC(sb.ftrow, sizeof(float2) * (size_t)nb * box * Dc);
+            rc = fft2d_crop_real_dev(ctx, images_dev + (long long)b0 * N, ctx->prepst.as<PrepStats>(), 0, 1,
+                                     p->imgmask, box, nb, o0, Dc, o0, Dc, 1, nullptr, sb.ftrow.as<float2>(),
+                                     sb.pcrop.as<float2>());
+            if (rc) return rc;
+            rc = amp_expfit_general(ctx, sb.pcrop.as<float2>(), (long long)Dc * Dc, Dc, R + 1, nb, p->minR, R,
+                                    sb.fcut0.as<float>());
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_crop_shift, dim3((Dc * Dc + 255) / 256, nb), dim3(256), 0, ctx->stream,
+                               sb.pcrop.as<float2>(), box, R, Dc, ctx->parts.as<float>());
+            CIM_HIP(hipGetLastError());
+        } else if (o0 >= 0 && o0 + Dc <= box) {
+            // the same with a prepared copy of the images (CIM_PREP_FUSED=0)
+            CIM_ALLOC(ctx->ft, sizeof(float2) * N * nb);
+            float2* ft = ctx->ft.as<float2>();
             rc = prep_images_dev(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), ft, 0, 1);
             if (rc) return rc;
             CIM_ALLOC(sb.ftrow, sizeof(float2) * (size_t)nb * box * Dc);
@@ -2062,6 +2080,8 @@ static int search_global_dev(cim_ctx* ctx, const cim_params* p, int n, const flo
                                sb.pcrop.as<float2>(), box, R, Dc, ctx->parts.as<float>());
             CIM_HIP(hipGetLastError());
         } else {
+            CIM_ALLOC(ctx->ft, sizeof(float2) * N * nb);
+            float2* ft = ctx->ft.as<float2>();
             rc = preprocess_images(ctx, p, nb, images_dev + (long long)b0 * N, ctx->parts.as<float>(), ft, 0, 1, 1);
             if (rc) return rc;
             rc = amp_expfit(ctx, ft, box, nb, p->minR, R, sb.fcut0.as<float>());

@@ -283,3 +283,48 @@ def test_tabulated_search_second_windows(ctx, monkeypatch):
 def test_tabulated_search_needs_closed_grid(ctx):
     with pytest.raises(lib.CimError):
         _search_pair(ctx, lib.SEARCH_TABLES, 0, n=2, step=0.7)
+
+
+def test_fused_preparation_same_bits(ctx, monkeypatch):
+    """The pruned transforms of the search and the insert prepare the images as
+    the row pass reads them (the default) or from a prepared copy
+    (CIM_PREP_FUSED=0): the search's records and the inserted maps are the same
+    bits either way, with the normalisation before the mask, after it, and off."""
+    import torch
+    from cryoim_amd import synth
+    box, apix, step, n = 128, 1.5, 1.0, 96
+    ph = synth.make_phantom(box, nblobs=20, sigma=(2.0, 4.0), radius=(40.0, 55.0), seed=111)
+    tang = synth.random_au_orientations(4, np.random.default_rng(112))
+    timgs = synth.project(ph, tang)
+    ps = synth.make_particles_torch(ph, n, seed=113, shift_max=3.0, df_range=(1.0, 3.0), snr=0.1, apix=apix,
+                                    mask_radius=box / 2 - 4)
+    d = pipelines.derived(box, apix, maxRes=8.0, minRes=300.0, imgmask_A=(box / 2 - 4) * apix)
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 3:5] = ps.shifts
+    P[:, 6:8] = ps.df
+    P[:, 11] = 1.0
+    maxR = box // 2 - 6
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("CIM_PREP_FUSED", fused)
+        p = lib.make_params(box, d["minR"], d["maxR"], apply_ctf=True, mask_on=True, apix=apix, imgmask=d["mask"],
+                            searchstep=step, search_method=lib.SEARCH_TABLES)
+        ctx.search_templates(p, timgs, np.c_[tang, np.zeros((4, 2))])
+        res = ctx.search_global(p, ps.images, P)
+        out["search", fused] = res
+        Q = P.copy()
+        Q[:, 0:3] = res[:, 0:3]
+        for norm in (0, 1, 2):
+            pi_ = lib.make_params(box, 0, maxR, apply_ctf=True, mask_on=True, norm_on=norm, apix=apix,
+                                  imgmask=d["mask"])
+            ctx.rec_begin(box, maxR, layout=16, rcut=maxR, wg_init=0.01)   # 16: CIM_REC_COMPENSATED
+            ctx.rec_insert(pi_, ps.images, Q)
+            m = torch.empty((box, box, box), dtype=torch.float32, device="cuda:0")
+            ctx.rec_finish(box, m)
+            torch.cuda.synchronize()
+            out["map", norm, fused] = m.cpu().numpy()
+    assert np.array_equal(out["search", "1"], out["search", "0"])
+    for norm in (0, 1, 2):
+        a, b = out["map", norm, "1"], out["map", norm, "0"]
+        assert np.isfinite(a).all() and np.abs(a).max() > 0
+        assert np.array_equal(a, b), f"norm_on={norm}: max |diff| {np.abs(a - b).max()}"
