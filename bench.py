@@ -962,8 +962,10 @@ def _roofline(r, args):
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), **base, "algorithmic_bytes_per_launch": int(tb),
                      "bytes_rule": "every coarse and fine table entry of the batch read once",
-                     "note": "the sums read each table row once per (AU point, line) from the caches: the kernel "
-                             "is bound by cache-line throughput, not by HBM"}
+                     "note": "the sums read each table row once per (AU point, line) from L2, not HBM (limits: "
+                             "hbm_frac); the scattered 2- and 4-byte reads keep the texture-address units and L2 "
+                             "channels busy (ta_busy, l2_req_per_channel_cycle) and the VALU at its issue rate "
+                             "(valu_issue_per_simd_cycle against valu_full_rate): those bound it together"}
     # the least HBM traffic of a launch: each particle's staged sample values read once
     # ({a ctf, b ctf, ctf^2, in}, 16 B on the (2R+1)(R+1) half-plane grid) and the accumulator's
     # hi and lo cells within radius R+1 of the origin read and written once (16 B each)
@@ -1007,14 +1009,24 @@ def _limits(keys, roof, tr, n):
     def tot(c):
         return sum(sq[k]["per_launch"].get(c, 0.0) for k in keys)
     cyc = CLOCK_GHZ * 1e9 * roof["avg_launch_ms"] / 1e3
-    out = {"valu_busy": round(4 * tot("SQ_ACTIVE_INST_VALU") / (SIMDS * cyc), 3),
+    out = {"valu_issue_per_simd_cycle": round(tot("SQ_ACTIVE_INST_VALU") / (SIMDS * cyc), 4),
+           "valu_full_rate": 0.25,
            "mfma_busy": round(tot("SQ_VALU_MFMA_BUSY_CYCLES") / (SIMDS * cyc), 3),
            "lds_instr_per_cu_cycle": round(tot("SQ_INSTS_LDS") / (CUS * cyc), 4),
            "lds_bank_conflict_frac": round(tot("SQ_LDS_BANK_CONFLICT") / (CUS * cyc), 3)}
     if roof.get("traffic"):
         out["hbm_frac"] = round(roof["traffic"] / (roof["avg_launch_ms"] / 1e3) / (PEAK_HBM_GBS * 1e9), 4)
+    # texture-address and L2 rates within the counter run itself (its own clock: GRBM_GUI_ACTIVE
+    # sums the 8 XCDs; TA_BUSY_avr averages the CUs' address units; 16 L2 channels per XCD)
+    gui = tot("GRBM_GUI_ACTIVE") / len(keys) / 8
+    if gui > 0:
+        out["ta_busy"] = round(tot("TA_BUSY_avr") / len(keys) / gui, 3)
+        out["l2_req_per_channel_cycle"] = round(tot("TCC_REQ_sum") / (128 * gui * len(keys)), 3)
+        req = tot("TCC_REQ_sum")
+        out["l2_hit"] = round(tot("TCC_HIT_sum") / req, 3) if req else None
     out["rule"] = (f"counts per launch from profiles/sq_counters_box384.json over {CLOCK_GHZ} GHz x the live launch "
-                   f"time; {SIMDS} SIMDs, {CUS} CUs")
+                   f"time; {SIMDS} SIMDs, {CUS} CUs; a SIMD issues one full-rate wave64 FP32 instruction per 4 "
+                   f"cycles (valu_full_rate)")
     return out
 
 
