@@ -29,6 +29,8 @@ FSC between the two maps.
 of those particles; the default run reports its rates under extra.
 `--stage lbox` measures local_box's stack mode on 256 of them: per particle
 the 96-pixel boxes around its 12 five-fold vertices (local_z 140 px).
+`--stage lref` measures local_refine on those 3,072 vertex sub-particles:
+the C5 candidates, then 12 cycles of 0.25 deg angle and shift steps.
 
 Inputs are resident in HBM before the timed region.  The roofline block
 describes the kernel that takes most of a chosen step (its HIP-event time in
@@ -84,10 +86,11 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="time the chosen path step after step (search, then insert) on one stream")
-    ap.add_argument("--stage", choices=("main", "center", "lbox", "missrate", "c1"), default="main",
+    ap.add_argument("--stage", choices=("main", "center", "lbox", "lref", "missrate", "c1"), default="main",
                     help="center: only the centre finding of icosprocess mode 3 at box 384; "
                          "c1: the box-128 workload (1 deg grid, 2,000 particles per step), chosen vs plain; "
                          "lbox: only local_box's stack mode (12 vertex boxes of 96 pixels per particle); "
+                         "lref: local_refine of those vertex sub-particles (C5, 12 cycles of 0.25 deg); "
                          "missrate: chosen vs plain search on --particles distinct particles")
     ap.add_argument("--particles", type=int, default=5120,
                     help="missrate: distinct particles (generated on the GPU in batches of --batch); "
@@ -831,6 +834,149 @@ def lbox_main(args):
     print(json.dumps(line))
 
 
+LREF_BATCH, LREF_MAXRES_A, LREF_STEP, LREF_NCYCLE, LREF_SYM = 256, 6.0, 0.25, 12, "c5"
+LREF_METRIC = ("sub-particles/s local_refine (refine_PR), 96^2 vertex sub-particles of box-384 particles: "
+               "C5 candidates, then 12 cycles of 0.25 deg / shift steps")
+
+
+def _lref_setup(ph, ps, coords):
+    """The vertex sub-particles of LREF_BATCH particles (coords: the
+    local_box_coords records of the particles at local_z LBOX_Z, (n, 12, 8)),
+    started 1.5 deg off their local angles, and the 96^3 model of the phantom
+    around the vertex (refine_PR.f90:24-54 takes its transform)."""
+    from cryoim_amd import lib, synth
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pipelines
+    n = LREF_BATCH
+    idx = np.arange(n) % len(ps.images)
+    ops = vertex_ops()
+    x0 = np.array([0.0, 0.0, LBOX_Z], np.float32)
+    P = np.zeros((n, lib.PART_STRIDE), np.float32)
+    P[:, 0:3], P[:, 3:5] = ps.angles[idx], ps.shifts[idx]
+    rec = np.asarray(coords(P, ops, x0), np.float32).reshape(-1, 8)
+    m = len(rec)
+    lx = np.repeat(idx, len(ops))
+    d = pipelines.lref_derived(LBOX_SUB, APIX, maxRes=LREF_MAXRES_A, imgmask_A=(LBOX_SUB / 2 - 4) * APIX)
+    p = lib.make_params(LBOX_SUB, 0, LBOX_SUB // 2 - 2, apply_ctf=True, mask_on=True, conj_out=False, apix=APIX,
+                        imgmask=d["mask"], vol=300.0, df_scale=1.0, ncycle=LREF_NCYCLE, delta_angle=LREF_STEP,
+                        delta_trans=d["refine_shift"])
+    rng = np.random.default_rng(11)
+    L = np.zeros((m, lib.PART_STRIDE), np.float32)
+    L[:, 0:3] = rec[:, 0:3] + rng.uniform(-1.5, 1.5, (m, 3)).astype(np.float32)
+    L[:, 3:5] = rec[:, 5:7]
+    L[:, lib.P_DF1] = ps.df[lx, 0] * 1.0e4            # Angstrom, as local_refine reads them
+    L[:, lib.P_DF2] = ps.df[lx, 1] * 1.0e4
+    L[:, lib.P_BOXX:lib.P_BOXY + 1] = rec[:, 3:5]
+    model = synth.phantom_map(synth.Phantom(ph.centres - x0.astype(np.float64), ph.sigmas, ph.amps, LBOX_SUB))
+    return dict(idx=idx, lx=lx, L=L, p=p, d=d, model=model.astype(np.float32), m=m)
+
+
+def lref_run(ps, s, steps: int, warmup: int):
+    """cim_lref_refine on the m sub-particles (host records and images, as the
+    Fortran program passes them): the refinement kernel's HIP events and the
+    whole call's host clock (upload, preparation, refinement, records back)."""
+    from cryoim_amd import lib
+    ctx = lib.Context(0)
+    ctx.lref_model(s["p"], s["model"])
+    full = np.ascontiguousarray(ps.images[s["lx"]])
+    for _ in range(warmup):
+        ctx.lref_refine(s["p"], full, s["L"], sym=LREF_SYM, convF=0.8, maxres=s["d"]["maxres"])
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ctx.lref_refine(s["p"], full, s["L"], sym=LREF_SYM, convF=0.8, maxres=s["d"]["maxres"])
+    wall = (time.perf_counter() - t0) / steps
+    kt = ctx.timing_get("lref_refine")
+    ctx.timing(False)
+    ctx.close()
+    dang = np.abs((out[:, 0:3] - s["L"][:, 0:3] + 180.0) % 360.0 - 180.0)
+    return out, {"kernel_ms": kt[0], "launches": kt[1], "call_ms": wall * 1e3,
+                 "median_angle_change_deg": float(np.median(dang)), "median_pr_deg": float(np.median(out[:, 5]))}
+
+
+def _port_lref_job(k):
+    """local_refine of sub-particle k by the C oracle (FP32): its preparation
+    (norm, cut, mask, FFT) and the same C5 + 12-cycle refinement."""
+    O, pl = _oracle32()
+    st = _PORT_STATE
+    s, L = st["s"], st["s"]["L"]
+    t0 = time.perf_counter()
+    bx, by, d1, d2 = st["cols"]
+    a20, b20, w = pl.lref_particle(O, st["images"][s["lx"][k]], LBOX_SUB, int(L[k, bx]), int(L[k, by]),
+                                   float(L[k, 3]), float(L[k, 4]), s["d"]["mask"], s["d"]["maxres"])
+    ctf = O.ctf2d(LBOX_SUB, float(L[k, d1]), float(L[k, d2]), 0.0, APIX, vol=300.0, df_scale=1.0)
+    O.lref_refine(st["fr"], st["fi"], a20, b20, ctf, w, st["symops"], LREF_NCYCLE, LREF_STEP,
+                  s["d"]["refine_shift"], 0.8, s["d"]["maxres"], L[k, 0:5].astype(np.float64))
+    return time.perf_counter() - t0
+
+
+def lref_cpu_baseline(ph, ps):
+    """The port's own local_box_coords, preparation and refinement of a
+    sample of the sub-particles on the host cores."""
+    from cryoim_amd import lib
+    O, pl = _oracle32()
+    s = _lref_setup(ph, ps, lambda P, ops, x0: np.stack(
+        [O.local_box_coords(P[i, 0:3], P[i, 3:5], ops, x0, BOX, APIX) for i in range(len(P))]))
+    fr, fi = pl.lref_model_ft(O, s["model"], s["d"]["mask"])
+    ncores = min(8, os.cpu_count() or 1)
+    _PORT_STATE.update(s=s, images=ps.images, fr=fr, fi=fi, symops=O.lref_symops(0, 5),
+                       cols=(lib.P_BOXX, lib.P_BOXY, lib.P_DF1, lib.P_DF2))
+    K = 64
+    secs, wall = _port_pool(_port_lref_job, [i % s["m"] for i in range(K * ncores)], ncores)
+    _PORT_STATE.clear()
+    per = float(np.mean(secs))
+    return {"value": round(ncores / per, 2), "unit": "sub-particles/s", "cores": ncores, "kind": "port",
+            "sample": (f"C oracle (FP32) local_refine in {ncores} processes, {K} sub-particles each: preparation + "
+                       f"C5 + {LREF_NCYCLE} cycles (mean {per * 1e3:.1f} ms per sub-particle, {wall:.1f} s wall)")}
+
+
+def lref_main(args):
+    ph, _, _, ps = make_workload(args.batch, args.templates)
+    cpu = None if args.no_cpu_baseline else lref_cpu_baseline(ph, ps)     # before any GPU use
+    from cryoim_amd import lib
+    ctx = lib.Context(0)
+    s = _lref_setup(ph, ps, lambda P, ops, x0: ctx.local_box_coords(P, ops, x0, BOX, APIX))
+    ctx.close()
+    out, r = lref_run(ps, s, args.steps, args.warmup)
+    m, nq = s["m"], int(np.count_nonzero(
+        np.hypot(*np.meshgrid(np.arange(-48, 48), np.arange(-48, 48))) <= s["d"]["maxres"]))
+    ms = r["kernel_ms"]
+    # per sub-particle: 5 symmetry candidates + 7 slices per cycle (the centre slice feeds the 5
+    # shift candidates), 8 trilinear taps of 8 bytes per disk pixel; HBM: the model transform
+    # once per launch, and per sub-particle its disk values g (8 B) and template factor (4 B)
+    slices = 5 + 7 * LREF_NCYCLE
+    tap_bytes = m * slices * nq * 8 * 8
+    hbm_bytes = LBOX_SUB ** 3 * 8 + m * nq * 12
+    achieved = hbm_bytes / (ms / 1e3) / 1e9
+    line = {"metric": LREF_METRIC, "value": round(m / (ms / 1e3), 1), "unit": "sub-particles/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (the main workload's phantom and particles; 96^3 model of the phantom around the vertex)",
+            "config": {"workload": (f"local_refine of {m} sub-particles ({LREF_BATCH} box-{BOX} particles x 12 "
+                                    f"vertices at local_z {LBOX_Z} px), box {LBOX_SUB}, maxres {s['d']['maxres']:.1f} px "
+                                    f"({nq} disk pixels), {LREF_SYM} then {LREF_NCYCLE} cycles from 1.5 deg off"),
+                       "box": LBOX_SUB, "sub_particles": m, "ncycle": LREF_NCYCLE, "delta_angle": LREF_STEP,
+                       "sym": LREF_SYM, "disk_pixels": nq},
+            "roofline": {"bound": "hbm", "kernel": "k_lref_refine (central slices of the model transform + phase "
+                                                   "residual of 11 candidates per cycle, one workgroup per sub-particle)",
+                         "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                         "algorithmic_bytes_per_launch": hbm_bytes, "avg_launch_ms": round(ms, 4),
+                         "launches": r["launches"], "timing": "HIP events on the library's stream",
+                         "slice_taps_GBps": round(tap_bytes / (ms / 1e3) / 1e9, 1),
+                         "note": ("HBM bytes are the model transform once and each sub-particle's disk values; the "
+                                  "slices' 8-tap gathers (slice_taps_GBps) are served by the L2 and the MALL")},
+            "cpu_baseline": cpu,
+            "extra": {"call_sub_particles_per_s": round(m / (r["call_ms"] / 1e3), 1),
+                      "call_ms": round(r["call_ms"], 2),
+                      "call_note": "cim_lref_refine from host images and records: upload of the full images, "
+                                   "preparation, refinement, records back",
+                      "median_angle_change_deg": round(r["median_angle_change_deg"], 3),
+                      "median_pr_deg": round(r["median_pr_deg"], 3)}}
+    print(json.dumps(line))
+
+
 def _grid_work(step: float) -> int:
     from cryoim_amd import geometry
     pts = geometry.au_grid_points(step)
@@ -1041,6 +1187,9 @@ def main():
         return
     if args.stage == "lbox":
         lbox_main(args)
+        return
+    if args.stage == "lref":
+        lref_main(args)
         return
     if args.stage == "missrate":
         missrate_main(args)

@@ -117,9 +117,9 @@ struct SearchState {
 struct LrefState {
     bool ready = false;
     int box = 0;
-    DevBuf model, acc, disk, symops, mparts, g, tfac;
+    DevBuf model, model4, acc, disk, symops, mparts, g, tfac;   // model4: tap-paired copy of model
     void release() {
-        for (DevBuf* b : {&model, &acc, &disk, &symops, &mparts, &g, &tfac}) b->release();
+        for (DevBuf* b : {&model, &model4, &acc, &disk, &symops, &mparts, &g, &tfac}) b->release();
     }
 };
 

@@ -113,7 +113,14 @@ __global__ void __launch_bounds__(1024) k_cut_norm(const float* __restrict__ img
     }
 }
 
-// Particle values on the disk: g = FT * w_lp and the template factor
+// The particle side of a phase-residual term: {|g|, atan2(g)}.
+__device__ __forceinline__ float2 g_polar(float2 g) {
+    return make_float2(sqrtf(g.x * g.x + g.y * g.y), atan2f(g.y, g.x));
+}
+
+// Particle values on the disk, g = FT * w_lp, kept as {|g|, atan2(g)}: the
+// particle side of every phase-residual term (cal_PR_FT.f90:15-25), formed
+// once here instead of once per candidate and cycle; and the template factor
 // shift2d * CTF * w_lp (refine_PR.f90:43-48,133-134,151-152).
 __global__ void k_lref_disk(const float2* __restrict__ ft, int n, const int2* __restrict__ disk, int nq,
                             const float* __restrict__ parts, int apply_ctf, CtfConst cc, float maxres,
@@ -126,7 +133,7 @@ __global__ void k_lref_disk(const float2* __restrict__ ft, int n, const int2* __
     const float r2 = (float)(hk.x * hk.x + hk.y * hk.y) / 2.0f / (maxres * maxres);
     const float w = expf(-r2);
     const float2 v = ft[(long long)p * n * n + (long long)Y * n + X];
-    g[(long long)p * nq + q] = make_float2(v.x * w, v.y * w);
+    g[(long long)p * nq + q] = g_polar(make_float2(v.x * w, v.y * w));
     float c = 1.f;
     if (apply_ctf) {
         const float* pr = parts + (long long)p * CIM_PART_STRIDE;
@@ -140,10 +147,11 @@ __global__ void k_lref_disk(const float2* __restrict__ ft, int n, const int2* __
 
 struct LrefArgs {
     const float2* model;   // n^3 model transform
+    const float4* model4;  // its tap-paired copy (k_pair_model)
     int n;
     const int2* disk;      // nq pixels (hx, ky) centred, radius <= maxres
     int nq;
-    const float2* g;       // particle values, per particle nq
+    const float2* g;       // particle values {|g|, atan2(g)}, per particle nq
     const float* tfac;     // template factor, per particle nq
     const float* symops;   // nsym x 9 (float) or null
     int nsym;
@@ -174,16 +182,53 @@ __device__ __forceinline__ float2 slice_at(const float2* __restrict__ m, int n, 
                        w1 * t1.y + w2 * t2.y + w3 * t3.y + w4 * t4.y + w5 * t5.y + w6 * t6.y + w7 * t7.y + w8 * t8.y);
 }
 
+// slice_at on the tap-paired model m4[i] = {m[i], m[i + 1]}: the same taps,
+// weights and sums, from 4 loads of 16 bytes instead of 8 of 8 (the radius
+// test keeps ix <= n - 2, so the pair never crosses a row).
+__device__ __forceinline__ float2 slice_at_p(const float4* __restrict__ m4, int n, const float* R, int hx, int ky) {
+    const float fx = (float)hx, fy = (float)ky;
+    float X = R[0] * fx + R[1] * fy, Y = R[3] * fx + R[4] * fy, Z = R[6] * fx + R[7] * fy;
+    if (sqrtf(X * X + Y * Y + Z * Z) > (float)(n / 2 - 2)) return make_float2(0.f, 0.f);
+    X += (float)(n / 2);
+    Y += (float)(n / 2);
+    Z += (float)(n / 2);
+    const int ix = (int)X, iy = (int)Y, iz = (int)Z;
+    const float xb = X - ix, yb = Y - iy, zb = Z - iz;
+    const long long n2 = (long long)n * n;
+    const float4* b = m4 + (long long)iz * n2 + (long long)iy * n + ix;
+    const float4 u1 = b[0], u3 = b[n], u5 = b[n2], u7 = b[n2 + n];
+    const float2 t1 = make_float2(u1.x, u1.y), t2 = make_float2(u1.z, u1.w);
+    const float2 t3 = make_float2(u3.x, u3.y), t4 = make_float2(u3.z, u3.w);
+    const float2 t5 = make_float2(u5.x, u5.y), t6 = make_float2(u5.z, u5.w);
+    const float2 t7 = make_float2(u7.x, u7.y), t8 = make_float2(u7.z, u7.w);
+    const float w1 = (1.f - xb) * (1.f - yb) * (1.f - zb), w2 = xb * (1.f - yb) * (1.f - zb);
+    const float w3 = (1.f - xb) * yb * (1.f - zb), w4 = xb * yb * (1.f - zb);
+    const float w5 = (1.f - xb) * (1.f - yb) * zb, w6 = xb * (1.f - yb) * zb;
+    const float w7 = (1.f - xb) * yb * zb, w8 = xb * yb * zb;
+    return make_float2(w1 * t1.x + w2 * t2.x + w3 * t3.x + w4 * t4.x + w5 * t5.x + w6 * t6.x + w7 * t7.x + w8 * t8.x,
+                       w1 * t1.y + w2 * t2.y + w3 * t3.y + w4 * t4.y + w5 * t5.y + w6 * t6.y + w7 * t7.y + w8 * t8.y);
+}
+
+// Tap-paired copy of an n^3 transform: out[i] = {m[i], m[i + 1]} (the last
+// element repeats itself).
+__global__ void k_pair_model(const float2* __restrict__ m, long long N, float4* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float2 a = m[i], b = m[i + 1 < N ? i + 1 : i];
+    out[i] = make_float4(a.x, a.y, b.x, b.y);
+}
+
 // One term of cal_PR_FT.f90:15-25: weight |F1| + |F2|, phase difference
-// folded into [0, pi].  A zero sample has phase atan2(0, 0) = 0 there.
-__device__ __forceinline__ void pr_term(float2 t, float2 g, float& num, float& den) {
-    const float a1 = sqrtf(t.x * t.x + t.y * t.y), a2 = sqrtf(g.x * g.x + g.y * g.y);
+// folded into [0, pi].  A zero sample has phase atan2(0, 0) = 0 there.  The
+// particle value g comes as gp = g_polar(g).
+__device__ __forceinline__ void pr_term(float2 t, float2 gp, float& num, float& den) {
+    const float a1 = sqrtf(t.x * t.x + t.y * t.y), a2 = gp.x, p2 = gp.y;
     float d;
     if (a1 > 0.f && a2 > 0.f) {
-        const float p1 = atan2f(t.y, t.x), p2 = atan2f(g.y, g.x);
+        const float p1 = atan2f(t.y, t.x);
         d = fabsf(p2 - p1);
     } else {
-        d = fabsf(a1 > 0.f ? atan2f(t.y, t.x) : atan2f(g.y, g.x));
+        d = fabsf(a1 > 0.f ? atan2f(t.y, t.x) : p2);
     }
     const float pi = 3.14159265358979323846f, tpi = 6.28318530717958647692f;
     if (d >= tpi) d -= tpi;
@@ -250,7 +295,7 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
             float num[1] = {0.f}, den[1] = {0.f};
             for (int q = tid; q < a.nq; q += blockDim.x) {
                 const int2 hk = a.disk[q];
-                float2 z = slice_at(a.model, a.n, R, hk.x, hk.y);
+                float2 z = slice_at_p(a.model4, a.n, R, hk.x, hk.y);
                 z = apply_shift(z, shift_phase(hk.x, hk.y, a.n, cur[3], cur[4], 0));
                 pr_term(make_float2(z.x * tf[q], z.y * tf[q]), g[q], num[0], den[0]);
             }
@@ -298,18 +343,21 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
         for (int c = 5; c <= 10; ++c) euler_f(cand[c][0], cand[c][1], cand[c][2], Rs[c - 4]);
         float num[11], den[11];
         for (int c = 0; c < 11; ++c) { num[c] = 0.f; den[c] = 0.f; }
+        // candidates 0 and 5-10 keep the current shift (cand[c][3..4] are copies of cur[3..4]), so
+        // one shift phase serves all seven
         for (int q = tid; q < a.nq; q += blockDim.x) {
             const int2 hk = a.disk[q];
             const float2 gq = g[q];
             const float t = tf[q];
-            const float2 z0 = slice_at(a.model, a.n, Rs[0], hk.x, hk.y);
+            const float2 z0 = slice_at_p(a.model4, a.n, Rs[0], hk.x, hk.y);
+            const float2 ph0 = shift_phase(hk.x, hk.y, a.n, cand[0][3], cand[0][4], 0);
             for (int c = 0; c <= 4; ++c) {
-                const float2 z = apply_shift(z0, shift_phase(hk.x, hk.y, a.n, cand[c][3], cand[c][4], 0));
+                const float2 z = apply_shift(z0, c ? shift_phase(hk.x, hk.y, a.n, cand[c][3], cand[c][4], 0) : ph0);
                 pr_term(make_float2(z.x * t, z.y * t), gq, num[c], den[c]);
             }
             for (int c = 5; c <= 10; ++c) {
-                float2 z = slice_at(a.model, a.n, Rs[c - 4], hk.x, hk.y);
-                z = apply_shift(z, shift_phase(hk.x, hk.y, a.n, cand[c][3], cand[c][4], 0));
+                float2 z = slice_at_p(a.model4, a.n, Rs[c - 4], hk.x, hk.y);
+                z = apply_shift(z, ph0);
                 pr_term(make_float2(z.x * t, z.y * t), gq, num[c], den[c]);
             }
         }
@@ -591,6 +639,10 @@ int cim_lref_model(cim_ctx* ctx, const cim_params* p, const float* vol) {
     hipLaunchKernelGGL(k_checker3d, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->lref.model.as<float2>(), n);
     CIM_HIP(hipGetLastError());
+    CIM_ALLOC(ctx->lref.model4, sizeof(float4) * N);
+    hipLaunchKernelGGL(k_pair_model, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->lref.model.as<float2>(), N, ctx->lref.model4.as<float4>());
+    CIM_HIP(hipGetLastError());
     CIM_HIP(hipStreamSynchronize(ctx->stream));
     ctx->lref.box = n;
     ctx->lref.ready = true;
@@ -690,6 +742,7 @@ int cim_lref_refine(cim_ctx* ctx, const cim_params* p, int n, const float* image
     const int nb = p->box;
     LrefArgs a{};
     a.model = L.model.as<float2>();
+    a.model4 = L.model4.as<float4>();
     a.n = nb;
     a.disk = L.disk.as<int2>();
     a.nq = nq;
@@ -732,6 +785,7 @@ int cim_lref_defocus(cim_ctx* ctx, const cim_params* p, int n, const float* imag
     LrefState& L = ctx->lref;
     LrefArgs a{};
     a.model = L.model.as<float2>();
+    a.model4 = L.model4.as<float4>();
     a.n = p->box;
     a.disk = L.disk.as<int2>();
     a.nq = nq;
