@@ -949,6 +949,11 @@ def lref_main(args):
     tap_bytes = m * slices * nq * 8 * 8
     hbm_bytes = LBOX_SUB ** 3 * 8 + m * nq * 12
     achieved = hbm_bytes / (ms / 1e3) / 1e9
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_lref_box96.json")) as f:
+            traffic = json.load(f)["k_lref_refine"]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        traffic = None
     line = {"metric": LREF_METRIC, "value": round(m / (ms / 1e3), 1), "unit": "sub-particles/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -961,7 +966,9 @@ def lref_main(args):
             "roofline": {"bound": "hbm", "kernel": "k_lref_refine (central slices of the model transform + phase "
                                                    "residual of 11 candidates per cycle, one workgroup per sub-particle)",
                          "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                         "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                         "traffic_source": "profiles/hbm_traffic_lref_box96.json (rocprofv3 FETCH_SIZE and "
+                                           "WRITE_SIZE, copy-calibrated)",
                          "algorithmic_bytes_per_launch": hbm_bytes, "avg_launch_ms": round(ms, 4),
                          "launches": r["launches"], "timing": "HIP events on the library's stream",
                          "slice_taps_GBps": round(tap_bytes / (ms / 1e3) / 1e9, 1),

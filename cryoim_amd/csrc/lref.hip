@@ -266,6 +266,9 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
     __shared__ float cand[11][5];
     __shared__ float cur[6];
     __shared__ float symang[3];
+    // the rotations of candidates 0 and 5-10 of a cycle, formed once per workgroup (an array indexed
+    // by the candidate loop in each thread would live in scratch memory)
+    __shared__ float rot[7][9];
     const int p = blockIdx.x, tid = threadIdx.x;
     const float2* g = a.g + (long long)p * a.nq;
     const float* tf = a.tfac + (long long)p * a.nq;
@@ -338,9 +341,11 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
             for (int k = 0; k < 5; ++k) cand[tid][k] = o[k];
         }
         __syncthreads();
-        float Rs[7][9];
-        euler_f(cand[0][0], cand[0][1], cand[0][2], Rs[0]);
-        for (int c = 5; c <= 10; ++c) euler_f(cand[c][0], cand[c][1], cand[c][2], Rs[c - 4]);
+        if (tid < 7) {
+            const int c = tid ? tid + 4 : 0;
+            euler_f(cand[c][0], cand[c][1], cand[c][2], rot[tid]);
+        }
+        __syncthreads();
         float num[11], den[11];
         for (int c = 0; c < 11; ++c) { num[c] = 0.f; den[c] = 0.f; }
         // candidates 0 and 5-10 keep the current shift (cand[c][3..4] are copies of cur[3..4]), so
@@ -349,14 +354,14 @@ __global__ void __launch_bounds__(256) k_lref_refine(LrefArgs a) {
             const int2 hk = a.disk[q];
             const float2 gq = g[q];
             const float t = tf[q];
-            const float2 z0 = slice_at_p(a.model4, a.n, Rs[0], hk.x, hk.y);
+            const float2 z0 = slice_at_p(a.model4, a.n, rot[0], hk.x, hk.y);
             const float2 ph0 = shift_phase(hk.x, hk.y, a.n, cand[0][3], cand[0][4], 0);
             for (int c = 0; c <= 4; ++c) {
                 const float2 z = apply_shift(z0, c ? shift_phase(hk.x, hk.y, a.n, cand[c][3], cand[c][4], 0) : ph0);
                 pr_term(make_float2(z.x * t, z.y * t), gq, num[c], den[c]);
             }
             for (int c = 5; c <= 10; ++c) {
-                float2 z = slice_at_p(a.model4, a.n, Rs[c - 4], hk.x, hk.y);
+                float2 z = slice_at_p(a.model4, a.n, rot[c - 4], hk.x, hk.y);
                 z = apply_shift(z, ph0);
                 pr_term(make_float2(z.x * t, z.y * t), gq, num[c], den[c]);
             }
